@@ -1,0 +1,178 @@
+/*
+ * uniphot_amd -- C ABI of the MI355X (gfx950) forced-photometry fit path.
+ *
+ * Drop-in boundary for the hot path of uniphot's ``tract7dt.fit`` island
+ * architecture.  All array arguments are device pointers (row-major,
+ * C-contiguous) owned by the caller; the library owns nothing between calls.
+ * Every entry point is stream-ordered on ``stream`` (a hipStream_t; NULL =
+ * the default stream) and performs no host synchronisation unless stated.
+ *
+ * Status codes: UPH_OK, UPH_ERR_INVALID_ARGUMENT, UPH_ERR_OOM (the caller may
+ * halve the island batch and retry, as the reference halves a chunk on
+ * RESOURCE_EXHAUSTED, tract7dt/fit/grid_gpu/island_solver.py:3134-3230) and
+ * UPH_ERR_INTERNAL.  ``uph_last_error()`` returns the calling thread's last
+ * error message.
+ */
+#ifndef UNIPHOT_AMD_H
+#define UNIPHOT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UPH_OK 0
+#define UPH_ERR_INVALID_ARGUMENT 1
+#define UPH_ERR_OOM 2
+#define UPH_ERR_INTERNAL 3
+
+#define UPH_PRECISION_FP32 0 /* pixel arithmetic in float32 (production) */
+#define UPH_PRECISION_FP64 1 /* pixel arithmetic in float64 (GPU oracle)  */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int uph_version(void);
+
+/* Last error message of the calling thread ("" when none). */
+const char* uph_last_error(void);
+
+/*
+ * Legacy LM constants of one island class.  Mirrors the knobs the reference
+ * bakes into its compiled step program (make_class_kernels,
+ * tract7dt/fit/grid_gpu/island_solver.py:1874-1922; values
+ * island_plan.py:55-74 and island_solver.py:146-152).
+ */
+typedef struct uph_lm_options {
+  float reg;            /* ridge on real member diagonals (1e-9)          */
+  float pos_step_max;   /* per-step clip on x, y (0.5 px)                  */
+  float shape_step_max; /* per-step clip on logre, ee1, ee2 (0.15)         */
+  float un_step_max;    /* per-step clip on u_n (0.5)                      */
+  float ftol_rel;       /* prong 1: dchi2 < ftol_rel * max(chi2, 1) (1e-4) */
+  float step_tol;       /* prong 2: position step < step_tol px (0.1)      */
+  float shape_step_tol; /* prong 2: shape steps < shape_step_tol (0.01)    */
+  int max_steps;        /* LM steps per island (8 main + 22 re-queue = 30) */
+} uph_lm_options;
+
+/*
+ * One island class (static shape) and its chunk of islands.  Replaces the
+ * per-class compiled program family of ``_make_bucket_kernels`` plus
+ * ``run_class_chunk`` (island_solver.py:221-1848, :2793-3131) for the
+ * legacy step protocol.  Layouts follow ``class_host_arrays``
+ * (island_plan.py:682-699) and the padded field stacks
+ * (island_stage.py:1711-1751).
+ */
+typedef struct uph_island_class {
+  int n_isl;      /* islands in this call                                  */
+  int N;          /* nsrc class (member slots per island), <= 64           */
+  int B;          /* window side (bbox class), multiple of 16              */
+  int T;          /* stamp tier (odd)                                      */
+  int n_terms;    /* PSF polynomial terms (1, 3, 6, 10, 15 or 21)          */
+  int n_bands;    /* bands                                                 */
+  int n_live_max; /* max live parameters per island (2N star, 6N mixed)   */
+  int Hp, Wp;     /* padded field-stack height / width                     */
+  int mixed;      /* 0: star-only class, 1: class with extended members    */
+} uph_island_class;
+
+typedef struct uph_island_inputs {
+  const float* dstack;    /* (n_bands, Hp, Wp) data - frozen background    */
+  const float* wstack;    /* (n_bands, Hp, Wp) inverse variance (0 = bad)  */
+  const float* psf_coefs; /* (n_bands, n_terms, T, T) planar PSF coefs     */
+  const float* psf_geo;   /* (n_bands, 4) x_origin, x_scale, y_origin, y_scale */
+  const float* sersic;    /* Sersic mixture table block; NULL for star-only classes */
+  const int32_t* ox;      /* (n_isl) window origin x in the padded stack   */
+  const int32_t* oy;      /* (n_isl)                                        */
+  const float* x0b;       /* (n_isl) window origin x in image coordinates  */
+  const float* y0b;       /* (n_isl)                                        */
+  const int32_t* stype;   /* (n_isl, N) 0 STAR, 1 EXP, 2 DEV, 3 SERSIC     */
+  const float* n_frozen;  /* (n_isl, N) Sersic index when not free          */
+  const uint8_t* free_n;  /* (n_isl, N)                                     */
+  const uint8_t* mmask;   /* (n_isl, N) real member slots                   */
+  const uint8_t* keep;    /* (n_isl, N, 6) fitted parameters                */
+  const float* p0;        /* (n_isl, 6N) seeds x, y, logre, ee1, ee2, u_n   */
+  const float* lower;     /* (n_isl, 6N) box bounds                         */
+  const float* upper;     /* (n_isl, 6N)                                    */
+  const int32_t* order;   /* (n_isl) processing order (work queue), or NULL */
+} uph_island_inputs;
+
+typedef struct uph_island_outputs {
+  float* p;          /* (n_isl, 6N) fitted parameters                       */
+  float* fluxes;     /* (n_isl, n_bands, N) VarPro member fluxes            */
+  float* ferr;       /* (n_isl, n_bands, N) sqrt(diag(A^-1))                */
+  float* chi2;       /* (n_isl)                                             */
+  float* lam;        /* (n_isl) final LM damping                            */
+  uint8_t* conv;     /* (n_isl)                                             */
+  uint8_t* stall;    /* (n_isl) converged by 4 consecutive rejects          */
+  int32_t* nst;      /* (n_isl) LM steps taken                              */
+  int32_t* consec;   /* (n_isl)                                             */
+  int32_t* nrej;     /* (n_isl)                                             */
+  int32_t* n_good;   /* (n_isl) good pixels summed over bands               */
+} uph_island_outputs;
+
+/* Bytes of device workspace one persistent workgroup needs for a class. */
+size_t uph_island_scratch_per_block(int N, int B, int T, int n_live_max, int precision);
+
+/*
+ * Fit every island of the chunk: fresh LM state (lam 1e-3, chi2 at p0), up
+ * to max_steps legacy steps with device-side accept/reject and convergence,
+ * then final fluxes.  One persistent kernel; islands are pulled from a work
+ * queue in ``order``.  ``scratch`` holds n_blocks workgroups' workspace
+ * (n_blocks = scratch_bytes / uph_island_scratch_per_block(...)); ``counter``
+ * is one device int32 the library resets.  Replaces
+ * ``run_class_chunk`` (island_solver.py:2793) with fresh state.
+ */
+int uph_island_fit(const uph_island_class* cls, const uph_island_inputs* in,
+                   const uph_island_outputs* out, const uph_lm_options* opt, int precision,
+                   void* scratch, size_t scratch_bytes, int32_t* counter, void* stream);
+
+/*
+ * Batched unit-flux stamp renderer: PSF polynomial at (x, y), galaxy
+ * mixture convolution for extended sources, unit-sum renormalisation when
+ * ``renorm`` (all-star grids), Lanczos-3 shift by the remainder of
+ * (pos - origin).  Replaces render_source_stamps / render_star_stamps /
+ * forced_phot_sparse.render_stamps (source_model.py:331-390,
+ * forced_phot_sparse.py:126-220).
+ *   src_band (n) : band index of each stamp; x, y (n) full-image position;
+ *   gx0, gy0 (n) : origin the sub-pixel remainder is taken against;
+ *   stype, logre, ee1, ee2, sersic_n (n) : morphology;
+ *   out (n, T, T).
+ */
+int uph_render_stamps(int n, int T, int n_terms, const float* psf_coefs, const float* psf_geo,
+                      const float* sersic, const int32_t* src_band, const float* x,
+                      const float* y, const float* gx0, const float* gy0,
+                      const int32_t* stype, const float* logre, const float* ee1,
+                      const float* ee2, const float* sersic_n, int renorm, float* out,
+                      void* stream);
+
+/*
+ * Batched per grid-band linear solve (member fluxes + polynomial background
+ * in one normal-equation system, float64 solve and inverse diagonal),
+ * replacing solve_band_sparse / launch_band_sparse
+ * (forced_phot_sparse.py:583-709, solver_inner :342-554).  One problem per
+ * (grid, band).  Problem q owns sources [src_off[q], src_off[q+1]) of the
+ * flattened per-problem source arrays and the grid window
+ * win[q] = (band, x0, y0, h, w) of the (n_bands, H, W) image arrays, whose
+ * pixel (0, 0) sits at full-image pixel (img_x0, img_y0).
+ *   data, invvar (n_bands, H, W) f32; good (n_bands, H, W) u8;
+ *   background basis X^m Y^n of order bg_order in X = (x - bg_xo) / bg_xs,
+ *   Y = (y - bg_yo) / bg_ys (full-image pixel coordinates, computed in
+ *   float64 and rounded to float32 as bg_polynomial.make_bg_basis does);
+ *   stamps (total_src, T, T) f32 (from uph_render_stamps);
+ *   ix_top, iy_top (total_src) i32 window-local stamp top-left;
+ *   nbr_idx (total_src, k_max) i32 problem-local neighbour (-1 = none);
+ *   outputs fluxes, flux_var (total_src) and bg, bg_var (n_prob, M) f32.
+ * ``scratch`` must hold uph_band_scratch_bytes(n_prob, max_src, bg_order).
+ */
+size_t uph_band_scratch_bytes(int n_prob, int max_src, int bg_order);
+int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const int64_t* src_off,
+                   const int32_t* win, int H, int W, int img_x0, int img_y0, const float* data,
+                   const float* invvar, const uint8_t* good, double bg_xo, double bg_xs, double bg_yo, double bg_ys,
+                   const float* stamps, const int32_t* ix_top, const int32_t* iy_top,
+                   const int32_t* nbr_idx, float reg, float* fluxes, float* flux_var, float* bg,
+                   float* bg_var, void* scratch, size_t scratch_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UNIPHOT_AMD_H */

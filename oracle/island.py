@@ -1,0 +1,279 @@
+"""FP64 restatement of the per-island VarPro-LM (TEST INFRASTRUCTURE).
+
+Reference: ``tract7dt/fit/grid_gpu/island_solver.py`` -- ``models_band``
+(:480-562), ``_fluxes_band`` / ``residual_band`` / ``chi2_of``
+(:570-684), ``hg_scan`` (:812-865), legacy ``one_step`` (:921-1002),
+``final_fluxes`` (:1467-1482), ``final_errors`` (:1484-1547), and the
+chunk loop of ``run_class_chunk`` (:2890-2977).  Host input layout:
+``island_plan.class_host_arrays`` (:493-699).
+
+The Jacobian is formed the way the reference forms it: one explicit
+whitened-residual tangent per live parameter, per pixel, through the
+closed-form VarPro flux solve (d f = A^-1 (d b - d A f)), then
+H = J J^T and g = J r.  (The HIP kernel uses Gram-matrix identities
+instead, so the two are independent formulations.)
+
+Parity pins: tests/test_oracle_island.py checks the VarPro fluxes against
+``numpy.linalg.lstsq``, the Jacobian against float64 central finite
+differences of the residual, and the LM invariants (chi2 non-increasing
+over accepts, bounds and clips respected, frozen parameters untouched,
+padded slots inert, convergence prongs).  Trajectories of the reference
+JAX program itself are parity unpinned (JAX is not importable here).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import core
+
+LM_LAMBDA0 = 1e-3
+LM_UP, LM_DOWN = 10.0, 0.1
+STALL_REJECT_K = 4
+
+
+@dataclass
+class LMOptions:
+    """Legacy-step constants (island_solver.py:146-152, island_plan.py:55-74)."""
+    reg: float = 1e-9
+    pos_step_max: float = 0.5
+    shape_step_max: float = 0.15
+    un_step_max: float = 0.5
+    ftol_rel: float = 1e-4
+    step_tol: float = 0.1
+    shape_step_tol: float = 0.01
+    max_steps: int = 30
+
+
+def clip_vec(N: int, o: LMOptions) -> np.ndarray:
+    return np.tile([o.pos_step_max, o.pos_step_max, o.shape_step_max,
+                    o.shape_step_max, o.shape_step_max, o.un_step_max], N)
+
+
+_PNAMES = ("x", "y", "lr", "e1", "e2", "u")
+
+
+class Island:
+    """One island: window data + member description (float64 copies).
+
+    Fields follow the ``isl`` pytree of island_solver.py:2742-2778:
+    data, w (nb,B,B); x0, y0; stype, n_frozen, free_n, mmask (N,);
+    keep (N,6); lower, upper (6N,).  ``psf`` is a list over bands of dicts
+    (coefs (T,T,nt), x_origin, x_scale, y_origin, y_scale).
+    """
+
+    def __init__(self, data, w, x0, y0, stype, n_frozen, free_n, mmask, keep,
+                 lower, upper, psf, T):
+        self.data = np.asarray(data, np.float64)
+        self.w = np.asarray(w, np.float64)
+        self.nb, self.B = self.data.shape[0], self.data.shape[1]
+        self.x0, self.y0 = float(x0), float(y0)
+        self.stype = np.asarray(stype, np.int64)
+        self.N = self.stype.size
+        self.n_frozen = np.asarray(n_frozen, np.float64)
+        self.free_n = np.asarray(free_n, bool)
+        self.mmask = np.asarray(mmask, bool)
+        self.keep = np.asarray(keep, bool) & self.mmask[:, None]
+        self.lower = np.asarray(lower, np.float64)
+        self.upper = np.asarray(upper, np.float64)
+        self.psf = psf
+        self.T = int(T)
+
+    def live(self):
+        """(member, param) index pairs of kept parameters (keep & mmask)."""
+        return [(k, j) for k in range(self.N) for j in range(6) if self.keep[k, j]]
+
+
+def models_band(p, isl: Island, b: int, want_grad: bool):
+    """Placed unit-flux member models (N,B,B) for band b and, with
+    ``want_grad``, a dict {(k, j): placed tangent plane} for live params."""
+    P = p.reshape(isl.N, 6)
+    B, T = isl.B, isl.T
+    m = np.zeros((isl.N, B, B))
+    tan = {}
+    live = isl.live() if want_grad else []
+    for k in range(isl.N):
+        if not isl.mmask[k]:
+            continue
+        x, y = P[k, 0], P[k, 1]
+        st = int(isl.stype[k])
+        if st == core.SOURCE_STAR:
+            lr, e1, e2 = 0.0, 1.0, 0.0
+        else:
+            lr, e1, e2 = P[k, 2], P[k, 3], P[k, 4]
+            if e1 == 0.0 and e2 == 0.0:
+                e1 = 1e-4
+        if isl.free_n[k]:
+            n_val = float(core.sersic_u_to_n(P[k, 5]))
+        else:
+            n_val = isl.n_frozen[k]
+        jk = [j for (kk, j) in live if kk == k]
+        want = []
+        for j in jk:
+            want.append({0: "x", 1: "y", 2: "lr", 3: "e1", 4: "e2", 5: "n"}[j])
+        stamp, grads = core.member_stamp(isl.psf[b], x, y, isl.x0, isl.y0, st,
+                                         lr, e1, e2, n_val, want=tuple(want))
+        ix, iy = core.stamp_top_left(x, y, isl.x0, isl.y0, T)
+        m[k] = core.place(stamp, ix, iy, B)
+        for j in jk:
+            name = {0: "x", 1: "y", 2: "lr", 3: "e1", 4: "e2", 5: "n"}[j]
+            g = grads[name]
+            if j == 5:
+                nmin, nmax = core.SERSIC_N_MIN, core.SERSIC_N_MAX
+                g = g * (n_val - nmin) * (nmax - n_val) / (nmax - nmin)
+            tan[(k, j)] = core.place(g, ix, iy, B)
+    return m, tan
+
+
+def fluxes_band(m, d, w, mmask, reg):
+    """Closed-form member fluxes (island_solver.py:570-590).  Returns
+    (f (N,), A64 (N,N) with the padded/ridge diagonal, A_raw)."""
+    N = m.shape[0]
+    wm = w[None] * m
+    A = np.einsum("kij,lij->kl", wm, m)
+    b = np.einsum("kij,ij->k", wm, d)
+    A64 = A + np.diag(np.where(mmask, reg, 1.0))
+    f = np.linalg.solve(A64, b) * mmask
+    return f, A64, A
+
+
+def chi2_of(p, isl: Island, o: LMOptions) -> float:
+    c = 0.0
+    for b in range(isl.nb):
+        m, _ = models_band(p, isl, b, False)
+        f, _, _ = fluxes_band(m, isl.data[b], isl.w[b], isl.mmask, o.reg)
+        r = np.sqrt(isl.w[b]) * (isl.data[b] - np.einsum("k,kij->ij", f, m))
+        c += float(np.sum(r * r))
+    return c
+
+
+def hg(p, isl: Island, o: LMOptions):
+    """Un-masked, un-damped flux-marginalised Gauss-Newton (H, g) over the
+    live parameters, plus the chi2 at p (island_solver.py:812-865)."""
+    live = isl.live()
+    L = len(live)
+    H = np.zeros((L, L))
+    g = np.zeros(L)
+    chi2 = 0.0
+    for b in range(isl.nb):
+        d, w = isl.data[b], isl.w[b]
+        sw = np.sqrt(w)
+        m, tan = models_band(p, isl, b, True)
+        f, A64, _ = fluxes_band(m, d, w, isl.mmask, o.reg)
+        model = np.einsum("k,kij->ij", f, m)
+        r = sw * (d - model)
+        chi2 += float(np.sum(r * r))
+        J = np.zeros((L, isl.B * isl.B))
+        for i, (k, j) in enumerate(live):
+            t = tan[(k, j)]
+            # dA = e_k u^T + u e_k^T, u_l = sum w m_l t ; db = e_k sum w t d
+            u = np.einsum("lij,ij->l", w[None] * m, t)
+            dA = np.zeros_like(A64)
+            dA[k, :] += u
+            dA[:, k] += u
+            db = np.zeros(isl.N)
+            db[k] = np.sum(w * t * d)
+            df = np.linalg.solve(A64, db - dA @ f) * isl.mmask
+            dmodel = np.einsum("l,lij->ij", df, m) + f[k] * t
+            J[i] = (-sw * dmodel).ravel()
+        H += J @ J.T
+        g += J @ r.ravel()
+    return H, g, chi2, live
+
+
+def one_step(state: dict, isl: Island, o: LMOptions) -> dict:
+    """Legacy damped LM step with accept/reject and the three convergence
+    prongs (island_solver.py:921-1002).  ``state`` keys: p, lam, chi2,
+    consec, nrej, conv, stall, nst."""
+    if state["conv"]:
+        return dict(state)
+    N = isl.N
+    nP = 6 * N
+    p = state["p"]
+    Hl, gl, _, live = hg(p, isl, o)
+    H = np.zeros((nP, nP))
+    g = np.zeros(nP)
+    idx = np.array([6 * k + j for (k, j) in live], dtype=np.int64)
+    if idx.size:
+        H[np.ix_(idx, idx)] = Hl
+        g[idx] = gl
+    keep = np.zeros(nP, bool)
+    keep[idx] = True
+    H = np.where(keep[:, None] & keep[None, :], H, 0.0) + np.diag(np.where(keep, 0.0, 1.0))
+    g = np.where(keep, g, 0.0)
+    Hd = H + state["lam"] * np.diag(np.diag(H)) + 1e-8 * np.eye(nP)
+    delta = -np.linalg.solve(Hd, g)
+    delta = np.where(keep, delta, 0.0)
+    cv = clip_vec(N, o)
+    delta = np.clip(delta, -cv, cv)
+    p_try = np.clip(p + delta, isl.lower, isl.upper)
+    chi2_try = chi2_of(p_try, isl, o)
+    chi2_cur = state["chi2"]
+    accept = chi2_try < chi2_cur
+    p_new = p_try if accept else p
+    lam = state["lam"]
+    lam_new = max(lam * LM_DOWN, 1e-9) if accept else min(lam * LM_UP, 1e6)
+    chi2_new = chi2_try if accept else chi2_cur
+    dchi2 = chi2_cur - chi2_try
+    step = (p_new - p).reshape(N, 6)
+    mmf = isl.mmask.astype(np.float64)
+    pos_step = float(np.max(mmf * np.hypot(step[:, 0], step[:, 1]))) if N else 0.0
+    shape_step = float(np.max(mmf[:, None] * np.abs(step[:, 2:6]))) if N else 0.0
+    small = accept and (dchi2 < o.ftol_rel * max(chi2_cur, 1.0))
+    consec = state["consec"] + 1 if small else (0 if accept else state["consec"])
+    step_small = accept and pos_step < o.step_tol and shape_step < o.shape_step_tol
+    nrej = 0 if accept else state["nrej"] + 1
+    stall_now = nrej >= STALL_REJECT_K
+    conv = bool(consec >= 2 or step_small or stall_now)
+    return dict(p=p_new, lam=lam_new, chi2=chi2_new, consec=consec, nrej=nrej,
+                conv=conv, stall=stall_now, nst=state["nst"] + 1,
+                accept=accept, chi2_try=chi2_try, p_try=p_try)
+
+
+def run_island(p0, isl: Island, o: LMOptions, max_steps=None, trace=False):
+    """Fresh-chunk LM loop for one island (run_class_chunk, :2890-2977):
+    lam = 1e-3, chi2 = chi2(p0), until converged or ``max_steps``."""
+    max_steps = o.max_steps if max_steps is None else max_steps
+    st = dict(p=np.asarray(p0, np.float64).copy(), lam=LM_LAMBDA0,
+              chi2=chi2_of(p0, isl, o), consec=0, nrej=0, conv=False,
+              stall=False, nst=0)
+    hist = []
+    for _ in range(max_steps):
+        if st["conv"]:
+            break
+        st = one_step(st, isl, o)
+        if trace:
+            hist.append(dict(st))
+    st["history"] = hist
+    return st
+
+
+def final_fluxes(p, isl: Island, o: LMOptions):
+    """Per-band fluxes and sqrt(diag(A^-1)) (island_solver.py:1467-1482)."""
+    f_all = np.zeros((isl.nb, isl.N))
+    e_all = np.zeros((isl.nb, isl.N))
+    for b in range(isl.nb):
+        m, _ = models_band(p, isl, b, False)
+        f, A64, _ = fluxes_band(m, isl.data[b], isl.w[b], isl.mmask, o.reg)
+        f_all[b] = f
+        e_all[b] = np.sqrt(np.maximum(np.diag(np.linalg.inv(A64)), 1e-30))
+    return f_all, e_all
+
+
+def final_errors(p, isl: Island, o: LMOptions):
+    """Marginal variances of the nonlinear parameters from the inverse of
+    the keep-masked GN Hessian (island_solver.py:1484-1547, no priors).
+    Returns (N, 7): var_x, var_y, var_lr, var_e1, var_e2, var_u, cov_e1e2."""
+    N = isl.N
+    nP = 6 * N
+    Hl, _, _, live = hg(p, isl, o)
+    H = np.eye(nP)
+    idx = np.array([6 * k + j for (k, j) in live], dtype=np.int64)
+    if idx.size:
+        H[np.ix_(idx, idx)] = Hl
+    S = np.linalg.inv(H)
+    var6 = np.diag(S).reshape(N, 6)
+    s6 = 6 * np.arange(N)
+    cov = S[s6 + 3, s6 + 4]
+    return np.concatenate([var6, cov[:, None]], axis=1)

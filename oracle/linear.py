@@ -1,0 +1,119 @@
+"""FP64 restatement of the per grid-band linear flux + background solve
+(TEST INFRASTRUCTURE).
+
+Reference: ``tract7dt/fit/grid_gpu/forced_phot_sparse.py`` --
+``render_stamps`` (:126-220, unit-sum renorm for all-star grids),
+``stamp_top_left`` (:255-264), ``build_neighbor_list`` (:272-334),
+``solver_inner`` (:342-554); background basis
+``grid_gpu/bg_polynomial.py:125-178``.
+
+Parity pins: tests/test_oracle_linear.py solves the same system as a dense
+weighted least-squares problem over the full design matrix with
+``numpy.linalg.lstsq`` (exact when no neighbour list truncates), and checks
+the zero-coverage known answer of the reference's
+``tests/test_output_contract_nodata.py:142-222`` (a source whose stamp has
+no valid pixel gets flux 0 while covered sources stay finite).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import core
+
+
+def bg_basis(h, w, x0, y0, xo, xs, yo, ys, order=1):
+    """(h*w, M) polynomial background basis in normalised full-image
+    coordinates (bg_polynomial.py:125-178)."""
+    yy, xx = np.mgrid[:h, :w].astype(np.float64)
+    X = (xx + x0 - xo) / xs
+    Y = (yy + y0 - yo) / ys
+    cols = [(X ** m * Y ** n).ravel() for (m, n) in core.terms_for_order(order)]
+    return np.stack(cols, axis=-1)
+
+
+def neighbor_list(px, py, H, W, k_max=32):
+    """Sources whose stamp boxes overlap (|dx| < W and |dy| < H), the k_max
+    nearest kept (forced_phot_sparse.py:272-334).  Brute force O(n^2)."""
+    n = px.size
+    idx = np.full((n, k_max), -1, np.int64)
+    msk = np.zeros((n, k_max), bool)
+    for s in range(n):
+        dx = np.abs(px - px[s])
+        dy = np.abs(py - py[s])
+        cand = np.where((dx < W) & (dy < H))[0]
+        cand = cand[cand != s]
+        if cand.size > k_max:
+            d2 = (px[cand] - px[s]) ** 2 + (py[cand] - py[s]) ** 2
+            cand = cand[np.argsort(d2, kind="stable")[:k_max]]
+        idx[s, :cand.size] = cand
+        msk[s, :cand.size] = True
+    return idx, msk
+
+
+def render_grid_stamps(psf, px, py, gx0, gy0, stype, lr, e1, e2, sn):
+    """Unit-flux stamps for the linear solve: all-star grids renormalise the
+    raw PSF to unit sum before the shift; grids with any extended source do
+    not (forced_phot_sparse.py:624-637)."""
+    all_star = bool(np.all(np.asarray(stype) == core.SOURCE_STAR))
+    out = []
+    for i in range(px.size):
+        s, _ = core.member_stamp(psf, float(px[i]), float(py[i]), gx0, gy0,
+                                 int(stype[i]), float(lr[i]), float(e1[i]),
+                                 float(e2[i]), float(sn[i]), renorm=all_star)
+        out.append(s)
+    return np.stack(out) if out else np.zeros((0,) + psf["coefs"].shape[:2])
+
+
+def solve_band(data, invvar, pix_good, stamps, ix_top, iy_top, nbr_idx, nbr_mask,
+               basis, reg=1e-9):
+    """One grid-band solve (forced_phot_sparse.py:342-554).
+    Returns fluxes (n,), bg (M,), flux_var (n,), bg_var (M,)."""
+    h, w = data.shape
+    n, T = stamps.shape[0], stamps.shape[1]
+    M = basis.shape[1]
+    wg = np.where(pix_good, invvar, 0.0)
+    Bg = basis.reshape(h, w, M)
+    ky = np.arange(T)
+    diag = np.zeros(n)
+    tdv = np.zeros(n)
+    tb = np.zeros((n, M))
+    eff = np.zeros_like(stamps)
+    wu = np.zeros_like(stamps)
+    for s in range(n):
+        ys = iy_top[s] + ky[:, None]
+        xs = ix_top[s] + ky[None, :]
+        inb = (ys >= 0) & (ys < h) & (xs >= 0) & (xs < w)
+        yc, xc = np.clip(ys, 0, h - 1), np.clip(xs, 0, w - 1)
+        wu[s] = np.where(inb, wg[yc, xc], 0.0)
+        du = data[yc, xc]
+        bu = Bg[yc, xc, :]
+        eff[s] = np.where(inb, stamps[s], 0.0)
+        ws = wu[s] * eff[s]
+        diag[s] = np.sum(ws * eff[s])
+        tdv[s] = np.sum(ws * du)
+        tb[s] = np.einsum("ij,ijm->m", ws, bu)
+    A = np.zeros((n + M, n + M))
+    A[np.arange(n), np.arange(n)] = diag
+    for s in range(n):
+        for kk in range(nbr_idx.shape[1]):
+            if not nbr_mask[s, kk]:
+                continue
+            t = nbr_idx[s, kk]
+            di = iy_top[s] - iy_top[t]
+            dj = ix_top[s] - ix_top[t]
+            tky = ky[:, None] + di
+            tkx = ky[None, :] + dj
+            in_t = (tky >= 0) & (tky < T) & (tkx >= 0) & (tkx < T)
+            tv = np.where(in_t, eff[t][np.clip(tky, 0, T - 1), np.clip(tkx, 0, T - 1)], 0.0)
+            A[s, t] += np.sum(wu[s] * eff[s] * tv)
+    A[:n, n:] = tb
+    A[n:, :n] = tb.T
+    bw = basis * wg.reshape(-1)[:, None]
+    A[n:, n:] = bw.T @ basis
+    rhs = np.concatenate([tdv, bw.T @ data.reshape(-1)])
+    dg = np.diag(A)
+    A = A + np.diag(reg * np.maximum(np.abs(dg), 1e-30))
+    x = np.linalg.solve(A, rhs)
+    inv = np.linalg.inv(A)
+    var = np.diag(inv)
+    return x[:n], x[n:], var[:n], var[n:]

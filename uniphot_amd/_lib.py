@@ -1,0 +1,117 @@
+"""ctypes binding of ``libuniphot_amd.so`` (the C ABI in include/uniphot_amd.h).
+
+The library is built in-tree by ``python -m uniphot_amd.build``.  Loading
+fails loudly when it is missing: there is no CPU fallback on the product
+path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libuniphot_amd.so"
+
+UPH_OK, UPH_ERR_INVALID_ARGUMENT, UPH_ERR_OOM, UPH_ERR_INTERNAL = 0, 1, 2, 3
+UPH_PRECISION_FP32, UPH_PRECISION_FP64 = 0, 1
+
+# Every exported entry point declared in include/uniphot_amd.h.
+EXPORTS = (
+    "uph_version", "uph_last_error", "uph_island_scratch_per_block", "uph_island_fit",
+    "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
+)
+
+
+class UphError(RuntimeError):
+    """A non-OK status from the C ABI; ``code`` is the status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"uniphot_amd status {code}: {msg}")
+        self.code = code
+
+
+class UphOOM(UphError):
+    """UPH_ERR_OOM: the caller may halve the batch and retry."""
+
+
+class LMOptions(C.Structure):
+    _fields_ = [("reg", C.c_float), ("pos_step_max", C.c_float),
+                ("shape_step_max", C.c_float), ("un_step_max", C.c_float),
+                ("ftol_rel", C.c_float), ("step_tol", C.c_float),
+                ("shape_step_tol", C.c_float), ("max_steps", C.c_int)]
+
+
+class IslandClass(C.Structure):
+    _fields_ = [("n_isl", C.c_int), ("N", C.c_int), ("B", C.c_int), ("T", C.c_int),
+                ("n_terms", C.c_int), ("n_bands", C.c_int), ("n_live_max", C.c_int),
+                ("Hp", C.c_int), ("Wp", C.c_int), ("mixed", C.c_int)]
+
+
+_P = C.c_void_p
+
+
+class IslandInputs(C.Structure):
+    _fields_ = [(n, _P) for n in (
+        "dstack", "wstack", "psf_coefs", "psf_geo", "sersic", "ox", "oy", "x0b", "y0b",
+        "stype", "n_frozen", "free_n", "mmask", "keep", "p0", "lower", "upper", "order")]
+
+
+class IslandOutputs(C.Structure):
+    _fields_ = [(n, _P) for n in (
+        "p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
+        "n_good")]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load (once) and return the library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m uniphot_amd.build` "
+            "(the fit path has no CPU fallback)")
+    L = C.CDLL(str(LIB_PATH))
+    L.uph_version.restype = C.c_int
+    L.uph_last_error.restype = C.c_char_p
+    L.uph_island_scratch_per_block.restype = C.c_size_t
+    L.uph_island_scratch_per_block.argtypes = [C.c_int] * 5
+    L.uph_island_fit.restype = C.c_int
+    L.uph_island_fit.argtypes = [C.POINTER(IslandClass), C.POINTER(IslandInputs),
+                                 C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,
+                                 _P, C.c_size_t, _P, _P]
+    L.uph_render_stamps.restype = C.c_int
+    L.uph_render_stamps.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P,
+                                    _P, _P, _P, _P, _P, C.c_int, _P, _P]
+    L.uph_band_scratch_bytes.restype = C.c_size_t
+    L.uph_band_scratch_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.uph_band_solve.restype = C.c_int
+    L.uph_band_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int,
+                                 C.c_int, C.c_int, _P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
+                                 _P, _P, _P, _P, C.c_float, _P, _P, _P, _P, _P, C.c_size_t, _P]
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status == UPH_OK:
+        return
+    msg = lib().uph_last_error().decode(errors="replace")
+    if status == UPH_ERR_OOM:
+        raise UphOOM(status, msg)
+    raise UphError(status, msg)
+
+
+def ptr(t) -> int | None:
+    """Device pointer of a torch tensor (None for None)."""
+    if t is None:
+        return None
+    return int(t.data_ptr())
+
+
+def stream_ptr(stream=None) -> int | None:
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)

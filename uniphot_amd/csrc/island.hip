@@ -1,0 +1,716 @@
+// Persistent per-island VarPro Levenberg-Marquardt kernel (gfx950).
+//
+// One 256-thread workgroup owns one island at a time and pulls the next
+// island from a device work queue, so islands of very different cost share
+// one launch and the LM loop never returns to the host.  Per band the
+// workgroup gathers the island window, renders every member's unit-flux
+// model and its live tangent planes into per-workgroup scratch (L2
+// resident), reduces the weighted Gram matrix of [data, models, tangents],
+// solves the member fluxes in float64, forms the residual, and accumulates
+// the flux-marginalised Gauss-Newton (H, g) from Gram identities:
+//   D_j = A^-1 (e_k s_j - f_k U_j)
+//   H_ij += D_i.A D_j + f_kj D_i.U_j + f_ki D_j.U_i + f_ki f_kj G_ij
+//   g_j  += -(D_j.c + f_kj s_j)
+// with A = M^T W M, U = M^T W T, G = T^T W T, c = M^T W R, s = T^T W R.
+// Algorithm followed: tract7dt/fit/grid_gpu/island_solver.py models_band
+// (:480-562), _fluxes_band/residual_band/chi2_of (:570-684), hg_scan
+// (:812-865), one_step (:921-1002), final_fluxes (:1467-1482).
+#include <hip/hip_runtime.h>
+
+#include "uph_capi_internal.hpp"
+#include "uph_device.hpp"
+#include "uph_galaxy.hpp"
+#include "uph_island.hpp"
+
+namespace uph {
+
+template <typename Real>
+struct IslandSh {
+  float p[kMaxLive], ptry[kMaxLive], lo[kMaxLive], hi[kMaxLive];
+  int mem[kMaxMembers];  // slot index of real member r
+  int st[kMaxMembers];
+  float nfz[kMaxMembers];
+  uint8_t fn[kMaxMembers];
+  int live_r[kMaxLive];  // real-member index of live parameter
+  int live_j[kMaxLive];  // parameter 0..5
+  int first_live[kMaxMembers + 1];
+  Real taps[28];
+  Real bas[63];          // PSF basis, d/dX, d/dY (21 each)
+  Real fR[kMaxMembers];  // member fluxes rounded to Real
+  double red[UPH_WAVES * 32];
+  double redd[UPH_WAVES];
+  int Nr, nL, item, pivot;
+  // LM state
+  float lam, chi2, chi2_try;
+  int consec, nrej, nst, conv, stall;
+};
+
+struct Small {
+  double *Araw, *LU, *b, *f, *c, *U, *Gt, *s, *D, *AD, *H, *g, *Hd, *delta, *colws;
+  int *pivA, *pivH;
+};
+
+__device__ inline Small carve_small(char* base, int N, int nL) {
+  double* d = reinterpret_cast<double*>(base);
+  Small s;
+  s.Araw = d; d += N * N;
+  s.LU = d; d += N * N;
+  s.b = d; d += N;
+  s.f = d; d += N;
+  s.c = d; d += N;
+  s.U = d; d += N * nL;
+  s.Gt = d; d += nL * nL;
+  s.s = d; d += nL;
+  s.D = d; d += nL * N;
+  s.AD = d; d += nL * N;
+  s.H = d; d += nL * nL;
+  s.g = d; d += nL;
+  s.Hd = d; d += nL * nL;
+  s.delta = d; d += nL;
+  s.colws = d; d += 256 * (nL > N ? nL : N);
+  int* ip = reinterpret_cast<int*>(d + 8);
+  s.pivA = ip; ip += N;
+  s.pivH = ip;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Weighted pair reductions: out = sum_p W[p] * Z[a][p] * Z[b][p].
+// GRAM mode: all a <= b over planes [0, C) (symmetric fill of out, C x C).
+// VEC mode: pairs (q, zr) for q in [0, C) into out[q].
+// Chunks of 32 pairs; per-thread split-K accumulation, fixed-order
+// wave + block reduction (deterministic).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void pair_of(int e, int C, int& a, int& b) {
+  a = 0;
+  while (e >= C - a) {
+    e -= C - a;
+    ++a;
+  }
+  b = a + e;
+}
+
+template <typename Real, bool GRAM>
+__device__ void weighted_pairs(const Real* __restrict__ W, const Real* __restrict__ Z, int P,
+                               int C, int z0, int zr, double* out, double* red) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int E = GRAM ? C * (C + 1) / 2 : C;
+  int a0 = 0, b0 = 0;  // first pair of the current chunk (GRAM)
+  for (int e0 = 0; e0 < E; e0 += 32) {
+    const int ne = min(32, E - e0);
+    int ia[32], ib[32];
+    {
+      int a = a0, b = b0;
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        if (GRAM) {
+          ia[q] = z0 + a;
+          ib[q] = z0 + b;
+          if (++b >= C) {
+            ++a;
+            b = a;
+          }
+        } else {
+          ia[q] = z0 + e0 + q;
+          ib[q] = zr;
+        }
+      }
+      if (GRAM) {
+        a0 = a;
+        b0 = b;
+      }
+    }
+    // products in Real, accumulation in float64 (exact-product FMA): the
+    // pixel sums carry no float32 summation error into the flux solve.
+    double acc[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q) acc[q] = 0.0;
+    for (int p = tid; p < P; p += UPH_BLOCK) {
+      const Real wv = W[p];
+#pragma unroll
+      for (int q = 0; q < 32; ++q) {
+        if (q < ne) {
+          const Real za = Z[(size_t)ia[q] * P + p];
+          const Real zb = Z[(size_t)ib[q] * P + p];
+          acc[q] = fma(double(wv * za), double(zb), acc[q]);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 32; ++q) {
+      double v = wave_sum(acc[q]);
+      if (lane == 0) red[wid * 32 + q] = v;
+    }
+    __syncthreads();
+    if (tid < ne) {
+      double s = red[tid];
+#pragma unroll
+      for (int w = 1; w < UPH_WAVES; ++w) s += red[w * 32 + tid];
+      if (GRAM) {
+        int pa, pb;
+        pair_of(e0 + tid, C, pa, pb);
+        out[pa * C + pb] = s;
+        out[pb * C + pa] = s;
+      } else {
+        out[e0 + tid] = s;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Star member: PSF polynomial at (x, y), separable Lanczos-3 shift by the
+// remainder of (x - x0b, y - y0b), placed on the B x B window at
+// round(pos - origin) - T//2 (source_model.py:322-328, island_solver.py:512).
+// Only the part of the stamp that lands in the window is evaluated.  With
+// tangents, also d/dx and d/dy (polynomial derivative + tap derivative).
+// ---------------------------------------------------------------------------
+template <typename Real>
+__device__ void render_star(const IslandArgs& a, int b, float xf, float yf, float x0f, float y0f,
+                            Real* __restrict__ m_out, Real* __restrict__ tx_out,
+                            Real* __restrict__ ty_out, Real* __restrict__ ws, Real* taps,
+                            Real* bas) {
+  const int T = a.T, B = a.B, nt = a.nt, tid = threadIdx.x;
+  const bool grad = tx_out != nullptr;
+  const float* geo = a.psf_geo + 4 * b;
+  const Real x = Real(xf), y = Real(yf);
+  const Real xg = Real(xf - x0f), yg = Real(yf - y0f);
+  const Real rxg = rint(xg), ryg = rint(yg);
+  const Real dx = xg - rxg, dy = yg - ryg;
+  const int ix = int(rxg) - T / 2, iy = int(ryg) - T / 2;
+  const Real xs = Real(geo[1]), ys = Real(geo[3]);
+  if (tid == 0) lanczos_taps(dx, taps, grad ? taps + 7 : (Real*)nullptr);
+  if (tid == 64) lanczos_taps(dy, taps + 14, grad ? taps + 21 : (Real*)nullptr);
+  if (tid == 128) {
+    const Real X = (x - Real(geo[0])) / xs, Y = (y - Real(geo[2])) / ys;
+    psf_basis(X, Y, nt, bas, bas + 21, bas + 42);
+  }
+  __syncthreads();
+  const int wx0 = max(0, ix), wx1 = min(B, ix + T), wy0 = max(0, iy), wy1 = min(B, iy + T);
+  if (wx1 <= wx0 || wy1 <= wy0) return;  // uniform across the block
+  const int sx0 = wx0 - ix, sx1 = wx1 - ix, sy0 = wy0 - iy, sy1 = wy1 - iy;
+  const int rx0 = max(0, sx0 - 3), rx1 = min(T, sx1 + 3);
+  const int ry0 = max(0, sy0 - 3), ry1 = min(T, sy1 + 3);
+  const int RW = rx1 - rx0, RH = ry1 - ry0, SW = sx1 - sx0, SH = sy1 - sy0;
+  const float* Cf = a.psf_coefs + (size_t)b * nt * T * T;
+  const int TT = T * T;
+  Real* P = ws;
+  Real* Px = ws + TT;
+  Real* Py = ws + 2 * TT;
+  Real* H0 = ws + 3 * TT;
+  Real* Hx = ws + 4 * TT;
+  Real* Hy = ws + 5 * TT;
+  for (int e = tid; e < RH * RW; e += UPH_BLOCK) {
+    const int r = ry0 + e / RW, c = rx0 + e % RW;
+    const float* cp = Cf + r * T + c;
+    Real s = 0, sx = 0, sy = 0;
+    for (int t = 0; t < nt; ++t) {
+      const Real cv = Real(cp[(size_t)t * TT]);
+      s = fma(cv, bas[t], s);
+      if (grad) {
+        sx = fma(cv, bas[21 + t], sx);
+        sy = fma(cv, bas[42 + t], sy);
+      }
+    }
+    P[e] = s;
+    if (grad) {
+      Px[e] = sx / xs;
+      Py[e] = sy / ys;
+    }
+  }
+  __syncthreads();
+  Real Lx[7], dLx[7], Ly[7], dLy[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    Lx[k] = taps[k];
+    Ly[k] = taps[14 + k];
+    dLx[k] = grad ? taps[7 + k] : Real(0);
+    dLy[k] = grad ? taps[21 + k] : Real(0);
+  }
+  for (int e = tid; e < RH * SW; e += UPH_BLOCK) {
+    const int rr = e / SW, c = sx0 + e % SW;
+    Real h = 0, hx = 0, hy = 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int cc = c + k - 3;
+      if (cc < rx0 || cc >= rx1) continue;
+      const int o = rr * RW + cc - rx0;
+      const Real v = P[o];
+      h = fma(Lx[k], v, h);
+      if (grad) {
+        hx = fma(Lx[k], Px[o], fma(dLx[k], v, hx));
+        hy = fma(Lx[k], Py[o], hy);
+      }
+    }
+    H0[e] = h;
+    if (grad) {
+      Hx[e] = hx;
+      Hy[e] = hy;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < SH * SW; e += UPH_BLOCK) {
+    const int r = sy0 + e / SW, cc = e % SW;
+    Real m = 0, tx = 0, ty = 0;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int rr = r + k - 3;
+      if (rr < ry0 || rr >= ry1) continue;
+      const int o = (rr - ry0) * SW + cc;
+      const Real h = H0[o];
+      m = fma(Ly[k], h, m);
+      if (grad) {
+        tx = fma(Ly[k], Hx[o], tx);
+        ty = fma(Ly[k], Hy[o], fma(dLy[k], h, ty));
+      }
+    }
+    const int wi = (r + iy) * B + (cc + sx0 + ix);
+    m_out[wi] = m;
+    if (grad) {
+      tx_out[wi] = tx;
+      ty_out[wi] = ty;
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// One band of one island: window copy, member renders, Gram, VarPro fluxes,
+// residual, chi2 and (with grads) accumulation of H, g.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int b,
+                          const float* pv, bool grads, bool accumulate, Real* planes, Real* ws,
+                          double* G, const Small& sm, float* flux_out, float* ferr_out,
+                          int* ngood) {
+  const int B = a.B, P = B * B, tid = threadIdx.x;
+  const int Nr = sh.Nr, nL = grads ? sh.nL : 0;
+  Real* Wp = planes;
+  Real* Rp = planes + P;
+  Real* Dp = planes + 2 * P;
+  Real* Mp = planes + 3 * P;
+  Real* Tp = Mp + (size_t)Nr * P;
+  const int ox = a.ox[isl], oy = a.oy[isl];
+  int cnt = 0;
+  for (int e = tid; e < P; e += UPH_BLOCK) {
+    const int r = e / B, c = e % B;
+    const size_t gi = ((size_t)b * a.Hp + oy + r) * a.Wp + ox + c;
+    const float wv = a.wstack[gi];
+    Wp[e] = Real(wv);
+    Dp[e] = Real(a.dstack[gi]);
+    cnt += wv > 0.0f;
+  }
+  if (ngood) {
+    double tot = block_sum(double(cnt), sh.redd);
+    if (tid == 0) *ngood += int(tot);
+  }
+  for (size_t e = tid; e < (size_t)(Nr + nL) * P; e += UPH_BLOCK) Mp[e] = Real(0);
+  __syncthreads();
+  const float x0f = a.x0b[isl], y0f = a.y0b[isl];
+  for (int r = 0; r < Nr; ++r) {
+    const int k = sh.mem[r];
+    const float xk = pv[6 * k], yk = pv[6 * k + 1];
+    Real* tx = nullptr;
+    Real* ty = nullptr;
+    if (grads) {
+      // star members: live params are x, y (in that order) when kept
+      for (int l = sh.first_live[r]; l < sh.first_live[r + 1]; ++l) {
+        if (sh.live_j[l] == 0) tx = Tp + (size_t)l * P;
+        if (sh.live_j[l] == 1) ty = Tp + (size_t)l * P;
+      }
+    }
+    if (sh.st[r] == 0) {
+      Real* wsx = ws;
+      if (grads && (tx == nullptr || ty == nullptr)) {
+        // a frozen coordinate: render both, discard into the R plane
+        if (tx == nullptr) tx = Rp;
+        if (ty == nullptr) ty = Rp;
+      }
+      render_star<Real>(a, b, xk, yk, x0f, y0f, Mp + (size_t)r * P, grads ? tx : nullptr,
+                        grads ? ty : nullptr, wsx, sh.taps, sh.bas);
+    } else {
+      render_galaxy<Real>(a, b, pv + 6 * k, sh.st[r], sh.nfz[r], sh.fn[r], x0f, y0f,
+                          Mp + (size_t)r * P, grads ? Tp : nullptr, sh.live_j, sh.first_live[r],
+                          sh.first_live[r + 1], ws, sh.taps, sh.bas);
+    }
+  }
+  // Gram of [d, models, tangents] under W.
+  const int C = 1 + Nr + nL;
+  weighted_pairs<Real, true>(Wp, planes, P, C, 2, 0, G, sh.red);
+  // VarPro fluxes (float64 LU), A64 = A + reg I on real members.
+  const double reg = double(a.reg);
+  for (int e = tid; e < Nr * Nr; e += UPH_BLOCK) {
+    const int i = e / Nr, j = e % Nr;
+    const double v = G[(1 + i) * C + 1 + j];
+    sm.Araw[e] = v;
+    sm.LU[e] = v + (i == j ? reg : 0.0);
+  }
+  for (int i = tid; i < Nr; i += UPH_BLOCK) sm.b[i] = G[1 + i];
+  __syncthreads();
+  block_lu(sm.LU, Nr, sm.pivA, &sh.pivot);
+  if (tid == 0) {
+    for (int i = 0; i < Nr; ++i) sm.f[i] = sm.b[i];
+    lu_solve_one(sm.LU, sm.pivA, Nr, sm.f);
+    for (int i = 0; i < Nr; ++i) {
+      sh.fR[i] = Real(sm.f[i]);
+      sm.f[i] = double(sh.fR[i]);
+    }
+  }
+  __syncthreads();
+  if (flux_out) {
+    // final fluxes and sqrt(diag(A^-1))
+    lu_inverse_diag(sm.LU, sm.pivA, Nr, sm.s, sm.colws);
+    __syncthreads();
+    for (int r = tid; r < Nr; r += UPH_BLOCK) {
+      flux_out[sh.mem[r]] = float(sh.fR[r]);
+      ferr_out[sh.mem[r]] = float(sqrt(fmax(sm.s[r], 1e-30)));
+    }
+    __syncthreads();
+    return Real(0);
+  }
+  // residual R = d - sum_k f_k m_k ; chi2 = sum (sqrt(w) R)^2
+  double acc = 0.0;
+  for (int p = tid; p < P; p += UPH_BLOCK) {
+    Real model = 0;
+    for (int r = 0; r < Nr; ++r) model = fma(sh.fR[r], Mp[(size_t)r * P + p], model);
+    const Real R = Dp[p] - model;
+    const Real rw = sqrt(Wp[p]) * R;
+    acc = fma(double(rw), double(rw), acc);
+    Rp[p] = R;
+  }
+  const Real chi2b = Real(block_sum(acc, sh.red));
+  if (!grads) return chi2b;
+  // c = M^T W R, s = T^T W R (one pass over the Nr + nL planes)
+  weighted_pairs<Real, false>(Wp, planes, P, Nr + nL, 3, 1, sm.colws, sh.red);
+  for (int i = tid; i < Nr + nL; i += UPH_BLOCK) {
+    if (i < Nr)
+      sm.c[i] = sm.colws[i];
+    else
+      sm.s[i - Nr] = sm.colws[i];
+  }
+  for (int e = tid; e < Nr * nL; e += UPH_BLOCK) {
+    const int r = e / nL, j = e % nL;
+    sm.U[e] = G[(1 + r) * C + 1 + Nr + j];
+  }
+  for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
+    const int i = e / nL, j = e % nL;
+    sm.Gt[e] = G[(1 + Nr + i) * C + 1 + Nr + j];
+  }
+  __syncthreads();
+  // D_j = A64^-1 (e_k s_j - f_k U_j), one right-hand side per thread
+  for (int j = tid; j < nL; j += UPH_BLOCK) {
+    double* x = sm.D + (size_t)j * Nr;
+    const int kr = sh.live_r[j];
+    for (int r = 0; r < Nr; ++r) x[r] = -sm.f[kr] * sm.U[r * nL + j];
+    x[kr] += sm.s[j];
+    lu_solve_one(sm.LU, sm.pivA, Nr, x);
+  }
+  __syncthreads();
+  for (int e = tid; e < nL * Nr; e += UPH_BLOCK) {
+    const int j = e / Nr, r = e % Nr;
+    double s = 0.0;
+    for (int q = 0; q < Nr; ++q) s += sm.Araw[r * Nr + q] * sm.D[j * Nr + q];
+    sm.AD[e] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
+    const int i = e / nL, j = e % nL;
+    const double* Di = sm.D + (size_t)i * Nr;
+    const double* Dj = sm.D + (size_t)j * Nr;
+    const double* ADj = sm.AD + (size_t)j * Nr;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+    for (int r = 0; r < Nr; ++r) {
+      t0 += Di[r] * ADj[r];
+      t1 += Di[r] * sm.U[r * nL + j];
+      t2 += Dj[r] * sm.U[r * nL + i];
+    }
+    const double fi = sm.f[sh.live_r[i]], fj = sm.f[sh.live_r[j]];
+    const double h = t0 + fj * t1 + fi * t2 + fi * fj * sm.Gt[e];
+    sm.H[e] = accumulate ? sm.H[e] + h : h;
+  }
+  for (int j = tid; j < nL; j += UPH_BLOCK) {
+    const double* Dj = sm.D + (size_t)j * Nr;
+    double t = 0.0;
+    for (int r = 0; r < Nr; ++r) t += Dj[r] * sm.c[r];
+    const double gj = -(t + sm.f[sh.live_r[j]] * sm.s[j]);
+    sm.g[j] = accumulate ? sm.g[j] + gj : gj;
+  }
+  __syncthreads();
+  return chi2b;
+}
+
+template <typename Real>
+__device__ float chi2_all(const IslandArgs& a, IslandSh<Real>& sh, int isl, const float* pv,
+                          Real* planes, Real* ws, double* G, const Small& sm) {
+  Real c = 0;
+  for (int b = 0; b < a.nb; ++b)
+    c += eval_band<Real>(a, sh, isl, b, pv, false, false, planes, ws, G, sm, nullptr, nullptr,
+                         nullptr);
+  return float(c);
+}
+
+template <typename Real>
+__global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
+  __shared__ IslandSh<Real> sh;
+  const int tid = threadIdx.x;
+  const int N = a.N, nP = 6 * N;
+  const IslandScratch lay = island_scratch_layout(N, a.B, a.T, a.nL_max, sizeof(Real));
+  char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
+  Real* planes = reinterpret_cast<Real*>(base + lay.planes);
+  Real* ws = reinterpret_cast<Real*>(base + lay.render);
+  double* G = reinterpret_cast<double*>(base + lay.gram);
+  const Small sm = carve_small(base + lay.small, N, a.nL_max);
+  const float clipv[6] = {a.pos_step_max, a.pos_step_max, a.shape_step_max,
+                          a.shape_step_max, a.shape_step_max, a.un_step_max};
+  for (;;) {
+    if (tid == 0) sh.item = atomicAdd(a.counter, 1);
+    __syncthreads();
+    const int item = sh.item;
+    if (item >= a.n_isl) break;
+    const int isl = a.order ? a.order[item] : item;
+    // ---- island description into LDS ----
+    for (int q = tid; q < nP; q += UPH_BLOCK) {
+      const size_t g = (size_t)isl * nP + q;
+      sh.p[q] = a.p0[g];
+      sh.lo[q] = a.lower[g];
+      sh.hi[q] = a.upper[g];
+    }
+    if (tid == 0) {
+      int nr = 0, nl = 0;
+      for (int k = 0; k < N; ++k) {
+        const size_t g = (size_t)isl * N + k;
+        if (!a.mmask[g]) continue;
+        sh.mem[nr] = k;
+        sh.st[nr] = a.stype[g];
+        sh.nfz[nr] = a.n_frozen[g];
+        sh.fn[nr] = a.free_n[g];
+        sh.first_live[nr] = nl;
+        for (int j = 0; j < 6; ++j) {
+          if (a.keep[g * 6 + j]) {
+            sh.live_r[nl] = nr;
+            sh.live_j[nl] = j;
+            ++nl;
+          }
+        }
+        ++nr;
+      }
+      sh.first_live[nr] = nl;
+      sh.Nr = nr;
+      sh.nL = nl;
+      sh.lam = 1e-3f;
+      sh.consec = sh.nrej = sh.nst = 0;
+      sh.conv = sh.stall = 0;
+    }
+    __syncthreads();
+    if (sh.Nr == 0) sh.conv = 1;  // nothing to fit (all slots padded)
+    __syncthreads();
+    if (sh.Nr > 0) {
+      const float c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, G, sm);
+      if (tid == 0) sh.chi2 = c0;
+      __syncthreads();
+    } else if (tid == 0) {
+      sh.chi2 = 0.0f;
+    }
+    for (int it = 0; it < a.max_steps; ++it) {
+      if (sh.conv) break;
+      const int nL = sh.nL;
+      // H, g at p (un-damped, un-masked over the live set)
+      for (int b = 0; b < a.nb; ++b)
+        eval_band<Real>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, G, sm, nullptr, nullptr,
+                        nullptr);
+      // damped float64 solve: (H + lam diag(H) + 1e-8 I) delta = -g
+      const double lam = double(sh.lam);
+      for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
+        const int i = e / nL, j = e % nL;
+        double v = sm.H[e];
+        if (i == j) v = v + lam * sm.H[e] + 1e-8;
+        sm.Hd[e] = v;
+      }
+      __syncthreads();
+      block_lu(sm.Hd, nL, sm.pivH, &sh.pivot);
+      if (tid == 0) {
+        for (int j = 0; j < nL; ++j) sm.delta[j] = sm.g[j];
+        lu_solve_one(sm.Hd, sm.pivH, nL, sm.delta);
+      }
+      __syncthreads();
+      for (int q = tid; q < nP; q += UPH_BLOCK) sh.ptry[q] = sh.p[q];
+      __syncthreads();
+      if (tid == 0) {
+        for (int l = 0; l < nL; ++l) {
+          const int q = 6 * sh.mem[sh.live_r[l]] + sh.live_j[l];
+          float d = float(-sm.delta[l]);
+          const float cl = clipv[sh.live_j[l]];
+          d = fminf(fmaxf(d, -cl), cl);
+          sh.ptry[q] = sh.p[q] + d;
+        }
+        for (int q = 0; q < nP; ++q) sh.ptry[q] = fminf(fmaxf(sh.ptry[q], sh.lo[q]), sh.hi[q]);
+      }
+      __syncthreads();
+      const float ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, G, sm);
+      if (tid == 0) {
+        const float cur = sh.chi2;
+        const bool accept = ct < cur;
+        float pos_step = 0.0f, shape_step = 0.0f;
+        if (accept) {
+          for (int r = 0; r < sh.Nr; ++r) {
+            const int k = sh.mem[r];
+            const float sx = sh.ptry[6 * k] - sh.p[6 * k];
+            const float sy = sh.ptry[6 * k + 1] - sh.p[6 * k + 1];
+            pos_step = fmaxf(pos_step, hypotf(sx, sy));
+            for (int j = 2; j < 6; ++j)
+              shape_step = fmaxf(shape_step, fabsf(sh.ptry[6 * k + j] - sh.p[6 * k + j]));
+          }
+          for (int q = 0; q < nP; ++q) sh.p[q] = sh.ptry[q];
+          sh.lam = fmaxf(sh.lam * 0.1f, 1e-9f);
+          sh.chi2 = ct;
+        } else {
+          sh.lam = fminf(sh.lam * 10.0f, 1e6f);
+        }
+        const float dchi2 = cur - ct;
+        const bool small = accept && (dchi2 < a.ftol_rel * fmaxf(cur, 1.0f));
+        sh.consec = small ? sh.consec + 1 : (accept ? 0 : sh.consec);
+        const bool step_small = accept && pos_step < a.step_tol && shape_step < a.shape_step_tol;
+        sh.nrej = accept ? 0 : sh.nrej + 1;
+        const bool stall_now = sh.nrej >= 4;
+        sh.stall = stall_now;
+        sh.conv = (sh.consec >= 2) || step_small || stall_now;
+        sh.nst += 1;
+      }
+      __syncthreads();
+    }
+    // ---- final fluxes + outputs ----
+    float* fo = a.flux_out + (size_t)isl * a.nb * N;
+    float* eo = a.ferr_out + (size_t)isl * a.nb * N;
+    for (int e = tid; e < a.nb * N; e += UPH_BLOCK) {
+      fo[e] = 0.0f;
+      eo[e] = 1.0f;
+    }
+    __syncthreads();
+    int ngood = 0;
+    if (sh.Nr > 0) {
+      for (int b = 0; b < a.nb; ++b)
+        eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, G, sm, fo + b * N,
+                        eo + b * N, &ngood);
+    }
+    for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[(size_t)isl * nP + q] = sh.p[q];
+    if (tid == 0) {
+      a.chi2_out[isl] = sh.chi2;
+      a.lam_out[isl] = sh.lam;
+      a.conv_out[isl] = uint8_t(sh.conv);
+      a.stall_out[isl] = uint8_t(sh.stall);
+      a.nst_out[isl] = sh.nst;
+      a.consec_out[isl] = sh.consec;
+      a.nrej_out[isl] = sh.nrej;
+      a.ngood_out[isl] = ngood;
+    }
+    __syncthreads();
+  }
+}
+
+}  // namespace uph
+
+extern "C" size_t uph_island_scratch_per_block(int N, int B, int T, int n_live_max,
+                                               int precision) {
+  const size_t rb = precision == UPH_PRECISION_FP64 ? 8 : 4;
+  size_t base = uph::island_scratch_layout(N, B, T, n_live_max, rb).total;
+  return base + uph::galaxy_scratch_bytes(T, rb);
+}
+
+extern "C" int uph_island_fit(const uph_island_class* cls, const uph_island_inputs* in,
+                              const uph_island_outputs* out, const uph_lm_options* opt,
+                              int precision, void* scratch, size_t scratch_bytes,
+                              int32_t* counter, void* stream) {
+  UPH_CHECK_ARG(cls && in && out && opt, "uph_island_fit: null argument");
+  UPH_CHECK_ARG(cls->N >= 1 && cls->N <= uph::kMaxMembers, "uph_island_fit: N=%d outside 1..%d",
+                cls->N, uph::kMaxMembers);
+  UPH_CHECK_ARG(cls->B >= 16 && cls->B % 16 == 0, "uph_island_fit: B=%d must be a multiple of 16",
+                cls->B);
+  UPH_CHECK_ARG(cls->T >= 7 && cls->T % 2 == 1, "uph_island_fit: T=%d must be odd >= 7", cls->T);
+  UPH_CHECK_ARG(cls->n_terms == 1 || cls->n_terms == 3 || cls->n_terms == 6 ||
+                    cls->n_terms == 10 || cls->n_terms == 15 || cls->n_terms == 21,
+                "uph_island_fit: unsupported n_terms=%d", cls->n_terms);
+  UPH_CHECK_ARG(cls->n_bands >= 1, "uph_island_fit: n_bands=%d", cls->n_bands);
+  UPH_CHECK_ARG(cls->n_live_max >= 1 && cls->n_live_max <= 6 * cls->N,
+                "uph_island_fit: n_live_max=%d outside 1..6N", cls->n_live_max);
+  UPH_CHECK_ARG(precision == UPH_PRECISION_FP32 || precision == UPH_PRECISION_FP64,
+                "uph_island_fit: precision=%d", precision);
+  UPH_CHECK_ARG(cls->Hp >= cls->B && cls->Wp >= cls->B, "uph_island_fit: stack %dx%d < B=%d",
+                cls->Hp, cls->Wp, cls->B);
+  UPH_CHECK_ARG(in->dstack && in->wstack && in->psf_coefs && in->psf_geo && in->ox && in->oy &&
+                    in->x0b && in->y0b && in->stype && in->n_frozen && in->free_n &&
+                    in->mmask && in->keep && in->p0 && in->lower && in->upper,
+                "uph_island_fit: null input pointer");
+  UPH_CHECK_ARG(out->p && out->fluxes && out->ferr && out->chi2 && out->lam && out->conv &&
+                    out->stall && out->nst && out->consec && out->nrej && out->n_good,
+                "uph_island_fit: null output pointer");
+  UPH_CHECK_ARG(opt->max_steps >= 0, "uph_island_fit: max_steps=%d", opt->max_steps);
+  UPH_CHECK_ARG(cls->mixed == 0, "uph_island_fit: classes with extended members are not supported "
+                "by this build");
+  UPH_CHECK_ARG(counter != nullptr && scratch != nullptr, "uph_island_fit: null scratch/counter");
+  if (cls->n_isl <= 0) return UPH_OK;
+  const size_t per = uph_island_scratch_per_block(cls->N, cls->B, cls->T, cls->n_live_max,
+                                                  precision);
+  const size_t nblk_s = scratch_bytes / per;
+  UPH_CHECK_ARG(nblk_s >= 1, "uph_island_fit: scratch of %zu bytes < one workgroup (%zu)",
+                scratch_bytes, per);
+  const int nblk = (int)(nblk_s < (size_t)cls->n_isl ? nblk_s : (size_t)cls->n_isl);
+  uph::IslandArgs a{};
+  a.n_isl = cls->n_isl;
+  a.N = cls->N;
+  a.B = cls->B;
+  a.T = cls->T;
+  a.nt = cls->n_terms;
+  a.nb = cls->n_bands;
+  a.nL_max = cls->n_live_max;
+  a.Hp = cls->Hp;
+  a.Wp = cls->Wp;
+  a.max_steps = opt->max_steps;
+  a.reg = opt->reg;
+  a.pos_step_max = opt->pos_step_max;
+  a.shape_step_max = opt->shape_step_max;
+  a.un_step_max = opt->un_step_max;
+  a.ftol_rel = opt->ftol_rel;
+  a.step_tol = opt->step_tol;
+  a.shape_step_tol = opt->shape_step_tol;
+  a.dstack = in->dstack;
+  a.wstack = in->wstack;
+  a.psf_coefs = in->psf_coefs;
+  a.psf_geo = in->psf_geo;
+  a.sersic = in->sersic;
+  a.ox = in->ox;
+  a.oy = in->oy;
+  a.x0b = in->x0b;
+  a.y0b = in->y0b;
+  a.stype = in->stype;
+  a.n_frozen = in->n_frozen;
+  a.free_n = in->free_n;
+  a.mmask = in->mmask;
+  a.keep = in->keep;
+  a.p0 = in->p0;
+  a.lower = in->lower;
+  a.upper = in->upper;
+  a.order = in->order;
+  a.p_out = out->p;
+  a.flux_out = out->fluxes;
+  a.ferr_out = out->ferr;
+  a.chi2_out = out->chi2;
+  a.lam_out = out->lam;
+  a.conv_out = out->conv;
+  a.stall_out = out->stall;
+  a.nst_out = out->nst;
+  a.consec_out = out->consec;
+  a.nrej_out = out->nrej;
+  a.ngood_out = out->n_good;
+  a.scratch = static_cast<char*>(scratch);
+  a.scratch_per_block = per;
+  a.counter = counter;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int st = uph::hip_status(hipMemsetAsync(counter, 0, sizeof(int32_t), s), "uph_island_fit memset");
+  if (st != UPH_OK) return st;
+  if (precision == UPH_PRECISION_FP32)
+    hipLaunchKernelGGL(uph::island_lm_kernel<float>, dim3(nblk), dim3(UPH_BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL(uph::island_lm_kernel<double>, dim3(nblk), dim3(UPH_BLOCK), 0, s, a);
+  return uph::hip_status(hipGetLastError(), "uph_island_fit launch");
+}

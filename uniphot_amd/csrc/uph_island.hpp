@@ -1,0 +1,80 @@
+// Island VarPro-LM argument block and per-block scratch layout, shared by
+// the kernel and the C-ABI launcher.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+namespace uph {
+
+constexpr int kMaxMembers = 64;     // nsrc class cap (joint fits use <= 32)
+constexpr int kMaxLive = 6 * kMaxMembers;
+
+struct IslandArgs {
+  int n_isl, N, B, T, nt, nb, nL_max, Hp, Wp;
+  int max_steps;
+  float reg, pos_step_max, shape_step_max, un_step_max, ftol_rel, step_tol, shape_step_tol;
+  const float* dstack;
+  const float* wstack;
+  const float* psf_coefs;  // (nb, nt, T, T)
+  const float* psf_geo;    // (nb, 4): x_origin, x_scale, y_origin, y_scale
+  const float* sersic;     // Sersic table block (see uph_galaxy.hpp), may be null
+  const int* ox;
+  const int* oy;
+  const float* x0b;
+  const float* y0b;
+  const int* stype;
+  const float* n_frozen;
+  const uint8_t* free_n;
+  const uint8_t* mmask;
+  const uint8_t* keep;
+  const float* p0;
+  const float* lower;
+  const float* upper;
+  const int* order;
+  float* p_out;
+  float* flux_out;
+  float* ferr_out;
+  float* chi2_out;
+  float* lam_out;
+  uint8_t* conv_out;
+  uint8_t* stall_out;
+  int* nst_out;
+  int* consec_out;
+  int* nrej_out;
+  int* ngood_out;
+  char* scratch;
+  size_t scratch_per_block;
+  int* counter;
+};
+
+__host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Per-block scratch layout (bytes).  Planes are plane-major B*B arrays:
+// [w, d, R, models (N), tangents (nL_max)].
+struct IslandScratch {
+  size_t planes, render, gram, small, total;
+  int cmax;  // Gram columns: d + models + tangents
+};
+
+__host__ __device__ inline IslandScratch island_scratch_layout(int N, int B, int T, int nL,
+                                                               size_t real_bytes) {
+  IslandScratch s;
+  size_t off = 0;
+  s.planes = off;
+  off = align256(off + (size_t)(3 + N + nL) * B * B * real_bytes);
+  s.render = off;
+  off = align256(off + (size_t)6 * T * T * real_bytes);
+  s.cmax = 1 + N + nL;
+  s.gram = off;
+  off = align256(off + (size_t)s.cmax * s.cmax * 8);
+  s.small = off;
+  const int L = nL > N ? nL : N;
+  // A_raw, LU, b, f, c, U, Gt, s, D, AD, H, g, Hd, delta, inverse-column ws, pivots
+  size_t nd = (size_t)2 * N * N + 3 * N + (size_t)N * nL + (size_t)nL * nL + nL +
+              (size_t)2 * nL * N + (size_t)2 * nL * nL + 2 * nL + (size_t)256 * L + 8;
+  off = align256(off + nd * 8 + (size_t)(N + nL + 8) * 4);
+  s.total = off;
+  return s;
+}
+
+}  // namespace uph

@@ -1,0 +1,163 @@
+"""Per grid-band linear flux + background solve (host facade).
+
+Mirrors ``tract7dt/fit/grid_gpu/forced_phot_sparse.py``:
+``build_neighbor_list`` (:272-334, host kd-tree, rectangular stamp-overlap
+predicate, k_max nearest kept), ``stamp_top_left`` (:255-264) and the
+per-band solve ``solve_band_sparse`` (:683-709) with its result type.  The
+device work is batched: ``solve_problems`` renders every (source, band)
+stamp with ``uph_render_stamps`` and solves every (grid, band) problem in
+one ``uph_band_solve`` launch.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from . import _lib
+from .source_model import SOURCE_STAR
+
+DEFAULT_K_MAX = 32
+
+
+@dataclass
+class SparseForcedPhotResult:
+    fluxes: np.ndarray
+    bg_coefs: np.ndarray
+    flux_var: np.ndarray
+    bg_var: np.ndarray
+    converged: bool
+
+
+def build_neighbor_list(pos_x, pos_y, H_psf: int, W_psf: int, k_max: int = DEFAULT_K_MAX):
+    """(idx (n, k_max) int32 with -1 padding, mask (n, k_max) bool): sources
+    whose stamp boxes overlap (|dx| < W_psf and |dy| < H_psf), keeping the
+    k_max nearest when more qualify."""
+    px = np.asarray(pos_x, np.float64)
+    py = np.asarray(pos_y, np.float64)
+    n = px.size
+    idx = np.full((n, k_max), -1, np.int32)
+    msk = np.zeros((n, k_max), bool)
+    fin = np.where(np.isfinite(px) & np.isfinite(py))[0]
+    if fin.size == 0:
+        return idx, msk
+    tree = cKDTree(np.column_stack([px[fin], py[fin]]))
+    lists = tree.query_ball_point(np.column_stack([px[fin], py[fin]]),
+                                  r=float(np.hypot(W_psf, H_psf)))
+    for a, s in enumerate(fin):
+        cand = fin[np.asarray(lists[a], np.int64)]
+        cand = cand[cand != s]
+        if cand.size == 0:
+            continue
+        cand = cand[(np.abs(px[cand] - px[s]) < W_psf) & (np.abs(py[cand] - py[s]) < H_psf)]
+        if cand.size > k_max:
+            d2 = (px[cand] - px[s]) ** 2 + (py[cand] - py[s]) ** 2
+            cand = cand[np.argsort(d2)[:k_max]]
+        idx[s, :cand.size] = cand
+        msk[s, :cand.size] = True
+    return idx, msk
+
+
+def stamp_top_left(pos_x, pos_y, grid_x0, grid_y0, H_psf: int, W_psf: int):
+    """round(pos - grid origin) - size//2 in float32 arithmetic, half to even."""
+    ix = np.rint(np.asarray(pos_x, np.float32) - np.float32(grid_x0)).astype(np.int32) - W_psf // 2
+    iy = np.rint(np.asarray(pos_y, np.float32) - np.float32(grid_y0)).astype(np.int32) - H_psf // 2
+    return ix, iy
+
+
+@dataclass
+class BandProblem:
+    """One (grid, band) problem over a window of the device images."""
+    band: int
+    x0: int
+    y0: int
+    h: int
+    w: int
+    pos_x: np.ndarray      # (n,) float32 full-image coordinates
+    pos_y: np.ndarray
+    nbr_idx: np.ndarray    # (n, k_max) problem-local, -1 padding
+    stype: np.ndarray | None = None
+    logre: np.ndarray | None = None
+    ee1: np.ndarray | None = None
+    ee2: np.ndarray | None = None
+    sersic_n: np.ndarray | None = None
+    origin: tuple | None = None   # (gx0, gy0) for the sub-pixel remainder
+
+
+def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, data_dev,
+                   invvar_dev, good_dev, *, bg_order: int = 1, bg_geo=(0.0, 1.0, 0.0, 1.0),
+                   img_origin=(0, 0), reg: float = 1e-9, k_max: int = DEFAULT_K_MAX,
+                   stream=None, sersic_dev=None):
+    """Solve a batch of problems sharing one stamp tier T.  Returns a list of
+    SparseForcedPhotResult (host numpy)."""
+    import torch
+    if not problems:
+        return []
+    dev = data_dev.device
+    nb, H, W = (int(s) for s in data_dev.shape)
+    M = (bg_order + 1) * (bg_order + 2) // 2
+    ns = np.array([p.pos_x.size for p in problems], np.int64)
+    off = np.concatenate([[0], np.cumsum(ns)]).astype(np.int64)
+    ntot = int(off[-1])
+    cat = lambda f, dt: np.concatenate([f(p) for p in problems]).astype(dt) if ntot else \
+        np.zeros(0, dt)
+    x = cat(lambda p: p.pos_x, np.float32)
+    y = cat(lambda p: p.pos_y, np.float32)
+    org = [p.origin if p.origin is not None else (p.x0, p.y0) for p in problems]
+    gx0 = np.concatenate([np.full(p.pos_x.size, o[0], np.float32) for p, o in
+                          zip(problems, org)]) if ntot else np.zeros(0, np.float32)
+    gy0 = np.concatenate([np.full(p.pos_x.size, o[1], np.float32) for p, o in
+                          zip(problems, org)]) if ntot else np.zeros(0, np.float32)
+    band = cat(lambda p: np.full(p.pos_x.size, p.band, np.int32), np.int32)
+    ixt = np.rint(x - gx0).astype(np.int32) - T // 2
+    iyt = np.rint(y - gy0).astype(np.int32) - T // 2
+    # window-local top-left (window origin may differ from the rounding origin)
+    wx0 = np.concatenate([np.full(p.pos_x.size, p.x0 - o[0], np.int32) for p, o in
+                          zip(problems, org)]) if ntot else np.zeros(0, np.int32)
+    wy0 = np.concatenate([np.full(p.pos_x.size, p.y0 - o[1], np.int32) for p, o in
+                          zip(problems, org)]) if ntot else np.zeros(0, np.int32)
+    ixt -= wx0
+    iyt -= wy0
+    nbr = np.concatenate([p.nbr_idx[:, :k_max] for p in problems]).astype(np.int32) if ntot \
+        else np.zeros((0, k_max), np.int32)
+    win = np.array([[p.band, p.x0, p.y0, p.h, p.w] for p in problems], np.int32)
+    all_star = [p.stype is None or bool(np.all(p.stype == SOURCE_STAR)) for p in problems]
+    if not all(all_star):
+        raise NotImplementedError("extended sources in the linear solve are not part of "
+                                  "this build")
+    up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    L = _lib.lib()
+    P = _lib.ptr
+    st = _lib.stream_ptr(stream)
+    stamps = torch.empty((max(ntot, 1), T, T), dtype=torch.float32, device=dev)
+    d_x, d_y, d_gx, d_gy, d_b = up(x), up(y), up(gx0), up(gy0), up(band)
+    if ntot:
+        _lib.check(L.uph_render_stamps(ntot, T, n_terms, P(psf_coefs_dev), P(psf_geo_dev),
+                                       P(sersic_dev), P(d_b), P(d_x), P(d_y), P(d_gx),
+                                       P(d_gy), None, None, None, None, None, 1, P(stamps),
+                                       st))
+    max_src = int(ns.max())
+    sbytes = L.uph_band_scratch_bytes(len(problems), max_src, bg_order)
+    scratch = torch.empty(max(sbytes, 1), dtype=torch.uint8, device=dev)
+    f32 = dict(dtype=torch.float32, device=dev)
+    fl = torch.empty(max(ntot, 1), **f32)
+    fv = torch.empty(max(ntot, 1), **f32)
+    bgc = torch.empty((len(problems), M), **f32)
+    bgv = torch.empty((len(problems), M), **f32)
+    d_off, d_win, d_ix, d_iy, d_nbr = up(off), up(win), up(ixt), up(iyt), up(nbr)
+    _lib.check(L.uph_band_solve(len(problems), T, bg_order, k_max, P(d_off), P(d_win), H, W,
+                                int(img_origin[0]), int(img_origin[1]), P(data_dev),
+                                P(invvar_dev), P(good_dev), *map(float, bg_geo), P(stamps),
+                                P(d_ix), P(d_iy), P(d_nbr), float(reg), P(fl), P(fv), P(bgc),
+                                P(bgv), P(scratch), scratch.numel(), st))
+    flh, fvh = fl.cpu().numpy(), fv.cpu().numpy()
+    bgh, bvh = bgc.cpu().numpy(), bgv.cpu().numpy()
+    out = []
+    for q in range(len(problems)):
+        f = flh[off[q]:off[q + 1]]
+        v = fvh[off[q]:off[q + 1]]
+        out.append(SparseForcedPhotResult(f, bgh[q], v, bvh[q],
+                                          bool(np.all(np.isfinite(f)) and
+                                               np.all(np.isfinite(bgh[q])))))
+    return out

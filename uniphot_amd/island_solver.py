@@ -1,0 +1,168 @@
+"""Per-island VarPro-LM on the GPU (host facade over ``uph_island_fit``).
+
+Mirrors the reference pair ``make_class_kernels`` / ``run_class_chunk``
+(``tract7dt/fit/grid_gpu/island_solver.py:1874-1922``, ``:2793-3131``) for
+the legacy step protocol: same host dict layout (``class_host_arrays``),
+same padded field stacks, same per-row output dict.  Differences by
+design: there is no vmap width (one persistent kernel pulls islands from a
+device work queue, so ``width`` is accepted and ignored), and the LM loop
+runs on the device with no per-step host synchronisation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .island_plan import (DEFAULT_CHI2_FTOL_REL, DEFAULT_MAX_STEPS, DEFAULT_POS_STEP_MAX,
+                          DEFAULT_REG, DEFAULT_SHAPE_STEP_MAX, DEFAULT_SHAPE_STEP_TOL,
+                          DEFAULT_STEP_TOL_PIX, DEFAULT_UN_STEP_MAX)
+
+# Cap on the persistent workgroups of one launch (4 per CU on 256 CUs) and
+# on the workspace they may hold.
+MAX_BLOCKS = 1024
+MAX_SCRATCH_BYTES = 4 << 30
+
+
+@dataclass
+class ClassKernels:
+    """Static description of one island class (the reference's cache key)."""
+    B: int
+    T: int
+    N: int
+    n_bands: int
+    n_terms: int
+    kind: str = "star"
+    reg: float = DEFAULT_REG
+    pos_step_max: float = DEFAULT_POS_STEP_MAX
+    chi2_ftol_rel: float = DEFAULT_CHI2_FTOL_REL
+    step_tol_pix: float = DEFAULT_STEP_TOL_PIX
+    shape_step_max: float = DEFAULT_SHAPE_STEP_MAX
+    un_step_max: float = DEFAULT_UN_STEP_MAX
+    shape_step_tol: float = DEFAULT_SHAPE_STEP_TOL
+    precision: str = "fp32"
+
+    @property
+    def n_live_max(self) -> int:
+        return 2 * self.N if self.kind == "star" else 6 * self.N
+
+    def options(self, max_steps: int) -> _lib.LMOptions:
+        return _lib.LMOptions(self.reg, self.pos_step_max, self.shape_step_max,
+                              self.un_step_max, self.chi2_ftol_rel, self.step_tol_pix,
+                              self.shape_step_tol, int(max_steps))
+
+
+def make_class_kernels(plan, n_bands: int, n_terms: int, *, reg: float = DEFAULT_REG,
+                       pos_step_max_pix: float = DEFAULT_POS_STEP_MAX,
+                       chi2_ftol_rel: float = DEFAULT_CHI2_FTOL_REL,
+                       step_tol_pix: float = DEFAULT_STEP_TOL_PIX,
+                       precision: str = "fp32", **_ignored) -> ClassKernels:
+    return ClassKernels(B=int(plan.bbox_class), T=int(plan.tier), N=int(plan.nsrc_class),
+                        n_bands=int(n_bands), n_terms=int(n_terms), kind=str(plan.kind),
+                        reg=float(reg), pos_step_max=float(pos_step_max_pix),
+                        chi2_ftol_rel=float(chi2_ftol_rel), step_tol_pix=float(step_tol_pix),
+                        precision=precision)
+
+
+@dataclass
+class PsfTree:
+    """Device PSF of one tier: coefs (nb, nt, T, T), geo (nb, 4)."""
+    coefs: object
+    geo: object
+    T: int
+    n_terms: int
+
+
+def psf_tree_device(psfs, tier: int, device="cuda") -> PsfTree:
+    import torch
+    from .psf_polynomial import psf_device_arrays
+    c, g = psf_device_arrays(psfs, tier)
+    return PsfTree(torch.from_numpy(c).to(device), torch.from_numpy(g).to(device), tier,
+                   c.shape[1])
+
+
+def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
+                    width: int | None, max_steps: int, dstack_dev, wstack_dev, pad: int,
+                    init_state=None, stream=None, sersic_dev=None) -> dict:
+    """Fit rows ``rows`` of a class host dict on the device and return the
+    reference's per-row output dict (p, fluxes, ferr, chi2, conv, stall,
+    nst, lam, consec, nrej, n_good, steps_run, wall_s)."""
+    import torch
+    if init_state is not None:
+        raise NotImplementedError("resuming LM state is not part of this build")
+    if host.get("has_priors"):
+        raise NotImplementedError("catalog priors are not part of this build")
+    t0 = time.perf_counter()
+    rows = np.asarray(rows, np.int64)
+    n_c = int(rows.size)
+    N, B, T = kernels.N, kernels.B, kernels.T
+    nb = kernels.n_bands
+    nP = 6 * N
+    if psf_tree.T != T:
+        raise ValueError(f"psf tree tier {psf_tree.T} != class tier {T}")
+    if tuple(dstack_dev.shape) != tuple(wstack_dev.shape) or dstack_dev.shape[0] != nb:
+        raise ValueError("dstack/wstack must both be (n_bands, Hp, Wp)")
+    Hp, Wp = int(dstack_dev.shape[1]), int(dstack_dev.shape[2])
+    ox = host["ox"][rows]
+    oy = host["oy"][rows]
+    if n_c and (ox.min() < 0 or oy.min() < 0 or ox.max() + B > Wp or oy.max() + B > Hp):
+        raise ValueError("island window outside the padded stacks")
+    if kernels.kind != "star" and sersic_dev is None:
+        raise ValueError("mixed classes need the device Sersic table")
+    dev = dstack_dev.device
+
+    def up(a, dt):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev, non_blocking=False)
+
+    mm = host["mmask"][rows]
+    # LPT order: most real members first
+    order = np.argsort(-mm.sum(axis=1), kind="stable").astype(np.int32)
+    ins = dict(
+        ox=up(ox, np.int32), oy=up(oy, np.int32), x0b=up(host["x0b"][rows], np.float32),
+        y0b=up(host["y0b"][rows], np.float32), stype=up(host["st"][rows], np.int32),
+        n_frozen=up(host["sn"][rows], np.float32), free_n=up(host["fn_flag"][rows], np.uint8),
+        mmask=up(mm, np.uint8), keep=up(host["keep"][rows], np.uint8),
+        p0=up(host["p0"][rows].reshape(n_c, nP), np.float32),
+        lower=up(host["lower"][rows].reshape(n_c, nP), np.float32),
+        upper=up(host["upper"][rows].reshape(n_c, nP), np.float32), order=up(order, np.int32))
+    f32 = dict(dtype=torch.float32, device=dev)
+    outs = dict(
+        p=torch.empty(n_c, nP, **f32), fluxes=torch.empty(n_c, nb, N, **f32),
+        ferr=torch.empty(n_c, nb, N, **f32), chi2=torch.empty(n_c, **f32),
+        lam=torch.empty(n_c, **f32), conv=torch.empty(n_c, dtype=torch.uint8, device=dev),
+        stall=torch.empty(n_c, dtype=torch.uint8, device=dev),
+        nst=torch.empty(n_c, dtype=torch.int32, device=dev),
+        consec=torch.empty(n_c, dtype=torch.int32, device=dev),
+        nrej=torch.empty(n_c, dtype=torch.int32, device=dev),
+        n_good=torch.empty(n_c, dtype=torch.int32, device=dev))
+    prec = _lib.UPH_PRECISION_FP64 if kernels.precision == "fp64" else _lib.UPH_PRECISION_FP32
+    L = _lib.lib()
+    per = L.uph_island_scratch_per_block(N, B, T, kernels.n_live_max, prec)
+    nblk = max(1, min(n_c, MAX_BLOCKS, MAX_SCRATCH_BYTES // max(per, 1)))
+    scratch = torch.empty(per * nblk, dtype=torch.uint8, device=dev)
+    counter = torch.zeros(1, dtype=torch.int32, device=dev)
+    cls = _lib.IslandClass(n_c, N, B, T, psf_tree.n_terms, nb, kernels.n_live_max, Hp, Wp,
+                           0 if kernels.kind == "star" else 1)
+    P = _lib.ptr
+    inp = _lib.IslandInputs(P(dstack_dev), P(wstack_dev), P(psf_tree.coefs), P(psf_tree.geo),
+                            P(sersic_dev), *(P(ins[k]) for k in (
+                                "ox", "oy", "x0b", "y0b", "stype", "n_frozen", "free_n",
+                                "mmask", "keep", "p0", "lower", "upper", "order")))
+    out = _lib.IslandOutputs(*(P(outs[k]) for k in (
+        "p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
+        "n_good")))
+    if n_c:
+        _lib.check(L.uph_island_fit(C.byref(cls), C.byref(inp), C.byref(out),
+                                    C.byref(kernels.options(max_steps)), prec, P(scratch),
+                                    scratch.numel(), P(counter), _lib.stream_ptr(stream)))
+    res = {k: v.cpu().numpy() for k, v in outs.items()}
+    res["p"] = res["p"].reshape(n_c, N, 6)
+    for k in ("conv", "stall"):
+        res[k] = res[k].astype(bool)
+    res["n_good"] = res["n_good"].astype(np.int64)
+    res["steps_run"] = int(res["nst"].max()) if n_c else 0
+    res["wall_s"] = time.perf_counter() - t0
+    return res
