@@ -126,6 +126,36 @@ int uph_island_fit(const uph_island_class* cls, const uph_island_inputs* in,
                    void* scratch, size_t scratch_bytes, int32_t* counter, void* stream);
 
 /*
+ * Heterogeneous island batch: islands of different classes (N, B, stamp
+ * tier) in one persistent launch, pulled from the work queue in ``order``
+ * (longest first is the caller's choice).  Member arrays of
+ * uph_island_inputs / outputs are concatenated over islands: island i owns
+ * member slots [mem_off[i], mem_off[i] + N_i) (stype, n_frozen, free_n,
+ * mmask; keep, p0, lower, upper and p at 6x that) and flux elements
+ * [flux_off[i], flux_off[i] + n_bands * N_i).  Per-island scalars (ox, oy,
+ * x0b, y0b, order and the scalar outputs) are indexed by island.
+ * ``psf_coefs`` of the inputs is ignored: tier t uses psf_coefs[t].
+ */
+typedef struct uph_island_batch {
+  int n_isl, n_bands, n_terms, Hp, Wp;
+  int max_B;                   /* largest window side in the batch                */
+  int mixed;                   /* any extended member                             */
+  int n_tiers;                 /* 1..3                                            */
+  int tier_T[3];               /* stamp size of each tier                         */
+  const float* psf_coefs[3];   /* (n_bands, n_terms, T_t, T_t) per tier           */
+  const int32_t* shape;        /* (n_isl, 4): N, B, tier index, n_live_max        */
+  const int64_t* mem_off;      /* (n_isl)                                          */
+  const int64_t* flux_off;     /* (n_isl)                                          */
+} uph_island_batch;
+
+/* ``scratch_per_block`` = max over the batch's shapes of
+   uph_island_scratch_per_block (a multiple of 256). */
+int uph_island_fit_batch(const uph_island_batch* batch, const uph_island_inputs* in,
+                         const uph_island_outputs* out, const uph_lm_options* opt, int precision,
+                         void* scratch, size_t scratch_bytes, size_t scratch_per_block,
+                         int32_t* counter, void* stream);
+
+/*
  * Batched unit-flux stamp renderer: PSF polynomial at (x, y), galaxy
  * mixture convolution for extended sources, unit-sum renormalisation when
  * ``renorm`` (all-star grids), Lanczos-3 shift by the remainder of

@@ -51,3 +51,23 @@ def test_island_fit_matches_oracle(cuda, precision):
             assert res["n_good"][r] == int((isl.w > 0).sum())
             n_checked += 1
     assert n_checked > 0
+
+
+def test_batch_launch_equals_per_class(cuda):
+    """All classes in one heterogeneous persistent launch give bit-identical
+    results to one launch per class (same per-island arithmetic)."""
+    import torch
+    from uniphot_amd.island_solver import (make_class_kernels, psf_tree_device,
+                                           run_class_chunk, run_island_batch)
+    sc = small_star_scene(seed=3, n_src=24)
+    plans, hosts, pad, d, w = star_classes(sc)
+    assert len(plans) >= 2
+    dd, ww = torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda)
+    tree = psf_tree_device(sc.psfs, 55, cuda)
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms), h, np.arange(p.n_islands))
+             for p, h in zip(plans, hosts)]
+    got = run_island_batch(batch, {55: tree}, 30, dd, ww)
+    for (kern, host, rows), rb in zip(batch, got):
+        ra = run_class_chunk(kern, tree, host, rows, None, 30, dd, ww, pad)
+        for k in ("p", "fluxes", "ferr", "chi2", "nst", "conv", "stall", "n_good"):
+            assert np.array_equal(ra[k], rb[k]), k

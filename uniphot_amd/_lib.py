@@ -17,6 +17,7 @@ UPH_PRECISION_FP32, UPH_PRECISION_FP64 = 0, 1
 # Every exported entry point declared in include/uniphot_amd.h.
 EXPORTS = (
     "uph_version", "uph_last_error", "uph_island_scratch_per_block", "uph_island_fit",
+    "uph_island_fit_batch",
     "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
 )
 
@@ -47,6 +48,13 @@ class IslandClass(C.Structure):
 
 
 _P = C.c_void_p
+
+
+class IslandBatch(C.Structure):
+    _fields_ = [("n_isl", C.c_int), ("n_bands", C.c_int), ("n_terms", C.c_int),
+                ("Hp", C.c_int), ("Wp", C.c_int), ("max_B", C.c_int), ("mixed", C.c_int),
+                ("n_tiers", C.c_int), ("tier_T", C.c_int * 3), ("psf_coefs", _P * 3),
+                ("shape", _P), ("mem_off", _P), ("flux_off", _P)]
 
 
 class IslandInputs(C.Structure):
@@ -82,6 +90,10 @@ def lib() -> C.CDLL:
     L.uph_island_fit.argtypes = [C.POINTER(IslandClass), C.POINTER(IslandInputs),
                                  C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,
                                  _P, C.c_size_t, _P, _P]
+    L.uph_island_fit_batch.restype = C.c_int
+    L.uph_island_fit_batch.argtypes = [C.POINTER(IslandBatch), C.POINTER(IslandInputs),
+                                       C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,
+                                       _P, C.c_size_t, C.c_size_t, _P, _P]
     L.uph_render_stamps.restype = C.c_int
     L.uph_render_stamps.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P,
                                     _P, _P, _P, _P, _P, C.c_int, _P, _P]

@@ -40,6 +40,9 @@ struct IslandSh {
   double red[UPH_WAVES * 32];
   double redd[UPH_WAVES];
   int Nr, nL, item, pivot;
+  int N, B, T, nLc;          // this island's class shape
+  long long moff, foff;      // member-slot and flux-block offsets
+  const float* coefs;        // PSF coefficients of the island's tier
   // LM state
   float lam, chi2, chi2_try;
   int consec, nrej, nst, conv, stall;
@@ -160,6 +163,97 @@ __device__ void weighted_pairs(const Real* __restrict__ W, const Real* __restric
 }
 
 // ---------------------------------------------------------------------------
+// Weighted Gram on the matrix cores: out = Z^T W Z over planes [z0, z0 + C).
+// v_mfma_f32_16x16x4f32 tiles of 16 planes; lane l feeds plane (l % 16) at
+// four consecutive pixels (float4), lane group l / 16 selects which four, so
+// one K-step of four MFMAs consumes 16 pixels.  Products and short sums run
+// in float32; every 16 K-steps (256 products per entry) the tile is flushed
+// into float64.  Tile pairs (ti <= tj) and, when there are fewer tile pairs
+// than waves, pixel splits are spread over the 4 waves; split partials are
+// summed in a fixed order (deterministic).
+// ---------------------------------------------------------------------------
+typedef float uph_f4 __attribute__((ext_vector_type(4)));
+
+__device__ void gram_mfma(const float* __restrict__ W, const float* __restrict__ Z, int P, int C,
+                          int z0, double* __restrict__ out, double* __restrict__ part) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nt = (C + 15) >> 4, ntp = nt * (nt + 1) / 2;
+  const int S = ntp >= UPH_WAVES ? 1 : UPH_WAVES / ntp;
+  const int units = ntp * S;
+  const int KS = P >> 4;
+  const int col = lane & 15, grp = lane >> 4;
+  for (int u = wid; u < units; u += UPH_WAVES) {
+    const int tp = u / S, sp = u % S;
+    int ti, tj;
+    pair_of(tp, nt, ti, tj);
+    const int ks0 = (KS * sp) / S, ks1 = (KS * (sp + 1)) / S;
+    const int a = ti * 16 + col, b = tj * 16 + col;
+    const bool va = a < C, vb = b < C;
+    const float* za = Z + (size_t)(z0 + (va ? a : 0)) * P;
+    const float* zb = Z + (size_t)(z0 + (vb ? b : 0)) * P;
+    double accd[4] = {0.0, 0.0, 0.0, 0.0};
+    uph_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    int cnt = 0;
+    for (int ks = ks0; ks < ks1; ++ks) {
+      const int px = ks * 16 + 4 * grp;
+      const uph_f4 wv = *reinterpret_cast<const uph_f4*>(W + px);
+      uph_f4 xa = {0.0f, 0.0f, 0.0f, 0.0f}, xb = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (va) xa = *reinterpret_cast<const uph_f4*>(za + px);
+      if (ti == tj)
+        xb = xa;
+      else if (vb)
+        xb = *reinterpret_cast<const uph_f4*>(zb + px);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.x, wv.x * xb.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.y, wv.y * xb.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.z, wv.z * xb.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa.w, wv.w * xb.w, acc, 0, 0, 0);
+      if (++cnt == 16) {
+        accd[0] += acc.x;
+        accd[1] += acc.y;
+        accd[2] += acc.z;
+        accd[3] += acc.w;
+        acc = uph_f4{0.0f, 0.0f, 0.0f, 0.0f};
+        cnt = 0;
+      }
+    }
+    accd[0] += acc.x;
+    accd[1] += acc.y;
+    accd[2] += acc.z;
+    accd[3] += acc.w;
+    // D[4 * grp + r][col] of the 16 x 16 tile
+    double* pt = part + (size_t)u * 256;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pt[(4 * grp + r) * 16 + col] = accd[r];
+  }
+  __syncthreads();
+  for (int e = tid; e < ntp * 256; e += UPH_BLOCK) {
+    const int tp = e >> 8, i = (e >> 4) & 15, j = e & 15;
+    int ti, tj;
+    pair_of(tp, nt, ti, tj);
+    if (ti == tj && i > j) continue;  // diagonal tiles: upper entries only
+    const int a = ti * 16 + i, b = tj * 16 + j;
+    if (a >= C || b >= C) continue;
+    double v = 0.0;
+    for (int sp = 0; sp < S; ++sp) v += part[((size_t)tp * S + sp) * 256 + i * 16 + j];
+    out[a * C + b] = v;
+    out[b * C + a] = v;
+  }
+  __syncthreads();
+}
+
+template <typename Real>
+__device__ __forceinline__ void gram(const Real* W, const Real* Z, int P, int C, int z0,
+                                     double* out, double* part, double* red) {
+  if constexpr (sizeof(Real) == 4) {
+    (void)red;
+    gram_mfma(W, Z, P, C, z0, out, part);
+  } else {
+    (void)part;
+    weighted_pairs<Real, true>(W, Z, P, C, z0, 0, out, red);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Star member: PSF polynomial at (x, y), separable Lanczos-3 shift by the
 // remainder of (x - x0b, y - y0b), placed on the B x B window at
 // round(pos - origin) - T//2 (source_model.py:322-328, island_solver.py:512).
@@ -167,11 +261,11 @@ __device__ void weighted_pairs(const Real* __restrict__ W, const Real* __restric
 // tangents, also d/dx and d/dy (polynomial derivative + tap derivative).
 // ---------------------------------------------------------------------------
 template <typename Real>
-__device__ void render_star(const IslandArgs& a, int b, float xf, float yf, float x0f, float y0f,
-                            Real* __restrict__ m_out, Real* __restrict__ tx_out,
-                            Real* __restrict__ ty_out, Real* __restrict__ ws, Real* taps,
-                            Real* bas) {
-  const int T = a.T, B = a.B, nt = a.nt, tid = threadIdx.x;
+__device__ void render_star(const IslandArgs& a, int T, int B, const float* coefs, int b, float xf,
+                            float yf, float x0f, float y0f, Real* __restrict__ m_out,
+                            Real* __restrict__ tx_out, Real* __restrict__ ty_out,
+                            Real* __restrict__ ws, Real* taps, Real* bas) {
+  const int nt = a.nt, tid = threadIdx.x;
   const bool grad = tx_out != nullptr;
   const float* geo = a.psf_geo + 4 * b;
   const Real x = Real(xf), y = Real(yf);
@@ -193,7 +287,7 @@ __device__ void render_star(const IslandArgs& a, int b, float xf, float yf, floa
   const int rx0 = max(0, sx0 - 3), rx1 = min(T, sx1 + 3);
   const int ry0 = max(0, sy0 - 3), ry1 = min(T, sy1 + 3);
   const int RW = rx1 - rx0, RH = ry1 - ry0, SW = sx1 - sx0, SH = sy1 - sy0;
-  const float* Cf = a.psf_coefs + (size_t)b * nt * T * T;
+  const float* Cf = coefs + (size_t)b * nt * T * T;
   const int TT = T * T;
   Real* P = ws;
   Real* Px = ws + TT;
@@ -282,9 +376,9 @@ __device__ void render_star(const IslandArgs& a, int b, float xf, float yf, floa
 template <typename Real>
 __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int b,
                           const float* pv, bool grads, bool accumulate, Real* planes, Real* ws,
-                          double* G, const Small& sm, float* flux_out, float* ferr_out,
-                          int* ngood) {
-  const int B = a.B, P = B * B, tid = threadIdx.x;
+                          double* G, double* part, const Small& sm, float* flux_out,
+                          float* ferr_out, int* ngood) {
+  const int B = sh.B, P = B * B, tid = threadIdx.x;
   const int Nr = sh.Nr, nL = grads ? sh.nL : 0;
   Real* Wp = planes;
   Real* Rp = planes + P;
@@ -327,8 +421,8 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
         if (tx == nullptr) tx = Rp;
         if (ty == nullptr) ty = Rp;
       }
-      render_star<Real>(a, b, xk, yk, x0f, y0f, Mp + (size_t)r * P, grads ? tx : nullptr,
-                        grads ? ty : nullptr, wsx, sh.taps, sh.bas);
+      render_star<Real>(a, sh.T, B, sh.coefs, b, xk, yk, x0f, y0f, Mp + (size_t)r * P,
+                        grads ? tx : nullptr, grads ? ty : nullptr, wsx, sh.taps, sh.bas);
     } else {
       render_galaxy<Real>(a, b, pv + 6 * k, sh.st[r], sh.nfz[r], sh.fn[r], x0f, y0f,
                           Mp + (size_t)r * P, grads ? Tp : nullptr, sh.live_j, sh.first_live[r],
@@ -337,7 +431,7 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
   }
   // Gram of [d, models, tangents] under W.
   const int C = 1 + Nr + nL;
-  weighted_pairs<Real, true>(Wp, planes, P, C, 2, 0, G, sh.red);
+  gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
   // VarPro fluxes (float64 LU), A64 = A + reg I on real members.
   const double reg = double(a.reg);
   for (int e = tid; e < Nr * Nr; e += UPH_BLOCK) {
@@ -352,8 +446,25 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
   if (tid == 0) {
     for (int i = 0; i < Nr; ++i) sm.f[i] = sm.b[i];
     lu_solve_one(sm.LU, sm.pivA, Nr, sm.f);
+    for (int i = 0; i < Nr; ++i) sh.fR[i] = Real(sm.f[i]);
+  }
+  __syncthreads();
+  // One refinement step: c0 = M^T W (d - M f0) accumulated in float64 from
+  // the per-pixel residual, f = f0 + A64^-1 (c0 - reg f0).  The flux then
+  // carries the accuracy of the residual sums, not of the Gram.
+  for (int p = tid; p < P; p += UPH_BLOCK) {
+    Real model = 0;
+    for (int r = 0; r < Nr; ++r) model = fma(sh.fR[r], Mp[(size_t)r * P + p], model);
+    Rp[p] = Dp[p] - model;
+  }
+  __syncthreads();
+  weighted_pairs<Real, false>(Wp, planes, P, Nr, 3, 1, sm.colws, sh.red);
+  if (tid == 0) {
+    double* dx = sm.c;
+    for (int i = 0; i < Nr; ++i) dx[i] = sm.colws[i] - reg * double(sh.fR[i]);
+    lu_solve_one(sm.LU, sm.pivA, Nr, dx);
     for (int i = 0; i < Nr; ++i) {
-      sh.fR[i] = Real(sm.f[i]);
+      sh.fR[i] = Real(double(sh.fR[i]) + dx[i]);
       sm.f[i] = double(sh.fR[i]);
     }
   }
@@ -442,11 +553,11 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
 
 template <typename Real>
 __device__ float chi2_all(const IslandArgs& a, IslandSh<Real>& sh, int isl, const float* pv,
-                          Real* planes, Real* ws, double* G, const Small& sm) {
+                          Real* planes, Real* ws, double* G, double* part, const Small& sm) {
   Real c = 0;
   for (int b = 0; b < a.nb; ++b)
-    c += eval_band<Real>(a, sh, isl, b, pv, false, false, planes, ws, G, sm, nullptr, nullptr,
-                         nullptr);
+    c += eval_band<Real>(a, sh, isl, b, pv, false, false, planes, ws, G, part, sm, nullptr,
+                         nullptr, nullptr);
   return float(c);
 }
 
@@ -454,24 +565,50 @@ template <typename Real>
 __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
   __shared__ IslandSh<Real> sh;
   const int tid = threadIdx.x;
-  const int N = a.N, nP = 6 * N;
-  const IslandScratch lay = island_scratch_layout(N, a.B, a.T, a.nL_max, sizeof(Real));
   char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
-  Real* planes = reinterpret_cast<Real*>(base + lay.planes);
-  Real* ws = reinterpret_cast<Real*>(base + lay.render);
-  double* G = reinterpret_cast<double*>(base + lay.gram);
-  const Small sm = carve_small(base + lay.small, N, a.nL_max);
   const float clipv[6] = {a.pos_step_max, a.pos_step_max, a.shape_step_max,
                           a.shape_step_max, a.shape_step_max, a.un_step_max};
   for (;;) {
-    if (tid == 0) sh.item = atomicAdd(a.counter, 1);
+    if (tid == 0) {
+      const int item = atomicAdd(a.counter, 1);
+      sh.item = item;
+      if (item < a.n_isl) {
+        const int isl = a.order ? a.order[item] : item;
+        if (a.shape) {
+          const int32_t* sp = a.shape + 4 * (size_t)isl;
+          sh.N = sp[0];
+          sh.B = sp[1];
+          sh.T = a.tier_T[sp[2]];
+          sh.coefs = a.tier_coefs[sp[2]];
+          sh.nLc = sp[3];
+          sh.moff = a.mem_off[isl];
+          sh.foff = a.flux_off[isl];
+        } else {
+          sh.N = a.N;
+          sh.B = a.B;
+          sh.T = a.T;
+          sh.coefs = a.psf_coefs;
+          sh.nLc = a.nL_max;
+          sh.moff = (long long)isl * a.N;
+          sh.foff = (long long)isl * a.nb * a.N;
+        }
+      }
+    }
     __syncthreads();
     const int item = sh.item;
     if (item >= a.n_isl) break;
     const int isl = a.order ? a.order[item] : item;
+    const int N = sh.N, nP = 6 * N;
+    const IslandScratch lay = island_scratch_layout(N, sh.B, sh.T, sh.nLc, sizeof(Real));
+    Real* planes = reinterpret_cast<Real*>(base + lay.planes);
+    Real* ws = reinterpret_cast<Real*>(base + lay.render);
+    double* G = reinterpret_cast<double*>(base + lay.gram);
+    double* part = reinterpret_cast<double*>(base + lay.part);
+    const Small sm = carve_small(base + lay.small, N, sh.nLc);
+    const size_t m0 = (size_t)sh.moff;
     // ---- island description into LDS ----
     for (int q = tid; q < nP; q += UPH_BLOCK) {
-      const size_t g = (size_t)isl * nP + q;
+      const size_t g = m0 * 6 + q;
       sh.p[q] = a.p0[g];
       sh.lo[q] = a.lower[g];
       sh.hi[q] = a.upper[g];
@@ -479,7 +616,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     if (tid == 0) {
       int nr = 0, nl = 0;
       for (int k = 0; k < N; ++k) {
-        const size_t g = (size_t)isl * N + k;
+        const size_t g = m0 + k;
         if (!a.mmask[g]) continue;
         sh.mem[nr] = k;
         sh.st[nr] = a.stype[g];
@@ -506,7 +643,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     if (sh.Nr == 0) sh.conv = 1;  // nothing to fit (all slots padded)
     __syncthreads();
     if (sh.Nr > 0) {
-      const float c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, G, sm);
+      const float c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, G, part, sm);
       if (tid == 0) sh.chi2 = c0;
       __syncthreads();
     } else if (tid == 0) {
@@ -517,8 +654,8 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
       const int nL = sh.nL;
       // H, g at p (un-damped, un-masked over the live set)
       for (int b = 0; b < a.nb; ++b)
-        eval_band<Real>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, G, sm, nullptr, nullptr,
-                        nullptr);
+        eval_band<Real>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, G, part, sm, nullptr,
+                        nullptr, nullptr);
       // damped float64 solve: (H + lam diag(H) + 1e-8 I) delta = -g
       const double lam = double(sh.lam);
       for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
@@ -547,7 +684,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
         for (int q = 0; q < nP; ++q) sh.ptry[q] = fminf(fmaxf(sh.ptry[q], sh.lo[q]), sh.hi[q]);
       }
       __syncthreads();
-      const float ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, G, sm);
+      const float ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, G, part, sm);
       if (tid == 0) {
         const float cur = sh.chi2;
         const bool accept = ct < cur;
@@ -580,8 +717,8 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
       __syncthreads();
     }
     // ---- final fluxes + outputs ----
-    float* fo = a.flux_out + (size_t)isl * a.nb * N;
-    float* eo = a.ferr_out + (size_t)isl * a.nb * N;
+    float* fo = a.flux_out + (size_t)sh.foff;
+    float* eo = a.ferr_out + (size_t)sh.foff;
     for (int e = tid; e < a.nb * N; e += UPH_BLOCK) {
       fo[e] = 0.0f;
       eo[e] = 1.0f;
@@ -590,10 +727,10 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     int ngood = 0;
     if (sh.Nr > 0) {
       for (int b = 0; b < a.nb; ++b)
-        eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, G, sm, fo + b * N,
+        eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, G, part, sm, fo + b * N,
                         eo + b * N, &ngood);
     }
-    for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[(size_t)isl * nP + q] = sh.p[q];
+    for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[m0 * 6 + q] = sh.p[q];
     if (tid == 0) {
       a.chi2_out[isl] = sh.chi2;
       a.lam_out[isl] = sh.lam;
@@ -617,54 +754,26 @@ extern "C" size_t uph_island_scratch_per_block(int N, int B, int T, int n_live_m
   return base + uph::galaxy_scratch_bytes(T, rb);
 }
 
-extern "C" int uph_island_fit(const uph_island_class* cls, const uph_island_inputs* in,
-                              const uph_island_outputs* out, const uph_lm_options* opt,
-                              int precision, void* scratch, size_t scratch_bytes,
-                              int32_t* counter, void* stream) {
-  UPH_CHECK_ARG(cls && in && out && opt, "uph_island_fit: null argument");
-  UPH_CHECK_ARG(cls->N >= 1 && cls->N <= uph::kMaxMembers, "uph_island_fit: N=%d outside 1..%d",
-                cls->N, uph::kMaxMembers);
-  UPH_CHECK_ARG(cls->B >= 16 && cls->B % 16 == 0, "uph_island_fit: B=%d must be a multiple of 16",
-                cls->B);
-  UPH_CHECK_ARG(cls->T >= 7 && cls->T % 2 == 1, "uph_island_fit: T=%d must be odd >= 7", cls->T);
-  UPH_CHECK_ARG(cls->n_terms == 1 || cls->n_terms == 3 || cls->n_terms == 6 ||
-                    cls->n_terms == 10 || cls->n_terms == 15 || cls->n_terms == 21,
-                "uph_island_fit: unsupported n_terms=%d", cls->n_terms);
-  UPH_CHECK_ARG(cls->n_bands >= 1, "uph_island_fit: n_bands=%d", cls->n_bands);
-  UPH_CHECK_ARG(cls->n_live_max >= 1 && cls->n_live_max <= 6 * cls->N,
-                "uph_island_fit: n_live_max=%d outside 1..6N", cls->n_live_max);
-  UPH_CHECK_ARG(precision == UPH_PRECISION_FP32 || precision == UPH_PRECISION_FP64,
-                "uph_island_fit: precision=%d", precision);
-  UPH_CHECK_ARG(cls->Hp >= cls->B && cls->Wp >= cls->B, "uph_island_fit: stack %dx%d < B=%d",
-                cls->Hp, cls->Wp, cls->B);
-  UPH_CHECK_ARG(in->dstack && in->wstack && in->psf_coefs && in->psf_geo && in->ox && in->oy &&
-                    in->x0b && in->y0b && in->stype && in->n_frozen && in->free_n &&
-                    in->mmask && in->keep && in->p0 && in->lower && in->upper,
+static int island_launch(uph::IslandArgs& a, const uph_island_inputs* in,
+                         const uph_island_outputs* out, const uph_lm_options* opt, int precision,
+                         void* scratch, size_t scratch_bytes, size_t per, int32_t* counter,
+                         void* stream) {
+  UPH_CHECK_ARG(in->dstack && in->wstack && in->psf_geo && in->ox && in->oy && in->x0b &&
+                    in->y0b && in->stype && in->n_frozen && in->free_n && in->mmask &&
+                    in->keep && in->p0 && in->lower && in->upper,
                 "uph_island_fit: null input pointer");
   UPH_CHECK_ARG(out->p && out->fluxes && out->ferr && out->chi2 && out->lam && out->conv &&
                     out->stall && out->nst && out->consec && out->nrej && out->n_good,
                 "uph_island_fit: null output pointer");
   UPH_CHECK_ARG(opt->max_steps >= 0, "uph_island_fit: max_steps=%d", opt->max_steps);
-  UPH_CHECK_ARG(cls->mixed == 0, "uph_island_fit: classes with extended members are not supported "
-                "by this build");
+  UPH_CHECK_ARG(precision == UPH_PRECISION_FP32 || precision == UPH_PRECISION_FP64,
+                "uph_island_fit: precision=%d", precision);
   UPH_CHECK_ARG(counter != nullptr && scratch != nullptr, "uph_island_fit: null scratch/counter");
-  if (cls->n_isl <= 0) return UPH_OK;
-  const size_t per = uph_island_scratch_per_block(cls->N, cls->B, cls->T, cls->n_live_max,
-                                                  precision);
+  if (a.n_isl <= 0) return UPH_OK;
   const size_t nblk_s = scratch_bytes / per;
   UPH_CHECK_ARG(nblk_s >= 1, "uph_island_fit: scratch of %zu bytes < one workgroup (%zu)",
                 scratch_bytes, per);
-  const int nblk = (int)(nblk_s < (size_t)cls->n_isl ? nblk_s : (size_t)cls->n_isl);
-  uph::IslandArgs a{};
-  a.n_isl = cls->n_isl;
-  a.N = cls->N;
-  a.B = cls->B;
-  a.T = cls->T;
-  a.nt = cls->n_terms;
-  a.nb = cls->n_bands;
-  a.nL_max = cls->n_live_max;
-  a.Hp = cls->Hp;
-  a.Wp = cls->Wp;
+  const int nblk = (int)(nblk_s < (size_t)a.n_isl ? nblk_s : (size_t)a.n_isl);
   a.max_steps = opt->max_steps;
   a.reg = opt->reg;
   a.pos_step_max = opt->pos_step_max;
@@ -675,7 +784,6 @@ extern "C" int uph_island_fit(const uph_island_class* cls, const uph_island_inpu
   a.shape_step_tol = opt->shape_step_tol;
   a.dstack = in->dstack;
   a.wstack = in->wstack;
-  a.psf_coefs = in->psf_coefs;
   a.psf_geo = in->psf_geo;
   a.sersic = in->sersic;
   a.ox = in->ox;
@@ -713,4 +821,82 @@ extern "C" int uph_island_fit(const uph_island_class* cls, const uph_island_inpu
   else
     hipLaunchKernelGGL(uph::island_lm_kernel<double>, dim3(nblk), dim3(UPH_BLOCK), 0, s, a);
   return uph::hip_status(hipGetLastError(), "uph_island_fit launch");
+}
+
+static bool valid_terms(int nt) {
+  return nt == 1 || nt == 3 || nt == 6 || nt == 10 || nt == 15 || nt == 21;
+}
+
+extern "C" int uph_island_fit(const uph_island_class* cls, const uph_island_inputs* in,
+                              const uph_island_outputs* out, const uph_lm_options* opt,
+                              int precision, void* scratch, size_t scratch_bytes,
+                              int32_t* counter, void* stream) {
+  UPH_CHECK_ARG(cls && in && out && opt, "uph_island_fit: null argument");
+  UPH_CHECK_ARG(cls->N >= 1 && cls->N <= uph::kMaxMembers, "uph_island_fit: N=%d outside 1..%d",
+                cls->N, uph::kMaxMembers);
+  UPH_CHECK_ARG(cls->B >= 16 && cls->B % 16 == 0, "uph_island_fit: B=%d must be a multiple of 16",
+                cls->B);
+  UPH_CHECK_ARG(cls->T >= 7 && cls->T % 2 == 1, "uph_island_fit: T=%d must be odd >= 7", cls->T);
+  UPH_CHECK_ARG(valid_terms(cls->n_terms), "uph_island_fit: unsupported n_terms=%d",
+                cls->n_terms);
+  UPH_CHECK_ARG(cls->n_bands >= 1, "uph_island_fit: n_bands=%d", cls->n_bands);
+  UPH_CHECK_ARG(cls->n_live_max >= 1 && cls->n_live_max <= 6 * cls->N,
+                "uph_island_fit: n_live_max=%d outside 1..6N", cls->n_live_max);
+  UPH_CHECK_ARG(cls->Hp >= cls->B && cls->Wp >= cls->B, "uph_island_fit: stack %dx%d < B=%d",
+                cls->Hp, cls->Wp, cls->B);
+  UPH_CHECK_ARG(cls->mixed == 0, "uph_island_fit: classes with extended members are not "
+                "supported by this build");
+  UPH_CHECK_ARG(in->psf_coefs != nullptr, "uph_island_fit: null psf_coefs");
+  uph::IslandArgs a{};
+  a.n_isl = cls->n_isl;
+  a.N = cls->N;
+  a.B = cls->B;
+  a.T = cls->T;
+  a.nt = cls->n_terms;
+  a.nb = cls->n_bands;
+  a.nL_max = cls->n_live_max;
+  a.Hp = cls->Hp;
+  a.Wp = cls->Wp;
+  a.psf_coefs = in->psf_coefs;
+  const size_t per = uph_island_scratch_per_block(cls->N, cls->B, cls->T, cls->n_live_max,
+                                                  precision);
+  return island_launch(a, in, out, opt, precision, scratch, scratch_bytes, per, counter, stream);
+}
+
+extern "C" int uph_island_fit_batch(const uph_island_batch* bat, const uph_island_inputs* in,
+                                    const uph_island_outputs* out, const uph_lm_options* opt,
+                                    int precision, void* scratch, size_t scratch_bytes,
+                                    size_t scratch_per_block, int32_t* counter, void* stream) {
+  UPH_CHECK_ARG(bat && in && out && opt, "uph_island_fit_batch: null argument");
+  UPH_CHECK_ARG(bat->shape && bat->mem_off && bat->flux_off,
+                "uph_island_fit_batch: null shape/offset arrays");
+  UPH_CHECK_ARG(bat->n_tiers >= 1 && bat->n_tiers <= 3, "uph_island_fit_batch: n_tiers=%d",
+                bat->n_tiers);
+  UPH_CHECK_ARG(valid_terms(bat->n_terms), "uph_island_fit_batch: unsupported n_terms=%d",
+                bat->n_terms);
+  UPH_CHECK_ARG(bat->n_bands >= 1, "uph_island_fit_batch: n_bands=%d", bat->n_bands);
+  UPH_CHECK_ARG(bat->max_B >= 16 && bat->Hp >= bat->max_B && bat->Wp >= bat->max_B,
+                "uph_island_fit_batch: stack %dx%d smaller than max_B=%d", bat->Hp, bat->Wp,
+                bat->max_B);
+  UPH_CHECK_ARG(bat->mixed == 0, "uph_island_fit_batch: extended members are not supported by "
+                "this build");
+  uph::IslandArgs a{};
+  for (int t = 0; t < bat->n_tiers; ++t) {
+    UPH_CHECK_ARG(bat->tier_T[t] >= 7 && bat->tier_T[t] % 2 == 1 && bat->psf_coefs[t],
+                  "uph_island_fit_batch: tier %d: T=%d / coefs", t, bat->tier_T[t]);
+    a.tier_coefs[t] = bat->psf_coefs[t];
+    a.tier_T[t] = bat->tier_T[t];
+  }
+  UPH_CHECK_ARG(scratch_per_block >= 256 && scratch_per_block % 256 == 0,
+                "uph_island_fit_batch: scratch_per_block=%zu", scratch_per_block);
+  a.n_isl = bat->n_isl;
+  a.nt = bat->n_terms;
+  a.nb = bat->n_bands;
+  a.Hp = bat->Hp;
+  a.Wp = bat->Wp;
+  a.shape = bat->shape;
+  a.mem_off = bat->mem_off;
+  a.flux_off = bat->flux_off;
+  return island_launch(a, in, out, opt, precision, scratch, scratch_bytes, scratch_per_block,
+                       counter, stream);
 }

@@ -45,6 +45,12 @@ struct IslandArgs {
   char* scratch;
   size_t scratch_per_block;
   int* counter;
+  // heterogeneous batches (NULL shape = every island has the class shape)
+  const int32_t* shape;    // (n_isl, 4): N, B, tier index, n_live_max
+  const int64_t* mem_off;  // (n_isl) first member slot of the island
+  const int64_t* flux_off; // (n_isl) first element of its (nb, N) flux block
+  const float* tier_coefs[3];
+  int tier_T[3];
 };
 
 __host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
@@ -52,7 +58,7 @@ __host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~(size
 // Per-block scratch layout (bytes).  Planes are plane-major B*B arrays:
 // [w, d, R, models (N), tangents (nL_max)].
 struct IslandScratch {
-  size_t planes, render, gram, small, total;
+  size_t planes, render, gram, part, small, total;
   int cmax;  // Gram columns: d + models + tangents
 };
 
@@ -67,6 +73,9 @@ __host__ __device__ inline IslandScratch island_scratch_layout(int N, int B, int
   s.cmax = 1 + N + nL;
   s.gram = off;
   off = align256(off + (size_t)s.cmax * s.cmax * 8);
+  const int nt = (s.cmax + 15) / 16;
+  s.part = off;  // matrix-core Gram partial tiles: tile pairs (+ pixel splits)
+  off = align256(off + (size_t)(nt * (nt + 1) / 2 + 4) * 256 * 8);
   s.small = off;
   const int L = nL > N ? nL : N;
   // A_raw, LU, b, f, c, U, Gt, s, D, AD, H, g, Hd, delta, inverse-column ws, pivots

@@ -88,7 +88,7 @@ class BandProblem:
 def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, data_dev,
                    invvar_dev, good_dev, *, bg_order: int = 1, bg_geo=(0.0, 1.0, 0.0, 1.0),
                    img_origin=(0, 0), reg: float = 1e-9, k_max: int = DEFAULT_K_MAX,
-                   stream=None, sersic_dev=None):
+                   stream=None, sersic_dev=None, timer=None):
     """Solve a batch of problems sharing one stamp tier T.  Returns a list of
     SparseForcedPhotResult (host numpy)."""
     import torch
@@ -133,10 +133,13 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
     stamps = torch.empty((max(ntot, 1), T, T), dtype=torch.float32, device=dev)
     d_x, d_y, d_gx, d_gy, d_b = up(x), up(y), up(gx0), up(gy0), up(band)
     if ntot:
+        ev = timer.begin("render_stamps_kernel") if timer is not None else None
         _lib.check(L.uph_render_stamps(ntot, T, n_terms, P(psf_coefs_dev), P(psf_geo_dev),
                                        P(sersic_dev), P(d_b), P(d_x), P(d_y), P(d_gx),
                                        P(d_gy), None, None, None, None, None, 1, P(stamps),
                                        st))
+        if ev is not None:
+            timer.end(ev, dict(kind="render", n=ntot, T=T, n_terms=n_terms))
     max_src = int(ns.max())
     sbytes = L.uph_band_scratch_bytes(len(problems), max_src, bg_order)
     scratch = torch.empty(max(sbytes, 1), dtype=torch.uint8, device=dev)
@@ -146,11 +149,15 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
     bgc = torch.empty((len(problems), M), **f32)
     bgv = torch.empty((len(problems), M), **f32)
     d_off, d_win, d_ix, d_iy, d_nbr = up(off), up(win), up(ixt), up(iyt), up(nbr)
+    ev = timer.begin("band_solve_kernel") if timer is not None else None
     _lib.check(L.uph_band_solve(len(problems), T, bg_order, k_max, P(d_off), P(d_win), H, W,
                                 int(img_origin[0]), int(img_origin[1]), P(data_dev),
                                 P(invvar_dev), P(good_dev), *map(float, bg_geo), P(stamps),
                                 P(d_ix), P(d_iy), P(d_nbr), float(reg), P(fl), P(fv), P(bgc),
                                 P(bgv), P(scratch), scratch.numel(), st))
+    if ev is not None:
+        timer.end(ev, dict(kind="band", n_prob=len(problems), n_src=ntot, T=T,
+                           pix=int(sum(p.h * p.w for p in problems))))
     flh, fvh = fl.cpu().numpy(), fv.cpu().numpy()
     bgh, bvh = bgc.cpu().numpy(), bgv.cpu().numpy()
     out = []

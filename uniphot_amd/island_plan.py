@@ -137,6 +137,7 @@ def class_host_arrays(plan: IslandClassPlan, *, pos_x, pos_y, source_type, logre
                 x0b=(ox - pad).astype(np.float32), y0b=(oy - pad).astype(np.float32),
                 st=st, lr=lr, e1=e1, e2=e2, sn=sn, fn_flag=fn, is_ext=is_ext, mmask=mm,
                 keep=keep, p0=p0, lower=lower, upper=upper, prior_mean=p0.copy(),
+                area=(wb * hb).astype(np.int64),
                 prior_inv_sigma=np.zeros_like(p0), has_priors=False)
 
 

@@ -86,7 +86,7 @@ def psf_tree_device(psfs, tier: int, device="cuda") -> PsfTree:
 
 def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
                     width: int | None, max_steps: int, dstack_dev, wstack_dev, pad: int,
-                    init_state=None, stream=None, sersic_dev=None) -> dict:
+                    init_state=None, stream=None, sersic_dev=None, timer=None) -> dict:
     """Fit rows ``rows`` of a class host dict on the device and return the
     reference's per-row output dict (p, fluxes, ferr, chi2, conv, stall,
     nst, lam, consec, nrej, n_good, steps_run, wall_s)."""
@@ -155,9 +155,13 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
         "p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
         "n_good")))
     if n_c:
+        ev = timer.begin("island_lm_kernel") if timer is not None else None
         _lib.check(L.uph_island_fit(C.byref(cls), C.byref(inp), C.byref(out),
                                     C.byref(kernels.options(max_steps)), prec, P(scratch),
                                     scratch.numel(), P(counter), _lib.stream_ptr(stream)))
+        if ev is not None:
+            timer.end(ev, dict(kind="island", n=n_c, B=B, N=N, nb=nb, outs=outs, host=host,
+                               rows=rows))
     res = {k: v.cpu().numpy() for k, v in outs.items()}
     res["p"] = res["p"].reshape(n_c, N, 6)
     for k in ("conv", "stall"):
@@ -166,3 +170,139 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
     res["steps_run"] = int(res["nst"].max()) if n_c else 0
     res["wall_s"] = time.perf_counter() - t0
     return res
+
+
+def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
+                     precision: str = "fp32", stream=None, sersic_dev=None, timer=None):
+    """Fit the islands of several classes in ONE persistent launch.
+
+    ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
+    maps stamp tier -> PsfTree.  Islands are ordered longest-first
+    (members x window area) for the work queue.  Returns one result dict
+    per class, in the same layout as :func:`run_class_chunk`.
+    """
+    import torch
+    t0 = time.perf_counter()
+    if not classes:
+        return []
+    k0 = classes[0][0]
+    nb, nt = k0.n_bands, k0.n_terms
+    if any(k.kind != "star" for k, _, _ in classes) and sersic_dev is None:
+        raise ValueError("mixed classes need the device Sersic table")
+    tiers = sorted({k.T for k, _, _ in classes})
+    if len(tiers) > 3:
+        raise ValueError(f"at most 3 stamp tiers per batch, got {tiers}")
+    tier_idx = {T: i for i, T in enumerate(tiers)}
+    Hp, Wp = int(dstack_dev.shape[1]), int(dstack_dev.shape[2])
+    dev = dstack_dev.device
+    shape, moff, foff, cost = [], [], [], []
+    cat = {k: [] for k in ("ox", "oy", "x0b", "y0b", "st", "sn", "fn", "mm", "keep", "p0",
+                           "lo", "hi")}
+    m_acc = f_acc = 0
+    spans = []
+    per = 0
+    prec = _lib.UPH_PRECISION_FP64 if precision == "fp64" else _lib.UPH_PRECISION_FP32
+    L = _lib.lib()
+    for kern, host, rows in classes:
+        rows = np.asarray(rows, np.int64)
+        n_c, N, B = rows.size, kern.N, kern.B
+        if kern.n_bands != nb or kern.n_terms != nt:
+            raise ValueError("classes of one batch must share n_bands and n_terms")
+        ox, oy = host["ox"][rows], host["oy"][rows]
+        if n_c and (ox.min() < 0 or oy.min() < 0 or ox.max() + B > Wp or oy.max() + B > Hp):
+            raise ValueError("island window outside the padded stacks")
+        mm = host["mmask"][rows]
+        spans.append((len(shape), n_c, m_acc, f_acc, N))
+        for i in range(n_c):
+            shape.append((N, B, tier_idx[kern.T], kern.n_live_max))
+            moff.append(m_acc + i * N)
+            foff.append(f_acc + i * nb * N)
+        cost.extend((mm.sum(axis=1) * B * B).tolist())
+        m_acc += n_c * N
+        f_acc += n_c * nb * N
+        cat["ox"].append(ox)
+        cat["oy"].append(oy)
+        cat["x0b"].append(host["x0b"][rows])
+        cat["y0b"].append(host["y0b"][rows])
+        cat["st"].append(host["st"][rows].ravel())
+        cat["sn"].append(host["sn"][rows].ravel())
+        cat["fn"].append(host["fn_flag"][rows].ravel())
+        cat["mm"].append(mm.ravel())
+        cat["keep"].append(host["keep"][rows].ravel())
+        cat["p0"].append(host["p0"][rows].ravel())
+        cat["lo"].append(host["lower"][rows].ravel())
+        cat["hi"].append(host["upper"][rows].ravel())
+        per = max(per, L.uph_island_scratch_per_block(N, B, kern.T, kern.n_live_max, prec))
+    n_isl = len(shape)
+    per = (per + 255) // 256 * 256
+
+    def up(a, dt):
+        return torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+
+    order = np.argsort(-np.asarray(cost, np.float64), kind="stable").astype(np.int32)
+    c = {k: np.concatenate(v) for k, v in cat.items()}
+    ins = dict(ox=up(c["ox"], np.int32), oy=up(c["oy"], np.int32), x0b=up(c["x0b"], np.float32),
+               y0b=up(c["y0b"], np.float32), stype=up(c["st"], np.int32),
+               n_frozen=up(c["sn"], np.float32), free_n=up(c["fn"], np.uint8),
+               mmask=up(c["mm"], np.uint8), keep=up(c["keep"], np.uint8),
+               p0=up(c["p0"], np.float32), lower=up(c["lo"], np.float32),
+               upper=up(c["hi"], np.float32), order=up(order, np.int32))
+    d_shape = up(np.asarray(shape, np.int32), np.int32)
+    d_moff = up(np.asarray(moff, np.int64), np.int64)
+    d_foff = up(np.asarray(foff, np.int64), np.int64)
+    f32 = dict(dtype=torch.float32, device=dev)
+    outs = dict(
+        p=torch.empty(m_acc * 6, **f32), fluxes=torch.empty(f_acc, **f32),
+        ferr=torch.empty(f_acc, **f32), chi2=torch.empty(n_isl, **f32),
+        lam=torch.empty(n_isl, **f32), conv=torch.empty(n_isl, dtype=torch.uint8, device=dev),
+        stall=torch.empty(n_isl, dtype=torch.uint8, device=dev),
+        nst=torch.empty(n_isl, dtype=torch.int32, device=dev),
+        consec=torch.empty(n_isl, dtype=torch.int32, device=dev),
+        nrej=torch.empty(n_isl, dtype=torch.int32, device=dev),
+        n_good=torch.empty(n_isl, dtype=torch.int32, device=dev))
+    nblk = max(1, min(n_isl, MAX_BLOCKS, MAX_SCRATCH_BYTES // per))
+    scratch = torch.empty(per * nblk, dtype=torch.uint8, device=dev)
+    counter = torch.zeros(1, dtype=torch.int32, device=dev)
+    P = _lib.ptr
+    bat = _lib.IslandBatch()
+    bat.n_isl, bat.n_bands, bat.n_terms, bat.Hp, bat.Wp = n_isl, nb, nt, Hp, Wp
+    bat.max_B = max(s[1] for s in shape)
+    bat.mixed = int(any(k.kind != "star" for k, _, _ in classes))
+    bat.n_tiers = len(tiers)
+    for i, T in enumerate(tiers):
+        bat.tier_T[i] = T
+        bat.psf_coefs[i] = P(psf_trees[T].coefs)
+    bat.shape, bat.mem_off, bat.flux_off = P(d_shape), P(d_moff), P(d_foff)
+    geo = psf_trees[tiers[0]].geo
+    inp = _lib.IslandInputs(P(dstack_dev), P(wstack_dev), None, P(geo), P(sersic_dev),
+                            *(P(ins[k]) for k in ("ox", "oy", "x0b", "y0b", "stype",
+                                                  "n_frozen", "free_n", "mmask", "keep", "p0",
+                                                  "lower", "upper", "order")))
+    out = _lib.IslandOutputs(*(P(outs[k]) for k in (
+        "p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
+        "n_good")))
+    opts = k0.options(max_steps)
+    ev = timer.begin("island_lm_kernel") if timer is not None else None
+    _lib.check(L.uph_island_fit_batch(C.byref(bat), C.byref(inp), C.byref(out), C.byref(opts),
+                                      prec, P(scratch), scratch.numel(), per, P(counter),
+                                      _lib.stream_ptr(stream)))
+    if ev is not None:
+        area = np.concatenate([h["area"][np.asarray(r)] for _, h, r in classes])
+        mmc = np.concatenate([h["mmask"][np.asarray(r)].sum(axis=1) for _, h, r in classes])
+        timer.end(ev, dict(kind="island_batch", n=n_isl, nb=nb, outs=outs, area=area,
+                           nreal=mmc))
+    h = {k: v.cpu().numpy() for k, v in outs.items()}
+    results = []
+    for (kern, host, rows), (i0, n_c, m0, f0, N) in zip(classes, spans):
+        r = dict(p=h["p"][m0 * 6:(m0 + n_c * N) * 6].reshape(n_c, N, 6),
+                 fluxes=h["fluxes"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N),
+                 ferr=h["ferr"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N))
+        for k in ("chi2", "lam", "conv", "stall", "nst", "consec", "nrej", "n_good"):
+            r[k] = h[k][i0:i0 + n_c]
+        for k in ("conv", "stall"):
+            r[k] = r[k].astype(bool)
+        r["n_good"] = r["n_good"].astype(np.int64)
+        r["steps_run"] = int(r["nst"].max()) if n_c else 0
+        r["wall_s"] = time.perf_counter() - t0
+        results.append(r)
+    return results

@@ -108,7 +108,7 @@ def _src_tier(stype, logre, sersic_n, tiers):
 
 
 def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: FitConfig,
-                tiers):
+                tiers, timer=None):
     """Per-grid, per-band linear solve; returns fluxes/var (n, nb) for each
     source from the grid that owns it, and bg coefs (n_grid, nb, M)."""
     n = x.size
@@ -139,7 +139,7 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
         coefs, geo = fld.psf(T)
         res = fps.solve_problems(problems, coefs, geo, T, fld.n_terms, fld.data, fld.invvar,
                                  fld.good, bg_order=cfg.bg_order, bg_geo=bg_geo, reg=cfg.reg,
-                                 k_max=cfg.k_max)
+                                 k_max=cfg.k_max, timer=timer)
         for (g, b), r in zip(owners, res):
             own = g.source_indices.size
             flux[g.source_indices, b] = r.fluxes[:own]
@@ -175,7 +175,7 @@ def padded_stacks(fld: DeviceField, grids, bg, bg_order, pad):
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,
-              sync=None) -> FitResult:
+              sync=None, timer=None, keep_artifacts=False) -> FitResult:
     """Full island-mode fit of one field (catalog seeds in, fitted fluxes
     out).  ``sync`` is called between stages for timing (e.g.
     torch.cuda.synchronize)."""
@@ -196,7 +196,8 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
         halo = max(halo, int(_src_tier(stype, logre, sn, tiers).max()) // 2 + 3)
     grids = divide_image_into_grids(fld.H, fld.W, cfg.n_rows, cfg.n_cols, halo)
     assign_sources_to_grids(grids, x, y)
-    s1_flux, s1_var, bg = linear_pass(fld, grids, x, y, stype, logre, ee1, ee2, sn, cfg, tiers)
+    s1_flux, s1_var, bg = linear_pass(fld, grids, x, y, stype, logre, ee1, ee2, sn, cfg, tiers,
+                                      timer)
     sync()
     tm["pass1"] = time.perf_counter() - t0
     # ---- islands ----
@@ -232,6 +233,10 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     for gid in giants:
         giant[iset.members[int(gid)]] = True
     n_fitted_islands = 0
+    classes = []
+    batch = []
+    trees = {}
+    from .island_solver import PsfTree, run_island_batch
     for plan in plans:
         if plan.kind != "star":
             raise NotImplementedError("extended members in the island fit are not part of "
@@ -241,11 +246,14 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                                  pos_max_shift_pix=cfg.pos_max_shift_pix, pad=pad)
         kern = make_class_kernels(plan, fld.nb, fld.n_terms, reg=cfg.reg,
                                   precision=cfg.precision)
-        coefs, geo = fld.psf(plan.tier)
-        from .island_solver import PsfTree
-        tree = PsfTree(coefs, geo, plan.tier, fld.n_terms)
-        res = run_class_chunk(kern, tree, host, np.arange(plan.n_islands), None,
-                              cfg.max_steps, dstack, wstack, pad)
+        if plan.tier not in trees:
+            coefs, geo = fld.psf(plan.tier)
+            trees[plan.tier] = PsfTree(coefs, geo, plan.tier, fld.n_terms)
+        batch.append((kern, host, np.arange(plan.n_islands)))
+        classes.append((plan, host))
+    results = run_island_batch(batch, trees, cfg.max_steps, dstack, wstack,
+                               precision=cfg.precision, timer=timer)
+    for (plan, host), res in zip(classes, results):
         n_fitted_islands += plan.n_islands
         mm = plan.mmask
         src = plan.midx[mm]
@@ -265,7 +273,8 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     t3 = time.perf_counter()
     grids2 = divide_image_into_grids(fld.H, fld.W, cfg.n_rows, cfg.n_cols, halo)
     assign_sources_to_grids(grids2, fx, fy)
-    flux, var, _ = linear_pass(fld, grids2, fx, fy, stype, flr, fe1, fe2, fsn, cfg, tiers)
+    flux, var, _ = linear_pass(fld, grids2, fx, fy, stype, flr, fe1, fe2, fsn, cfg, tiers,
+                               timer)
     sync()
     tm["pass2"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
@@ -274,5 +283,8 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                    (p.bbox_class, p.nsrc_class, p.kind, p.tier, p.n_islands) for p in plans],
                knobs_off=["scene_subtract", "blank_response_correction", "wing_peeling",
                           "param_errors", "source_metrics", "marginal_flux_err", "priors"])
+    if keep_artifacts:
+        rec["_artifacts"] = dict(classes=classes, dstack=dstack, wstack=wstack, pad=pad,
+                                 grids=grids, island_set=iset)
     return FitResult(flux, np.sqrt(np.maximum(var, 0.0)), s1_flux, s1_var, fx, fy, flr, fe1,
                      fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec)

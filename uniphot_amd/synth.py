@@ -253,7 +253,7 @@ def scene_c2(seed=SEEDS["c2"], H=2048, W=2048, n_bands=20, n_src=5000, star_only
               for _ in range(n_bands)]
     psfs = [f.fit_polynomial(55, order=order) for f in fields]
     # Thomas process: parents uniform, 1 + Poisson children around them
-    n_par = max(1, int(round(n_src * parent_frac)))
+    n_par = max(1, int(round(n_src * parent_frac * 1.15)))  # margin for edge losses
     px = rng.uniform(0, W - 1, n_par)
     py = rng.uniform(0, H - 1, n_par)
     kids = 1 + rng.poisson(1.0 / parent_frac - 1.0, n_par)
