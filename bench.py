@@ -8,7 +8,7 @@ the dominant kernel's roofline position (algorithmic bytes per launch over
 its measured average duration, HIP events on its stream) and a CPU baseline
 (the float64 oracle timed on a bounded sample of the same workload).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2-stars]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c2-stars|c1]
 """
 from __future__ import annotations
 
@@ -173,11 +173,17 @@ def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0):
         g = grids[rng.integers(len(grids))]
         b = int(rng.integers(fld.nb))
         fi = g.fit_indices
+        T = art["gtier"][g.grid_id]
+        ps = dict(psfs[b], coefs=ocore.pad_psf_coefs(psfs[b]["coefs"], T))
         xs, ys = sc.x[fi], sc.y[fi]
-        nbr, msk = fps.build_neighbor_list(xs, ys, 55, 55)
-        stamps = np.stack([ocore.member_stamp(psfs[b], float(a), float(c), g.x0_pad, g.y0_pad,
-                                              0, renorm=True)[0] for a, c in zip(xs, ys)])
-        ix, iy = fps.stamp_top_left(xs, ys, g.x0_pad, g.y0_pad, 55, 55)
+        all_star = bool(np.all(sc.stype[fi] == 0))
+        nbr, msk = fps.build_neighbor_list(xs, ys, T, T)
+        stamps = np.stack([ocore.member_stamp(ps, float(sc.x[i]), float(sc.y[i]), g.x0_pad,
+                                              g.y0_pad, int(sc.stype[i]), float(sc.logre[i]),
+                                              float(sc.ee1[i]), float(sc.ee2[i]),
+                                              float(sc.sersic_n[i]), renorm=all_star)[0]
+                           for i in fi])
+        ix, iy = fps.stamp_top_left(xs, ys, g.x0_pad, g.y0_pad, T, T)
         basis = olin.bg_basis(g.pad_h, g.pad_w, g.x0_pad, g.y0_pad, *bg_geo, order=1)
         sl = (b, slice(g.y0_pad, g.y1_pad), slice(g.x0_pad, g.x1_pad))
         olin.solve_band(data[sl].astype(np.float64), iv[sl].astype(np.float64), gd[sl], stamps,
@@ -198,7 +204,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2-stars")
+    ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()

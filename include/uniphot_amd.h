@@ -79,7 +79,9 @@ typedef struct uph_island_inputs {
   const float* wstack;    /* (n_bands, Hp, Wp) inverse variance (0 = bad)  */
   const float* psf_coefs; /* (n_bands, n_terms, T, T) planar PSF coefs     */
   const float* psf_geo;   /* (n_bands, 4) x_origin, x_scale, y_origin, y_scale */
-  const float* sersic;    /* Sersic mixture table block; NULL for star-only classes */
+  const double* sersic;   /* Sersic mixture table block (float64): n_knots, n_min,
+                             n_max, K, knots, amps (n_knots, K), log_vars
+                             (n_knots, K), core (n_knots); NULL for star-only */
   const int32_t* ox;      /* (n_isl) window origin x in the padded stack   */
   const int32_t* oy;      /* (n_isl)                                        */
   const float* x0b;       /* (n_isl) window origin x in image coordinates  */
@@ -158,21 +160,25 @@ int uph_island_fit_batch(const uph_island_batch* batch, const uph_island_inputs*
 /*
  * Batched unit-flux stamp renderer: PSF polynomial at (x, y), galaxy
  * mixture convolution for extended sources, unit-sum renormalisation when
- * ``renorm`` (all-star grids), Lanczos-3 shift by the remainder of
+ * ``renorm`` (all-star grids only), Lanczos-3 shift by the remainder of
  * (pos - origin).  Replaces render_source_stamps / render_star_stamps /
  * forced_phot_sparse.render_stamps (source_model.py:331-390,
  * forced_phot_sparse.py:126-220).
  *   src_band (n) : band index of each stamp; x, y (n) full-image position;
  *   gx0, gy0 (n) : origin the sub-pixel remainder is taken against;
- *   stype, logre, ee1, ee2, sersic_n (n) : morphology;
- *   out (n, T, T).
+ *   stype (n) or NULL (all stars); logre, ee1, ee2, sersic_n (n) :
+ *   morphology of extended sources; out (n, T, T).
+ * Extended sources, and stamps too large for one workgroup's LDS (T > 111),
+ * need ``scratch`` of uph_render_scratch_bytes(T, k) bytes (k persistent
+ * workgroups, k >= 1).
  */
+size_t uph_render_scratch_bytes(int T, int n_blocks);
 int uph_render_stamps(int n, int T, int n_terms, const float* psf_coefs, const float* psf_geo,
-                      const float* sersic, const int32_t* src_band, const float* x,
+                      const double* sersic, const int32_t* src_band, const float* x,
                       const float* y, const float* gx0, const float* gy0,
                       const int32_t* stype, const float* logre, const float* ee1,
                       const float* ee2, const float* sersic_n, int renorm, float* out,
-                      void* stream);
+                      void* scratch, size_t scratch_bytes, void* stream);
 
 /*
  * Batched per grid-band linear solve (member fluxes + polynomial background

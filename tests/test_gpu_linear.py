@@ -33,7 +33,7 @@ def test_render_stamps_match_oracle(cuda):
     tc, tg, tb, tx, ty, tgx, tgy = map(up, (c, g, band, sc.x, sc.y, gx0, gy0))
     _lib.check(L.uph_render_stamps(n, 55, c.shape[1], P(tc), P(tg), None, P(tb), P(tx), P(ty),
                                    P(tgx), P(tgy), None, None, None, None, None, 1, P(out),
-                                   _lib.stream_ptr()))
+                                   None, 0, _lib.stream_ptr()))
     got = out.cpu().numpy()
     for i in range(n):
         ref, _ = ocore.member_stamp(_psf_dict(sc.psfs[band[i]]), float(sc.x[i]), float(sc.y[i]),

@@ -51,7 +51,7 @@ def test_fit_field_star_scene(cuda):
     # pass 2 at the GPU-fitted positions: oracle linear solve per grid-band
     halo = cfg.halo
     grids = divide_image_into_grids(240, 240, 2, 2, halo)
-    assign_sources_to_grids(grids, res.x, res.y)
+    assign_sources_to_grids(grids, sc.x, sc.y)   # pass-2 keeps the pass-1 grid fit sets
     bg_geo = (0.5 * 239, 120.0, 0.5 * 239, 120.0)
     for g in grids:
         fi = g.fit_indices

@@ -18,7 +18,7 @@ UPH_PRECISION_FP32, UPH_PRECISION_FP64 = 0, 1
 EXPORTS = (
     "uph_version", "uph_last_error", "uph_island_scratch_per_block", "uph_island_fit",
     "uph_island_fit_batch",
-    "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
+    "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
 )
 
 
@@ -94,9 +94,11 @@ def lib() -> C.CDLL:
     L.uph_island_fit_batch.argtypes = [C.POINTER(IslandBatch), C.POINTER(IslandInputs),
                                        C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,
                                        _P, C.c_size_t, C.c_size_t, _P, _P]
+    L.uph_render_scratch_bytes.restype = C.c_size_t
+    L.uph_render_scratch_bytes.argtypes = [C.c_int, C.c_int]
     L.uph_render_stamps.restype = C.c_int
     L.uph_render_stamps.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P, _P,
-                                    _P, _P, _P, _P, _P, C.c_int, _P, _P]
+                                    _P, _P, _P, _P, _P, C.c_int, _P, _P, C.c_size_t, _P]
     L.uph_band_scratch_bytes.restype = C.c_size_t
     L.uph_band_scratch_bytes.argtypes = [C.c_int, C.c_int, C.c_int]
     L.uph_band_solve.restype = C.c_int

@@ -36,6 +36,7 @@ struct IslandSh {
   int first_live[kMaxMembers + 1];
   Real taps[28];
   Real bas[63];          // PSF basis, d/dX, d/dY (21 each)
+  GalParams<Real> gp;    // extended member being rendered
   Real fR[kMaxMembers];  // member fluxes rounded to Real
   double red[UPH_WAVES * 32];
   double redd[UPH_WAVES];
@@ -370,13 +371,145 @@ __device__ void render_star(const IslandArgs& a, int T, int B, const float* coef
 }
 
 // ---------------------------------------------------------------------------
+// Extended member: PSF polynomial over the whole stamp (and its x/y
+// derivatives), the analytic-FT galaxy convolution with the raw tangents of
+// the member's live parameters, then the same separable Lanczos-3 shift and
+// placement as a star (source_model.py:264-328 with the fft backend,
+// island_solver.py:441-562).  Tangent planes of live parameter l go to
+// tan_base + l * B^2.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, const float* p6,
+                              int r, float x0f, float y0f, Real* __restrict__ m_out,
+                              Real* __restrict__ tan_base, Real* __restrict__ gws) {
+  const int T = sh.T, B = sh.B, nt = a.nt, tid = threadIdx.x, TT = T * T, P = B * B;
+  const bool grads = tan_base != nullptr;
+  int need = 0;
+  if (grads)
+    for (int l = sh.first_live[r]; l < sh.first_live[r + 1]; ++l) need |= 1 << sh.live_j[l];
+  const bool gxy = need & 3;
+  const float* geo = a.psf_geo + 4 * b;
+  const float xf = p6[0], yf = p6[1];
+  const Real x = Real(xf), y = Real(yf);
+  const Real xg = Real(xf - x0f), yg = Real(yf - y0f);
+  const Real rxg = rint(xg), ryg = rint(yg);
+  const Real dx = xg - rxg, dy = yg - ryg;
+  const int ix = int(rxg) - T / 2, iy = int(ryg) - T / 2;
+  const Real xs = Real(geo[1]), ys = Real(geo[3]);
+  if (tid == 0) {
+    GalParams<Real>& gp = sh.gp;
+    double n = double(sh.nfz[r]);
+    gp.dndu = 0.0;
+    if (sh.fn[r]) {
+      const double u = double(p6[5]);
+      n = kNMin + (kNMax - kNMin) / (1.0 + exp(-u));
+      gp.dndu = (n - kNMin) * (kNMax - n) / (kNMax - kNMin);
+    }
+    gp.n = n;
+    gp.K = int(a.sersic[3]);
+    mog_lookup(a.sersic, n, gp.a, gp.v, gp.core, gp.da, gp.dv, gp.dcore);
+    ellipse_terms(double(p6[2]), double(p6[3]), double(p6[4]), gp.t, gp.dt);
+  }
+  if (tid == 64) lanczos_taps(dx, sh.taps, grads ? sh.taps + 7 : (Real*)nullptr);
+  if (tid == 128) lanczos_taps(dy, sh.taps + 14, grads ? sh.taps + 21 : (Real*)nullptr);
+  if (tid == 192) {
+    const Real X = (x - Real(geo[0])) / xs, Y = (y - Real(geo[2])) / ys;
+    psf_basis(X, Y, nt, sh.bas, sh.bas + 21, sh.bas + 42);
+  }
+  __syncthreads();
+  const GalLayout L = galaxy_layout(T);
+  const float* Cf = sh.coefs + (size_t)b * nt * TT;
+  for (int e = tid; e < TT; e += UPH_BLOCK) {
+    Real v = 0, vx = 0, vy = 0;
+    for (int t = 0; t < nt; ++t) {
+      const Real cv = Real(Cf[(size_t)t * TT + e]);
+      v = fma(cv, sh.bas[t], v);
+      if (gxy) {
+        vx = fma(cv, sh.bas[21 + t], vx);
+        vy = fma(cv, sh.bas[42 + t], vy);
+      }
+    }
+    gws[L.P + e] = v;
+    if (gxy) {
+      gws[L.Px + e] = vx / xs;
+      gws[L.Py + e] = vy / ys;
+    }
+  }
+  __syncthreads();
+  galaxy_raw(sh.gp, T, need, gws);
+  // ---- separable shift + placement of G and the raw tangents ----
+  const int wx0 = max(0, ix), wx1 = min(B, ix + T), wy0 = max(0, iy), wy1 = min(B, iy + T);
+  if (wx1 <= wx0 || wy1 <= wy0) return;  // uniform
+  const int sx0 = wx0 - ix, sx1 = wx1 - ix, sy0 = wy0 - iy, sy1 = wy1 - iy;
+  const int ry0 = max(0, sy0 - 3), ry1 = min(T, sy1 + 3);
+  const int RH = ry1 - ry0, SW = sx1 - sx0, SH = sy1 - sy0;
+  Real Lx[7], dLx[7], Ly[7], dLy[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    Lx[k] = sh.taps[k];
+    Ly[k] = sh.taps[14 + k];
+    dLx[k] = grads ? sh.taps[7 + k] : Real(0);
+    dLy[k] = grads ? sh.taps[21 + k] : Real(0);
+  }
+  const Real* G = gws + L.G;
+  Real* Hh = gws + L.Hh;  // plane q of the horizontal pass at Hh + q * T^2
+  const int RS = RH * SW;
+  for (int e = tid; e < RS; e += UPH_BLOCK) {
+    const int rr = ry0 + e / SW, c = sx0 + e % SW;
+    Real h[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int cc = c + k - 3;
+      if (cc < 0 || cc >= T) continue;
+      const int o = rr * T + cc;
+      h[0] = fma(Lx[k], G[o], h[0]);
+      if (need & 1) h[1] = fma(Lx[k], G[TT + o], fma(dLx[k], G[o], h[1]));
+#pragma unroll
+      for (int j = 1; j < 6; ++j)
+        if (need & (1 << j)) h[1 + j] = fma(Lx[k], G[(size_t)(1 + j) * TT + o], h[1 + j]);
+    }
+    Hh[e] = h[0];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      if (need & (1 << j)) Hh[(size_t)(1 + j) * TT + e] = h[1 + j];
+  }
+  __syncthreads();
+  const Real dndu = Real(sh.gp.dndu);
+  for (int e = tid; e < SH * SW; e += UPH_BLOCK) {
+    const int rw = sy0 + e / SW, cc = e % SW;
+    Real v[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int rr = rw + k - 3;
+      if (rr < ry0 || rr >= ry1) continue;
+      const int o = (rr - ry0) * SW + cc;
+      const Real h0 = Hh[o];
+      v[0] = fma(Ly[k], h0, v[0]);
+      if (need & 2) v[2] = fma(Ly[k], Hh[(size_t)2 * TT + o], fma(dLy[k], h0, v[2]));
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        if (j != 1 && (need & (1 << j))) v[1 + j] = fma(Ly[k], Hh[(size_t)(1 + j) * TT + o], v[1 + j]);
+    }
+    const int wi = (rw + iy) * B + (cc + sx0 + ix);
+    m_out[wi] = v[0];
+    if (grads) {
+      for (int l = sh.first_live[r]; l < sh.first_live[r + 1]; ++l) {
+        const int j = sh.live_j[l];
+        tan_base[(size_t)l * P + wi] = j == 5 ? v[6] * dndu : v[1 + j];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
 // One band of one island: window copy, member renders, Gram, VarPro fluxes,
 // residual, chi2 and (with grads) accumulation of H, g.
 // ---------------------------------------------------------------------------
 template <typename Real>
 __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int b,
                           const float* pv, bool grads, bool accumulate, Real* planes, Real* ws,
-                          double* G, double* part, const Small& sm, float* flux_out,
+                          Real* gws, double* G, double* part, const Small& sm, float* flux_out,
                           float* ferr_out, int* ngood) {
   const int B = sh.B, P = B * B, tid = threadIdx.x;
   const int Nr = sh.Nr, nL = grads ? sh.nL : 0;
@@ -424,9 +557,8 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
       render_star<Real>(a, sh.T, B, sh.coefs, b, xk, yk, x0f, y0f, Mp + (size_t)r * P,
                         grads ? tx : nullptr, grads ? ty : nullptr, wsx, sh.taps, sh.bas);
     } else {
-      render_galaxy<Real>(a, b, pv + 6 * k, sh.st[r], sh.nfz[r], sh.fn[r], x0f, y0f,
-                          Mp + (size_t)r * P, grads ? Tp : nullptr, sh.live_j, sh.first_live[r],
-                          sh.first_live[r + 1], ws, sh.taps, sh.bas);
+      render_galaxy<Real>(a, sh, b, pv + 6 * k, r, x0f, y0f, Mp + (size_t)r * P,
+                          grads ? Tp : nullptr, gws);
     }
   }
   // Gram of [d, models, tangents] under W.
@@ -553,10 +685,11 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
 
 template <typename Real>
 __device__ float chi2_all(const IslandArgs& a, IslandSh<Real>& sh, int isl, const float* pv,
-                          Real* planes, Real* ws, double* G, double* part, const Small& sm) {
+                          Real* planes, Real* ws, Real* gws, double* G, double* part,
+                          const Small& sm) {
   Real c = 0;
   for (int b = 0; b < a.nb; ++b)
-    c += eval_band<Real>(a, sh, isl, b, pv, false, false, planes, ws, G, part, sm, nullptr,
+    c += eval_band<Real>(a, sh, isl, b, pv, false, false, planes, ws, gws, G, part, sm, nullptr,
                          nullptr, nullptr);
   return float(c);
 }
@@ -605,6 +738,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     double* G = reinterpret_cast<double*>(base + lay.gram);
     double* part = reinterpret_cast<double*>(base + lay.part);
     const Small sm = carve_small(base + lay.small, N, sh.nLc);
+    Real* gws = reinterpret_cast<Real*>(base + lay.total);  // galaxy workspace (mixed)
     const size_t m0 = (size_t)sh.moff;
     // ---- island description into LDS ----
     for (int q = tid; q < nP; q += UPH_BLOCK) {
@@ -643,7 +777,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     if (sh.Nr == 0) sh.conv = 1;  // nothing to fit (all slots padded)
     __syncthreads();
     if (sh.Nr > 0) {
-      const float c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, G, part, sm);
+      const float c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, gws, G, part, sm);
       if (tid == 0) sh.chi2 = c0;
       __syncthreads();
     } else if (tid == 0) {
@@ -654,7 +788,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
       const int nL = sh.nL;
       // H, g at p (un-damped, un-masked over the live set)
       for (int b = 0; b < a.nb; ++b)
-        eval_band<Real>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, G, part, sm, nullptr,
+        eval_band<Real>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, gws, G, part, sm, nullptr,
                         nullptr, nullptr);
       // damped float64 solve: (H + lam diag(H) + 1e-8 I) delta = -g
       const double lam = double(sh.lam);
@@ -684,7 +818,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
         for (int q = 0; q < nP; ++q) sh.ptry[q] = fminf(fmaxf(sh.ptry[q], sh.lo[q]), sh.hi[q]);
       }
       __syncthreads();
-      const float ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, G, part, sm);
+      const float ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, gws, G, part, sm);
       if (tid == 0) {
         const float cur = sh.chi2;
         const bool accept = ct < cur;
@@ -727,7 +861,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     int ngood = 0;
     if (sh.Nr > 0) {
       for (int b = 0; b < a.nb; ++b)
-        eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, G, part, sm, fo + b * N,
+        eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, gws, G, part, sm, fo + b * N,
                         eo + b * N, &ngood);
     }
     for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[m0 * 6 + q] = sh.p[q];
@@ -751,7 +885,8 @@ extern "C" size_t uph_island_scratch_per_block(int N, int B, int T, int n_live_m
                                                int precision) {
   const size_t rb = precision == UPH_PRECISION_FP64 ? 8 : 4;
   size_t base = uph::island_scratch_layout(N, B, T, n_live_max, rb).total;
-  return base + uph::galaxy_scratch_bytes(T, rb);
+  // n_live_max > 2N marks a class with extended members
+  return base + (n_live_max > 2 * N ? uph::galaxy_scratch_bytes(T, rb) : 0);
 }
 
 static int island_launch(uph::IslandArgs& a, const uph_island_inputs* in,
@@ -844,8 +979,8 @@ extern "C" int uph_island_fit(const uph_island_class* cls, const uph_island_inpu
                 "uph_island_fit: n_live_max=%d outside 1..6N", cls->n_live_max);
   UPH_CHECK_ARG(cls->Hp >= cls->B && cls->Wp >= cls->B, "uph_island_fit: stack %dx%d < B=%d",
                 cls->Hp, cls->Wp, cls->B);
-  UPH_CHECK_ARG(cls->mixed == 0, "uph_island_fit: classes with extended members are not "
-                "supported by this build");
+  UPH_CHECK_ARG(!cls->mixed || (in->sersic != nullptr && cls->n_live_max > 2 * cls->N),
+                "uph_island_fit: mixed classes need the Sersic table and n_live_max > 2N");
   UPH_CHECK_ARG(in->psf_coefs != nullptr, "uph_island_fit: null psf_coefs");
   uph::IslandArgs a{};
   a.n_isl = cls->n_isl;
@@ -878,8 +1013,8 @@ extern "C" int uph_island_fit_batch(const uph_island_batch* bat, const uph_islan
   UPH_CHECK_ARG(bat->max_B >= 16 && bat->Hp >= bat->max_B && bat->Wp >= bat->max_B,
                 "uph_island_fit_batch: stack %dx%d smaller than max_B=%d", bat->Hp, bat->Wp,
                 bat->max_B);
-  UPH_CHECK_ARG(bat->mixed == 0, "uph_island_fit_batch: extended members are not supported by "
-                "this build");
+  UPH_CHECK_ARG(!bat->mixed || in->sersic != nullptr,
+                "uph_island_fit_batch: mixed islands need the Sersic table");
   uph::IslandArgs a{};
   for (int t = 0; t < bat->n_tiers; ++t) {
     UPH_CHECK_ARG(bat->tier_T[t] >= 7 && bat->tier_T[t] % 2 == 1 && bat->psf_coefs[t],

@@ -16,6 +16,7 @@
 
 #include "uph_capi_internal.hpp"
 #include "uph_device.hpp"
+#include "uph_galaxy.hpp"
 #include "uph_island.hpp"
 
 namespace uph {
@@ -23,14 +24,15 @@ namespace uph {
 __global__ __launch_bounds__(UPH_BLOCK) void render_stamps_kernel(
     int n, int T, int nt, const float* __restrict__ coefs, const float* __restrict__ geo_all,
     const int32_t* __restrict__ src_band, const float* __restrict__ xs_, const float* __restrict__ ys_,
-    const float* __restrict__ gx0, const float* __restrict__ gy0, int renorm,
-    float* __restrict__ out) {
+    const float* __restrict__ gx0, const float* __restrict__ gy0, const int32_t* __restrict__ stype,
+    int renorm, float* __restrict__ out) {
   extern __shared__ float lds[];
   __shared__ float taps[14];
   __shared__ float bas[21];
   __shared__ float red[UPH_WAVES];
   const int s = blockIdx.x, tid = threadIdx.x, TT = T * T;
   if (s >= n) return;
+  if (stype && stype[s] != 0) return;  // extended sources: render_galaxy_stamps_kernel
   const int b = src_band[s];
   const float* geo = geo_all + 4 * b;
   const float x = xs_[s], y = ys_[s];
@@ -86,6 +88,90 @@ __global__ __launch_bounds__(UPH_BLOCK) void render_stamps_kernel(
       if (rr >= 0 && rr < T) v = fmaf(Ly[k], h[rr * T + c], v);
     }
     o[e] = v;
+  }
+}
+
+// Extended-source stamps (EXP/DEV/SERSIC): raw PSF at (x, y), analytic-FT
+// galaxy convolution, separable Lanczos-3 shift; never renormalised
+// (source_model.py:264-328 fft backend, forced_phot_sparse.py:624-637).
+// Persistent: each workgroup loops over the stamps with its own workspace.
+__global__ __launch_bounds__(UPH_BLOCK) void render_galaxy_stamps_kernel(
+    int n, int T, int nt, const float* __restrict__ coefs, const float* __restrict__ geo_all,
+    const double* __restrict__ sersic, const int32_t* __restrict__ src_band,
+    const float* __restrict__ xs_, const float* __restrict__ ys_, const float* __restrict__ gx0,
+    const float* __restrict__ gy0, const int32_t* __restrict__ stype,
+    const float* __restrict__ logre, const float* __restrict__ ee1,
+    const float* __restrict__ ee2, const float* __restrict__ sersic_n, int stars_too, int renorm,
+    float* __restrict__ out, char* scratch, size_t per_block) {
+  __shared__ GalParams<float> gp;
+  __shared__ float taps[14];
+  __shared__ float bas[21];
+  __shared__ float red[UPH_WAVES];
+  const int tid = threadIdx.x, TT = T * T;
+  float* ws = reinterpret_cast<float*>(scratch + (size_t)blockIdx.x * per_block);
+  const GalLayout L = galaxy_layout(T);
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    const bool star = stype == nullptr || stype[s] == 0;
+    if (star && !stars_too) continue;  // uniform per block
+    const int b = src_band[s];
+    const float* geo = geo_all + 4 * b;
+    const float x = xs_[s], y = ys_[s];
+    const float xg = x - gx0[s], yg = y - gy0[s];
+    const float dx = xg - rintf(xg), dy = yg - rintf(yg);
+    if (tid == 0 && !star) {
+      gp.n = double(sersic_n[s]);
+      gp.dndu = 0.0;
+      gp.K = int(sersic[3]);
+      mog_lookup(sersic, gp.n, gp.a, gp.v, gp.core, gp.da, gp.dv, gp.dcore);
+      ellipse_terms(double(logre[s]), double(ee1[s]), double(ee2[s]), gp.t, gp.dt);
+    }
+    if (tid == 64) lanczos_taps(dx, taps, (float*)nullptr);
+    if (tid == 128) lanczos_taps(dy, taps + 7, (float*)nullptr);
+    if (tid == 192) {
+      const float X = (x - geo[0]) / geo[1], Y = (y - geo[2]) / geo[3];
+      psf_basis(X, Y, nt, bas, (float*)nullptr, (float*)nullptr);
+    }
+    __syncthreads();
+    const float* C = coefs + (size_t)b * nt * TT;
+    float part = 0.0f;
+    float* rawp = star ? ws + L.G : ws + L.P;
+    for (int e = tid; e < TT; e += UPH_BLOCK) {
+      float v = 0.0f;
+      for (int t = 0; t < nt; ++t) v = fmaf(C[(size_t)t * TT + e], bas[t], v);
+      rawp[e] = v;
+      part += v;
+    }
+    if (star && renorm) {
+      const float den = fmaxf(block_sum(part, red), 1e-12f);
+      for (int e = tid; e < TT; e += UPH_BLOCK) rawp[e] = rawp[e] / den;
+    }
+    __syncthreads();
+    if (!star) galaxy_raw(gp, T, 0, ws);
+    const float* G = ws + L.G;
+    float* h = ws + L.Hh;
+    for (int e = tid; e < TT; e += UPH_BLOCK) {
+      const int r = e / T, c = e % T;
+      float v = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const int cc = c + k - 3;
+        if (cc >= 0 && cc < T) v = fmaf(taps[k], G[r * T + cc], v);
+      }
+      h[e] = v;
+    }
+    __syncthreads();
+    float* o = out + (size_t)s * TT;
+    for (int e = tid; e < TT; e += UPH_BLOCK) {
+      const int r = e / T, c = e % T;
+      float v = 0.0f;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const int rr = r + k - 3;
+        if (rr >= 0 && rr < T) v = fmaf(taps[7 + k], h[rr * T + c], v);
+      }
+      o[e] = v;
+    }
+    __syncthreads();
   }
 }
 
@@ -319,28 +405,29 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_solve_kernel(BandArgs a) {
 
 static int n_terms_of_order(int order) { return (order + 1) * (order + 2) / 2; }
 
+extern "C" size_t uph_render_scratch_bytes(int T, int n_blocks) {
+  return (size_t)n_blocks * uph::galaxy_scratch_bytes(T, sizeof(float));
+}
+
 extern "C" int uph_render_stamps(int n, int T, int n_terms, const float* psf_coefs,
-                                 const float* psf_geo, const float* sersic,
+                                 const float* psf_geo, const double* sersic,
                                  const int32_t* src_band, const float* x, const float* y,
                                  const float* gx0, const float* gy0, const int32_t* stype,
                                  const float* logre, const float* ee1, const float* ee2,
-                                 const float* sersic_n, int renorm, float* out, void* stream) {
-  (void)sersic;
-  (void)logre;
-  (void)ee1;
-  (void)ee2;
-  (void)sersic_n;
-  (void)stype;
+                                 const float* sersic_n, int renorm, float* out, void* scratch,
+                                 size_t scratch_bytes, void* stream) {
   UPH_CHECK_ARG(n >= 0, "uph_render_stamps: n=%d", n);
-  UPH_CHECK_ARG(T >= 7 && T % 2 == 1 && T <= 111, "uph_render_stamps: T=%d must be odd in 7..111",
+  UPH_CHECK_ARG(T >= 7 && T % 2 == 1 && T <= 511, "uph_render_stamps: T=%d must be odd in 7..511",
                 T);
   UPH_CHECK_ARG(n_terms == 1 || n_terms == 3 || n_terms == 6 || n_terms == 10 || n_terms == 15 ||
                     n_terms == 21,
                 "uph_render_stamps: unsupported n_terms=%d", n_terms);
   UPH_CHECK_ARG(psf_coefs && psf_geo && src_band && x && y && gx0 && gy0 && out,
                 "uph_render_stamps: null pointer");
-  UPH_CHECK_ARG(stype == nullptr, "uph_render_stamps: extended sources are not supported by this "
-                "build (pass stype = NULL for an all-star batch)");
+  const bool ext = stype != nullptr;
+  UPH_CHECK_ARG(!ext || (logre && ee1 && ee2 && sersic_n && sersic && scratch),
+                "uph_render_stamps: extended sources need logre, ee1, ee2, sersic_n, the Sersic "
+                "table and scratch");
   if (n == 0) return UPH_OK;
   const size_t lds = (size_t)2 * T * T * sizeof(float);
   int dev = 0, lds_max = 0;
@@ -350,13 +437,28 @@ extern "C" int uph_render_stamps(int n, int T, int n_terms, const float* psf_coe
       hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
       "uph_render_stamps attribute");
   if (st0 != UPH_OK) return st0;
-  UPH_CHECK_ARG(lds + 512 <= (size_t)lds_max,
-                "uph_render_stamps: T=%d needs %zu bytes of LDS, device allows %d", T, lds,
-                lds_max);
+  // stars whose stamp does not fit LDS go through the persistent workspace path
+  const bool stars_lds = lds + 512 <= (size_t)lds_max;
+  UPH_CHECK_ARG(stars_lds || scratch != nullptr,
+                "uph_render_stamps: T=%d stamps need scratch (uph_render_scratch_bytes)", T);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  hipLaunchKernelGGL(uph::render_stamps_kernel, dim3(n), dim3(UPH_BLOCK), lds, s, n, T, n_terms,
-                     psf_coefs, psf_geo, src_band, x, y, gx0, gy0, renorm, out);
-  return uph::hip_status(hipGetLastError(), "uph_render_stamps launch");
+  if (stars_lds) {
+    hipLaunchKernelGGL(uph::render_stamps_kernel, dim3(n), dim3(UPH_BLOCK), lds, s, n, T,
+                       n_terms, psf_coefs, psf_geo, src_band, x, y, gx0, gy0, stype, renorm,
+                       out);
+    int st = uph::hip_status(hipGetLastError(), "uph_render_stamps launch");
+    if (st != UPH_OK || !ext) return st;
+  }
+  const size_t per = uph::galaxy_scratch_bytes(T, sizeof(float));
+  const size_t nb_s = scratch_bytes / per;
+  UPH_CHECK_ARG(nb_s >= 1, "uph_render_stamps: scratch of %zu bytes < one workgroup (%zu)",
+                scratch_bytes, per);
+  const int nblk = (int)(nb_s < (size_t)n ? nb_s : (size_t)n);
+  hipLaunchKernelGGL(uph::render_galaxy_stamps_kernel, dim3(nblk), dim3(UPH_BLOCK), 0, s, n, T,
+                     n_terms, psf_coefs, psf_geo, sersic, src_band, x, y, gx0, gy0, stype, logre,
+                     ee1, ee2, sersic_n, stars_lds ? 0 : 1, renorm, out,
+                     static_cast<char*>(scratch), per);
+  return uph::hip_status(hipGetLastError(), "uph_render_stamps galaxy launch");
 }
 
 extern "C" size_t uph_band_scratch_bytes(int n_prob, int max_src, int bg_order) {

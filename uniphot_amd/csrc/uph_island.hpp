@@ -17,7 +17,7 @@ struct IslandArgs {
   const float* wstack;
   const float* psf_coefs;  // (nb, nt, T, T)
   const float* psf_geo;    // (nb, 4): x_origin, x_scale, y_origin, y_scale
-  const float* sersic;     // Sersic table block (see uph_galaxy.hpp), may be null
+  const double* sersic;    // Sersic table block (see uph_galaxy.hpp), may be null
   const int* ox;
   const int* oy;
   const float* x0b;

@@ -122,24 +122,53 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
     nbr = np.concatenate([p.nbr_idx[:, :k_max] for p in problems]).astype(np.int32) if ntot \
         else np.zeros((0, k_max), np.int32)
     win = np.array([[p.band, p.x0, p.y0, p.h, p.w] for p in problems], np.int32)
-    all_star = [p.stype is None or bool(np.all(p.stype == SOURCE_STAR)) for p in problems]
-    if not all(all_star):
-        raise NotImplementedError("extended sources in the linear solve are not part of "
-                                  "this build")
+    mixed_flag = np.array([p.stype is not None and bool(np.any(p.stype != SOURCE_STAR))
+                           for p in problems])
     up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     L = _lib.lib()
     P = _lib.ptr
     st = _lib.stream_ptr(stream)
     stamps = torch.empty((max(ntot, 1), T, T), dtype=torch.float32, device=dev)
-    d_x, d_y, d_gx, d_gy, d_b = up(x), up(y), up(gx0), up(gy0), up(band)
-    if ntot:
+    # all-star problems renormalise their stamps, problems with an extended
+    # source do not (forced_phot_sparse.py:624-637): one render call per kind
+    for mixed in (False, True):
+        sel = np.where(mixed_flag == mixed)[0]
+        idx = (np.concatenate([np.arange(off[q], off[q + 1]) for q in sel]).astype(np.int64)
+               if sel.size else np.zeros(0, np.int64))
+        if idx.size == 0:
+            continue
+        r_x, r_y, r_gx, r_gy, r_b = (up(v[idx]) for v in (x, y, gx0, gy0, band))
+        s_out = torch.empty((idx.size, T, T), dtype=torch.float32, device=dev)
+        ext = [None] * 5
+        scratch_r, sb = None, 0
+        if T > 111:   # stamps beyond one workgroup's LDS use the workspace path
+            sb = L.uph_render_scratch_bytes(T, max(1, min(int(idx.size), 256)))
+            scratch_r = torch.empty(sb, dtype=torch.uint8, device=dev)
+        if mixed:
+            def gather(attr, dt, fill):
+                return np.concatenate([
+                    getattr(problems[q], attr) if getattr(problems[q], attr) is not None
+                    else np.full(problems[q].pos_x.size, fill) for q in sel]).astype(dt)
+            ext = [up(gather("stype", np.int32, 0)), up(gather("logre", np.float32, 0.0)),
+                   up(gather("ee1", np.float32, 0.0)), up(gather("ee2", np.float32, 0.0)),
+                   up(gather("sersic_n", np.float32, 1.0))]
+            n_gal = int((ext[0] != SOURCE_STAR).sum().item())
+            if scratch_r is None:
+                sb = L.uph_render_scratch_bytes(T, max(1, min(n_gal, 1024)))
+                scratch_r = torch.empty(sb, dtype=torch.uint8, device=dev)
+            if sersic_dev is None:
+                from .sersic_table import device_table
+                sersic_dev = device_table(dev)
         ev = timer.begin("render_stamps_kernel") if timer is not None else None
-        _lib.check(L.uph_render_stamps(ntot, T, n_terms, P(psf_coefs_dev), P(psf_geo_dev),
-                                       P(sersic_dev), P(d_b), P(d_x), P(d_y), P(d_gx),
-                                       P(d_gy), None, None, None, None, None, 1, P(stamps),
-                                       st))
+        _lib.check(L.uph_render_stamps(int(idx.size), T, n_terms, P(psf_coefs_dev),
+                                       P(psf_geo_dev), P(sersic_dev) if mixed else None,
+                                       P(r_b), P(r_x), P(r_y), P(r_gx), P(r_gy),
+                                       *(P(e) for e in ext), 0 if mixed else 1, P(s_out),
+                                       P(scratch_r), sb, st))
         if ev is not None:
-            timer.end(ev, dict(kind="render", n=ntot, T=T, n_terms=n_terms))
+            timer.end(ev, dict(kind="render", n=int(idx.size), T=T, n_terms=n_terms,
+                               mixed=bool(mixed)))
+        stamps.index_copy_(0, up(idx), s_out)
     max_src = int(ns.max())
     sbytes = L.uph_band_scratch_bytes(len(problems), max_src, bg_order)
     scratch = torch.empty(max(sbytes, 1), dtype=torch.uint8, device=dev)

@@ -107,23 +107,30 @@ def _src_tier(stype, logre, sersic_n, tiers):
     return tier
 
 
+def grid_tiers(fld: DeviceField, grids, stype, logre, sn, tiers):
+    """Stamp tier of each grid: the largest requirement of its fit sources
+    at the catalog seeds (pipeline.py:1609-1621, :2014-2019)."""
+    src_tier = _src_tier(stype, logre, sn, tiers)
+    return {g.grid_id: (int(src_tier[g.fit_indices].max()) if g.fit_indices.size else
+                        fld.native) for g in grids}
+
+
 def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: FitConfig,
-                tiers, timer=None):
+                gtier: dict, timer=None):
     """Per-grid, per-band linear solve; returns fluxes/var (n, nb) for each
-    source from the grid that owns it, and bg coefs (n_grid, nb, M)."""
+    source from the grid that owns it, and bg coefs (n_grid, nb, M).  The
+    grids, their fit sets and stamp tiers are those of pass 1 (pass 2 only
+    replaces the source parameters, island_stage.py:3331-3363)."""
     n = x.size
     nb = fld.nb
     M = (cfg.bg_order + 1) * (cfg.bg_order + 2) // 2
     flux = np.full((n, nb), np.nan, np.float32)
     var = np.full((n, nb), np.nan, np.float32)
     bg = np.zeros((len(grids), nb, M), np.float32)
-    src_tier = _src_tier(stype, logre, sn, tiers)
     bg_geo = (0.5 * (fld.W - 1), max(0.5 * fld.W, 1.0), 0.5 * (fld.H - 1), max(0.5 * fld.H, 1.0))
     by_tier = {}
     for g in grids:
-        fi = g.fit_indices
-        T = int(src_tier[fi].max()) if fi.size else fld.native
-        by_tier.setdefault(T, []).append(g)
+        by_tier.setdefault(gtier[g.grid_id], []).append(g)
     for T, gl in sorted(by_tier.items()):
         problems, owners = [], []
         for g in gl:
@@ -196,7 +203,8 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
         halo = max(halo, int(_src_tier(stype, logre, sn, tiers).max()) // 2 + 3)
     grids = divide_image_into_grids(fld.H, fld.W, cfg.n_rows, cfg.n_cols, halo)
     assign_sources_to_grids(grids, x, y)
-    s1_flux, s1_var, bg = linear_pass(fld, grids, x, y, stype, logre, ee1, ee2, sn, cfg, tiers,
+    gtier = grid_tiers(fld, grids, stype, logre, sn, tiers)
+    s1_flux, s1_var, bg = linear_pass(fld, grids, x, y, stype, logre, ee1, ee2, sn, cfg, gtier,
                                       timer)
     sync()
     tm["pass1"] = time.perf_counter() - t0
@@ -238,9 +246,6 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     trees = {}
     from .island_solver import PsfTree, run_island_batch
     for plan in plans:
-        if plan.kind != "star":
-            raise NotImplementedError("extended members in the island fit are not part of "
-                                      "this build")
         host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
                                  ee1=ee1, ee2=ee2, sersic_n=sn, free_n_all=free_n,
                                  pos_max_shift_pix=cfg.pos_max_shift_pix, pad=pad)
@@ -251,8 +256,12 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
             trees[plan.tier] = PsfTree(coefs, geo, plan.tier, fld.n_terms)
         batch.append((kern, host, np.arange(plan.n_islands)))
         classes.append((plan, host))
+    sersic_dev = None
+    if any(p.kind != "star" for p in plans):
+        from .sersic_table import device_table
+        sersic_dev = device_table(dstack.device)
     results = run_island_batch(batch, trees, cfg.max_steps, dstack, wstack,
-                               precision=cfg.precision, timer=timer)
+                               precision=cfg.precision, timer=timer, sersic_dev=sersic_dev)
     for (plan, host), res in zip(classes, results):
         n_fitted_islands += plan.n_islands
         mm = plan.mmask
@@ -271,10 +280,7 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     tm["island_fit"] = time.perf_counter() - t2
     # ---- pass 2 ----
     t3 = time.perf_counter()
-    grids2 = divide_image_into_grids(fld.H, fld.W, cfg.n_rows, cfg.n_cols, halo)
-    assign_sources_to_grids(grids2, fx, fy)
-    flux, var, _ = linear_pass(fld, grids2, fx, fy, stype, flr, fe1, fe2, fsn, cfg, tiers,
-                               timer)
+    flux, var, _ = linear_pass(fld, grids, fx, fy, stype, flr, fe1, fe2, fsn, cfg, gtier, timer)
     sync()
     tm["pass2"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
@@ -285,6 +291,6 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                           "param_errors", "source_metrics", "marginal_flux_err", "priors"])
     if keep_artifacts:
         rec["_artifacts"] = dict(classes=classes, dstack=dstack, wstack=wstack, pad=pad,
-                                 grids=grids, island_set=iset)
+                                 grids=grids, island_set=iset, gtier=gtier)
     return FitResult(flux, np.sqrt(np.maximum(var, 0.0)), s1_flux, s1_var, fx, fy, flr, fe1,
                      fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec)

@@ -12,7 +12,7 @@ below n = 0.8 are fits from The Tractor that exist only as data (the
 expansion itself degrades there), so this table starts at n = 0.8 and
 lookups below it clamp to the n = 0.8 mixture.
 
-Device block layout (float32, see uph_galaxy.hpp):
+Device block layout (float64, see uph_galaxy.hpp):
   [0] n_knots, [1] n_min, [2] n_max, [3] K, then knots (n_knots),
   amps (n_knots, K), log_vars (n_knots, K), core (n_knots).
 """
@@ -97,9 +97,21 @@ def mog_params_at(n: float):
 
 
 def device_block() -> np.ndarray:
-    """Flat float32 block of the table for the kernels."""
+    """Flat float64 block of the table for the kernels."""
     t = table()
     nk = t["n_knots"].size
     head = np.array([nk, t["n_min"], t["n_max"], K_MAX], np.float64)
     return np.concatenate([head, t["n_knots"], t["amps"].ravel(), t["log_vars"].ravel(),
-                           t["core_amp"]]).astype(np.float32)
+                           t["core_amp"]]).astype(np.float64)
+
+
+_DEVICE_BLOCKS = {}
+
+
+def device_table(device="cuda"):
+    """The float64 table block resident on ``device`` (uploaded once)."""
+    import torch
+    key = str(device)
+    if key not in _DEVICE_BLOCKS:
+        _DEVICE_BLOCKS[key] = torch.from_numpy(device_block()).to(device)
+    return _DEVICE_BLOCKS[key]
