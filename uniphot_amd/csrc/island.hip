@@ -436,7 +436,7 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
     }
   }
   __syncthreads();
-  galaxy_raw(sh.gp, T, need, gws);
+  galaxy_raw<Real, false>(sh.gp, T, need, gws);
   // ---- separable shift + placement of G and the raw tangents ----
   const int wx0 = max(0, ix), wx1 = min(B, ix + T), wy0 = max(0, iy), wy1 = min(B, iy + T);
   if (wx1 <= wx0 || wy1 <= wy0) return;  // uniform

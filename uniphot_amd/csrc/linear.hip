@@ -146,7 +146,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void render_galaxy_stamps_kernel(
       for (int e = tid; e < TT; e += UPH_BLOCK) rawp[e] = rawp[e] / den;
     }
     __syncthreads();
-    if (!star) galaxy_raw(gp, T, 0, ws);
+    if (!star) galaxy_raw<float, true>(gp, T, 0, ws);
     const float* G = ws + L.G;
     float* h = ws + L.Hh;
     for (int e = tid; e < TT; e += UPH_BLOCK) {

@@ -175,8 +175,72 @@ __device__ __forceinline__ void sincospi_r(Real x, Real* s, Real* c) {
     sincospi(x, s, c);
 }
 
+// LDS capacity of the line-batch FFT (complex elements) and the largest
+// transform length it serves.
+constexpr int kFftLdsComplex = 6144;
+constexpr int kFftMaxN = 512;
+
 template <typename Real>
-__device__ void fft_lines(Real* buf, int n, int logn, int nl, int ls, int es, int sign) {
+__device__ void fft_lines_lds(Real* buf, int n, int logn, int nl, int ls, int es, int sign) {
+  // Lines are processed in batches that fit LDS: load (bit-reversed),
+  // log2(n) radix-2 stages in LDS with a shared twiddle table, store.
+  constexpr int CAP = kFftLdsComplex * 4 / (int)sizeof(Real);
+  __shared__ Real lb[2 * CAP];
+  __shared__ Real tw[kFftMaxN];  // n/2 complex twiddles exp(sign 2 pi i j / n)
+  const int tid = threadIdx.x;
+  for (int j = tid; j < n / 2; j += UPH_BLOCK) {
+    Real sn, cs;
+    sincospi_r(Real(2 * j) / Real(n), &sn, &cs);
+    tw[2 * j] = cs;
+    tw[2 * j + 1] = Real(sign) * sn;
+  }
+  const int chunk = CAP / n;
+  const bool col = (ls == 1);  // adjacent lines adjacent in memory
+  for (int l0 = 0; l0 < nl; l0 += chunk) {
+    const int lc = min(chunk, nl - l0);
+    for (int e = tid; e < lc * n; e += UPH_BLOCK) {
+      const int line = col ? e % lc : e / n;
+      const int i = col ? e / lc : e % n;
+      const int j = (int)(__brev((unsigned)i) >> (32 - logn));
+      const Real* src = buf + 2 * ((size_t)(l0 + line) * ls + (size_t)i * es);
+      lb[2 * (line * n + j)] = src[0];
+      lb[2 * (line * n + j) + 1] = src[1];
+    }
+    __syncthreads();
+    const int nh = n >> 1;
+    for (int s = 1; s <= logn; ++s) {
+      const int half = 1 << (s - 1);
+      const int tstep = n >> s;
+      for (int e = tid; e < lc * nh; e += UPH_BLOCK) {
+        const int line = e / nh, k = e % nh;
+        const int j = k & (half - 1);
+        const int i0 = line * n + ((k >> (s - 1)) << s) + j, i1 = i0 + half;
+        const Real wr = tw[2 * j * tstep], wi = tw[2 * j * tstep + 1];
+        const Real qr = lb[2 * i1], qi = lb[2 * i1 + 1];
+        const Real br = wr * qr - wi * qi, bi = wr * qi + wi * qr;
+        const Real ar = lb[2 * i0], ai = lb[2 * i0 + 1];
+        lb[2 * i0] = ar + br;
+        lb[2 * i0 + 1] = ai + bi;
+        lb[2 * i1] = ar - br;
+        lb[2 * i1 + 1] = ai - bi;
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < lc * n; e += UPH_BLOCK) {
+      const int line = col ? e % lc : e / n;
+      const int i = col ? e / lc : e % n;
+      Real* dst = buf + 2 * ((size_t)(l0 + line) * ls + (size_t)i * es);
+      dst[0] = lb[2 * (line * n + i)];
+      dst[1] = lb[2 * (line * n + i) + 1];
+    }
+    __syncthreads();
+  }
+}
+
+// Block-cooperative radix-2 FFT working directly on the (global) buffer:
+// in-place bit reversal, then log2(n) stages with a barrier each.
+template <typename Real>
+__device__ void fft_lines_block(Real* buf, int n, int logn, int nl, int ls, int es, int sign) {
   const int tid = threadIdx.x;
   for (int e = tid; e < nl * n; e += UPH_BLOCK) {
     const int l = e / n, i = e % n;
@@ -215,9 +279,118 @@ __device__ void fft_lines(Real* buf, int n, int logn, int nl, int ls, int es, in
   }
 }
 
+// Wave-level FFT of length n = 64 E: each wavefront owns one line at a time,
+// lane l holding elements l + 64 e (e < E) in registers.  Bit-reversed
+// gather on load, then decimation-in-time stages: spans 1..32 across lanes
+// (shuffles), spans 64.. inside the lane; natural-order store.  No LDS
+// allocation and no barrier inside a line; the 4 waves take lines
+// round-robin.  Twiddles depend only on the lane (and e for in-lane spans)
+// and are computed once per call.
+template <typename Real, int E>
+__device__ void fft_lines_wave(Real* buf, int nl, int ls, int es, int sign) {
+  constexpr int LOGE = E == 1 ? 0 : (E == 2 ? 1 : (E == 4 ? 2 : 3));
+  constexpr int LOGN = 6 + LOGE;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  Real cw[6][2];        // cross-lane stage twiddles (span h = 1 .. 32)
+  Real iw[LOGE + 1][E][2];  // in-lane stage twiddles (span 64 he)
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int h = 1 << s;
+    Real sn, cs;
+    sincospi_r(Real(lane & (h - 1)) / Real(h), &sn, &cs);
+    cw[s][0] = cs;
+    cw[s][1] = Real(sign) * sn;
+  }
+#pragma unroll
+  for (int t = 0; t < LOGE; ++t) {
+    const int he = 1 << t;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = lane + 64 * (e & (he - 1));
+      Real sn, cs;
+      sincospi_r(Real(j) / Real(64 * he), &sn, &cs);
+      iw[t][e][0] = cs;
+      iw[t][e][1] = Real(sign) * sn;
+    }
+  }
+  for (int line = wid; line < nl; line += UPH_WAVES) {
+    Real re[E], im[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = lane + 64 * e;
+      const int src = (int)(__brev((unsigned)i) >> (32 - LOGN));
+      const Real* p = buf + 2 * ((size_t)line * ls + (size_t)src * es);
+      re[e] = p[0];
+      im[e] = p[1];
+    }
+#pragma unroll
+    for (int s = 0; s < 6; ++s) {
+      const int h = 1 << s;
+      const Real wr = cw[s][0], wi = cw[s][1];
+      const bool upper = lane & h;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const Real pr = __shfl_xor(re[e], h, 64), pim = __shfl_xor(im[e], h, 64);
+        if (!upper) {
+          re[e] = re[e] + (wr * pr - wi * pim);
+          im[e] = im[e] + (wr * pim + wi * pr);
+        } else {
+          const Real tr = wr * re[e] - wi * im[e], ti = wr * im[e] + wi * re[e];
+          re[e] = pr - tr;
+          im[e] = pim - ti;
+        }
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < LOGE; ++t) {
+      const int he = 1 << t;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if (e & he) continue;
+        const int f = e + he;
+        const Real wr = iw[t][e][0], wi = iw[t][e][1];
+        const Real br = wr * re[f] - wi * im[f], bi = wr * im[f] + wi * re[f];
+        const Real ar = re[e], ai = im[e];
+        re[e] = ar + br;
+        im[e] = ai + bi;
+        re[f] = ar - br;
+        im[f] = ai - bi;
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int i = lane + 64 * e;
+      Real* p = buf + 2 * ((size_t)line * ls + (size_t)i * es);
+      p[0] = re[e];
+      p[1] = im[e];
+    }
+  }
+  __syncthreads();
+}
+
+// In-place FFT of nl lines (natural order in and out).  kLds selects the
+// LDS line-batch variant (used by the grid stamp renderer); otherwise the
+// wave-level register FFT (n = 128, 256, 512) or the block FFT.
+template <typename Real, bool kLds>
+__device__ void fft_lines(Real* buf, int n, int logn, int nl, int ls, int es, int sign) {
+  if constexpr (kLds) {
+    fft_lines_lds(buf, n, logn, nl, ls, es, sign);
+  } else {
+    __syncthreads();  // lines may have been written by other waves
+    if (n == 128)
+      fft_lines_wave<Real, 2>(buf, nl, ls, es, sign);
+    else if (n == 256)
+      fft_lines_wave<Real, 4>(buf, nl, ls, es, sign);
+    else if (n == 512)
+      fft_lines_wave<Real, 8>(buf, nl, ls, es, sign);
+    else
+      fft_lines_block(buf, n, logn, nl, ls, es, sign);
+  }
+}
+
 // rfft2 of the (T, T) real plane R zero-padded to (pad, pad): spectrum S of
 // (pad, H = pad/2 + 1) complex, row-major.  RB: (T, pad) complex scratch.
-template <typename Real>
+template <typename Real, bool kLds>
 __device__ void rfft2_pad(const Real* R, int T, int pad, int logp, Real* RB, Real* S) {
   const int tid = threadIdx.x, H = pad / 2 + 1;
   for (int e = tid; e < T * pad; e += UPH_BLOCK) {
@@ -226,21 +399,21 @@ __device__ void rfft2_pad(const Real* R, int T, int pad, int logp, Real* RB, Rea
     RB[2 * e + 1] = Real(0);
   }
   __syncthreads();
-  fft_lines(RB, pad, logp, T, pad, 1, -1);
+  fft_lines<Real, kLds>(RB, pad, logp, T, pad, 1, -1);
   for (int e = tid; e < pad * H; e += UPH_BLOCK) {
     const int r = e / H, k = e % H;
     S[2 * e] = r < T ? RB[2 * (r * pad + k)] : Real(0);
     S[2 * e + 1] = r < T ? RB[2 * (r * pad + k) + 1] : Real(0);
   }
   __syncthreads();
-  fft_lines(S, pad, logp, H, 1, H, -1);
+  fft_lines<Real, kLds>(S, pad, logp, H, 1, H, -1);
 }
 
 // Real output O (T, T) = irfft2(Z, (pad, pad))[:T, :T]; Z is destroyed.
-template <typename Real>
+template <typename Real, bool kLds>
 __device__ void irfft2_crop(Real* Z, int T, int pad, int logp, Real* RB, Real* O) {
   const int tid = threadIdx.x, H = pad / 2 + 1;
-  fft_lines(Z, pad, logp, H, 1, H, +1);
+  fft_lines<Real, kLds>(Z, pad, logp, H, 1, H, +1);
   for (int e = tid; e < T * pad; e += UPH_BLOCK) {
     const int r = e / pad, c = e % pad;
     Real re, im;
@@ -255,7 +428,7 @@ __device__ void irfft2_crop(Real* Z, int T, int pad, int logp, Real* RB, Real* O
     RB[2 * e + 1] = im;
   }
   __syncthreads();
-  fft_lines(RB, pad, logp, T, pad, 1, +1);
+  fft_lines<Real, kLds>(RB, pad, logp, T, pad, 1, +1);
   const Real norm = Real(1) / (Real(pad) * Real(pad));
   for (int e = tid; e < T * T; e += UPH_BLOCK) {
     const int r = e / T, c = e % T;
@@ -328,16 +501,16 @@ __device__ void spec_shape(const Real* S, const Real* F1, const GalParams<Real>&
 // `need` (bit 0 x, 1 y, 2 logre, 3 ee1, 4 ee2, 5 n).  Planes in ws at
 // GalLayout offsets: G + 0 (model), G + (1 + j) T^2 for parameter j.
 // Requires P (and Px, Py for x/y) already in ws and gp filled.
-template <typename Real>
+template <typename Real, bool kLds>
 __device__ void galaxy_raw(const GalParams<Real>& gp, int T, int need, Real* ws) {
   const GalLayout L = galaxy_layout(T);
   const int pad = L.pad, logp = ilog2(pad), n = pad * L.H, TT = T * T;
   const bool shape = need & (4 | 8 | 16), nder = need & 32;
   fill_fourier(gp, pad, shape, nder, ws + L.F);
-  rfft2_pad(ws + L.P, T, pad, logp, ws + L.RB, ws + L.SP);
+  rfft2_pad<Real, kLds>(ws + L.P, T, pad, logp, ws + L.RB, ws + L.SP);
   Real* Gm = ws + L.G;
   spec_scale(ws + L.SP, ws + L.F, n, ws + L.Z);
-  irfft2_crop(ws + L.Z, T, pad, logp, ws + L.RB, Gm);
+  irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, Gm);
   const Real core = Real(gp.core);
   for (int e = threadIdx.x; e < TT; e += UPH_BLOCK) Gm[e] += core * ws[L.P + e];
   __syncthreads();
@@ -345,22 +518,22 @@ __device__ void galaxy_raw(const GalParams<Real>& gp, int T, int need, Real* ws)
     if (!(need & (1 << j))) continue;
     Real* src = ws + (j == 0 ? L.Px : L.Py);
     Real* spec = ws + (j == 0 ? L.SPx : L.SPy);
-    rfft2_pad(src, T, pad, logp, ws + L.RB, spec);
+    rfft2_pad<Real, kLds>(src, T, pad, logp, ws + L.RB, spec);
     spec_scale(spec, ws + L.F, n, ws + L.Z);
     Real* o = Gm + (size_t)(1 + j) * TT;
-    irfft2_crop(ws + L.Z, T, pad, logp, ws + L.RB, o);
+    irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, o);
     for (int e = threadIdx.x; e < TT; e += UPH_BLOCK) o[e] += core * src[e];
     __syncthreads();
   }
   for (int p = 0; p < 3; ++p) {
     if (!(need & (4 << p))) continue;
     spec_shape(ws + L.SP, ws + L.F + n, gp, p, pad, ws + L.Z);
-    irfft2_crop(ws + L.Z, T, pad, logp, ws + L.RB, Gm + (size_t)(3 + p) * TT);
+    irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, Gm + (size_t)(3 + p) * TT);
   }
   if (nder) {
     spec_scale(ws + L.SP, ws + L.F + 2 * n, n, ws + L.Z);
     Real* o = Gm + (size_t)6 * TT;
-    irfft2_crop(ws + L.Z, T, pad, logp, ws + L.RB, o);
+    irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, o);
     const Real dcore = Real(gp.dcore);
     for (int e = threadIdx.x; e < TT; e += UPH_BLOCK) o[e] += dcore * ws[L.P + e];
     __syncthreads();
