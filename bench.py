@@ -112,6 +112,20 @@ def algorithmic_cost(info):
     return 0.0, 0.0
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of ``kernel`` from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE runs, each scaled by what the same counter
+    read for a plain copy of known size), or None if there is none."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    fetch_scale = t["copy_bytes"] / (t["copy_fetch_kib"] * 1024.0)
+    write_scale = t["copy_bytes"] / (t["copy_write_kib"] * 1024.0)
+    return (t["fetch_kib"] * 1024.0 * fetch_scale + t["write_kib"] * 1024.0 * write_scale)
+
+
 def make_scene(config):
     from uniphot_amd import synth
     if config == "c2-stars":
@@ -207,6 +221,9 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="run one plain device copy of this many MiB before the steps, as the "
+                         "known-size reference for FETCH_SIZE / WRITE_SIZE counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -223,6 +240,13 @@ def main():
     fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
     cfg = FitConfig(**gcfg)
     sync = torch.cuda.synchronize
+    if args.copy_calib_mib > 0:
+        src = torch.ones(args.copy_calib_mib << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        sync()
+        dst.copy_(src)
+        sync()
+        del src, dst
     args_fit = (sc.x, sc.y, sc.stype, sc.logre, sc.ee1, sc.ee2, sc.sersic_n, cfg)
     for i in range(args.warmup):
         r = fit_field(fld, *args_fit, sync=sync)
@@ -256,7 +280,8 @@ def main():
                    "grid": [cfg.n_rows, cfg.n_cols], "islands": res.record["n_islands"],
                    "stage_s": {k: round(v, 4) for k, v in res.timings.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                     "traffic": measured_traffic(args.config, "island_lm_kernel"),
                      "kernel": "island_lm_kernel", "avg_launch_ms": round(avg_s * 1e3, 4),
                      "launches": isl["launches"],
                      "algorithmic_bytes_per_launch": per_launch_bytes,
