@@ -1,0 +1,219 @@
+"""Host-side checks that need no GPU: the C-ABI library loads and exports
+every entry point the header declares, the ctypes structs have the C
+layout (compiled against the header with gcc), argument errors come back
+as status codes with a message before any device work, and the host
+planning logic (islands, buckets, grids, stamp tiers, PSF device layout)
+agrees with brute-force restatements."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from uniphot_amd import _lib
+from uniphot_amd.grid import assign_sources_to_grids, divide_image_into_grids
+from uniphot_amd.island_plan import plan_island_classes
+from uniphot_amd.islands import (ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands,
+                                 build_islands)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "uniphot_amd.h")
+
+
+def _header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(uph_[a-z0-9_]+)\s*\(", txt)))
+
+
+@pytest.fixture(scope="module")
+def L():
+    if not _lib.LIB_PATH.exists():
+        from uniphot_amd import build
+        build.build()
+    return _lib.lib()
+
+
+def test_library_exports_every_declared_entry_point(L):
+    declared = _header_functions()
+    assert declared == sorted(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.uph_version() >= 100
+
+
+_STRUCTS = {
+    "uph_lm_options": _lib.LMOptions,
+    "uph_island_class": _lib.IslandClass,
+    "uph_island_inputs": _lib.IslandInputs,
+    "uph_island_outputs": _lib.IslandOutputs,
+    "uph_island_batch": _lib.IslandBatch,
+}
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="needs gcc")
+def test_ctypes_structs_match_the_c_layout(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"',
+             "int main(void) {"]
+    for cname, py in _STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'  printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for ln in out.splitlines():
+        s, f, v = ln.split()
+        got[(s, f)] = int(v)
+    for cname, py in _STRUCTS.items():
+        assert got[(cname, "size")] == C.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])
+
+
+def test_scratch_size_queries(L):
+    a = L.uph_island_scratch_per_block(4, 48, 55, 8, 0)
+    b = L.uph_island_scratch_per_block(4, 48, 55, 8, 1)
+    assert a % 256 == 0 and b > a                       # float64 planes are larger
+    assert L.uph_island_scratch_per_block(4, 64, 55, 8, 0) > a
+    gal = L.uph_island_scratch_per_block(4, 48, 55, 24, 0)
+    assert gal > L.uph_island_scratch_per_block(4, 48, 55, 8, 0)  # galaxy workspace
+    r1, r4 = L.uph_render_scratch_bytes(111, 1), L.uph_render_scratch_bytes(111, 4)
+    assert r4 == 4 * r1 > 0
+    assert L.uph_band_scratch_bytes(10, 40, 1) > L.uph_band_scratch_bytes(10, 20, 1) > 0
+
+
+def test_argument_errors_return_status_and_message(L):
+    st = L.uph_island_fit(None, None, None, None, 0, None, 0, None, None)
+    assert st == 1 and b"null argument" in L.uph_last_error()
+    opts = _lib.LMOptions(1e-9, 0.5, 0.15, 0.5, 1e-4, 0.1, 0.01, 30)
+    cls = _lib.IslandClass()
+    cls.n_isl, cls.N, cls.B, cls.T, cls.n_terms, cls.n_bands = 1, 3, 40, 55, 15, 2
+    st = L.uph_island_fit(C.byref(cls), C.byref(_lib.IslandInputs()),
+                          C.byref(_lib.IslandOutputs()), C.byref(opts), 0, None, 0, None, None)
+    assert st == 1 and b"multiple of 16" in L.uph_last_error()
+    with pytest.raises(_lib.UphError) as ei:
+        _lib.check(L.uph_render_stamps(4, 54, 15, *([None] * 13), 0, None, None, 0, None))
+    assert ei.value.code == 1 and "odd" in str(ei.value)
+    with pytest.raises(_lib.UphError):
+        _lib.check(L.uph_band_solve(-1, 55, 1, 32, *([None] * 2), 0, 0, 0, 0, *([None] * 3),
+                                    0.0, 1.0, 0.0, 1.0, *([None] * 4), 1e-9, *([None] * 4),
+                                    None, 0, None))
+
+
+# ---------------------------------------------------------------------------
+# host planning logic
+
+
+def _union_find_labels(px, py, r):
+    n = px.size
+    parent = list(range(n))
+
+    def find(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for i in range(n):
+        for j in range(i + 1, n):
+            if (px[i] - px[j]) ** 2 + (py[i] - py[j]) ** 2 < (r[i] + r[j]) ** 2:
+                a, b = find(i), find(j)
+                if a != b:
+                    parent[max(a, b)] = min(a, b)
+    roots = [find(i) for i in range(n)]
+    # label islands in order of their smallest member index
+    order = {}
+    for i in range(n):
+        order.setdefault(roots[i], len(order))
+    return np.array([order[roots[i]] for i in range(n)])
+
+
+def test_islands_match_brute_force_union_find():
+    rng = np.random.default_rng(5)
+    par = rng.uniform(0, 300, (60, 2))
+    pts = np.concatenate([par, par[rng.integers(60, size=90)] + rng.normal(0, 5, (90, 2))])
+    px, py = pts[:, 0], pts[:, 1]
+    r = rng.uniform(2, 9, px.size)
+    iset = build_islands(px, py, None, None, None, 27.5, radius=r)
+    ref = _union_find_labels(px, py, r)
+    np.testing.assert_array_equal(iset.labels, ref)
+    for k, m in enumerate(iset.members):
+        assert m.min() == np.where(ref == k)[0].min()
+        assert iset.bbox[k, 0] == np.floor((px[m] - r[m]).min())
+        assert iset.bbox[k, 3] == np.ceil((py[m] + r[m]).max())
+
+
+def test_bucketing_picks_the_smallest_class_or_giant():
+    rng = np.random.default_rng(8)
+    par = rng.uniform(0, 2000, (400, 2))
+    k = rng.integers(1, 40, 400)
+    pts = np.concatenate([par[i] + rng.normal(0, 2 + k[i] / 4, (k[i], 2)) for i in range(400)])
+    r = np.full(len(pts), 4.0)
+    iset = build_islands(pts[:, 0], pts[:, 1], None, None, None, 27.5, radius=r)
+    bk = bucket_islands(iset)
+    seen = np.zeros(iset.n_islands, int)
+    for (B, N), b in bk.buckets.items():
+        for row, i in enumerate(b.island_ids):
+            seen[i] += 1
+            size, n = iset.bbox_size[i], iset.n_src[i]
+            assert size <= B and n <= N
+            smaller_b = [c for c in ISLAND_BBOX_CLASSES if c < B]
+            smaller_n = [c for c in ISLAND_NSRC_CLASSES if c < N]
+            assert not smaller_b or size > max(smaller_b)
+            assert not smaller_n or n > max(smaller_n)
+            assert b.member_mask[row].sum() == n
+    for i in bk.giant_ids:
+        seen[i] += 1
+        assert iset.bbox_size[i] > max(ISLAND_BBOX_CLASSES) or iset.n_src[i] > max(
+            ISLAND_NSRC_CLASSES)
+    assert np.all(seen == 1)
+
+
+def test_grid_tiling_owns_each_source_once():
+    grids = divide_image_into_grids(2048, 2000, 8, 7, 30)
+    rng = np.random.default_rng(1)
+    x, y = rng.uniform(0, 2000, 3000), rng.uniform(0, 2048, 3000)
+    assign_sources_to_grids(grids, x, y)
+    owned = np.concatenate([g.source_indices for g in grids])
+    assert np.array_equal(np.sort(owned), np.arange(3000))
+    cover = np.zeros((2048, 2000), int)
+    for g in grids:
+        cover[g.y0_core:g.y1_core, g.x0_core:g.x1_core] += 1
+        assert g.x0_pad == max(0, g.x0_core - 30) and g.y1_pad == min(2048, g.y1_core + 30)
+        h = g.halo_source_indices
+        assert np.all((x[h] >= g.x0_pad) & (x[h] < g.x1_pad) & (y[h] >= g.y0_pad)
+                      & (y[h] < g.y1_pad))
+    assert np.all(cover == 1)
+
+
+def test_stamp_tier_covers_twice_the_profile_extent():
+    rng = np.random.default_rng(2)
+    n = 300
+    px, py = rng.uniform(0, 3000, n), rng.uniform(0, 3000, n)
+    st = rng.integers(0, 4, n).astype(np.int32)
+    sn = np.where(st == 2, 4.0, np.where(st == 3, rng.uniform(0.5, 6, n), 1.0))
+    lr = np.log(rng.uniform(1, 12, n))
+    iset = build_islands(px, py, st, lr, sn, 27.5, radius=np.full(n, 20.0))
+    plans, capped = plan_island_classes(iset, bucket_islands(iset), st, lr, sn, 55)
+    for p in plans:
+        for row in range(p.n_islands):
+            m = p.midx[row][p.mmask[row]]
+            gal = m[st[m] != 0]
+            if p.kind == "star":
+                assert gal.size == 0 and p.tier == 55
+                continue
+            nre = np.where(sn[gal] < 1.5, 4.0, np.where(sn[gal] < 3.0, 6.0, 8.0))
+            need = 2 * (nre * np.exp(lr[gal])).max()
+            ladder = (55, 111, 221)
+            want = next((t for t in ladder if need <= t), 221)
+            assert p.tier == want

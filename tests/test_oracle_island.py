@@ -1,0 +1,197 @@
+"""Pins of the float64 island oracle that do not restate its formulas:
+VarPro fluxes against numpy.linalg.lstsq, the analytic Gauss-Newton (H, g)
+against central finite differences of the VarPro residual, and LM
+behaviour (monotone chi2 over accepts, bounds, clips, frozen and padded
+slots, the stall and step-size convergence prongs) on islands with known
+answers."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import core
+from oracle import island as oisl
+
+from _scenes import mixed_classes, oracle_island, small_mixed_scene, small_star_scene, star_classes
+
+
+def _psf_dict(p, T):
+    return dict(coefs=core.pad_psf_coefs(p.coefs.astype(np.float64), T), x_origin=p.x_origin,
+                x_scale=p.x_scale, y_origin=p.y_origin, y_scale=p.y_scale)
+
+
+def test_varpro_fluxes_match_lstsq():
+    rng = np.random.default_rng(3)
+    N, B = 5, 24
+    m = rng.uniform(0, 1, (N, B, B)) ** 3
+    d = rng.normal(0, 1, (B, B)) + np.einsum("k,kij->ij", rng.uniform(1, 9, N), m)
+    w = rng.uniform(0.2, 2.0, (B, B))
+    w[3:7, 5:11] = 0.0                       # masked pixels carry no weight
+    mmask = np.array([1, 1, 0, 1, 1], bool)  # slot 2 is padding: no model
+    m[~mmask] = 0.0
+    f, A64, _ = oisl.fluxes_band(m, d, w, mmask, 1e-9)
+    sw = np.sqrt(w).ravel()
+    X = (m[mmask].reshape(mmask.sum(), -1) * sw).T
+    ref, *_ = np.linalg.lstsq(X, d.ravel() * sw, rcond=None)
+    np.testing.assert_allclose(f[mmask], ref, rtol=1e-7, atol=1e-9)
+    assert f[2] == 0.0
+    assert A64[2, 2] == A64[2, 2] and A64[2, 2] >= 1.0  # padded slot: unit diagonal
+
+
+def _varpro_residual(p, isl, o):
+    out = []
+    for b in range(isl.nb):
+        m, _ = oisl.models_band(p, isl, b, False)
+        f, _, _ = oisl.fluxes_band(m, isl.data[b], isl.w[b], isl.mmask, o.reg)
+        out.append((np.sqrt(isl.w[b]) * (isl.data[b] - np.einsum("k,kij->ij", f, m))).ravel())
+    return np.concatenate(out)
+
+
+def _away_from_half(p, isl):
+    """Shift positions so that x - x0 is not near a half-integer (the
+    integer placement changes there, which finite differences straddle)."""
+    P = p.reshape(isl.N, 6).copy()
+    for k in range(isl.N):
+        for j, o0 in ((0, isl.x0), (1, isl.y0)):
+            fr = (P[k, j] - o0) % 1.0
+            if abs(fr - 0.5) < 0.08:
+                P[k, j] += 0.15
+    return P.ravel()
+
+
+@pytest.mark.parametrize("kind", ["star", "mixed"])
+def test_gauss_newton_matches_finite_differences(kind):
+    if kind == "star":
+        sc = small_star_scene(seed=11, n_bands=2, n_src=6, order=1)
+        plans, hosts, pad, d, w = star_classes(sc)
+    else:
+        sc = small_mixed_scene(seed=5, n_bands=2, n_src=6, order=1, types=(0, 1, 3))
+        free = np.asarray(sc.stype == 3)
+        plans, hosts, pad, d, w = mixed_classes(sc, free_n=free)
+    # the island with the most live parameters
+    best = max(((pi, r) for pi, pl in enumerate(plans) for r in range(pl.n_islands)),
+               key=lambda t: int(hosts[t[0]]["keep"][t[1]].sum()))
+    plan, host = plans[best[0]], hosts[best[0]]
+    isl = oracle_island(plan, host, best[1], d, w, sc.psfs)
+    o = oisl.LMOptions()
+    p = _away_from_half(host["p0"][best[1]].ravel().astype(np.float64), isl)
+    H, g, chi2, live = oisl.hg(p, isl, o)
+    r0 = _varpro_residual(p, isl, o)
+    assert chi2 == pytest.approx(float(r0 @ r0), rel=1e-12)
+    J = np.zeros((len(live), r0.size))
+    for i, (k, j) in enumerate(live):
+        h = 1e-5
+        pp, pm = p.copy(), p.copy()
+        pp[6 * k + j] += h
+        pm[6 * k + j] -= h
+        J[i] = (_varpro_residual(pp, isl, o) - _varpro_residual(pm, isl, o)) / (2 * h)
+    g_fd = J @ r0
+    H_fd = J @ J.T
+    scale_g = np.abs(g_fd).max()
+    np.testing.assert_allclose(g, g_fd, rtol=2e-4, atol=2e-5 * scale_g)
+    np.testing.assert_allclose(H, H_fd, rtol=2e-4, atol=2e-5 * np.abs(H_fd).max())
+    if kind == "mixed":
+        assert any(j >= 2 for (_, j) in live)   # shape parameters were exercised
+
+
+def _noiseless_island(offset, seed=2):
+    """One two-star island whose data is exactly the model at the truth
+    (float64 oracle render, unit weights); seeds offset by ``offset`` px."""
+    sc = small_star_scene(seed=seed, n_bands=2, n_src=6, order=1)
+    plans, hosts, pad, d, w = star_classes(sc)
+    pi, r = next((pi, r) for pi, pl in enumerate(plans) for r in range(pl.n_islands)
+                 if hosts[pi]["mmask"][r].sum() == 2)
+    plan, host = plans[pi], hosts[pi]
+    isl = oracle_island(plan, host, r, d, w, sc.psfs)
+    p_true = _away_from_half(host["p0"][r].ravel().astype(np.float64), isl)
+    fl = np.array([[2e4, 5e3], [1e4, 8e3]])
+    mm = isl.mmask
+    for b in range(isl.nb):
+        m, _ = oisl.models_band(p_true, isl, b, False)
+        f = np.zeros(isl.N)
+        f[np.where(mm)[0]] = fl[b]
+        isl.data[b] = np.einsum("k,kij->ij", f, m)
+        isl.w[b] = np.ones_like(isl.w[b])
+    p0 = p_true.copy()
+    P = p0.reshape(isl.N, 6)
+    P[mm, 0] += offset
+    P[mm, 1] -= offset
+    return isl, p_true, p0
+
+
+def test_lm_converges_on_step_prong_from_a_small_offset():
+    isl, p_true, p0 = _noiseless_island(0.03)
+    o = oisl.LMOptions()
+    st = oisl.run_island(p0, isl, o, trace=True)
+    h = st["history"]
+    assert h[0]["accept"] and st["conv"] and not st["stall"]
+    assert st["nst"] == 1                       # pos step 0.04 px < 0.1, shape step 0
+    P = st["p"].reshape(isl.N, 6)[isl.mmask]
+    T = p_true.reshape(isl.N, 6)[isl.mmask]
+    assert np.abs(P[:, :2] - T[:, :2]).max() < 1e-3
+    f, _ = oisl.final_fluxes(st["p"], isl, o)
+    np.testing.assert_allclose(f[:, isl.mmask], [[2e4, 5e3], [1e4, 8e3]], rtol=1e-3)
+
+
+def test_lm_stalls_after_four_rejects_when_steps_are_clipped_to_zero():
+    isl, _, p0 = _noiseless_island(0.3)
+    o = oisl.LMOptions(pos_step_max=0.0, shape_step_max=0.0, un_step_max=0.0)
+    st = oisl.run_island(p0, isl, o, trace=True)
+    h = st["history"]
+    # p_try == p gives chi2_try == chi2, which the strict test rejects
+    assert [s["accept"] for s in h] == [False] * 4
+    assert [s["nrej"] for s in h] == [1, 2, 3, 4]
+    np.testing.assert_allclose([s["lam"] for s in h], [1e-2, 1e-1, 1.0, 10.0])
+    assert st["conv"] and st["stall"] and st["nst"] == 4
+    np.testing.assert_array_equal(st["p"], p0)
+
+
+def test_lm_invariants_on_a_noisy_mixed_island():
+    sc = small_mixed_scene(seed=9, n_bands=2, n_src=8, order=1)
+    plans, hosts, pad, d, w = mixed_classes(sc)
+    pi, r = max(((pi, r) for pi, pl in enumerate(plans) for r in range(pl.n_islands)),
+                key=lambda t: int(hosts[t[0]]["mmask"][t[1]].sum()))
+    plan, host = plans[pi], hosts[pi]
+    isl = oracle_island(plan, host, r, d, w, sc.psfs)
+    o = oisl.LMOptions(max_steps=8)
+    p0 = host["p0"][r].ravel().astype(np.float64)
+    st = oisl.run_island(p0, isl, o, trace=True)
+    cv = oisl.clip_vec(isl.N, o)
+    prev_p, prev_chi2 = p0, oisl.chi2_of(p0, isl, o)
+    keep = isl.keep.ravel()
+    for s in st["history"]:
+        assert s["chi2"] <= prev_chi2                     # never increases
+        d_try = s["p_try"] - prev_p
+        assert np.all(np.abs(d_try) <= cv + 1e-12)        # per-parameter clips
+        assert np.all(s["p_try"] >= isl.lower - 1e-12) and np.all(s["p_try"] <= isl.upper + 1e-12)
+        assert np.all(s["p"][~keep] == p0[~keep])         # frozen / padded / star shape
+        if s["accept"]:
+            assert s["chi2"] == s["chi2_try"] < prev_chi2
+            np.testing.assert_array_equal(s["p"], s["p_try"])
+        else:
+            np.testing.assert_array_equal(s["p"], prev_p)
+        prev_p, prev_chi2 = s["p"], s["chi2"]
+    f, e = oisl.final_fluxes(st["p"], isl, o)
+    assert np.all(f[:, ~isl.mmask] == 0.0)
+    assert np.all(np.isfinite(e)) and np.all(e > 0)
+
+
+def test_final_errors_invert_the_finite_difference_fisher():
+    sc = small_star_scene(seed=4, n_bands=1, n_src=4, order=1)
+    plans, hosts, pad, d, w = star_classes(sc)
+    pi, r = 0, 0
+    isl = oracle_island(plans[pi], hosts[pi], r, d, w, sc.psfs)
+    o = oisl.LMOptions()
+    p = _away_from_half(hosts[pi]["p0"][r].ravel().astype(np.float64), isl)
+    ev = oisl.final_errors(p, isl, o)
+    live = isl.live()
+    r0 = _varpro_residual(p, isl, o)
+    J = np.zeros((len(live), r0.size))
+    for i, (k, j) in enumerate(live):
+        pp, pm = p.copy(), p.copy()
+        pp[6 * k + j] += 1e-5
+        pm[6 * k + j] -= 1e-5
+        J[i] = (_varpro_residual(pp, isl, o) - _varpro_residual(pm, isl, o)) / 2e-5
+    C = np.linalg.inv(J @ J.T)
+    for i, (k, j) in enumerate(live):
+        assert ev[k, j] == pytest.approx(C[i, i], rel=1e-3)
