@@ -221,6 +221,9 @@ def main():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--island-split", type=int, default=1,
+                    help="1 = band-split island LM (each island's bands spread over all CUs), "
+                         "0 = one persistent workgroup per island")
     ap.add_argument("--copy-calib-mib", type=int, default=0,
                     help="run one plain device copy of this many MiB before the steps, as the "
                          "known-size reference for FETCH_SIZE / WRITE_SIZE counter runs")
@@ -238,7 +241,7 @@ def main():
           f"{sc.data.shape[1]}x{sc.data.shape[2]} in {time.perf_counter() - t_scene:.1f}s",
           file=sys.stderr, flush=True)
     fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
-    cfg = FitConfig(**gcfg)
+    cfg = FitConfig(**gcfg, island_split=bool(args.island_split))
     sync = torch.cuda.synchronize
     if args.copy_calib_mib > 0:
         src = torch.ones(args.copy_calib_mib << 20, dtype=torch.uint8, device="cuda")

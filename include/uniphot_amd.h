@@ -158,6 +158,41 @@ int uph_island_fit_batch(const uph_island_batch* batch, const uph_island_inputs*
                          int32_t* counter, void* stream);
 
 /*
+ * Band-split form of uph_island_fit_batch, for islands whose cost would make
+ * one workgroup the critical path of a batch: every evaluation pass (chi2
+ * at p0, H and g at p, chi2 at the trial, final fluxes) runs as
+ * (island, band) items spread over all compute units, and a per-island
+ * launch sums the band blocks in band order and takes the legacy LM step.
+ * Same inputs, outputs and results (bit for bit) as uph_island_fit_batch;
+ * the LM loop is 4 * max_steps + 3 launches on ``stream`` with no host
+ * synchronisation.  Replaces run_class_chunk's per-step program
+ * (island_solver.py:2890-2977) with band-parallel evaluation.
+ *   item_isl, item_band (n_items = n_isl * n_bands): every (island, band)
+ *     pair once, in the order the work queue should take them;
+ *   hg_off (n_isl): first double of island i's n_bands blocks of
+ *     (n_live_max^2 + n_live_max + 2) doubles each, inside hg (hg_doubles);
+ *   p_try: laid out like outputs->p;
+ *   counters: uph_island_split_counters(max_steps) int32 (reset here; the
+ *     last one is set to 1 if an island's blocks did not fit in hg).
+ */
+typedef struct uph_island_split {
+  int n_items;
+  const int32_t* item_isl;
+  const int32_t* item_band;
+  const int64_t* hg_off;
+  double* hg;
+  size_t hg_doubles;
+  float* p_try;
+  int32_t* counters;
+} uph_island_split;
+
+size_t uph_island_split_counters(int max_steps);
+int uph_island_fit_split(const uph_island_batch* batch, const uph_island_inputs* in,
+                         const uph_island_outputs* out, const uph_lm_options* opt, int precision,
+                         const uph_island_split* split, void* scratch, size_t scratch_bytes,
+                         size_t scratch_per_block, void* stream);
+
+/*
  * Batched unit-flux stamp renderer: PSF polynomial at (x, y), galaxy
  * mixture convolution for extended sources, unit-sum renormalisation when
  * ``renorm`` (all-star grids only), Lanczos-3 shift by the remainder of

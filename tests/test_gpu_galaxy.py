@@ -79,3 +79,21 @@ def test_mixed_island_fit_matches_oracle(cuda, precision):
             rel = np.abs(rr["fluxes"][r][:, mm] - f[:, mm]) / np.maximum(np.abs(f[:, mm]),
                                                                           e[:, mm])
             assert rel.max() < 1e-5
+
+
+def test_mixed_band_split_equals_persistent(cuda):
+    import torch
+    from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
+    from uniphot_amd.sersic_table import device_table
+    sc = small_mixed_scene()
+    plans, hosts, pad, d, w = mixed_classes(sc, free_n=(sc.stype == 3))
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms), h, np.arange(p.n_islands))
+             for p, h in zip(plans, hosts)]
+    trees = {T: psf_tree_device(sc.psfs, T, cuda) for T in {p.tier for p in plans}}
+    args = (batch, trees, 30, torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda))
+    ref = run_island_batch(*args, sersic_dev=device_table(cuda))
+    print("split mixed batch", [(p.kind, p.n_islands) for p in plans], flush=True)
+    got = run_island_batch(*args, sersic_dev=device_table(cuda), split=True)
+    for ra, rb in zip(ref, got):
+        for k in ("p", "fluxes", "ferr", "chi2", "lam", "nst", "conv", "stall", "n_good"):
+            assert np.array_equal(ra[k], rb[k]), k

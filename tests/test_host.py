@@ -53,6 +53,7 @@ _STRUCTS = {
     "uph_island_inputs": _lib.IslandInputs,
     "uph_island_outputs": _lib.IslandOutputs,
     "uph_island_batch": _lib.IslandBatch,
+    "uph_island_split": _lib.IslandSplit,
 }
 
 

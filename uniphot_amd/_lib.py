@@ -17,7 +17,7 @@ UPH_PRECISION_FP32, UPH_PRECISION_FP64 = 0, 1
 # Every exported entry point declared in include/uniphot_amd.h.
 EXPORTS = (
     "uph_version", "uph_last_error", "uph_island_scratch_per_block", "uph_island_fit",
-    "uph_island_fit_batch",
+    "uph_island_fit_batch", "uph_island_split_counters", "uph_island_fit_split",
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
 )
 
@@ -69,6 +69,11 @@ class IslandOutputs(C.Structure):
         "n_good")]
 
 
+class IslandSplit(C.Structure):
+    _fields_ = [("n_items", C.c_int), ("item_isl", _P), ("item_band", _P), ("hg_off", _P),
+                ("hg", _P), ("hg_doubles", C.c_size_t), ("p_try", _P), ("counters", _P)]
+
+
 _lib = None
 
 
@@ -94,6 +99,12 @@ def lib() -> C.CDLL:
     L.uph_island_fit_batch.argtypes = [C.POINTER(IslandBatch), C.POINTER(IslandInputs),
                                        C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,
                                        _P, C.c_size_t, C.c_size_t, _P, _P]
+    L.uph_island_split_counters.restype = C.c_size_t
+    L.uph_island_split_counters.argtypes = [C.c_int]
+    L.uph_island_fit_split.restype = C.c_int
+    L.uph_island_fit_split.argtypes = [C.POINTER(IslandBatch), C.POINTER(IslandInputs),
+                                       C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,
+                                       C.POINTER(IslandSplit), _P, C.c_size_t, C.c_size_t, _P]
     L.uph_render_scratch_bytes.restype = C.c_size_t
     L.uph_render_scratch_bytes.argtypes = [C.c_int, C.c_int]
     L.uph_render_stamps.restype = C.c_int

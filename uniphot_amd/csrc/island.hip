@@ -694,38 +694,140 @@ __device__ float chi2_all(const IslandArgs& a, IslandSh<Real>& sh, int isl, cons
   return float(c);
 }
 
+// Class shape and offsets of island `isl` into sh (called by one thread).
+template <typename Real>
+__device__ void read_shape(const IslandArgs& a, IslandSh<Real>& sh, int isl) {
+  if (a.shape) {
+    const int32_t* sp = a.shape + 4 * (size_t)isl;
+    sh.N = sp[0];
+    sh.B = sp[1];
+    sh.T = a.tier_T[sp[2]];
+    sh.coefs = a.tier_coefs[sp[2]];
+    sh.nLc = sp[3];
+    sh.moff = a.mem_off[isl];
+    sh.foff = a.flux_off[isl];
+  } else {
+    sh.N = a.N;
+    sh.B = a.B;
+    sh.T = a.T;
+    sh.coefs = a.psf_coefs;
+    sh.nLc = a.nL_max;
+    sh.moff = (long long)isl * a.N;
+    sh.foff = (long long)isl * a.nb * a.N;
+  }
+}
+
+// Parameters (from psrc, laid out like p0), box bounds and the real-member
+// and live-parameter lists of the island whose shape is in sh.
+// Block-cooperative; ends with a barrier.
+template <typename Real>
+__device__ void describe_island(const IslandArgs& a, IslandSh<Real>& sh, const float* psrc) {
+  const int tid = threadIdx.x, N = sh.N, nP = 6 * N;
+  const size_t m0 = (size_t)sh.moff;
+  for (int q = tid; q < nP; q += UPH_BLOCK) {
+    const size_t g = m0 * 6 + q;
+    sh.p[q] = psrc[g];
+    sh.lo[q] = a.lower[g];
+    sh.hi[q] = a.upper[g];
+  }
+  if (tid == 0) {
+    int nr = 0, nl = 0;
+    for (int k = 0; k < N; ++k) {
+      const size_t g = m0 + k;
+      if (!a.mmask[g]) continue;
+      sh.mem[nr] = k;
+      sh.st[nr] = a.stype[g];
+      sh.nfz[nr] = a.n_frozen[g];
+      sh.fn[nr] = a.free_n[g];
+      sh.first_live[nr] = nl;
+      for (int j = 0; j < 6; ++j) {
+        if (a.keep[g * 6 + j]) {
+          sh.live_r[nl] = nr;
+          sh.live_j[nl] = j;
+          ++nl;
+        }
+      }
+      ++nr;
+    }
+    sh.first_live[nr] = nl;
+    sh.Nr = nr;
+    sh.nL = nl;
+  }
+  __syncthreads();
+}
+
+// Damped diagonal entry of the LM system: H_ii + lam H_ii + 1e-8.
+__device__ __forceinline__ double damp_diag(double h, double lam) { return fma(lam, h, h) + 1e-8; }
+
+// LM proposal: LU of the damped system Hd (in place), delta = Hd^-1 rhs
+// (rhs overwritten), p_try = clip(p - delta) within the box bounds.
+template <typename Real>
+__device__ void lm_propose(const IslandArgs& a, IslandSh<Real>& sh, double* Hd, double* rhs,
+                           int* piv) {
+  const int tid = threadIdx.x, nL = sh.nL, nP = 6 * sh.N;
+  block_lu(Hd, nL, piv, &sh.pivot);
+  if (tid == 0) lu_solve_one(Hd, piv, nL, rhs);
+  __syncthreads();
+  for (int q = tid; q < nP; q += UPH_BLOCK) sh.ptry[q] = sh.p[q];
+  __syncthreads();
+  if (tid == 0) {
+    const float clipv[6] = {a.pos_step_max, a.pos_step_max, a.shape_step_max,
+                            a.shape_step_max, a.shape_step_max, a.un_step_max};
+    for (int l = 0; l < nL; ++l) {
+      const int q = 6 * sh.mem[sh.live_r[l]] + sh.live_j[l];
+      float d = float(-rhs[l]);
+      const float cl = clipv[sh.live_j[l]];
+      d = fminf(fmaxf(d, -cl), cl);
+      sh.ptry[q] = sh.p[q] + d;
+    }
+    for (int q = 0; q < nP; ++q) sh.ptry[q] = fminf(fmaxf(sh.ptry[q], sh.lo[q]), sh.hi[q]);
+  }
+  __syncthreads();
+}
+
+// Accept / reject the trial with chi2 `ct` and update the convergence
+// prongs (one thread).
+template <typename Real>
+__device__ void lm_accept(const IslandArgs& a, IslandSh<Real>& sh, float ct) {
+  const float cur = sh.chi2;
+  const bool accept = ct < cur;
+  float pos_step = 0.0f, shape_step = 0.0f;
+  if (accept) {
+    for (int r = 0; r < sh.Nr; ++r) {
+      const int k = sh.mem[r];
+      const float sx = sh.ptry[6 * k] - sh.p[6 * k];
+      const float sy = sh.ptry[6 * k + 1] - sh.p[6 * k + 1];
+      pos_step = fmaxf(pos_step, hypotf(sx, sy));
+      for (int j = 2; j < 6; ++j)
+        shape_step = fmaxf(shape_step, fabsf(sh.ptry[6 * k + j] - sh.p[6 * k + j]));
+    }
+    for (int q = 0; q < 6 * sh.N; ++q) sh.p[q] = sh.ptry[q];
+    sh.lam = fmaxf(sh.lam * 0.1f, 1e-9f);
+    sh.chi2 = ct;
+  } else {
+    sh.lam = fminf(sh.lam * 10.0f, 1e6f);
+  }
+  const float dchi2 = cur - ct;
+  const bool small = accept && (dchi2 < a.ftol_rel * fmaxf(cur, 1.0f));
+  sh.consec = small ? sh.consec + 1 : (accept ? 0 : sh.consec);
+  const bool step_small = accept && pos_step < a.step_tol && shape_step < a.shape_step_tol;
+  sh.nrej = accept ? 0 : sh.nrej + 1;
+  const bool stall_now = sh.nrej >= 4;
+  sh.stall = stall_now;
+  sh.conv = (sh.consec >= 2) || step_small || stall_now;
+  sh.nst += 1;
+}
+
 template <typename Real>
 __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
   __shared__ IslandSh<Real> sh;
   const int tid = threadIdx.x;
   char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
-  const float clipv[6] = {a.pos_step_max, a.pos_step_max, a.shape_step_max,
-                          a.shape_step_max, a.shape_step_max, a.un_step_max};
   for (;;) {
     if (tid == 0) {
       const int item = atomicAdd(a.counter, 1);
       sh.item = item;
-      if (item < a.n_isl) {
-        const int isl = a.order ? a.order[item] : item;
-        if (a.shape) {
-          const int32_t* sp = a.shape + 4 * (size_t)isl;
-          sh.N = sp[0];
-          sh.B = sp[1];
-          sh.T = a.tier_T[sp[2]];
-          sh.coefs = a.tier_coefs[sp[2]];
-          sh.nLc = sp[3];
-          sh.moff = a.mem_off[isl];
-          sh.foff = a.flux_off[isl];
-        } else {
-          sh.N = a.N;
-          sh.B = a.B;
-          sh.T = a.T;
-          sh.coefs = a.psf_coefs;
-          sh.nLc = a.nL_max;
-          sh.moff = (long long)isl * a.N;
-          sh.foff = (long long)isl * a.nb * a.N;
-        }
-      }
+      if (item < a.n_isl) read_shape(a, sh, a.order ? a.order[item] : item);
     }
     __syncthreads();
     const int item = sh.item;
@@ -741,40 +843,13 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     Real* gws = reinterpret_cast<Real*>(base + lay.total);  // galaxy workspace (mixed)
     const size_t m0 = (size_t)sh.moff;
     // ---- island description into LDS ----
-    for (int q = tid; q < nP; q += UPH_BLOCK) {
-      const size_t g = m0 * 6 + q;
-      sh.p[q] = a.p0[g];
-      sh.lo[q] = a.lower[g];
-      sh.hi[q] = a.upper[g];
-    }
+    describe_island(a, sh, a.p0);
     if (tid == 0) {
-      int nr = 0, nl = 0;
-      for (int k = 0; k < N; ++k) {
-        const size_t g = m0 + k;
-        if (!a.mmask[g]) continue;
-        sh.mem[nr] = k;
-        sh.st[nr] = a.stype[g];
-        sh.nfz[nr] = a.n_frozen[g];
-        sh.fn[nr] = a.free_n[g];
-        sh.first_live[nr] = nl;
-        for (int j = 0; j < 6; ++j) {
-          if (a.keep[g * 6 + j]) {
-            sh.live_r[nl] = nr;
-            sh.live_j[nl] = j;
-            ++nl;
-          }
-        }
-        ++nr;
-      }
-      sh.first_live[nr] = nl;
-      sh.Nr = nr;
-      sh.nL = nl;
       sh.lam = 1e-3f;
       sh.consec = sh.nrej = sh.nst = 0;
       sh.conv = sh.stall = 0;
+      if (sh.Nr == 0) sh.conv = 1;  // nothing to fit (all slots padded)
     }
-    __syncthreads();
-    if (sh.Nr == 0) sh.conv = 1;  // nothing to fit (all slots padded)
     __syncthreads();
     if (sh.Nr > 0) {
       const float c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, gws, G, part, sm);
@@ -783,6 +858,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     } else if (tid == 0) {
       sh.chi2 = 0.0f;
     }
+    __syncthreads();
     for (int it = 0; it < a.max_steps; ++it) {
       if (sh.conv) break;
       const int nL = sh.nL;
@@ -794,60 +870,14 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
       const double lam = double(sh.lam);
       for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
         const int i = e / nL, j = e % nL;
-        double v = sm.H[e];
-        if (i == j) v = v + lam * sm.H[e] + 1e-8;
-        sm.Hd[e] = v;
+        const double v = sm.H[e];
+        sm.Hd[e] = i == j ? damp_diag(v, lam) : v;
       }
+      for (int j = tid; j < nL; j += UPH_BLOCK) sm.delta[j] = sm.g[j];
       __syncthreads();
-      block_lu(sm.Hd, nL, sm.pivH, &sh.pivot);
-      if (tid == 0) {
-        for (int j = 0; j < nL; ++j) sm.delta[j] = sm.g[j];
-        lu_solve_one(sm.Hd, sm.pivH, nL, sm.delta);
-      }
-      __syncthreads();
-      for (int q = tid; q < nP; q += UPH_BLOCK) sh.ptry[q] = sh.p[q];
-      __syncthreads();
-      if (tid == 0) {
-        for (int l = 0; l < nL; ++l) {
-          const int q = 6 * sh.mem[sh.live_r[l]] + sh.live_j[l];
-          float d = float(-sm.delta[l]);
-          const float cl = clipv[sh.live_j[l]];
-          d = fminf(fmaxf(d, -cl), cl);
-          sh.ptry[q] = sh.p[q] + d;
-        }
-        for (int q = 0; q < nP; ++q) sh.ptry[q] = fminf(fmaxf(sh.ptry[q], sh.lo[q]), sh.hi[q]);
-      }
-      __syncthreads();
+      lm_propose(a, sh, sm.Hd, sm.delta, sm.pivH);
       const float ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, gws, G, part, sm);
-      if (tid == 0) {
-        const float cur = sh.chi2;
-        const bool accept = ct < cur;
-        float pos_step = 0.0f, shape_step = 0.0f;
-        if (accept) {
-          for (int r = 0; r < sh.Nr; ++r) {
-            const int k = sh.mem[r];
-            const float sx = sh.ptry[6 * k] - sh.p[6 * k];
-            const float sy = sh.ptry[6 * k + 1] - sh.p[6 * k + 1];
-            pos_step = fmaxf(pos_step, hypotf(sx, sy));
-            for (int j = 2; j < 6; ++j)
-              shape_step = fmaxf(shape_step, fabsf(sh.ptry[6 * k + j] - sh.p[6 * k + j]));
-          }
-          for (int q = 0; q < nP; ++q) sh.p[q] = sh.ptry[q];
-          sh.lam = fmaxf(sh.lam * 0.1f, 1e-9f);
-          sh.chi2 = ct;
-        } else {
-          sh.lam = fminf(sh.lam * 10.0f, 1e6f);
-        }
-        const float dchi2 = cur - ct;
-        const bool small = accept && (dchi2 < a.ftol_rel * fmaxf(cur, 1.0f));
-        sh.consec = small ? sh.consec + 1 : (accept ? 0 : sh.consec);
-        const bool step_small = accept && pos_step < a.step_tol && shape_step < a.shape_step_tol;
-        sh.nrej = accept ? 0 : sh.nrej + 1;
-        const bool stall_now = sh.nrej >= 4;
-        sh.stall = stall_now;
-        sh.conv = (sh.consec >= 2) || step_small || stall_now;
-        sh.nst += 1;
-      }
+      if (tid == 0) lm_accept(a, sh, ct);
       __syncthreads();
     }
     // ---- final fluxes + outputs ----
@@ -879,6 +909,178 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Band-split LM.  The same step as island_lm_kernel, with each evaluation
+// pass spread over (island, band) items so that one island's bands run on
+// many compute units at once; a per-island step launch reduces the band
+// blocks in band order (the order of the persistent kernel's accumulation,
+// so both paths produce the same bits) and runs the damped solve or the
+// accept / reject.  Used for islands whose single-workgroup latency would
+// dominate a batch.
+// ---------------------------------------------------------------------------
+template <typename Real>
+__global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(IslandArgs a) {
+  __shared__ IslandSh<Real> sh;
+  const int tid = threadIdx.x;
+  char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
+  const int mode = a.mode;
+  for (;;) {
+    if (tid == 0) {
+      const int item = atomicAdd(a.counter, 1);
+      sh.item = item;
+      if (item < a.n_items) {
+        const int isl = a.item_isl[item];
+        read_shape(a, sh, isl);
+        sh.conv = (mode == kSplitEvalGrad || mode == kSplitEvalTry) ? int(a.conv_out[isl]) : 0;
+      }
+    }
+    __syncthreads();
+    const int item = sh.item;
+    if (item >= a.n_items) break;
+    if (sh.conv) {
+      __syncthreads();
+      continue;
+    }
+    const int isl = a.item_isl[item], b = a.item_band[item];
+    const float* psrc = mode == kSplitEvalInit ? a.p0 : (mode == kSplitEvalTry ? a.p_try : a.p_out);
+    describe_island(a, sh, psrc);
+    const int N = sh.N, nL = sh.nL;
+    const size_t blk = split_band_block(sh.nLc);
+    const size_t off = (size_t)a.hg_off[isl] + (size_t)b * blk;
+    if (off + blk > a.hg_doubles || nL > sh.nLc) {
+      if (tid == 0) atomicOr(a.err, 1);
+      __syncthreads();
+      continue;
+    }
+    double* slot = a.hg + off;
+    if (sh.Nr == 0) {
+      if (tid == 0) slot[(size_t)nL * nL + nL] = 0.0;
+      __syncthreads();
+      continue;
+    }
+    const IslandScratch lay = island_scratch_layout(N, sh.B, sh.T, sh.nLc, sizeof(Real));
+    Real* planes = reinterpret_cast<Real*>(base + lay.planes);
+    Real* ws = reinterpret_cast<Real*>(base + lay.render);
+    double* G = reinterpret_cast<double*>(base + lay.gram);
+    double* part = reinterpret_cast<double*>(base + lay.part);
+    const Small sm = carve_small(base + lay.small, N, sh.nLc);
+    Real* gws = reinterpret_cast<Real*>(base + lay.total);
+    if (mode == kSplitEvalFinal) {
+      float* fo = a.flux_out + (size_t)sh.foff + (size_t)b * N;
+      float* eo = a.ferr_out + (size_t)sh.foff + (size_t)b * N;
+      int ngood = 0;
+      eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, gws, G, part, sm, fo, eo,
+                      &ngood);
+      if (tid == 0) atomicAdd(a.ngood_out + isl, ngood);
+    } else {
+      const bool grads = mode == kSplitEvalGrad;
+      const Real c = eval_band<Real>(a, sh, isl, b, sh.p, grads, false, planes, ws, gws, G, part,
+                                     sm, nullptr, nullptr, nullptr);
+      if (grads) {
+        for (int e = tid; e < nL * nL; e += UPH_BLOCK) slot[e] = sm.H[e];
+        for (int j = tid; j < nL; j += UPH_BLOCK) slot[(size_t)nL * nL + j] = sm.g[j];
+      }
+      if (tid == 0) slot[(size_t)nL * nL + nL] = double(c);
+    }
+    __syncthreads();
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
+  __shared__ IslandSh<Real> sh;
+  __shared__ int piv[kMaxLive];
+  __shared__ double rhs[kMaxLive];
+  const int tid = threadIdx.x, isl = blockIdx.x, mode = a.mode;
+  if (mode != kSplitStepInit && a.conv_out[isl]) return;  // uniform over the block
+  if (tid == 0) read_shape(a, sh, isl);
+  __syncthreads();
+  describe_island(a, sh, mode == kSplitStepInit ? a.p0 : a.p_out);
+  const int N = sh.N, nP = 6 * N, nL = sh.nL, nb = a.nb;
+  const size_t m0 = (size_t)sh.moff;
+  const size_t blk = split_band_block(sh.nLc);
+  const size_t off0 = (size_t)a.hg_off[isl];
+  if (off0 + (size_t)nb * blk > a.hg_doubles || nL > sh.nLc) {
+    if (tid == 0) {
+      atomicOr(a.err, 1);
+      a.conv_out[isl] = 1;
+    }
+    return;
+  }
+  double* h0 = a.hg + off0;
+  const size_t c_at = (size_t)nL * nL + nL;  // chi2 entry of a band block
+  if (mode == kSplitStepInit) {
+    if (tid == 0) {
+      Real c = 0;
+      if (sh.Nr > 0)
+        for (int b = 0; b < nb; ++b) c += Real(h0[(size_t)b * blk + c_at]);
+      a.chi2_out[isl] = sh.Nr > 0 ? float(c) : 0.0f;
+      a.lam_out[isl] = 1e-3f;
+      a.consec_out[isl] = 0;
+      a.nrej_out[isl] = 0;
+      a.nst_out[isl] = 0;
+      a.conv_out[isl] = uint8_t(sh.Nr == 0);
+      a.stall_out[isl] = 0;
+      a.ngood_out[isl] = 0;
+    }
+    for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[m0 * 6 + q] = sh.p[q];
+    float* fo = a.flux_out + (size_t)sh.foff;
+    float* eo = a.ferr_out + (size_t)sh.foff;
+    for (int e = tid; e < nb * N; e += UPH_BLOCK) {
+      fo[e] = 0.0f;
+      eo[e] = 1.0f;
+    }
+    return;
+  }
+  if (tid == 0) {
+    sh.chi2 = a.chi2_out[isl];
+    sh.lam = a.lam_out[isl];
+    sh.consec = a.consec_out[isl];
+    sh.nrej = a.nrej_out[isl];
+    sh.nst = a.nst_out[isl];
+    sh.conv = a.conv_out[isl];
+    sh.stall = a.stall_out[isl];
+  }
+  __syncthreads();
+  if (mode == kSplitStepSolve) {
+    // H = sum_b H_b and g = sum_b g_b in band order; damped in place in block 0
+    const double lam = double(sh.lam);
+    for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
+      double v = h0[e];
+      for (int b = 1; b < nb; ++b) v = v + h0[(size_t)b * blk + e];
+      const int i = e / nL, j = e % nL;
+      h0[e] = i == j ? damp_diag(v, lam) : v;
+    }
+    for (int j = tid; j < nL; j += UPH_BLOCK) {
+      const size_t o = (size_t)nL * nL + j;
+      double v = h0[o];
+      for (int b = 1; b < nb; ++b) v = v + h0[(size_t)b * blk + o];
+      rhs[j] = v;
+    }
+    __syncthreads();
+    lm_propose(a, sh, h0, rhs, piv);
+    for (int q = tid; q < nP; q += UPH_BLOCK) a.p_try[m0 * 6 + q] = sh.ptry[q];
+    return;
+  }
+  // kSplitStepAccept
+  for (int q = tid; q < nP; q += UPH_BLOCK) sh.ptry[q] = a.p_try[m0 * 6 + q];
+  __syncthreads();
+  if (tid == 0) {
+    Real c = 0;
+    for (int b = 0; b < nb; ++b) c += Real(h0[(size_t)b * blk + c_at]);
+    lm_accept(a, sh, float(c));
+    a.chi2_out[isl] = sh.chi2;
+    a.lam_out[isl] = sh.lam;
+    a.consec_out[isl] = sh.consec;
+    a.nrej_out[isl] = sh.nrej;
+    a.nst_out[isl] = sh.nst;
+    a.conv_out[isl] = uint8_t(sh.conv);
+    a.stall_out[isl] = uint8_t(sh.stall);
+  }
+  __syncthreads();
+  for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[m0 * 6 + q] = sh.p[q];
+}
+
 }  // namespace uph
 
 extern "C" size_t uph_island_scratch_per_block(int N, int B, int T, int n_live_max,
@@ -889,10 +1091,11 @@ extern "C" size_t uph_island_scratch_per_block(int N, int B, int T, int n_live_m
   return base + (n_live_max > 2 * N ? uph::galaxy_scratch_bytes(T, rb) : 0);
 }
 
-static int island_launch(uph::IslandArgs& a, const uph_island_inputs* in,
-                         const uph_island_outputs* out, const uph_lm_options* opt, int precision,
-                         void* scratch, size_t scratch_bytes, size_t per, int32_t* counter,
-                         void* stream) {
+// Validate and copy the inputs, outputs and options shared by every island
+// entry point into `a`.
+static int fill_island_args(uph::IslandArgs& a, const uph_island_inputs* in,
+                            const uph_island_outputs* out, const uph_lm_options* opt,
+                            int precision, void* scratch, size_t per) {
   UPH_CHECK_ARG(in->dstack && in->wstack && in->psf_geo && in->ox && in->oy && in->x0b &&
                     in->y0b && in->stype && in->n_frozen && in->free_n && in->mmask &&
                     in->keep && in->p0 && in->lower && in->upper,
@@ -903,12 +1106,7 @@ static int island_launch(uph::IslandArgs& a, const uph_island_inputs* in,
   UPH_CHECK_ARG(opt->max_steps >= 0, "uph_island_fit: max_steps=%d", opt->max_steps);
   UPH_CHECK_ARG(precision == UPH_PRECISION_FP32 || precision == UPH_PRECISION_FP64,
                 "uph_island_fit: precision=%d", precision);
-  UPH_CHECK_ARG(counter != nullptr && scratch != nullptr, "uph_island_fit: null scratch/counter");
-  if (a.n_isl <= 0) return UPH_OK;
-  const size_t nblk_s = scratch_bytes / per;
-  UPH_CHECK_ARG(nblk_s >= 1, "uph_island_fit: scratch of %zu bytes < one workgroup (%zu)",
-                scratch_bytes, per);
-  const int nblk = (int)(nblk_s < (size_t)a.n_isl ? nblk_s : (size_t)a.n_isl);
+  UPH_CHECK_ARG(scratch != nullptr, "uph_island_fit: null scratch");
   a.max_steps = opt->max_steps;
   a.reg = opt->reg;
   a.pos_step_max = opt->pos_step_max;
@@ -947,6 +1145,21 @@ static int island_launch(uph::IslandArgs& a, const uph_island_inputs* in,
   a.ngood_out = out->n_good;
   a.scratch = static_cast<char*>(scratch);
   a.scratch_per_block = per;
+  return UPH_OK;
+}
+
+static int island_launch(uph::IslandArgs& a, const uph_island_inputs* in,
+                         const uph_island_outputs* out, const uph_lm_options* opt, int precision,
+                         void* scratch, size_t scratch_bytes, size_t per, int32_t* counter,
+                         void* stream) {
+  int st0 = fill_island_args(a, in, out, opt, precision, scratch, per);
+  if (st0 != UPH_OK) return st0;
+  UPH_CHECK_ARG(counter != nullptr, "uph_island_fit: null counter");
+  if (a.n_isl <= 0) return UPH_OK;
+  const size_t nblk_s = scratch_bytes / per;
+  UPH_CHECK_ARG(nblk_s >= 1, "uph_island_fit: scratch of %zu bytes < one workgroup (%zu)",
+                scratch_bytes, per);
+  const int nblk = (int)(nblk_s < (size_t)a.n_isl ? nblk_s : (size_t)a.n_isl);
   a.counter = counter;
   hipStream_t s = static_cast<hipStream_t>(stream);
   int st = uph::hip_status(hipMemsetAsync(counter, 0, sizeof(int32_t), s), "uph_island_fit memset");
@@ -998,11 +1211,9 @@ extern "C" int uph_island_fit(const uph_island_class* cls, const uph_island_inpu
   return island_launch(a, in, out, opt, precision, scratch, scratch_bytes, per, counter, stream);
 }
 
-extern "C" int uph_island_fit_batch(const uph_island_batch* bat, const uph_island_inputs* in,
-                                    const uph_island_outputs* out, const uph_lm_options* opt,
-                                    int precision, void* scratch, size_t scratch_bytes,
-                                    size_t scratch_per_block, int32_t* counter, void* stream) {
-  UPH_CHECK_ARG(bat && in && out && opt, "uph_island_fit_batch: null argument");
+// Heterogeneous batch description -> `a` (shape arrays, tiers, sizes).
+static int fill_batch_args(uph::IslandArgs& a, const uph_island_batch* bat,
+                           const uph_island_inputs* in, size_t scratch_per_block) {
   UPH_CHECK_ARG(bat->shape && bat->mem_off && bat->flux_off,
                 "uph_island_fit_batch: null shape/offset arrays");
   UPH_CHECK_ARG(bat->n_tiers >= 1 && bat->n_tiers <= 3, "uph_island_fit_batch: n_tiers=%d",
@@ -1015,7 +1226,6 @@ extern "C" int uph_island_fit_batch(const uph_island_batch* bat, const uph_islan
                 bat->max_B);
   UPH_CHECK_ARG(!bat->mixed || in->sersic != nullptr,
                 "uph_island_fit_batch: mixed islands need the Sersic table");
-  uph::IslandArgs a{};
   for (int t = 0; t < bat->n_tiers; ++t) {
     UPH_CHECK_ARG(bat->tier_T[t] >= 7 && bat->tier_T[t] % 2 == 1 && bat->psf_coefs[t],
                   "uph_island_fit_batch: tier %d: T=%d / coefs", t, bat->tier_T[t]);
@@ -1032,6 +1242,94 @@ extern "C" int uph_island_fit_batch(const uph_island_batch* bat, const uph_islan
   a.shape = bat->shape;
   a.mem_off = bat->mem_off;
   a.flux_off = bat->flux_off;
+  return UPH_OK;
+}
+
+extern "C" int uph_island_fit_batch(const uph_island_batch* bat, const uph_island_inputs* in,
+                                    const uph_island_outputs* out, const uph_lm_options* opt,
+                                    int precision, void* scratch, size_t scratch_bytes,
+                                    size_t scratch_per_block, int32_t* counter, void* stream) {
+  UPH_CHECK_ARG(bat && in && out && opt, "uph_island_fit_batch: null argument");
+  UPH_CHECK_ARG(bat->shape && bat->mem_off && bat->flux_off,
+                "uph_island_fit_batch: null shape/offset arrays");
+  uph::IslandArgs a{};
+  int st = fill_batch_args(a, bat, in, scratch_per_block);
+  if (st != UPH_OK) return st;
   return island_launch(a, in, out, opt, precision, scratch, scratch_bytes, scratch_per_block,
                        counter, stream);
+}
+
+template <typename Real>
+static void split_eval(uph::IslandArgs a, int mode, int* counter, int nblk, hipStream_t s) {
+  a.mode = mode;
+  a.counter = counter;
+  hipLaunchKernelGGL(uph::island_eval_kernel<Real>, dim3(nblk), dim3(UPH_BLOCK), 0, s, a);
+}
+
+template <typename Real>
+static void split_step(uph::IslandArgs a, int mode, hipStream_t s) {
+  a.mode = mode;
+  hipLaunchKernelGGL(uph::island_step_kernel<Real>, dim3(a.n_isl), dim3(UPH_BLOCK), 0, s, a);
+}
+
+template <typename Real>
+static void split_sequence(const uph::IslandArgs& a, int* counters, int nblk, hipStream_t s) {
+  split_eval<Real>(a, uph::kSplitEvalInit, counters + 0, nblk, s);
+  split_step<Real>(a, uph::kSplitStepInit, s);
+  for (int it = 0; it < a.max_steps; ++it) {
+    split_eval<Real>(a, uph::kSplitEvalGrad, counters + 2 + 2 * it, nblk, s);
+    split_step<Real>(a, uph::kSplitStepSolve, s);
+    split_eval<Real>(a, uph::kSplitEvalTry, counters + 3 + 2 * it, nblk, s);
+    split_step<Real>(a, uph::kSplitStepAccept, s);
+  }
+  split_eval<Real>(a, uph::kSplitEvalFinal, counters + 1, nblk, s);
+}
+
+extern "C" size_t uph_island_split_counters(int max_steps) {
+  return (size_t)(2 * (max_steps > 0 ? max_steps : 0) + 3);
+}
+
+extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_island_inputs* in,
+                                    const uph_island_outputs* out, const uph_lm_options* opt,
+                                    int precision, const uph_island_split* sp, void* scratch,
+                                    size_t scratch_bytes, size_t scratch_per_block,
+                                    void* stream) {
+  UPH_CHECK_ARG(bat && in && out && opt && sp, "uph_island_fit_split: null argument");
+  UPH_CHECK_ARG(bat->shape && bat->mem_off && bat->flux_off,
+                "uph_island_fit_split: null shape/offset arrays");
+  UPH_CHECK_ARG(sp->n_items >= 0 && (sp->n_items == 0 || (sp->item_isl && sp->item_band)),
+                "uph_island_fit_split: bad item arrays");
+  UPH_CHECK_ARG(sp->hg_off && sp->hg && sp->p_try && sp->counters,
+                "uph_island_fit_split: null work arrays");
+  uph::IslandArgs a{};
+  int st = fill_batch_args(a, bat, in, scratch_per_block);
+  if (st != UPH_OK) return st;
+  st = fill_island_args(a, in, out, opt, precision, scratch, scratch_per_block);
+  if (st != UPH_OK) return st;
+  if (a.n_isl <= 0) return UPH_OK;
+  UPH_CHECK_ARG(sp->n_items == a.n_isl * a.nb,
+                "uph_island_fit_split: n_items=%d must be n_isl * n_bands = %d", sp->n_items,
+                a.n_isl * a.nb);
+  const size_t nblk_s = scratch_bytes / scratch_per_block;
+  UPH_CHECK_ARG(nblk_s >= 1, "uph_island_fit_split: scratch of %zu bytes < one workgroup (%zu)",
+                scratch_bytes, scratch_per_block);
+  const int nblk = (int)(nblk_s < (size_t)sp->n_items ? nblk_s : (size_t)sp->n_items);
+  a.n_items = sp->n_items;
+  a.item_isl = sp->item_isl;
+  a.item_band = sp->item_band;
+  a.hg_off = sp->hg_off;
+  a.hg = sp->hg;
+  a.hg_doubles = sp->hg_doubles;
+  a.p_try = sp->p_try;
+  const size_t nc = uph_island_split_counters(a.max_steps);
+  a.err = sp->counters + nc - 1;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  st = uph::hip_status(hipMemsetAsync(sp->counters, 0, nc * sizeof(int32_t), s),
+                       "uph_island_fit_split memset");
+  if (st != UPH_OK) return st;
+  if (precision == UPH_PRECISION_FP32)
+    split_sequence<float>(a, sp->counters, nblk, s);
+  else
+    split_sequence<double>(a, sp->counters, nblk, s);
+  return uph::hip_status(hipGetLastError(), "uph_island_fit_split launch");
 }

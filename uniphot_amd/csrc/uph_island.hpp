@@ -51,7 +51,33 @@ struct IslandArgs {
   const int64_t* flux_off; // (n_isl) first element of its (nb, N) flux block
   const float* tier_coefs[3];
   int tier_T[3];
+  // band-split path: (island, band) evaluation items and per-band H, g, chi2
+  int mode;                // kSplit* phase of the launch
+  int n_items;
+  const int32_t* item_isl;
+  const int32_t* item_band;
+  const int64_t* hg_off;   // (n_isl) first double of island i's band blocks
+  double* hg;
+  size_t hg_doubles;
+  float* p_try;            // trial parameters, laid out like p_out
+  int* err;                // set to 1 when an island's band blocks overflow hg
 };
+
+// Band-split phases.  Evaluation launches run one (island, band) item per
+// workgroup visit; step launches run one workgroup per island.
+constexpr int kSplitEvalInit = 0;   // chi2 per band at p0
+constexpr int kSplitEvalGrad = 1;   // H, g, chi2 per band at p
+constexpr int kSplitEvalTry = 2;    // chi2 per band at p_try
+constexpr int kSplitEvalFinal = 3;  // fluxes, flux errors, good-pixel count at p
+constexpr int kSplitStepInit = 4;   // LM state from the initial chi2
+constexpr int kSplitStepSolve = 5;  // damped solve -> p_try
+constexpr int kSplitStepAccept = 6; // accept / reject, convergence
+
+// Doubles of one band block of an island with n_live_max = nLc:
+// H (nL x nL, row-major with the actual nL), g (nL), chi2, one spare.
+__host__ __device__ inline size_t split_band_block(int nLc) {
+  return (size_t)nLc * nLc + nLc + 2;
+}
 
 __host__ __device__ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 

@@ -173,13 +173,18 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
 
 
 def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
-                     precision: str = "fp32", stream=None, sersic_dev=None, timer=None):
+                     precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
+                     split: bool = False):
     """Fit the islands of several classes in ONE persistent launch.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
     maps stamp tier -> PsfTree.  Islands are ordered longest-first
     (members x window area) for the work queue.  Returns one result dict
     per class, in the same layout as :func:`run_class_chunk`.
+
+    ``split=True`` runs the band-split form (``uph_island_fit_split``):
+    every evaluation pass is spread over (island, band) items, so one large
+    island no longer sets the wall time; results are bit-identical.
     """
     import torch
     t0 = time.perf_counter()
@@ -260,7 +265,8 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         consec=torch.empty(n_isl, dtype=torch.int32, device=dev),
         nrej=torch.empty(n_isl, dtype=torch.int32, device=dev),
         n_good=torch.empty(n_isl, dtype=torch.int32, device=dev))
-    nblk = max(1, min(n_isl, MAX_BLOCKS, MAX_SCRATCH_BYTES // per))
+    n_work = n_isl * nb if split else n_isl
+    nblk = max(1, min(n_work, MAX_BLOCKS, MAX_SCRATCH_BYTES // per))
     scratch = torch.empty(per * nblk, dtype=torch.uint8, device=dev)
     counter = torch.zeros(1, dtype=torch.int32, device=dev)
     P = _lib.ptr
@@ -283,15 +289,36 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         "n_good")))
     opts = k0.options(max_steps)
     ev = timer.begin("island_lm_kernel") if timer is not None else None
-    _lib.check(L.uph_island_fit_batch(C.byref(bat), C.byref(inp), C.byref(out), C.byref(opts),
-                                      prec, P(scratch), scratch.numel(), per, P(counter),
-                                      _lib.stream_ptr(stream)))
+    if split:
+        nlc = np.asarray([s_[3] for s_ in shape], np.int64)
+        blk = nlc * nlc + nlc + 2
+        hg_off = np.concatenate([[0], np.cumsum(blk * nb)[:-1]]).astype(np.int64)
+        hg_n = int((blk * nb).sum())
+        items = np.repeat(order.astype(np.int64), nb)
+        bands = np.tile(np.arange(nb, dtype=np.int32), n_isl)
+        hg = torch.empty(hg_n, dtype=torch.float64, device=dev)
+        p_try = torch.empty(m_acc * 6, **f32)
+        n_cnt = int(L.uph_island_split_counters(int(max_steps)))
+        counters = torch.empty(n_cnt, dtype=torch.int32, device=dev)
+        d_items, d_bands = up(items, np.int32), up(bands, np.int32)
+        d_hgoff = up(hg_off, np.int64)
+        sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
+                              P(p_try), P(counters))
+        _lib.check(L.uph_island_fit_split(C.byref(bat), C.byref(inp), C.byref(out),
+                                          C.byref(opts), prec, C.byref(sp), P(scratch),
+                                          scratch.numel(), per, _lib.stream_ptr(stream)))
+    else:
+        _lib.check(L.uph_island_fit_batch(C.byref(bat), C.byref(inp), C.byref(out),
+                                          C.byref(opts), prec, P(scratch), scratch.numel(), per,
+                                          P(counter), _lib.stream_ptr(stream)))
     if ev is not None:
         area = np.concatenate([h["area"][np.asarray(r)] for _, h, r in classes])
         mmc = np.concatenate([h["mmask"][np.asarray(r)].sum(axis=1) for _, h, r in classes])
         timer.end(ev, dict(kind="island_batch", n=n_isl, nb=nb, outs=outs, area=area,
                            nreal=mmc))
     h = {k: v.cpu().numpy() for k, v in outs.items()}
+    if split and int(counters[-1].item()) != 0:
+        raise _lib.UphError(_lib.UPH_ERR_INTERNAL, "island band blocks overflowed their buffer")
     results = []
     for (kern, host, rows), (i0, n_c, m0, f0, N) in zip(classes, spans):
         r = dict(p=h["p"][m0 * 6:(m0 + n_c * N) * 6].reshape(n_c, N, 6),

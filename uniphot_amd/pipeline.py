@@ -48,6 +48,7 @@ class FitConfig:
     k_max: int = 32
     precision: str = "fp32"
     stamp_tiers: tuple = (55, 111, 221)
+    island_split: bool = True    # band-split island LM (uph_island_fit_split)
 
 
 @dataclass
@@ -261,7 +262,8 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
         from .sersic_table import device_table
         sersic_dev = device_table(dstack.device)
     results = run_island_batch(batch, trees, cfg.max_steps, dstack, wstack,
-                               precision=cfg.precision, timer=timer, sersic_dev=sersic_dev)
+                               precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
+                               split=cfg.island_split)
     for (plan, host), res in zip(classes, results):
         n_fitted_islands += plan.n_islands
         mm = plan.mmask
