@@ -48,7 +48,11 @@ class KernelTimer:
             return
         e1 = torch.cuda.Event(enable_timing=True)
         e1.record()
-        self.recs.append((ev[0], ev[1], e1, info))
+        self.recs.append((info.get("name") or ev[0], ev[1], e1, info))
+
+    def record(self, name, e0, e1, info):
+        """An event pair recorded inside the library (per-kernel launches)."""
+        self.recs.append((name, e0, e1, info))
 
     def per_launch(self):
         import torch
@@ -80,6 +84,8 @@ def algorithmic_cost(info):
             per evaluation pass: initial chi2, 2 per LM step, final fluxes)
             + PSF coefficients once; flops per band per step 4N R + 24 N^2 A
             with R = 2 T^2 n_t + 28 T^2 (star members).
+    island_eval: one band-split evaluation launch: nb * A_i * 8 for every
+            island box it evaluates (bytes computed by the island facade).
     band:   data + invvar + good per window pixel (9 B) + stamps read.
     render: PSF coefficients + stamp written per stamp."""
     k = info["kind"]
@@ -104,6 +110,8 @@ def algorithmic_cost(info):
         R = 2 * T * T * 15 + 28 * T * T
         fl = float(np.sum(nb * (2 * nst + 2) * (4 * nreal * R / 2 + 24 * nreal ** 2 * area)))
         return by, fl
+    if k == "island_eval":
+        return info["bytes"], 0.0
     if k == "band":
         return info["pix"] * 9.0 + info["n_src"] * info["T"] ** 2 * 4.0, 0.0
     if k == "render":
@@ -266,9 +274,12 @@ def main():
     value = n * args.steps / el
     ks = timer.summary()
     if os.environ.get("UPH_BENCH_VERBOSE"):
-        for name, ms, info in timer.per_launch()[:80]:
+        for name, ms, info in timer.per_launch()[:400]:
             print(f"[bench] {name} {ms:.3f} ms {info}", file=sys.stderr)
-    isl = ks.get("island_lm_kernel", dict(launches=0, ms=0.0, bytes=0.0, flops=0.0))
+    # dominant kernel: the most total time among single kernels (the split
+    # island fit's whole launch sequence is recorded for reference only)
+    kern = max((k for k in ks if k != "island_split_sequence"), key=lambda k: ks[k]["ms"])
+    isl = ks[kern]
     avg_s = isl["ms"] / 1e3 / max(isl["launches"], 1)
     per_launch_bytes = isl["bytes"] / max(isl["launches"], 1)
     achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
@@ -284,8 +295,8 @@ def main():
                    "stage_s": {k: round(v, 4) for k, v in res.timings.items()}},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                     "traffic": measured_traffic(args.config, "island_lm_kernel"),
-                     "kernel": "island_lm_kernel", "avg_launch_ms": round(avg_s * 1e3, 4),
+                     "traffic": measured_traffic(args.config, kern),
+                     "kernel": kern, "avg_launch_ms": round(avg_s * 1e3, 4),
                      "launches": isl["launches"],
                      "algorithmic_bytes_per_launch": per_launch_bytes,
                      "valu_tflops": round(isl["flops"] / max(isl["ms"] / 1e3, 1e-12) / 1e12, 4)},

@@ -184,6 +184,10 @@ typedef struct uph_island_split {
   size_t hg_doubles;
   float* p_try;
   int32_t* counters;
+  void* const* eval_events; /* NULL, or 2 * (2 * max_steps + 2) hipEvent_t
+                               recorded before / after each evaluation launch
+                               (chi2 at p0, then H/g and trial per step, then
+                               final fluxes), for per-kernel timing        */
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);

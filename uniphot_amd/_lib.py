@@ -71,7 +71,8 @@ class IslandOutputs(C.Structure):
 
 class IslandSplit(C.Structure):
     _fields_ = [("n_items", C.c_int), ("item_isl", _P), ("item_band", _P), ("hg_off", _P),
-                ("hg", _P), ("hg_doubles", C.c_size_t), ("p_try", _P), ("counters", _P)]
+                ("hg", _P), ("hg_doubles", C.c_size_t), ("p_try", _P), ("counters", _P),
+                ("eval_events", _P)]
 
 
 _lib = None

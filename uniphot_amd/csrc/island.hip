@@ -1272,17 +1272,30 @@ static void split_step(uph::IslandArgs a, int mode, hipStream_t s) {
   hipLaunchKernelGGL(uph::island_step_kernel<Real>, dim3(a.n_isl), dim3(UPH_BLOCK), 0, s, a);
 }
 
+// Evaluation launch k of the sequence, bracketed by the caller's timing
+// events 2k and 2k + 1 when given.
 template <typename Real>
-static void split_sequence(const uph::IslandArgs& a, int* counters, int nblk, hipStream_t s) {
-  split_eval<Real>(a, uph::kSplitEvalInit, counters + 0, nblk, s);
+static void split_eval_timed(const uph::IslandArgs& a, int mode, int* counter, int nblk,
+                             hipStream_t s, void* const* ev, int& k) {
+  if (ev) hipEventRecord(static_cast<hipEvent_t>(ev[2 * k]), s);
+  split_eval<Real>(a, mode, counter, nblk, s);
+  if (ev) hipEventRecord(static_cast<hipEvent_t>(ev[2 * k + 1]), s);
+  ++k;
+}
+
+template <typename Real>
+static void split_sequence(const uph::IslandArgs& a, int* counters, int nblk, hipStream_t s,
+                           void* const* ev) {
+  int k = 0;
+  split_eval_timed<Real>(a, uph::kSplitEvalInit, counters + 0, nblk, s, ev, k);
   split_step<Real>(a, uph::kSplitStepInit, s);
   for (int it = 0; it < a.max_steps; ++it) {
-    split_eval<Real>(a, uph::kSplitEvalGrad, counters + 2 + 2 * it, nblk, s);
+    split_eval_timed<Real>(a, uph::kSplitEvalGrad, counters + 2 + 2 * it, nblk, s, ev, k);
     split_step<Real>(a, uph::kSplitStepSolve, s);
-    split_eval<Real>(a, uph::kSplitEvalTry, counters + 3 + 2 * it, nblk, s);
+    split_eval_timed<Real>(a, uph::kSplitEvalTry, counters + 3 + 2 * it, nblk, s, ev, k);
     split_step<Real>(a, uph::kSplitStepAccept, s);
   }
-  split_eval<Real>(a, uph::kSplitEvalFinal, counters + 1, nblk, s);
+  split_eval_timed<Real>(a, uph::kSplitEvalFinal, counters + 1, nblk, s, ev, k);
 }
 
 extern "C" size_t uph_island_split_counters(int max_steps) {
@@ -1328,8 +1341,8 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
                        "uph_island_fit_split memset");
   if (st != UPH_OK) return st;
   if (precision == UPH_PRECISION_FP32)
-    split_sequence<float>(a, sp->counters, nblk, s);
+    split_sequence<float>(a, sp->counters, nblk, s, sp->eval_events);
   else
-    split_sequence<double>(a, sp->counters, nblk, s);
+    split_sequence<double>(a, sp->counters, nblk, s, sp->eval_events);
   return uph::hip_status(hipGetLastError(), "uph_island_fit_split launch");
 }

@@ -279,19 +279,21 @@ __device__ void fft_lines_block(Real* buf, int n, int logn, int nl, int ls, int 
   }
 }
 
-// Wave-level FFT of length n = 64 E: each wavefront owns one line at a time,
-// lane l holding elements l + 64 e (e < E) in registers.  Bit-reversed
-// gather on load, then decimation-in-time stages: spans 1..32 across lanes
-// (shuffles), spans 64.. inside the lane; natural-order store.  No LDS
-// allocation and no barrier inside a line; the 4 waves take lines
-// round-robin.  Twiddles depend only on the lane (and e for in-lane spans)
-// and are computed once per call.
-template <typename Real, int E>
+// Wave-level FFT of length n = 64 E: each wavefront owns G lines at a
+// time, lane l holding elements l + 64 e (e < E) of each in registers.
+// Bit-reversed gather on load, then decimation-in-time stages: spans 1..32
+// across lanes (shuffles), spans 64.. inside the lane; natural-order store.
+// The G lines' loads are issued together (one memory latency per batch,
+// and adjacent lines of a column pass share cache lines), then the G
+// transforms interleave.  No LDS allocation and no barrier inside a batch;
+// the 4 waves take batches round-robin.  Twiddles depend only on the lane
+// (and e for in-lane spans) and are computed once per call.
+template <typename Real, int E, int G>
 __device__ void fft_lines_wave(Real* buf, int nl, int ls, int es, int sign) {
   constexpr int LOGE = E == 1 ? 0 : (E == 2 ? 1 : (E == 4 ? 2 : 3));
   constexpr int LOGN = 6 + LOGE;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  Real cw[6][2];        // cross-lane stage twiddles (span h = 1 .. 32)
+  Real cw[6][2];            // cross-lane stage twiddles (span h = 1 .. 32)
   Real iw[LOGE + 1][E][2];  // in-lane stage twiddles (span 64 he)
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
@@ -313,15 +315,20 @@ __device__ void fft_lines_wave(Real* buf, int nl, int ls, int es, int sign) {
       iw[t][e][1] = Real(sign) * sn;
     }
   }
-  for (int line = wid; line < nl; line += UPH_WAVES) {
-    Real re[E], im[E];
+  for (int l0 = wid * G; l0 < nl; l0 += UPH_WAVES * G) {
+    Real re[G][E], im[G][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = lane + 64 * e;
-      const int src = (int)(__brev((unsigned)i) >> (32 - LOGN));
-      const Real* p = buf + 2 * ((size_t)line * ls + (size_t)src * es);
-      re[e] = p[0];
-      im[e] = p[1];
+    for (int g = 0; g < G; ++g) {
+      if (l0 + g < nl) {  // wave-uniform
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = lane + 64 * e;
+          const int src = (int)(__brev((unsigned)i) >> (32 - LOGN));
+          const Real* p = buf + 2 * ((size_t)(l0 + g) * ls + (size_t)src * es);
+          re[g][e] = p[0];
+          im[g][e] = p[1];
+        }
+      }
     }
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
@@ -329,15 +336,19 @@ __device__ void fft_lines_wave(Real* buf, int nl, int ls, int es, int sign) {
       const Real wr = cw[s][0], wi = cw[s][1];
       const bool upper = lane & h;
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const Real pr = __shfl_xor(re[e], h, 64), pim = __shfl_xor(im[e], h, 64);
-        if (!upper) {
-          re[e] = re[e] + (wr * pr - wi * pim);
-          im[e] = im[e] + (wr * pim + wi * pr);
-        } else {
-          const Real tr = wr * re[e] - wi * im[e], ti = wr * im[e] + wi * re[e];
-          re[e] = pr - tr;
-          im[e] = pim - ti;
+      for (int g = 0; g < G; ++g) {
+        if (l0 + g >= nl) continue;  // wave-uniform
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const Real pr = __shfl_xor(re[g][e], h, 64), pim = __shfl_xor(im[g][e], h, 64);
+          if (!upper) {
+            re[g][e] = re[g][e] + (wr * pr - wi * pim);
+            im[g][e] = im[g][e] + (wr * pim + wi * pr);
+          } else {
+            const Real tr = wr * re[g][e] - wi * im[g][e], ti = wr * im[g][e] + wi * re[g][e];
+            re[g][e] = pr - tr;
+            im[g][e] = pim - ti;
+          }
         }
       }
     }
@@ -345,24 +356,32 @@ __device__ void fft_lines_wave(Real* buf, int nl, int ls, int es, int sign) {
     for (int t = 0; t < LOGE; ++t) {
       const int he = 1 << t;
 #pragma unroll
-      for (int e = 0; e < E; ++e) {
-        if (e & he) continue;
-        const int f = e + he;
-        const Real wr = iw[t][e][0], wi = iw[t][e][1];
-        const Real br = wr * re[f] - wi * im[f], bi = wr * im[f] + wi * re[f];
-        const Real ar = re[e], ai = im[e];
-        re[e] = ar + br;
-        im[e] = ai + bi;
-        re[f] = ar - br;
-        im[f] = ai - bi;
+      for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          if (e & he) continue;
+          const int f = e + he;
+          const Real wr = iw[t][e][0], wi = iw[t][e][1];
+          const Real br = wr * re[g][f] - wi * im[g][f], bi = wr * im[g][f] + wi * re[g][f];
+          const Real ar = re[g][e], ai = im[g][e];
+          re[g][e] = ar + br;
+          im[g][e] = ai + bi;
+          re[g][f] = ar - br;
+          im[g][f] = ai - bi;
+        }
       }
     }
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int i = lane + 64 * e;
-      Real* p = buf + 2 * ((size_t)line * ls + (size_t)i * es);
-      p[0] = re[e];
-      p[1] = im[e];
+    for (int g = 0; g < G; ++g) {
+      if (l0 + g < nl) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+          const int i = lane + 64 * e;
+          Real* p = buf + 2 * ((size_t)(l0 + g) * ls + (size_t)i * es);
+          p[0] = re[g][e];
+          p[1] = im[g][e];
+        }
+      }
     }
   }
   __syncthreads();
@@ -378,11 +397,11 @@ __device__ void fft_lines(Real* buf, int n, int logn, int nl, int ls, int es, in
   } else {
     __syncthreads();  // lines may have been written by other waves
     if (n == 128)
-      fft_lines_wave<Real, 2>(buf, nl, ls, es, sign);
+      fft_lines_wave<Real, 2, 8>(buf, nl, ls, es, sign);
     else if (n == 256)
-      fft_lines_wave<Real, 4>(buf, nl, ls, es, sign);
+      fft_lines_wave<Real, 4, 4>(buf, nl, ls, es, sign);
     else if (n == 512)
-      fft_lines_wave<Real, 8>(buf, nl, ls, es, sign);
+      fft_lines_wave<Real, 8, 2>(buf, nl, ls, es, sign);
     else
       fft_lines_block(buf, n, logn, nl, ls, es, sign);
   }

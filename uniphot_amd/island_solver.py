@@ -302,8 +302,18 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         counters = torch.empty(n_cnt, dtype=torch.int32, device=dev)
         d_items, d_bands = up(items, np.int32), up(bands, np.int32)
         d_hgoff = up(hg_off, np.int64)
+        n_eval = 2 * int(max_steps) + 2
+        evs = None
+        ev_arr = None
+        if timer is not None:
+            # per-evaluation-launch timing events (created by a first record)
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_eval)]
+            for e in evs:
+                e.record()
+            ev_arr = (C.c_void_p * len(evs))(*[e.cuda_event for e in evs])
         sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
-                              P(p_try), P(counters))
+                              P(p_try), P(counters),
+                              C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None)
         _lib.check(L.uph_island_fit_split(C.byref(bat), C.byref(inp), C.byref(out),
                                           C.byref(opts), prec, C.byref(sp), P(scratch),
                                           scratch.numel(), per, _lib.stream_ptr(stream)))
@@ -311,14 +321,27 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         _lib.check(L.uph_island_fit_batch(C.byref(bat), C.byref(inp), C.byref(out),
                                           C.byref(opts), prec, P(scratch), scratch.numel(), per,
                                           P(counter), _lib.stream_ptr(stream)))
+    area = np.concatenate([h["area"][np.asarray(r)] for _, h, r in classes])
+    mmc = np.concatenate([h["mmask"][np.asarray(r)].sum(axis=1) for _, h, r in classes])
     if ev is not None:
-        area = np.concatenate([h["area"][np.asarray(r)] for _, h, r in classes])
-        mmc = np.concatenate([h["mmask"][np.asarray(r)].sum(axis=1) for _, h, r in classes])
         timer.end(ev, dict(kind="island_batch", n=n_isl, nb=nb, outs=outs, area=area,
-                           nreal=mmc))
+                           nreal=mmc, name="island_split_sequence" if split else None))
     h = {k: v.cpu().numpy() for k, v in outs.items()}
     if split and int(counters[-1].item()) != 0:
         raise _lib.UphError(_lib.UPH_ERR_INTERNAL, "island band blocks overflowed their buffer")
+    if split and evs is not None:
+        # algorithmic bytes of each evaluation launch: data + weight of every
+        # island box it evaluates (INIT and FINAL: all islands; the H/g and
+        # trial launches of step `it`: the islands that took step `it`)
+        nst = h["nst"].astype(np.int64)
+        box = nb * area.astype(np.float64) * 8.0
+        kinds = ["init"] + ["grad", "try"] * int(max_steps) + ["final"]
+        for k, kind in enumerate(kinds):
+            it = (k - 1) // 2
+            sel = np.ones(n_isl, bool) if kind in ("init", "final") else nst > it
+            timer.record("island_eval_kernel", evs[2 * k], evs[2 * k + 1],
+                         dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
+                              n_isl=int(sel.sum())))
     results = []
     for (kern, host, rows), (i0, n_c, m0, f0, N) in zip(classes, spans):
         r = dict(p=h["p"][m0 * 6:(m0 + n_c * N) * 6].reshape(n_c, N, 6),
