@@ -173,7 +173,9 @@ int uph_island_fit_batch(const uph_island_batch* batch, const uph_island_inputs*
  *     (n_live_max^2 + n_live_max + 2) doubles each, inside hg (hg_doubles);
  *   p_try: laid out like outputs->p;
  *   counters: uph_island_split_counters(max_steps) int32 (reset here; the
- *     last one is set to 1 if an island's blocks did not fit in hg).
+ *     last one is set to 1 if an island's or a galaxy's blocks did not fit).
+ * With the galaxy pre-render, scratch_per_block must also cover
+ * uph_galaxy_prerender_scratch_bytes(T) of every stamp tier.
  */
 typedef struct uph_island_split {
   int n_items;
@@ -184,13 +186,35 @@ typedef struct uph_island_split {
   size_t hg_doubles;
   float* p_try;
   int32_t* counters;
-  void* const* eval_events; /* NULL, or 2 * (2 * max_steps + 2) hipEvent_t
-                               recorded before / after each evaluation launch
-                               (chi2 at p0, then H/g and trial per step, then
-                               final fluxes), for per-kernel timing        */
+  void* const* eval_events; /* NULL, or 4 * (2 * max_steps + 2) hipEvent_t:
+                               per evaluation pass (chi2 at p0, then H/g and
+                               trial per step, then final fluxes) one at its
+                               start, one between the galaxy spectra and
+                               planes launches, one before the (island, band)
+                               evaluation and one at its end               */
+  /* galaxy pre-render (float32 only; n_gal = 0 renders galaxies in place):
+     before each pass, the raw planes of every extended member are rendered
+     by (galaxy, band) items and the evaluation only shifts and places them */
+  int n_gal;
+  const int32_t* gal_isl;     /* (n_gal) island                                */
+  const int32_t* gal_slot;    /* (n_gal) member slot within the island         */
+  const int64_t* gal_off;     /* (n_gal) first float of its n_bands x 7 x T^2
+                                 planes in gbuf                                */
+  const int64_t* gal_spec;    /* (n_gal) first float of its 3 x (pad/2+1) x pad
+                                 Fourier spectra in fspec                      */
+  const int32_t* gal_of_slot; /* (total member slots) galaxy index, -1 else    */
+  float* gbuf;
+  size_t gbuf_elems;
+  float* fspec;
+  size_t fspec_elems;
+  double* gdt;                /* (n_gal, 9) scratch                            */
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);
+/* Per-workgroup scratch of the galaxy pre-render for stamp tier T (0 when
+   the pre-render does not support T), and its spectra floats per galaxy. */
+size_t uph_galaxy_prerender_scratch_bytes(int T);
+size_t uph_galaxy_spectra_floats(int T);
 int uph_island_fit_split(const uph_island_batch* batch, const uph_island_inputs* in,
                          const uph_island_outputs* out, const uph_lm_options* opt, int precision,
                          const uph_island_split* split, void* scratch, size_t scratch_bytes,

@@ -93,7 +93,44 @@ def test_mixed_band_split_equals_persistent(cuda):
     args = (batch, trees, 30, torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda))
     ref = run_island_batch(*args, sersic_dev=device_table(cuda))
     print("split mixed batch", [(p.kind, p.n_islands) for p in plans], flush=True)
-    got = run_island_batch(*args, sersic_dev=device_table(cuda), split=True)
+    got = run_island_batch(*args, sersic_dev=device_table(cuda), split=True, prerender=False)
     for ra, rb in zip(ref, got):
         for k in ("p", "fluxes", "ferr", "chi2", "lam", "nst", "conv", "stall", "n_good"):
             assert np.array_equal(ra[k], rb[k]), k
+
+
+def test_galaxy_prerender_matches_oracle(cuda):
+    """Band-split fit with the galaxy planes rendered by their own launches
+    (float32 wave-level FFTs in LDS) against the float64 oracle, at the bar
+    of the in-place render."""
+    import torch
+    from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
+    from uniphot_amd.sersic_table import device_table
+    sc = small_mixed_scene()
+    plans, hosts, pad, d, w = mixed_classes(sc, free_n=(sc.stype == 3))
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms), h, np.arange(p.n_islands))
+             for p, h in zip(plans, hosts)]
+    trees = {T: psf_tree_device(sc.psfs, T, cuda) for T in {p.tier for p in plans}}
+    print("prerender mixed batch", [(p.kind, p.tier, p.n_islands) for p in plans], flush=True)
+    res = run_island_batch(batch, trees, 30, torch.from_numpy(d).to(cuda),
+                           torch.from_numpy(w).to(cuda), sersic_dev=device_table(cuda),
+                           split=True, prerender=True)
+    o = oisl.LMOptions()
+    n_gal = 0
+    for (plan, host), rr in zip(zip(plans, hosts), res):
+        for r in range(plan.n_islands):
+            isl = oracle_island(plan, host, r, d, w, sc.psfs)
+            st = oisl.run_island(host["p0"][r].ravel().astype(np.float64), isl, o)
+            f, e = oisl.final_fluxes(st["p"], isl, o)
+            mm = host["mmask"][r]
+            n_gal += int((host["st"][r][mm] != 0).sum())
+            assert rr["nst"][r] == st["nst"] and rr["conv"][r] == st["conv"]
+            assert rr["stall"][r] == st["stall"]
+            pg, po = rr["p"][r][mm], st["p"].reshape(-1, 6)[mm]
+            assert np.abs(pg[:, :2] - po[:, :2]).max() < 1e-4
+            keep = host["keep"][r][mm]
+            assert np.abs((pg - po)[:, 2:][keep[:, 2:]]).max(initial=0) < 1e-4
+            rel = np.abs(rr["fluxes"][r][:, mm] - f[:, mm]) / np.maximum(np.abs(f[:, mm]),
+                                                                          e[:, mm])
+            assert rel.max() < 1e-5
+    assert n_gal > 0

@@ -20,6 +20,7 @@
 #include "uph_capi_internal.hpp"
 #include "uph_device.hpp"
 #include "uph_galaxy.hpp"
+#include "uph_galaxy_pre.hpp"
 #include "uph_island.hpp"
 
 namespace uph {
@@ -261,7 +262,9 @@ __device__ __forceinline__ void gram(const Real* W, const Real* Z, int P, int C,
 // Only the part of the stamp that lands in the window is evaluated.  With
 // tangents, also d/dx and d/dy (polynomial derivative + tap derivative).
 // ---------------------------------------------------------------------------
-template <typename Real>
+// kCopy separates the copies used by the two evaluation variants (each then
+// has a single caller and is inlined into it).
+template <typename Real, int kCopy>
 __device__ void render_star(const IslandArgs& a, int T, int B, const float* coefs, int b, float xf,
                             float yf, float x0f, float y0f, Real* __restrict__ m_out,
                             Real* __restrict__ tx_out, Real* __restrict__ ty_out,
@@ -378,7 +381,7 @@ __device__ void render_star(const IslandArgs& a, int T, int B, const float* coef
 // island_solver.py:441-562).  Tangent planes of live parameter l go to
 // tan_base + l * B^2.
 // ---------------------------------------------------------------------------
-template <typename Real>
+template <typename Real, bool kPre>
 __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, const float* p6,
                               int r, float x0f, float y0f, Real* __restrict__ m_out,
                               Real* __restrict__ tan_base, Real* __restrict__ gws) {
@@ -396,47 +399,76 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
   const Real dx = xg - rxg, dy = yg - ryg;
   const int ix = int(rxg) - T / 2, iy = int(ryg) - T / 2;
   const Real xs = Real(geo[1]), ys = Real(geo[3]);
-  if (tid == 0) {
-    GalParams<Real>& gp = sh.gp;
-    double n = double(sh.nfz[r]);
-    gp.dndu = 0.0;
-    if (sh.fn[r]) {
-      const double u = double(p6[5]);
-      n = kNMin + (kNMax - kNMin) / (1.0 + exp(-u));
-      gp.dndu = (n - kNMin) * (kNMax - n) / (kNMax - kNMin);
+  const Real* Gsrc;
+  if constexpr (!kPre) {
+    if (tid == 0) {
+      GalParams<Real>& gp = sh.gp;
+      double n = double(sh.nfz[r]);
+      gp.dndu = 0.0;
+      if (sh.fn[r]) {
+        const double u = double(p6[5]);
+        n = kNMin + (kNMax - kNMin) / (1.0 + exp(-u));
+        gp.dndu = (n - kNMin) * (kNMax - n) / (kNMax - kNMin);
+      }
+      gp.n = n;
+      gp.K = int(a.sersic[3]);
+      mog_lookup(a.sersic, n, gp.a, gp.v, gp.core, gp.da, gp.dv, gp.dcore);
+      ellipse_terms(double(p6[2]), double(p6[3]), double(p6[4]), gp.t, gp.dt);
     }
-    gp.n = n;
-    gp.K = int(a.sersic[3]);
-    mog_lookup(a.sersic, n, gp.a, gp.v, gp.core, gp.da, gp.dv, gp.dcore);
-    ellipse_terms(double(p6[2]), double(p6[3]), double(p6[4]), gp.t, gp.dt);
-  }
-  if (tid == 64) lanczos_taps(dx, sh.taps, grads ? sh.taps + 7 : (Real*)nullptr);
-  if (tid == 128) lanczos_taps(dy, sh.taps + 14, grads ? sh.taps + 21 : (Real*)nullptr);
-  if (tid == 192) {
-    const Real X = (x - Real(geo[0])) / xs, Y = (y - Real(geo[2])) / ys;
-    psf_basis(X, Y, nt, sh.bas, sh.bas + 21, sh.bas + 42);
-  }
-  __syncthreads();
-  const GalLayout L = galaxy_layout(T);
-  const float* Cf = sh.coefs + (size_t)b * nt * TT;
-  for (int e = tid; e < TT; e += UPH_BLOCK) {
-    Real v = 0, vx = 0, vy = 0;
-    for (int t = 0; t < nt; ++t) {
-      const Real cv = Real(Cf[(size_t)t * TT + e]);
-      v = fma(cv, sh.bas[t], v);
+    if (tid == 64) lanczos_taps(dx, sh.taps, grads ? sh.taps + 7 : (Real*)nullptr);
+    if (tid == 128) lanczos_taps(dy, sh.taps + 14, grads ? sh.taps + 21 : (Real*)nullptr);
+    if (tid == 192) {
+      const Real X = (x - Real(geo[0])) / xs, Y = (y - Real(geo[2])) / ys;
+      psf_basis(X, Y, nt, sh.bas, sh.bas + 21, sh.bas + 42);
+    }
+    __syncthreads();
+    const GalLayout L = galaxy_layout(T);
+    const float* Cf = sh.coefs + (size_t)b * nt * TT;
+    for (int e = tid; e < TT; e += UPH_BLOCK) {
+      Real v = 0, vx = 0, vy = 0;
+      for (int t = 0; t < nt; ++t) {
+        const Real cv = Real(Cf[(size_t)t * TT + e]);
+        v = fma(cv, sh.bas[t], v);
+        if (gxy) {
+          vx = fma(cv, sh.bas[21 + t], vx);
+          vy = fma(cv, sh.bas[42 + t], vy);
+        }
+      }
+      gws[L.P + e] = v;
       if (gxy) {
-        vx = fma(cv, sh.bas[21 + t], vx);
-        vy = fma(cv, sh.bas[42 + t], vy);
+        gws[L.Px + e] = vx / xs;
+        gws[L.Py + e] = vy / ys;
       }
     }
-    gws[L.P + e] = v;
-    if (gxy) {
-      gws[L.Px + e] = vx / xs;
-      gws[L.Py + e] = vy / ys;
+    __syncthreads();
+    galaxy_raw<Real, false>(sh.gp, T, need, gws);
+    Gsrc = gws + L.G;
+  } else {
+    // raw planes pre-rendered for this pass by the galaxy planes kernel
+    (void)nt;
+    (void)gxy;
+    (void)x;
+    (void)y;
+    (void)xs;
+    (void)ys;
+    if (tid == 0) {
+      sh.gp.dndu = 0.0;
+      if (sh.fn[r]) {
+        const double u = double(p6[5]);
+        const double n = kNMin + (kNMax - kNMin) / (1.0 + exp(-u));
+        sh.gp.dndu = (n - kNMin) * (kNMax - n) / (kNMax - kNMin);
+      }
     }
+    if (tid == 64) lanczos_taps(dx, sh.taps, grads ? sh.taps + 7 : (Real*)nullptr);
+    if (tid == 128) lanczos_taps(dy, sh.taps + 14, grads ? sh.taps + 21 : (Real*)nullptr);
+    __syncthreads();
+    // only the pre-render evaluation kernel instantiates this branch, with
+    // its IslandArgsPre argument
+    const IslandArgsPre& ap = static_cast<const IslandArgsPre&>(a);
+    const int g = ap.gal_of_slot[(size_t)sh.moff + sh.mem[r]];
+    Gsrc = static_cast<const Real*>(ap.gbuf) + ap.gal_off[g] + (size_t)b * 7 * TT;
   }
-  __syncthreads();
-  galaxy_raw<Real, false>(sh.gp, T, need, gws);
+  const GalLayout L = galaxy_layout(T);
   // ---- separable shift + placement of G and the raw tangents ----
   const int wx0 = max(0, ix), wx1 = min(B, ix + T), wy0 = max(0, iy), wy1 = min(B, iy + T);
   if (wx1 <= wx0 || wy1 <= wy0) return;  // uniform
@@ -451,7 +483,7 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
     dLx[k] = grads ? sh.taps[7 + k] : Real(0);
     dLy[k] = grads ? sh.taps[21 + k] : Real(0);
   }
-  const Real* G = gws + L.G;
+  const Real* G = Gsrc;
   Real* Hh = gws + L.Hh;  // plane q of the horizontal pass at Hh + q * T^2
   const int RS = RH * SW;
   for (int e = tid; e < RS; e += UPH_BLOCK) {
@@ -506,7 +538,7 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
 // One band of one island: window copy, member renders, Gram, VarPro fluxes,
 // residual, chi2 and (with grads) accumulation of H, g.
 // ---------------------------------------------------------------------------
-template <typename Real>
+template <typename Real, bool kPre>
 __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int b,
                           const float* pv, bool grads, bool accumulate, Real* planes, Real* ws,
                           Real* gws, double* G, double* part, const Small& sm, float* flux_out,
@@ -554,10 +586,10 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
         if (tx == nullptr) tx = Rp;
         if (ty == nullptr) ty = Rp;
       }
-      render_star<Real>(a, sh.T, B, sh.coefs, b, xk, yk, x0f, y0f, Mp + (size_t)r * P,
+      render_star<Real, kPre ? 1 : 0>(a, sh.T, B, sh.coefs, b, xk, yk, x0f, y0f, Mp + (size_t)r * P,
                         grads ? tx : nullptr, grads ? ty : nullptr, wsx, sh.taps, sh.bas);
     } else {
-      render_galaxy<Real>(a, sh, b, pv + 6 * k, r, x0f, y0f, Mp + (size_t)r * P,
+      render_galaxy<Real, kPre>(a, sh, b, pv + 6 * k, r, x0f, y0f, Mp + (size_t)r * P,
                           grads ? Tp : nullptr, gws);
     }
   }
@@ -689,7 +721,7 @@ __device__ float chi2_all(const IslandArgs& a, IslandSh<Real>& sh, int isl, cons
                           const Small& sm) {
   Real c = 0;
   for (int b = 0; b < a.nb; ++b)
-    c += eval_band<Real>(a, sh, isl, b, pv, false, false, planes, ws, gws, G, part, sm, nullptr,
+    c += eval_band<Real, false>(a, sh, isl, b, pv, false, false, planes, ws, gws, G, part, sm, nullptr,
                          nullptr, nullptr);
   return float(c);
 }
@@ -864,7 +896,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
       const int nL = sh.nL;
       // H, g at p (un-damped, un-masked over the live set)
       for (int b = 0; b < a.nb; ++b)
-        eval_band<Real>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, gws, G, part, sm, nullptr,
+        eval_band<Real, false>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, gws, G, part, sm, nullptr,
                         nullptr, nullptr);
       // damped float64 solve: (H + lam diag(H) + 1e-8 I) delta = -g
       const double lam = double(sh.lam);
@@ -891,7 +923,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     int ngood = 0;
     if (sh.Nr > 0) {
       for (int b = 0; b < a.nb; ++b)
-        eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, gws, G, part, sm, fo + b * N,
+        eval_band<Real, false>(a, sh, isl, b, sh.p, false, false, planes, ws, gws, G, part, sm, fo + b * N,
                         eo + b * N, &ngood);
     }
     for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[m0 * 6 + q] = sh.p[q];
@@ -918,8 +950,19 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
 // accept / reject.  Used for islands whose single-workgroup latency would
 // dominate a batch.
 // ---------------------------------------------------------------------------
-template <typename Real>
-__global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(IslandArgs a) {
+// Argument block of the evaluation kernel: the pre-render variant also
+// carries the galaxy plane buffers.
+template <bool kPre>
+struct EvalArgs {
+  using type = IslandArgs;
+};
+template <>
+struct EvalArgs<true> {
+  using type = IslandArgsPre;
+};
+
+template <typename Real, bool kPre>
+__global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(typename EvalArgs<kPre>::type a) {
   __shared__ IslandSh<Real> sh;
   const int tid = threadIdx.x;
   char* base = a.scratch + (size_t)blockIdx.x * a.scratch_per_block;
@@ -969,12 +1012,12 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(IslandArgs a) {
       float* fo = a.flux_out + (size_t)sh.foff + (size_t)b * N;
       float* eo = a.ferr_out + (size_t)sh.foff + (size_t)b * N;
       int ngood = 0;
-      eval_band<Real>(a, sh, isl, b, sh.p, false, false, planes, ws, gws, G, part, sm, fo, eo,
+      eval_band<Real, kPre>(a, sh, isl, b, sh.p, false, false, planes, ws, gws, G, part, sm, fo, eo,
                       &ngood);
       if (tid == 0) atomicAdd(a.ngood_out + isl, ngood);
     } else {
       const bool grads = mode == kSplitEvalGrad;
-      const Real c = eval_band<Real>(a, sh, isl, b, sh.p, grads, false, planes, ws, gws, G, part,
+      const Real c = eval_band<Real, kPre>(a, sh, isl, b, sh.p, grads, false, planes, ws, gws, G, part,
                                      sm, nullptr, nullptr, nullptr);
       if (grads) {
         for (int e = tid; e < nL * nL; e += UPH_BLOCK) slot[e] = sm.H[e];
@@ -1263,7 +1306,15 @@ template <typename Real>
 static void split_eval(uph::IslandArgs a, int mode, int* counter, int nblk, hipStream_t s) {
   a.mode = mode;
   a.counter = counter;
-  hipLaunchKernelGGL(uph::island_eval_kernel<Real>, dim3(nblk), dim3(UPH_BLOCK), 0, s, a);
+  hipLaunchKernelGGL((uph::island_eval_kernel<Real, false>), dim3(nblk), dim3(UPH_BLOCK), 0, s, a);
+}
+
+static void split_eval_pre(uph::IslandArgsPre a, int mode, int* counter, int nblk,
+                           hipStream_t s) {
+  a.mode = mode;
+  a.counter = counter;
+  hipLaunchKernelGGL((uph::island_eval_kernel<float, true>), dim3(nblk), dim3(UPH_BLOCK), 0, s,
+                     a);
 }
 
 template <typename Real>
@@ -1272,34 +1323,51 @@ static void split_step(uph::IslandArgs a, int mode, hipStream_t s) {
   hipLaunchKernelGGL(uph::island_step_kernel<Real>, dim3(a.n_isl), dim3(UPH_BLOCK), 0, s, a);
 }
 
-// Evaluation launch k of the sequence, bracketed by the caller's timing
-// events 2k and 2k + 1 when given.
+// Evaluation pass k of the sequence: the galaxy pre-render (when `pre` is
+// given) then the (island, band) evaluation, on work-queue counters 2k
+// (evaluation) and 2k + 1 (planes); the caller's timing events 4k (start),
+// 4k + 1 (between spectra and planes), 4k + 2 (before the evaluation) and
+// 4k + 3 (end) when given.
 template <typename Real>
-static void split_eval_timed(const uph::IslandArgs& a, int mode, int* counter, int nblk,
-                             hipStream_t s, void* const* ev, int& k) {
-  if (ev) hipEventRecord(static_cast<hipEvent_t>(ev[2 * k]), s);
-  split_eval<Real>(a, mode, counter, nblk, s);
-  if (ev) hipEventRecord(static_cast<hipEvent_t>(ev[2 * k + 1]), s);
+static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, int mode,
+                       int* counters, int nblk, hipStream_t s, void* const* ev, int& k) {
+  auto rec = [&](int q) {
+    if (ev) hipEventRecord(static_cast<hipEvent_t>(ev[4 * k + q]), s);
+  };
+  rec(0);
+  if (pre) {
+    uph::galaxy_prerender(*pre, mode, counters + 2 * k + 1, nblk, s,
+                          ev ? static_cast<hipEvent_t>(ev[4 * k + 1]) : nullptr);
+  } else {
+    rec(1);
+  }
+  rec(2);
+  if (pre)
+    split_eval_pre(*pre, mode, counters + 2 * k, nblk, s);
+  else
+    split_eval<Real>(a, mode, counters + 2 * k, nblk, s);
+  rec(3);
   ++k;
 }
 
 template <typename Real>
-static void split_sequence(const uph::IslandArgs& a, int* counters, int nblk, hipStream_t s,
-                           void* const* ev) {
+static void split_sequence(const uph::IslandArgs& a, const uph::IslandArgsPre* pre,
+                           int* counters, int nblk, hipStream_t s, void* const* ev) {
   int k = 0;
-  split_eval_timed<Real>(a, uph::kSplitEvalInit, counters + 0, nblk, s, ev, k);
+  split_pass<Real>(a, pre, uph::kSplitEvalInit, counters, nblk, s, ev, k);
   split_step<Real>(a, uph::kSplitStepInit, s);
   for (int it = 0; it < a.max_steps; ++it) {
-    split_eval_timed<Real>(a, uph::kSplitEvalGrad, counters + 2 + 2 * it, nblk, s, ev, k);
+    split_pass<Real>(a, pre, uph::kSplitEvalGrad, counters, nblk, s, ev, k);
     split_step<Real>(a, uph::kSplitStepSolve, s);
-    split_eval_timed<Real>(a, uph::kSplitEvalTry, counters + 3 + 2 * it, nblk, s, ev, k);
+    split_pass<Real>(a, pre, uph::kSplitEvalTry, counters, nblk, s, ev, k);
     split_step<Real>(a, uph::kSplitStepAccept, s);
   }
-  split_eval_timed<Real>(a, uph::kSplitEvalFinal, counters + 1, nblk, s, ev, k);
+  split_pass<Real>(a, pre, uph::kSplitEvalFinal, counters, nblk, s, ev, k);
 }
 
 extern "C" size_t uph_island_split_counters(int max_steps) {
-  return (size_t)(2 * (max_steps > 0 ? max_steps : 0) + 3);
+  const size_t passes = (size_t)(2 * (max_steps > 0 ? max_steps : 0) + 2);
+  return 2 * passes + 1;  // evaluation + planes queue per pass, then the error flag
 }
 
 extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_island_inputs* in,
@@ -1336,13 +1404,39 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
   a.p_try = sp->p_try;
   const size_t nc = uph_island_split_counters(a.max_steps);
   a.err = sp->counters + nc - 1;
+  uph::IslandArgsPre pre{};
+  const bool use_pre = sp->n_gal > 0;
+  if (use_pre) {
+    UPH_CHECK_ARG(precision == UPH_PRECISION_FP32,
+                  "uph_island_fit_split: the galaxy pre-render is float32 only");
+    UPH_CHECK_ARG(sp->gal_isl && sp->gal_slot && sp->gal_off && sp->gal_spec &&
+                      sp->gal_of_slot && sp->gbuf && sp->fspec && sp->gdt,
+                  "uph_island_fit_split: null galaxy arrays");
+    for (int t = 0; t < bat->n_tiers; ++t)
+      UPH_CHECK_ARG(uph::galaxy_prerender_supports(bat->tier_T[t]) &&
+                        uph::galaxy_prerender_scratch_bytes(bat->tier_T[t]) <= scratch_per_block,
+                    "uph_island_fit_split: stamp tier %d not supported by the galaxy pre-render",
+                    bat->tier_T[t]);
+    static_cast<uph::IslandArgs&>(pre) = a;
+    pre.n_gal = sp->n_gal;
+    pre.gal_isl = sp->gal_isl;
+    pre.gal_slot = sp->gal_slot;
+    pre.gal_off = sp->gal_off;
+    pre.gal_spec = sp->gal_spec;
+    pre.gal_of_slot = sp->gal_of_slot;
+    pre.gbuf = sp->gbuf;
+    pre.gbuf_elems = sp->gbuf_elems;
+    pre.fspec = sp->fspec;
+    pre.fspec_elems = sp->fspec_elems;
+    pre.gdt = sp->gdt;
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
   st = uph::hip_status(hipMemsetAsync(sp->counters, 0, nc * sizeof(int32_t), s),
                        "uph_island_fit_split memset");
   if (st != UPH_OK) return st;
   if (precision == UPH_PRECISION_FP32)
-    split_sequence<float>(a, sp->counters, nblk, s, sp->eval_events);
+    split_sequence<float>(a, use_pre ? &pre : nullptr, sp->counters, nblk, s, sp->eval_events);
   else
-    split_sequence<double>(a, sp->counters, nblk, s, sp->eval_events);
+    split_sequence<double>(a, nullptr, sp->counters, nblk, s, sp->eval_events);
   return uph::hip_status(hipGetLastError(), "uph_island_fit_split launch");
 }

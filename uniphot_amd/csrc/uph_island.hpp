@@ -63,6 +63,26 @@ struct IslandArgs {
   int* err;                // set to 1 when an island's band blocks overflow hg
 };
 
+// Galaxy pre-render (band-split path, float32): raw planes of every
+// extended member, n_bands x 7 x T^2 floats per galaxy, rendered before each
+// evaluation pass by the galaxy kernels and read by the evaluation kernel
+// instead of rendering in place.  Only the pre-render launches see these
+// fields; every other kernel takes the base IslandArgs.
+struct IslandArgsPre : IslandArgs {
+  int n_gal;
+  const int32_t* gal_isl;      // (n_gal) island of the galaxy
+  const int32_t* gal_slot;     // (n_gal) member slot within the island
+  const int64_t* gal_off;      // (n_gal) first float of the galaxy's planes in gbuf
+  const int64_t* gal_spec;     // (n_gal) first float of its Fourier spectra in fspec
+  const int32_t* gal_of_slot;  // (total member slots) galaxy index, -1 for stars
+  void* gbuf;                  // raw planes (NULL = render galaxies in place)
+  size_t gbuf_elems;
+  float* fspec;                // per galaxy: F + core, F1, Fn + dcore on the (H, pad) grid
+  size_t fspec_elems;
+  double* gdt;                 // (n_gal, 9) ellipse-term derivatives of each galaxy
+  int* gal_counter;            // work queue of the planes kernel
+};
+
 // Band-split phases.  Evaluation launches run one (island, band) item per
 // workgroup visit; step launches run one workgroup per island.
 constexpr int kSplitEvalInit = 0;   // chi2 per band at p0

@@ -172,9 +172,51 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
     return res
 
 
+def _galaxy_prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prerender,
+                             dev, up):
+    """Galaxy index arrays and buffers of the band-split pre-render: every
+    real extended member, in the islands' work-queue order."""
+    import torch
+    L = _lib.lib()
+    P = _lib.ptr
+    empty = dict(args=(0, None, None, None, None, None, None, 0, None, 0, None),
+                 isl=np.zeros(0, np.int64), T=np.zeros(0, np.int64),
+                 spec_floats=np.zeros(0, np.float64))
+    if not prerender or precision == "fp64":
+        return empty
+    st_all, mm_all = cat["st"], cat["mm"].astype(bool)
+    gal_isl, gal_slot = [], []
+    slot_gal = np.full(m_acc, -1, np.int32)
+    for i in order:
+        m0, N = moff[i], shape[i][0]
+        for k in np.nonzero(mm_all[m0:m0 + N] & (st_all[m0:m0 + N] != 0))[0]:
+            slot_gal[m0 + k] = len(gal_isl)
+            gal_isl.append(int(i))
+            gal_slot.append(int(k))
+    if not gal_isl:
+        return empty
+    gi = np.asarray(gal_isl, np.int64)
+    T = np.asarray([tiers[shape[i][2]] for i in gal_isl], np.int64)
+    if any(L.uph_galaxy_prerender_scratch_bytes(int(t)) == 0 for t in set(T.tolist())):
+        return empty
+    psz = nb * 7 * T * T
+    ssz = np.asarray([L.uph_galaxy_spectra_floats(int(t)) for t in T], np.int64)
+    poff = np.concatenate([[0], np.cumsum(psz)[:-1]]).astype(np.int64)
+    soff = np.concatenate([[0], np.cumsum(ssz)[:-1]]).astype(np.int64)
+    gbuf = torch.empty(int(psz.sum()), dtype=torch.float32, device=dev)
+    fspec = torch.empty(int(ssz.sum()), dtype=torch.float32, device=dev)
+    gdt = torch.empty(len(gal_isl) * 9, dtype=torch.float64, device=dev)
+    keep = [up(np.asarray(gal_isl, np.int32), np.int32), up(np.asarray(gal_slot, np.int32), np.int32),
+            up(poff, np.int64), up(soff, np.int64), up(slot_gal, np.int32)]
+    args = (len(gal_isl), *(P(t) for t in keep), P(gbuf), int(gbuf.numel()), P(fspec),
+            int(fspec.numel()), P(gdt))
+    return dict(args=args, keepalive=(keep, gbuf, fspec, gdt), isl=gi, T=T,
+                spec_floats=ssz.astype(np.float64))
+
+
 def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                      precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
-                     split: bool = False):
+                     split: bool = False, prerender: bool = True):
     """Fit the islands of several classes in ONE persistent launch.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
@@ -184,7 +226,11 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
 
     ``split=True`` runs the band-split form (``uph_island_fit_split``):
     every evaluation pass is spread over (island, band) items, so one large
-    island no longer sets the wall time; results are bit-identical.
+    island no longer sets the wall time; results are bit-identical.  With
+    ``prerender`` (float32 only) the extended members' raw planes are
+    rendered by separate (galaxy, band) launches before each pass (LDS
+    wave-level FFTs) instead of inside the evaluation kernel; results then
+    agree with the in-place render to float32 rounding.
     """
     import torch
     t0 = time.perf_counter()
@@ -239,6 +285,12 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         cat["hi"].append(host["upper"][rows].ravel())
         per = max(per, L.uph_island_scratch_per_block(N, B, kern.T, kern.n_live_max, prec))
     n_isl = len(shape)
+    if split and prerender and precision != "fp64" and any(k.kind != "star" for k, _, _ in classes):
+        for T in tiers:
+            need = int(L.uph_galaxy_prerender_scratch_bytes(int(T)))
+            if need == 0:
+                prerender = False  # a stamp tier the pre-render does not serve
+            per = max(per, need)
     per = (per + 255) // 256 * 256
 
     def up(a, dt):
@@ -306,14 +358,17 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         evs = None
         ev_arr = None
         if timer is not None:
-            # per-evaluation-launch timing events (created by a first record)
-            evs = [torch.cuda.Event(enable_timing=True) for _ in range(2 * n_eval)]
+            # per-pass timing events (created by a first record)
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(4 * n_eval)]
             for e in evs:
                 e.record()
             ev_arr = (C.c_void_p * len(evs))(*[e.cuda_event for e in evs])
+        gal = _galaxy_prerender_arrays(order, shape, moff, c, tiers, nb, m_acc, precision,
+                                       prerender, dev, up)
         sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
                               P(p_try), P(counters),
-                              C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None)
+                              C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None,
+                              *gal["args"])
         _lib.check(L.uph_island_fit_split(C.byref(bat), C.byref(inp), C.byref(out),
                                           C.byref(opts), prec, C.byref(sp), P(scratch),
                                           scratch.numel(), per, _lib.stream_ptr(stream)))
@@ -336,10 +391,26 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         nst = h["nst"].astype(np.int64)
         box = nb * area.astype(np.float64) * 8.0
         kinds = ["init"] + ["grad", "try"] * int(max_steps) + ["final"]
+        g_isl = gal["isl"]
         for k, kind in enumerate(kinds):
             it = (k - 1) // 2
             sel = np.ones(n_isl, bool) if kind in ("init", "final") else nst > it
-            timer.record("island_eval_kernel", evs[2 * k], evs[2 * k + 1],
+            if g_isl.size:
+                gsel = sel[g_isl]
+                tg = gal["T"][gsel].astype(np.float64)
+                # spectra written once per galaxy; per (galaxy, band): PSF
+                # coefficients read, raw planes written
+                nspec = 3.0 if kind == "grad" else 1.0
+                sbytes = float(np.sum(gal["spec_floats"][gsel] / 3.0 * nspec * 4.0))
+                planes = 7.0 if kind == "grad" else 1.0
+                pbytes = float(nb * np.sum(tg * tg * (nt * 4.0 + planes * 4.0)))
+                timer.record("galaxy_spectra_kernel", evs[4 * k], evs[4 * k + 1],
+                             dict(kind="island_eval", phase=kind, bytes=sbytes,
+                                  n_gal=int(gsel.sum())))
+                timer.record("galaxy_planes_kernel", evs[4 * k + 1], evs[4 * k + 2],
+                             dict(kind="island_eval", phase=kind, bytes=pbytes,
+                                  n_gal=int(gsel.sum())))
+            timer.record("island_eval_kernel", evs[4 * k + 2], evs[4 * k + 3],
                          dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
                               n_isl=int(sel.sum())))
     results = []

@@ -49,6 +49,7 @@ class FitConfig:
     precision: str = "fp32"
     stamp_tiers: tuple = (55, 111, 221)
     island_split: bool = True    # band-split island LM (uph_island_fit_split)
+    galaxy_prerender: bool = True  # split path: galaxies rendered by their own launches
 
 
 @dataclass
@@ -263,7 +264,7 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
         sersic_dev = device_table(dstack.device)
     results = run_island_batch(batch, trees, cfg.max_steps, dstack, wstack,
                                precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
-                               split=cfg.island_split)
+                               split=cfg.island_split, prerender=cfg.galaxy_prerender)
     for (plan, host), res in zip(classes, results):
         n_fitted_islands += plan.n_islands
         mm = plan.mmask
