@@ -78,7 +78,8 @@ struct LogE {
 // Forward-transform twiddles of this lane for a line of 64 E points.
 template <int E>
 struct WaveTw {
-  float cw[6][2];
+  float cw[6][2];  // cross-lane stages: this lane's scale (twiddle or 1)
+  float cs[6];     // and sign (+1 lower lane, -1 upper lane of its pair)
   float iw[LogE<E>::v + 1][E][2];
 };
 
@@ -88,10 +89,12 @@ __device__ __forceinline__ void wave_tw(WaveTw<E>& t) {
 #pragma unroll
   for (int s = 0; s < 6; ++s) {
     const int h = 1 << s;
+    const bool upper = lane & h;
     float sn, cs;
     sincospif(float(lane & (h - 1)) / float(h), &sn, &cs);
-    t.cw[s][0] = cs;
-    t.cw[s][1] = -sn;
+    t.cw[s][0] = upper ? cs : 1.0f;
+    t.cw[s][1] = upper ? -sn : 0.0f;
+    t.cs[s] = upper ? -1.0f : 1.0f;
   }
 #pragma unroll
   for (int q = 0; q < LogE<E>::v; ++q) {
@@ -107,31 +110,68 @@ __device__ __forceinline__ void wave_tw(WaveTw<E>& t) {
   }
 }
 
+// x of lane (l ^ H) for power-of-two H < 64 without the LDS crossbar:
+// DPP quad permutes and row (half) mirrors for H <= 8, a bit-mode swizzle
+// for 16, the 32-lane swap for 32.
+template <int H>
+__device__ __forceinline__ float lane_xor(float x) {
+  const int v = __float_as_int(x);
+  int r;
+  if constexpr (H == 1) {
+    r = __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad [1,0,3,2]
+  } else if constexpr (H == 2) {
+    r = __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad [2,3,0,1]
+  } else if constexpr (H == 4) {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // i ^ 7
+    r = __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);             // i ^ 3
+  } else if constexpr (H == 8) {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // i ^ 15
+    r = __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);            // i ^ 7
+  } else if constexpr (H == 16) {
+    r = __builtin_amdgcn_ds_swizzle(v, (0x10 << 10) | 0x1F);  // bit mode: xor 16
+  } else {
+    static_assert(H == 32, "lane_xor: H in 1..32");
+    const auto pr = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    r = (threadIdx.x & 32) ? int(pr[0]) : int(pr[1]);
+  }
+  return __int_as_float(r);
+}
+
+// One cross-lane radix-2 stage of span 2^S.  Each lane scales its own
+// element by w_l (the twiddle on the upper lane of a pair, 1 on the lower
+// lane), swaps the product with its partner, and keeps partner + sg * own
+// (sg = +1 lower, -1 upper): the lower lane ends with a + w b, the upper
+// with a - w b, for a the lower and b the upper element of the pair.
+template <int S, int E>
+__device__ __forceinline__ void lane_stage(float (&re)[E], float (&im)[E], const float wr,
+                                           const float wi, const float sg) {
+  constexpr int h = 1 << S;
+  float zr[E], zi[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    zr[e] = wr * re[e] - wi * im[e];
+    zi[e] = wr * im[e] + wi * re[e];
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float qr = lane_xor<h>(zr[e]), qi = lane_xor<h>(zi[e]);
+    re[e] = fmaf(sg, zr[e], qr);
+    im[e] = fmaf(sg, zi[e], qi);
+  }
+}
+
 // In-register forward FFT of one line held by a wavefront: on entry lane l
 // holds the bit-reversed-order elements l + 64 e, on exit the natural-order
 // outputs l + 64 e.  Radix-2 decimation in time: spans 1..32 across lanes
 // (shuffles), spans 64.. inside the lane.
 template <int E>
 __device__ __forceinline__ void wave_fft(float (&re)[E], float (&im)[E], const WaveTw<E>& t) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int s = 0; s < 6; ++s) {
-    const int h = 1 << s;
-    const float wr = t.cw[s][0], wi = t.cw[s][1];
-    const bool upper = lane & h;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const float pr = __shfl_xor(re[e], h, 64), pim = __shfl_xor(im[e], h, 64);
-      if (!upper) {
-        re[e] = re[e] + (wr * pr - wi * pim);
-        im[e] = im[e] + (wr * pim + wi * pr);
-      } else {
-        const float tr = wr * re[e] - wi * im[e], ti = wr * im[e] + wi * re[e];
-        re[e] = pr - tr;
-        im[e] = pim - ti;
-      }
-    }
-  }
+  lane_stage<0, E>(re, im, t.cw[0][0], t.cw[0][1], t.cs[0]);
+  lane_stage<1, E>(re, im, t.cw[1][0], t.cw[1][1], t.cs[1]);
+  lane_stage<2, E>(re, im, t.cw[2][0], t.cw[2][1], t.cs[2]);
+  lane_stage<3, E>(re, im, t.cw[3][0], t.cw[3][1], t.cs[3]);
+  lane_stage<4, E>(re, im, t.cw[4][0], t.cw[4][1], t.cs[4]);
+  lane_stage<5, E>(re, im, t.cw[5][0], t.cw[5][1], t.cs[5]);
 #pragma unroll
   for (int q = 0; q < LogE<E>::v; ++q) {
     const int he = 1 << q;
@@ -225,41 +265,73 @@ struct PlaneJob {
   const double* dt;
 };
 
+// Lines of one pass handled together by a wavefront (G E = 8 complex values
+// in flight per lane): the gathers of G lines are issued before any of
+// their transforms, so one memory latency covers the batch.
+template <int E>
+struct Batch {
+  static constexpr int G = 8 / E;
+};
+
 // Forward transform of the T x T real plane src into S (H columns of pad).
 template <int E>
 __device__ void forward_plane(const PlaneJob& j, const float* src, const WaveTw<E>& tw) {
+  constexpr int G = Batch<E>::G;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int T = j.T, H = j.H, pad = j.pad, logp = j.logp;
   // rows: A[k][r] = sum_c P[r][c] e^{-2 pi i k c / pad}, k < H
-  for (int r = wid; r < T; r += kGalWaves) {
-    float re[E], im[E];
+  for (int r0 = wid * G; r0 < T; r0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int c = brev_bits(lane + 64 * e, logp);
-      re[e] = c < T ? src[r * T + c] : 0.0f;
-      im[e] = 0.0f;
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = brev_bits(lane + 64 * e, logp);
+        re[g][e] = (r < T && c < T) ? src[r * T + c] : 0.0f;
+        im[g][e] = 0.0f;
+      }
     }
-    wave_fft<E>(re, im, tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int k = lane + 64 * e;
-      if (k < H) j.A[k * T + r] = make_float2(re[e], im[e]);
+    for (int g = 0; g < G; ++g)
+      if (r0 + g < T) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+      if (r >= T) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int k = lane + 64 * e;
+        if (k < H) j.A[k * T + r] = make_float2(re[g][e], im[g][e]);
+      }
     }
   }
   __syncthreads();
   // columns: S[k][m] = sum_r A[k][r] e^{-2 pi i m r / pad}
-  for (int k = wid; k < H; k += kGalWaves) {
-    float re[E], im[E];
+  for (int k0 = wid * G; k0 < H; k0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int r = brev_bits(lane + 64 * e, logp);
-      const float2 v = r < T ? j.A[k * T + r] : make_float2(0.0f, 0.0f);
-      re[e] = v.x;
-      im[e] = v.y;
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int r = brev_bits(lane + 64 * e, logp);
+        const float2 v = (k < H && r < T) ? j.A[k * T + r] : make_float2(0.0f, 0.0f);
+        re[g][e] = v.x;
+        im[g][e] = v.y;
+      }
     }
-    wave_fft<E>(re, im, tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) j.S[(size_t)k * pad + lane + 64 * e] = make_float2(re[e], im[e]);
+    for (int g = 0; g < G; ++g)
+      if (k0 + g < H) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      if (k >= H) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        j.S[(size_t)k * pad + lane + 64 * e] = make_float2(re[g][e], im[g][e]);
+    }
   }
   __syncthreads();
 }
@@ -269,62 +341,91 @@ __device__ void forward_plane(const PlaneJob& j, const float* src, const WaveTw<
 // the T x T plane dst: dst = irfft2(S X, (pad, pad))[:T, :T].
 template <int E>
 __device__ void inverse_plane(const PlaneJob& j, int kind, float* dst, const WaveTw<E>& tw) {
+  constexpr int G = Batch<E>::G;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int T = j.T, H = j.H, pad = j.pad, logp = j.logp, n = H * pad;
   const float* X = j.F + (kind == 0 ? 0 : (kind == 4 ? 2 * n : n));
   const double c2 = -2.0 * 9.869604401089358;
   const int p = kind - 1;
   // columns: Y[k][r] = sum_m S[k][m] X[k][m] e^{+2 pi i m r / pad}, r < T
-  for (int k = wid; k < H; k += kGalWaves) {
-    const double v = double(k) / pad;
-    float re[E], im[E];
+  for (int k0 = wid * G; k0 < H; k0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int m = brev_bits(lane + 64 * e, logp);
-      const size_t o = (size_t)k * pad + m;
-      const float2 s = j.S[o];
-      float x = X[o];
-      if (kind >= 1 && kind <= 3) {
-        const double w = double(m < pad / 2 ? m : m - pad) / pad;
-        const double dex = c2 * (j.dt[0 * 3 + p] * v * v + j.dt[1 * 3 + p] * w * w +
-                                 2.0 * j.dt[2 * 3 + p] * v * w);
-        x = float(double(x) * dex);
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      const double v = double(k) / pad;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int m = brev_bits(lane + 64 * e, logp);
+        float2 sv = make_float2(0.0f, 0.0f);
+        float x = 0.0f;
+        if (k < H) {
+          const size_t o = (size_t)k * pad + m;
+          sv = j.S[o];
+          x = X[o];
+          if (kind >= 1 && kind <= 3) {
+            const double w = double(m < pad / 2 ? m : m - pad) / pad;
+            const double dex = c2 * (j.dt[0 * 3 + p] * v * v + j.dt[1 * 3 + p] * w * w +
+                                     2.0 * j.dt[2 * 3 + p] * v * w);
+            x = float(double(x) * dex);
+          }
+        }
+        // inverse via conj(FFT(conj(z)))
+        re[g][e] = sv.x * x;
+        im[g][e] = -(sv.y * x);
       }
-      // inverse via conj(FFT(conj(z)))
-      re[e] = s.x * x;
-      im[e] = -(s.y * x);
     }
-    wave_fft<E>(re, im, tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int r = lane + 64 * e;
-      if (r < T) j.A[k * T + r] = make_float2(re[e], -im[e]);
+    for (int g = 0; g < G; ++g)
+      if (k0 + g < H) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      if (k >= H) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int r = lane + 64 * e;
+        if (r < T) j.A[k * T + r] = make_float2(re[g][e], -im[g][e]);
+      }
     }
   }
   __syncthreads();
   // rows: out[r][c] = Re sum_k X_r[k] e^{+2 pi i k c / pad} / pad^2 with
   // the Hermitian extension X_r[k] = Y[k][r] (k < H), conj(Y[pad - k][r])
   const float norm = 1.0f / (float(pad) * float(pad));
-  for (int r = wid; r < T; r += kGalWaves) {
-    float re[E], im[E];
+  for (int r0 = wid * G; r0 < T; r0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int c = brev_bits(lane + 64 * e, logp);
-      float2 y;
-      if (c < H) {
-        y = j.A[c * T + r];
-      } else {
-        y = j.A[(pad - c) * T + r];
-        y.y = -y.y;
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = brev_bits(lane + 64 * e, logp);
+        float2 y = make_float2(0.0f, 0.0f);
+        if (r < T) {
+          if (c < H) {
+            y = j.A[c * T + r];
+          } else {
+            y = j.A[(pad - c) * T + r];
+            y.y = -y.y;
+          }
+        }
+        re[g][e] = y.x;
+        im[g][e] = -y.y;
       }
-      re[e] = y.x;
-      im[e] = -y.y;
     }
-    wave_fft<E>(re, im, tw);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int c = lane + 64 * e;
-      if (c < T) dst[r * T + c] = re[e] * norm;
+    for (int g = 0; g < G; ++g)
+      if (r0 + g < T) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+      if (r >= T) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = lane + 64 * e;
+        if (c < T) dst[r * T + c] = re[g][e] * norm;
+      }
     }
   }
   __syncthreads();
@@ -427,12 +528,19 @@ __global__ __launch_bounds__(kGalBlock) void galaxy_planes_kernel(IslandArgsPre 
     const bool gxy = need & 3;
     for (int e = tid; e < TT; e += kGalBlock) {
       float v = 0.0f, vx = 0.0f, vy = 0.0f;
-      for (int t = 0; t < nt; ++t) {
-        const float cv = Cf[(size_t)t * TT + e];
-        v = fmaf(cv, bas[t], v);
-        if (gxy) {
-          vx = fmaf(cv, bas[21 + t], vx);
-          vy = fmaf(cv, bas[42 + t], vy);
+      // coefficient loads in groups of 8 terms, all issued before their FMAs
+      for (int t0 = 0; t0 < nt; t0 += 8) {
+        float cv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) cv[u] = t0 + u < nt ? Cf[(size_t)(t0 + u) * TT + e] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          if (t0 + u >= nt) break;
+          v = fmaf(cv[u], bas[t0 + u], v);
+          if (gxy) {
+            vx = fmaf(cv[u], bas[21 + t0 + u], vx);
+            vy = fmaf(cv[u], bas[42 + t0 + u], vy);
+          }
         }
       }
       j.P3[e] = v;
