@@ -238,7 +238,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by independent islands and grid tiles; see DESIGN.md)",
+              "(the fit would shard by independent islands and grid tiles across ranks)",
               file=sys.stderr)
         sys.exit(2)
     import torch

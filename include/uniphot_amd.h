@@ -174,8 +174,8 @@ int uph_island_fit_batch(const uph_island_batch* batch, const uph_island_inputs*
  *   p_try: laid out like outputs->p;
  *   counters: uph_island_split_counters(max_steps) int32 (reset here; the
  *     last one is set to 1 if an island's or a galaxy's blocks did not fit).
- * With the galaxy pre-render, scratch_per_block must also cover
- * uph_galaxy_prerender_scratch_bytes(T) of every stamp tier.
+ * With the member pre-render, scratch_per_block must also cover
+ * uph_prerender_scratch_bytes(T) of every stamp tier.
  */
 typedef struct uph_island_split {
   int n_items;
@@ -186,35 +186,44 @@ typedef struct uph_island_split {
   size_t hg_doubles;
   float* p_try;
   int32_t* counters;
-  void* const* eval_events; /* NULL, or 4 * (2 * max_steps + 2) hipEvent_t:
+  void* const* eval_events; /* NULL, or 5 * (2 * max_steps + 2) hipEvent_t:
                                per evaluation pass (chi2 at p0, then H/g and
                                trial per step, then final fluxes) one at its
-                               start, one between the galaxy spectra and
-                               planes launches, one before the (island, band)
-                               evaluation and one at its end               */
-  /* galaxy pre-render (float32 only; n_gal = 0 renders galaxies in place):
-     before each pass, the raw planes of every extended member are rendered
-     by (galaxy, band) items and the evaluation only shifts and places them */
-  int n_gal;
-  const int32_t* gal_isl;     /* (n_gal) island                                */
-  const int32_t* gal_slot;    /* (n_gal) member slot within the island         */
-  const int64_t* gal_off;     /* (n_gal) first float of its n_bands x 7 x T^2
-                                 planes in gbuf                                */
-  const int64_t* gal_spec;    /* (n_gal) first float of its 3 x (pad/2+1) x pad
-                                 Fourier spectra in fspec                      */
-  const int32_t* gal_of_slot; /* (total member slots) galaxy index, -1 else    */
-  float* gbuf;
-  size_t gbuf_elems;
+                               start, one after the pre-render taps/spectra
+                               launch, one after the extended-member planes,
+                               one after the star planes (before the
+                               (island, band) evaluation) and one at its end */
+  /* member pre-render (float32 only; n_pre = 0 renders members in place):
+     before each pass every real member of the active islands is rendered
+     and shifted by (member, band) items into its planes, and the evaluation
+     only places their window part */
+  int n_pre;
+  int n_pre_ext;              /* members [0, n_pre_ext) are extended, the
+                                 rest stars                                    */
+  const int32_t* pre_isl;     /* (n_pre) island                                */
+  const int32_t* pre_slot;    /* (n_pre) member slot within the island         */
+  const int64_t* pre_off;     /* (n_pre) first float of its n_bands x np x T^2
+                                 planes in pbuf, np = 7 for an extended member
+                                 (model and 6 tangents), 3 for a star (model,
+                                 d/dx, d/dy)                                   */
+  const int64_t* pre_spec;    /* (n_pre) first float of its 3 x (pad/2+1) x pad
+                                 Fourier spectra in fspec; -1 for stars        */
+  const int32_t* pre_of_slot; /* (total member slots) pre-render index, -1
+                                 for padding slots                             */
+  float* pbuf;
+  size_t pbuf_elems;
   float* fspec;
   size_t fspec_elems;
-  double* gdt;                /* (n_gal, 9) scratch                            */
+  double* gdt;                /* (n_pre, 9) scratch                            */
+  float* gtap;                /* (n_pre, 28) scratch: Lanczos taps per pass    */
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);
-/* Per-workgroup scratch of the galaxy pre-render for stamp tier T (0 when
-   the pre-render does not support T), and its spectra floats per galaxy. */
-size_t uph_galaxy_prerender_scratch_bytes(int T);
-size_t uph_galaxy_spectra_floats(int T);
+/* Per-workgroup scratch of the member pre-render for stamp tier T (0 when
+   the pre-render does not support T), and the spectra floats of one
+   extended member. */
+size_t uph_prerender_scratch_bytes(int T);
+size_t uph_prerender_spectra_floats(int T);
 int uph_island_fit_split(const uph_island_batch* batch, const uph_island_inputs* in,
                          const uph_island_outputs* out, const uph_lm_options* opt, int precision,
                          const uph_island_split* split, void* scratch, size_t scratch_bytes,

@@ -18,7 +18,7 @@ UPH_PRECISION_FP32, UPH_PRECISION_FP64 = 0, 1
 EXPORTS = (
     "uph_version", "uph_last_error", "uph_island_scratch_per_block", "uph_island_fit",
     "uph_island_fit_batch", "uph_island_split_counters", "uph_island_fit_split",
-    "uph_galaxy_prerender_scratch_bytes", "uph_galaxy_spectra_floats",
+    "uph_prerender_scratch_bytes", "uph_prerender_spectra_floats",
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
 )
 
@@ -73,10 +73,10 @@ class IslandOutputs(C.Structure):
 class IslandSplit(C.Structure):
     _fields_ = [("n_items", C.c_int), ("item_isl", _P), ("item_band", _P), ("hg_off", _P),
                 ("hg", _P), ("hg_doubles", C.c_size_t), ("p_try", _P), ("counters", _P),
-                ("eval_events", _P), ("n_gal", C.c_int), ("gal_isl", _P), ("gal_slot", _P),
-                ("gal_off", _P), ("gal_spec", _P), ("gal_of_slot", _P), ("gbuf", _P),
-                ("gbuf_elems", C.c_size_t), ("fspec", _P), ("fspec_elems", C.c_size_t),
-                ("gdt", _P)]
+                ("eval_events", _P), ("n_pre", C.c_int), ("n_pre_ext", C.c_int),
+                ("pre_isl", _P), ("pre_slot", _P), ("pre_off", _P), ("pre_spec", _P),
+                ("pre_of_slot", _P), ("pbuf", _P), ("pbuf_elems", C.c_size_t), ("fspec", _P),
+                ("fspec_elems", C.c_size_t), ("gdt", _P), ("gtap", _P)]
 
 
 _lib = None
@@ -106,10 +106,10 @@ def lib() -> C.CDLL:
                                        _P, C.c_size_t, C.c_size_t, _P, _P]
     L.uph_island_split_counters.restype = C.c_size_t
     L.uph_island_split_counters.argtypes = [C.c_int]
-    L.uph_galaxy_prerender_scratch_bytes.restype = C.c_size_t
-    L.uph_galaxy_prerender_scratch_bytes.argtypes = [C.c_int]
-    L.uph_galaxy_spectra_floats.restype = C.c_size_t
-    L.uph_galaxy_spectra_floats.argtypes = [C.c_int]
+    L.uph_prerender_scratch_bytes.restype = C.c_size_t
+    L.uph_prerender_scratch_bytes.argtypes = [C.c_int]
+    L.uph_prerender_spectra_floats.restype = C.c_size_t
+    L.uph_prerender_spectra_floats.argtypes = [C.c_int]
     L.uph_island_fit_split.restype = C.c_int
     L.uph_island_fit_split.argtypes = [C.POINTER(IslandBatch), C.POINTER(IslandInputs),
                                        C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,

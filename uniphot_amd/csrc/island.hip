@@ -20,7 +20,7 @@
 #include "uph_capi_internal.hpp"
 #include "uph_device.hpp"
 #include "uph_galaxy.hpp"
-#include "uph_galaxy_pre.hpp"
+#include "uph_prerender.hpp"
 #include "uph_island.hpp"
 
 namespace uph {
@@ -262,9 +262,7 @@ __device__ __forceinline__ void gram(const Real* W, const Real* Z, int P, int C,
 // Only the part of the stamp that lands in the window is evaluated.  With
 // tangents, also d/dx and d/dy (polynomial derivative + tap derivative).
 // ---------------------------------------------------------------------------
-// kCopy separates the copies used by the two evaluation variants (each then
-// has a single caller and is inlined into it).
-template <typename Real, int kCopy>
+template <typename Real>
 __device__ void render_star(const IslandArgs& a, int T, int B, const float* coefs, int b, float xf,
                             float yf, float x0f, float y0f, Real* __restrict__ m_out,
                             Real* __restrict__ tx_out, Real* __restrict__ ty_out,
@@ -303,12 +301,14 @@ __device__ void render_star(const IslandArgs& a, int T, int B, const float* coef
     const int r = ry0 + e / RW, c = rx0 + e % RW;
     const float* cp = Cf + r * T + c;
     Real s = 0, sx = 0, sy = 0;
-    for (int t = 0; t < nt; ++t) {
-      const Real cv = Real(cp[(size_t)t * TT]);
-      s = fma(cv, bas[t], s);
-      if (grad) {
-        sx = fma(cv, bas[21 + t], sx);
-        sy = fma(cv, bas[42 + t], sy);
+    {
+      for (int t = 0; t < nt; ++t) {
+        const Real cv = Real(cp[(size_t)t * TT]);
+        s = fma(cv, bas[t], s);
+        if (grad) {
+          sx = fma(cv, bas[21 + t], sx);
+          sy = fma(cv, bas[42 + t], sy);
+        }
       }
     }
     P[e] = s;
@@ -381,7 +381,7 @@ __device__ void render_star(const IslandArgs& a, int T, int B, const float* coef
 // island_solver.py:441-562).  Tangent planes of live parameter l go to
 // tan_base + l * B^2.
 // ---------------------------------------------------------------------------
-template <typename Real, bool kPre>
+template <typename Real>
 __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, const float* p6,
                               int r, float x0f, float y0f, Real* __restrict__ m_out,
                               Real* __restrict__ tan_base, Real* __restrict__ gws) {
@@ -399,8 +399,7 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
   const Real dx = xg - rxg, dy = yg - ryg;
   const int ix = int(rxg) - T / 2, iy = int(ryg) - T / 2;
   const Real xs = Real(geo[1]), ys = Real(geo[3]);
-  const Real* Gsrc;
-  if constexpr (!kPre) {
+  {
     if (tid == 0) {
       GalParams<Real>& gp = sh.gp;
       double n = double(sh.nfz[r]);
@@ -442,31 +441,6 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
     }
     __syncthreads();
     galaxy_raw<Real, false>(sh.gp, T, need, gws);
-    Gsrc = gws + L.G;
-  } else {
-    // raw planes pre-rendered for this pass by the galaxy planes kernel
-    (void)nt;
-    (void)gxy;
-    (void)x;
-    (void)y;
-    (void)xs;
-    (void)ys;
-    if (tid == 0) {
-      sh.gp.dndu = 0.0;
-      if (sh.fn[r]) {
-        const double u = double(p6[5]);
-        const double n = kNMin + (kNMax - kNMin) / (1.0 + exp(-u));
-        sh.gp.dndu = (n - kNMin) * (kNMax - n) / (kNMax - kNMin);
-      }
-    }
-    if (tid == 64) lanczos_taps(dx, sh.taps, grads ? sh.taps + 7 : (Real*)nullptr);
-    if (tid == 128) lanczos_taps(dy, sh.taps + 14, grads ? sh.taps + 21 : (Real*)nullptr);
-    __syncthreads();
-    // only the pre-render evaluation kernel instantiates this branch, with
-    // its IslandArgsPre argument
-    const IslandArgsPre& ap = static_cast<const IslandArgsPre&>(a);
-    const int g = ap.gal_of_slot[(size_t)sh.moff + sh.mem[r]];
-    Gsrc = static_cast<const Real*>(ap.gbuf) + ap.gal_off[g] + (size_t)b * 7 * TT;
   }
   const GalLayout L = galaxy_layout(T);
   // ---- separable shift + placement of G and the raw tangents ----
@@ -483,7 +457,7 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
     dLx[k] = grads ? sh.taps[7 + k] : Real(0);
     dLy[k] = grads ? sh.taps[21 + k] : Real(0);
   }
-  const Real* G = Gsrc;
+  const Real* G = gws + L.G;
   Real* Hh = gws + L.Hh;  // plane q of the horizontal pass at Hh + q * T^2
   const int RS = RH * SW;
   for (int e = tid; e < RS; e += UPH_BLOCK) {
@@ -534,6 +508,35 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
   __syncthreads();
 }
 
+// Pre-render variant of the member render: copy the window part of the
+// member's shifted planes (model, then the live tangents; 7 planes per band
+// for an extended member, 3 for a star) into the island planes.  The stamp
+// sits at rint(x - x0) - T / 2 as in the in-place render.
+template <typename Real>
+__device__ void place_member(const IslandArgs& a, IslandSh<Real>& sh, int b, const float* p6,
+                             int r, float x0f, float y0f, Real* __restrict__ m_out,
+                             Real* __restrict__ tan_base) {
+  const IslandArgsPre& ap = static_cast<const IslandArgsPre&>(a);
+  const int T = sh.T, B = sh.B, TT = T * T, P = B * B, tid = threadIdx.x;
+  const Real xg = Real(p6[0] - x0f), yg = Real(p6[1] - y0f);
+  const int ix = int(rint(xg)) - T / 2, iy = int(rint(yg)) - T / 2;
+  const int wx0 = max(0, ix), wx1 = min(B, ix + T), wy0 = max(0, iy), wy1 = min(B, iy + T);
+  if (wx1 <= wx0 || wy1 <= wy0) return;  // uniform
+  const int sx0 = wx0 - ix, sy0 = wy0 - iy, SW = wx1 - wx0, SH = wy1 - wy0;
+  const int g = ap.pre_of_slot[(size_t)sh.moff + sh.mem[r]];
+  const float* blk = ap.pbuf + ap.pre_off[g] + (size_t)b * pre_planes(sh.st[r] == 0) * TT;
+  const int l0 = sh.first_live[r], l1 = sh.first_live[r + 1];
+  for (int e = tid; e < SH * SW; e += UPH_BLOCK) {
+    const int rs = sy0 + e / SW, cs = sx0 + e % SW;
+    const int wi = (rs + iy) * B + (cs + ix), so = rs * T + cs;
+    m_out[wi] = Real(blk[so]);
+    if (tan_base)
+      for (int l = l0; l < l1; ++l)
+        tan_base[(size_t)l * P + wi] = Real(blk[(size_t)(1 + sh.live_j[l]) * TT + so]);
+  }
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------------------
 // One band of one island: window copy, member renders, Gram, VarPro fluxes,
 // residual, chi2 and (with grads) accumulation of H, g.
@@ -579,17 +582,20 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
         if (sh.live_j[l] == 1) ty = Tp + (size_t)l * P;
       }
     }
-    if (sh.st[r] == 0) {
+    if constexpr (kPre) {
+      place_member<Real>(a, sh, b, pv + 6 * k, r, x0f, y0f, Mp + (size_t)r * P,
+                         grads ? Tp : nullptr);
+    } else if (sh.st[r] == 0) {
       Real* wsx = ws;
       if (grads && (tx == nullptr || ty == nullptr)) {
         // a frozen coordinate: render both, discard into the R plane
         if (tx == nullptr) tx = Rp;
         if (ty == nullptr) ty = Rp;
       }
-      render_star<Real, kPre ? 1 : 0>(a, sh.T, B, sh.coefs, b, xk, yk, x0f, y0f, Mp + (size_t)r * P,
+      render_star<Real>(a, sh.T, B, sh.coefs, b, xk, yk, x0f, y0f, Mp + (size_t)r * P,
                         grads ? tx : nullptr, grads ? ty : nullptr, wsx, sh.taps, sh.bas);
     } else {
-      render_galaxy<Real, kPre>(a, sh, b, pv + 6 * k, r, x0f, y0f, Mp + (size_t)r * P,
+      render_galaxy<Real>(a, sh, b, pv + 6 * k, r, x0f, y0f, Mp + (size_t)r * P,
                           grads ? Tp : nullptr, gws);
     }
   }
@@ -1323,30 +1329,32 @@ static void split_step(uph::IslandArgs a, int mode, hipStream_t s) {
   hipLaunchKernelGGL(uph::island_step_kernel<Real>, dim3(a.n_isl), dim3(UPH_BLOCK), 0, s, a);
 }
 
-// Evaluation pass k of the sequence: the galaxy pre-render (when `pre` is
+// Evaluation pass k of the sequence: the member pre-render (when `pre` is
 // given) then the (island, band) evaluation, on work-queue counters 2k
-// (evaluation) and 2k + 1 (planes); the caller's timing events 4k (start),
-// 4k + 1 (between spectra and planes), 4k + 2 (before the evaluation) and
-// 4k + 3 (end) when given.
+// (evaluation) and 2k + 1 (extended-member planes); the caller's timing
+// events 5k (start), 5k + 1 (after the taps/spectra launch), 5k + 2 (after
+// the extended-member planes), 5k + 3 (after the star planes, before the
+// evaluation) and 5k + 4 (end) when given.
 template <typename Real>
 static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, int mode,
                        int* counters, int nblk, hipStream_t s, void* const* ev, int& k) {
+  auto evk = [&](int q) { return ev ? static_cast<hipEvent_t>(ev[5 * k + q]) : nullptr; };
   auto rec = [&](int q) {
-    if (ev) hipEventRecord(static_cast<hipEvent_t>(ev[4 * k + q]), s);
+    if (ev) (void)hipEventRecord(evk(q), s);
   };
   rec(0);
   if (pre) {
-    uph::galaxy_prerender(*pre, mode, counters + 2 * k + 1, nblk, s,
-                          ev ? static_cast<hipEvent_t>(ev[4 * k + 1]) : nullptr);
+    uph::prerender_pass(*pre, mode, counters + 2 * k + 1, nblk, s, evk(1), evk(2));
   } else {
     rec(1);
+    rec(2);
   }
-  rec(2);
+  rec(3);
   if (pre)
     split_eval_pre(*pre, mode, counters + 2 * k, nblk, s);
   else
     split_eval<Real>(a, mode, counters + 2 * k, nblk, s);
-  rec(3);
+  rec(4);
   ++k;
 }
 
@@ -1405,30 +1413,35 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
   const size_t nc = uph_island_split_counters(a.max_steps);
   a.err = sp->counters + nc - 1;
   uph::IslandArgsPre pre{};
-  const bool use_pre = sp->n_gal > 0;
+  const bool use_pre = sp->n_pre > 0;
   if (use_pre) {
     UPH_CHECK_ARG(precision == UPH_PRECISION_FP32,
-                  "uph_island_fit_split: the galaxy pre-render is float32 only");
-    UPH_CHECK_ARG(sp->gal_isl && sp->gal_slot && sp->gal_off && sp->gal_spec &&
-                      sp->gal_of_slot && sp->gbuf && sp->fspec && sp->gdt,
-                  "uph_island_fit_split: null galaxy arrays");
+                  "uph_island_fit_split: the member pre-render is float32 only");
+    UPH_CHECK_ARG(sp->pre_isl && sp->pre_slot && sp->pre_off && sp->pre_spec &&
+                      sp->pre_of_slot && sp->pbuf && sp->fspec && sp->gdt && sp->gtap,
+                  "uph_island_fit_split: null pre-render arrays");
+    UPH_CHECK_ARG(sp->n_pre_ext >= 0 && sp->n_pre_ext <= sp->n_pre,
+                  "uph_island_fit_split: n_pre_ext=%d outside [0, n_pre=%d]", sp->n_pre_ext,
+                  sp->n_pre);
     for (int t = 0; t < bat->n_tiers; ++t)
-      UPH_CHECK_ARG(uph::galaxy_prerender_supports(bat->tier_T[t]) &&
-                        uph::galaxy_prerender_scratch_bytes(bat->tier_T[t]) <= scratch_per_block,
-                    "uph_island_fit_split: stamp tier %d not supported by the galaxy pre-render",
+      UPH_CHECK_ARG(uph::prerender_supports(bat->tier_T[t]) &&
+                        uph::prerender_scratch(bat->tier_T[t]) <= scratch_per_block,
+                    "uph_island_fit_split: stamp tier %d not supported by the member pre-render",
                     bat->tier_T[t]);
     static_cast<uph::IslandArgs&>(pre) = a;
-    pre.n_gal = sp->n_gal;
-    pre.gal_isl = sp->gal_isl;
-    pre.gal_slot = sp->gal_slot;
-    pre.gal_off = sp->gal_off;
-    pre.gal_spec = sp->gal_spec;
-    pre.gal_of_slot = sp->gal_of_slot;
-    pre.gbuf = sp->gbuf;
-    pre.gbuf_elems = sp->gbuf_elems;
+    pre.n_pre = sp->n_pre;
+    pre.n_pre_ext = sp->n_pre_ext;
+    pre.pre_isl = sp->pre_isl;
+    pre.pre_slot = sp->pre_slot;
+    pre.pre_off = sp->pre_off;
+    pre.pre_spec = sp->pre_spec;
+    pre.pre_of_slot = sp->pre_of_slot;
+    pre.pbuf = sp->pbuf;
+    pre.pbuf_elems = sp->pbuf_elems;
     pre.fspec = sp->fspec;
     pre.fspec_elems = sp->fspec_elems;
     pre.gdt = sp->gdt;
+    pre.gtap = sp->gtap;
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
   st = uph::hip_status(hipMemsetAsync(sp->counters, 0, nc * sizeof(int32_t), s),

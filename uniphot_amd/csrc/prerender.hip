@@ -1,0 +1,806 @@
+// Member pre-render for the band-split island fit (gfx950, float32).
+//
+// Before each evaluation pass every real member of the active islands is
+// rendered and shifted here, and the evaluation launch only places the
+// window part of its planes.  Two launches per pass:
+//   * spectra: one workgroup per extended member computes, once for all
+//     bands, the Fourier mixture F + core, F1 = sum a v E and Fn + dcore on
+//     the (H = pad/2 + 1, pad) grid (column-major: frequency k along the
+//     stamp row, then m), and the ellipse-term derivatives;
+//   * planes: (member, band) items.  A star's raw planes are the PSF stamp
+//     P of the band at its position and dP/dx, dP/dy; an extended member's
+//     are the PSF-convolved mixture and the tangents the pass needs, from
+//     wave-level FFTs: rows of P -> LDS line buffer -> columns -> spectrum
+//     S (LDS for pad <= 128, per-workgroup global columns for pad 256); each
+//     output plane is one column pass over S times its spectrum into the LDS
+//     line buffer and one Hermitian row pass.  The raw planes are then
+//     shifted by the member's sub-pixel offset (separable Lanczos-3 with its
+//     tap derivatives) into the member's output planes.
+// The core term is folded into the spectra (irfft2(S (F + core)) =
+// irfft2(S F) + core P for the cropped stamp).
+// Algorithm followed: tract7dt/fit/grid_gpu/sersic_mog.py
+// (render_mog_stamp_fft :153-194, _ellipse_cov_terms :108-150),
+// source_model.py:264-328 (PSF at the source position) and
+// island_solver.py:441-562 (member stamps and their tangents).
+#include <hip/hip_runtime.h>
+
+#include "uph_capi_internal.hpp"
+#include "uph_device.hpp"
+#include "uph_galaxy.hpp"
+#include "uph_prerender.hpp"
+#include "uph_island.hpp"
+
+namespace uph {
+
+constexpr int kPreBlock = 512;  // threads of the planes kernel (8 waves)
+constexpr int kGalWaves = kPreBlock / 64;
+// LDS line buffer of the planes kernel (complex elements): holds the
+// (H, T) row/column buffer plus, for pad <= 128, the (H, pad) spectrum.
+constexpr int kGalLdsComplex = 14336;
+
+__host__ __device__ inline bool galaxy_fits(int T, int pad) {
+  const int H = pad / 2 + 1;
+  const int need = H * T + (pad <= 128 ? H * pad : 0);
+  return pad >= 64 && pad <= 256 && need <= kGalLdsComplex;
+}
+
+__host__ __device__ inline size_t round64(size_t v) { return (v + 63) / 64 * 64; }
+
+// Per-workgroup scratch of the planes kernel (floats, 64-aligned regions):
+// P, dP/dx, dP/dy | 7 raw planes | 7 horizontally shifted planes |
+// spectrum columns (pad 256 only).
+struct PreScratch {
+  size_t P3, R, Hh, S, total;
+};
+
+__host__ __device__ inline PreScratch pre_scratch(int T) {
+  const int pad = fft_pad_for(T), H = pad / 2 + 1;
+  const size_t TT = (size_t)T * T;
+  PreScratch o;
+  o.P3 = 0;
+  o.R = round64(3 * TT);
+  o.Hh = o.R + round64(7 * TT);
+  o.S = o.Hh + round64(7 * TT);
+  o.total = o.S + (pad > 128 ? (size_t)2 * H * pad : 0);
+  return o;
+}
+
+__host__ __device__ inline size_t spectra_floats(int T) {
+  const int pad = fft_pad_for(T);
+  return (size_t)3 * (pad / 2 + 1) * pad;
+}
+
+bool prerender_supports(int T) { return galaxy_fits(T, fft_pad_for(T)); }
+size_t prerender_scratch(int T) { return align256(pre_scratch(T).total * sizeof(float)); }
+size_t prerender_spectra_floats(int T) { return spectra_floats(T); }
+
+__device__ __forceinline__ int brev_bits(int i, int logn) {
+  return (int)(__brev((unsigned)i) >> (32 - logn));
+}
+
+template <int E>
+struct LogE {
+  static constexpr int v = E == 1 ? 0 : (E == 2 ? 1 : (E == 4 ? 2 : 3));
+};
+
+// Forward-transform twiddles of this lane for a line of 64 E points.
+template <int E>
+struct WaveTw {
+  float cw[6][2];  // cross-lane stages: this lane's scale (twiddle or 1)
+  float cs[6];     // and sign (+1 lower lane, -1 upper lane of its pair)
+  float iw[LogE<E>::v + 1][E][2];
+};
+
+template <int E>
+__device__ __forceinline__ void wave_tw(WaveTw<E>& t) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const int h = 1 << s;
+    const bool upper = lane & h;
+    float sn, cs;
+    sincospif(float(lane & (h - 1)) / float(h), &sn, &cs);
+    t.cw[s][0] = upper ? cs : 1.0f;
+    t.cw[s][1] = upper ? -sn : 0.0f;
+    t.cs[s] = upper ? -1.0f : 1.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < LogE<E>::v; ++q) {
+    const int he = 1 << q;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = lane + 64 * (e & (he - 1));
+      float sn, cs;
+      sincospif(float(j) / float(64 * he), &sn, &cs);
+      t.iw[q][e][0] = cs;
+      t.iw[q][e][1] = -sn;
+    }
+  }
+}
+
+// x of lane (l ^ H) for power-of-two H < 64 without the LDS crossbar:
+// DPP quad permutes and row (half) mirrors for H <= 8, a bit-mode swizzle
+// for 16, the 32-lane swap for 32.
+template <int H>
+__device__ __forceinline__ float lane_xor(float x) {
+  const int v = __float_as_int(x);
+  int r;
+  if constexpr (H == 1) {
+    r = __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad [1,0,3,2]
+  } else if constexpr (H == 2) {
+    r = __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad [2,3,0,1]
+  } else if constexpr (H == 4) {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // i ^ 7
+    r = __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);             // i ^ 3
+  } else if constexpr (H == 8) {
+    const int t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // i ^ 15
+    r = __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);            // i ^ 7
+  } else if constexpr (H == 16) {
+    r = __builtin_amdgcn_ds_swizzle(v, (0x10 << 10) | 0x1F);  // bit mode: xor 16
+  } else {
+    static_assert(H == 32, "lane_xor: H in 1..32");
+    const auto pr = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    r = (threadIdx.x & 32) ? int(pr[0]) : int(pr[1]);
+  }
+  return __int_as_float(r);
+}
+
+// One cross-lane radix-2 stage of span 2^S.  Each lane scales its own
+// element by w_l (the twiddle on the upper lane of a pair, 1 on the lower
+// lane), swaps the product with its partner, and keeps partner + sg * own
+// (sg = +1 lower, -1 upper): the lower lane ends with a + w b, the upper
+// with a - w b, for a the lower and b the upper element of the pair.
+template <int S, int E>
+__device__ __forceinline__ void lane_stage(float (&re)[E], float (&im)[E], const float wr,
+                                           const float wi, const float sg) {
+  constexpr int h = 1 << S;
+  float zr[E], zi[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    zr[e] = wr * re[e] - wi * im[e];
+    zi[e] = wr * im[e] + wi * re[e];
+  }
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const float qr = lane_xor<h>(zr[e]), qi = lane_xor<h>(zi[e]);
+    re[e] = fmaf(sg, zr[e], qr);
+    im[e] = fmaf(sg, zi[e], qi);
+  }
+}
+
+// In-register forward FFT of one line held by a wavefront: on entry lane l
+// holds the bit-reversed-order elements l + 64 e, on exit the natural-order
+// outputs l + 64 e.  Radix-2 decimation in time: spans 1..32 across lanes
+// (shuffles), spans 64.. inside the lane.
+template <int E>
+__device__ __forceinline__ void wave_fft(float (&re)[E], float (&im)[E], const WaveTw<E>& t) {
+  lane_stage<0, E>(re, im, t.cw[0][0], t.cw[0][1], t.cs[0]);
+  lane_stage<1, E>(re, im, t.cw[1][0], t.cw[1][1], t.cs[1]);
+  lane_stage<2, E>(re, im, t.cw[2][0], t.cw[2][1], t.cs[2]);
+  lane_stage<3, E>(re, im, t.cw[3][0], t.cw[3][1], t.cs[3]);
+  lane_stage<4, E>(re, im, t.cw[4][0], t.cw[4][1], t.cs[4]);
+  lane_stage<5, E>(re, im, t.cw[5][0], t.cw[5][1], t.cs[5]);
+#pragma unroll
+  for (int q = 0; q < LogE<E>::v; ++q) {
+    const int he = 1 << q;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (e & he) continue;
+      const int f = e + he;
+      const float wr = t.iw[q][e][0], wi = t.iw[q][e][1];
+      const float br = wr * re[f] - wi * im[f], bi = wr * im[f] + wi * re[f];
+      const float ar = re[e], ai = im[e];
+      re[e] = ar + br;
+      im[e] = ai + bi;
+      re[f] = ar - br;
+      im[f] = ai - bi;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Taps and spectra: one workgroup per pre-rendered member.  Every member
+// gets its Lanczos taps for the pass (band-independent: the sub-pixel
+// offset is taken against the island window origin, where the evaluation
+// kernel places the stamp at rint(x - x0) - T / 2); extended members also
+// get their Fourier spectra.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(UPH_BLOCK) void prerender_spectra_kernel(IslandArgsPre a) {
+  __shared__ GalParams<float> gp;
+  __shared__ int s_skip, s_T;
+  const int g = blockIdx.x, tid = threadIdx.x, mode = a.mode;
+  const float* psrc = mode == kSplitEvalInit ? a.p0 : (mode == kSplitEvalTry ? a.p_try : a.p_out);
+  const int isl = a.pre_isl[g];
+  if ((mode == kSplitEvalGrad || mode == kSplitEvalTry) && a.conv_out[isl]) return;  // uniform
+  const size_t slot = (size_t)a.mem_off[isl] + a.pre_slot[g];
+  const float* p6 = psrc + slot * 6;
+  if (tid == 0 || tid == 64) {
+    // x taps on wave 0, y taps on wave 1
+    const float d = tid == 0 ? p6[0] - a.x0b[isl] : p6[1] - a.y0b[isl];
+    float* t = a.gtap + (size_t)g * 28 + (tid == 0 ? 0 : 14);
+    lanczos_taps(d - rintf(d), t, t + 7);
+  }
+  if (a.pre_spec[g] < 0) return;  // star: no spectra (uniform)
+  if (tid == 0) {
+    int skip = 0;
+    const int32_t* sp = a.shape + 4 * (size_t)isl;
+    const int T = a.tier_T[sp[2]];
+    s_T = T;
+    if ((size_t)a.pre_spec[g] + spectra_floats(T) > a.fspec_elems) {
+      atomicOr(a.err, 1);
+      skip = 1;
+    }
+    if (!skip) {
+      double n = double(a.n_frozen[slot]);
+      if (a.free_n[slot]) {
+        const double u = double(p6[5]);
+        n = kNMin + (kNMax - kNMin) / (1.0 + exp(-u));
+      }
+      gp.n = n;
+      gp.K = int(a.sersic[3]);
+      mog_lookup(a.sersic, n, gp.a, gp.v, gp.core, gp.da, gp.dv, gp.dcore);
+      ellipse_terms(double(p6[2]), double(p6[3]), double(p6[4]), gp.t, gp.dt);
+      for (int i = 0; i < 9; ++i) a.gdt[(size_t)g * 9 + i] = gp.dt[i / 3][i % 3];
+    }
+    s_skip = skip;
+  }
+  __syncthreads();
+  if (s_skip) return;
+  const int T = s_T, pad = fft_pad_for(T), H = pad / 2 + 1, n = H * pad;
+  const bool grads = mode == kSplitEvalGrad;
+  float* F = a.fspec + a.pre_spec[g];
+  const double c2 = -2.0 * 9.869604401089358;  // -2 pi^2
+  for (int e = tid; e < n; e += UPH_BLOCK) {
+    const int k = e / pad, m = e % pad;
+    const double v = double(k) / pad;
+    const double w = double(m < pad / 2 ? m : m - pad) / pad;
+    const double ex = c2 * (gp.t[0] * v * v + gp.t[1] * w * w + 2.0 * gp.t[2] * v * w);
+    double f = 0.0, f1 = 0.0, fn = 0.0;
+    for (int j = 0; j < gp.K; ++j) {
+      const double E = exp(gp.v[j] * ex);
+      f += gp.a[j] * E;
+      f1 += gp.a[j] * gp.v[j] * E;
+      fn += gp.da[j] * E + gp.a[j] * E * ex * gp.dv[j];
+    }
+    F[e] = float(f + gp.core);
+    if (grads) {
+      F[n + e] = float(f1);
+      F[2 * n + e] = float(fn + gp.dcore);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Planes: (member, band) items.
+// ---------------------------------------------------------------------------
+struct PlaneJob {
+  int g, b, T, pad, logp, H, need;
+  const float* coefs;
+  const float* p6;
+  float* out;     // the item's 7 raw planes (T^2 each)
+  float* P3;      // per-workgroup P, dP/dx, dP/dy (global scratch)
+  float2* S;      // spectrum (H, pad): LDS or global scratch
+  float2* A;      // LDS line buffer (H, T)
+  const float* F; // the galaxy's spectra
+  const double* dt;
+};
+
+// Lines of one pass handled together by a wavefront (G E = 8 complex values
+// in flight per lane): the gathers of G lines are issued before any of
+// their transforms, so one memory latency covers the batch.
+template <int E>
+struct Batch {
+  static constexpr int G = 8 / E;
+};
+
+// Forward transform of the T x T real plane src into S (H columns of pad).
+template <int E>
+__device__ void forward_plane(const PlaneJob& j, const float* src, const WaveTw<E>& tw) {
+  constexpr int G = Batch<E>::G;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int T = j.T, H = j.H, pad = j.pad, logp = j.logp;
+  // rows: A[k][r] = sum_c P[r][c] e^{-2 pi i k c / pad}, k < H
+  for (int r0 = wid * G; r0 < T; r0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = brev_bits(lane + 64 * e, logp);
+        re[g][e] = (r < T && c < T) ? src[r * T + c] : 0.0f;
+        im[g][e] = 0.0f;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (r0 + g < T) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+      if (r >= T) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int k = lane + 64 * e;
+        if (k < H) j.A[k * T + r] = make_float2(re[g][e], im[g][e]);
+      }
+    }
+  }
+  __syncthreads();
+  // columns: S[k][m] = sum_r A[k][r] e^{-2 pi i m r / pad}
+  for (int k0 = wid * G; k0 < H; k0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int r = brev_bits(lane + 64 * e, logp);
+        const float2 v = (k < H && r < T) ? j.A[k * T + r] : make_float2(0.0f, 0.0f);
+        re[g][e] = v.x;
+        im[g][e] = v.y;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (k0 + g < H) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      if (k >= H) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        j.S[(size_t)k * pad + lane + 64 * e] = make_float2(re[g][e], im[g][e]);
+    }
+  }
+  __syncthreads();
+}
+
+// Inverse transform of S times the spectrum of output kind `kind`
+// (0: F + core, 1..3: F1 d(expo)/d(logre, ee1, ee2), 4: Fn + dcore) into
+// the T x T plane dst: dst = irfft2(S X, (pad, pad))[:T, :T].
+template <int E>
+__device__ void inverse_plane(const PlaneJob& j, int kind, float* dst, const WaveTw<E>& tw) {
+  constexpr int G = Batch<E>::G;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int T = j.T, H = j.H, pad = j.pad, logp = j.logp, n = H * pad;
+  const float* X = j.F + (kind == 0 ? 0 : (kind == 4 ? 2 * n : n));
+  const double c2 = -2.0 * 9.869604401089358;
+  const int p = kind - 1;
+  // columns: Y[k][r] = sum_m S[k][m] X[k][m] e^{+2 pi i m r / pad}, r < T
+  for (int k0 = wid * G; k0 < H; k0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      const double v = double(k) / pad;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int m = brev_bits(lane + 64 * e, logp);
+        float2 sv = make_float2(0.0f, 0.0f);
+        float x = 0.0f;
+        if (k < H) {
+          const size_t o = (size_t)k * pad + m;
+          sv = j.S[o];
+          x = X[o];
+          if (kind >= 1 && kind <= 3) {
+            const double w = double(m < pad / 2 ? m : m - pad) / pad;
+            const double dex = c2 * (j.dt[0 * 3 + p] * v * v + j.dt[1 * 3 + p] * w * w +
+                                     2.0 * j.dt[2 * 3 + p] * v * w);
+            x = float(double(x) * dex);
+          }
+        }
+        // inverse via conj(FFT(conj(z)))
+        re[g][e] = sv.x * x;
+        im[g][e] = -(sv.y * x);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (k0 + g < H) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      if (k >= H) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int r = lane + 64 * e;
+        if (r < T) j.A[k * T + r] = make_float2(re[g][e], -im[g][e]);
+      }
+    }
+  }
+  __syncthreads();
+  // rows: out[r][c] = Re sum_k X_r[k] e^{+2 pi i k c / pad} / pad^2 with
+  // the Hermitian extension X_r[k] = Y[k][r] (k < H), conj(Y[pad - k][r])
+  const float norm = 1.0f / (float(pad) * float(pad));
+  for (int r0 = wid * G; r0 < T; r0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = brev_bits(lane + 64 * e, logp);
+        float2 y = make_float2(0.0f, 0.0f);
+        if (r < T) {
+          if (c < H) {
+            y = j.A[c * T + r];
+          } else {
+            y = j.A[(pad - c) * T + r];
+            y.y = -y.y;
+          }
+        }
+        re[g][e] = y.x;
+        im[g][e] = -y.y;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (r0 + g < T) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+      if (r >= T) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = lane + 64 * e;
+        if (c < T) dst[r * T + c] = re[g][e] * norm;
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <int E>
+__device__ void render_planes(const PlaneJob& j) {
+  WaveTw<E> tw;
+  wave_tw<E>(tw);
+  const int TT = j.T * j.T, need = j.need;
+  // source P: model (plane 0), shape tangents (planes 3..5), n (plane 6)
+  forward_plane<E>(j, j.P3, tw);
+  inverse_plane<E>(j, 0, j.out, tw);
+  for (int q = 0; q < 3; ++q)
+    if (need & (4 << q)) inverse_plane<E>(j, 1 + q, j.out + (size_t)(3 + q) * TT, tw);
+  if (need & 32) inverse_plane<E>(j, 4, j.out + (size_t)6 * TT, tw);
+  // sources dP/dx, dP/dy: position tangents (planes 1, 2)
+  for (int q = 0; q < 2; ++q) {
+    if (!(need & (1 << q))) continue;
+    forward_plane<E>(j, j.P3 + (size_t)(1 + q) * TT, tw);
+    inverse_plane<E>(j, 0, j.out + (size_t)(1 + q) * TT, tw);
+  }
+}
+
+// Part of a member's T x T stamp that lands in its island window: the
+// window-covered box [sx0, sx1) x [sy0, sy1) in stamp coordinates, and the
+// box [rx0, rx1) x [ry0, ry1) of raw-plane pixels the 7-tap shift reads
+// for it (the window box grown by 3 and clipped to the stamp).
+struct StampClip {
+  int sx0, sx1, sy0, sy1, rx0, rx1, ry0, ry1;
+};
+
+// Separable Lanczos-3 shift of the raw planes R (plane 0 the stamp, 1 and 2
+// its x and y raw tangents, 3..6 the shape and n tangents) into out:
+// horizontal pass into Hh, vertical pass into out, over the window box of
+// the stamp (raw inputs outside [0, T) are zero; only the box c.r* is read).
+// Plane 1 of out gets the x tap derivative applied to plane 0, plane 2 the y
+// tap derivative; plane 6 is scaled by dn/du.  Planes of tangents not in
+// `need`, and pixels outside the window box, are not written.
+template <int kThreads>
+__device__ void shift_planes(const float* R, float* Hh, float* out, int T, int need,
+                             const float* taps, float dndu, const StampClip& c) {
+  const int tid = threadIdx.x, TT = T * T;
+  float Lx[7], dLx[7], Ly[7], dLy[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    Lx[k] = taps[k];
+    dLx[k] = taps[7 + k];
+    Ly[k] = taps[14 + k];
+    dLy[k] = taps[21 + k];
+  }
+  const int SW = c.sx1 - c.sx0, RH = c.ry1 - c.ry0, SH = c.sy1 - c.sy0;
+  for (int q = tid; q < RH * SW; q += kThreads) {
+    const int rr = c.ry0 + q / SW, cx = c.sx0 + q % SW, e = rr * T + cx;
+    float h[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int cc = cx + k - 3;
+      if (cc < c.rx0 || cc >= c.rx1) continue;
+      const int o = rr * T + cc;
+      h[0] = fmaf(Lx[k], R[o], h[0]);
+      if (need & 1) h[1] = fmaf(Lx[k], R[TT + o], fmaf(dLx[k], R[o], h[1]));
+#pragma unroll
+      for (int j = 1; j < 6; ++j)
+        if (need & (1 << j)) h[1 + j] = fmaf(Lx[k], R[(size_t)(1 + j) * TT + o], h[1 + j]);
+    }
+    Hh[e] = h[0];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      if (need & (1 << j)) Hh[(size_t)(1 + j) * TT + e] = h[1 + j];
+  }
+  __syncthreads();
+  for (int q = tid; q < SH * SW; q += kThreads) {
+    const int rw = c.sy0 + q / SW, cc = c.sx0 + q % SW, e = rw * T + cc;
+    float v[7] = {0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int rr = rw + k - 3;
+      if (rr < c.ry0 || rr >= c.ry1) continue;
+      const int o = rr * T + cc;
+      const float h0 = Hh[o];
+      v[0] = fmaf(Ly[k], h0, v[0]);
+      if (need & 2) v[2] = fmaf(Ly[k], Hh[(size_t)2 * TT + o], fmaf(dLy[k], h0, v[2]));
+#pragma unroll
+      for (int j = 0; j < 6; ++j)
+        if (j != 1 && (need & (1 << j))) v[1 + j] = fmaf(Ly[k], Hh[(size_t)(1 + j) * TT + o], v[1 + j]);
+    }
+    out[e] = v[0];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      if (need & (1 << j)) out[(size_t)(1 + j) * TT + e] = j == 5 ? v[6] * dndu : v[1 + j];
+  }
+  __syncthreads();
+}
+
+// Setup shared by the planes kernels, from any one thread: the item's
+// member, stamp tier, clip box and validity.  Returns false when the item
+// has nothing to render (converged island, stamp outside the window, or a
+// buffer overflow, which also raises the error flag).
+struct PreItem {
+  int g, b, isl, T, need;
+  size_t slot;
+  StampClip clip;
+};
+
+__device__ __forceinline__ bool pre_item(const IslandArgsPre& a, int item, bool star,
+                                         const float* psrc, PreItem& it) {
+  const int nb = a.nb;
+  it.g = item / nb;
+  it.b = item % nb;
+  it.isl = a.pre_isl[it.g];
+  const int mode = a.mode;
+  if ((mode == kSplitEvalGrad || mode == kSplitEvalTry) && a.conv_out[it.isl]) return false;
+  const int32_t* sp = a.shape + 4 * (size_t)it.isl;
+  const int T = a.tier_T[sp[2]], B = sp[1];
+  it.T = T;
+  it.slot = (size_t)a.mem_off[it.isl] + a.pre_slot[it.g];
+  int need = 0;
+  if (mode == kSplitEvalGrad)
+    for (int q = 0; q < 6; ++q)
+      if (a.keep[it.slot * 6 + q]) need |= 1 << q;
+  it.need = star ? need & 3 : need;
+  const float* p6 = psrc + it.slot * 6;
+  const float xg = p6[0] - a.x0b[it.isl], yg = p6[1] - a.y0b[it.isl];
+  const int ix = int(rintf(xg)) - T / 2, iy = int(rintf(yg)) - T / 2;
+  StampClip& c = it.clip;
+  c.sx0 = max(0, ix) - ix;
+  c.sx1 = min(B, ix + T) - ix;
+  c.sy0 = max(0, iy) - iy;
+  c.sy1 = min(B, iy + T) - iy;
+  c.rx0 = max(0, c.sx0 - 3);
+  c.rx1 = min(T, c.sx1 + 3);
+  c.ry0 = max(0, c.sy0 - 3);
+  c.ry1 = min(T, c.sy1 + 3);
+  // a stamp entirely outside the window is never placed
+  return c.sx1 > c.sx0 && c.sy1 > c.sy0;
+}
+
+__device__ __forceinline__ bool pre_item_fits(const IslandArgsPre& a, const PreItem& it,
+                                              bool star) {
+  const int T = it.T;
+  // the launch's member kind must be the member's own (its spectra offset
+  // and plane count follow from it)
+  return (a.stype[it.slot] == 0) == star && (a.pre_spec[it.g] < 0) == star &&
+         (star || galaxy_fits(T, fft_pad_for(T))) &&
+         pre_scratch(T).total * sizeof(float) <= a.scratch_per_block &&
+         (size_t)a.pre_off[it.g] + (size_t)a.nb * pre_planes(star) * T * T <= a.pbuf_elems &&
+         (star || (size_t)a.pre_spec[it.g] + spectra_floats(T) <= a.fspec_elems);
+}
+
+// PSF stamp of band b at the member position, and d/dx, d/dy when `gxy`,
+// over the box [x0, x0 + W) x [y0, y0 + H) of the T x T stamp into
+// dst[0], dst[T^2], dst[2 T^2] (row stride T).
+template <int kThreads>
+__device__ __forceinline__ void psf_box(const float* Cf, int nt, int T, const float* bas, float xs,
+                                        float ys, bool gxy, int x0, int y0, int W, int H,
+                                        float* dst) {
+  const int TT = T * T;
+  for (int q = threadIdx.x; q < W * H; q += kThreads) {
+    const int e = (y0 + q / W) * T + x0 + q % W;
+    float v = 0.0f, vx = 0.0f, vy = 0.0f;
+    // coefficient loads in groups of 8 terms, all issued before their FMAs
+    for (int t0 = 0; t0 < nt; t0 += 8) {
+      float cv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) cv[u] = t0 + u < nt ? Cf[(size_t)(t0 + u) * TT + e] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (t0 + u >= nt) break;
+        v = fmaf(cv[u], bas[t0 + u], v);
+        if (gxy) {
+          vx = fmaf(cv[u], bas[21 + t0 + u], vx);
+          vy = fmaf(cv[u], bas[42 + t0 + u], vy);
+        }
+      }
+    }
+    dst[e] = v;
+    if (gxy) {
+      dst[TT + e] = vx / xs;
+      dst[2 * TT + e] = vy / ys;
+    }
+  }
+}
+
+// Extended members: (member, band) items [0, n_pre_ext * nb) from a work
+// queue; the whole stamp goes through the Fourier mixture in LDS.
+__global__ __launch_bounds__(kPreBlock) void prerender_planes_kernel(IslandArgsPre a) {
+  __shared__ float2 lds[kGalLdsComplex];
+  __shared__ float bas[63];
+  __shared__ float taps[28];
+  __shared__ int s_item, s_ok;
+  __shared__ PreItem s_it;
+  __shared__ float s_dndu, s_xs, s_ys;
+  const int tid = threadIdx.x, mode = a.mode, nb = a.nb, nt = a.nt;
+  float* base = reinterpret_cast<float*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
+  const int n_items = a.n_pre_ext * nb;
+  const float* psrc = mode == kSplitEvalInit ? a.p0 : (mode == kSplitEvalTry ? a.p_try : a.p_out);
+  for (;;) {
+    if (tid == 0) {
+      const int item = atomicAdd(a.pre_counter, 1);
+      s_item = item;
+      if (item < n_items) {
+        PreItem it;
+        int ok = pre_item(a, item, false, psrc, it);
+        if (ok && !pre_item_fits(a, it, false)) {
+          atomicOr(a.err, 1);
+          ok = 0;
+        }
+        s_it = it;
+        s_ok = ok;
+        if (ok) {
+          const float* geo = a.psf_geo + 4 * it.b;
+          const float* p6 = psrc + it.slot * 6;
+          const float X = (p6[0] - geo[0]) / geo[1], Y = (p6[1] - geo[2]) / geo[3];
+          psf_basis(X, Y, nt, bas, bas + 21, bas + 42);
+          s_xs = geo[1];
+          s_ys = geo[3];
+          float dndu = 0.0f;
+          if (a.free_n[it.slot]) {
+            const double u = double(p6[5]);
+            const double n = kNMin + (kNMax - kNMin) / (1.0 + exp(-u));
+            dndu = float((n - kNMin) * (kNMax - n) / (kNMax - kNMin));
+          }
+          s_dndu = dndu;
+        }
+      }
+    }
+    __syncthreads();
+    const int item = s_item;
+    if (item >= n_items) break;
+    if (!s_ok) {
+      __syncthreads();
+      continue;
+    }
+    const int g = s_it.g, b = s_it.b, T = s_it.T, need = s_it.need;
+    const size_t slot = s_it.slot;
+    if (tid < 28) taps[tid] = a.gtap[(size_t)g * 28 + tid];
+    const PreScratch L = pre_scratch(T);
+    float* P3 = base + L.P3;
+    float* R = base + L.R;
+    psf_box<kPreBlock>(a.tier_coefs[a.shape[4 * (size_t)s_it.isl + 2]] + (size_t)b * nt * T * T,
+                       nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
+    __syncthreads();
+    const int pad = fft_pad_for(T), H = pad / 2 + 1;
+    PlaneJob j;
+    j.g = g;
+    j.b = b;
+    j.T = T;
+    j.pad = pad;
+    j.logp = ilog2(pad);
+    j.H = H;
+    j.need = need;
+    j.coefs = nullptr;
+    j.p6 = psrc + slot * 6;
+    j.out = R;
+    j.P3 = P3;
+    j.A = lds;
+    j.S = pad <= 128 ? lds + H * T : reinterpret_cast<float2*>(base + L.S);
+    j.F = a.fspec + a.pre_spec[g];
+    j.dt = a.gdt + (size_t)g * 9;
+    if (pad == 64)
+      render_planes<1>(j);
+    else if (pad == 128)
+      render_planes<2>(j);
+    else
+      render_planes<4>(j);
+    // the item fields are read from LDS again here, not held across the
+    // transforms
+    {
+      const int g2 = s_it.g, T2 = s_it.T;
+      const PreScratch L2 = pre_scratch(T2);
+      shift_planes<kPreBlock>(base + L2.R, base + L2.Hh,
+                              a.pbuf + a.pre_off[g2] + (size_t)s_it.b * 7 * T2 * T2, T2,
+                              s_it.need, taps, s_dndu, s_it.clip);
+    }
+  }
+}
+
+// Stars: (member, band) items [n_pre_ext * nb, n_pre * nb), strided over
+// the grid.  A star needs only the PSF pixels the shift reads for the
+// window box, and no LDS beyond its basis and taps, so several workgroups
+// share a CU.
+constexpr int kStarBlock = 256;
+
+__global__ __launch_bounds__(kStarBlock) void prerender_stars_kernel(IslandArgsPre a) {
+  __shared__ float bas[63];
+  __shared__ float taps[28];
+  const int tid = threadIdx.x, mode = a.mode, nb = a.nb, nt = a.nt;
+  float* base = reinterpret_cast<float*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
+  const int i0 = a.n_pre_ext * nb, i1 = a.n_pre * nb;
+  const float* psrc = mode == kSplitEvalInit ? a.p0 : (mode == kSplitEvalTry ? a.p_try : a.p_out);
+  for (int item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
+    // every thread derives the (uniform) item setup itself
+    PreItem it;
+    if (!pre_item(a, item, true, psrc, it)) continue;
+    if (!pre_item_fits(a, it, true)) {
+      if (tid == 0) atomicOr(a.err, 1);
+      continue;
+    }
+    const int T = it.T, b = it.b;
+    const float* geo = a.psf_geo + 4 * b;
+    const float* p6 = psrc + it.slot * 6;
+    if (tid < nt) {
+      // basis term t = tid of the spatial polynomial, as psf_basis forms it
+      const float X = (p6[0] - geo[0]) / geo[1], Y = (p6[1] - geo[2]) / geo[3];
+      const int m = kTermM[tid], n = kTermN[tid];
+      float px[6], py[6];
+      px[0] = 1.0f;
+      py[0] = 1.0f;
+#pragma unroll
+      for (int i = 1; i < 6; ++i) {
+        px[i] = px[i - 1] * X;
+        py[i] = py[i - 1] * Y;
+      }
+      bas[tid] = px[m] * py[n];
+      bas[21 + tid] = m > 0 ? float(m) * px[m - 1] * py[n] : 0.0f;
+      bas[42 + tid] = n > 0 ? float(n) * px[m] * py[n - 1] : 0.0f;
+    } else if (tid >= 64 && tid < 64 + 28) {
+      taps[tid - 64] = a.gtap[(size_t)it.g * 28 + tid - 64];
+    }
+    __syncthreads();
+    const PreScratch L = pre_scratch(T);
+    float* R = base + L.R;
+    const StampClip& c = it.clip;
+    psf_box<kStarBlock>(a.tier_coefs[a.shape[4 * (size_t)it.isl + 2]] + (size_t)b * nt * T * T,
+                        nt, T, bas, geo[1], geo[3], it.need & 3, c.rx0, c.ry0, c.rx1 - c.rx0,
+                        c.ry1 - c.ry0, R);
+    __syncthreads();
+    shift_planes<kStarBlock>(R, base + L.Hh, a.pbuf + a.pre_off[it.g] + (size_t)b * 3 * T * T,
+                             T, it.need, taps, 0.0f, c);
+  }
+}
+
+void prerender_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hipStream_t s,
+                    hipEvent_t ev_spec, hipEvent_t ev_ext) {
+  IslandArgsPre p = a;
+  p.mode = mode;
+  p.pre_counter = counter;
+  if (p.n_pre > 0)
+    hipLaunchKernelGGL(prerender_spectra_kernel, dim3(p.n_pre), dim3(UPH_BLOCK), 0, s, p);
+  if (ev_spec) (void)hipEventRecord(ev_spec, s);
+  const long long n_ext = (long long)p.n_pre_ext * p.nb;
+  const int g_ext = (int)(n_ext < nblk ? n_ext : nblk);
+  if (g_ext > 0) hipLaunchKernelGGL(prerender_planes_kernel, dim3(g_ext), dim3(kPreBlock), 0, s, p);
+  if (ev_ext) (void)hipEventRecord(ev_ext, s);
+  const long long n_star = (long long)(p.n_pre - p.n_pre_ext) * p.nb;
+  const int g_star = (int)(n_star < nblk ? n_star : nblk);
+  if (g_star > 0)
+    hipLaunchKernelGGL(prerender_stars_kernel, dim3(g_star), dim3(kStarBlock), 0, s, p);
+}
+
+}  // namespace uph
+
+extern "C" size_t uph_prerender_scratch_bytes(int T) {
+  return uph::prerender_supports(T) ? uph::prerender_scratch(T) : 0;
+}
+
+extern "C" size_t uph_prerender_spectra_floats(int T) { return uph::spectra_floats(T); }

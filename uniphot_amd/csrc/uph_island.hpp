@@ -63,24 +63,31 @@ struct IslandArgs {
   int* err;                // set to 1 when an island's band blocks overflow hg
 };
 
-// Galaxy pre-render (band-split path, float32): raw planes of every
-// extended member, n_bands x 7 x T^2 floats per galaxy, rendered before each
-// evaluation pass by the galaxy kernels and read by the evaluation kernel
-// instead of rendering in place.  Only the pre-render launches see these
-// fields; every other kernel takes the base IslandArgs.
+// Member pre-render (band-split path, float32): before each evaluation pass
+// every real member of the active islands is rendered by (member, band)
+// items -- stars from the PSF polynomial, extended members through the
+// Fourier mixture -- and shifted to its sub-pixel position, into
+// n_bands x np x T^2 floats per member (the model, then the tangents of x,
+// y and, for an extended member (np = 7), logre, ee1, ee2, n; np = 3 for a
+// star); only the part that lands in the island window is written, and the
+// evaluation kernel only places that part.
+// Only the pre-render launches see these fields; every other kernel takes
+// the base IslandArgs.
 struct IslandArgsPre : IslandArgs {
-  int n_gal;
-  const int32_t* gal_isl;      // (n_gal) island of the galaxy
-  const int32_t* gal_slot;     // (n_gal) member slot within the island
-  const int64_t* gal_off;      // (n_gal) first float of the galaxy's planes in gbuf
-  const int64_t* gal_spec;     // (n_gal) first float of its Fourier spectra in fspec
-  const int32_t* gal_of_slot;  // (total member slots) galaxy index, -1 for stars
-  void* gbuf;                  // raw planes (NULL = render galaxies in place)
-  size_t gbuf_elems;
-  float* fspec;                // per galaxy: F + core, F1, Fn + dcore on the (H, pad) grid
+  int n_pre;
+  int n_pre_ext;               // members [0, n_pre_ext) are extended, the rest stars
+  const int32_t* pre_isl;      // (n_pre) island of the member
+  const int32_t* pre_slot;     // (n_pre) member slot within the island
+  const int64_t* pre_off;      // (n_pre) first float of the member's planes in pbuf
+  const int64_t* pre_spec;     // (n_pre) first float of its Fourier spectra in fspec, -1 for stars
+  const int32_t* pre_of_slot;  // (total member slots) pre-render index, -1 for padding
+  float* pbuf;                 // shifted planes
+  size_t pbuf_elems;
+  float* fspec;                // per extended member: F + core, F1, Fn + dcore on the (H, pad) grid
   size_t fspec_elems;
-  double* gdt;                 // (n_gal, 9) ellipse-term derivatives of each galaxy
-  int* gal_counter;            // work queue of the planes kernel
+  double* gdt;                 // (n_pre, 9) ellipse-term derivatives of extended members
+  float* gtap;                 // (n_pre, 28) Lanczos taps and tap derivatives, x then y
+  int* pre_counter;            // work queue of the planes kernel
 };
 
 // Band-split phases.  Evaluation launches run one (island, band) item per
@@ -92,6 +99,9 @@ constexpr int kSplitEvalFinal = 3;  // fluxes, flux errors, good-pixel count at 
 constexpr int kSplitStepInit = 4;   // LM state from the initial chi2
 constexpr int kSplitStepSolve = 5;  // damped solve -> p_try
 constexpr int kSplitStepAccept = 6; // accept / reject, convergence
+
+// Planes per band of a pre-rendered member.
+__host__ __device__ inline int pre_planes(bool star) { return star ? 3 : 7; }
 
 // Doubles of one band block of an island with n_live_max = nLc:
 // H (nL x nL, row-major with the actual nL), g (nL), chi2, one spare.

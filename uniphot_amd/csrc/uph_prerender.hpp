@@ -1,0 +1,27 @@
+// Member pre-render of the band-split island fit (float32): entry points of
+// prerender.hip used by the island launcher.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "uph_island.hpp"
+
+namespace uph {
+
+// Whether the pre-render serves stamp tier T (its line buffer must fit LDS).
+bool prerender_supports(int T);
+
+// Per-workgroup scratch bytes the planes kernel needs for stamp tier T.
+size_t prerender_scratch(int T);
+
+// Floats of one extended member's Fourier spectra for tier T.
+size_t prerender_spectra_floats(int T);
+
+// Render and shift the planes of every pre-rendered member for one
+// evaluation pass (`mode` is a kSplitEval* phase): the taps/spectra launch
+// (one workgroup per member), the extended-member (member, band) launch on
+// its work queue `counter`, then the star (member, band) launch; `ev_spec`
+// and `ev_ext`, when given, are recorded after the first and second.
+void prerender_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hipStream_t s,
+                    hipEvent_t ev_spec, hipEvent_t ev_ext);
+
+}  // namespace uph

@@ -172,46 +172,68 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
     return res
 
 
-def _galaxy_prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prerender,
-                             dev, up):
-    """Galaxy index arrays and buffers of the band-split pre-render: every
-    real extended member, in the islands' work-queue order."""
+def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prerender, dev, up):
+    """Index arrays and buffers of the band-split member pre-render: every
+    real member, extended members first, each kind in the islands'
+    work-queue order; extended members also get a block of Fourier
+    spectra.  Planes per band: 7 for an extended member, 3 for a star."""
     import torch
     L = _lib.lib()
     P = _lib.ptr
-    empty = dict(args=(0, None, None, None, None, None, None, 0, None, 0, None),
-                 isl=np.zeros(0, np.int64), T=np.zeros(0, np.int64),
-                 spec_floats=np.zeros(0, np.float64))
+    empty = dict(args=(0, 0, *([None] * 6), 0, None, 0, None, None),
+                 isl=np.zeros(0, np.int64), T=np.zeros(0, np.int64), star=np.zeros(0, bool),
+                 spec_floats=np.zeros(0, np.float64), n_ext=0)
     if not prerender or precision == "fp64":
         return empty
     st_all, mm_all = cat["st"], cat["mm"].astype(bool)
-    gal_isl, gal_slot = [], []
-    slot_gal = np.full(m_acc, -1, np.int32)
+    ext, stars = [], []
     for i in order:
         m0, N = moff[i], shape[i][0]
-        for k in np.nonzero(mm_all[m0:m0 + N] & (st_all[m0:m0 + N] != 0))[0]:
-            slot_gal[m0 + k] = len(gal_isl)
-            gal_isl.append(int(i))
-            gal_slot.append(int(k))
-    if not gal_isl:
+        for k in np.nonzero(mm_all[m0:m0 + N])[0]:
+            (stars if st_all[m0 + k] == 0 else ext).append((int(i), int(k)))
+    members = ext + stars
+    if not members:
         return empty
-    gi = np.asarray(gal_isl, np.int64)
-    T = np.asarray([tiers[shape[i][2]] for i in gal_isl], np.int64)
-    if any(L.uph_galaxy_prerender_scratch_bytes(int(t)) == 0 for t in set(T.tolist())):
+    pre_isl = [i for i, _ in members]
+    pre_slot = [k for _, k in members]
+    slot_pre = np.full(m_acc, -1, np.int32)
+    for g, (i, k) in enumerate(members):
+        slot_pre[moff[i] + k] = g
+    gi = np.asarray(pre_isl, np.int64)
+    T = np.asarray([tiers[shape[i][2]] for i in pre_isl], np.int64)
+    if any(L.uph_prerender_scratch_bytes(int(t)) == 0 for t in set(T.tolist())):
         return empty
-    psz = nb * 7 * T * T
-    ssz = np.asarray([L.uph_galaxy_spectra_floats(int(t)) for t in T], np.int64)
+    star = np.arange(len(members)) >= len(ext)
+    psz = nb * np.where(star, 3, 7) * T * T
+    ssz = np.where(star, 0, [L.uph_prerender_spectra_floats(int(t)) for t in T]).astype(np.int64)
     poff = np.concatenate([[0], np.cumsum(psz)[:-1]]).astype(np.int64)
     soff = np.concatenate([[0], np.cumsum(ssz)[:-1]]).astype(np.int64)
-    gbuf = torch.empty(int(psz.sum()), dtype=torch.float32, device=dev)
-    fspec = torch.empty(int(ssz.sum()), dtype=torch.float32, device=dev)
-    gdt = torch.empty(len(gal_isl) * 9, dtype=torch.float64, device=dev)
-    keep = [up(np.asarray(gal_isl, np.int32), np.int32), up(np.asarray(gal_slot, np.int32), np.int32),
-            up(poff, np.int64), up(soff, np.int64), up(slot_gal, np.int32)]
-    args = (len(gal_isl), *(P(t) for t in keep), P(gbuf), int(gbuf.numel()), P(fspec),
-            int(fspec.numel()), P(gdt))
-    return dict(args=args, keepalive=(keep, gbuf, fspec, gdt), isl=gi, T=T,
-                spec_floats=ssz.astype(np.float64))
+    soff = np.where(star, -1, soff).astype(np.int64)
+    pbuf = torch.empty(int(psz.sum()), dtype=torch.float32, device=dev)
+    fspec = torch.empty(max(int(ssz.sum()), 1), dtype=torch.float32, device=dev)
+    gdt = torch.empty(len(members) * 9, dtype=torch.float64, device=dev)
+    gtap = torch.empty(len(members) * 28, dtype=torch.float32, device=dev)
+    keep = [up(np.asarray(pre_isl, np.int32), np.int32), up(np.asarray(pre_slot, np.int32), np.int32),
+            up(poff, np.int64), up(soff, np.int64), up(slot_pre, np.int32)]
+    args = (len(members), len(ext), *(P(t) for t in keep), P(pbuf), int(pbuf.numel()), P(fspec),
+            int(fspec.numel()), P(gdt), P(gtap))
+    # window part and shift-read box of each stamp at the initial position
+    # (positions move by well under a pixel), for the bytes accounting
+    slot = np.asarray(moff, np.int64)[gi] + np.asarray(pre_slot, np.int64)
+    Bw = np.asarray([shape[i][1] for i in pre_isl], np.int64)
+    p0 = cat["p0"].reshape(-1, 6)[slot]
+    ix = np.rint(p0[:, 0].astype(np.float32) - cat["x0b"][gi]).astype(np.int64) - T // 2
+    iy = np.rint(p0[:, 1].astype(np.float32) - cat["y0b"][gi]).astype(np.int64) - T // 2
+    sw = np.clip(np.minimum(Bw, ix + T) - np.maximum(0, ix), 0, None)
+    sh = np.clip(np.minimum(Bw, iy + T) - np.maximum(0, iy), 0, None)
+    rw = np.minimum(T, np.minimum(Bw, ix + T) - ix + 3) - np.maximum(0, np.maximum(0, ix) - ix - 3)
+    rh = np.minimum(T, np.minimum(Bw, iy + T) - iy + 3) - np.maximum(0, np.maximum(0, iy) - iy - 3)
+    live_xy = cat["keep"].reshape(-1, 6)[slot][:, :2].sum(axis=1)
+    live = cat["keep"].reshape(-1, 6)[slot].sum(axis=1)
+    return dict(args=args, keepalive=(keep, pbuf, fspec, gdt, gtap), isl=gi, T=T, star=star,
+                spec_floats=ssz.astype(np.float64), n_ext=len(ext),
+                win=(sw * sh).astype(np.float64), reg=np.clip(rw * rh, 0, None).astype(np.float64),
+                grad_planes=np.where(star, 1 + live_xy, 1 + live).astype(np.float64))
 
 
 def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
@@ -227,10 +249,10 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
     ``split=True`` runs the band-split form (``uph_island_fit_split``):
     every evaluation pass is spread over (island, band) items, so one large
     island no longer sets the wall time; results are bit-identical.  With
-    ``prerender`` (float32 only) the extended members' raw planes are
-    rendered by separate (galaxy, band) launches before each pass (LDS
-    wave-level FFTs) instead of inside the evaluation kernel; results then
-    agree with the in-place render to float32 rounding.
+    ``prerender`` (float32 only) every member is rendered and shifted by
+    separate (member, band) launches before each pass (extended members
+    through LDS wave-level FFTs) and the evaluation only places the planes;
+    results then agree with the in-place render to float32 rounding.
     """
     import torch
     t0 = time.perf_counter()
@@ -285,9 +307,9 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         cat["hi"].append(host["upper"][rows].ravel())
         per = max(per, L.uph_island_scratch_per_block(N, B, kern.T, kern.n_live_max, prec))
     n_isl = len(shape)
-    if split and prerender and precision != "fp64" and any(k.kind != "star" for k, _, _ in classes):
+    if split and prerender and precision != "fp64":
         for T in tiers:
-            need = int(L.uph_galaxy_prerender_scratch_bytes(int(T)))
+            need = int(L.uph_prerender_scratch_bytes(int(T)))
             if need == 0:
                 prerender = False  # a stamp tier the pre-render does not serve
             per = max(per, need)
@@ -359,12 +381,12 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         ev_arr = None
         if timer is not None:
             # per-pass timing events (created by a first record)
-            evs = [torch.cuda.Event(enable_timing=True) for _ in range(4 * n_eval)]
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(5 * n_eval)]
             for e in evs:
                 e.record()
             ev_arr = (C.c_void_p * len(evs))(*[e.cuda_event for e in evs])
-        gal = _galaxy_prerender_arrays(order, shape, moff, c, tiers, nb, m_acc, precision,
-                                       prerender, dev, up)
+        gal = _prerender_arrays(order, shape, moff, c, tiers, nb, m_acc, precision, prerender,
+                                dev, up)
         sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
                               P(p_try), P(counters),
                               C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None,
@@ -395,22 +417,38 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         for k, kind in enumerate(kinds):
             it = (k - 1) // 2
             sel = np.ones(n_isl, bool) if kind in ("init", "final") else nst > it
+            e5 = evs[5 * k:5 * k + 5]
             if g_isl.size:
                 gsel = sel[g_isl]
-                tg = gal["T"][gsel].astype(np.float64)
-                # spectra written once per galaxy; per (galaxy, band): PSF
-                # coefficients read, raw planes written
+                st_, ex_ = gsel & gal["star"], gsel & ~gal["star"]
+                tg = gal["T"].astype(np.float64)
+                planes = gal["grad_planes"] if kind == "grad" else np.ones(len(g_isl))
+                # taps of every member; spectra of extended members (all
+                # three on H/g passes, F alone otherwise)
                 nspec = 3.0 if kind == "grad" else 1.0
-                sbytes = float(np.sum(gal["spec_floats"][gsel] / 3.0 * nspec * 4.0))
-                planes = 7.0 if kind == "grad" else 1.0
-                pbytes = float(nb * np.sum(tg * tg * (nt * 4.0 + planes * 4.0)))
-                timer.record("galaxy_spectra_kernel", evs[4 * k], evs[4 * k + 1],
+                sbytes = float(np.sum(gal["spec_floats"][ex_] / 3.0 * nspec * 4.0)
+                               + 28 * 4.0 * gsel.sum())
+                # per (member, band): PSF coefficients over the whole stamp
+                # (extended, convolved whole) or the shift-read box (stars),
+                # spectra read back (extended), window part of the planes
+                # written
+                ebytes = float(nb * np.sum(
+                    (tg * tg * nt * 4.0 + gal["spec_floats"] / 3.0 * nspec * 4.0
+                     + gal["win"] * planes * 4.0)[ex_]))
+                tbytes = float(nb * np.sum(
+                    (gal["reg"] * nt * 4.0 + gal["win"] * planes * 4.0)[st_]))
+                timer.record("prerender_spectra_kernel", e5[0], e5[1],
                              dict(kind="island_eval", phase=kind, bytes=sbytes,
-                                  n_gal=int(gsel.sum())))
-                timer.record("galaxy_planes_kernel", evs[4 * k + 1], evs[4 * k + 2],
-                             dict(kind="island_eval", phase=kind, bytes=pbytes,
-                                  n_gal=int(gsel.sum())))
-            timer.record("island_eval_kernel", evs[4 * k + 2], evs[4 * k + 3],
+                                  n_pre=int(gsel.sum())))
+                if gal["n_ext"]:
+                    timer.record("prerender_planes_kernel", e5[1], e5[2],
+                                 dict(kind="island_eval", phase=kind, bytes=ebytes,
+                                      n_pre=int(ex_.sum())))
+                if gal["n_ext"] < g_isl.size:
+                    timer.record("prerender_stars_kernel", e5[2], e5[3],
+                                 dict(kind="island_eval", phase=kind, bytes=tbytes,
+                                      n_pre=int(st_.sum())))
+            timer.record("island_eval_kernel", e5[3], e5[4],
                          dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
                               n_isl=int(sel.sum())))
     results = []
