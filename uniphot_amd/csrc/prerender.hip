@@ -65,6 +65,12 @@ __host__ __device__ inline PreScratch pre_scratch(int T) {
   return o;
 }
 
+// A star item's scratch: its raw planes (P, dP/dx, dP/dy) then the
+// horizontal pass, T^2 floats each, row stride T.
+__host__ __device__ inline size_t star_scratch_floats(int T) {
+  return round64(3 * (size_t)T * T) + 3 * (size_t)T * T;
+}
+
 __host__ __device__ inline size_t spectra_floats(int T) {
   const int pad = fft_pad_for(T);
   return (size_t)3 * (pad / 2 + 1) * pad;
@@ -591,7 +597,8 @@ __device__ __forceinline__ bool pre_item_fits(const IslandArgsPre& a, const PreI
   // and plane count follow from it)
   return (a.stype[it.slot] == 0) == star && (a.pre_spec[it.g] < 0) == star &&
          (star || galaxy_fits(T, fft_pad_for(T))) &&
-         pre_scratch(T).total * sizeof(float) <= a.scratch_per_block &&
+         (star ? star_scratch_floats(T) * sizeof(float) <= a.star_stride
+               : pre_scratch(T).total * sizeof(float) <= a.scratch_per_block) &&
          (size_t)a.pre_off[it.g] + (size_t)a.nb * pre_planes(star) * T * T <= a.pbuf_elems &&
          (star || (size_t)a.pre_spec[it.g] + spectra_floats(T) <= a.fspec_elems);
 }
@@ -729,12 +736,13 @@ __global__ __launch_bounds__(kPreBlock) void prerender_planes_kernel(IslandArgsP
 // window box, and no LDS beyond its basis and taps, so several workgroups
 // share a CU.
 constexpr int kStarBlock = 256;
+constexpr long long kStarMaxBlocks = 8192;
 
 __global__ __launch_bounds__(kStarBlock) void prerender_stars_kernel(IslandArgsPre a) {
   __shared__ float bas[63];
   __shared__ float taps[28];
   const int tid = threadIdx.x, mode = a.mode, nb = a.nb, nt = a.nt;
-  float* base = reinterpret_cast<float*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
+  float* base = reinterpret_cast<float*>(a.scratch + (size_t)blockIdx.x * a.star_stride);
   const int i0 = a.n_pre_ext * nb, i1 = a.n_pre * nb;
   const float* psrc = mode == kSplitEvalInit ? a.p0 : (mode == kSplitEvalTry ? a.p_try : a.p_out);
   for (int item = i0 + blockIdx.x; item < i1; item += gridDim.x) {
@@ -767,15 +775,15 @@ __global__ __launch_bounds__(kStarBlock) void prerender_stars_kernel(IslandArgsP
       taps[tid - 64] = a.gtap[(size_t)it.g * 28 + tid - 64];
     }
     __syncthreads();
-    const PreScratch L = pre_scratch(T);
-    float* R = base + L.R;
+    float* R = base;
     const StampClip& c = it.clip;
     psf_box<kStarBlock>(a.tier_coefs[a.shape[4 * (size_t)it.isl + 2]] + (size_t)b * nt * T * T,
                         nt, T, bas, geo[1], geo[3], it.need & 3, c.rx0, c.ry0, c.rx1 - c.rx0,
                         c.ry1 - c.ry0, R);
     __syncthreads();
-    shift_planes<kStarBlock>(R, base + L.Hh, a.pbuf + a.pre_off[it.g] + (size_t)b * 3 * T * T,
-                             T, it.need, taps, 0.0f, c);
+    shift_planes<kStarBlock>(R, base + round64(3 * (size_t)T * T),
+                             a.pbuf + a.pre_off[it.g] + (size_t)b * 3 * T * T, T, it.need, taps,
+                             0.0f, c);
   }
 }
 
@@ -791,8 +799,18 @@ void prerender_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hi
   const int g_ext = (int)(n_ext < nblk ? n_ext : nblk);
   if (g_ext > 0) hipLaunchKernelGGL(prerender_planes_kernel, dim3(g_ext), dim3(kPreBlock), 0, s, p);
   if (ev_ext) (void)hipEventRecord(ev_ext, s);
+  // the star launch carves the same scratch into smaller per-workgroup
+  // blocks (raw planes and horizontal pass of the largest tier only), so
+  // more of its workgroups are resident
+  int t_max = 0;
+  for (int t = 0; t < 3; ++t) t_max = p.tier_T[t] > t_max ? p.tier_T[t] : t_max;
+  p.star_stride = align256(star_scratch_floats(t_max) * sizeof(float));
+  const size_t total = (size_t)nblk * p.scratch_per_block;
+  const long long cap = (long long)(total / p.star_stride) < kStarMaxBlocks
+                            ? (long long)(total / p.star_stride)
+                            : kStarMaxBlocks;
   const long long n_star = (long long)(p.n_pre - p.n_pre_ext) * p.nb;
-  const int g_star = (int)(n_star < nblk ? n_star : nblk);
+  const int g_star = (int)(n_star < cap ? n_star : cap);
   if (g_star > 0)
     hipLaunchKernelGGL(prerender_stars_kernel, dim3(g_star), dim3(kStarBlock), 0, s, p);
 }

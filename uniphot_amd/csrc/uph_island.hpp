@@ -87,6 +87,7 @@ struct IslandArgsPre : IslandArgs {
   size_t fspec_elems;
   double* gdt;                 // (n_pre, 9) ellipse-term derivatives of extended members
   float* gtap;                 // (n_pre, 28) Lanczos taps and tap derivatives, x then y
+  size_t star_stride;          // per-workgroup scratch bytes of the star launch
   int* pre_counter;            // work queue of the planes kernel
 };
 
