@@ -95,19 +95,23 @@ __device__ __forceinline__ void pair_of(int e, int C, int& a, int& b) {
   b = a + e;
 }
 
-template <typename Real, bool GRAM>
+// Q entries (pairs, or columns against plane zr) are accumulated per pass
+// over the pixels; each entry sums the same per-pixel terms in the same
+// order whatever Q is, so Q only trades registers (2 Q doubles of
+// accumulators and Q plane addresses) against passes over W.
+template <typename Real, bool GRAM, int Q = 32>
 __device__ void weighted_pairs(const Real* __restrict__ W, const Real* __restrict__ Z, int P,
                                int C, int z0, int zr, double* out, double* red) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int E = GRAM ? C * (C + 1) / 2 : C;
   int a0 = 0, b0 = 0;  // first pair of the current chunk (GRAM)
-  for (int e0 = 0; e0 < E; e0 += 32) {
-    const int ne = min(32, E - e0);
-    int ia[32], ib[32];
+  for (int e0 = 0; e0 < E; e0 += Q) {
+    const int ne = min(Q, E - e0);
+    int ia[Q], ib[Q];
     {
       int a = a0, b = b0;
 #pragma unroll
-      for (int q = 0; q < 32; ++q) {
+      for (int q = 0; q < Q; ++q) {
         if (GRAM) {
           ia[q] = z0 + a;
           ib[q] = z0 + b;
@@ -127,13 +131,13 @@ __device__ void weighted_pairs(const Real* __restrict__ W, const Real* __restric
     }
     // products in Real, accumulation in float64 (exact-product FMA): the
     // pixel sums carry no float32 summation error into the flux solve.
-    double acc[32];
+    double acc[Q];
 #pragma unroll
-    for (int q = 0; q < 32; ++q) acc[q] = 0.0;
+    for (int q = 0; q < Q; ++q) acc[q] = 0.0;
     for (int p = tid; p < P; p += UPH_BLOCK) {
       const Real wv = W[p];
 #pragma unroll
-      for (int q = 0; q < 32; ++q) {
+      for (int q = 0; q < Q; ++q) {
         if (q < ne) {
           const Real za = Z[(size_t)ia[q] * P + p];
           const Real zb = Z[(size_t)ib[q] * P + p];
@@ -142,7 +146,7 @@ __device__ void weighted_pairs(const Real* __restrict__ W, const Real* __restric
       }
     }
 #pragma unroll
-    for (int q = 0; q < 32; ++q) {
+    for (int q = 0; q < Q; ++q) {
       double v = wave_sum(acc[q]);
       if (lane == 0) red[wid * 32 + q] = v;
     }
@@ -541,6 +545,11 @@ __device__ void place_member(const IslandArgs& a, IslandSh<Real>& sh, int b, con
 // One band of one island: window copy, member renders, Gram, VarPro fluxes,
 // residual, chi2 and (with grads) accumulation of H, g.
 // ---------------------------------------------------------------------------
+// Residual projections of the pre-render evaluation run 8 columns per pixel
+// pass: 112 VGPRs, four waves per SIMD (32 columns need 280 with the
+// accumulators and addresses, one wave per SIMD).
+constexpr int kPreQ = 8;
+
 template <typename Real, bool kPre>
 __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int b,
                           const float* pv, bool grads, bool accumulate, Real* planes, Real* ws,
@@ -628,7 +637,7 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
     Rp[p] = Dp[p] - model;
   }
   __syncthreads();
-  weighted_pairs<Real, false>(Wp, planes, P, Nr, 3, 1, sm.colws, sh.red);
+  weighted_pairs<Real, false, kPre ? kPreQ : 32>(Wp, planes, P, Nr, 3, 1, sm.colws, sh.red);
   if (tid == 0) {
     double* dx = sm.c;
     for (int i = 0; i < Nr; ++i) dx[i] = sm.colws[i] - reg * double(sh.fR[i]);
@@ -663,7 +672,7 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
   const Real chi2b = Real(block_sum(acc, sh.red));
   if (!grads) return chi2b;
   // c = M^T W R, s = T^T W R (one pass over the Nr + nL planes)
-  weighted_pairs<Real, false>(Wp, planes, P, Nr + nL, 3, 1, sm.colws, sh.red);
+  weighted_pairs<Real, false, kPre ? kPreQ : 32>(Wp, planes, P, Nr + nL, 3, 1, sm.colws, sh.red);
   for (int i = tid; i < Nr + nL; i += UPH_BLOCK) {
     if (i < Nr)
       sm.c[i] = sm.colws[i];
