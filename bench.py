@@ -11,7 +11,7 @@ placed against whichever of the 8 TB/s HBM and 157.3 TFLOP/s fp32 peaks it
 is closer to -- and a CPU baseline (the float64 oracle timed on a bounded
 sample of the same workload).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c2-stars|c1]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c2-stars|c1|c3]
 """
 from __future__ import annotations
 
@@ -225,12 +225,144 @@ def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0):
                         f"extrapolated per source (islands + 2 passes x {fld.nb} bands)"))
 
 
+def roofline_of(ks, config):
+    """Dominant kernel (most total time; the split sequence's own record is
+    for reference only) placed against the HBM or fp32 roof it is closer to."""
+    kern = max((k for k in ks if k != "island_split_sequence"), key=lambda k: ks[k]["ms"])
+    isl = ks[kern]
+    avg_s = isl["ms"] / 1e3 / max(isl["launches"], 1)
+    per_launch_bytes = isl["bytes"] / max(isl["launches"], 1)
+    achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
+    tflops = isl["flops"] / max(isl["ms"] / 1e3, 1e-12) / 1e12
+    compute = tflops / FP32_PEAK_TFLOPS > achieved / HBM_PEAK_GBS
+    traffic = measured_traffic(config, kern)
+    if compute:
+        roof = {"bound": "mfma", "achieved": round(tflops, 4), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 6),
+                "traffic": traffic, "hbm_gbs": round(achieved, 3),
+                "hbm_frac": round(achieved / HBM_PEAK_GBS, 6)}
+    else:
+        roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
+                "fp32_tflops": round(tflops, 4),
+                "fp32_frac": round(tflops / FP32_PEAK_TFLOPS, 6)}
+    roof.update({"kernel": kern, "avg_launch_ms": round(avg_s * 1e3, 4),
+                 "launches": isl["launches"], "algorithmic_bytes_per_launch": per_launch_bytes,
+                 "algorithmic_flops_per_launch": isl["flops"] / max(isl["launches"], 1)})
+    return roof
+
+
+def cpu_baseline_cutouts(cut, res, budget_s=12.0, seed=0):
+    """Float64 oracle on random islands of the cutout batch (bounded time),
+    extrapolated to seconds per source."""
+    from oracle import island as oisl
+    from oracle import core as ocore
+    rng = np.random.default_rng(seed)
+    d = cut.dstack
+    w = cut.wstack
+    o = oisl.LMOptions()
+    psfs = [dict(coefs=p.coefs.astype(np.float64), x_origin=p.x_origin, x_scale=p.x_scale,
+                 y_origin=p.y_origin, y_scale=p.y_scale) for p in cut.psfs]
+    items = [(pi, r) for pi, (plan, host) in enumerate(res.classes)
+             for r in range(plan.n_islands)]
+    order = rng.permutation(len(items))
+    t0 = time.perf_counter()
+    n_src = n_isl = 0
+    for q in order:
+        if time.perf_counter() - t0 > budget_s:
+            break
+        plan, host = res.classes[items[q][0]]
+        r = items[q][1]
+        B, T = plan.bbox_class, plan.tier
+        ox, oy = int(host["ox"][r]), int(host["oy"][r])
+        ps = [dict(p, coefs=ocore.pad_psf_coefs(p["coefs"], T)) for p in psfs]
+        dw = d[:, oy:oy + B, ox:ox + B].cpu().numpy()
+        ww = w[:, oy:oy + B, ox:ox + B].cpu().numpy()
+        isl = oisl.Island(dw, ww, host["x0b"][r], host["y0b"][r], host["st"][r], host["sn"][r],
+                          host["fn_flag"][r], host["mmask"][r], host["keep"][r],
+                          host["lower"][r].ravel(), host["upper"][r].ravel(), ps, T)
+        st = oisl.run_island(host["p0"][r].ravel().astype(np.float64), isl, o)
+        oisl.final_fluxes(st["p"], isl, o)
+        n_src += int(host["mmask"][r].sum())
+        n_isl += 1
+    el = time.perf_counter() - t0
+    return dict(value=n_src / max(el, 1e-9), unit="sources/s", cores=1, kind="port",
+                sample=f"float64 oracle, 1 thread: {n_isl} random islands ({n_src} sources) of "
+                       f"the batch in {el:.1f}s, per source")
+
+
+def bench_cutouts(args, torch):
+    """Batching workload: independent island cutouts fitted in band-split
+    launch sequences; the step is the whole batch's island fit."""
+    from uniphot_amd.pipeline import FitConfig, cutout_accuracy, fit_cutouts, psf_bank
+    from uniphot_amd.synth_cutouts import scene_cutouts
+    sync = torch.cuda.synchronize
+    t_scene = time.perf_counter()
+    cut = scene_cutouts(n_isl=args.islands, n_bands=10, device="cuda")
+    sync()
+    n = int(cut.x.size)
+    print(f"[bench] scene c3: {args.islands} islands, {n} sources, 10 bands, mosaic "
+          f"{cut.H}x{cut.W} in {time.perf_counter() - t_scene:.1f}s", file=sys.stderr, flush=True)
+    psf_of = psf_bank(cut.psfs, cut.dstack.device)
+    cfg = FitConfig()
+    for i in range(args.warmup):
+        r = fit_cutouts(cut, cfg, sync=sync, psf_of=psf_of)
+        print(f"[bench] warmup {i}: chunks {r.chunks}", file=sys.stderr, flush=True)
+    timer = KernelTimer()
+    sync()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        res = fit_cutouts(cut, cfg, sync=sync, psf_of=psf_of, timer=timer)
+        print(f"[bench] step {i}: chunks {res.chunks}", file=sys.stderr, flush=True)
+    sync()
+    el = time.perf_counter() - t0
+    ks = timer.summary()
+    rng = np.random.default_rng(1)
+    sample = np.sort(rng.choice(args.islands, min(args.sample, args.islands), replace=False))
+    acc = cutout_accuracy(cut, cfg, sample, sync=sync, psf_of=psf_of)
+    print(f"[bench] accuracy {acc}", file=sys.stderr, flush=True)
+    # the same sample through one persistent workgroup per island, one launch
+    # per class (the plain per-class form)
+    import dataclasses
+    sub_src = int(cut.island_set.n_src[sample].sum())
+    t1 = time.perf_counter()
+    fit_cutouts(cut, cfg, sync=sync, psf_of=psf_of, island_ids=sample)
+    t_split = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    fit_cutouts(cut, dataclasses.replace(cfg, island_split=False), sync=sync, psf_of=psf_of,
+                island_ids=sample)
+    t_plain = time.perf_counter() - t1
+    line = {
+        "metric": "sources fitted/sec at <=1e-5 flux rel-err vs FP64; p50/p99 tile latency",
+        "value": round(n * args.steps / el, 2), "unit": "sources/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c3", "n_islands": args.islands, "n_sources": n, "n_bands": 10,
+                   "members": "1-64, P(k)~k^-2", "windows": "15-95 px",
+                   "launch_sequences": [c for c, _ in res.chunks]},
+        "roofline": roofline_of(ks, "c3"),
+        "accuracy_vs_fp64": acc,
+        "sample_split_vs_persistent": {"islands": int(sample.size), "sources": sub_src,
+                                       "band_split_s": round(t_split, 3),
+                                       "persistent_per_island_s": round(t_plain, 3)},
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                        for kk, vv in v.items()} for k, v in ks.items()},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_cutouts(cut, res, budget_s=args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2")
+    ap.add_argument("--config", default="c2", help="c2 (default), c2-stars, c1 or c3")
+    ap.add_argument("--islands", type=int, default=200_000, help="c3: island cutouts")
+    ap.add_argument("--sample", type=int, default=300,
+                    help="c3: islands of the FP64 accuracy and persistent-form comparisons")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--island-split", type=int, default=1,
@@ -246,6 +378,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.config == "c3":
+        bench_cutouts(args, torch)
+        return
     from uniphot_amd.pipeline import DeviceField, FitConfig, fit_field
     t_scene = time.perf_counter()
     sc, gcfg = make_scene(args.config)
@@ -280,30 +415,7 @@ def main():
     if os.environ.get("UPH_BENCH_VERBOSE"):
         for name, ms, info in timer.per_launch()[:400]:
             print(f"[bench] {name} {ms:.3f} ms {info}", file=sys.stderr)
-    # dominant kernel: the most total time among single kernels (the split
-    # island fit's whole launch sequence is recorded for reference only)
-    kern = max((k for k in ks if k != "island_split_sequence"), key=lambda k: ks[k]["ms"])
-    isl = ks[kern]
-    avg_s = isl["ms"] / 1e3 / max(isl["launches"], 1)
-    per_launch_bytes = isl["bytes"] / max(isl["launches"], 1)
-    achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
-    tflops = isl["flops"] / max(isl["ms"] / 1e3, 1e-12) / 1e12
-    # the roofline the kernel sits closer to: HBM bytes or fp32 arithmetic
-    compute = tflops / FP32_PEAK_TFLOPS > achieved / HBM_PEAK_GBS
-    traffic = measured_traffic(args.config, kern)
-    if compute:
-        roof = {"bound": "mfma", "achieved": round(tflops, 4), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 6),
-                "traffic": traffic, "hbm_gbs": round(achieved, 3),
-                "hbm_frac": round(achieved / HBM_PEAK_GBS, 6)}
-    else:
-        roof = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": traffic,
-                "fp32_tflops": round(tflops, 4),
-                "fp32_frac": round(tflops / FP32_PEAK_TFLOPS, 6)}
-    roof.update({"kernel": kern, "avg_launch_ms": round(avg_s * 1e3, 4),
-                 "launches": isl["launches"], "algorithmic_bytes_per_launch": per_launch_bytes,
-                 "algorithmic_flops_per_launch": isl["flops"] / max(isl["launches"], 1)})
+    roof = roofline_of(ks, args.config)
     line = {
         "metric": "sources fitted/sec at <=1e-5 flux rel-err vs FP64; p50/p99 tile latency",
         "value": round(value, 2), "unit": "sources/s", "n_gpus": 1, "steps": args.steps,

@@ -202,10 +202,12 @@ typedef struct uph_island_split {
                                  rest stars                                    */
   const int32_t* pre_isl;     /* (n_pre) island                                */
   const int32_t* pre_slot;    /* (n_pre) member slot within the island         */
-  const int64_t* pre_off;     /* (n_pre) first float of its n_bands x np x T^2
-                                 planes in pbuf, np = 7 for an extended member
-                                 (model and 6 tangents), 3 for a star (model,
-                                 d/dx, d/dy)                                   */
+  const int64_t* pre_off;     /* (n_pre) first float of its n_bands x np planes
+                                 of min(T, B)^2 floats in pbuf, np = 7 for an
+                                 extended member (model and 6 tangents), 3 for
+                                 a star (model, d/dx, d/dy); a plane holds the
+                                 part of the stamp inside the island window,
+                                 stored from that part's corner                */
   const int64_t* pre_spec;    /* (n_pre) first float of its 3 x (pad/2+1) x pad
                                  Fourier spectra in fspec; -1 for stars        */
   const int32_t* pre_of_slot; /* (total member slots) pre-render index, -1

@@ -521,22 +521,23 @@ __device__ void place_member(const IslandArgs& a, IslandSh<Real>& sh, int b, con
                              int r, float x0f, float y0f, Real* __restrict__ m_out,
                              Real* __restrict__ tan_base) {
   const IslandArgsPre& ap = static_cast<const IslandArgsPre&>(a);
-  const int T = sh.T, B = sh.B, TT = T * T, P = B * B, tid = threadIdx.x;
+  const int T = sh.T, B = sh.B, P = B * B, tid = threadIdx.x;
   const Real xg = Real(p6[0] - x0f), yg = Real(p6[1] - y0f);
   const int ix = int(rint(xg)) - T / 2, iy = int(rint(yg)) - T / 2;
   const int wx0 = max(0, ix), wx1 = min(B, ix + T), wy0 = max(0, iy), wy1 = min(B, iy + T);
   if (wx1 <= wx0 || wy1 <= wy0) return;  // uniform
   const int sx0 = wx0 - ix, sy0 = wy0 - iy, SW = wx1 - wx0, SH = wy1 - wy0;
   const int g = ap.pre_of_slot[(size_t)sh.moff + sh.mem[r]];
-  const float* blk = ap.pbuf + ap.pre_off[g] + (size_t)b * pre_planes(sh.st[r] == 0) * TT;
+  const int side = pre_side(T, B), PP = side * side;
+  const float* blk = ap.pbuf + ap.pre_off[g] + (size_t)b * pre_planes(sh.st[r] == 0) * PP;
   const int l0 = sh.first_live[r], l1 = sh.first_live[r + 1];
   for (int e = tid; e < SH * SW; e += UPH_BLOCK) {
     const int rs = sy0 + e / SW, cs = sx0 + e % SW;
-    const int wi = (rs + iy) * B + (cs + ix), so = rs * T + cs;
+    const int wi = (rs + iy) * B + (cs + ix), so = (e / SW) * side + e % SW;
     m_out[wi] = Real(blk[so]);
     if (tan_base)
       for (int l = l0; l < l1; ++l)
-        tan_base[(size_t)l * P + wi] = Real(blk[(size_t)(1 + sh.live_j[l]) * TT + so]);
+        tan_base[(size_t)l * P + wi] = Real(blk[(size_t)(1 + sh.live_j[l]) * PP + so]);
   }
   __syncthreads();
 }

@@ -489,10 +489,11 @@ struct StampClip {
 // horizontal pass into Hh, vertical pass into out, over the window box of
 // the stamp (raw inputs outside [0, T) are zero; only the box c.r* is read).
 // Plane 1 of out gets the x tap derivative applied to plane 0, plane 2 the y
-// tap derivative; plane 6 is scaled by dn/du.  Planes of tangents not in
-// `need`, and pixels outside the window box, are not written.
+// tap derivative; plane 6 is scaled by dn/du.  out holds planes of side x
+// side floats, the window box stored from its corner (c.sx0, c.sy0).  Planes
+// of tangents not in `need` are not written.
 template <int kThreads>
-__device__ void shift_planes(const float* R, float* Hh, float* out, int T, int need,
+__device__ void shift_planes(const float* R, float* Hh, float* out, int T, int side, int need,
                              const float* taps, float dndu, const StampClip& c) {
   const int tid = threadIdx.x, TT = T * T;
   float Lx[7], dLx[7], Ly[7], dLy[7];
@@ -524,8 +525,9 @@ __device__ void shift_planes(const float* R, float* Hh, float* out, int T, int n
       if (need & (1 << j)) Hh[(size_t)(1 + j) * TT + e] = h[1 + j];
   }
   __syncthreads();
+  const int PP = side * side;
   for (int q = tid; q < SH * SW; q += kThreads) {
-    const int rw = c.sy0 + q / SW, cc = c.sx0 + q % SW, e = rw * T + cc;
+    const int rw = c.sy0 + q / SW, cc = c.sx0 + q % SW, e = (q / SW) * side + q % SW;
     float v[7] = {0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
@@ -542,7 +544,7 @@ __device__ void shift_planes(const float* R, float* Hh, float* out, int T, int n
     out[e] = v[0];
 #pragma unroll
     for (int j = 0; j < 6; ++j)
-      if (need & (1 << j)) out[(size_t)(1 + j) * TT + e] = j == 5 ? v[6] * dndu : v[1 + j];
+      if (need & (1 << j)) out[(size_t)(1 + j) * PP + e] = j == 5 ? v[6] * dndu : v[1 + j];
   }
   __syncthreads();
 }
@@ -552,7 +554,7 @@ __device__ void shift_planes(const float* R, float* Hh, float* out, int T, int n
 // has nothing to render (converged island, stamp outside the window, or a
 // buffer overflow, which also raises the error flag).
 struct PreItem {
-  int g, b, isl, T, need;
+  int g, b, isl, T, side, need;
   size_t slot;
   StampClip clip;
 };
@@ -568,6 +570,7 @@ __device__ __forceinline__ bool pre_item(const IslandArgsPre& a, int item, bool 
   const int32_t* sp = a.shape + 4 * (size_t)it.isl;
   const int T = a.tier_T[sp[2]], B = sp[1];
   it.T = T;
+  it.side = pre_side(T, B);
   it.slot = (size_t)a.mem_off[it.isl] + a.pre_slot[it.g];
   int need = 0;
   if (mode == kSplitEvalGrad)
@@ -599,7 +602,8 @@ __device__ __forceinline__ bool pre_item_fits(const IslandArgsPre& a, const PreI
          (star || galaxy_fits(T, fft_pad_for(T))) &&
          (star ? star_scratch_floats(T) * sizeof(float) <= a.star_stride
                : pre_scratch(T).total * sizeof(float) <= a.scratch_per_block) &&
-         (size_t)a.pre_off[it.g] + (size_t)a.nb * pre_planes(star) * T * T <= a.pbuf_elems &&
+         (size_t)a.pre_off[it.g] + (size_t)a.nb * pre_planes(star) * it.side * it.side <=
+             a.pbuf_elems &&
          (star || (size_t)a.pre_spec[it.g] + spectra_floats(T) <= a.fspec_elems);
 }
 
@@ -724,9 +728,10 @@ __global__ __launch_bounds__(kPreBlock) void prerender_planes_kernel(IslandArgsP
     {
       const int g2 = s_it.g, T2 = s_it.T;
       const PreScratch L2 = pre_scratch(T2);
+      const int side = s_it.side;
       shift_planes<kPreBlock>(base + L2.R, base + L2.Hh,
-                              a.pbuf + a.pre_off[g2] + (size_t)s_it.b * 7 * T2 * T2, T2,
-                              s_it.need, taps, s_dndu, s_it.clip);
+                              a.pbuf + a.pre_off[g2] + (size_t)s_it.b * 7 * side * side, T2,
+                              side, s_it.need, taps, s_dndu, s_it.clip);
     }
   }
 }
@@ -782,8 +787,8 @@ __global__ __launch_bounds__(kStarBlock) void prerender_stars_kernel(IslandArgsP
                         c.ry1 - c.ry0, R);
     __syncthreads();
     shift_planes<kStarBlock>(R, base + round64(3 * (size_t)T * T),
-                             a.pbuf + a.pre_off[it.g] + (size_t)b * 3 * T * T, T, it.need, taps,
-                             0.0f, c);
+                             a.pbuf + a.pre_off[it.g] + (size_t)b * 3 * it.side * it.side, T,
+                             it.side, it.need, taps, 0.0f, c);
   }
 }
 

@@ -67,10 +67,10 @@ struct IslandArgs {
 // every real member of the active islands is rendered by (member, band)
 // items -- stars from the PSF polynomial, extended members through the
 // Fourier mixture -- and shifted to its sub-pixel position, into
-// n_bands x np x T^2 floats per member (the model, then the tangents of x,
-// y and, for an extended member (np = 7), logre, ee1, ee2, n; np = 3 for a
-// star); only the part that lands in the island window is written, and the
-// evaluation kernel only places that part.
+// n_bands x np planes of min(T, B)^2 floats per member (the model, then the
+// tangents of x, y and, for an extended member (np = 7), logre, ee1, ee2, n;
+// np = 3 for a star); a plane holds the part of the stamp that lands in the
+// island window (pre_side), which the evaluation kernel places.
 // Only the pre-render launches see these fields; every other kernel takes
 // the base IslandArgs.
 struct IslandArgsPre : IslandArgs {
@@ -103,6 +103,11 @@ constexpr int kSplitStepAccept = 6; // accept / reject, convergence
 
 // Planes per band of a pre-rendered member.
 __host__ __device__ inline int pre_planes(bool star) { return star ? 3 : 7; }
+
+// Side of one stored plane: the part of a T x T stamp inside a B x B window
+// is at most min(T, B) on a side.  Pixel (r, c) of the stamp's window part
+// [sx0, sx1) x [sy0, sy1) is stored at (r - sy0) * side + (c - sx0).
+__host__ __device__ inline int pre_side(int T, int B) { return T < B ? T : B; }
 
 // Doubles of one band block of an island with n_live_max = nLc:
 // H (nL x nL, row-major with the actual nL), g (nL), chi2, one spare.

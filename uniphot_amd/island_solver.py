@@ -185,7 +185,8 @@ def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prer
     """Index arrays and buffers of the band-split member pre-render: every
     real member, extended members first, each kind in the islands'
     work-queue order; extended members also get a block of Fourier
-    spectra.  Planes per band: 7 for an extended member, 3 for a star."""
+    spectra.  Planes per band: 7 for an extended member, 3 for a star, each
+    min(T, B)^2 floats."""
     import torch
     L = _lib.lib()
     P = _lib.ptr
@@ -213,7 +214,9 @@ def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prer
     if any(L.uph_prerender_scratch_bytes(int(t)) == 0 for t in set(T.tolist())):
         return empty
     star = np.arange(len(members)) >= len(ext)
-    psz = nb * np.where(star, 3, 7) * T * T
+    # one plane holds the window part of the stamp: at most min(T, B) a side
+    side = np.minimum(T, np.asarray([shape[i][1] for i in pre_isl], np.int64))
+    psz = nb * np.where(star, 3, 7) * side * side
     ssz = np.where(star, 0, [L.uph_prerender_spectra_floats(int(t)) for t in T]).astype(np.int64)
     poff = np.concatenate([[0], np.cumsum(psz)[:-1]]).astype(np.int64)
     soff = np.concatenate([[0], np.cumsum(ssz)[:-1]]).astype(np.int64)

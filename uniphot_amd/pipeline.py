@@ -31,7 +31,7 @@ from .island_solver import make_class_kernels, psf_tree_device, run_class_chunk
 from .islands import (ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands,
                       build_islands, flux_aware_radii, psf_radial_profile)
 from .psf_polynomial import evaluate_psf_at_np, psf_device_arrays
-from .source_model import SOURCE_STAR
+from .source_model import SOURCE_STAR, sersic_u_to_n
 
 
 @dataclass
@@ -183,6 +183,144 @@ def padded_stacks(fld: DeviceField, grids, bg, bg_order, pad):
     return d, w
 
 
+@dataclass
+class IslandSetFit:
+    """Per-source results of the island LM over an island set."""
+    x: np.ndarray
+    y: np.ndarray
+    logre: np.ndarray
+    ee1: np.ndarray
+    ee2: np.ndarray
+    sersic_n: np.ndarray      # fitted n of free-n members, the catalog n otherwise
+    p: np.ndarray             # (n, 6) fitted parameter vector (x, y, logre, ee1, ee2, u)
+    flux: np.ndarray          # (n, nb) island VarPro fluxes, NaN where not fitted
+    flux_err: np.ndarray
+    conv: np.ndarray
+    stall: np.ndarray
+    nst: np.ndarray
+    fitted: np.ndarray        # (n,) bool: member of a jointly fitted island
+    n_fitted_islands: int
+    classes: list             # (plan, host) per class
+    chunks: list              # (n_islands, seconds) per launch sequence
+
+
+def _chunk_batch(batch, nb, budget_bytes):
+    """Split a class batch [(kernels, host, rows)] into launch sequences whose
+    pre-render planes fit ``budget_bytes``; classes are taken in order of
+    their per-workgroup scratch so large classes share sequences."""
+    from . import _lib
+    L = _lib.lib()
+    keyed = []
+    for kern, host, rows in batch:
+        per = L.uph_island_scratch_per_block(kern.N, kern.B, kern.T, kern.n_live_max, 0)
+        side = min(kern.T, kern.B)
+        st = host["st"][rows]
+        mm = host["mmask"][rows].astype(bool)
+        planes = np.where(st == SOURCE_STAR, 3, 7) * mm
+        cost = nb * side * side * 4.0 * planes.sum(axis=1)
+        keyed.append((per, kern, host, np.asarray(rows), cost))
+    keyed.sort(key=lambda t: t[0])
+    chunks, cur, used = [], [], 0.0
+    for per, kern, host, rows, cost in keyed:
+        start = 0
+        while start < rows.size:
+            room = budget_bytes - used
+            take = int(np.searchsorted(np.cumsum(cost[start:]), room, side="right"))
+            if take == 0 and cur:
+                chunks.append(cur)
+                cur, used = [], 0.0
+                continue
+            take = max(take, 1)
+            cur.append((kern, host, rows[start:start + take]))
+            used += float(cost[start:start + take].sum())
+            start += take
+    if cur:
+        chunks.append(cur)
+    return chunks
+
+
+def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans, x, y, stype,
+                   logre, ee1, ee2, sn, free_n, cfg: FitConfig, timer=None,
+                   budget_bytes: float = 48e9, sync=None, p_start=None) -> IslandSetFit:
+    """Island VarPro-LM over the planned classes of an island set
+    (island_solver.py:3233-3595, one device): class host arrays, the device
+    Sersic table and PSF trees, then the band-split fit in launch sequences
+    of at most ``budget_bytes`` of pre-rendered planes each.  ``psf_of(T)``
+    gives the device (coefs, geo) of stamp tier T.  ``p_start`` (n, 6), when
+    given, replaces the starting parameters of the real members after the
+    classes (stamp tiers, bounds) were planned from the catalog seeds."""
+    from .island_solver import PsfTree, run_island_batch
+    n = x.size
+    fx, fy = x.copy(), y.copy()
+    flr, fe1, fe2 = logre.copy(), ee1.copy(), ee2.copy()
+    fsn = np.asarray(sn, np.float32).copy()
+    pfit = np.full((n, 6), np.nan, np.float32)
+    flux = np.full((n, nb), np.nan, np.float32)
+    ferr = np.full((n, nb), np.nan, np.float32)
+    conv = np.zeros(n, bool)
+    stall = np.zeros(n, bool)
+    nst = np.zeros(n, np.int32)
+    fitted = np.zeros(n, bool)
+    classes, batch, trees = [], [], {}
+    for plan in plans:
+        host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
+                                 ee1=ee1, ee2=ee2, sersic_n=sn, free_n_all=free_n,
+                                 pos_max_shift_pix=cfg.pos_max_shift_pix, pad=pad)
+        kern = make_class_kernels(plan, nb, n_terms, reg=cfg.reg, precision=cfg.precision)
+        if p_start is not None:
+            mm = plan.mmask.astype(bool)
+            host["p0"][mm] = np.asarray(p_start, np.float32)[plan.midx[mm]]
+        if plan.tier not in trees:
+            coefs, geo = psf_of(plan.tier)
+            trees[plan.tier] = PsfTree(coefs, geo, plan.tier, n_terms)
+        batch.append((kern, host, np.arange(plan.n_islands)))
+        classes.append((plan, host))
+    sersic_dev = None
+    if any(p.kind != "star" for p in plans):
+        from .sersic_table import device_table
+        sersic_dev = device_table(dstack.device)
+    if cfg.island_split and cfg.galaxy_prerender and cfg.precision != "fp64":
+        chunks = _chunk_batch(batch, nb, budget_bytes)
+    else:
+        chunks = [batch]
+    plan_of = {id(h): p for p, h in classes}
+    timing = []
+    for chunk in chunks:
+        if not chunk:
+            continue
+        t0 = time.perf_counter()
+        results = run_island_batch(chunk, trees, cfg.max_steps, dstack, wstack,
+                                   precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
+                                   split=cfg.island_split, prerender=cfg.galaxy_prerender)
+        if sync is not None:
+            sync()
+        timing.append((int(sum(np.asarray(r).size for _, _, r in chunk)),
+                       time.perf_counter() - t0))
+        for (kern, host, rows), res in zip(chunk, results):
+            plan = plan_of[id(host)]
+            mm = plan.mmask[rows]
+            src = plan.midx[rows][mm]
+            fx[src] = res["p"][:, :, 0][mm]
+            fy[src] = res["p"][:, :, 1][mm]
+            ext = (stype[src] != SOURCE_STAR)
+            flr[src[ext]] = res["p"][:, :, 2][mm][ext]
+            fe1[src[ext]] = res["p"][:, :, 3][mm][ext]
+            fe2[src[ext]] = res["p"][:, :, 4][mm][ext]
+            fnm = host["fn_flag"][rows][mm].astype(bool)
+            fsn[src[fnm]] = sersic_u_to_n(res["p"][:, :, 5][mm][fnm]).astype(np.float32)
+            pfit[src] = res["p"][mm]
+            flux[src] = res["fluxes"].transpose(0, 2, 1)[mm]
+            ferr[src] = res["ferr"].transpose(0, 2, 1)[mm]
+            r_of = np.repeat(np.arange(rows.size), mm.sum(axis=1))
+            conv[src] = res["conv"][r_of]
+            stall[src] = res["stall"][r_of]
+            nst[src] = res["nst"][r_of]
+            fitted[src] = True
+    n_fit = sum(p.n_islands for p in plans)
+    return IslandSetFit(fx, fy, flr, fe1, fe2, fsn, pfit, flux, ferr, conv, stall, nst, fitted,
+                        n_fit, classes, timing)
+
+
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,
               sync=None, timer=None, keep_artifacts=False) -> FitResult:
     """Full island-mode fit of one field (catalog seeds in, fitted fluxes
@@ -234,51 +372,16 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     tm["islands"] = time.perf_counter() - t1
     # ---- island LM ----
     t2 = time.perf_counter()
-    fx, fy = x.copy(), y.copy()
-    flr, fe1, fe2, fsn = logre.copy(), ee1.copy(), ee2.copy(), sn.copy()
-    conv = np.zeros(n, bool)
-    stall = np.zeros(n, bool)
-    nst = np.zeros(n, np.int32)
+    isf = fit_island_set(fld.psf, fld.nb, fld.n_terms, fld.native, dstack, wstack, pad, iset,
+                         plans, x, y, stype, logre, ee1, ee2, sn, free_n, cfg, timer=timer)
+    fx, fy, flr, fe1, fe2 = isf.x, isf.y, isf.logre, isf.ee1, isf.ee2
+    fsn = sn.copy()
+    conv, stall, nst = isf.conv, isf.stall, isf.nst
     giant = np.zeros(n, bool)
     for gid in giants:
         giant[iset.members[int(gid)]] = True
-    n_fitted_islands = 0
-    classes = []
-    batch = []
-    trees = {}
-    from .island_solver import PsfTree, run_island_batch
-    for plan in plans:
-        host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
-                                 ee1=ee1, ee2=ee2, sersic_n=sn, free_n_all=free_n,
-                                 pos_max_shift_pix=cfg.pos_max_shift_pix, pad=pad)
-        kern = make_class_kernels(plan, fld.nb, fld.n_terms, reg=cfg.reg,
-                                  precision=cfg.precision)
-        if plan.tier not in trees:
-            coefs, geo = fld.psf(plan.tier)
-            trees[plan.tier] = PsfTree(coefs, geo, plan.tier, fld.n_terms)
-        batch.append((kern, host, np.arange(plan.n_islands)))
-        classes.append((plan, host))
-    sersic_dev = None
-    if any(p.kind != "star" for p in plans):
-        from .sersic_table import device_table
-        sersic_dev = device_table(dstack.device)
-    results = run_island_batch(batch, trees, cfg.max_steps, dstack, wstack,
-                               precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
-                               split=cfg.island_split, prerender=cfg.galaxy_prerender)
-    for (plan, host), res in zip(classes, results):
-        n_fitted_islands += plan.n_islands
-        mm = plan.mmask
-        src = plan.midx[mm]
-        fx[src] = res["p"][:, :, 0][mm]
-        fy[src] = res["p"][:, :, 1][mm]
-        ext = (stype[src] != SOURCE_STAR)
-        flr[src[ext]] = res["p"][:, :, 2][mm][ext]
-        fe1[src[ext]] = res["p"][:, :, 3][mm][ext]
-        fe2[src[ext]] = res["p"][:, :, 4][mm][ext]
-        rows = np.repeat(np.arange(plan.n_islands), mm.sum(axis=1))
-        conv[src] = res["conv"][rows]
-        stall[src] = res["stall"][rows]
-        nst[src] = res["nst"][rows]
+    n_fitted_islands = isf.n_fitted_islands
+    classes = isf.classes
     sync()
     tm["island_fit"] = time.perf_counter() - t2
     # ---- pass 2 ----
@@ -297,3 +400,87 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                                  grids=grids, island_set=iset, gtier=gtier)
     return FitResult(flux, np.sqrt(np.maximum(var, 0.0)), s1_flux, s1_var, fx, fy, flr, fe1,
                      fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec)
+
+
+def psf_bank(psfs, device="cuda"):
+    """``psf_of(T)``: device (coefs, geo) of the band PSFs padded to tier T,
+    built once per tier."""
+    import torch
+    cache = {}
+
+    def psf_of(T):
+        if T not in cache:
+            c, g = psf_device_arrays(psfs, T)
+            cache[T] = (torch.from_numpy(c).to(device), torch.from_numpy(g).to(device))
+        return cache[T]
+    return psf_of
+
+
+def fit_cutouts(cut, cfg: FitConfig, timer=None, sync=None, joint_nsrc_cap=None,
+                budget_bytes: float = 48e9, psf_of=None, island_ids=None,
+                p_start=None) -> IslandSetFit:
+    """Batched island fit of independent island cutouts (each island in its
+    own zero-weight-bordered cell of a mosaic; see synth_cutouts): class
+    planning with every island fitted jointly up to 64 members
+    (``joint_nsrc_cap=None``), then :func:`fit_island_set`.  With no linear
+    pass before it, the free-n gate's S/N is the true flux over the
+    sky-limited noise of a 50 px^2 aperture.  ``island_ids`` restricts the
+    fit to those islands."""
+    from .islands import IslandSet
+    iset = cut.island_set
+    if island_ids is not None:
+        ids = np.asarray(island_ids, np.int64)
+        iset = IslandSet(n_sources=iset.n_sources, n_islands=ids.size, labels=iset.labels,
+                         radius=iset.radius, members=[iset.members[i] for i in ids],
+                         n_src=iset.n_src[ids], bbox=iset.bbox[ids], r_psf=iset.r_psf)
+    native = int(cut.psfs[0].H_psf)
+    buckets = bucket_islands(iset, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
+    plans, capped = plan_island_classes(iset, buckets, cut.stype, cut.logre, cut.sersic_n,
+                                        native, joint_nsrc_cap=joint_nsrc_cap)
+    var = (np.asarray(cut.sky, np.float64) + 25.0)[None, :] * 50.0
+    free_n = free_n_gate(cut.stype, cut.true_flux, np.broadcast_to(var, cut.true_flux.shape))
+    psf_of = psf_of or psf_bank(cut.psfs, cut.dstack.device)
+    return fit_island_set(psf_of, cut.n_bands, int(cut.psfs[0].n_terms), native, cut.dstack,
+                          cut.wstack, cut.pad, iset, plans, cut.x, cut.y, cut.stype, cut.logre,
+                          cut.ee1, cut.ee2, cut.sersic_n, free_n, cfg, timer=timer,
+                          budget_bytes=budget_bytes, sync=sync, p_start=p_start)
+
+
+def _rel_stats(f, ref, ref_err):
+    rel = np.abs(f - ref) / np.maximum(np.abs(ref), ref_err)
+    r = np.nanmax(rel, axis=1) if rel.ndim == 2 else rel
+    r = r[np.isfinite(r)]
+    if r.size == 0:
+        return dict(n=0)
+    return dict(n=int(r.size), p50=float(np.percentile(r, 50)), p99=float(np.percentile(r, 99)),
+                max=float(r.max()), frac_le_1e5=float((r <= 1e-5).mean()))
+
+
+def cutout_accuracy(cut, cfg: FitConfig, island_ids, sync=None, psf_of=None) -> dict:
+    """Flux error of the float32 fit against the float64 fit on the given
+    islands, per source as max over bands of |f32 - f64| / max(|f64|,
+    sigma64):
+      * ``fit``: both full LM fits.  Where the chi2 landscape of a blend is
+        flat, float32 and float64 trajectories may stop at different
+        positions, and the fluxes follow them.
+      * ``flux_solve``: the float32 flux solve (no LM step) at the float64
+        fit's parameters -- the arithmetic precision of the fluxes."""
+    import dataclasses
+    psf_of = psf_of or psf_bank(cut.psfs, cut.dstack.device)
+    c32 = dataclasses.replace(cfg, precision="fp32")
+    c64 = dataclasses.replace(cfg, precision="fp64")
+    f32 = fit_cutouts(cut, c32, island_ids=island_ids, sync=sync, psf_of=psf_of)
+    f64 = fit_cutouts(cut, c64, island_ids=island_ids, sync=sync, psf_of=psf_of)
+    fz = fit_cutouts(cut, dataclasses.replace(c32, max_steps=0), island_ids=island_ids,
+                     sync=sync, psf_of=psf_of, p_start=f64.p)
+    fz64 = fit_cutouts(cut, dataclasses.replace(c64, max_steps=0), island_ids=island_ids,
+                       sync=sync, psf_of=psf_of, p_start=f64.p)
+    ok = f32.fitted & f64.fitted
+    same = ok & (f32.nst == f64.nst) & (f32.conv == f64.conv) & (f32.stall == f64.stall)
+    return dict(n_islands=int(np.asarray(island_ids).size),
+                same_trajectory=float(same.sum() / max(ok.sum(), 1)),
+                fit=_rel_stats(f32.flux[ok], f64.flux[ok], f64.flux_err[ok]),
+                flux_solve=_rel_stats(fz.flux[ok], f64.flux[ok], f64.flux_err[ok]),
+                flux_solve_fp64_self=_rel_stats(fz64.flux[ok], f64.flux[ok], f64.flux_err[ok]),
+                position_max_px=float(np.max(np.hypot(f32.x[ok] - f64.x[ok],
+                                                       f32.y[ok] - f64.y[ok]), initial=0.0)))

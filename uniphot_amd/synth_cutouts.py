@@ -1,0 +1,241 @@
+"""Synthetic island cutouts for the batching workload: many independent
+islands of 1-64 sources (P(k) ~ k^-2), 10 bands, the source mix, PSF and
+noise of the 20-band 2048^2 field scene.  Each island sits in its own cell
+of a mosaic; pixels outside the cells carry zero weight, so no island sees
+another's light.  The truth is rendered on the GPU with torch (about 6 M
+(source, band) profiles; the numpy renderer of the field scenes would take
+tens of minutes), deterministically: every cell receives one profile per
+member slot at a time.
+
+Recipe: members uniform in a disk of radius 4 sqrt(k) px; window side = the
+smallest odd number >= 15 that holds every member's +-5 px box, capped at
+95; the window sits centred in a slot of its island class side B, and the
+slots are packed in rows of a mosaic whose PSF polynomial spans the whole
+mosaic, so islands (shuffled before packing) see the PSF of a random field
+position.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .islands import ISLAND_BBOX_CLASSES, IslandSet
+from .sersic_table import mog_params_at
+from .source_model import SOURCE_DEV, SOURCE_EXP, SOURCE_SERSIC, SOURCE_STAR
+from .synth import SEEDS, MixturePsfField, _cov_terms, _galaxy_shapes, _seed_catalog
+
+
+@dataclass
+class Cutouts:
+    dstack: object            # torch (nb, H + 2 pad, W + 2 pad) float32 data, zero outside cells
+    wstack: object            # torch, inverse variance, zero outside cells
+    pad: int
+    H: int                    # mosaic size without the border
+    W: int
+    psfs: list                # per-band PsfPolynomial in the mosaic frame
+    x: np.ndarray             # catalog seeds
+    y: np.ndarray
+    stype: np.ndarray
+    logre: np.ndarray
+    ee1: np.ndarray
+    ee2: np.ndarray
+    sersic_n: np.ndarray
+    true_x: np.ndarray
+    true_y: np.ndarray
+    true_flux: np.ndarray     # (n, nb)
+    island_set: IslandSet
+    sky: np.ndarray           # (nb,)
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n_bands(self) -> int:
+        return int(self.sky.size)
+
+
+def island_sizes(rng, n_isl: int, k_max: int = 64) -> np.ndarray:
+    """Members per island, P(k) proportional to k^-2 on 1..k_max."""
+    k = np.arange(1, k_max + 1)
+    p = k.astype(np.float64) ** -2.0
+    return rng.choice(k, n_isl, p=p / p.sum()).astype(np.int64)
+
+
+def _window_sides(ux, uy, starts, box, w_min, w_max):
+    xmin, xmax = np.minimum.reduceat(ux, starts), np.maximum.reduceat(ux, starts)
+    ymin, ymax = np.minimum.reduceat(uy, starts), np.maximum.reduceat(uy, starts)
+    ext = np.maximum(xmax - xmin, ymax - ymin) + 2.0 * box
+    w = np.ceil(ext).astype(np.int64)
+    w += (w % 2 == 0)
+    return np.clip(w, w_min, w_max), 0.5 * (xmin + xmax), 0.5 * (ymin + ymax)
+
+
+def _pack_slots(bcls, order, mosaic_w):
+    """Slot origins: islands grouped by class side, each group in rows of
+    slots of its side, groups stacked vertically; `order` sets which island
+    takes which slot within its group."""
+    sx = np.zeros(bcls.size, np.int64)
+    sy = np.zeros(bcls.size, np.int64)
+    y0 = 0
+    for B in sorted(set(bcls.tolist())):
+        ids = order[bcls[order] == B]
+        per_row = max(1, mosaic_w // B)
+        sx[ids] = (np.arange(ids.size) % per_row) * B
+        sy[ids] = y0 + (np.arange(ids.size) // per_row) * B
+        y0 += -(-ids.size // per_row) * B
+    return sx, sy, y0
+
+
+def scene_cutouts(seed=SEEDS["c3"], n_isl=200_000, n_bands=10, k_max=64, device="cuda",
+                  mosaic_w=16384, box=5.0, w_min=15, w_max=95, order=4,
+                  max_profile_elems=1 << 27) -> Cutouts:
+    import torch
+    rng = np.random.default_rng(seed)
+    k = island_sizes(rng, n_isl, k_max)
+    n = int(k.sum())
+    starts = np.concatenate([[0], np.cumsum(k)[:-1]])
+    isl_of = np.repeat(np.arange(n_isl), k)
+    slot_of = np.arange(n) - starts[isl_of]
+    rad = 4.0 * np.sqrt(k[isl_of]) * np.sqrt(rng.uniform(0.0, 1.0, n))
+    th = rng.uniform(0.0, 2.0 * np.pi, n)
+    ux, uy = rad * np.cos(th), rad * np.sin(th)
+    w, mx, my = _window_sides(ux, uy, starts, box, w_min, w_max)
+    classes = np.asarray(ISLAND_BBOX_CLASSES)
+    bcls = classes[np.searchsorted(classes, w, side="left")]
+    sx, sy, H = _pack_slots(bcls, rng.permutation(n_isl), mosaic_w)
+    W = int(mosaic_w)
+    cx0 = sx + (bcls - w) // 2
+    cy0 = sy + (bcls - w) // 2
+    tx = cx0[isl_of] + 0.5 * (w[isl_of] - 1) + (ux - mx[isl_of])
+    ty = cy0[isl_of] + 0.5 * (w[isl_of] - 1) + (uy - my[isl_of])
+    # sources: the field scene's mix, fluxes, shapes and PSF
+    skies = rng.uniform(50.0, 500.0, n_bands)
+    fields = [MixturePsfField(H, W, (1.0, 2.0, 4.0), (0.7, 0.25, 0.05), amp=0.1, rng=rng)
+              for _ in range(n_bands)]
+    psfs = [f.fit_polynomial(55, order=order) for f in fields]
+    u = rng.uniform(0, 1, n)
+    st = np.where(u < 0.70, SOURCE_STAR, np.where(u < 0.85, SOURCE_EXP, np.where(
+        u < 0.95, SOURCE_DEV, SOURCE_SERSIC))).astype(np.int32)
+    sn = np.where(st == SOURCE_DEV, 4.0, np.where(st == SOURCE_SERSIC,
+                                                   rng.uniform(0.8, 6.0, n), 1.0))
+    sn = sn.astype(np.float32)
+    base = np.exp(rng.uniform(np.log(1e2), np.log(1e6), n))
+    flux = base[:, None] * rng.uniform(0.5, 2.0, (n, n_bands))
+    lr, e1, e2 = _galaxy_shapes(rng, n)
+    gal = st != SOURCE_STAR
+    lr = np.where(gal, lr, 0.0).astype(np.float32)
+    e1 = np.where(gal, e1, 0.0).astype(np.float32)
+    e2 = np.where(gal, e2, 0.0).astype(np.float32)
+    # per (source, band) PSF mixture and per-galaxy Fourier-free mixture
+    sig = np.stack([f.params(tx, ty)[0] for f in fields], axis=1)   # (n, nb, 3)
+    wgt = np.stack([f.params(tx, ty)[1] for f in fields], axis=1)
+    cxx, cyy, cxy = _cov_terms(lr.astype(np.float64), e1.astype(np.float64),
+                               e2.astype(np.float64))
+    K = 0
+    mog = {}
+    for v in np.unique(sn[gal]):
+        a, var, core = mog_params_at(float(v))
+        mog[float(v)] = (a, var, core)
+        K = max(K, a.size)
+    ga = np.zeros((n, K))
+    gv = np.zeros((n, K))
+    gcore = np.zeros(n)
+    for i in np.where(gal)[0]:
+        a, var, core = mog[float(sn[i])]
+        ga[i, :a.size], gv[i, :a.size], gcore[i] = a, var, core
+    pad = int(classes.max()) + 8
+    f64 = dict(dtype=torch.float64, device=device)
+    dstack = torch.zeros((n_bands, H + 2 * pad, W + 2 * pad), dtype=torch.float32, device=device)
+    wstack = torch.zeros_like(dstack)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    sky_t = torch.tensor(skies, **f64)
+    t_sig = torch.tensor(sig, **f64)
+    t_wgt = torch.tensor(wgt, **f64)
+    t_flux = torch.tensor(flux, **f64)
+    t_cov = torch.tensor(np.stack([cxx, cyy, cxy], axis=1), **f64)
+    t_ga, t_gv = torch.tensor(ga, **f64), torch.tensor(gv, **f64)
+    t_core = torch.tensor(gcore, **f64)
+    for wv in np.unique(w):
+        ids_all = np.where(w == wv)[0]
+        kmax = int(k[ids_all].max())
+        per = max(1, max_profile_elems // (n_bands * wv * wv * max(3 * K + 3, 1)))
+        for c0 in range(0, ids_all.size, per):
+            ids = ids_all[c0:c0 + per]
+            cells = torch.zeros((ids.size, n_bands, wv, wv), **f64)
+            pix = torch.arange(wv, **f64)
+            edges = torch.arange(wv + 1, **f64) - 0.5
+            for j in range(kmax):
+                has = k[ids] > j
+                if not has.any():
+                    break
+                loc = np.where(has)[0]
+                m = starts[ids[loc]] + j
+                lx = torch.tensor(tx[m] - cx0[ids[loc]], **f64)
+                ly = torch.tensor(ty[m] - cy0[ids[loc]], **f64)
+                s = t_sig[m]                                  # (q, nb, 3)
+                amp = t_flux[m][:, :, None] * t_wgt[m]        # (q, nb, 3)
+                prof = torch.zeros((loc.size, n_bands, wv, wv), **f64)
+                star = torch.tensor(st[m] == SOURCE_STAR, device=device)
+                if bool(star.any()):
+                    # pixel-integrated (erf) Gaussians, separable
+                    zx = (edges[None, None, None, :] - lx[:, None, None, None]) / (
+                        np.sqrt(2.0) * s[..., None])
+                    zy = (edges[None, None, None, :] - ly[:, None, None, None]) / (
+                        np.sqrt(2.0) * s[..., None])
+                    px = 0.5 * torch.diff(torch.special.erf(zx), dim=-1)   # (q, nb, 3, w)
+                    py = 0.5 * torch.diff(torch.special.erf(zy), dim=-1)
+                    sp = torch.einsum("qbk,qbky,qbkx->qbyx", amp, py, px)
+                    prof += sp * star[:, None, None, None]
+                gsel = ~star
+                if bool(gsel.any()):
+                    gi = torch.nonzero(gsel).flatten()
+                    mg = torch.as_tensor(m, device=device)[gi]
+                    ddx = pix[None, None, :] - lx[gi][:, None, None]     # (g, 1, w)
+                    ddy = pix[None, :, None] - ly[gi][:, None, None]     # (g, w, 1)
+                    acc = torch.zeros((gi.numel(), n_bands, wv, wv), **f64)
+                    cv = t_cov[mg]
+                    for kk in range(3):
+                        s2 = s[gi][:, :, kk] ** 2 + 1.0 / 12.0          # (g, nb)
+                        ampk = amp[gi][:, :, kk]
+                        r2 = (ddx ** 2 + ddy ** 2)[:, None]             # (g, 1, w, w)
+                        acc += (t_core[mg][:, None] * ampk / (2 * np.pi * s2))[..., None, None] \
+                            * torch.exp(-0.5 * r2 / s2[..., None, None])
+                        for q in range(K):
+                            aq = t_ga[mg, q]
+                            if not bool((aq != 0).any()):
+                                continue
+                            vq = t_gv[mg, q]
+                            c00 = vq[:, None] * cv[:, 0, None] + s2
+                            c11 = vq[:, None] * cv[:, 1, None] + s2
+                            c01 = (vq * cv[:, 2])[:, None].expand_as(s2)
+                            det = c00 * c11 - c01 * c01
+                            qf = (c11[..., None, None] * (ddx ** 2)[:, None]
+                                  - 2 * c01[..., None, None] * (ddx * ddy)[:, None]
+                                  + c00[..., None, None] * (ddy ** 2)[:, None]) \
+                                / det[..., None, None]
+                            acc += (aq[:, None] * ampk / (2 * np.pi * torch.sqrt(det)))[
+                                ..., None, None] * torch.exp(-0.5 * qf)
+                    prof[gi] += acc
+                cells.index_add_(0, torch.as_tensor(loc, device=device), prof)
+            # noise: Poisson(truth + sky) + N(0, 5^2) - sky; sky-only inverse variance
+            lam = torch.clamp(cells + sky_t[None, :, None, None], min=0.0)
+            d = torch.poisson(lam, generator=gen)
+            d += torch.randn(d.shape, generator=gen, **f64) * 5.0 - sky_t[None, :, None, None]
+            iv = (1.0 / (sky_t + 25.0))[None, :, None, None].expand_as(d)
+            rows = torch.as_tensor(pad + cy0[ids], device=device)[:, None] + torch.arange(
+                wv, device=device)[None, :]
+            cols = torch.as_tensor(pad + cx0[ids], device=device)[:, None] + torch.arange(
+                wv, device=device)[None, :]
+            r_ix, c_ix = rows[:, :, None], cols[:, None, :]
+            for b in range(n_bands):
+                dstack[b][r_ix, c_ix] = d[:, b].float()
+                wstack[b][r_ix, c_ix] = iv[:, b].float()
+    x, y, lr_s, e1_s, e2_s = _seed_catalog(rng, tx, ty, st, lr, e1, e2)
+    members = np.split(np.arange(n), starts[1:])
+    iset = IslandSet(n_sources=n, n_islands=n_isl, labels=isl_of.astype(np.int64),
+                     radius=np.full(n, float(box)), members=members, n_src=k,
+                     bbox=np.stack([cx0, cy0, cx0 + w, cy0 + w], axis=1).astype(np.int64),
+                     r_psf=27.5)
+    return Cutouts(dstack, wstack, pad, H, W, psfs, x, y, st, lr_s, e1_s, e2_s, sn, tx, ty, flux,
+                   iset, skies, meta=dict(config="c3", seed=seed, n_islands=n_isl,
+                                          n_bands=n_bands, k_max=k_max, slot_of=slot_of))
