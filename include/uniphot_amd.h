@@ -186,9 +186,10 @@ typedef struct uph_island_split {
   size_t hg_doubles;
   float* p_try;
   int32_t* counters;
-  void* const* eval_events; /* NULL, or 5 * (2 * max_steps + 2) hipEvent_t:
-                               per evaluation pass (chi2 at p0, then H/g and
-                               trial per step, then final fluxes) one at its
+  void* const* eval_events; /* NULL, or 5 * (2 * max_steps + 2 + post)
+                               hipEvent_t: per evaluation pass (chi2 at p0,
+                               then H/g and trial per step, then final
+                               fluxes, then the post-fit pass) one at its
                                start, one after the pre-render taps/spectra
                                launch, one after the extended-member planes,
                                one after the star planes (before the
@@ -218,6 +219,26 @@ typedef struct uph_island_split {
   size_t fspec_elems;
   double* gdt;                /* (n_pre, 9) scratch                            */
   float* gtap;                /* (n_pre, 28) scratch: Lanczos taps per pass    */
+  /* post-fit pass (post = 1; needs the pre-render): after the final fluxes,
+     one more evaluation at the converged parameters and a per-island
+     covariance launch (island_solver.py:1484-1715 final_errors,
+     final_flux_marg, source_metrics without priors):
+       post_off (n_isl): first double of the island's block of
+         nLc * (n_bands * N + nLc) doubles in post_buf (post_doubles);
+       post_var (member slots, 7): marginal variances of x, y, logre, ee1,
+         ee2, u and cov(ee1, ee2) (1 and 0 where a parameter is not live);
+       post_dvar: laid out like outputs->fluxes, the flux variance increment
+         of marginalising over extended members' parameters (0 for stars);
+       post_metrics (member slots, n_bands, 6): per band sum m, sum (f m)^2,
+         sum (model - f m) f m, sum m over zero-weight pixels,
+         sum m w (d - model)^2 and f, m the unit model on the window.     */
+  int post;
+  const int64_t* post_off;
+  double* post_buf;
+  size_t post_doubles;
+  float* post_var;
+  float* post_dvar;
+  float* post_metrics;
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);

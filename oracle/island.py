@@ -3,7 +3,8 @@
 Reference: ``tract7dt/fit/grid_gpu/island_solver.py`` -- ``models_band``
 (:480-562), ``_fluxes_band`` / ``residual_band`` / ``chi2_of``
 (:570-684), ``hg_scan`` (:812-865), legacy ``one_step`` (:921-1002),
-``final_fluxes`` (:1467-1482), ``final_errors`` (:1484-1547), and the
+``final_fluxes`` (:1467-1482), ``final_errors`` (:1484-1547), ``final_flux_marg``
+(:1549-1612), ``source_metrics`` (:1643-1715), and the
 chunk loop of ``run_class_chunk`` (:2890-2977).  Host input layout:
 ``island_plan.class_host_arrays`` (:493-699).
 
@@ -277,3 +278,86 @@ def final_errors(p, isl: Island, o: LMOptions):
     s6 = 6 * np.arange(N)
     cov = S[s6 + 3, s6 + 4]
     return np.concatenate([var6, cov[:, None]], axis=1)
+
+
+def flux_sensitivities(p, isl: Island, o: LMOptions, b: int):
+    """dF*/dtheta (N, L) of band b's VarPro fluxes over the live parameters:
+    the closed-form derivative of the flux solve, df = A^-1 (db - dA f),
+    with dA = e_k u^T + u e_k^T, u_l = sum w m_l t, db = e_k sum w t d."""
+    live = isl.live()
+    d, w = isl.data[b], isl.w[b]
+    m, tan = models_band(p, isl, b, True)
+    f, A64, _ = fluxes_band(m, d, w, isl.mmask, o.reg)
+    J = np.zeros((isl.N, len(live)))
+    for i, (k, j) in enumerate(live):
+        t = tan[(k, j)]
+        u = np.einsum("lij,ij->l", w[None] * m, t)
+        dA = np.zeros_like(A64)
+        dA[k, :] += u
+        dA[:, k] += u
+        db = np.zeros(isl.N)
+        db[k] = np.sum(w * t * d)
+        J[:, i] = np.linalg.solve(A64, db - dA @ f) * isl.mmask
+    return J, live
+
+
+def final_flux_marg(p, isl: Island, o: LMOptions):
+    """Per-band shape-marginalisation flux variance increment (nb, N)
+    (island_solver.py:1549-1612, no priors): diag(J Sigma J^T) over the
+    extended members' live parameters, Sigma the inverse of the GN Hessian
+    final_errors inverts; star rows and padded slots 0."""
+    Hl, _, _, live = hg(p, isl, o)
+    out = np.zeros((isl.nb, isl.N))
+    if not live:
+        return out
+    S = np.linalg.inv(Hl)
+    ext = (isl.stype != core.SOURCE_STAR) & isl.mmask
+    cols = np.array([ext[k] for (k, _) in live], np.float64)
+    for b in range(isl.nb):
+        J, _ = flux_sensitivities(p, isl, o, b)
+        Jm = J * cols[None]
+        dv = np.einsum("ka,ab,kb->k", Jm, S, Jm)
+        out[b] = np.where(ext, np.maximum(dv, 0.0), 0.0)
+    return out
+
+
+def source_metrics(p, isl: Island, o: LMOptions):
+    """Per-member (rchisq, fracflux, fracin, fracmasked) (N, 4) and the
+    per-band window flux f * sum(m) (nb, N) at p (island_solver.py:1643-
+    1715): unit models m on the window, VarPro fluxes f, model = sum f m;
+    per band fin = clip(sum m, 0, 1), fracflux ratio sum (model - f m) f m /
+    sum (f m)^2, masked fraction clip(sum over zero-weight pixels of m, 0, 1),
+    rchisq numerator sum m w (d - model)^2 over denominator |sum m|."""
+    N, nb = isl.N, isl.nb
+    z = np.zeros(N)
+    ff_n, ff_d, fin_n, fin_d, fm_n, rc_n, rc_d = (z.copy() for _ in range(7))
+    fwin = np.zeros((nb, N))
+    for b in range(nb):
+        d, w = isl.data[b], isl.w[b]
+        m, _ = models_band(p, isl, b, False)
+        f, _, _ = fluxes_band(m, d, w, isl.mmask, o.reg)
+        model = np.einsum("k,kij->ij", f, m)
+        own = f[:, None, None] * m
+        s_m = m.sum(axis=(1, 2))
+        fin = np.clip(s_m, 0.0, 1.0)
+        bot = (own * own).sum(axis=(1, 2))
+        top = ((model[None] - own) * own).sum(axis=(1, 2))
+        ratio = np.where(bot > 0, top / np.where(bot > 0, bot, 1.0), 0.0)
+        fm = np.clip(((w == 0.0)[None] * m).sum(axis=(1, 2)), 0.0, 1.0)
+        rc = (m * (w * (d - model) ** 2)[None]).sum(axis=(1, 2))
+        ff_n += fin * ratio
+        ff_d += fin
+        fin_n += np.abs(s_m)
+        fin_d += 1.0
+        fm_n += fin * fm
+        rc_n += np.abs(rc)
+        rc_d += np.abs(s_m)
+        fwin[b] = f * s_m
+    mm = isl.mmask
+    with np.errstate(divide="ignore", invalid="ignore"):
+        fracflux = np.where((ff_d > 0) & mm, ff_n / ff_d, 0.0)
+        fracin = np.where((fin_d > 0) & mm, fin_n / fin_d, np.nan)
+        fracmasked = np.where((ff_d > 0) & mm, fm_n / ff_d, 0.0)
+        rchisq = np.where((rc_d > 0) & mm, rc_n / rc_d, np.nan)
+    return np.stack([rchisq, fracflux, fracin, fracmasked], axis=1), np.where(mm[None], fwin,
+                                                                             np.nan)

@@ -152,3 +152,91 @@ def test_member_prerender_equals_inplace_render(cuda):
             fa, fb = ra["fluxes"][r][:, mm], rb["fluxes"][r][:, mm]
             scale = np.maximum(np.abs(fa), ra["ferr"][r][:, mm])
             assert (np.abs(fa - fb) / scale).max() < 1e-5
+
+
+def test_split_sequence_graph_replays_equal_eager(cuda):
+    """The band-split sequence (member pre-render included) captured once
+    into a HIP graph: three replays, with eager fits of the same batch on
+    other buffers between them, each reproduce the eager result bit for
+    bit."""
+    import torch
+    from uniphot_amd.island_solver import (IslandBatchGraph, make_class_kernels,
+                                           psf_tree_device, run_island_batch)
+    sc = small_star_scene(seed=3, n_src=24)
+    plans, hosts, pad, d, w = star_classes(sc)
+    dd, ww = torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda)
+    tree = psf_tree_device(sc.psfs, 55, cuda)
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms), h, np.arange(p.n_islands))
+             for p, h in zip(plans, hosts)]
+    ref = run_island_batch(batch, {55: tree}, 30, dd, ww, split=True, prerender=True)
+    print("capturing the split sequence", flush=True)
+    g = IslandBatchGraph(batch, {55: tree}, 30, dd, ww)
+    for rep in range(3):
+        g.replay()
+        got = g.collect()
+        print("replay", rep, flush=True)
+        for ra, rb in zip(ref, got):
+            for k in ("p", "fluxes", "ferr", "chi2", "nst", "conv", "stall", "n_good"):
+                assert np.array_equal(ra[k], rb[k]), (rep, k)
+        eager = run_island_batch(batch, {55: tree}, 30, dd, ww, split=True, prerender=True)
+        torch.cuda.synchronize()
+        for ra, rb in zip(ref, eager):
+            assert np.array_equal(ra["fluxes"], rb["fluxes"])
+
+
+@pytest.mark.parametrize("kind", ["star", "mixed"])
+def test_post_fit_pass_matches_oracle(cuda, kind):
+    """Post-fit pass at the converged parameters: marginal parameter
+    variances and cov(ee1, ee2), the shape-marginalisation flux variance
+    increment and the per-source metrics against the float64 oracle
+    evaluated at the same (device) parameters."""
+    import torch
+    from _scenes import mixed_classes, small_mixed_scene
+    from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
+    from uniphot_amd.pipeline import metrics_from_sums
+    from uniphot_amd.sersic_table import device_table
+    if kind == "star":
+        sc = small_star_scene()
+        plans, hosts, pad, d, w = star_classes(sc)
+    else:
+        sc = small_mixed_scene(seed=5, n_bands=2, n_src=6, order=1, types=(0, 1, 3))
+        plans, hosts, pad, d, w = mixed_classes(sc, free_n=np.asarray(sc.stype == 3))
+    dd, ww = torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda)
+    trees = {p.tier: psf_tree_device(sc.psfs, p.tier, cuda) for p in plans}
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms), h, np.arange(p.n_islands))
+             for p, h in zip(plans, hosts)]
+    print("post-fit", kind, [(p.kind, p.nsrc_class, p.n_islands) for p in plans], flush=True)
+    res = run_island_batch(batch, trees, 30, dd, ww, split=True, prerender=True, post=True,
+                           sersic_dev=device_table(cuda) if kind == "mixed" else None)
+    o = oisl.LMOptions()
+    worst = dict(var=0.0, dvar=0.0, met=0.0)
+    for (plan, host), rr in zip(zip(plans, hosts), res):
+        for r in range(plan.n_islands):
+            isl = oracle_island(plan, host, r, d, w, sc.psfs)
+            p = rr["p"][r].ravel().astype(np.float64)
+            mm = host["mmask"][r]
+            live = isl.live()
+            ev = oisl.final_errors(p, isl, o)
+            gv = rr["post_var"][r]
+            for (k, j) in live:
+                e = abs(gv[k, j] - ev[k, j]) / abs(ev[k, j])
+                worst["var"] = max(worst["var"], e)
+            assert np.all(gv[~mm, :6] == 1.0)
+            dv = oisl.final_flux_marg(p, isl, o)
+            _, ferr = oisl.final_fluxes(p, isl, o)
+            gd = rr["post_dvar"][r]
+            scale = np.maximum(dv, 1e-3 * ferr ** 2)
+            worst["dvar"] = max(worst["dvar"], float((np.abs(gd - dv) / scale)[:, mm].max()))
+            assert np.all(gd[:, mm][:, isl.stype[mm] == 0] == 0.0)
+            met, fwin = oisl.source_metrics(p, isl, o)
+            gm, gw = metrics_from_sums(rr["post_metrics"][r])
+            for q in range(4):
+                a, b = gm[mm, q], met[mm, q]
+                sc_q = np.maximum(np.abs(b), 1e-3)
+                worst["met"] = max(worst["met"], float((np.abs(a - b) / sc_q).max()))
+            np.testing.assert_allclose(gw.T[:, mm], fwin[:, mm], rtol=1e-4,
+                                       atol=1e-4 * np.abs(fwin[:, mm]).max())
+    print("worst relative errors", worst, flush=True)
+    assert worst["var"] < 2e-3
+    assert worst["dvar"] < 2e-3
+    assert worst["met"] < 1e-3

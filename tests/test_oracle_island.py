@@ -195,3 +195,82 @@ def test_final_errors_invert_the_finite_difference_fisher():
     C = np.linalg.inv(J @ J.T)
     for i, (k, j) in enumerate(live):
         assert ev[k, j] == pytest.approx(C[i, i], rel=1e-3)
+
+
+def _fluxes_of(p, isl, o, b):
+    m, _ = oisl.models_band(p, isl, b, False)
+    f, _, _ = oisl.fluxes_band(m, isl.data[b], isl.w[b], isl.mmask, o.reg)
+    return f
+
+
+def test_flux_marginalisation_uses_finite_difference_flux_sensitivities():
+    """final_flux_marg = diag(J Sigma J^T) with J the flux solve's
+    sensitivity to the extended members' parameters: J from central finite
+    differences of the fluxes, Sigma from the finite-difference Fisher
+    matrix; star rows exactly 0."""
+    sc = small_mixed_scene(seed=5, n_bands=2, n_src=6, order=1, types=(0, 1, 3))
+    plans, hosts, pad, d, w = mixed_classes(sc, free_n=np.asarray(sc.stype == 3))
+    best = max(((pi, r) for pi, pl in enumerate(plans) for r in range(pl.n_islands)
+                if pl.kind == "mixed"), key=lambda t: int(hosts[t[0]]["keep"][t[1]].sum()))
+    plan, host = plans[best[0]], hosts[best[0]]
+    isl = oracle_island(plan, host, best[1], d, w, sc.psfs)
+    o = oisl.LMOptions()
+    p = _away_from_half(host["p0"][best[1]].ravel().astype(np.float64), isl)
+    live = isl.live()
+    ext = (isl.stype != core.SOURCE_STAR) & isl.mmask
+    assert ext.any() and (isl.mmask & ~ext).any() or ext.any()
+    r0 = _varpro_residual(p, isl, o)
+    Jr = np.zeros((len(live), r0.size))
+    Jf = np.zeros((isl.nb, isl.N, len(live)))
+    for i, (k, j) in enumerate(live):
+        pp, pm = p.copy(), p.copy()
+        pp[6 * k + j] += 1e-5
+        pm[6 * k + j] -= 1e-5
+        Jr[i] = (_varpro_residual(pp, isl, o) - _varpro_residual(pm, isl, o)) / 2e-5
+        for b in range(isl.nb):
+            Jf[b, :, i] = (_fluxes_of(pp, isl, o, b) - _fluxes_of(pm, isl, o, b)) / 2e-5
+    S = np.linalg.inv(Jr @ Jr.T)
+    cols = np.array([ext[k] for (k, _) in live], np.float64)
+    dv = oisl.final_flux_marg(p, isl, o)
+    for b in range(isl.nb):
+        J, _ = oisl.flux_sensitivities(p, isl, o, b)
+        np.testing.assert_allclose(J, Jf[b], rtol=1e-4, atol=1e-6 * np.abs(Jf[b]).max())
+        Jm = Jf[b] * cols[None]
+        ref = np.where(ext, np.einsum("ka,ab,kb->k", Jm, S, Jm), 0.0)
+        np.testing.assert_allclose(dv[b], ref, rtol=2e-3, atol=1e-9 * np.abs(ref).max())
+        assert np.all(dv[b][~ext] == 0.0)
+
+
+def test_source_metrics_closed_forms():
+    """Isolated member: fracflux 0; data equal to the model: rchisq 0; one
+    band: fracin = |sum m|, window flux f sum m; zero weight on a pixel set:
+    fracmasked is the clipped unit-model sum over it."""
+    sc = small_star_scene(seed=4, n_bands=1, n_src=4, order=1)
+    plans, hosts, pad, d, w = star_classes(sc)
+    pi, r = 0, 0
+    isl = oracle_island(plans[pi], hosts[pi], r, d, w, sc.psfs)
+    k0 = int(np.where(isl.mmask)[0][0])
+    isl.mmask[:] = False                        # every other member a padded slot
+    isl.mmask[k0] = True
+    isl.keep &= isl.mmask[:, None]
+    o = oisl.LMOptions()
+    p = _away_from_half(hosts[pi]["p0"][r].ravel().astype(np.float64), isl)
+    k = int(np.where(isl.mmask)[0][0])
+    m, _ = oisl.models_band(p, isl, 0, False)
+    isl.data[0] = 3e4 * m[k]
+    isl.w[0] = np.full_like(isl.w[0], 0.01)
+    met, fwin = oisl.source_metrics(p, isl, o)
+    assert met[k, 1] == 0.0                     # fracflux of an isolated member
+    # rchisq of an exact model: only the ridge of the flux solve remains,
+    # f = b / (A + reg), R = 3e4 m reg / (A + reg)
+    A = 0.01 * np.sum(m[k] ** 2)
+    c = 3e4 * o.reg / (A + o.reg)
+    assert met[k, 0] == pytest.approx(0.01 * c * c * np.sum(m[k] ** 3) / abs(m[k].sum()),
+                                      rel=1e-6)
+    assert fwin[0, k] == pytest.approx((3e4 - c) * m[k].sum(), rel=1e-9)
+    assert met[k, 2] == pytest.approx(abs(m[k].sum()), rel=1e-12)   # fracin, one band
+    isl.w[0][:, : isl.B // 2] = 0.0             # masked: the left half of the window
+    met, _ = oisl.source_metrics(p, isl, o)
+    expect = np.clip(m[k][:, : isl.B // 2].sum(), 0.0, 1.0)
+    assert met[k, 3] == pytest.approx(expect, rel=1e-12)
+    assert 0.0 < expect < 1.0

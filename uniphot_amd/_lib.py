@@ -76,7 +76,9 @@ class IslandSplit(C.Structure):
                 ("eval_events", _P), ("n_pre", C.c_int), ("n_pre_ext", C.c_int),
                 ("pre_isl", _P), ("pre_slot", _P), ("pre_off", _P), ("pre_spec", _P),
                 ("pre_of_slot", _P), ("pbuf", _P), ("pbuf_elems", C.c_size_t), ("fspec", _P),
-                ("fspec_elems", C.c_size_t), ("gdt", _P), ("gtap", _P)]
+                ("fspec_elems", C.c_size_t), ("gdt", _P), ("gtap", _P), ("post", C.c_int),
+                ("post_off", _P), ("post_buf", _P), ("post_doubles", C.c_size_t),
+                ("post_var", _P), ("post_dvar", _P), ("post_metrics", _P)]
 
 
 _lib = None

@@ -977,6 +977,52 @@ struct EvalArgs<true> {
   using type = IslandArgsPre;
 };
 
+// Post-fit pass, one band of one island at the converged parameters
+// (island_solver.py:1549-1612 final_flux_marg, :1643-1715 source_metrics):
+// the flux sensitivities dF/dtheta_j = D_j = A^-1 (e_k s_j - f_k U_j) of the
+// live parameters (padded slots 0) into the island's post block, and per
+// real member the band's metric sums over the window: sum m (the unit
+// model), sum (f m)^2, sum (model - f m) f m, sum of m on zero-weight
+// pixels, sum m w (d - model)^2, and f.
+template <typename Real>
+__device__ void post_band(const IslandArgsPre& a, IslandSh<Real>& sh, int isl, int b,
+                          const Real* planes, const Small& sm) {
+  const int tid = threadIdx.x, P = sh.B * sh.B, Nr = sh.Nr, nL = sh.nL, N = sh.N;
+  const Real* Wp = planes;
+  const Real* Rp = planes + P;
+  const Real* Dp = planes + 2 * P;
+  const Real* Mp = planes + 3 * P;
+  double* D = a.post_buf + a.post_off[isl] + (size_t)b * sh.nLc * N;
+  for (int e = tid; e < nL * N; e += UPH_BLOCK) D[e] = 0.0;
+  __syncthreads();
+  for (int e = tid; e < nL * Nr; e += UPH_BLOCK) {
+    const int j = e / Nr, r = e % Nr;
+    D[(size_t)j * N + sh.mem[r]] = sm.D[(size_t)j * Nr + r];
+  }
+  for (int r = 0; r < Nr; ++r) {
+    const double f = double(sh.fR[r]);
+    const Real* M = Mp + (size_t)r * P;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int p = tid; p < P; p += UPH_BLOCK) {
+      const double m = double(M[p]), w = double(Wp[p]), R = double(Rp[p]);
+      const double own = f * m, model = double(Dp[p]) - R;
+      v[0] += m;
+      v[1] += own * own;
+      v[2] += (model - own) * own;
+      v[3] += w == 0.0 ? m : 0.0;
+      v[4] += m * w * R * R;
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) v[q] = block_sum(v[q], sh.redd);
+    if (tid == 0) {
+      float* o = a.post_metrics + ((size_t)(sh.moff + sh.mem[r]) * a.nb + b) * 6;
+      for (int q = 0; q < 5; ++q) o[q] = float(v[q]);
+      o[5] = float(f);
+    }
+  }
+  __syncthreads();
+}
+
 template <typename Real, bool kPre>
 __global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(typename EvalArgs<kPre>::type a) {
   __shared__ IslandSh<Real> sh;
@@ -1032,7 +1078,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(typename EvalArg
                       &ngood);
       if (tid == 0) atomicAdd(a.ngood_out + isl, ngood);
     } else {
-      const bool grads = mode == kSplitEvalGrad;
+      const bool grads = split_grads(mode);
       const Real c = eval_band<Real, kPre>(a, sh, isl, b, sh.p, grads, false, planes, ws, gws, G, part,
                                      sm, nullptr, nullptr, nullptr);
       if (grads) {
@@ -1040,6 +1086,9 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(typename EvalArg
         for (int j = tid; j < nL; j += UPH_BLOCK) slot[(size_t)nL * nL + j] = sm.g[j];
       }
       if (tid == 0) slot[(size_t)nL * nL + nL] = double(c);
+      if constexpr (kPre) {
+        if (mode == kSplitEvalPost) post_band<Real>(a, sh, isl, b, planes, sm);
+      }
     }
     __syncthreads();
   }
@@ -1138,6 +1187,77 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
   }
   __syncthreads();
   for (int q = tid; q < nP; q += UPH_BLOCK) a.p_out[m0 * 6 + q] = sh.p[q];
+}
+
+// Post-fit covariance, one workgroup per island (island_solver.py:1484-1547
+// final_errors without priors, :1549-1612 final_flux_marg): Sigma = inverse
+// of the un-damped flux-marginalised GN Hessian summed over bands (f64 LU),
+// the marginal variances of the live parameters and cov(ee1, ee2) per
+// member slot (1 and 0 where a parameter is not live), and per band the
+// flux variance increment diag(D Sigma D^T) of each extended member over
+// the extended members' live parameters (0 for stars).
+__global__ __launch_bounds__(UPH_BLOCK) void island_post_kernel(IslandArgsPre a) {
+  __shared__ IslandSh<float> sh;
+  __shared__ int piv[kMaxLive];
+  __shared__ int s_p;
+  const int tid = threadIdx.x, isl = blockIdx.x, nb = a.nb;
+  if (tid == 0) read_shape(a, sh, isl);
+  __syncthreads();
+  describe_island(a, sh, a.p_out);
+  const int N = sh.N, nL = sh.nL;
+  const size_t blk = split_band_block(sh.nLc);
+  const size_t off0 = (size_t)a.hg_off[isl];
+  const size_t poff = (size_t)a.post_off[isl];
+  if (off0 + (size_t)nb * blk > a.hg_doubles || nL > sh.nLc ||
+      poff + post_block(nb, sh.nLc, N) > a.post_doubles) {
+    if (tid == 0) atomicOr(a.err, 1);
+    return;  // uniform
+  }
+  for (int e = tid; e < N * 7; e += UPH_BLOCK)
+    a.post_var[(size_t)sh.moff * 7 + e] = e % 7 == 6 ? 0.0f : 1.0f;
+  for (int e = tid; e < nb * N; e += UPH_BLOCK) a.post_dvar[(size_t)sh.foff + e] = 0.0f;
+  if (nL == 0 || sh.Nr == 0) return;  // uniform
+  double* h0 = a.hg + off0;
+  const double* D = a.post_buf + poff;
+  double* S = a.post_buf + poff + (size_t)nb * sh.nLc * N;
+  __syncthreads();
+  for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
+    double v = h0[e];
+    for (int b = 1; b < nb; ++b) v = v + h0[(size_t)b * blk + e];
+    h0[e] = v;
+  }
+  __syncthreads();
+  block_lu(h0, nL, piv, &s_p);
+  for (int j = tid; j < nL; j += UPH_BLOCK) {
+    double* x = S + (size_t)j * nL;  // column j of the (symmetric) inverse
+    for (int i = 0; i < nL; ++i) x[i] = i == j ? 1.0 : 0.0;
+    lu_solve_one(h0, piv, nL, x);
+  }
+  __syncthreads();
+  for (int l = tid; l < nL; l += UPH_BLOCK) {
+    const int r = sh.live_r[l], q = sh.live_j[l];
+    float* o = a.post_var + ((size_t)sh.moff + sh.mem[r]) * 7;
+    o[q] = float(S[(size_t)l * nL + l]);
+    if (q == 3)
+      for (int m = sh.first_live[r]; m < sh.first_live[r + 1]; ++m)
+        if (sh.live_j[m] == 4) o[6] = float(S[(size_t)l * nL + m]);
+  }
+  for (int e = tid; e < nb * sh.Nr; e += UPH_BLOCK) {
+    const int b = e / sh.Nr, r = e % sh.Nr;
+    if (sh.st[r] == 0) continue;  // star rows carry no shape marginalisation
+    const double* Db = D + (size_t)b * sh.nLc * N;
+    const int k = sh.mem[r];
+    double dv = 0.0;
+    for (int i = 0; i < nL; ++i) {
+      if (sh.st[sh.live_r[i]] == 0) continue;
+      const double di = Db[(size_t)i * N + k];
+      double t = 0.0;
+      for (int c = 0; c < nL; ++c)
+        if (sh.st[sh.live_r[c]] != 0) t += S[(size_t)i * nL + c] * Db[(size_t)c * N + k];
+      dv += di * t;
+    }
+    a.post_dvar[(size_t)sh.foff + (size_t)b * N + k] = float(fmax(dv, 0.0));
+  }
 }
 
 }  // namespace uph
@@ -1318,6 +1438,13 @@ extern "C" int uph_island_fit_batch(const uph_island_batch* bat, const uph_islan
                        counter, stream);
 }
 
+namespace uph {
+// dst[0, n) = v, one workgroup.
+__global__ __launch_bounds__(UPH_BLOCK) void fill_i32_kernel(int32_t* dst, int n, int v) {
+  for (int i = threadIdx.x; i < n; i += UPH_BLOCK) dst[i] = v;
+}
+}  // namespace uph
+
 template <typename Real>
 static void split_eval(uph::IslandArgs a, int mode, int* counter, int nblk, hipStream_t s) {
   a.mode = mode;
@@ -1381,10 +1508,15 @@ static void split_sequence(const uph::IslandArgs& a, const uph::IslandArgsPre* p
     split_step<Real>(a, uph::kSplitStepAccept, s);
   }
   split_pass<Real>(a, pre, uph::kSplitEvalFinal, counters, nblk, s, ev, k);
+  if (pre && pre->post) {
+    split_pass<Real>(a, pre, uph::kSplitEvalPost, counters, nblk, s, ev, k);
+    hipLaunchKernelGGL(uph::island_post_kernel, dim3(a.n_isl), dim3(UPH_BLOCK), 0, s, *pre);
+  }
 }
 
 extern "C" size_t uph_island_split_counters(int max_steps) {
-  const size_t passes = (size_t)(2 * (max_steps > 0 ? max_steps : 0) + 2);
+  // evaluation passes: initial, two per step, final, post-fit
+  const size_t passes = (size_t)(2 * (max_steps > 0 ? max_steps : 0) + 3);
   return 2 * passes + 1;  // evaluation + planes queue per pass, then the error flag
 }
 
@@ -1424,6 +1556,8 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
   a.err = sp->counters + nc - 1;
   uph::IslandArgsPre pre{};
   const bool use_pre = sp->n_pre > 0;
+  UPH_CHECK_ARG(!sp->post || use_pre,
+                "uph_island_fit_split: the post-fit pass needs the member pre-render");
   if (use_pre) {
     UPH_CHECK_ARG(precision == UPH_PRECISION_FP32,
                   "uph_island_fit_split: the member pre-render is float32 only");
@@ -1452,10 +1586,25 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
     pre.fspec_elems = sp->fspec_elems;
     pre.gdt = sp->gdt;
     pre.gtap = sp->gtap;
+    pre.post = sp->post;
+    if (sp->post) {
+      UPH_CHECK_ARG(sp->post_off && sp->post_buf && sp->post_var && sp->post_dvar &&
+                        sp->post_metrics,
+                    "uph_island_fit_split: null post-fit arrays");
+      pre.post_off = sp->post_off;
+      pre.post_buf = sp->post_buf;
+      pre.post_doubles = sp->post_doubles;
+      pre.post_var = sp->post_var;
+      pre.post_dvar = sp->post_dvar;
+      pre.post_metrics = sp->post_metrics;
+    }
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
-  st = uph::hip_status(hipMemsetAsync(sp->counters, 0, nc * sizeof(int32_t), s),
-                       "uph_island_fit_split memset");
+  // counters cleared by a kernel, not a memset node, so that the whole
+  // sequence can be captured into a graph and replayed
+  hipLaunchKernelGGL(uph::fill_i32_kernel, dim3(1), dim3(UPH_BLOCK), 0, s, sp->counters,
+                     (int)nc, 0);
+  st = uph::hip_status(hipGetLastError(), "uph_island_fit_split counter reset");
   if (st != UPH_OK) return st;
   if (precision == UPH_PRECISION_FP32)
     split_sequence<float>(a, use_pre ? &pre : nullptr, sp->counters, nblk, s, sp->eval_events);

@@ -253,7 +253,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void prerender_spectra_kernel(IslandArgs
   __syncthreads();
   if (s_skip) return;
   const int T = s_T, pad = fft_pad_for(T), H = pad / 2 + 1, n = H * pad;
-  const bool grads = mode == kSplitEvalGrad;
+  const bool grads = split_grads(mode);
   float* F = a.fspec + a.pre_spec[g];
   const double c2 = -2.0 * 9.869604401089358;  // -2 pi^2
   for (int e = tid; e < n; e += UPH_BLOCK) {
@@ -573,7 +573,7 @@ __device__ __forceinline__ bool pre_item(const IslandArgsPre& a, int item, bool 
   it.side = pre_side(T, B);
   it.slot = (size_t)a.mem_off[it.isl] + a.pre_slot[it.g];
   int need = 0;
-  if (mode == kSplitEvalGrad)
+  if (split_grads(mode))
     for (int q = 0; q < 6; ++q)
       if (a.keep[it.slot * 6 + q]) need |= 1 << q;
   it.need = star ? need & 3 : need;

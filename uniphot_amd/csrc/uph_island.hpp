@@ -89,6 +89,15 @@ struct IslandArgsPre : IslandArgs {
   float* gtap;                 // (n_pre, 28) Lanczos taps and tap derivatives, x then y
   size_t star_stride;          // per-workgroup scratch bytes of the star launch
   int* pre_counter;            // work queue of the planes kernel
+  // post-fit pass at the converged parameters (post != 0):
+  int post;
+  const int64_t* post_off;     // (n_isl) first double of the island's block in post_buf:
+                               // D (n_bands, nLc, N) then Sigma (nLc, nLc)
+  double* post_buf;
+  size_t post_doubles;
+  float* post_var;             // (member slots, 7) var x, y, logre, ee1, ee2, u, cov(ee1, ee2)
+  float* post_dvar;            // flux layout: shape-marginalisation variance increment
+  float* post_metrics;         // (member slots, n_bands, 6) per-band metric sums
 };
 
 // Band-split phases.  Evaluation launches run one (island, band) item per
@@ -100,6 +109,18 @@ constexpr int kSplitEvalFinal = 3;  // fluxes, flux errors, good-pixel count at 
 constexpr int kSplitStepInit = 4;   // LM state from the initial chi2
 constexpr int kSplitStepSolve = 5;  // damped solve -> p_try
 constexpr int kSplitStepAccept = 6; // accept / reject, convergence
+constexpr int kSplitEvalPost = 7;   // H, g, flux sensitivities and metric sums at p (converged)
+
+// Passes that need the tangent planes.
+__host__ __device__ inline bool split_grads(int mode) {
+  return mode == kSplitEvalGrad || mode == kSplitEvalPost;
+}
+
+// Doubles of one island's post-fit block: the per-band flux sensitivities
+// D (n_bands, nLc, N) then the parameter covariance (nLc, nLc).
+__host__ __device__ inline size_t post_block(int nb, int nLc, int N) {
+  return (size_t)nLc * ((size_t)nb * N + nLc);
+}
 
 // Planes per band of a pre-rendered member.
 __host__ __device__ inline int pre_planes(bool star) { return star ? 3 : 7; }

@@ -248,28 +248,36 @@ def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prer
                 grad_planes=np.where(star, 1 + live_xy, 1 + live).astype(np.float64))
 
 
-def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
-                     precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
-                     split: bool = False, prerender: bool = True):
-    """Fit the islands of several classes in ONE persistent launch.
+def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
+                         precision: str = "fp32", sersic_dev=None, timer=None,
+                         split: bool = False, prerender: bool = True,
+                         post: bool = False) -> "PreparedIslandBatch":
+    """Stage the islands of several classes for ONE launch sequence.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
     maps stamp tier -> PsfTree.  Islands are ordered longest-first
-    (members x window area) for the work queue.  Returns one result dict
-    per class, in the same layout as :func:`run_class_chunk`.
+    (members x window area) for the work queue.  The returned batch's
+    ``collect()`` gives one result dict per class, in the same layout as
+    :func:`run_class_chunk`.
 
-    ``split=True`` runs the band-split form (``uph_island_fit_split``):
-    every evaluation pass is spread over (island, band) items, so one large
-    island no longer sets the wall time; results are bit-identical.  With
-    ``prerender`` (float32 only) every member is rendered and shifted by
-    separate (member, band) launches before each pass (extended members
-    through LDS wave-level FFTs) and the evaluation only places the planes;
-    results then agree with the in-place render to float32 rounding.
+    ``split=False`` is the persistent form (``uph_island_fit_batch``, one
+    workgroup per island).  ``split=True`` runs the band-split form
+    (``uph_island_fit_split``): every evaluation pass is spread over
+    (island, band) items, so one large island no longer sets the wall time;
+    results are bit-identical.  With ``prerender`` (float32 only) every
+    member is rendered and shifted by separate (member, band) launches
+    before each pass (extended members through LDS wave-level FFTs) and the
+    evaluation only places the planes; results then agree with the in-place
+    render to float32 rounding.  A ``timer`` gets per-launch events of the
+    split sequence.  ``post`` (split + pre-render only) adds the post-fit
+    pass: per-member parameter variances (``post_var`` (n, N, 7)), the
+    shape-marginalisation flux variance increment (``post_dvar`` (n, nb, N))
+    and the per-band metric sums (``post_metrics`` (n, N, nb, 6)).
     """
     import torch
     t0 = time.perf_counter()
     if not classes:
-        return []
+        return PreparedIslandBatch(lambda stream=None: None, lambda: None, lambda: [])
     k0 = classes[0][0]
     nb, nt = k0.n_bands, k0.n_terms
     if any(k.kind != "star" for k, _, _ in classes) and sersic_dev is None:
@@ -374,7 +382,6 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         "p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
         "n_good")))
     opts = k0.options(max_steps)
-    ev = timer.begin("island_lm_kernel") if timer is not None else None
     if split:
         nlc = np.asarray([s_[3] for s_ in shape], np.int64)
         blk = nlc * nlc + nlc + 2
@@ -389,6 +396,10 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
         d_items, d_bands = up(items, np.int32), up(bands, np.int32)
         d_hgoff = up(hg_off, np.int64)
         n_eval = 2 * int(max_steps) + 2
+        gal = _prerender_arrays(order, shape, moff, c, tiers, nb, m_acc, precision, prerender,
+                                dev, up)
+        post_on = bool(post and gal["args"][0] > 0)
+        n_eval += int(post_on)
         evs = None
         ev_arr = None
         if timer is not None:
@@ -397,112 +408,209 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
             for e in evs:
                 e.record()
             ev_arr = (C.c_void_p * len(evs))(*[e.cuda_event for e in evs])
-        gal = _prerender_arrays(order, shape, moff, c, tiers, nb, m_acc, precision, prerender,
-                                dev, up)
+        post_args = (0, None, None, 0, None, None, None)
+        if post_on:
+            Ns = np.asarray([s_[0] for s_ in shape], np.int64)
+            pblk = nlc * (nb * Ns + nlc)
+            post_off = np.concatenate([[0], np.cumsum(pblk)[:-1]]).astype(np.int64)
+            post_t = dict(off=up(post_off, np.int64),
+                          buf=torch.empty(max(int(pblk.sum()), 1), dtype=torch.float64,
+                                          device=dev),
+                          var=torch.full((m_acc * 7,), float("nan"), **f32),
+                          dvar=torch.full((f_acc,), float("nan"), **f32),
+                          metrics=torch.full((m_acc * nb * 6,), float("nan"), **f32))
+            post_args = (1, P(post_t["off"]), P(post_t["buf"]), int(post_t["buf"].numel()),
+                         P(post_t["var"]), P(post_t["dvar"]), P(post_t["metrics"]))
         sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
                               P(p_try), P(counters),
                               C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None,
-                              *gal["args"])
-        _lib.check(L.uph_island_fit_split(C.byref(bat), C.byref(inp), C.byref(out),
-                                          C.byref(opts), prec, C.byref(sp), P(scratch),
-                                          scratch.numel(), per, _lib.stream_ptr(stream)))
+                              *gal["args"], *post_args)
     else:
-        _lib.check(L.uph_island_fit_batch(C.byref(bat), C.byref(inp), C.byref(out),
-                                          C.byref(opts), prec, P(scratch), scratch.numel(), per,
-                                          P(counter), _lib.stream_ptr(stream)))
+        evs = None
+        post_on = False
+
+    def launch(stream=None):
+        if split:
+            _lib.check(L.uph_island_fit_split(C.byref(bat), C.byref(inp), C.byref(out),
+                                              C.byref(opts), prec, C.byref(sp), P(scratch),
+                                              scratch.numel(), per, _lib.stream_ptr(stream)))
+        else:
+            _lib.check(L.uph_island_fit_batch(C.byref(bat), C.byref(inp), C.byref(out),
+                                              C.byref(opts), prec, P(scratch), scratch.numel(),
+                                              per, P(counter), _lib.stream_ptr(stream)))
+
     area = np.concatenate([h["area"][np.asarray(r)] for _, h, r in classes])
     mmc = np.concatenate([h["mmask"][np.asarray(r)].sum(axis=1) for _, h, r in classes])
+
+    def timer_info():
+        return dict(kind="island_batch", n=n_isl, nb=nb, outs=outs, area=area, nreal=mmc,
+                    name="island_split_sequence" if split else None)
+
+    def collect():
+        return _collect()
+
+    def _collect():
+        h = {k: v.cpu().numpy() for k, v in outs.items()}
+        if split and int(counters[-1].item()) != 0:
+            raise _lib.UphError(_lib.UPH_ERR_INTERNAL,
+                                "island band blocks overflowed their buffer")
+        if split and evs is not None:
+            # Algorithmic cost of every launch of the sequence, recorded with
+            # its events.  Bytes are the unique HBM bytes the launch must move;
+            # flops count multiply-adds as 2 and a complex FFT of n points as
+            # 5 n log2 n.
+            #   evaluation: data + weight of every island box it evaluates (INIT
+            #     and FINAL: all islands; the H/g and trial launches of step it:
+            #     the islands that took step it); per band the Gram of the
+            #     1 + N + nL planes, the residual and its projections.
+            #   taps/spectra: spectra written (all three on H/g passes, F alone
+            #     otherwise) and 28 taps per member.
+            #   extended planes: PSF coefficient table once, spectra read, window
+            #     part of the planes written; per (member, band) the polynomial
+            #     over the stamp, forward transforms of P (and dP/dx, dP/dy),
+            #     one spectral product and inverse transform per output plane,
+            #     the 7-tap shift of the window part.
+            #   stars: coefficient table once, planes written; the polynomial
+            #     over the shift-read box and the shift.
+            nst = h["nst"].astype(np.int64)
+            box = nb * area.astype(np.float64) * 8.0
+            kp = c["keep"].reshape(-1, 6).astype(np.float64) * c["mm"].reshape(-1, 1)
+            nlive = np.asarray([kp[moff[i]:moff[i] + shape[i][0]].sum() for i in range(n_isl)])
+            npix = area.astype(np.float64)
+            nr = mmc.astype(np.float64)
+            kinds = ["init"] + ["grad", "try"] * int(max_steps) + ["final"] + ["post"] * post_on
+            g_isl = gal["isl"]
+            coef_tbl = {T: nb * nt * T * T * 4.0 for T in tiers}
+
+            def fft_cost(n):
+                return 5.0 * n * np.log2(n)
+
+            for k, kind in enumerate(kinds):
+                it = (k - 1) // 2
+                sel = np.ones(n_isl, bool) if kind in ("init", "final", "post") else nst > it
+                e5 = evs[5 * k:5 * k + 5]
+                grad = kind in ("grad", "post")
+                if g_isl.size:
+                    gsel = sel[g_isl]
+                    st_, ex_ = gsel & gal["star"], gsel & ~gal["star"]
+                    tg = gal["T"].astype(np.float64)
+                    planes = gal["grad_planes"] if grad else np.ones(len(g_isl))
+                    nspec = 3.0 if grad else 1.0
+                    spec_b = gal["spec_floats"] / 3.0 * nspec * 4.0
+                    sbytes = float(np.sum(spec_b[ex_]) + 28 * 4.0 * gsel.sum())
+                    pad = np.asarray([_fft_pad(int(t)) for t in gal["T"]], np.float64)
+                    hh = pad // 2 + 1
+                    npoly = 3.0 if grad else 1.0
+                    nfwd = np.where(grad & (gal["grad_planes"] > 1), 3.0, 1.0) if grad else 1.0
+                    e_fl = (2.0 * nt * tg * tg * npoly + nfwd * (tg + hh) * fft_cost(pad)
+                            + planes * (6.0 * hh * pad + (hh + tg) * fft_cost(pad))
+                            + 28.0 * gal["win"] * planes)
+                    t_fl = 2.0 * nt * gal["reg"] * npoly + 28.0 * gal["win"] * planes
+                    tiers_of = lambda m: sum(coef_tbl[int(t)] for t in set(gal["T"][m].tolist()))
+                    ebytes = float(tiers_of(ex_) + np.sum(spec_b[ex_])
+                                   + nb * np.sum((gal["win"] * planes * 4.0)[ex_]))
+                    tbytes = float(tiers_of(st_) + nb * np.sum((gal["win"] * planes * 4.0)[st_]))
+                    timer.record("prerender_spectra_kernel", e5[0], e5[1],
+                                 dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
+                                      n_pre=int(gsel.sum())))
+                    if gal["n_ext"]:
+                        timer.record("prerender_planes_kernel", e5[1], e5[2],
+                                     dict(kind="island_eval", phase=kind, bytes=ebytes,
+                                          flops=float(nb * np.sum(e_fl[ex_])),
+                                          n_pre=int(ex_.sum())))
+                    if gal["n_ext"] < g_isl.size:
+                        timer.record("prerender_stars_kernel", e5[2], e5[3],
+                                     dict(kind="island_eval", phase=kind, bytes=tbytes,
+                                          flops=float(nb * np.sum(t_fl[st_])),
+                                          n_pre=int(st_.sum())))
+                ncol = 1 + nr + (nlive if grad else 0)
+                ev_fl = nb * npix * (ncol * (ncol + 1) + 4.0 * nr + (2.0 * (nr + nlive) if grad else 0))
+                timer.record("island_eval_kernel", e5[3], e5[4],
+                             dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
+                                  flops=float(ev_fl[sel].sum()), n_isl=int(sel.sum())))
+        hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics")} if post_on
+              else None)
+        results = []
+        for (kern, host, rows), (i0, n_c, m0, f0, N) in zip(classes, spans):
+            r = dict(p=h["p"][m0 * 6:(m0 + n_c * N) * 6].reshape(n_c, N, 6),
+                     fluxes=h["fluxes"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N),
+                     ferr=h["ferr"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N))
+            for k in ("chi2", "lam", "conv", "stall", "nst", "consec", "nrej", "n_good"):
+                r[k] = h[k][i0:i0 + n_c]
+            for k in ("conv", "stall"):
+                r[k] = r[k].astype(bool)
+            r["n_good"] = r["n_good"].astype(np.int64)
+            if post_on:
+                r["post_var"] = hp["var"][m0 * 7:(m0 + n_c * N) * 7].reshape(n_c, N, 7)
+                r["post_dvar"] = hp["dvar"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N)
+                r["post_metrics"] = hp["metrics"][m0 * nb * 6:(m0 + n_c * N) * nb * 6].reshape(
+                    n_c, N, nb, 6)
+            r["steps_run"] = int(r["nst"].max()) if n_c else 0
+            r["wall_s"] = time.perf_counter() - t0
+            results.append(r)
+        return results
+
+    # every device buffer the launches address: the launch closure holds
+    # raw pointers only, and a captured graph replays them later
+    keep = [ins, d_shape, d_moff, d_foff, outs, scratch, counter, dstack_dev, wstack_dev,
+            psf_trees, sersic_dev]
+    if split:
+        keep += [hg, p_try, counters, d_items, d_bands, d_hgoff, gal]
+        if post_on:
+            keep.append(post_t)
+    return PreparedIslandBatch(launch, timer_info, collect, keep)
+
+
+@dataclass
+class PreparedIslandBatch:
+    """A batch of island classes staged on the device: ``launch(stream)``
+    enqueues the fit (it can be captured into a graph), ``timer_info()``
+    describes it for a kernel timer and ``collect()`` copies the results
+    back (one dict per class).  ``keepalive`` holds every device buffer the
+    launches address (the launch closure passes raw pointers)."""
+    launch: object
+    timer_info: object
+    collect: object
+    keepalive: object = None
+
+
+def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
+                     precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
+                     split: bool = False, prerender: bool = True, post: bool = False):
+    """Fit the islands of several classes in one launch sequence (see
+    :func:`prepare_island_batch`) and return one result dict per class, in
+    the same layout as :func:`run_class_chunk`."""
+    pb = prepare_island_batch(classes, psf_trees, max_steps, dstack_dev, wstack_dev,
+                              precision=precision, sersic_dev=sersic_dev, timer=timer,
+                              split=split, prerender=prerender, post=post)
+    ev = timer.begin("island_lm_kernel") if timer is not None else None
+    pb.launch(stream)
     if ev is not None:
-        timer.end(ev, dict(kind="island_batch", n=n_isl, nb=nb, outs=outs, area=area,
-                           nreal=mmc, name="island_split_sequence" if split else None))
-    h = {k: v.cpu().numpy() for k, v in outs.items()}
-    if split and int(counters[-1].item()) != 0:
-        raise _lib.UphError(_lib.UPH_ERR_INTERNAL, "island band blocks overflowed their buffer")
-    if split and evs is not None:
-        # Algorithmic cost of every launch of the sequence, recorded with
-        # its events.  Bytes are the unique HBM bytes the launch must move;
-        # flops count multiply-adds as 2 and a complex FFT of n points as
-        # 5 n log2 n.
-        #   evaluation: data + weight of every island box it evaluates (INIT
-        #     and FINAL: all islands; the H/g and trial launches of step it:
-        #     the islands that took step it); per band the Gram of the
-        #     1 + N + nL planes, the residual and its projections.
-        #   taps/spectra: spectra written (all three on H/g passes, F alone
-        #     otherwise) and 28 taps per member.
-        #   extended planes: PSF coefficient table once, spectra read, window
-        #     part of the planes written; per (member, band) the polynomial
-        #     over the stamp, forward transforms of P (and dP/dx, dP/dy),
-        #     one spectral product and inverse transform per output plane,
-        #     the 7-tap shift of the window part.
-        #   stars: coefficient table once, planes written; the polynomial
-        #     over the shift-read box and the shift.
-        nst = h["nst"].astype(np.int64)
-        box = nb * area.astype(np.float64) * 8.0
-        kp = c["keep"].reshape(-1, 6).astype(np.float64) * c["mm"].reshape(-1, 1)
-        nlive = np.asarray([kp[moff[i]:moff[i] + shape[i][0]].sum() for i in range(n_isl)])
-        npix = area.astype(np.float64)
-        nr = mmc.astype(np.float64)
-        kinds = ["init"] + ["grad", "try"] * int(max_steps) + ["final"]
-        g_isl = gal["isl"]
-        coef_tbl = {T: nb * nt * T * T * 4.0 for T in tiers}
+        timer.end(ev, pb.timer_info())
+    return pb.collect()
 
-        def fft_cost(n):
-            return 5.0 * n * np.log2(n)
 
-        for k, kind in enumerate(kinds):
-            it = (k - 1) // 2
-            sel = np.ones(n_isl, bool) if kind in ("init", "final") else nst > it
-            e5 = evs[5 * k:5 * k + 5]
-            grad = kind == "grad"
-            if g_isl.size:
-                gsel = sel[g_isl]
-                st_, ex_ = gsel & gal["star"], gsel & ~gal["star"]
-                tg = gal["T"].astype(np.float64)
-                planes = gal["grad_planes"] if grad else np.ones(len(g_isl))
-                nspec = 3.0 if grad else 1.0
-                spec_b = gal["spec_floats"] / 3.0 * nspec * 4.0
-                sbytes = float(np.sum(spec_b[ex_]) + 28 * 4.0 * gsel.sum())
-                pad = np.asarray([_fft_pad(int(t)) for t in gal["T"]], np.float64)
-                hh = pad // 2 + 1
-                npoly = 3.0 if grad else 1.0
-                nfwd = np.where(grad & (gal["grad_planes"] > 1), 3.0, 1.0) if grad else 1.0
-                e_fl = (2.0 * nt * tg * tg * npoly + nfwd * (tg + hh) * fft_cost(pad)
-                        + planes * (6.0 * hh * pad + (hh + tg) * fft_cost(pad))
-                        + 28.0 * gal["win"] * planes)
-                t_fl = 2.0 * nt * gal["reg"] * npoly + 28.0 * gal["win"] * planes
-                tiers_of = lambda m: sum(coef_tbl[int(t)] for t in set(gal["T"][m].tolist()))
-                ebytes = float(tiers_of(ex_) + np.sum(spec_b[ex_])
-                               + nb * np.sum((gal["win"] * planes * 4.0)[ex_]))
-                tbytes = float(tiers_of(st_) + nb * np.sum((gal["win"] * planes * 4.0)[st_]))
-                timer.record("prerender_spectra_kernel", e5[0], e5[1],
-                             dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
-                                  n_pre=int(gsel.sum())))
-                if gal["n_ext"]:
-                    timer.record("prerender_planes_kernel", e5[1], e5[2],
-                                 dict(kind="island_eval", phase=kind, bytes=ebytes,
-                                      flops=float(nb * np.sum(e_fl[ex_])),
-                                      n_pre=int(ex_.sum())))
-                if gal["n_ext"] < g_isl.size:
-                    timer.record("prerender_stars_kernel", e5[2], e5[3],
-                                 dict(kind="island_eval", phase=kind, bytes=tbytes,
-                                      flops=float(nb * np.sum(t_fl[st_])),
-                                      n_pre=int(st_.sum())))
-            ncol = 1 + nr + (nlive if grad else 0)
-            ev_fl = nb * npix * (ncol * (ncol + 1) + 4.0 * nr + (2.0 * (nr + nlive) if grad else 0))
-            timer.record("island_eval_kernel", e5[3], e5[4],
-                         dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
-                              flops=float(ev_fl[sel].sum()), n_isl=int(sel.sum())))
-    results = []
-    for (kern, host, rows), (i0, n_c, m0, f0, N) in zip(classes, spans):
-        r = dict(p=h["p"][m0 * 6:(m0 + n_c * N) * 6].reshape(n_c, N, 6),
-                 fluxes=h["fluxes"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N),
-                 ferr=h["ferr"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N))
-        for k in ("chi2", "lam", "conv", "stall", "nst", "consec", "nrej", "n_good"):
-            r[k] = h[k][i0:i0 + n_c]
-        for k in ("conv", "stall"):
-            r[k] = r[k].astype(bool)
-        r["n_good"] = r["n_good"].astype(np.int64)
-        r["steps_run"] = int(r["nst"].max()) if n_c else 0
-        r["wall_s"] = time.perf_counter() - t0
-        results.append(r)
-    return results
+class IslandBatchGraph:
+    """The launch sequence of a prepared batch captured once into a HIP
+    graph and replayed: every replay refits the batch from the seeds in its
+    device buffers (the counters are cleared by a kernel node, and nothing
+    in the sequence allocates or synchronises)."""
+
+    def __init__(self, classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
+                 precision: str = "fp32", sersic_dev=None, prerender: bool = True):
+        import torch
+        self.batch = prepare_island_batch(classes, psf_trees, max_steps, dstack_dev,
+                                          wstack_dev, precision=precision,
+                                          sersic_dev=sersic_dev, split=True,
+                                          prerender=prerender)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.batch.launch(torch.cuda.current_stream())
+
+    def replay(self):
+        self.graph.replay()
+
+    def collect(self):
+        return self.batch.collect()
+

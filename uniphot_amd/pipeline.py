@@ -50,6 +50,7 @@ class FitConfig:
     stamp_tiers: tuple = (55, 111, 221)
     island_split: bool = True    # band-split island LM (uph_island_fit_split)
     galaxy_prerender: bool = True  # split path: galaxies rendered by their own launches
+    post_fit: bool = False       # parameter variances, marginal flux errors, source metrics
 
 
 @dataclass
@@ -90,6 +91,28 @@ class DeviceField:
         self.device = device
         self.sigma_sky = float(np.median([
             np.median(1.0 / np.sqrt(v[::16, ::16][v[::16, ::16] > 0])) for v in iv]))
+
+    @classmethod
+    def from_device(cls, data, invvar, good, psfs, psf_dev=None):
+        """Wrap images already on the device (no copy); ``psf_dev`` maps
+        stamp tier -> (coefs, geo) device arrays to use instead of building
+        them from ``psfs``."""
+        import torch
+        self = cls.__new__(cls)
+        self.data = data
+        self.invvar = torch.where(good.bool(), invvar, torch.zeros((), dtype=invvar.dtype,
+                                                                  device=invvar.device))
+        self.good = good
+        self.psfs = list(psfs)
+        self.nb, self.H, self.W = (int(v) for v in data.shape)
+        self.native = int(psfs[0].H_psf)
+        self.n_terms = int(psfs[0].n_terms)
+        self._psf_dev = dict(psf_dev or {})
+        self.device = data.device
+        iv = self.invvar[:, ::16, ::16]
+        med = [torch.median(1.0 / torch.sqrt(v[v > 0])).item() for v in iv]
+        self.sigma_sky = float(np.median(med))
+        return self
 
     def psf(self, T):
         import torch
@@ -202,6 +225,33 @@ class IslandSetFit:
     n_fitted_islands: int
     classes: list             # (plan, host) per class
     chunks: list              # (n_islands, seconds) per launch sequence
+    # post-fit pass (cfg.post_fit; NaN where not computed)
+    param_var: np.ndarray = None      # (n, 7) var x, y, logre, ee1, ee2, u, cov(ee1, ee2)
+    flux_err_marg: np.ndarray = None  # (n, nb) sqrt(flux_err^2 + shape-marginal increment)
+    metrics: np.ndarray = None        # (n, 4) rchisq, fracflux, fracin, fracmasked
+    flux_window: np.ndarray = None    # (n, nb) flux times the unit model's window sum
+
+
+def metrics_from_sums(sums):
+    """Per-source (rchisq, fracflux, fracin, fracmasked) and the per-band
+    window flux from the post-fit pass's per-band sums (..., nb, 6) = [sum m,
+    sum (f m)^2, sum (model - f m) f m, sum m on zero-weight pixels,
+    sum m w (d - model)^2, f], combined over bands as the reference does
+    (island_solver.py:1672-1713): fracflux and fracmasked fin-weighted with
+    fin = clip(sum m, 0, 1), fracin and rchisq as ratios of band sums."""
+    sm, own2, nown, msk, rc, f = (sums[..., q].astype(np.float64) for q in range(6))
+    fin = np.clip(sm, 0.0, 1.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ffr = np.where(own2 > 0, nown / own2, 0.0)
+        ff_n, ff_d = (fin * ffr).sum(-1), fin.sum(-1)
+        fin_n, fin_d = np.abs(sm).sum(-1), np.ones_like(sm).sum(-1)
+        fm_n = (fin * np.clip(msk, 0.0, 1.0)).sum(-1)
+        rc_n, rc_d = np.abs(rc).sum(-1), np.abs(sm).sum(-1)
+        fracflux = np.where(ff_d > 0, ff_n / ff_d, 0.0)
+        fracin = np.where(fin_d > 0, fin_n / fin_d, np.nan)
+        fracmasked = np.where(ff_d > 0, fm_n / ff_d, 0.0)
+        rchisq = np.where(rc_d > 0, rc_n / rc_d, np.nan)
+    return np.stack([rchisq, fracflux, fracin, fracmasked], axis=-1), f * sm
 
 
 def _chunk_batch(batch, nb, budget_bytes):
@@ -261,6 +311,10 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     stall = np.zeros(n, bool)
     nst = np.zeros(n, np.int32)
     fitted = np.zeros(n, bool)
+    pvar = np.full((n, 7), np.nan, np.float32)
+    ferr_m = np.full((n, nb), np.nan, np.float32)
+    mets = np.full((n, 4), np.nan, np.float32)
+    fwin = np.full((n, nb), np.nan, np.float32)
     classes, batch, trees = [], [], {}
     for plan in plans:
         host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
@@ -291,7 +345,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         t0 = time.perf_counter()
         results = run_island_batch(chunk, trees, cfg.max_steps, dstack, wstack,
                                    precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
-                                   split=cfg.island_split, prerender=cfg.galaxy_prerender)
+                                   split=cfg.island_split, prerender=cfg.galaxy_prerender,
+                                   post=cfg.post_fit)
         if sync is not None:
             sync()
         timing.append((int(sum(np.asarray(r).size for _, _, r in chunk)),
@@ -316,9 +371,16 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             stall[src] = res["stall"][r_of]
             nst[src] = res["nst"][r_of]
             fitted[src] = True
+            if "post_var" in res:
+                pvar[src] = res["post_var"][mm]
+                dv = res["post_dvar"].transpose(0, 2, 1)[mm]
+                ferr_m[src] = np.sqrt(np.square(ferr[src]) + dv)
+                m4, fw = metrics_from_sums(res["post_metrics"][mm])
+                mets[src] = m4
+                fwin[src] = fw
     n_fit = sum(p.n_islands for p in plans)
     return IslandSetFit(fx, fy, flr, fe1, fe2, fsn, pfit, flux, ferr, conv, stall, nst, fitted,
-                        n_fit, classes, timing)
+                        n_fit, classes, timing, pvar, ferr_m, mets, fwin)
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,

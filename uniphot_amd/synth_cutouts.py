@@ -85,6 +85,81 @@ def _pack_slots(bcls, order, mosaic_w):
     return sx, sy, y0
 
 
+def render_profiles(lx, ly, size, sig, amp, is_star, cov, ga, gv, core):
+    """(q, nb, size, size) float64 profiles of q sources on a size x size
+    pixel grid (pixel centres 0..size-1) at local positions (lx, ly):
+    stars as pixel-integrated (erf) 3-Gaussian PSF mixtures, extended
+    sources as their Gaussian mixture (amplitudes ga, variances gv times the
+    ellipse covariance terms cov, plus a core) convolved analytically with
+    the PSF mixture, pixel integration approximated by s^2 + 1/12 (as the
+    numpy field renderer).  sig, amp: (q, nb, 3) PSF sigmas and per-component
+    amplitudes (flux x weight)."""
+    import torch
+    dev = lx.device
+    f64 = dict(dtype=torch.float64, device=dev)
+    q, nb = amp.shape[0], amp.shape[1]
+    prof = torch.zeros((q, nb, size, size), **f64)
+    pix = torch.arange(size, **f64)
+    edges = torch.arange(size + 1, **f64) - 0.5
+    if bool(is_star.any()):
+        zx = (edges[None, None, None, :] - lx[:, None, None, None]) / (np.sqrt(2.0) * sig[..., None])
+        zy = (edges[None, None, None, :] - ly[:, None, None, None]) / (np.sqrt(2.0) * sig[..., None])
+        px = 0.5 * torch.diff(torch.special.erf(zx), dim=-1)   # (q, nb, 3, size)
+        py = 0.5 * torch.diff(torch.special.erf(zy), dim=-1)
+        sp = torch.einsum("qbk,qbky,qbkx->qbyx", amp, py, px)
+        prof += sp * is_star[:, None, None, None]
+    gsel = ~is_star
+    if bool(gsel.any()):
+        gi = torch.nonzero(gsel).flatten()
+        ddx = pix[None, None, :] - lx[gi][:, None, None]     # (g, 1, size)
+        ddy = pix[None, :, None] - ly[gi][:, None, None]     # (g, size, 1)
+        acc = torch.zeros((gi.numel(), nb, size, size), **f64)
+        cv = cov[gi]
+        s_g, amp_g = sig[gi], amp[gi]
+        r2 = (ddx ** 2 + ddy ** 2)[:, None]
+        for kk in range(3):
+            s2 = s_g[:, :, kk] ** 2 + 1.0 / 12.0              # (g, nb)
+            ampk = amp_g[:, :, kk]
+            acc += (core[gi][:, None] * ampk / (2 * np.pi * s2))[..., None, None] \
+                * torch.exp(-0.5 * r2 / s2[..., None, None])
+            for j in range(ga.shape[1]):
+                aq = ga[gi, j]
+                if not bool((aq != 0).any()):
+                    continue
+                vq = gv[gi, j]
+                c00 = vq[:, None] * cv[:, 0, None] + s2
+                c11 = vq[:, None] * cv[:, 1, None] + s2
+                c01 = (vq * cv[:, 2])[:, None].expand_as(s2)
+                det = c00 * c11 - c01 * c01
+                qf = (c11[..., None, None] * (ddx ** 2)[:, None]
+                      - 2 * c01[..., None, None] * (ddx * ddy)[:, None]
+                      + c00[..., None, None] * (ddy ** 2)[:, None]) / det[..., None, None]
+                acc += (aq[:, None] * ampk / (2 * np.pi * torch.sqrt(det)))[..., None, None] \
+                    * torch.exp(-0.5 * qf)
+        prof[gi] += acc
+    return prof
+
+
+def galaxy_mixtures(stype, sersic_n):
+    """Per-source Gaussian-mixture amplitudes, variances (padded to the
+    longest mixture) and core amplitude of the extended sources."""
+    gal = stype != SOURCE_STAR
+    K = 0
+    mog = {}
+    for v in np.unique(sersic_n[gal]):
+        a, var, core = mog_params_at(float(v))
+        mog[float(v)] = (a, var, core)
+        K = max(K, a.size)
+    n = stype.size
+    ga = np.zeros((n, max(K, 1)))
+    gv = np.zeros((n, max(K, 1)))
+    gcore = np.zeros(n)
+    for i in np.where(gal)[0]:
+        a, var, core = mog[float(sersic_n[i])]
+        ga[i, :a.size], gv[i, :a.size], gcore[i] = a, var, core
+    return ga, gv, gcore
+
+
 def scene_cutouts(seed=SEEDS["c3"], n_isl=200_000, n_bands=10, k_max=64, device="cuda",
                   mosaic_w=16384, box=5.0, w_min=15, w_max=95, order=4,
                   max_profile_elems=1 << 27) -> Cutouts:
@@ -130,18 +205,8 @@ def scene_cutouts(seed=SEEDS["c3"], n_isl=200_000, n_bands=10, k_max=64, device=
     wgt = np.stack([f.params(tx, ty)[1] for f in fields], axis=1)
     cxx, cyy, cxy = _cov_terms(lr.astype(np.float64), e1.astype(np.float64),
                                e2.astype(np.float64))
-    K = 0
-    mog = {}
-    for v in np.unique(sn[gal]):
-        a, var, core = mog_params_at(float(v))
-        mog[float(v)] = (a, var, core)
-        K = max(K, a.size)
-    ga = np.zeros((n, K))
-    gv = np.zeros((n, K))
-    gcore = np.zeros(n)
-    for i in np.where(gal)[0]:
-        a, var, core = mog[float(sn[i])]
-        ga[i, :a.size], gv[i, :a.size], gcore[i] = a, var, core
+    ga, gv, gcore = galaxy_mixtures(st, sn)
+    K = ga.shape[1]
     pad = int(classes.max()) + 8
     f64 = dict(dtype=torch.float64, device=device)
     dstack = torch.zeros((n_bands, H + 2 * pad, W + 2 * pad), dtype=torch.float32, device=device)
@@ -162,8 +227,6 @@ def scene_cutouts(seed=SEEDS["c3"], n_isl=200_000, n_bands=10, k_max=64, device=
         for c0 in range(0, ids_all.size, per):
             ids = ids_all[c0:c0 + per]
             cells = torch.zeros((ids.size, n_bands, wv, wv), **f64)
-            pix = torch.arange(wv, **f64)
-            edges = torch.arange(wv + 1, **f64) - 0.5
             for j in range(kmax):
                 has = k[ids] > j
                 if not has.any():
@@ -174,48 +237,9 @@ def scene_cutouts(seed=SEEDS["c3"], n_isl=200_000, n_bands=10, k_max=64, device=
                 ly = torch.tensor(ty[m] - cy0[ids[loc]], **f64)
                 s = t_sig[m]                                  # (q, nb, 3)
                 amp = t_flux[m][:, :, None] * t_wgt[m]        # (q, nb, 3)
-                prof = torch.zeros((loc.size, n_bands, wv, wv), **f64)
-                star = torch.tensor(st[m] == SOURCE_STAR, device=device)
-                if bool(star.any()):
-                    # pixel-integrated (erf) Gaussians, separable
-                    zx = (edges[None, None, None, :] - lx[:, None, None, None]) / (
-                        np.sqrt(2.0) * s[..., None])
-                    zy = (edges[None, None, None, :] - ly[:, None, None, None]) / (
-                        np.sqrt(2.0) * s[..., None])
-                    px = 0.5 * torch.diff(torch.special.erf(zx), dim=-1)   # (q, nb, 3, w)
-                    py = 0.5 * torch.diff(torch.special.erf(zy), dim=-1)
-                    sp = torch.einsum("qbk,qbky,qbkx->qbyx", amp, py, px)
-                    prof += sp * star[:, None, None, None]
-                gsel = ~star
-                if bool(gsel.any()):
-                    gi = torch.nonzero(gsel).flatten()
-                    mg = torch.as_tensor(m, device=device)[gi]
-                    ddx = pix[None, None, :] - lx[gi][:, None, None]     # (g, 1, w)
-                    ddy = pix[None, :, None] - ly[gi][:, None, None]     # (g, w, 1)
-                    acc = torch.zeros((gi.numel(), n_bands, wv, wv), **f64)
-                    cv = t_cov[mg]
-                    for kk in range(3):
-                        s2 = s[gi][:, :, kk] ** 2 + 1.0 / 12.0          # (g, nb)
-                        ampk = amp[gi][:, :, kk]
-                        r2 = (ddx ** 2 + ddy ** 2)[:, None]             # (g, 1, w, w)
-                        acc += (t_core[mg][:, None] * ampk / (2 * np.pi * s2))[..., None, None] \
-                            * torch.exp(-0.5 * r2 / s2[..., None, None])
-                        for q in range(K):
-                            aq = t_ga[mg, q]
-                            if not bool((aq != 0).any()):
-                                continue
-                            vq = t_gv[mg, q]
-                            c00 = vq[:, None] * cv[:, 0, None] + s2
-                            c11 = vq[:, None] * cv[:, 1, None] + s2
-                            c01 = (vq * cv[:, 2])[:, None].expand_as(s2)
-                            det = c00 * c11 - c01 * c01
-                            qf = (c11[..., None, None] * (ddx ** 2)[:, None]
-                                  - 2 * c01[..., None, None] * (ddx * ddy)[:, None]
-                                  + c00[..., None, None] * (ddy ** 2)[:, None]) \
-                                / det[..., None, None]
-                            acc += (aq[:, None] * ampk / (2 * np.pi * torch.sqrt(det)))[
-                                ..., None, None] * torch.exp(-0.5 * qf)
-                    prof[gi] += acc
+                prof = render_profiles(lx, ly, wv, s, amp, torch.tensor(st[m] == SOURCE_STAR,
+                                                                        device=device),
+                                       t_cov[m], t_ga[m], t_gv[m], t_core[m])
                 cells.index_add_(0, torch.as_tensor(loc, device=device), prof)
             # noise: Poisson(truth + sky) + N(0, 5^2) - sky; sky-only inverse variance
             lam = torch.clamp(cells + sky_t[None, :, None, None], min=0.0)
