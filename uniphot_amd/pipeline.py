@@ -50,7 +50,7 @@ class FitConfig:
     stamp_tiers: tuple = (55, 111, 221)
     island_split: bool = True    # band-split island LM (uph_island_fit_split)
     galaxy_prerender: bool = True  # split path: galaxies rendered by their own launches
-    post_fit: bool = False       # parameter variances, marginal flux errors, source metrics
+    post_fit: bool = True        # parameter variances, marginal flux errors, source metrics
 
 
 @dataclass
@@ -72,6 +72,12 @@ class FitResult:
     island_giant: np.ndarray
     timings: dict = field(default_factory=dict)
     record: dict = field(default_factory=dict)
+    # island post-fit pass (NaN where not computed):
+    param_var: np.ndarray = None      # (n, 7) var x, y, logre, ee1, ee2, u, cov(ee1, ee2)
+    island_flux: np.ndarray = None    # (n, nb) island VarPro fluxes
+    island_flux_err_marg: np.ndarray = None  # (n, nb) shape-marginalised island flux error
+    metrics: np.ndarray = None        # (n, 4) rchisq, fracflux, fracin, fracmasked
+    flux_window: np.ndarray = None    # (n, nb)
 
 
 class DeviceField:
@@ -437,7 +443,7 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     isf = fit_island_set(fld.psf, fld.nb, fld.n_terms, fld.native, dstack, wstack, pad, iset,
                          plans, x, y, stype, logre, ee1, ee2, sn, free_n, cfg, timer=timer)
     fx, fy, flr, fe1, fe2 = isf.x, isf.y, isf.logre, isf.ee1, isf.ee2
-    fsn = sn.copy()
+    fsn = isf.sersic_n
     conv, stall, nst = isf.conv, isf.stall, isf.nst
     giant = np.zeros(n, bool)
     for gid in giants:
@@ -456,12 +462,14 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                n_giant_islands=int(giants.size), classes=[
                    (p.bbox_class, p.nsrc_class, p.kind, p.tier, p.n_islands) for p in plans],
                knobs_off=["scene_subtract", "blank_response_correction", "wing_peeling",
-                          "param_errors", "source_metrics", "marginal_flux_err", "priors"])
+                          "priors"] + ([] if cfg.post_fit else
+                                       ["param_errors", "source_metrics", "marginal_flux_err"]))
     if keep_artifacts:
         rec["_artifacts"] = dict(classes=classes, dstack=dstack, wstack=wstack, pad=pad,
                                  grids=grids, island_set=iset, gtier=gtier)
     return FitResult(flux, np.sqrt(np.maximum(var, 0.0)), s1_flux, s1_var, fx, fy, flr, fe1,
-                     fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec)
+                     fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec,
+                     isf.param_var, isf.flux, isf.flux_err_marg, isf.metrics, isf.flux_window)
 
 
 def psf_bank(psfs, device="cuda"):
