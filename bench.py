@@ -11,7 +11,7 @@ placed against whichever of the 8 TB/s HBM and 157.3 TFLOP/s fp32 peaks it
 is closer to -- and a CPU baseline (the float64 oracle timed on a bounded
 sample of the same workload).
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c2-stars|c1|c3|c5]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c2-stars|c1|c3|c4|c5]
 """
 from __future__ import annotations
 
@@ -391,16 +391,87 @@ def bench_serving(args, torch):
     print(json.dumps(line), flush=True)
 
 
+def bench_precision(args, torch):
+    """Precision workload: the 20-band 2048^2 field scaled to 8192^2 with 40
+    bands and 80 k sources (same source density, mix and PSF), fitted with
+    the island Gram on float32 matrix cores (value) and on bfloat16 ones
+    (three refinement steps, islands whose refinement does not settle
+    refitted on float32 operands); per-source flux and position error of
+    both against the float64 island fit on a sample of islands."""
+    import dataclasses
+    from uniphot_amd.pipeline import DeviceField, FitConfig, fit_field, island_accuracy
+    from uniphot_amd.synth import SEEDS
+    from uniphot_amd.synth_tiles import PsfBank, make_tile
+    sync = torch.cuda.synchronize
+    t_scene = time.perf_counter()
+    bank = PsfBank(SEEDS["c4"], n_bands=40)
+    side = args.side
+    req = make_tile(np.random.default_rng(SEEDS["c4"]), 0, 0.0, side, 40, bank,
+                    density=80000.0 / 8192.0 ** 2)
+    fld = DeviceField.from_device(req.data, req.invvar, req.good, req.psfs)
+    sync()
+    n = req.n_sources
+    print(f"[bench] scene c4: {side}^2, 40 bands, {n} sources in "
+          f"{time.perf_counter() - t_scene:.1f}s", file=sys.stderr, flush=True)
+    g = max(1, side // 256)
+    base = FitConfig(n_rows=g, n_cols=g)
+    cat = (req.x, req.y, req.stype, req.logre, req.ee1, req.ee2, req.sersic_n)
+    runs = {}
+    for i, gram in enumerate(("fp32", "bf16")):
+        cfg = dataclasses.replace(base, gram=gram)
+        if i == 0 and args.warmup:
+            r = fit_field(fld, *cat, cfg, sync=sync)
+            print(f"[bench] warmup: {r.timings}", file=sys.stderr, flush=True)
+        timer = KernelTimer()
+        sync()
+        t0 = time.perf_counter()
+        res = fit_field(fld, *cat, cfg, sync=sync, timer=timer, keep_artifacts=True)
+        sync()
+        el = time.perf_counter() - t0
+        print(f"[bench] {gram}: {el:.2f}s {res.timings}", file=sys.stderr, flush=True)
+        runs[gram] = (res, el, timer)
+    res, el, timer = runs["fp32"]
+    art = res.record["_artifacts"]
+    iset = art["island_set"]
+    rng = np.random.default_rng(2)
+    fit_ids = np.setdiff1d(np.arange(iset.n_islands), art["giants"])
+    sample = np.sort(rng.choice(fit_ids, min(args.sample, fit_ids.size), replace=False))
+    acc = island_accuracy(fld.psf, fld.nb, fld.n_terms, fld.native, art["dstack"],
+                          art["wstack"], art["pad"], iset, sample, cat, art["free_n"], base,
+                          grams=("bf16", "fp32"), sync=sync)
+    print(f"[bench] accuracy {acc}", file=sys.stderr, flush=True)
+    ks = timer.summary()
+    line = {
+        "metric": "sources fitted/sec at <=1e-5 flux rel-err vs FP64; p50/p99 tile latency",
+        "value": round(n / el, 2), "unit": "sources/s", "n_gpus": 1, "steps": 1,
+        "warmup": int(args.warmup), "ms_per_step": round(el * 1e3, 2), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "c4", "image": [side, side], "n_bands": 40, "n_sources": n,
+                   "grid": [g, g], "islands": res.record["n_islands"],
+                   "stage_s": {k: round(v, 4) for k, v in res.timings.items()}},
+        "bf16_gram": {"value": round(n / runs["bf16"][1], 2),
+                      "ms_per_step": round(runs["bf16"][1] * 1e3, 2),
+                      "stage_s": {k: round(v, 4) for k, v in runs["bf16"][0].timings.items()}},
+        "roofline": roofline_of(ks, "c4"),
+        "accuracy_vs_fp64": acc,
+        "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
+                        for kk, vv in v.items()} for k, v in ks.items()},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c2", help="c2 (default), c2-stars, c1, c3 or c5")
+    ap.add_argument("--config", default="c2", help="c2 (default), c2-stars, c1, c3, c4 or c5")
     ap.add_argument("--islands", type=int, default=200_000, help="c3: island cutouts")
     ap.add_argument("--requests", type=int, default=16, help="c5: tile requests")
     ap.add_argument("--rate", type=float, default=50.0, help="c5: mean arrivals per second")
     ap.add_argument("--inflight", type=int, default=4, help="c5: tiles in flight at most")
+    ap.add_argument("--side", type=int, default=8192, help="c4: field side (px)")
     ap.add_argument("--sample", type=int, default=300,
                     help="c3: islands of the FP64 accuracy and persistent-form comparisons")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -423,6 +494,9 @@ def main():
         return
     if args.config == "c5":
         bench_serving(args, torch)
+        return
+    if args.config == "c4":
+        bench_precision(args, torch)
         return
     from uniphot_amd.pipeline import DeviceField, FitConfig, fit_field
     t_scene = time.perf_counter()

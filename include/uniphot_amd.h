@@ -239,6 +239,16 @@ typedef struct uph_island_split {
   float* post_var;
   float* post_dvar;
   float* post_metrics;
+  /* gram_bf16 = 1 (needs the pre-render): the island Gram matrices on
+     bfloat16 matrix cores (operands rounded to nearest even, float32
+     accumulation flushed to float64 every 256 products); the float64 flux
+     solve then takes three refinement steps on float64 residual
+     projections instead of one.  gram_unsettled (n_isl int32, zeroed by
+     the caller) is set to 1 for an island where the last step still moved
+     a flux by more than 1e-6 of itself (Gram error times condition number
+     near 1); the caller refits those islands with gram_bf16 = 0. */
+  int gram_bf16;
+  int32_t* gram_unsettled;
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);

@@ -51,3 +51,19 @@ def test_cutout_fluxes_against_fp64(cut):
     # the rendering precision times the blend's conditioning
     assert acc["flux_solve"]["frac_le_1e5"] >= 0.95
     assert acc["flux_solve"]["max"] < 1e-3
+
+
+def test_bf16_gram_against_fp64(cut):
+    """Island Gram on bfloat16 matrix cores: after the float64 flux solve's
+    refinement step, the fluxes at fixed parameters stay at the float32
+    Gram's accuracy; whole fits follow the float32 ones."""
+    from uniphot_amd.pipeline import FitConfig, cutout_accuracy
+    k = cut.island_set.n_src
+    ids = np.where(k <= 4)[0]
+    acc = cutout_accuracy(cut, FitConfig(), ids, grams=("fp32", "bf16"))
+    print("bf16 vs fp32 Gram", acc, flush=True)
+    assert acc["flux_solve_fp64_self"]["max"] == 0.0
+    b16, f32 = acc["bf16"], acc["fp32"]
+    assert b16["flux_solve"]["p50"] < 1e-6
+    assert b16["flux_solve"]["frac_le_1e5"] >= f32["flux_solve"]["frac_le_1e5"] - 0.05
+    assert b16["same_trajectory"] >= 0.8

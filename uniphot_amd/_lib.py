@@ -78,7 +78,8 @@ class IslandSplit(C.Structure):
                 ("pre_of_slot", _P), ("pbuf", _P), ("pbuf_elems", C.c_size_t), ("fspec", _P),
                 ("fspec_elems", C.c_size_t), ("gdt", _P), ("gtap", _P), ("post", C.c_int),
                 ("post_off", _P), ("post_buf", _P), ("post_doubles", C.c_size_t),
-                ("post_var", _P), ("post_dvar", _P), ("post_metrics", _P)]
+                ("post_var", _P), ("post_dvar", _P), ("post_metrics", _P),
+                ("gram_bf16", C.c_int), ("gram_unsettled", _P)]
 
 
 _lib = None

@@ -247,6 +247,89 @@ __device__ void gram_mfma(const float* __restrict__ W, const float* __restrict__
   __syncthreads();
 }
 
+// The same Gram with bfloat16 inputs: one v_mfma_f32_16x16x16_bf16 takes
+// each lane's four consecutive pixels (lane l: plane l % 16, pixels
+// 4 (l / 16) .. + 3 of the 16-pixel step), so a K-step is one instruction
+// instead of four.  Operands are rounded to bfloat16 (nearest even): plane a
+// and W * plane b; products accumulate in float32 and are flushed into
+// float64 every 16 K-steps as in gram_mfma.  The flux solve then takes three
+// refinement steps on float64 residual projections instead of one, which
+// restores the fluxes' accuracy.
+typedef short uph_s4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ short bf16_rne(float v) {
+  unsigned u = __float_as_uint(v);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return short(u >> 16);
+}
+
+__device__ void gram_mfma_bf16(const float* __restrict__ W, const float* __restrict__ Z, int P,
+                               int C, int z0, double* __restrict__ out,
+                               double* __restrict__ part) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nt = (C + 15) >> 4, ntp = nt * (nt + 1) / 2;
+  const int S = ntp >= UPH_WAVES ? 1 : UPH_WAVES / ntp;
+  const int units = ntp * S;
+  const int KS = P >> 4;
+  const int col = lane & 15, grp = lane >> 4;
+  for (int u = wid; u < units; u += UPH_WAVES) {
+    const int tp = u / S, sp = u % S;
+    int ti, tj;
+    pair_of(tp, nt, ti, tj);
+    const int ks0 = (KS * sp) / S, ks1 = (KS * (sp + 1)) / S;
+    const int a = ti * 16 + col, b = tj * 16 + col;
+    const bool va = a < C, vb = b < C;
+    const float* za = Z + (size_t)(z0 + (va ? a : 0)) * P;
+    const float* zb = Z + (size_t)(z0 + (vb ? b : 0)) * P;
+    double accd[4] = {0.0, 0.0, 0.0, 0.0};
+    uph_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    int cnt = 0;
+    for (int ks = ks0; ks < ks1; ++ks) {
+      const int px = ks * 16 + 4 * grp;
+      const uph_f4 wv = *reinterpret_cast<const uph_f4*>(W + px);
+      uph_f4 xa = {0.0f, 0.0f, 0.0f, 0.0f}, xb = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (va) xa = *reinterpret_cast<const uph_f4*>(za + px);
+      if (ti == tj)
+        xb = xa;
+      else if (vb)
+        xb = *reinterpret_cast<const uph_f4*>(zb + px);
+      const uph_s4 av = {bf16_rne(xa.x), bf16_rne(xa.y), bf16_rne(xa.z), bf16_rne(xa.w)};
+      const uph_s4 bv = {bf16_rne(wv.x * xb.x), bf16_rne(wv.y * xb.y), bf16_rne(wv.z * xb.z),
+                         bf16_rne(wv.w * xb.w)};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+      if (++cnt == 16) {
+        accd[0] += acc.x;
+        accd[1] += acc.y;
+        accd[2] += acc.z;
+        accd[3] += acc.w;
+        acc = uph_f4{0.0f, 0.0f, 0.0f, 0.0f};
+        cnt = 0;
+      }
+    }
+    accd[0] += acc.x;
+    accd[1] += acc.y;
+    accd[2] += acc.z;
+    accd[3] += acc.w;
+    double* pt = part + (size_t)u * 256;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) pt[(4 * grp + r) * 16 + col] = accd[r];
+  }
+  __syncthreads();
+  for (int e = tid; e < ntp * 256; e += UPH_BLOCK) {
+    const int tp = e >> 8, i = (e >> 4) & 15, j = e & 15;
+    int ti, tj;
+    pair_of(tp, nt, ti, tj);
+    if (ti == tj && i > j) continue;
+    const int a = ti * 16 + i, b = tj * 16 + j;
+    if (a >= C || b >= C) continue;
+    double v = 0.0;
+    for (int sp = 0; sp < S; ++sp) v += part[((size_t)tp * S + sp) * 256 + i * 16 + j];
+    out[a * C + b] = v;
+    out[b * C + a] = v;
+  }
+  __syncthreads();
+}
+
 template <typename Real>
 __device__ __forceinline__ void gram(const Real* W, const Real* Z, int P, int C, int z0,
                                      double* out, double* part, double* red) {
@@ -611,7 +694,16 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
   }
   // Gram of [d, models, tangents] under W.
   const int C = 1 + Nr + nL;
-  gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
+  int bf16 = 0;  // bfloat16 Gram operands (pre-render variant, on request)
+  if constexpr (kPre) bf16 = static_cast<const IslandArgsPre&>(a).gram_bf16;
+  if constexpr (kPre) {
+    if (bf16)
+      gram_mfma_bf16(Wp, planes, P, C, 2, G, part);
+    else
+      gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
+  } else {
+    gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
+  }
   // VarPro fluxes (float64 LU), A64 = A + reg I on real members.
   const double reg = double(a.reg);
   for (int e = tid; e < Nr * Nr; e += UPH_BLOCK) {
@@ -649,6 +741,40 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
     }
   }
   __syncthreads();
+  if constexpr (kPre) {
+    // with the bfloat16 Gram two more refinement steps: each one multiplies
+    // the flux error by about (Gram error) x (condition number)
+    if (bf16) {
+      auto refine = [&]() {
+        for (int p = tid; p < P; p += UPH_BLOCK) {
+          Real model = 0;
+          for (int r = 0; r < Nr; ++r) model = fma(sh.fR[r], Mp[(size_t)r * P + p], model);
+          Rp[p] = Dp[p] - model;
+        }
+        __syncthreads();
+        weighted_pairs<Real, false, kPreQ>(Wp, planes, P, Nr, 3, 1, sm.colws, sh.red);
+        if (tid == 0) {
+          double* dx = sm.c;
+          double moved = 0.0;  // largest correction relative to its flux
+          for (int i = 0; i < Nr; ++i) dx[i] = sm.colws[i] - reg * double(sh.fR[i]);
+          lu_solve_one(sm.LU, sm.pivA, Nr, dx);
+          for (int i = 0; i < Nr; ++i) {
+            moved = fmax(moved, fabs(dx[i]) / fmax(fabs(double(sh.fR[i])), 1.0));
+            sh.fR[i] = Real(double(sh.fR[i]) + dx[i]);
+            sm.f[i] = double(sh.fR[i]);
+          }
+          sh.redd[0] = moved;
+        }
+        __syncthreads();
+      };
+      refine();
+      refine();
+      // the refinement has not settled (Gram error times the condition
+      // number near 1): the island is refitted on float32 operands
+      if (tid == 0 && sh.redd[0] > 1e-6)
+        atomicOr(static_cast<const IslandArgsPre&>(a).gram_unsettled + isl, 1);
+    }
+  }
   if (flux_out) {
     // final fluxes and sqrt(diag(A^-1))
     lu_inverse_diag(sm.LU, sm.pivA, Nr, sm.s, sm.colws);
@@ -1558,6 +1684,8 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
   const bool use_pre = sp->n_pre > 0;
   UPH_CHECK_ARG(!sp->post || use_pre,
                 "uph_island_fit_split: the post-fit pass needs the member pre-render");
+  UPH_CHECK_ARG(!sp->gram_bf16 || use_pre,
+                "uph_island_fit_split: the bfloat16 Gram needs the member pre-render");
   if (use_pre) {
     UPH_CHECK_ARG(precision == UPH_PRECISION_FP32,
                   "uph_island_fit_split: the member pre-render is float32 only");
@@ -1587,6 +1715,10 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
     pre.gdt = sp->gdt;
     pre.gtap = sp->gtap;
     pre.post = sp->post;
+    pre.gram_bf16 = sp->gram_bf16;
+    pre.gram_unsettled = sp->gram_unsettled;
+    UPH_CHECK_ARG(!sp->gram_bf16 || sp->gram_unsettled,
+                  "uph_island_fit_split: the bfloat16 Gram needs gram_unsettled");
     if (sp->post) {
       UPH_CHECK_ARG(sp->post_off && sp->post_buf && sp->post_var && sp->post_dvar &&
                         sp->post_metrics,

@@ -98,6 +98,8 @@ struct IslandArgsPre : IslandArgs {
   float* post_var;             // (member slots, 7) var x, y, logre, ee1, ee2, u, cov(ee1, ee2)
   float* post_dvar;            // flux layout: shape-marginalisation variance increment
   float* post_metrics;         // (member slots, n_bands, 6) per-band metric sums
+  int gram_bf16;               // island Gram on bfloat16 matrix cores (float32 accumulate)
+  int* gram_unsettled;         // (n_isl) set to 1 where a bfloat16 refinement did not settle
 };
 
 // Band-split phases.  Evaluation launches run one (island, band) item per

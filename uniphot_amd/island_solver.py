@@ -251,7 +251,7 @@ def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prer
 def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                          precision: str = "fp32", sersic_dev=None, timer=None,
                          split: bool = False, prerender: bool = True,
-                         post: bool = False) -> "PreparedIslandBatch":
+                         post: bool = False, gram: str = "fp32") -> "PreparedIslandBatch":
     """Stage the islands of several classes for ONE launch sequence.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
@@ -273,6 +273,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     pass: per-member parameter variances (``post_var`` (n, N, 7)), the
     shape-marginalisation flux variance increment (``post_dvar`` (n, nb, N))
     and the per-band metric sums (``post_metrics`` (n, N, nb, 6)).
+    ``gram="bf16"`` (split + pre-render) builds the island Gram matrices on
+    bfloat16 matrix cores with float32 accumulation.
     """
     import torch
     t0 = time.perf_counter()
@@ -409,6 +411,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 e.record()
             ev_arr = (C.c_void_p * len(evs))(*[e.cuda_event for e in evs])
         post_args = (0, None, None, 0, None, None, None)
+        bf16_on = bool(gram == "bf16" and gal["args"][0] > 0)
+        unsettled = torch.zeros(n_isl, dtype=torch.int32, device=dev)
         if post_on:
             Ns = np.asarray([s_[0] for s_ in shape], np.int64)
             pblk = nlc * (nb * Ns + nlc)
@@ -424,10 +428,11 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
                               P(p_try), P(counters),
                               C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None,
-                              *gal["args"], *post_args)
+                              *gal["args"], *post_args, int(bf16_on), P(unsettled))
     else:
         evs = None
         post_on = False
+        bf16_on = False
 
     def launch(stream=None):
         if split:
@@ -530,6 +535,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                                   flops=float(ev_fl[sel].sum()), n_isl=int(sel.sum())))
         hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics")} if post_on
               else None)
+        h_uns = unsettled.cpu().numpy() if split and bf16_on else None
         results = []
         for (kern, host, rows), (i0, n_c, m0, f0, N) in zip(classes, spans):
             r = dict(p=h["p"][m0 * 6:(m0 + n_c * N) * 6].reshape(n_c, N, 6),
@@ -540,6 +546,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
             for k in ("conv", "stall"):
                 r[k] = r[k].astype(bool)
             r["n_good"] = r["n_good"].astype(np.int64)
+            if split and bf16_on:
+                r["gram_unsettled"] = h_uns[i0:i0 + n_c].astype(bool)
             if post_on:
                 r["post_var"] = hp["var"][m0 * 7:(m0 + n_c * N) * 7].reshape(n_c, N, 7)
                 r["post_dvar"] = hp["dvar"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N)
@@ -555,7 +563,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     keep = [ins, d_shape, d_moff, d_foff, outs, scratch, counter, dstack_dev, wstack_dev,
             psf_trees, sersic_dev]
     if split:
-        keep += [hg, p_try, counters, d_items, d_bands, d_hgoff, gal]
+        keep += [hg, p_try, counters, d_items, d_bands, d_hgoff, gal, unsettled]
         if post_on:
             keep.append(post_t)
     return PreparedIslandBatch(launch, timer_info, collect, keep)
@@ -576,13 +584,14 @@ class PreparedIslandBatch:
 
 def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                      precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
-                     split: bool = False, prerender: bool = True, post: bool = False):
+                     split: bool = False, prerender: bool = True, post: bool = False,
+                     gram: str = "fp32"):
     """Fit the islands of several classes in one launch sequence (see
     :func:`prepare_island_batch`) and return one result dict per class, in
     the same layout as :func:`run_class_chunk`."""
     pb = prepare_island_batch(classes, psf_trees, max_steps, dstack_dev, wstack_dev,
                               precision=precision, sersic_dev=sersic_dev, timer=timer,
-                              split=split, prerender=prerender, post=post)
+                              split=split, prerender=prerender, post=post, gram=gram)
     ev = timer.begin("island_lm_kernel") if timer is not None else None
     pb.launch(stream)
     if ev is not None:

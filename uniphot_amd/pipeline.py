@@ -25,7 +25,7 @@ import numpy as np
 
 from . import forced_phot_sparse as fps
 from .grid import assign_sources_to_grids, divide_image_into_grids
-from .island_plan import (DEFAULT_MAX_STEPS, class_host_arrays, free_n_gate,
+from .island_plan import (DEFAULT_MAX_STEPS, JOINT_NSRC_CAP, class_host_arrays, free_n_gate,
                           plan_island_classes)
 from .island_solver import make_class_kernels, psf_tree_device, run_class_chunk
 from .islands import (ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands,
@@ -51,6 +51,8 @@ class FitConfig:
     island_split: bool = True    # band-split island LM (uph_island_fit_split)
     galaxy_prerender: bool = True  # split path: galaxies rendered by their own launches
     post_fit: bool = True        # parameter variances, marginal flux errors, source metrics
+    gram: str = "fp32"           # island Gram operands: "fp32" or "bf16" (matrix cores)
+    stamp_budget: float = 24e9   # bytes of rendered stamps per linear-solve launch
 
 
 @dataclass
@@ -175,9 +177,18 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
                     sersic_n=sn[fi]))
                 owners.append((g, b))
         coefs, geo = fld.psf(T)
-        res = fps.solve_problems(problems, coefs, geo, T, fld.n_terms, fld.data, fld.invvar,
-                                 fld.good, bg_order=cfg.bg_order, bg_geo=bg_geo, reg=cfg.reg,
-                                 k_max=cfg.k_max, timer=timer)
+        # launches of at most stamp_budget bytes of rendered stamps each
+        cost = np.array([pr.pos_x.size * T * T * 4.0 for pr in problems])
+        cut = np.concatenate([[0], np.nonzero(np.diff(np.floor(np.cumsum(cost) /
+                                                             cfg.stamp_budget)))[0] + 1,
+                              [len(problems)]])
+        res = []
+        for a, z in zip(cut[:-1], cut[1:]):
+            if z > a:
+                res += fps.solve_problems(problems[a:z], coefs, geo, T, fld.n_terms, fld.data,
+                                          fld.invvar, fld.good, bg_order=cfg.bg_order,
+                                          bg_geo=bg_geo, reg=cfg.reg, k_max=cfg.k_max,
+                                          timer=timer)
         for (g, b), r in zip(owners, res):
             own = g.source_indices.size
             flux[g.source_indices, b] = r.fluxes[:own]
@@ -230,7 +241,8 @@ class IslandSetFit:
     fitted: np.ndarray        # (n,) bool: member of a jointly fitted island
     n_fitted_islands: int
     classes: list             # (plan, host) per class
-    chunks: list              # (n_islands, seconds) per launch sequence
+    chunks: list              # (n_islands, seconds) per launch sequence; (-n, 0) marks n
+                              # islands refitted on float32 Gram operands
     # post-fit pass (cfg.post_fit; NaN where not computed)
     param_var: np.ndarray = None      # (n, 7) var x, y, logre, ee1, ee2, u, cov(ee1, ee2)
     flux_err_marg: np.ndarray = None  # (n, nb) sqrt(flux_err^2 + shape-marginal increment)
@@ -352,7 +364,32 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         results = run_island_batch(chunk, trees, cfg.max_steps, dstack, wstack,
                                    precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
                                    split=cfg.island_split, prerender=cfg.galaxy_prerender,
-                                   post=cfg.post_fit)
+                                   post=cfg.post_fit, gram=cfg.gram)
+        redo = [(kern, host, rows[res["gram_unsettled"]])
+                for (kern, host, rows), res in zip(chunk, results) if "gram_unsettled" in res
+                and res["gram_unsettled"].any()]
+        if redo:
+            # islands whose bfloat16 Gram refinement did not settle, refitted
+            # on float32 operands
+            again = run_island_batch(redo, trees, cfg.max_steps, dstack, wstack,
+                                     precision=cfg.precision, sersic_dev=sersic_dev,
+                                     split=cfg.island_split, prerender=cfg.galaxy_prerender,
+                                     post=cfg.post_fit, gram="fp32")
+            pos = {id(h): j for j, (_, h, _) in enumerate(redo)}
+            merged = []
+            for (kern, host, rows), res in zip(chunk, results):
+                if id(host) in pos and "gram_unsettled" in res and res["gram_unsettled"].any():
+                    sel = np.nonzero(res["gram_unsettled"])[0]
+                    ra = again[pos[id(host)]]
+                    res = {k: (v.copy() if isinstance(v, np.ndarray) else v)
+                           for k, v in res.items()}
+                    for k, v in ra.items():
+                        if isinstance(v, np.ndarray) and k in res and res[k].shape[0] == rows.size:
+                            res[k][sel] = v
+                merged.append(res)
+            results = merged
+            n_redo = sum(int(r.size) for _, _, r in redo)
+            timing.append((-n_redo, 0.0))
         if sync is not None:
             sync()
         timing.append((int(sum(np.asarray(r).size for _, _, r in chunk)),
@@ -466,7 +503,8 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                                        ["param_errors", "source_metrics", "marginal_flux_err"]))
     if keep_artifacts:
         rec["_artifacts"] = dict(classes=classes, dstack=dstack, wstack=wstack, pad=pad,
-                                 grids=grids, island_set=iset, gtier=gtier)
+                                 grids=grids, island_set=iset, gtier=gtier, free_n=free_n,
+                                 giants=giants)
     return FitResult(flux, np.sqrt(np.maximum(var, 0.0)), s1_flux, s1_var, fx, fy, flr, fe1,
                      fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec,
                      isf.param_var, isf.flux, isf.flux_err_marg, isf.metrics, isf.flux_window)
@@ -496,13 +534,9 @@ def fit_cutouts(cut, cfg: FitConfig, timer=None, sync=None, joint_nsrc_cap=None,
     pass before it, the free-n gate's S/N is the true flux over the
     sky-limited noise of a 50 px^2 aperture.  ``island_ids`` restricts the
     fit to those islands."""
-    from .islands import IslandSet
     iset = cut.island_set
     if island_ids is not None:
-        ids = np.asarray(island_ids, np.int64)
-        iset = IslandSet(n_sources=iset.n_sources, n_islands=ids.size, labels=iset.labels,
-                         radius=iset.radius, members=[iset.members[i] for i in ids],
-                         n_src=iset.n_src[ids], bbox=iset.bbox[ids], r_psf=iset.r_psf)
+        iset = sub_island_set(iset, island_ids)
     native = int(cut.psfs[0].H_psf)
     buckets = bucket_islands(iset, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
     plans, capped = plan_island_classes(iset, buckets, cut.stype, cut.logre, cut.sersic_n,
@@ -526,31 +560,71 @@ def _rel_stats(f, ref, ref_err):
                 max=float(r.max()), frac_le_1e5=float((r <= 1e-5).mean()))
 
 
-def cutout_accuracy(cut, cfg: FitConfig, island_ids, sync=None, psf_of=None) -> dict:
-    """Flux error of the float32 fit against the float64 fit on the given
-    islands, per source as max over bands of |f32 - f64| / max(|f64|,
-    sigma64):
-      * ``fit``: both full LM fits.  Where the chi2 landscape of a blend is
-        flat, float32 and float64 trajectories may stop at different
-        positions, and the fluxes follow them.
-      * ``flux_solve``: the float32 flux solve (no LM step) at the float64
-        fit's parameters -- the arithmetic precision of the fluxes."""
+def sub_island_set(iset, island_ids):
+    """The islands ``island_ids`` of an island set (members, boxes, sizes)."""
+    from .islands import IslandSet
+    ids = np.asarray(island_ids, np.int64)
+    return IslandSet(n_sources=iset.n_sources, n_islands=ids.size, labels=iset.labels,
+                     radius=iset.radius, members=[iset.members[i] for i in ids],
+                     n_src=iset.n_src[ids], bbox=iset.bbox[ids], r_psf=iset.r_psf)
+
+
+def island_accuracy(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, island_ids, cat,
+                    free_n, cfg: FitConfig, grams=("fp32",), sync=None,
+                    joint_nsrc_cap=JOINT_NSRC_CAP) -> dict:
+    """Flux error of float32 island fits (one per Gram operand precision in
+    ``grams``) against the float64 fit of the same islands, per source as
+    max over bands of |f - f64| / max(|f64|, sigma64):
+      * ``fit``: full LM fits.  Where the chi2 landscape of a blend is flat,
+        the trajectories may stop at different positions and the fluxes
+        follow them.
+      * ``flux_solve``: the flux solve (no LM step) at the float64 fit's
+        parameters -- the arithmetic precision of the fluxes.
+    ``cat`` = (x, y, stype, logre, ee1, ee2, sersic_n) catalog seeds."""
     import dataclasses
+    x, y, st, lr, e1, e2, sn = cat
+    sub = sub_island_set(iset, island_ids)
+    buckets = bucket_islands(sub, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
+    plans, _ = plan_island_classes(sub, buckets, st, lr, sn, native,
+                                   joint_nsrc_cap=joint_nsrc_cap)
+
+    def run(c, p_start=None):
+        return fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, sub, plans, x,
+                              y, st, lr, e1, e2, sn, free_n, c, sync=sync, p_start=p_start)
+    c64 = dataclasses.replace(cfg, precision="fp64", gram="fp32", post_fit=False)
+    f64 = run(c64)
+    fz64 = run(dataclasses.replace(c64, max_steps=0), f64.p)
+    out = dict(n_islands=int(sub.n_islands),
+               flux_solve_fp64_self=None)
+    ok = f64.fitted
+    out["flux_solve_fp64_self"] = _rel_stats(fz64.flux[ok], f64.flux[ok], f64.flux_err[ok])
+    for g in grams:
+        c32 = dataclasses.replace(cfg, precision="fp32", gram=g, post_fit=False)
+        f32 = run(c32)
+        fz = run(dataclasses.replace(c32, max_steps=0), f64.p)
+        ok = f32.fitted & f64.fitted
+        same = ok & (f32.nst == f64.nst) & (f32.conv == f64.conv) & (f32.stall == f64.stall)
+        out[g] = dict(same_trajectory=float(same.sum() / max(ok.sum(), 1)),
+                      fit=_rel_stats(f32.flux[ok], f64.flux[ok], f64.flux_err[ok]),
+                      flux_solve=_rel_stats(fz.flux[ok], f64.flux[ok], f64.flux_err[ok]),
+                      position_max_px=float(np.max(np.hypot(f32.x[ok] - f64.x[ok],
+                                                            f32.y[ok] - f64.y[ok]),
+                                                   initial=0.0)))
+    return out
+
+
+def cutout_accuracy(cut, cfg: FitConfig, island_ids, sync=None, psf_of=None,
+                    grams=("fp32",)) -> dict:
+    """:func:`island_accuracy` on island cutouts (every island fitted
+    jointly, free-n gate from the true flux as in :func:`fit_cutouts`)."""
     psf_of = psf_of or psf_bank(cut.psfs, cut.dstack.device)
-    c32 = dataclasses.replace(cfg, precision="fp32")
-    c64 = dataclasses.replace(cfg, precision="fp64")
-    f32 = fit_cutouts(cut, c32, island_ids=island_ids, sync=sync, psf_of=psf_of)
-    f64 = fit_cutouts(cut, c64, island_ids=island_ids, sync=sync, psf_of=psf_of)
-    fz = fit_cutouts(cut, dataclasses.replace(c32, max_steps=0), island_ids=island_ids,
-                     sync=sync, psf_of=psf_of, p_start=f64.p)
-    fz64 = fit_cutouts(cut, dataclasses.replace(c64, max_steps=0), island_ids=island_ids,
-                       sync=sync, psf_of=psf_of, p_start=f64.p)
-    ok = f32.fitted & f64.fitted
-    same = ok & (f32.nst == f64.nst) & (f32.conv == f64.conv) & (f32.stall == f64.stall)
-    return dict(n_islands=int(np.asarray(island_ids).size),
-                same_trajectory=float(same.sum() / max(ok.sum(), 1)),
-                fit=_rel_stats(f32.flux[ok], f64.flux[ok], f64.flux_err[ok]),
-                flux_solve=_rel_stats(fz.flux[ok], f64.flux[ok], f64.flux_err[ok]),
-                flux_solve_fp64_self=_rel_stats(fz64.flux[ok], f64.flux[ok], f64.flux_err[ok]),
-                position_max_px=float(np.max(np.hypot(f32.x[ok] - f64.x[ok],
-                                                       f32.y[ok] - f64.y[ok]), initial=0.0)))
+    var = (np.asarray(cut.sky, np.float64) + 25.0)[None, :] * 50.0
+    free_n = free_n_gate(cut.stype, cut.true_flux, np.broadcast_to(var, cut.true_flux.shape))
+    acc = island_accuracy(psf_of, cut.n_bands, int(cut.psfs[0].n_terms),
+                          int(cut.psfs[0].H_psf), cut.dstack, cut.wstack, cut.pad,
+                          cut.island_set, island_ids, (cut.x, cut.y, cut.stype, cut.logre,
+                                                       cut.ee1, cut.ee2, cut.sersic_n),
+                          free_n, cfg, grams=grams, sync=sync, joint_nsrc_cap=None)
+    if tuple(grams) == ("fp32",):
+        acc.update(acc.pop("fp32"))   # flat layout for the single-precision comparison
+    return acc
