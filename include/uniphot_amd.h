@@ -304,6 +304,21 @@ int uph_render_stamps(int n, int T, int n_terms, const float* psf_coefs, const f
  *   outputs fluxes, flux_var (total_src) and bg, bg_var (n_prob, M) f32.
  * ``scratch`` must hold uph_band_scratch_bytes(n_prob, max_src, bg_order).
  */
+/*
+ * Wing images of large extended sources for the linear solve's wing
+ * peeling (solver_grid.py:1578-1626 _compute_wing_images; the final solve
+ * fits data - wing, which by linearity is the solve of data minus the
+ * solve of the wing image).  Problem q writes its window win[q] = (band,
+ * x0, y0, h, w) of out (n_bands, H, W) f32: per pixel, the sum over its
+ * sources s in [src_off[q], src_off[q+1]) of flux[s] * (big[s] - small[s]),
+ * big (n, Tb, Tb) unit stamps at window-local top-left (ix[s], iy[s]) and
+ * small (n, Ts, Ts) unit stamps centred on the same pixel (Tb, Ts odd).
+ * Sources are summed in order; the windows of one call must be disjoint.
+ */
+int uph_wing_images(int n_prob, int Tb, int Ts, const int64_t* src_off, const int32_t* win,
+                    const float* big, const float* small, const int32_t* ix, const int32_t* iy,
+                    const float* flux, int H, int W, float* out, void* stream);
+
 size_t uph_band_scratch_bytes(int n_prob, int max_src, int bg_order);
 int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const int64_t* src_off,
                    const int32_t* win, int H, int W, int img_x0, int img_y0, const float* data,

@@ -117,3 +117,22 @@ def solve_band(data, invvar, pix_good, stamps, ix_top, iy_top, nbr_idx, nbr_mask
     inv = np.linalg.inv(A)
     var = np.diag(inv)
     return x[:n], x[n:], var[:n], var[n:]
+
+
+def wing_image(psf, px, py, gx0, gy0, stype, lr, e1, e2, sn, flux, h, w, Tb, Ts):
+    """Wing of the big sources on an (h, w) window whose pixel (0, 0) is
+    full-image (gx0, gy0) (solver_grid.py:1578-1626): sum over sources of
+    flux x (unit render at stamp size Tb - unit render at Ts), each render
+    placed at round(pos - origin) - size // 2 and clipped to the window."""
+    out = np.zeros((h, w))
+    for T, sign in ((Tb, 1.0), (Ts, -1.0)):
+        p = dict(psf, coefs=core.pad_psf_coefs(psf["coefs"], T))
+        for i in range(px.size):
+            s, _ = core.member_stamp(p, float(px[i]), float(py[i]), gx0, gy0, int(stype[i]),
+                                     float(lr[i]), float(e1[i]), float(e2[i]), float(sn[i]))
+            ix, iy = core.stamp_top_left(float(px[i]), float(py[i]), gx0, gy0, T)
+            y0, x0 = max(0, iy), max(0, ix)
+            y1, x1 = min(h, iy + T), min(w, ix + T)
+            if y1 > y0 and x1 > x0:
+                out[y0:y1, x0:x1] += sign * float(flux[i]) * s[y0 - iy:y1 - iy, x0 - ix:x1 - ix]
+    return out

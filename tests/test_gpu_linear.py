@@ -76,3 +76,65 @@ def test_band_solve_matches_oracle(cuda):
         assert np.allclose(r.flux_var, fv, rtol=1e-5)
         assert np.allclose(r.bg_coefs, bgc, rtol=1e-4, atol=1e-4 * np.sqrt(bv).max())
         assert r.converged
+
+
+def test_wing_peeling_matches_oracle(cuda):
+    """Wing peeling of one grid at the native stamp: the device path (S1
+    solve, wing images of the big sources from the S1 fluxes, solve of the
+    wing subtracted by linearity) against the oracle's S1 solve and its
+    direct solve of data - wing."""
+    import torch
+    from _scenes import small_mixed_scene
+    from uniphot_amd import forced_phot_sparse as fps
+    from uniphot_amd.grid import GridDef
+    from uniphot_amd.pipeline import DeviceField, FitConfig, wing_peel, wing_sources
+    sc = small_mixed_scene(seed=9, n_bands=2, n_src=12, order=1, types=(0, 1, 2))
+    st, lr, sn = sc.stype.copy(), sc.logre.copy(), sc.sersic_n.copy()
+    dev_i = int(np.where(st == 2)[0][0])          # a de Vaucouleurs galaxy made big
+    lr[dev_i] = np.float32(np.log(6.0))
+    nb, H, W = sc.data.shape
+    fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
+    g = GridDef(0, 0, 0, 20, 180, 20, 180, 5, 195, 5, 195, 15)
+    inside = (sc.x >= g.x0_pad) & (sc.x < g.x1_pad) & (sc.y >= g.y0_pad) & (sc.y < g.y1_pad)
+    fi = np.where(inside)[0]
+    g.source_indices = fi
+    g.halo_source_indices = np.zeros(0, np.int64)
+    assert dev_i in fi
+    big = wing_sources(st[fi], lr[fi], sn[fi], 55, 64)
+    assert big.size >= 1
+    xs, ys = sc.x[fi], sc.y[fi]
+    nbr, msk = fps.build_neighbor_list(xs, ys, 55, 55)
+    probs = [fps.BandProblem(b, g.x0_pad, g.y0_pad, g.pad_h, g.pad_w, xs, ys, nbr, stype=st[fi],
+                             logre=lr[fi], ee1=sc.ee1[fi], ee2=sc.ee2[fi], sersic_n=sn[fi])
+             for b in range(nb)]
+    bg_geo = (0.5 * (W - 1), 0.5 * W, 0.5 * (H - 1), 0.5 * H)
+    c55, g55 = fld.psf(55)
+    s1 = fps.solve_problems(probs, c55, g55, 55, fld.n_terms, fld.data, fld.invvar, fld.good,
+                            bg_order=1, bg_geo=bg_geo)
+    cfg = FitConfig()
+    fin = wing_peel(fld, 55, probs, [(g, b) for b in range(nb)], s1, sc.x, sc.y, st, lr,
+                    sc.ee1, sc.ee2, sn, cfg, bg_geo)
+    h, w = g.pad_h, g.pad_w
+    basis = olin.bg_basis(h, w, g.x0_pad, g.y0_pad, *bg_geo, order=1)
+    ix, iy = fps.stamp_top_left(xs, ys, g.x0_pad, g.y0_pad, 55, 55)
+    for b in range(nb):
+        psf = _psf_dict(sc.psfs[b])
+        stamps = olin.render_grid_stamps(psf, xs, ys, g.x0_pad, g.y0_pad, st[fi], lr[fi],
+                                         sc.ee1[fi], sc.ee2[fi], sn[fi])
+        sl = (b, slice(g.y0_pad, g.y1_pad), slice(g.x0_pad, g.x1_pad))
+        d, iv, gd = (sc.data[sl].astype(np.float64), sc.invvar[sl].astype(np.float64),
+                     sc.good[sl])
+        f1, _, _, _ = olin.solve_band(d, iv, gd, stamps, ix, iy, nbr, msk, basis)
+        bi = fi[big]
+        wing = olin.wing_image(psf, sc.x[bi], sc.y[bi], g.x0_pad, g.y0_pad, st[bi], lr[bi],
+                               sc.ee1[bi], sc.ee2[bi], sn[bi], f1[big], h, w, 221, 55)
+        assert np.abs(wing).max() > 0
+        f, bgc, fv, bv = olin.solve_band(d - wing, iv, gd, stamps, ix, iy, nbr, msk, basis)
+        r = fin[b]
+        sig = np.sqrt(fv)
+        err = np.max(np.abs(r.fluxes - f) / np.maximum(np.abs(f), sig))
+        moved = np.max(np.abs(f - f1) / np.maximum(np.abs(f), sig))
+        print("band", b, "wing moved fluxes by", moved, "device vs oracle", err, flush=True)
+        assert moved > 1e-4
+        assert err < 1e-4
+        assert np.allclose(r.flux_var, fv, rtol=1e-5)

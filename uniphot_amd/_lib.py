@@ -20,6 +20,7 @@ EXPORTS = (
     "uph_island_fit_batch", "uph_island_split_counters", "uph_island_fit_split",
     "uph_prerender_scratch_bytes", "uph_prerender_spectra_floats",
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
+    "uph_wing_images",
 )
 
 
@@ -128,6 +129,9 @@ def lib() -> C.CDLL:
     L.uph_band_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int,
                                  C.c_int, C.c_int, _P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
                                  _P, _P, _P, _P, C.c_float, _P, _P, _P, _P, _P, C.c_size_t, _P]
+    L.uph_wing_images.restype = C.c_int
+    L.uph_wing_images.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P,
+                                  C.c_int, C.c_int, _P, _P]
     _lib = L
     return L
 

@@ -401,6 +401,40 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_solve_kernel(BandArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Wing images of large extended sources (solver_grid.py:1578-1626): problem
+// q's window (band, x0, y0, h, w) gets, per pixel, the sum over its sources
+// s in [src_off[q], src_off[q+1]) of flux[s] * (big[s] - small[s]), big the
+// Tb x Tb unit stamp at window-local top-left (ix[s], iy[s]) and small the
+// Ts x Ts stamp centred on the same pixel.  Sources are summed in order per
+// pixel (deterministic); the windows of one launch must be disjoint.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(UPH_BLOCK) void wing_images_kernel(
+    int Tb, int Ts, const int64_t* __restrict__ src_off, const int32_t* __restrict__ win,
+    const float* __restrict__ big, const float* __restrict__ small,
+    const int32_t* __restrict__ ix, const int32_t* __restrict__ iy,
+    const float* __restrict__ flux, int H, int W, float* __restrict__ out) {
+  const int q = blockIdx.x;
+  const int band = win[q * 5], x0 = win[q * 5 + 1], y0 = win[q * 5 + 2];
+  const int h = win[q * 5 + 3], w = win[q * 5 + 4];
+  const int64_t s0 = src_off[q], s1 = src_off[q + 1];
+  const int o = Tb / 2 - Ts / 2;
+  float* img = out + (size_t)band * H * W;
+  for (int e = blockIdx.y * UPH_BLOCK + threadIdx.x; e < h * w; e += gridDim.y * UPH_BLOCK) {
+    const int py = e / w, px = e % w;
+    float v = 0.0f;
+    for (int64_t s = s0; s < s1; ++s) {
+      const int u = px - ix[s], t = py - iy[s];
+      if (u < 0 || u >= Tb || t < 0 || t >= Tb) continue;
+      float d = big[(size_t)s * Tb * Tb + (size_t)t * Tb + u];
+      const int us = u - o, ts = t - o;
+      if (us >= 0 && us < Ts && ts >= 0 && ts < Ts) d -= small[(size_t)s * Ts * Ts + ts * Ts + us];
+      v = fmaf(flux[s], d, v);
+    }
+    img[(size_t)(y0 + py) * W + x0 + px] = v;
+  }
+}
+
 }  // namespace uph
 
 static int n_terms_of_order(int order) { return (order + 1) * (order + 2) / 2; }
@@ -522,4 +556,19 @@ extern "C" int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const 
   else
     hipLaunchKernelGGL(uph::band_solve_kernel<6>, dim3(n_prob), dim3(UPH_BLOCK), 0, s, a);
   return uph::hip_status(hipGetLastError(), "uph_band_solve launch");
+}
+
+extern "C" int uph_wing_images(int n_prob, int Tb, int Ts, const int64_t* src_off,
+                               const int32_t* win, const float* big, const float* small,
+                               const int32_t* ix, const int32_t* iy, const float* flux, int H,
+                               int W, float* out, void* stream) {
+  UPH_CHECK_ARG(n_prob >= 0 && Tb >= Ts && Ts >= 1 && (Tb % 2) == 1 && (Ts % 2) == 1,
+                "uph_wing_images: n_prob=%d Tb=%d Ts=%d (odd, Tb >= Ts)", n_prob, Tb, Ts);
+  if (n_prob == 0) return UPH_OK;
+  UPH_CHECK_ARG(src_off && win && big && small && ix && iy && flux && out,
+                "uph_wing_images: null argument");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(uph::wing_images_kernel, dim3(n_prob, 16), dim3(UPH_BLOCK), 0, s, Tb, Ts,
+                     src_off, win, big, small, ix, iy, flux, H, W, out);
+  return uph::hip_status(hipGetLastError(), "uph_wing_images launch");
 }

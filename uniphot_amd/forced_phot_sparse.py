@@ -197,3 +197,34 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
                                           bool(np.all(np.isfinite(f)) and
                                                np.all(np.isfinite(bgh[q])))))
     return out
+
+
+def render_unit_stamps(psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, band, pos_x, pos_y,
+                       origin_x, origin_y, stype, logre, ee1, ee2, sersic_n, *, stream=None,
+                       sersic_dev=None):
+    """(n, T, T) unit-flux stamps on the device (``uph_render_stamps``, no
+    unit-sum renormalisation): PSF at the position for stars, the Fourier
+    mixture convolution for extended sources, Lanczos-3 shift by the
+    remainder of (pos - origin) (source_model.py:331-390)."""
+    import torch
+    dev = psf_coefs_dev.device
+    L = _lib.lib()
+    P = _lib.ptr
+    n = int(np.asarray(pos_x).size)
+    out = torch.empty((max(n, 1), T, T), dtype=torch.float32, device=dev)
+    if n == 0:
+        return out[:0]
+    up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+    if sersic_dev is None:
+        from .sersic_table import device_table
+        sersic_dev = device_table(dev)
+    sb = L.uph_render_scratch_bytes(T, max(1, min(n, 1024)))
+    scratch = torch.empty(max(sb, 1), dtype=torch.uint8, device=dev)
+    args = [up(band, np.int32), up(pos_x, np.float32), up(pos_y, np.float32),
+            up(origin_x, np.float32), up(origin_y, np.float32), up(stype, np.int32),
+            up(logre, np.float32), up(ee1, np.float32), up(ee2, np.float32),
+            up(sersic_n, np.float32)]
+    _lib.check(L.uph_render_stamps(n, T, n_terms, P(psf_coefs_dev), P(psf_geo_dev),
+                                   P(sersic_dev), *(P(a) for a in args), 0, P(out), P(scratch),
+                                   sb, _lib.stream_ptr(stream)))
+    return out

@@ -47,7 +47,10 @@ class FitConfig:
     reg: float = 1e-9
     k_max: int = 32
     precision: str = "fp32"
-    stamp_tiers: tuple = (55, 111, 221)
+    stamp_tiers: tuple = ()      # whole-grid stamp-tier promotion ladder; () = native only
+    wing_peeling: bool = True    # large galaxies: subtract tier-wing_tier wings (native solve)
+    wing_tier: int = 221
+    wing_max_sources: int = 64
     island_split: bool = True    # band-split island LM (uph_island_fit_split)
     galaxy_prerender: bool = True  # split path: galaxies rendered by their own launches
     post_fit: bool = True        # parameter variances, marginal flux errors, source metrics
@@ -148,6 +151,107 @@ def grid_tiers(fld: DeviceField, grids, stype, logre, sn, tiers):
                         fld.native) for g in grids}
 
 
+def wing_sources(stype, logre, sersic_n, native: int, max_sources: int) -> np.ndarray:
+    """Indices of the "big" sources whose wings are peeled: extended and
+    nre(n) Re > native / 2 with nre = 4 / 6 / 8 for n < 1.5 / < 3 / >= 3,
+    the largest ``max_sources`` by nre Re (solver_grid.py:1517-1548)."""
+    n = np.asarray(sersic_n, np.float64)
+    nre = np.where(n < 1.5, 4.0, np.where(n < 3.0, 6.0, 8.0))
+    req = nre * np.exp(np.asarray(logre, np.float64))
+    big = (np.asarray(stype) != SOURCE_STAR) & np.isfinite(req) & (req > 0.5 * float(native))
+    idx = np.where(big)[0]
+    if idx.size > int(max_sources):
+        idx = np.sort(idx[np.argsort(-req[idx], kind="stable")[:int(max_sources)]])
+    return idx.astype(np.int64)
+
+
+def _disjoint_colors(grids):
+    """Colour grids so that the padded windows of one colour are disjoint."""
+    colors, boxes = {}, []
+    for g in grids:
+        c = 0
+        while any(not (g.x1_pad <= b[0] or b[2] <= g.x0_pad or g.y1_pad <= b[1] or
+                       b[3] <= g.y0_pad) for b in boxes if b[4] == c):
+            c += 1
+        colors[g.grid_id] = c
+        boxes.append((g.x0_pad, g.y0_pad, g.x1_pad, g.y1_pad, c))
+    return colors
+
+
+def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
+              cfg: FitConfig, bg_geo, timer=None):
+    """Wing peeling of a grid pass at the native stamp (solver_grid.py:
+    2495-2546, :2826-2846): per grid, the big sources' wings -- their S1
+    flux times the tier-wing_tier render minus the native render -- are
+    subtracted from the data before the final solve.  The final solve is
+    linear in the data, so it equals the S1 solve minus the same solve of
+    the wing image: the wing images are built on the device
+    (``uph_wing_images``, grids of one colour at a time so windows are
+    disjoint), solved, and their fluxes and background subtracted."""
+    import torch
+    from . import _lib
+    if T != fld.native:
+        return res
+    nb, H, W = fld.nb, fld.H, fld.W
+    Tb, Ts = int(cfg.wing_tier), int(fld.native)
+    big_of = {}
+    for (g, b) in owners:
+        if g.grid_id not in big_of:
+            fi = g.fit_indices
+            big_of[g.grid_id] = wing_sources(stype[fi], logre[fi], sn[fi], Ts,
+                                             cfg.wing_max_sources)
+    qs = [q for q, (g, b) in enumerate(owners) if big_of[g.grid_id].size]
+    if not qs:
+        return res
+    colors = _disjoint_colors([g for g, _ in {id(g): (g, 0) for g, _ in owners}.values()])
+    L = _lib.lib()
+    P = _lib.ptr
+    dev = fld.data.device
+    up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+    cb, gb = fld.psf(Tb)
+    cs, gs = fld.psf(Ts)
+    out = list(res)
+    for c in sorted({colors[owners[q][0].grid_id] for q in qs}):
+        sel = [q for q in qs if colors[owners[q][0].grid_id] == c]
+        rows = []   # (problem, local index in its fit set, global source index)
+        for q in sel:
+            g, b = owners[q]
+            for k in big_of[g.grid_id]:
+                rows.append((q, int(k), int(g.fit_indices[k])))
+        qa = np.array([r[0] for r in rows])
+        src = np.array([r[2] for r in rows])
+        band = np.array([owners[q][1] for q in qa], np.int32)
+        ox = np.array([owners[q][0].x0_pad for q in qa], np.float32)
+        oy = np.array([owners[q][0].y0_pad for q in qa], np.float32)
+        cat = (x[src], y[src], stype[src], logre[src], ee1[src], ee2[src], sn[src])
+        s_big = fps.render_unit_stamps(cb, gb, Tb, fld.n_terms, band, cat[0], cat[1], ox, oy,
+                                       *cat[2:])
+        s_small = fps.render_unit_stamps(cs, gs, Ts, fld.n_terms, band, cat[0], cat[1], ox, oy,
+                                         *cat[2:])
+        ix, iy = fps.stamp_top_left(cat[0], cat[1], ox, oy, Tb, Tb)
+        fl = np.array([res[q].fluxes[k] for q, k, _ in rows], np.float32)
+        fl = np.where(np.isfinite(fl), fl, 0.0).astype(np.float32)
+        counts = np.bincount(np.searchsorted(sel, qa), minlength=len(sel))
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        win = np.array([[owners[q][1], problems[q].x0, problems[q].y0, problems[q].h,
+                         problems[q].w] for q in sel], np.int32)
+        wing = torch.zeros((nb, H, W), dtype=torch.float32, device=dev)
+        keep = [up(off, np.int64), up(win, np.int32), up(ix, np.int32), up(iy, np.int32),
+                up(fl, np.float32)]
+        _lib.check(L.uph_wing_images(len(sel), Tb, Ts, P(keep[0]), P(keep[1]), P(s_big),
+                                     P(s_small), P(keep[2]), P(keep[3]), P(keep[4]), H, W,
+                                     P(wing), _lib.stream_ptr(None)))
+        rw = fps.solve_problems([problems[q] for q in sel], cs, gs, Ts, fld.n_terms, wing,
+                                fld.invvar, fld.good, bg_order=cfg.bg_order, bg_geo=bg_geo,
+                                reg=cfg.reg, k_max=cfg.k_max, timer=timer)
+        for q, r in zip(sel, rw):
+            r0 = out[q]
+            out[q] = fps.SparseForcedPhotResult(r0.fluxes - r.fluxes, r0.bg_coefs - r.bg_coefs,
+                                                r0.flux_var, r0.bg_var,
+                                                r0.converged and r.converged)
+    return out
+
+
 def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: FitConfig,
                 gtier: dict, timer=None):
     """Per-grid, per-band linear solve; returns fluxes/var (n, nb) for each
@@ -185,10 +289,14 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
         res = []
         for a, z in zip(cut[:-1], cut[1:]):
             if z > a:
-                res += fps.solve_problems(problems[a:z], coefs, geo, T, fld.n_terms, fld.data,
+                part = fps.solve_problems(problems[a:z], coefs, geo, T, fld.n_terms, fld.data,
                                           fld.invvar, fld.good, bg_order=cfg.bg_order,
                                           bg_geo=bg_geo, reg=cfg.reg, k_max=cfg.k_max,
                                           timer=timer)
+                if cfg.wing_peeling:
+                    part = wing_peel(fld, T, problems[a:z], owners[a:z], part, x, y, stype,
+                                     logre, ee1, ee2, sn, cfg, bg_geo, timer)
+                res += part
         for (g, b), r in zip(owners, res):
             own = g.source_indices.size
             flux[g.source_indices, b] = r.fluxes[:own]
