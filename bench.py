@@ -161,6 +161,7 @@ def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0):
     art = res.record["_artifacts"]
     d = art["dstack"].cpu().numpy()
     w = art["wstack"].cpu().numpy()
+    scene = art.get("scene")
     o = oisl.LMOptions(reg=cfg.reg)
     psfs = [dict(coefs=p.coefs.astype(np.float64), x_origin=p.x_origin, x_scale=p.x_scale,
                  y_origin=p.y_origin, y_scale=p.y_scale) for p in fld.psfs]
@@ -177,7 +178,14 @@ def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0):
         B, T = plan.bbox_class, plan.tier
         ox, oy = int(host["ox"][r]), int(host["oy"][r])
         ps = [dict(q, coefs=ocore.pad_psf_coefs(q["coefs"], T)) for q in psfs]
-        isl = oisl.Island(d[:, oy:oy + B, ox:ox + B], w[:, oy:oy + B, ox:ox + B],
+        cut = d[:, oy:oy + B, ox:ox + B]
+        if scene is not None:
+            # member add-back of the scene subtraction
+            mem = plan.midx[r][host["mmask"][r]]
+            cut = cut + oisl.member_addback(ps, host["x0b"][r], host["y0b"][r], B,
+                                            scene.x[mem], scene.y[mem], scene.stype[mem],
+                                            scene.flux[mem])
+        isl = oisl.Island(cut, w[:, oy:oy + B, ox:ox + B],
                           host["x0b"][r], host["y0b"][r], host["st"][r], host["sn"][r],
                           host["fn_flag"][r], host["mmask"][r], host["keep"][r],
                           host["lower"][r].ravel(), host["upper"][r].ravel(), ps, T)
@@ -438,7 +446,7 @@ def bench_precision(args, torch):
     sample = np.sort(rng.choice(fit_ids, min(args.sample, fit_ids.size), replace=False))
     acc = island_accuracy(fld.psf, fld.nb, fld.n_terms, fld.native, art["dstack"],
                           art["wstack"], art["pad"], iset, sample, cat, art["free_n"], base,
-                          grams=("bf16", "fp32"), sync=sync)
+                          grams=("bf16", "fp32"), sync=sync, scene=art["scene"])
     print(f"[bench] accuracy {acc}", file=sys.stderr, flush=True)
     ks = timer.summary()
     line = {

@@ -319,6 +319,59 @@ int uph_wing_images(int n_prob, int Tb, int Ts, const int64_t* src_off, const in
                     const float* big, const float* small, const int32_t* ix, const int32_t* iy,
                     const float* flux, int H, int W, float* out, void* stream);
 
+/*
+ * Weighted stamp sums into disjoint windows of an image stack: the scene
+ * subtraction's field-wide STAR subtract at S1 flux (replaces the per-source
+ * scatter `dmb[bi, ...] -= _f * _st[...]`, island_stage.py:2040-2117) and the
+ * island member add-back (replaces addback_v, island_solver.py:1795-1818,
+ * applied at :2779-2789).  Window q = win[q] = (x0, y0, h, w) of band
+ * band0 + bl (bl < nbc) of out (n_bands, H, W) f32 gets
+ *   out += scale * sum_{k in [off[q], off[q+1])} flux[s*nbc + bl] * stamp(bl, s)
+ * with s = list[k], stamp(bl, s) the T x T block s of band chunk bl of
+ * stamps (nbc, n_st, T, T) f32 placed at top-left (ix[s], iy[s]) in out
+ * coordinates (zero outside it).  Windows lie inside out and are disjoint;
+ * max_pix is the largest h * w.  Entries are summed in list order.
+ */
+int uph_stamp_sum(int n_win, int max_pix, int T, int n_st, int nbc, int band0, int n_bands,
+                  const int64_t* off, const int32_t* win, const int32_t* list, const int32_t* ix,
+                  const int32_t* iy, const float* stamps, const float* flux, float scale, int H,
+                  int W, float* out, void* stream);
+
+/*
+ * Island window gather (replaces _gather_jit / gather, island_solver.py:
+ * 1820-1830): window q = win[q] = (sx, sy, dx, dy, side) copies the
+ * side x side block at (sx, sy) of every band of src_a (n_bands, Hs, Ws) to
+ * (dx, dy) of dst_a (n_bands, Hd, Wd), and likewise src_b -> dst_b when both
+ * are given.  Blocks lie inside both stacks; max_side is the largest side.
+ */
+int uph_gather_windows(int n_win, int max_side, int n_bands, const int32_t* win, int Hs, int Ws,
+                       const float* src_a, const float* src_b, int Hd, int Wd, float* dst_a,
+                       float* dst_b, void* stream);
+
+/*
+ * Blank-response (empty-aperture) correction of a pass-2 grid-band solve
+ * (solver_grid.py:2867-2943): for problem q with window win[q] = (band, x0,
+ * y0, h, w) of the (n_bands, H, W) images, the model of its sources
+ * [src_off[q], src_off[q+1]) -- flux[s] times the unit stamp s (T x T at
+ * window-local top-left (ix_top[s], iy_top[s])) plus the background bg[q]
+ * (order bg_order, basis as in uph_band_solve) -- the weighted residual
+ * w (d - wing - model) on good pixels (wing (n_bands, H, W) where
+ * use_wing[q], else none), resp = corr(w r, phi) / corr(w, phi^2) ('same'
+ * correlation with the template phi[q] (T x T, clipped >= 0, unit sum);
+ * NaN where the denominator is 0 or the pixel is not good) and
+ * b_hat[q] = nanmedian(resp) (NaN if no value).  The caller subtracts
+ * b_hat from the STAR fluxes.  work: uph_blank_work_floats(n_prob,
+ * max_pix) floats, max_pix >= every h * w.
+ */
+size_t uph_blank_work_floats(int n_prob, int max_pix);
+int uph_blank_response(int n_prob, int T, int bg_order, const int64_t* src_off,
+                       const int32_t* win, int H, int W, const float* data, const float* invvar,
+                       const uint8_t* good, const float* wing, const uint8_t* use_wing,
+                       const float* stamps, const int32_t* ix_top, const int32_t* iy_top,
+                       const float* flux, const float* bg, double bg_xo, double bg_xs,
+                       double bg_yo, double bg_ys, const float* phi, float* work,
+                       size_t work_floats, int max_pix, double* b_hat, void* stream);
+
 size_t uph_band_scratch_bytes(int n_prob, int max_src, int bg_order);
 int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const int64_t* src_off,
                    const int32_t* win, int H, int W, int img_x0, int img_y0, const float* data,

@@ -6,7 +6,9 @@ Reference: ``tract7dt/fit/grid_gpu/island_solver.py`` -- ``models_band``
 ``final_fluxes`` (:1467-1482), ``final_errors`` (:1484-1547), ``final_flux_marg``
 (:1549-1612), ``source_metrics`` (:1643-1715), and the
 chunk loop of ``run_class_chunk`` (:2890-2977).  Host input layout:
-``island_plan.class_host_arrays`` (:493-699).
+``island_plan.class_host_arrays`` (:493-699).  Scene subtraction:
+``scene_star_image`` (island_stage.py:2040-2117) and ``member_addback``
+(island_solver.py:1795-1818).
 
 The Jacobian is formed the way the reference forms it: one explicit
 whitened-residual tangent per live parameter, per pixel, through the
@@ -18,7 +20,10 @@ Parity pins: tests/test_oracle_island.py checks the VarPro fluxes against
 ``numpy.linalg.lstsq``, the Jacobian against float64 central finite
 differences of the residual, and the LM invariants (chi2 non-increasing
 over accepts, bounds and clips respected, frozen parameters untouched,
-padded slots inert, convergence prongs).  Trajectories of the reference
+padded slots inert, convergence prongs), and the scene subtraction against
+an independent Gaussian-PSF field (stars at integer positions cancel to
+rounding, extended light untouched, members restored, non-integer shifts to
+the Lanczos interpolation error).  Trajectories of the reference
 JAX program itself are parity unpinned (JAX is not importable here).
 """
 from __future__ import annotations
@@ -361,3 +366,55 @@ def source_metrics(p, isl: Island, o: LMOptions):
         rchisq = np.where((rc_d > 0) & mm, rc_n / rc_d, np.nan)
     return np.stack([rchisq, fracflux, fracin, fracmasked], axis=1), np.where(mm[None], fwin,
                                                                              np.nan)
+
+
+def scene_star_image(shape, pad: int, psfs, x, y, stype, flux):
+    """Field-wide STAR scene of the scene subtraction (island_stage.py:1846-
+    1851, :2040-2117) on a padded (nb, Hp, Wp) stack: every STAR k with
+    flux[k, b] != 0 contributes flux[k, b] times its native unit stamp (PSF at
+    (x, y), Lanczos-shifted by x - round(x), y - round(y)) at top-left
+    round(pos) - T // 2 + pad, clipped to the stack.  The stack minus this
+    image is what the islands gather."""
+    nb, Hp, Wp = shape
+    out = np.zeros(shape)
+    T = psfs[0]["coefs"].shape[0]
+    for k in range(len(x)):
+        if int(stype[k]) != core.SOURCE_STAR:
+            continue
+        ix, iy = core.stamp_top_left(float(x[k]), float(y[k]), 0.0, 0.0, T)
+        ix += pad
+        iy += pad
+        y0, x0 = max(0, iy), max(0, ix)
+        y1, x1 = min(Hp, iy + T), min(Wp, ix + T)
+        if y1 <= y0 or x1 <= x0:
+            continue
+        for b in range(nb):
+            f = float(flux[k][b])
+            if f == 0.0:
+                continue
+            st, _ = core.member_stamp(psfs[b], float(x[k]), float(y[k]), 0.0, 0.0,
+                                      core.SOURCE_STAR)
+            out[b, y0:y1, x0:x1] += f * st[y0 - iy:y1 - iy, x0 - ix:x1 - ix]
+    return out
+
+
+def member_addback(psfs_tier, x0b, y0b, B: int, xs, ys, stypes, flux):
+    """Member add-back of one island (island_solver.py:1795-1818 at the seeds,
+    member fluxes from island_plan.py:664-680): (nb, B, B) sum over member
+    STARS of flux[k, b] times the unit stamp at the class's stamp tier placed
+    in the B x B window at origin (x0b, y0b); extended members add nothing."""
+    nb = len(psfs_tier)
+    T = psfs_tier[0]["coefs"].shape[0]
+    out = np.zeros((nb, B, B))
+    for k in range(len(xs)):
+        if int(stypes[k]) != core.SOURCE_STAR:
+            continue
+        ix, iy = core.stamp_top_left(float(xs[k]), float(ys[k]), float(x0b), float(y0b), T)
+        for b in range(nb):
+            f = float(flux[k][b])
+            if f == 0.0:
+                continue
+            st, _ = core.member_stamp(psfs_tier[b], float(xs[k]), float(ys[k]), float(x0b),
+                                      float(y0b), core.SOURCE_STAR)
+            out[b] += f * core.place(st, ix, iy, B)
+    return out

@@ -136,3 +136,34 @@ def wing_image(psf, px, py, gx0, gy0, stype, lr, e1, e2, sn, flux, h, w, Tb, Ts)
             if y1 > y0 and x1 > x0:
                 out[y0:y1, x0:x1] += sign * float(flux[i]) * s[y0 - iy:y1 - iy, x0 - ix:x1 - ix]
     return out
+
+
+def blank_response(data, invvar, pix_good, stamps, ix_top, iy_top, flux, bg_coefs, basis, phi,
+                   wing=None):
+    """Blank-response median of one grid-band window (solver_grid.py:2867-
+    2943): model = sum flux x unit stamps (placed, clipped) + basis @ bg;
+    w = invvar on good pixels, r = d - wing - model there; resp =
+    corr(w r, phi) / corr(w, phi^2), 'same' correlation through
+    scipy.signal.fftconvolve with the flipped template as the reference
+    does; NaN off the good pixels or where the denominator is 0;
+    b = nanmedian(resp)."""
+    from scipy.signal import fftconvolve
+    h, w = data.shape
+    T = stamps.shape[1]
+    model = (basis @ np.asarray(bg_coefs, np.float64)).reshape(h, w)
+    for s in range(stamps.shape[0]):
+        y0, x0 = max(0, iy_top[s]), max(0, ix_top[s])
+        y1, x1 = min(h, iy_top[s] + T), min(w, ix_top[s] + T)
+        if y1 > y0 and x1 > x0:
+            model[y0:y1, x0:x1] += float(flux[s]) * stamps[s][y0 - iy_top[s]:y1 - iy_top[s],
+                                                               x0 - ix_top[s]:x1 - ix_top[s]]
+    d = data - (wing if wing is not None else 0.0)
+    good = pix_good & (invvar > 0)
+    wgt = np.where(good, invvar, 0.0)
+    r = np.where(good, d - model, 0.0)
+    num = fftconvolve(wgt * r, phi[::-1, ::-1], mode="same")
+    den = fftconvolve(wgt, (phi * phi)[::-1, ::-1], mode="same")
+    with np.errstate(divide="ignore", invalid="ignore"):
+        resp = np.where(den > 0, num / np.maximum(den, 1e-30), np.nan)
+    resp = np.where(good, resp, np.nan)
+    return float(np.nanmedian(resp)) if np.isfinite(resp).any() else float("nan")

@@ -112,8 +112,8 @@ def test_wing_peeling_matches_oracle(cuda):
     s1 = fps.solve_problems(probs, c55, g55, 55, fld.n_terms, fld.data, fld.invvar, fld.good,
                             bg_order=1, bg_geo=bg_geo)
     cfg = FitConfig()
-    fin = wing_peel(fld, 55, probs, [(g, b) for b in range(nb)], s1, sc.x, sc.y, st, lr,
-                    sc.ee1, sc.ee2, sn, cfg, bg_geo)
+    fin, _ = wing_peel(fld, 55, probs, [(g, b) for b in range(nb)], s1, sc.x, sc.y, st, lr,
+                       sc.ee1, sc.ee2, sn, cfg, bg_geo)
     h, w = g.pad_h, g.pad_w
     basis = olin.bg_basis(h, w, g.x0_pad, g.y0_pad, *bg_geo, order=1)
     ix, iy = fps.stamp_top_left(xs, ys, g.x0_pad, g.y0_pad, 55, 55)
@@ -138,3 +138,55 @@ def test_wing_peeling_matches_oracle(cuda):
         assert moved > 1e-4
         assert err < 1e-4
         assert np.allclose(r.flux_var, fv, rtol=1e-5)
+
+
+def test_blank_response_matches_oracle(cuda):
+    """Pass-2 blank-response median of each band of one grid: device
+    (uph_blank_response) against the oracle (scipy fftconvolve +
+    nanmedian) on the same final fluxes and background; the STAR fluxes
+    move by exactly b_hat."""
+    from uniphot_amd import forced_phot_sparse as fps
+    from uniphot_amd.grid import GridDef
+    from uniphot_amd.pipeline import DeviceField, FitConfig, blank_response_correct, _pad_center
+    from uniphot_amd.psf_polynomial import evaluate_psf_at_np
+    sc = small_star_scene(n_src=20)
+    nb, H, W = sc.data.shape
+    fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
+    g = GridDef(0, 0, 0, 15, 185, 10, 150, 5, 195, 2, 160, 10)
+    inside = (sc.x >= g.x0_pad) & (sc.x < g.x1_pad) & (sc.y >= g.y0_pad) & (sc.y < g.y1_pad)
+    fi = np.where(inside)[0]
+    g.source_indices = fi
+    g.halo_source_indices = np.zeros(0, np.int64)
+    xs, ys = sc.x[fi], sc.y[fi]
+    nbr, msk = fps.build_neighbor_list(xs, ys, 55, 55)
+    probs = [fps.BandProblem(b, g.x0_pad, g.y0_pad, g.pad_h, g.pad_w, xs, ys, nbr)
+             for b in range(nb)]
+    bg_geo = (0.5 * (W - 1), 0.5 * W, 0.5 * (H - 1), 0.5 * H)
+    c55, g55 = fld.psf(55)
+    res = fps.solve_problems(probs, c55, g55, 55, fld.n_terms, fld.data, fld.invvar, fld.good,
+                             bg_order=1, bg_geo=bg_geo)
+    st = sc.stype.astype(np.int32)
+    zero = np.zeros_like(sc.x)
+    corr, b_hat = blank_response_correct(fld, 55, probs, [(g, b) for b in range(nb)], res,
+                                         sc.x, sc.y, st, zero, zero + 1, zero, zero + 1,
+                                         FitConfig(), bg_geo)
+    h, w = g.pad_h, g.pad_w
+    basis = olin.bg_basis(h, w, g.x0_pad, g.y0_pad, *bg_geo, order=1)
+    ix, iy = fps.stamp_top_left(xs, ys, g.x0_pad, g.y0_pad, 55, 55)
+    for b in range(nb):
+        psf = _psf_dict(sc.psfs[b])
+        stamps = np.stack([ocore.member_stamp(psf, float(xx), float(yy), g.x0_pad, g.y0_pad, 0,
+                                              renorm=False)[0] for xx, yy in zip(xs, ys)])
+        phi = np.clip(evaluate_psf_at_np(sc.psfs[b], g.x0_pad + 0.5 * w,
+                                         g.y0_pad + 0.5 * h).astype(np.float64), 0.0, None)
+        phi = _pad_center(phi, 55)
+        phi /= phi.sum()
+        sl = (b, slice(g.y0_pad, g.y1_pad), slice(g.x0_pad, g.x1_pad))
+        ref = olin.blank_response(sc.data[sl].astype(np.float64), sc.invvar[sl].astype(np.float64),
+                                  sc.good[sl], stamps, ix, iy, res[b].fluxes.astype(np.float64),
+                                  res[b].bg_coefs, basis, phi)
+        sig = np.sqrt(np.median(res[b].flux_var))
+        print("band", b, "b_hat device", b_hat[b], "oracle", ref, "flux sigma", sig, flush=True)
+        assert abs(b_hat[b] - ref) < 1e-3 * sig + 1e-5 * abs(ref)
+        np.testing.assert_allclose(res[b].fluxes - corr[b].fluxes, np.float32(b_hat[b]),
+                                   rtol=1e-6, atol=2e-7 * np.abs(res[b].fluxes).max())

@@ -109,6 +109,14 @@ def test_argument_errors_return_status_and_message(L):
         _lib.check(L.uph_band_solve(-1, 55, 1, 32, *([None] * 2), 0, 0, 0, 0, *([None] * 3),
                                     0.0, 1.0, 0.0, 1.0, *([None] * 4), 1e-9, *([None] * 4),
                                     None, 0, None))
+    with pytest.raises(_lib.UphError) as ei:
+        _lib.check(L.uph_stamp_sum(1, 64, 55, 1, 3, 0, 2, *([None] * 7), -1.0, 64, 64, None,
+                                   None))
+    assert "bands [0, 3) of 2" in str(ei.value)
+    with pytest.raises(_lib.UphError) as ei:
+        _lib.check(L.uph_gather_windows(1, 32, 2, None, 64, 64, None, None, 64, 64, None, None,
+                                        None))
+    assert "null argument" in str(ei.value)
 
 
 # ---------------------------------------------------------------------------
@@ -218,3 +226,46 @@ def test_stamp_tier_covers_twice_the_profile_extent():
             ladder = (55, 111, 221)
             want = next((t for t in ladder if need <= t), 221)
             assert p.tier == want
+
+
+def test_scene_tiles_cover_each_clipped_stamp_once():
+    """Scene-subtraction tiles: every stamp pixel inside the image is in
+    exactly one tile entry of its stamp, tiles are disjoint and in the image,
+    entries of a tile are in stamp order."""
+    from uniphot_amd.scene import tile_lists
+    rng = np.random.default_rng(0)
+    H, W, T, tile = 300, 230, 55, 64
+    ix = rng.integers(-60, W + 5, 400)
+    iy = rng.integers(-60, H + 5, 400)
+    off, win, lst = tile_lists(ix, iy, T, H, W, tile)
+    cover = np.zeros((ix.size, H, W), np.int8)
+    seen = np.zeros((H, W), np.int32)
+    for q in range(win.shape[0]):
+        x0, y0, h, w = (int(v) for v in win[q])
+        assert 0 <= x0 and x0 + w <= W and 0 <= y0 and y0 + h <= H and h > 0 and w > 0
+        seen[y0:y0 + h, x0:x0 + w] += 1
+        ent = lst[off[q]:off[q + 1]]
+        assert np.all(np.diff(ent) > 0)
+        for s in ent:
+            ys0, ys1 = max(y0, iy[s]), min(y0 + h, iy[s] + T)
+            xs0, xs1 = max(x0, ix[s]), min(x0 + w, ix[s] + T)
+            assert ys1 > ys0 and xs1 > xs0
+            cover[s, ys0:ys1, xs0:xs1] += 1
+    assert seen.max() == 1
+    for s in range(ix.size):
+        area = max(0, min(H, iy[s] + T) - max(0, iy[s])) * max(0, min(W, ix[s] + T)
+                                                                 - max(0, ix[s]))
+        assert int(cover[s].sum()) == area and cover[s].max() <= 1
+
+
+def test_mosaic_shelves_are_disjoint():
+    from uniphot_amd.scene import shelf_pack
+    rng = np.random.default_rng(1)
+    sides = rng.choice([32, 64, 128, 256], 300, p=[0.5, 0.3, 0.15, 0.05])
+    mx, my, Hm, Wm = shelf_pack(sides)
+    img = np.zeros((Hm, Wm), np.int32)
+    for x, y, s in zip(mx, my, sides):
+        assert x + s <= Wm and y + s <= Hm
+        img[y:y + s, x:x + s] += 1
+    assert img.max() == 1
+    assert img.sum() / img.size > 0.7      # shelves of sorted sides pack densely

@@ -274,3 +274,93 @@ def test_source_metrics_closed_forms():
     expect = np.clip(m[k][:, : isl.B // 2].sum(), 0.0, 1.0)
     assert met[k, 3] == pytest.approx(expect, rel=1e-12)
     assert 0.0 < expect < 1.0
+
+
+def _gauss_psf(T=15, sigma=1.3):
+    r = np.arange(T) - T // 2
+    g = np.exp(-0.5 * (r[:, None] ** 2 + r[None, :] ** 2) / sigma ** 2)
+    z = g.sum()
+    psf = dict(coefs=(g / z)[:, :, None], x_origin=0.0, x_scale=1.0, y_origin=0.0, y_scale=1.0)
+    return psf, z
+
+
+def _gauss_field(H, W, x, y, sigma, z):
+    """Unit Gaussian sampled at the pixel centres of an H x W frame,
+    normalised by the stamp's discrete sum z (independent of the renderer)."""
+    yy, xx = np.mgrid[0:H, 0:W]
+    return np.exp(-0.5 * ((xx - x) ** 2 + (yy - y) ** 2) / sigma ** 2) / z
+
+
+def test_scene_subtract_removes_stars_and_keeps_extended_light():
+    """Stars at integer positions (the Lanczos shift is then the identity):
+    the stack minus the star scene is exactly the extended light, clipping
+    at the stack edge included; the member add-back restores the island's
+    own stars in its window and nothing else."""
+    T, sig, pad = 15, 1.3, 8
+    psf, z = _gauss_psf(T, sig)
+    psfs = [psf, psf]
+    nb, H, W = 2, 44, 50
+    Hp, Wp = H + 2 * pad, W + 2 * pad
+    rng = np.random.default_rng(3)
+    x = np.array([5.0, 20.0, 23.0, -6.0, 30.0, 45.0])
+    y = np.array([7.0, 20.0, 22.0, 10.0, 35.0, 40.0])
+    st = np.array([0, 0, 0, 0, 1, 0])             # source 4 extended
+    flux = rng.uniform(100, 1000, (6, nb))
+    flux[5, 1] = 0.0                              # zero S1 flux: not subtracted in band 1
+    gal = np.zeros((nb, Hp, Wp))
+    gal[:, pad + 30:pad + 40, pad + 25:pad + 36] = rng.uniform(1, 5, (nb, 10, 11))
+    stars = np.zeros((nb, Hp, Wp))
+    for k in range(6):
+        if st[k] == 0:
+            ix, iy = int(x[k]) - T // 2 + pad, int(y[k]) - T // 2 + pad
+            box = np.zeros((Hp, Wp))
+            y0, x0 = max(iy, 0), max(ix, 0)
+            y1, x1 = min(iy + T, Hp), min(ix + T, Wp)
+            g = _gauss_field(Hp, Wp, x[k] + pad, y[k] + pad, sig, z)
+            box[y0:y1, x0:x1] = g[y0:y1, x0:x1]
+            for b in range(nb):
+                stars[b] += flux[k, b] * box
+    scene = oisl.scene_star_image((nb, Hp, Wp), pad, psfs, x, y, st, flux)
+    left = stars - scene
+    # source 5's band-1 light stays, everything else cancels
+    ix5, iy5 = int(x[5]) - T // 2 + pad, int(y[5]) - T // 2 + pad
+    left[1, max(iy5, 0):iy5 + T, max(ix5, 0):ix5 + T] = 0.0
+    assert np.max(np.abs(left)) < 1e-12 * flux.max()
+    assert np.max(np.abs((stars + gal - scene)[0] - gal[0])) < 1e-12 * flux.max()
+    # island {1, 2, 4} in a 24 x 24 window at field origin (12, 14)
+    mem = np.array([1, 2, 4])
+    x0b, y0b, B = 12.0, 14.0, 24
+    add = oisl.member_addback(psfs, x0b, y0b, B, x[mem], y[mem], st[mem], flux[mem])
+    ox, oy = int(x0b) + pad, int(y0b) + pad
+    cut = (stars + gal - scene)[:, oy:oy + B, ox:ox + B] + add
+    own = np.zeros((nb, Hp, Wp))
+    for k in (1, 2):
+        g = _gauss_field(Hp, Wp, x[k] + pad, y[k] + pad, sig, z)
+        for b in range(nb):
+            own[b] += flux[k, b] * g
+    want = (own + gal)[:, oy:oy + B, ox:ox + B]
+    # the stars lie well inside the window, so their boxes are whole there
+    assert np.max(np.abs(cut - want)) < 1e-9 * flux.max()
+    # a larger stamp tier adds the same light (zero ring around the native box)
+    ptier = [dict(psf, coefs=core.pad_psf_coefs(psf["coefs"], 25))] * nb
+    add25 = oisl.member_addback(ptier, x0b, y0b, B, x[mem], y[mem], st[mem], flux[mem])
+    assert np.max(np.abs(add25 - add)) < 1e-12 * flux.max()
+
+
+def test_scene_subtract_shifts_stars_to_their_subpixel_position():
+    """Non-integer positions: the subtracted star matches the Gaussian
+    sampled at its true position to the Lanczos-3 interpolation error, and
+    the mirrored shift (a sign slip) is far outside that."""
+    T, sig, pad = 21, 1.6, 12
+    psf, z = _gauss_psf(T, sig)
+    Hp = Wp = 48 + 2 * pad
+    x, y = np.array([20.3]), np.array([25.6])
+    img = oisl.scene_star_image((1, Hp, Wp), pad, [psf], x, y, np.array([0]),
+                                np.array([[1.0]]))[0]
+    true = _gauss_field(Hp, Wp, x[0] + pad, y[0] + pad, sig, z)
+    mirror = _gauss_field(Hp, Wp, 2 * np.rint(x[0]) - x[0] + pad,
+                          2 * np.rint(y[0]) - y[0] + pad, sig, z)
+    peak = true.max()
+    err = np.max(np.abs(img - true))
+    assert err < 1e-2 * peak, err / peak          # measured 6.2e-3 at sigma 1.6 px
+    assert np.max(np.abs(img - mirror)) > 0.2 * peak

@@ -117,3 +117,35 @@ def test_neighbor_list_matches_brute_force_and_truncates_to_nearest():
         ref_idx, ref_msk = olin.neighbor_list(px, py, 21, 25, k_max=k_max)
         for s in range(px.size):
             assert sorted(idx[s][msk[s]]) == sorted(ref_idx[s][ref_msk[s]])
+
+
+def test_blank_response_closed_forms():
+    """A constant residual c on a 200 x 200 window (uniform weight): more
+    than half of the pixels see the whole template, where the response is
+    c / sum(phi^2) -- the median; data equal to the model: 0."""
+    rng = np.random.default_rng(3)
+    h = w = 200
+    T = 55
+    yy, xx = np.mgrid[:T, :T] - T // 2
+    phi = np.exp(-0.5 * (xx ** 2 + yy ** 2) / 2.0 ** 2)
+    phi /= phi.sum()
+    stamps = np.stack([phi, phi])
+    ix = np.array([30, 110])
+    iy = np.array([60, 20])
+    flux = np.array([500.0, 800.0])
+    basis = olin.bg_basis(h, w, 0, 0, 99.5, 100.0, 99.5, 100.0, order=1)
+    bg = np.array([3.0, 0.5, -0.2])
+    model = (basis @ bg).reshape(h, w)
+    for s in range(2):
+        model[iy[s]:iy[s] + T, ix[s]:ix[s] + T] += flux[s] * stamps[s]
+    iv = np.full((h, w), 0.04)
+    good = np.ones((h, w), bool)
+    assert abs(olin.blank_response(model, iv, good, stamps, ix, iy, flux, bg, basis, phi)) < 1e-9
+    c = 1.7
+    b = olin.blank_response(model + c, iv, good, stamps, ix, iy, flux, bg, basis, phi)
+    assert b == pytest.approx(c / np.sum(phi * phi), rel=1e-9)
+    # a wing is taken off the data first
+    wing = rng.normal(0, 1, (h, w))
+    b2 = olin.blank_response(model + c + wing, iv, good, stamps, ix, iy, flux, bg, basis, phi,
+                             wing=wing)
+    assert b2 == pytest.approx(b, rel=1e-9)

@@ -20,7 +20,8 @@ EXPORTS = (
     "uph_island_fit_batch", "uph_island_split_counters", "uph_island_fit_split",
     "uph_prerender_scratch_bytes", "uph_prerender_spectra_floats",
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
-    "uph_wing_images",
+    "uph_wing_images", "uph_blank_work_floats", "uph_blank_response", "uph_stamp_sum",
+    "uph_gather_windows",
 )
 
 
@@ -129,9 +130,20 @@ def lib() -> C.CDLL:
     L.uph_band_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int,
                                  C.c_int, C.c_int, _P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
                                  _P, _P, _P, _P, C.c_float, _P, _P, _P, _P, _P, C.c_size_t, _P]
+    L.uph_blank_work_floats.restype = C.c_size_t
+    L.uph_blank_work_floats.argtypes = [C.c_int, C.c_int]
+    L.uph_blank_response.restype = C.c_int
+    L.uph_blank_response.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int, _P, _P,
+                                     _P, _P, _P, _P, _P, _P, _P, _P, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, _P, _P, C.c_size_t, C.c_int, _P, _P]
     L.uph_wing_images.restype = C.c_int
     L.uph_wing_images.argtypes = [C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P, _P,
                                   C.c_int, C.c_int, _P, _P]
+    L.uph_stamp_sum.restype = C.c_int
+    L.uph_stamp_sum.argtypes = [C.c_int] * 7 + [_P] * 7 + [C.c_float, C.c_int, C.c_int, _P, _P]
+    L.uph_gather_windows.restype = C.c_int
+    L.uph_gather_windows.argtypes = [C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P,
+                                     C.c_int, C.c_int, _P, _P, _P]
     _lib = L
     return L
 

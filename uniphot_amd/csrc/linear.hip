@@ -435,6 +435,254 @@ __global__ __launch_bounds__(UPH_BLOCK) void wing_images_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Weighted stamp sums into disjoint image windows: the field-wide STAR
+// subtraction at S1 flux (island_stage.py:2040-2117) and an island's member
+// add-back (island_solver.py:1795-1818).  Window q = (x0, y0, h, w) of band
+// band0 + blockIdx.y of out (bands, H, W) gets scale times the sum over its
+// entries k in [off[q], off[q+1]) of flux[s * nbc + bl] * stamp s of band
+// chunk bl (T x T at top-left (ix[s], iy[s]) in out coordinates, zero outside),
+// s = list[k].  Entries are summed in list order per pixel, so the result does
+// not depend on the launch shape; windows of one launch must be disjoint.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(UPH_BLOCK) void stamp_sum_kernel(
+    int T, int n_st, int band0, const int64_t* __restrict__ off, const int32_t* __restrict__ win,
+    const int32_t* __restrict__ list, const int32_t* __restrict__ ix,
+    const int32_t* __restrict__ iy, const float* __restrict__ stamps,
+    const float* __restrict__ flux, float scale, int H, int W, float* __restrict__ out) {
+  const int q = blockIdx.x, bl = blockIdx.y, nbc = gridDim.y;
+  const int x0 = win[q * 4], y0 = win[q * 4 + 1], h = win[q * 4 + 2], w = win[q * 4 + 3];
+  const int64_t k0 = off[q], k1 = off[q + 1];
+  const float* st = stamps + (size_t)bl * n_st * T * T;
+  float* img = out + (size_t)(band0 + bl) * H * W;
+  for (int e = blockIdx.z * UPH_BLOCK + threadIdx.x; e < h * w; e += gridDim.z * UPH_BLOCK) {
+    const int py = y0 + e / w, px = x0 + e % w;
+    float v = 0.0f;
+    for (int64_t k = k0; k < k1; ++k) {
+      const int s = list[k];
+      const int u = px - ix[s], t = py - iy[s];
+      if (u < 0 || u >= T || t < 0 || t >= T) continue;
+      v = fmaf(flux[(size_t)s * nbc + bl], st[((size_t)s * T + t) * T + u], v);
+    }
+    float* o = img + (size_t)py * W + px;
+    *o = fmaf(scale, v, *o);
+  }
+}
+
+// Island window gather (island_solver.py:1820-1830): window q = (sx, sy, dx,
+// dy, side) copies the side x side block at (sx, sy) of every band of src_a
+// (and src_b) (bands, Hs, Ws) to (dx, dy) of dst_a (dst_b) (bands, Hd, Wd).
+__global__ __launch_bounds__(UPH_BLOCK) void gather_windows_kernel(
+    const int32_t* __restrict__ win, int Hs, int Ws, const float* __restrict__ src_a,
+    const float* __restrict__ src_b, int Hd, int Wd, float* __restrict__ dst_a,
+    float* __restrict__ dst_b) {
+  const int q = blockIdx.x, band = blockIdx.y;
+  const int sx = win[q * 5], sy = win[q * 5 + 1], dx = win[q * 5 + 2], dy = win[q * 5 + 3];
+  const int side = win[q * 5 + 4];
+  for (int e = blockIdx.z * UPH_BLOCK + threadIdx.x; e < side * side;
+       e += gridDim.z * UPH_BLOCK) {
+    const int r = e / side, c = e % side;
+    const size_t i = ((size_t)band * Hs + sy + r) * Ws + sx + c;
+    const size_t o = ((size_t)band * Hd + dy + r) * Wd + dx + c;
+    dst_a[o] = src_a[i];
+    if (dst_b) dst_b[o] = src_b[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Blank-response (empty-aperture) median of a grid-band problem
+// (solver_grid.py:2867-2943): with the final fluxes and background, the
+// weighted residual wr = w (d - wing - model) and w on the window (good
+// pixels, w = invvar), then resp = corr(wr, phi) / corr(w, phi^2) ('same'
+// correlation with the problem's PSF template phi, T x T, clipped >= 0 and
+// unit-sum), NaN where the denominator is 0 or the pixel is not good, and
+// b = nanmedian(resp).
+// ---------------------------------------------------------------------------
+struct BlankArgs {
+  int n_prob, T, M, H, W;
+  const int64_t* src_off;
+  const int32_t* win;
+  const float* data;
+  const float* invvar;
+  const uint8_t* good;
+  const float* wing;        // (n_bands, H, W) or null
+  const uint8_t* use_wing;  // (n_prob)
+  const float* stamps;      // (total_src, T, T) unit stamps
+  const int32_t* ix_top;
+  const int32_t* iy_top;
+  const float* flux;        // (total_src)
+  const float* bg;          // (n_prob, M)
+  double bxo, bxs, byo, bys;
+  const float* phi;         // (n_prob, T, T)
+  float* work;              // per problem: wr, w, resp planes of max_pix floats
+  size_t max_pix;
+  double* b_hat;            // (n_prob)
+};
+
+template <int M>
+__global__ __launch_bounds__(UPH_BLOCK) void blank_residual_kernel(BlankArgs a) {
+  const int q = blockIdx.x;
+  const int band = a.win[q * 5], x0 = a.win[q * 5 + 1], y0 = a.win[q * 5 + 2];
+  const int h = a.win[q * 5 + 3], w = a.win[q * 5 + 4], T = a.T, TT = T * T;
+  const int64_t s0 = a.src_off[q], s1 = a.src_off[q + 1];
+  const size_t plane = (size_t)a.H * a.W, off = (size_t)band * plane;
+  float* WR = a.work + (size_t)q * 3 * a.max_pix;
+  float* WW = WR + a.max_pix;
+  const float* bgq = a.bg + (size_t)q * M;
+  const bool wing = a.wing != nullptr && a.use_wing[q];
+  for (int e = blockIdx.y * UPH_BLOCK + threadIdx.x; e < h * w; e += gridDim.y * UPH_BLOCK) {
+    const int py = e / w, px = e % w;
+    const size_t gi = off + (size_t)(y0 + py) * a.W + x0 + px;
+    const float iv = a.invvar[gi];
+    const bool gd = a.good[gi] && iv > 0.0f;
+    float model = 0.0f;
+    for (int64_t s = s0; s < s1; ++s) {
+      const int u = px - a.ix_top[s], t = py - a.iy_top[s];
+      if (u < 0 || u >= T || t < 0 || t >= T) continue;
+      model = fmaf(a.flux[s], a.stamps[(size_t)s * TT + t * T + u], model);
+    }
+    const double X = (double(x0 + px) - a.bxo) / a.bxs, Y = (double(y0 + py) - a.byo) / a.bys;
+    constexpr int m_[6] = {0, 1, 0, 2, 1, 0};
+    constexpr int n_[6] = {0, 0, 1, 0, 1, 2};
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double v = 1.0;
+      for (int k = 0; k < m_[m]; ++k) v *= X;
+      for (int k = 0; k < n_[m]; ++k) v *= Y;
+      model = fmaf(bgq[m], float(v), model);
+    }
+    const float d = a.data[gi] - (wing ? a.wing[gi] : 0.0f);
+    const float wv = gd ? iv : 0.0f;
+    WR[e] = gd ? wv * (d - model) : 0.0f;
+    WW[e] = wv;
+  }
+}
+
+constexpr int kCorrTile = 16;
+
+// 'same' correlations of wr with phi and of w with phi^2 over one
+// kCorrTile^2 output tile per workgroup; inputs and template staged in LDS.
+__global__ __launch_bounds__(UPH_BLOCK) void blank_corr_kernel(BlankArgs a) {
+  extern __shared__ float lds[];
+  const int q = blockIdx.x;
+  const int band = a.win[q * 5], x0 = a.win[q * 5 + 1], y0 = a.win[q * 5 + 2];
+  const int h = a.win[q * 5 + 3], w = a.win[q * 5 + 4], T = a.T, c = T / 2;
+  const int tiles_x = (w + kCorrTile - 1) / kCorrTile, tiles_y = (h + kCorrTile - 1) / kCorrTile;
+  const int S = kCorrTile + T - 1;
+  float* P1 = lds;                // phi
+  float* P2 = P1 + T * T;         // phi^2
+  float* A1 = P2 + T * T;         // wr tile with halo
+  float* A2 = A1 + S * S;         // w tile with halo
+  const float* WR = a.work + (size_t)q * 3 * a.max_pix;
+  const float* WW = WR + a.max_pix;
+  float* RESP = a.work + (size_t)q * 3 * a.max_pix + 2 * a.max_pix;
+  const float* ph = a.phi + (size_t)q * T * T;
+  for (int e = threadIdx.x; e < T * T; e += UPH_BLOCK) {
+    P1[e] = ph[e];
+    P2[e] = ph[e] * ph[e];
+  }
+  const size_t plane = (size_t)a.H * a.W, off = (size_t)band * plane;
+  for (int tile = blockIdx.y; tile < tiles_x * tiles_y; tile += gridDim.y) {
+    const int ty0 = (tile / tiles_x) * kCorrTile, tx0 = (tile % tiles_x) * kCorrTile;
+    __syncthreads();
+    for (int e = threadIdx.x; e < S * S; e += UPH_BLOCK) {
+      const int yy = ty0 - c + e / S, xx = tx0 - c + e % S;
+      const bool in = yy >= 0 && yy < h && xx >= 0 && xx < w;
+      A1[e] = in ? WR[yy * w + xx] : 0.0f;
+      A2[e] = in ? WW[yy * w + xx] : 0.0f;
+    }
+    __syncthreads();
+    const int ly = threadIdx.x / kCorrTile, lx = threadIdx.x % kCorrTile;
+    const int oy = ty0 + ly, ox = tx0 + lx;
+    if (oy < h && ox < w) {
+      float num = 0.0f, den = 0.0f;
+      for (int ky = 0; ky < T; ++ky) {
+        const float* r1 = A1 + (ly + ky) * S + lx;
+        const float* r2 = A2 + (ly + ky) * S + lx;
+        const float* k1 = P1 + ky * T;
+        const float* k2 = P2 + ky * T;
+        for (int kx = 0; kx < T; ++kx) {
+          num = fmaf(r1[kx], k1[kx], num);
+          den = fmaf(r2[kx], k2[kx], den);
+        }
+      }
+      const size_t gi = off + (size_t)(y0 + oy) * a.W + x0 + ox;
+      const bool gd = a.good[gi] && a.invvar[gi] > 0.0f;
+      const float nanv = __int_as_float(0x7fc00000);
+      RESP[oy * w + ox] = (gd && den > 0.0f) ? num / fmaxf(den, 1e-30f) : nanv;
+    }
+  }
+}
+
+// Total order of floats as unsigned keys (negative values reversed).
+__device__ __forceinline__ unsigned float_key(float v) {
+  const unsigned u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ float key_float(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// The rank-th smallest (0-based) of the non-NaN values of v[0, n): radix
+// select, four 8-bit digits from the top, one LDS histogram per digit.
+__device__ float select_rank(const float* v, int n, int rank, unsigned* hist, unsigned* s_pref,
+                             int* s_rank) {
+  const int tid = threadIdx.x;
+  unsigned prefix = 0, mask = 0;
+  if (tid == 0) *s_rank = rank;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    for (int i = tid; i < 256; i += UPH_BLOCK) hist[i] = 0;
+    __syncthreads();
+    for (int i = tid; i < n; i += UPH_BLOCK) {
+      const float x = v[i];
+      if (x != x) continue;
+      const unsigned k = float_key(x);
+      if ((k & mask) == prefix) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+      int r = *s_rank;
+      unsigned d = 0;
+      while (d < 255 && r >= int(hist[d])) {
+        r -= int(hist[d]);
+        ++d;
+      }
+      *s_rank = r;
+      *s_pref = prefix | (d << shift);
+    }
+    __syncthreads();
+    prefix = *s_pref;
+    mask |= 255u << shift;
+  }
+  return key_float(prefix);
+}
+
+__global__ __launch_bounds__(UPH_BLOCK) void blank_median_kernel(BlankArgs a) {
+  __shared__ unsigned hist[256];
+  __shared__ unsigned s_pref;
+  __shared__ int s_rank;
+  __shared__ int s_cnt;
+  const int q = blockIdx.x;
+  const int h = a.win[q * 5 + 3], w = a.win[q * 5 + 4], n = h * w;
+  const float* RESP = a.work + (size_t)q * 3 * a.max_pix + 2 * a.max_pix;
+  if (threadIdx.x == 0) s_cnt = 0;
+  __syncthreads();
+  int cnt = 0;
+  for (int i = threadIdx.x; i < n; i += UPH_BLOCK) cnt += RESP[i] == RESP[i];
+  atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  const int m = s_cnt;
+  if (m == 0) {
+    if (threadIdx.x == 0) a.b_hat[q] = __longlong_as_double(0x7ff8000000000000ll);
+    return;
+  }
+  const float lo = select_rank(RESP, n, (m - 1) / 2, hist, &s_pref, &s_rank);
+  __syncthreads();
+  const float hi = select_rank(RESP, n, m / 2, hist, &s_pref, &s_rank);
+  if (threadIdx.x == 0) a.b_hat[q] = 0.5 * (double(lo) + double(hi));
+}
+
 }  // namespace uph
 
 static int n_terms_of_order(int order) { return (order + 1) * (order + 2) / 2; }
@@ -571,4 +819,104 @@ extern "C" int uph_wing_images(int n_prob, int Tb, int Ts, const int64_t* src_of
   hipLaunchKernelGGL(uph::wing_images_kernel, dim3(n_prob, 16), dim3(UPH_BLOCK), 0, s, Tb, Ts,
                      src_off, win, big, small, ix, iy, flux, H, W, out);
   return uph::hip_status(hipGetLastError(), "uph_wing_images launch");
+}
+
+extern "C" int uph_stamp_sum(int n_win, int max_pix, int T, int n_st, int nbc, int band0,
+                             int n_bands, const int64_t* off, const int32_t* win,
+                             const int32_t* list, const int32_t* ix, const int32_t* iy,
+                             const float* stamps, const float* flux, float scale, int H, int W,
+                             float* out, void* stream) {
+  UPH_CHECK_ARG(n_win >= 0 && max_pix >= 0 && T >= 1 && n_st >= 0 && nbc >= 1 && band0 >= 0 &&
+                    band0 + nbc <= n_bands && H >= 1 && W >= 1,
+                "uph_stamp_sum: n_win=%d max_pix=%d T=%d n_st=%d bands [%d, %d) of %d, %d x %d",
+                n_win, max_pix, T, n_st, band0, band0 + nbc, n_bands, H, W);
+  if (n_win == 0 || max_pix == 0) return UPH_OK;
+  UPH_CHECK_ARG(off && win && list && ix && iy && stamps && flux && out,
+                "uph_stamp_sum: null argument");
+  const int per = UPH_BLOCK * 16;
+  const int z = std::min(64, (max_pix + per - 1) / per);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(uph::stamp_sum_kernel, dim3(n_win, nbc, z), dim3(UPH_BLOCK), 0, s, T, n_st,
+                     band0, off, win, list, ix, iy, stamps, flux, scale, H, W, out);
+  return uph::hip_status(hipGetLastError(), "uph_stamp_sum launch");
+}
+
+extern "C" int uph_gather_windows(int n_win, int max_side, int n_bands, const int32_t* win,
+                                  int Hs, int Ws, const float* src_a, const float* src_b,
+                                  int Hd, int Wd, float* dst_a, float* dst_b, void* stream) {
+  UPH_CHECK_ARG(n_win >= 0 && max_side >= 0 && n_bands >= 1 && Hs >= 1 && Ws >= 1 && Hd >= 1 &&
+                    Wd >= 1,
+                "uph_gather_windows: n_win=%d max_side=%d n_bands=%d src %d x %d dst %d x %d",
+                n_win, max_side, n_bands, Hs, Ws, Hd, Wd);
+  if (n_win == 0 || max_side == 0) return UPH_OK;
+  UPH_CHECK_ARG(win && src_a && dst_a && ((src_b == nullptr) == (dst_b == nullptr)),
+                "uph_gather_windows: null argument");
+  const int per = UPH_BLOCK * 16;
+  const int z = std::min(64, (max_side * max_side + per - 1) / per);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(uph::gather_windows_kernel, dim3(n_win, n_bands, z), dim3(UPH_BLOCK), 0, s,
+                     win, Hs, Ws, src_a, src_b, Hd, Wd, dst_a, dst_b);
+  return uph::hip_status(hipGetLastError(), "uph_gather_windows launch");
+}
+
+extern "C" size_t uph_blank_work_floats(int n_prob, int max_pix) {
+  return (size_t)n_prob * 3 * (size_t)(max_pix > 0 ? max_pix : 0);
+}
+
+extern "C" int uph_blank_response(int n_prob, int T, int bg_order, const int64_t* src_off,
+                                  const int32_t* win, int H, int W, const float* data,
+                                  const float* invvar, const uint8_t* good, const float* wing,
+                                  const uint8_t* use_wing, const float* stamps,
+                                  const int32_t* ix_top, const int32_t* iy_top,
+                                  const float* flux, const float* bg, double bg_xo, double bg_xs,
+                                  double bg_yo, double bg_ys, const float* phi, float* work,
+                                  size_t work_floats, int max_pix, double* b_hat, void* stream) {
+  UPH_CHECK_ARG(n_prob >= 0 && T >= 1 && (T % 2) == 1 && bg_order >= 0 && bg_order <= 2,
+                "uph_blank_response: n_prob=%d T=%d bg_order=%d", n_prob, T, bg_order);
+  if (n_prob == 0) return UPH_OK;
+  UPH_CHECK_ARG(src_off && win && data && invvar && good && stamps && ix_top && iy_top && flux &&
+                    bg && phi && work && b_hat && (wing == nullptr || use_wing),
+                "uph_blank_response: null argument");
+  UPH_CHECK_ARG(work_floats >= uph_blank_work_floats(n_prob, max_pix),
+                "uph_blank_response: work of %zu floats < %zu", work_floats,
+                uph_blank_work_floats(n_prob, max_pix));
+  uph::BlankArgs a{};
+  a.n_prob = n_prob;
+  a.T = T;
+  a.M = (bg_order + 1) * (bg_order + 2) / 2;
+  a.H = H;
+  a.W = W;
+  a.src_off = src_off;
+  a.win = win;
+  a.data = data;
+  a.invvar = invvar;
+  a.good = good;
+  a.wing = wing;
+  a.use_wing = use_wing;
+  a.stamps = stamps;
+  a.ix_top = ix_top;
+  a.iy_top = iy_top;
+  a.flux = flux;
+  a.bg = bg;
+  a.bxo = bg_xo;
+  a.bxs = bg_xs;
+  a.byo = bg_yo;
+  a.bys = bg_ys;
+  a.phi = phi;
+  a.work = work;
+  a.max_pix = (size_t)max_pix;
+  a.b_hat = b_hat;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (a.M == 1)
+    hipLaunchKernelGGL(uph::blank_residual_kernel<1>, dim3(n_prob, 16), dim3(UPH_BLOCK), 0, s, a);
+  else if (a.M == 3)
+    hipLaunchKernelGGL(uph::blank_residual_kernel<3>, dim3(n_prob, 16), dim3(UPH_BLOCK), 0, s, a);
+  else
+    hipLaunchKernelGGL(uph::blank_residual_kernel<6>, dim3(n_prob, 16), dim3(UPH_BLOCK), 0, s, a);
+  const int S = uph::kCorrTile + T - 1;
+  const size_t lds = (size_t)(2 * T * T + 2 * S * S) * sizeof(float);
+  UPH_CHECK_ARG(lds <= 65536, "uph_blank_response: template T=%d too large", T);
+  hipLaunchKernelGGL(uph::blank_corr_kernel, dim3(n_prob, 32), dim3(UPH_BLOCK), lds, s, a);
+  hipLaunchKernelGGL(uph::blank_median_kernel, dim3(n_prob), dim3(UPH_BLOCK), 0, s, a);
+  return uph::hip_status(hipGetLastError(), "uph_blank_response launch");
 }

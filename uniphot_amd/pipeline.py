@@ -4,16 +4,19 @@ The single-GPU restatement of ``run_grid_gpu_fit`` in island mode
 (``tract7dt/fit/grid_gpu/pipeline.py:1352-2725``) and
 ``run_island_prestage`` (``island_stage.py:1338-3363``):
 
-  pass 1  per-grid linear S1 solve at the catalog seeds (fluxes + plane)
+  pass 1  per-grid linear S1 solve at the catalog seeds (fluxes + plane),
+          wing peeling of large galaxies
   islands flux-aware coupling radii (or a fixed linking radius), connected
           components, (B, N) buckets, star/mixed classes and tiers
-  stacks  padded field stacks: data - per-grid S1 plane on each grid core
-  fit     per-class persistent VarPro-LM kernel
-  pass 2  per-grid linear S1' solve at the fitted positions and shapes
+  stacks  padded field stacks: data - per-grid S1 plane on each grid core,
+          every catalogued star subtracted at its S1 flux (scene subtraction)
+  fit     band-split VarPro-LM over launch sequences of island cutouts (own
+          member stars added back), post-fit errors and source metrics
+  pass 2  per-grid linear S1' solve at the fitted positions and shapes,
+          blank-response correction of the star fluxes
 
-Knobs the reference turns on by default that this build leaves off (and
-says so in the run record): scene subtraction, blank-response correction,
-wing peeling, post-fit error/metric passes, catalog priors.
+Every default-on knob of the reference has a switch in FitConfig; catalog
+priors are not built (the run record lists what is off).
 Inputs stay resident on the device for the whole fit.
 """
 from __future__ import annotations
@@ -31,6 +34,7 @@ from .island_solver import make_class_kernels, psf_tree_device, run_class_chunk
 from .islands import (ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands,
                       build_islands, flux_aware_radii, psf_radial_profile)
 from .psf_polynomial import evaluate_psf_at_np, psf_device_arrays
+from .scene import SceneStars, island_mosaic, mosaic_bytes, star_flux, subtract_stars
 from .source_model import SOURCE_STAR, sersic_u_to_n
 
 
@@ -51,6 +55,8 @@ class FitConfig:
     wing_peeling: bool = True    # large galaxies: subtract tier-wing_tier wings (native solve)
     wing_tier: int = 221
     wing_max_sources: int = 64
+    blank_response: bool = True  # pass 2: subtract the blank-response median from STAR fluxes
+    scene_subtract: bool = True  # island fit: non-member stars removed at their S1 flux
     island_split: bool = True    # band-split island LM (uph_island_fit_split)
     galaxy_prerender: bool = True  # split path: galaxies rendered by their own launches
     post_fit: bool = True        # parameter variances, marginal flux errors, source metrics
@@ -178,8 +184,103 @@ def _disjoint_colors(grids):
     return colors
 
 
+def blank_response_correct(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
+                           cfg: FitConfig, bg_geo, wing=None, use_wing=None):
+    """Blank-response (empty-aperture) correction of the final solve of
+    pass 2 (solver_grid.py:2855-2943): per (grid, band), the median
+    matched-filter response of the final residual -- corr(w r, phi) /
+    corr(w, phi^2), phi the band PSF at the window centre clipped >= 0 and
+    unit-sum -- is subtracted from the STAR fluxes.  The model uses unit
+    stamps without renormalisation (the STAR rows of render_source_stamps)
+    and the final fluxes and background; with wing peeling the residual is
+    taken against data - wing (``wing`` (nb, H, W) for the problems flagged
+    in ``use_wing``).  Runs on the device (``uph_blank_response``) in
+    launches bounded by the stamp budget; returns the corrected results and
+    b_hat per problem."""
+    import torch
+    from . import _lib
+    from .psf_polynomial import evaluate_psf_at_np
+    L = _lib.lib()
+    P = _lib.ptr
+    dev = fld.data.device
+    nb, H, W = fld.nb, fld.H, fld.W
+    up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+    cs, gs = fld.psf(T)
+    out = list(res)
+    b_hat = np.full(len(problems), np.nan)
+    n_of = np.array([owners[q][0].fit_indices.size for q in range(len(problems))])
+    cost = n_of * T * T * 4.0 + np.array([pr.h * pr.w * 12.0 for pr in problems])
+    cut = np.concatenate([[0], np.nonzero(np.diff(np.floor(np.cumsum(cost) /
+                                                         cfg.stamp_budget)))[0] + 1,
+                          [len(problems)]])
+    for a, z in zip(cut[:-1], cut[1:]):
+        qs = np.arange(a, z)
+        if qs.size == 0:
+            continue
+        src = np.concatenate([owners[q][0].fit_indices for q in qs]).astype(np.int64)
+        counts = n_of[qs]
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        band = np.repeat([owners[q][1] for q in qs], counts).astype(np.int32)
+        ox = np.repeat([problems[q].x0 for q in qs], counts).astype(np.float32)
+        oy = np.repeat([problems[q].y0 for q in qs], counts).astype(np.float32)
+        ext = stype[src] != SOURCE_STAR
+        stamps = fps.render_unit_stamps(
+            cs, gs, T, fld.n_terms, band, x[src], y[src], ox, oy, stype[src],
+            np.where(ext, logre[src], 0.0), np.where(ext, ee1[src], 1.0),
+            np.where(ext, ee2[src], 0.0), sn[src])
+        ix, iy = fps.stamp_top_left(x[src], y[src], ox, oy, T, T)
+        flux = np.concatenate([out[q].fluxes for q in qs]).astype(np.float32)
+        flux = np.where(np.isfinite(flux), flux, 0.0).astype(np.float32)
+        bg = np.stack([out[q].bg_coefs for q in qs]).astype(np.float32)
+        win = np.array([[owners[q][1], problems[q].x0, problems[q].y0, problems[q].h,
+                         problems[q].w] for q in qs], np.int32)
+        phi = np.empty((qs.size, T, T), np.float32)
+        for j, q in enumerate(qs):
+            pr = problems[q]
+            t = np.clip(evaluate_psf_at_np(fld.psfs[owners[q][1]], pr.x0 + 0.5 * pr.w,
+                                           pr.y0 + 0.5 * pr.h).astype(np.float64), 0.0, None)
+            t = _pad_center(t, T)
+            phi[j] = (t / t.sum()).astype(np.float32)
+        max_pix = int(max(problems[q].h * problems[q].w for q in qs))
+        nw = int(L.uph_blank_work_floats(int(qs.size), max_pix))
+        work = torch.empty(max(nw, 1), dtype=torch.float32, device=dev)
+        bh = torch.empty(qs.size, dtype=torch.float64, device=dev)
+        uw = (np.asarray(use_wing, np.uint8)[qs] if use_wing is not None
+              else np.zeros(qs.size, np.uint8))
+        keep = [up(off, np.int64), up(win, np.int32), up(ix, np.int32), up(iy, np.int32),
+                up(flux, np.float32), up(bg, np.float32), up(phi, np.float32), up(uw, np.uint8)]
+        _lib.check(L.uph_blank_response(
+            int(qs.size), T, cfg.bg_order, P(keep[0]), P(keep[1]), H, W, P(fld.data),
+            P(fld.invvar), P(fld.good), P(wing) if wing is not None else None, P(keep[7]),
+            P(stamps), P(keep[2]), P(keep[3]), P(keep[4]), P(keep[5]), *map(float, bg_geo),
+            P(keep[6]), P(work), nw, max_pix, P(bh), _lib.stream_ptr(None)))
+        bq = bh.cpu().numpy()
+        for j, q in enumerate(qs):
+            b_hat[q] = bq[j]
+            if not np.isfinite(bq[j]):
+                continue
+            star = stype[owners[q][0].fit_indices] == SOURCE_STAR
+            r0 = out[q]
+            f = r0.fluxes.copy()
+            f[star] = f[star] - np.float32(bq[j])
+            out[q] = fps.SparseForcedPhotResult(f, r0.bg_coefs, r0.flux_var, r0.bg_var,
+                                                r0.converged)
+    return out, b_hat
+
+
+def _pad_center(t, T):
+    """Zero-pad (or keep) a square stamp to T x T, centred."""
+    n = t.shape[0]
+    if n == T:
+        return t
+    o = (T - n) // 2
+    out = np.zeros((T, T), t.dtype)
+    out[o:o + n, o:o + n] = t
+    return out
+
+
 def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
-              cfg: FitConfig, bg_geo, timer=None):
+              cfg: FitConfig, bg_geo, timer=None, blank=False):
     """Wing peeling of a grid pass at the native stamp (solver_grid.py:
     2495-2546, :2826-2846): per grid, the big sources' wings -- their S1
     flux times the tier-wing_tier render minus the native render -- are
@@ -187,11 +288,14 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
     linear in the data, so it equals the S1 solve minus the same solve of
     the wing image: the wing images are built on the device
     (``uph_wing_images``, grids of one colour at a time so windows are
-    disjoint), solved, and their fluxes and background subtracted."""
+    disjoint), solved, and their fluxes and background subtracted.  With
+    ``blank`` (pass 2) the blank-response correction follows, against data
+    minus each grid's own wing.  Returns (results, problems corrected)."""
     import torch
     from . import _lib
+    done = np.zeros(len(problems), bool)
     if T != fld.native:
-        return res
+        return res, done
     nb, H, W = fld.nb, fld.H, fld.W
     Tb, Ts = int(cfg.wing_tier), int(fld.native)
     big_of = {}
@@ -202,7 +306,7 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
                                              cfg.wing_max_sources)
     qs = [q for q, (g, b) in enumerate(owners) if big_of[g.grid_id].size]
     if not qs:
-        return res
+        return res, done
     colors = _disjoint_colors([g for g, _ in {id(g): (g, 0) for g, _ in owners}.values()])
     L = _lib.lib()
     P = _lib.ptr
@@ -249,11 +353,22 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
             out[q] = fps.SparseForcedPhotResult(r0.fluxes - r.fluxes, r0.bg_coefs - r.bg_coefs,
                                                 r0.flux_var, r0.bg_var,
                                                 r0.converged and r.converged)
-    return out
+        if blank:
+            use = np.zeros(len(problems), np.uint8)
+            use[sel] = 1
+            sub = [problems[q] for q in sel]
+            corr, _ = blank_response_correct(fld, T, sub, [owners[q] for q in sel],
+                                             [out[q] for q in sel], x, y, stype, logre, ee1,
+                                             ee2, sn, cfg, bg_geo, wing=wing,
+                                             use_wing=np.ones(len(sel), np.uint8))
+            for q, r in zip(sel, corr):
+                out[q] = r
+            done[sel] = True
+    return out, done
 
 
 def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: FitConfig,
-                gtier: dict, timer=None):
+                gtier: dict, timer=None, final=False):
     """Per-grid, per-band linear solve; returns fluxes/var (n, nb) for each
     source from the grid that owns it, and bg coefs (n_grid, nb, M).  The
     grids, their fit sets and stamp tiers are those of pass 1 (pass 2 only
@@ -293,9 +408,19 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
                                           fld.invvar, fld.good, bg_order=cfg.bg_order,
                                           bg_geo=bg_geo, reg=cfg.reg, k_max=cfg.k_max,
                                           timer=timer)
+                blank = final and cfg.blank_response and T == fld.native
+                done = np.zeros(len(part), bool)
                 if cfg.wing_peeling:
-                    part = wing_peel(fld, T, problems[a:z], owners[a:z], part, x, y, stype,
-                                     logre, ee1, ee2, sn, cfg, bg_geo, timer)
+                    part, done = wing_peel(fld, T, problems[a:z], owners[a:z], part, x, y,
+                                           stype, logre, ee1, ee2, sn, cfg, bg_geo, timer,
+                                           blank=blank)
+                if blank and not done.all():
+                    rest = np.nonzero(~done)[0]
+                    corr, _ = blank_response_correct(
+                        fld, T, [problems[a + q] for q in rest], [owners[a + q] for q in rest],
+                        [part[q] for q in rest], x, y, stype, logre, ee1, ee2, sn, cfg, bg_geo)
+                    for q, r in zip(rest, corr):
+                        part[q] = r
                 res += part
         for (g, b), r in zip(owners, res):
             own = g.source_indices.size
@@ -380,10 +505,11 @@ def metrics_from_sums(sums):
     return np.stack([rchisq, fracflux, fracin, fracmasked], axis=-1), f * sm
 
 
-def _chunk_batch(batch, nb, budget_bytes):
+def _chunk_batch(batch, nb, budget_bytes, mosaic=False):
     """Split a class batch [(kernels, host, rows)] into launch sequences whose
-    pre-render planes fit ``budget_bytes``; classes are taken in order of
-    their per-workgroup scratch so large classes share sequences."""
+    pre-render planes (and, with ``mosaic``, the island cutout mosaic) fit
+    ``budget_bytes``; classes are taken in order of their per-workgroup
+    scratch so large classes share sequences."""
     from . import _lib
     L = _lib.lib()
     keyed = []
@@ -394,6 +520,8 @@ def _chunk_batch(batch, nb, budget_bytes):
         mm = host["mmask"][rows].astype(bool)
         planes = np.where(st == SOURCE_STAR, 3, 7) * mm
         cost = nb * side * side * 4.0 * planes.sum(axis=1)
+        if mosaic:
+            cost = cost + mosaic_bytes(nb, kern.B)
         keyed.append((per, kern, host, np.asarray(rows), cost))
     keyed.sort(key=lambda t: t[0])
     chunks, cur, used = [], [], 0.0
@@ -417,14 +545,18 @@ def _chunk_batch(batch, nb, budget_bytes):
 
 def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans, x, y, stype,
                    logre, ee1, ee2, sn, free_n, cfg: FitConfig, timer=None,
-                   budget_bytes: float = 48e9, sync=None, p_start=None) -> IslandSetFit:
+                   budget_bytes: float = 48e9, sync=None, p_start=None,
+                   scene: SceneStars | None = None) -> IslandSetFit:
     """Island VarPro-LM over the planned classes of an island set
     (island_solver.py:3233-3595, one device): class host arrays, the device
     Sersic table and PSF trees, then the band-split fit in launch sequences
     of at most ``budget_bytes`` of pre-rendered planes each.  ``psf_of(T)``
     gives the device (coefs, geo) of stamp tier T.  ``p_start`` (n, 6), when
     given, replaces the starting parameters of the real members after the
-    classes (stamp tiers, bounds) were planned from the catalog seeds."""
+    classes (stamp tiers, bounds) were planned from the catalog seeds.  With
+    ``scene`` (scene subtraction), dstack already has every catalogued star
+    subtracted at its S1 flux and each launch sequence fits a mosaic of its
+    island cutouts with the members' own stars added back."""
     from .island_solver import PsfTree, run_island_batch
     n = x.size
     fx, fy = x.copy(), y.copy()
@@ -460,7 +592,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         from .sersic_table import device_table
         sersic_dev = device_table(dstack.device)
     if cfg.island_split and cfg.galaxy_prerender and cfg.precision != "fp64":
-        chunks = _chunk_batch(batch, nb, budget_bytes)
+        chunks = _chunk_batch(batch, nb, budget_bytes, mosaic=scene is not None)
     else:
         chunks = [batch]
     plan_of = {id(h): p for p, h in classes}
@@ -469,7 +601,13 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         if not chunk:
             continue
         t0 = time.perf_counter()
-        results = run_island_batch(chunk, trees, cfg.max_steps, dstack, wstack,
+        ds, ws = dstack, wstack
+        if scene is not None:
+            ds, ws, moved = island_mosaic(chunk, plan_of, dstack, wstack, scene)
+            for (_, h2, _), (_, h, _) in zip(moved, chunk):
+                plan_of[id(h2)] = plan_of[id(h)]
+            chunk = moved
+        results = run_island_batch(chunk, trees, cfg.max_steps, ds, ws,
                                    precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
                                    split=cfg.island_split, prerender=cfg.galaxy_prerender,
                                    post=cfg.post_fit, gram=cfg.gram)
@@ -479,7 +617,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         if redo:
             # islands whose bfloat16 Gram refinement did not settle, refitted
             # on float32 operands
-            again = run_island_batch(redo, trees, cfg.max_steps, dstack, wstack,
+            again = run_island_batch(redo, trees, cfg.max_steps, ds, ws,
                                      precision=cfg.precision, sersic_dev=sersic_dev,
                                      split=cfg.island_split, prerender=cfg.galaxy_prerender,
                                      post=cfg.post_fit, gram="fp32")
@@ -581,12 +719,19 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     free_n = free_n_gate(stype, np.nan_to_num(s1_flux, nan=0.0), np.nan_to_num(s1_var, nan=1e30))
     pad = int(max((p.bbox_class for p in plans), default=32)) + 8
     dstack, wstack = padded_stacks(fld, grids, bg, cfg.bg_order, pad)
+    dstack_raw = dstack.clone() if keep_artifacts and cfg.scene_subtract else dstack
+    scene = None
+    if cfg.scene_subtract:
+        coefs, geo = fld.psf(fld.native)
+        subtract_stars(dstack, pad, coefs, geo, fld.native, fld.n_terms, x, y, stype, s1_flux)
+        scene = SceneStars(x, y, stype, star_flux(s1_flux), fld.psf, fld.n_terms)
     sync()
     tm["islands"] = time.perf_counter() - t1
     # ---- island LM ----
     t2 = time.perf_counter()
     isf = fit_island_set(fld.psf, fld.nb, fld.n_terms, fld.native, dstack, wstack, pad, iset,
-                         plans, x, y, stype, logre, ee1, ee2, sn, free_n, cfg, timer=timer)
+                         plans, x, y, stype, logre, ee1, ee2, sn, free_n, cfg, timer=timer,
+                         scene=scene)
     fx, fy, flr, fe1, fe2 = isf.x, isf.y, isf.logre, isf.ee1, isf.ee2
     fsn = isf.sersic_n
     conv, stall, nst = isf.conv, isf.stall, isf.nst
@@ -599,18 +744,22 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     tm["island_fit"] = time.perf_counter() - t2
     # ---- pass 2 ----
     t3 = time.perf_counter()
-    flux, var, _ = linear_pass(fld, grids, fx, fy, stype, flr, fe1, fe2, fsn, cfg, gtier, timer)
+    flux, var, _ = linear_pass(fld, grids, fx, fy, stype, flr, fe1, fe2, fsn, cfg, gtier, timer,
+                               final=True)
     sync()
     tm["pass2"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
     rec = dict(n_sources=n, n_islands=int(iset.n_islands), n_fitted_islands=n_fitted_islands,
                n_giant_islands=int(giants.size), classes=[
                    (p.bbox_class, p.nsrc_class, p.kind, p.tier, p.n_islands) for p in plans],
-               knobs_off=["scene_subtract", "blank_response_correction", "wing_peeling",
-                          "priors"] + ([] if cfg.post_fit else
-                                       ["param_errors", "source_metrics", "marginal_flux_err"]))
+               knobs_off=["priors"] + ([] if cfg.scene_subtract else ["scene_subtract"])
+               + ([] if cfg.blank_response else ["blank_response_correction"])
+               + ([] if cfg.wing_peeling else ["wing_peeling"])
+               + ([] if cfg.post_fit else ["param_errors", "source_metrics",
+                                           "marginal_flux_err"]))
     if keep_artifacts:
-        rec["_artifacts"] = dict(classes=classes, dstack=dstack, wstack=wstack, pad=pad,
+        rec["_artifacts"] = dict(classes=classes, dstack=dstack, dstack_raw=dstack_raw,
+                                 scene=scene, wstack=wstack, pad=pad,
                                  grids=grids, island_set=iset, gtier=gtier, free_n=free_n,
                                  giants=giants)
     return FitResult(flux, np.sqrt(np.maximum(var, 0.0)), s1_flux, s1_var, fx, fy, flr, fe1,
@@ -679,7 +828,7 @@ def sub_island_set(iset, island_ids):
 
 def island_accuracy(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, island_ids, cat,
                     free_n, cfg: FitConfig, grams=("fp32",), sync=None,
-                    joint_nsrc_cap=JOINT_NSRC_CAP) -> dict:
+                    joint_nsrc_cap=JOINT_NSRC_CAP, scene: SceneStars | None = None) -> dict:
     """Flux error of float32 island fits (one per Gram operand precision in
     ``grams``) against the float64 fit of the same islands, per source as
     max over bands of |f - f64| / max(|f64|, sigma64):
@@ -688,7 +837,8 @@ def island_accuracy(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, isla
         follow them.
       * ``flux_solve``: the flux solve (no LM step) at the float64 fit's
         parameters -- the arithmetic precision of the fluxes.
-    ``cat`` = (x, y, stype, logre, ee1, ee2, sersic_n) catalog seeds."""
+    ``cat`` = (x, y, stype, logre, ee1, ee2, sersic_n) catalog seeds;
+    ``scene`` as for :func:`fit_island_set` (dstack then scene-subtracted)."""
     import dataclasses
     x, y, st, lr, e1, e2, sn = cat
     sub = sub_island_set(iset, island_ids)
@@ -698,7 +848,8 @@ def island_accuracy(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, isla
 
     def run(c, p_start=None):
         return fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, sub, plans, x,
-                              y, st, lr, e1, e2, sn, free_n, c, sync=sync, p_start=p_start)
+                              y, st, lr, e1, e2, sn, free_n, c, sync=sync, p_start=p_start,
+                              scene=scene)
     c64 = dataclasses.replace(cfg, precision="fp64", gram="fp32", post_fit=False)
     f64 = run(c64)
     fz64 = run(dataclasses.replace(c64, max_steps=0), f64.p)
