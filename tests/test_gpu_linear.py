@@ -152,7 +152,7 @@ def test_blank_response_matches_oracle(cuda):
     sc = small_star_scene(n_src=20)
     nb, H, W = sc.data.shape
     fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
-    g = GridDef(0, 0, 0, 15, 185, 10, 150, 5, 195, 2, 160, 10)
+    g = GridDef(0, 0, 0, 15, 145, 10, 150, 5, 155, 2, 158, 10)
     inside = (sc.x >= g.x0_pad) & (sc.x < g.x1_pad) & (sc.y >= g.y0_pad) & (sc.y < g.y1_pad)
     fi = np.where(inside)[0]
     g.source_indices = fi

@@ -215,6 +215,7 @@ struct BandArgs {
   float* bg_var;
   char* scratch;
   size_t per_problem;
+  int stage;  // weighted stamps staged in LDS (UPH_WAVES * T^2 floats of dynamic LDS)
 };
 
 // float32 background basis at full-image pixel (px, py): float64
@@ -269,63 +270,137 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_solve_kernel(BandArgs a) {
   const uint8_t* GD = a.good + (size_t)band * plane;
   for (size_t e = tid; e < (size_t)NT * NT; e += UPH_BLOCK) A[e] = 0.0;
   __syncthreads();
-  // ---- per-source self terms, one wave per source ----
-  for (int s = wid; s < n; s += UPH_WAVES) {
-    const float* st = a.stamps + (size_t)(s0 + s) * TT;
-    const int ix = a.ix_top[s0 + s], iy = a.iy_top[s0 + s];
-    float dg = 0.0f, td = 0.0f, tb[M];
+  // ---- per-source self terms and neighbour overlaps ----
+  if (a.stage) {
+    // rounds of one source per wave: the wave stages w * stamp of its source
+    // (zero off the window and on bad pixels) in LDS with the self terms,
+    // then takes each neighbour's dot product over the two stamps' overlap
+    // rectangle only, reading the neighbour stamp once
+    extern __shared__ float ws_lds[];
+    float* wsb = ws_lds + (size_t)wid * TT;
+    for (int r0 = 0; r0 < n; r0 += UPH_WAVES) {
+      const int s = r0 + wid;
+      if (s < n) {
+        const float* st = a.stamps + (size_t)(s0 + s) * TT;
+        const int ix = a.ix_top[s0 + s], iy = a.iy_top[s0 + s];
+        float dg = 0.0f, td = 0.0f, tb[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) tb[m] = 0.0f;
-    for (int e = lane; e < TT; e += 64) {
-      const int py = iy + e / T, px = ix + e % T;
-      if (py < 0 || py >= gh || px < 0 || px >= gw) continue;
-      const size_t gi = (size_t)(gy0 + py) * a.W + gx0 + px;
-      const float wv = GD[gi] ? IV[gi] : 0.0f;
-      if (wv == 0.0f) continue;
-      const float sv = st[e];
-      const float ws = wv * sv;
-      dg = fmaf(ws, sv, dg);
-      td = fmaf(ws, D[gi], td);
-      float br[M];
-      bg_row<M>(a, a.img_x0 + gx0 + px, a.img_y0 + gy0 + py, br);
+        for (int m = 0; m < M; ++m) tb[m] = 0.0f;
+        for (int e = lane; e < TT; e += 64) {
+          const int py = iy + e / T, px = ix + e % T;
+          float ws = 0.0f;
+          if (py >= 0 && py < gh && px >= 0 && px < gw) {
+            const size_t gi = (size_t)(gy0 + py) * a.W + gx0 + px;
+            const float wv = GD[gi] ? IV[gi] : 0.0f;
+            if (wv != 0.0f) {
+              const float sv = st[e];
+              ws = wv * sv;
+              dg = fmaf(ws, sv, dg);
+              td = fmaf(ws, D[gi], td);
+              float br[M];
+              bg_row<M>(a, a.img_x0 + gx0 + px, a.img_y0 + gy0 + py, br);
 #pragma unroll
-      for (int m = 0; m < M; ++m) tb[m] = fmaf(ws, br[m], tb[m]);
+              for (int m = 0; m < M; ++m) tb[m] = fmaf(ws, br[m], tb[m]);
+            }
+          }
+          wsb[e] = ws;
+        }
+        dg = wave_sum(dg);
+        td = wave_sum(td);
+#pragma unroll
+        for (int m = 0; m < M; ++m) tb[m] = wave_sum(tb[m]);
+        if (lane == 0) {
+          A[(size_t)s * NT + s] = double(dg);
+          rhs[s] = double(td);
+          for (int m = 0; m < M; ++m) {
+            A[(size_t)s * NT + n + m] = double(tb[m]);
+            A[(size_t)(n + m) * NT + s] = double(tb[m]);
+          }
+        }
+      }
+      __syncthreads();
+      if (s < n) {
+        const int ixs = a.ix_top[s0 + s], iys = a.iy_top[s0 + s];
+        for (int k = 0; k < a.k_max; ++k) {
+          const int t = a.nbr[(s0 + s) * a.k_max + k];
+          if (t < 0) continue;  // uniform within the wave
+          const float* tt = a.stamps + (size_t)(s0 + t) * TT;
+          const int di = iys - a.iy_top[s0 + t], dj = ixs - a.ix_top[s0 + t];
+          const int ky0 = max(0, -di), ky1 = min(T, T - di);
+          const int kx0 = max(0, -dj), kx1 = min(T, T - dj);
+          const int ow = kx1 - kx0, oh = ky1 - ky0;
+          float acc = 0.0f;
+          if (ow > 0 && oh > 0) {
+            for (int e = lane; e < ow * oh; e += 64) {
+              const int ky = ky0 + e / ow, kx = kx0 + e % ow;
+              acc = fmaf(wsb[ky * T + kx], tt[(ky + di) * T + kx + dj], acc);
+            }
+          }
+          acc = wave_sum(acc);
+          if (lane == 0) A[(size_t)s * NT + t] += double(acc);
+        }
+      }
+      __syncthreads();
     }
-    dg = wave_sum(dg);
-    td = wave_sum(td);
+  } else {
+    // ---- per-source self terms, one wave per source ----
+    for (int s = wid; s < n; s += UPH_WAVES) {
+      const float* st = a.stamps + (size_t)(s0 + s) * TT;
+      const int ix = a.ix_top[s0 + s], iy = a.iy_top[s0 + s];
+      float dg = 0.0f, td = 0.0f, tb[M];
 #pragma unroll
-    for (int m = 0; m < M; ++m) tb[m] = wave_sum(tb[m]);
-    if (lane == 0) {
-      A[(size_t)s * NT + s] = double(dg);
-      rhs[s] = double(td);
-      for (int m = 0; m < M; ++m) {
-        A[(size_t)s * NT + n + m] = double(tb[m]);
-        A[(size_t)(n + m) * NT + s] = double(tb[m]);
+      for (int m = 0; m < M; ++m) tb[m] = 0.0f;
+      for (int e = lane; e < TT; e += 64) {
+        const int py = iy + e / T, px = ix + e % T;
+        if (py < 0 || py >= gh || px < 0 || px >= gw) continue;
+        const size_t gi = (size_t)(gy0 + py) * a.W + gx0 + px;
+        const float wv = GD[gi] ? IV[gi] : 0.0f;
+        if (wv == 0.0f) continue;
+        const float sv = st[e];
+        const float ws = wv * sv;
+        dg = fmaf(ws, sv, dg);
+        td = fmaf(ws, D[gi], td);
+        float br[M];
+        bg_row<M>(a, a.img_x0 + gx0 + px, a.img_y0 + gy0 + py, br);
+#pragma unroll
+        for (int m = 0; m < M; ++m) tb[m] = fmaf(ws, br[m], tb[m]);
+      }
+      dg = wave_sum(dg);
+      td = wave_sum(td);
+#pragma unroll
+      for (int m = 0; m < M; ++m) tb[m] = wave_sum(tb[m]);
+      if (lane == 0) {
+        A[(size_t)s * NT + s] = double(dg);
+        rhs[s] = double(td);
+        for (int m = 0; m < M; ++m) {
+          A[(size_t)s * NT + n + m] = double(tb[m]);
+          A[(size_t)(n + m) * NT + s] = double(tb[m]);
+        }
       }
     }
-  }
-  // ---- neighbour overlaps, one wave per (source, slot) ----
-  for (int u = wid; u < n * a.k_max; u += UPH_WAVES) {
-    const int s = u / a.k_max;
-    const int t = a.nbr[(s0 + s) * a.k_max + u % a.k_max];
-    if (t < 0) continue;  // uniform within the wave
-    const float* ss = a.stamps + (size_t)(s0 + s) * TT;
-    const float* tt = a.stamps + (size_t)(s0 + t) * TT;
-    const int ixs = a.ix_top[s0 + s], iys = a.iy_top[s0 + s];
-    const int di = iys - a.iy_top[s0 + t], dj = ixs - a.ix_top[s0 + t];
-    float acc = 0.0f;
-    for (int e = lane; e < TT; e += 64) {
-      const int ky = e / T, kx = e % T;
-      const int ty = ky + di, tx = kx + dj;
-      if (ty < 0 || ty >= T || tx < 0 || tx >= T) continue;
-      const int py = iys + ky, px = ixs + kx;
-      if (py < 0 || py >= gh || px < 0 || px >= gw) continue;
-      const size_t gi = (size_t)(gy0 + py) * a.W + gx0 + px;
-      const float wv = GD[gi] ? IV[gi] : 0.0f;
-      acc = fmaf(wv * ss[e], tt[ty * T + tx], acc);
+    // ---- neighbour overlaps, one wave per (source, slot) ----
+    for (int u = wid; u < n * a.k_max; u += UPH_WAVES) {
+      const int s = u / a.k_max;
+      const int t = a.nbr[(s0 + s) * a.k_max + u % a.k_max];
+      if (t < 0) continue;  // uniform within the wave
+      const float* ss = a.stamps + (size_t)(s0 + s) * TT;
+      const float* tt = a.stamps + (size_t)(s0 + t) * TT;
+      const int ixs = a.ix_top[s0 + s], iys = a.iy_top[s0 + s];
+      const int di = iys - a.iy_top[s0 + t], dj = ixs - a.ix_top[s0 + t];
+      float acc = 0.0f;
+      for (int e = lane; e < TT; e += 64) {
+        const int ky = e / T, kx = e % T;
+        const int ty = ky + di, tx = kx + dj;
+        if (ty < 0 || ty >= T || tx < 0 || tx >= T) continue;
+        const int py = iys + ky, px = ixs + kx;
+        if (py < 0 || py >= gh || px < 0 || px >= gw) continue;
+        const size_t gi = (size_t)(gy0 + py) * a.W + gx0 + px;
+        const float wv = GD[gi] ? IV[gi] : 0.0f;
+        acc = fmaf(wv * ss[e], tt[ty * T + tx], acc);
+      }
+      acc = wave_sum(acc);
+      if (lane == 0) A[(size_t)s * NT + t] += double(acc);
     }
-    acc = wave_sum(acc);
-    if (lane == 0) A[(size_t)s * NT + t] += double(acc);
   }
   // ---- background Gram and target over the whole window ----
   {
@@ -796,13 +871,16 @@ extern "C" int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const 
   a.bg_var = bg_var;
   a.scratch = static_cast<char*>(scratch);
   a.per_problem = (scratch_bytes / (size_t)n_prob) & ~(size_t)255;
+  // stage the weighted stamps in LDS up to T = 63 (64 KiB per workgroup)
+  a.stage = T <= 63;
+  const size_t lds = a.stage ? (size_t)UPH_WAVES * T * T * sizeof(float) : 0;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (a.M == 1)
-    hipLaunchKernelGGL(uph::band_solve_kernel<1>, dim3(n_prob), dim3(UPH_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(uph::band_solve_kernel<1>, dim3(n_prob), dim3(UPH_BLOCK), lds, s, a);
   else if (a.M == 3)
-    hipLaunchKernelGGL(uph::band_solve_kernel<3>, dim3(n_prob), dim3(UPH_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(uph::band_solve_kernel<3>, dim3(n_prob), dim3(UPH_BLOCK), lds, s, a);
   else
-    hipLaunchKernelGGL(uph::band_solve_kernel<6>, dim3(n_prob), dim3(UPH_BLOCK), 0, s, a);
+    hipLaunchKernelGGL(uph::band_solve_kernel<6>, dim3(n_prob), dim3(UPH_BLOCK), lds, s, a);
   return uph::hip_status(hipGetLastError(), "uph_band_solve launch");
 }
 

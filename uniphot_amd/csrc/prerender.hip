@@ -484,6 +484,224 @@ struct StampClip {
   int sx0, sx1, sy0, sy1, rx0, rx1, ry0, ry1;
 };
 
+// ---------------------------------------------------------------------------
+// LDS-resident form of the planes pipeline (pad 128 with the raw planes,
+// the line buffer, the spectrum and two shifted planes all in LDS): the
+// horizontal Lanczos pass is applied to each output row of an inverse
+// transform while the row is in registers, and only the rows and columns
+// the window box needs are kept, so no raw plane is stored anywhere.
+// ---------------------------------------------------------------------------
+constexpr int kPoolFloats = 40704;  // LDS pool of the planes kernel (159 KiB)
+
+__host__ __device__ inline bool planes_in_lds(int T, int pad) {
+  const int H = pad / 2 + 1;
+  return pad == 128 && 2 * (H * T + H * pad) + 5 * T * T <= kPoolFloats;
+}
+
+// Horizontal 7-tap pass of one row held across the wave (element e of lane
+// l is column l + 64 e): h[e] = sum_k L[k] v(c + k - 3) and, when `both`,
+// hd[e] = sum_k dL[k] v(c + k - 3), over source columns in [c0, c1) (zero
+// outside).  Neighbour columns come from the other lanes by __shfl.
+template <int E>
+__device__ __forceinline__ void lanczos_row(const float (&v)[E], const float* L, const float* dL,
+                                            int c0, int c1, bool both, float (&h)[E],
+                                            float (&hd)[E]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int e = 0; e < E; ++e) h[e] = hd[e] = 0.0f;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    const int d = k - 3;
+    const int sl = (lane + d) & 63;
+    const int wrap = lane + d < 0 ? -1 : (lane + d >= 64 ? 1 : 0);
+    float s[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) s[e] = d == 0 ? v[e] : __shfl(v[e], sl, 64);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      float x = s[e];
+      if (wrap < 0) x = e > 0 ? s[e > 0 ? e - 1 : 0] : 0.0f;
+      if (wrap > 0) x = e + 1 < E ? s[e + 1 < E ? e + 1 : 0] : 0.0f;
+      const int c = lane + 64 * e + d;
+      if (c < c0 || c >= c1) x = 0.0f;
+      h[e] = fmaf(L[k], x, h[e]);
+      if (both) hd[e] = fmaf(dL[k], x, hd[e]);
+    }
+  }
+}
+
+// Column half of an inverse transform: A[k][r] = sum_m S[k][m] X[k][m]
+// e^{+2 pi i m r / pad} for r < T, X the spectrum of output kind `kind`
+// (0: F + core, 1..3: F1 d(expo)/d(logre, ee1, ee2), 4: Fn + dcore).
+template <int E>
+__device__ void inverse_columns(const PlaneJob& j, int kind, const WaveTw<E>& tw) {
+  constexpr int G = Batch<E>::G;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int T = j.T, H = j.H, pad = j.pad, logp = j.logp, n = H * pad;
+  const float* X = j.F + (kind == 0 ? 0 : (kind == 4 ? 2 * n : n));
+  const double c2 = -2.0 * 9.869604401089358;
+  const int p = kind - 1;
+  for (int k0 = wid * G; k0 < H; k0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      const double v = double(k) / pad;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int m = brev_bits(lane + 64 * e, logp);
+        float2 sv = make_float2(0.0f, 0.0f);
+        float x = 0.0f;
+        if (k < H) {
+          const size_t o = (size_t)k * pad + m;
+          sv = j.S[o];
+          x = X[o];
+          if (kind >= 1 && kind <= 3) {
+            const double w = double(m < pad / 2 ? m : m - pad) / pad;
+            const double dex = c2 * (j.dt[0 * 3 + p] * v * v + j.dt[1 * 3 + p] * w * w +
+                                     2.0 * j.dt[2 * 3 + p] * v * w);
+            x = float(double(x) * dex);
+          }
+        }
+        re[g][e] = sv.x * x;
+        im[g][e] = -(sv.y * x);
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (k0 + g < H) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int k = k0 + g;
+      if (k >= H) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int r = lane + 64 * e;
+        if (r < T) j.A[k * T + r] = make_float2(re[g][e], -im[g][e]);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Row half of an inverse transform with the horizontal shift fused: rows
+// [ry0, ry1) of the raw plane (Hermitian row transforms of A), filtered
+// with L (and dL into Hb when Hb is given) and stored for the window-box
+// columns as Ha[(r - ry0) * SW + c - sx0], SW = sx1 - sx0.
+template <int E>
+__device__ void inverse_rows_hshift(const PlaneJob& j, const StampClip& cl, const float* L,
+                                    const float* dL, float* Ha, float* Hb, const WaveTw<E>& tw) {
+  constexpr int G = Batch<E>::G;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int T = j.T, pad = j.pad, logp = j.logp, H = j.H;
+  const int SW = cl.sx1 - cl.sx0;
+  const bool both = Hb != nullptr;
+  const float norm = 1.0f / (float(pad) * float(pad));
+  for (int r0 = cl.ry0 + wid * G; r0 < cl.ry1; r0 += kGalWaves * G) {
+    float re[G][E], im[G][E];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = brev_bits(lane + 64 * e, logp);
+        float2 y = make_float2(0.0f, 0.0f);
+        if (r < cl.ry1) {
+          if (c < H) {
+            y = j.A[c * T + r];
+          } else {
+            y = j.A[(pad - c) * T + r];
+            y.y = -y.y;
+          }
+        }
+        re[g][e] = y.x;
+        im[g][e] = -y.y;
+      }
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+      if (r0 + g < cl.ry1) wave_fft<E>(re[g], im[g], tw);
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+      const int r = r0 + g;
+      if (r >= cl.ry1) continue;  // uniform across the wave
+      float v[E], h[E], hd[E];
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = re[g][e] * norm;
+      lanczos_row<E>(v, L, dL, cl.rx0, cl.rx1, both, h, hd);
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int c = lane + 64 * e;
+        if (c < cl.sx0 || c >= cl.sx1) continue;
+        const int o = (r - cl.ry0) * SW + c - cl.sx0;
+        Ha[o] = h[e];
+        if (both) Hb[o] = hd[e];
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Vertical 7-tap pass of a horizontally shifted buffer Hs (rows [ry0, ry1),
+// SW columns) into the window box of an output plane (side x side, from
+// its corner): out = scale * sum_k L[k] Hs(r + k - 3), or out += that sum.
+template <int kThreads>
+__device__ __forceinline__ void vshift_plane(const float* Hs, const StampClip& cl, const float* L,
+                                             float* out, int side, float scale, bool add) {
+  const int SW = cl.sx1 - cl.sx0, SH = cl.sy1 - cl.sy0;
+  for (int q = threadIdx.x; q < SH * SW; q += kThreads) {
+    const int rw = cl.sy0 + q / SW, cc = q % SW;
+    float v = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      const int rr = rw + k - 3;
+      if (rr < cl.ry0 || rr >= cl.ry1) continue;
+      v = fmaf(L[k], Hs[(rr - cl.ry0) * SW + cc], v);
+    }
+    const int e = (q / SW) * side + cc;
+    out[e] = add ? out[e] + v : v * scale;
+  }
+}
+
+// All planes of an extended member with the raw planes P3 and the shifted
+// buffers Ha, Hb in LDS: plane 0 = Ly (Lx R0); plane 1 = Ly (dLx R0) +
+// Ly (Lx R1); plane 2 = dLy (Lx R0) + Ly (Lx R2); planes 3..5 = Ly (Lx R);
+// plane 6 = dn/du Ly (Lx R6), R the raw planes of section render_planes.
+// `out` holds the member's 7 planes of side x side floats for this band.
+template <int E>
+__device__ void render_planes_lds(const PlaneJob& j, const StampClip& cl, const float* taps,
+                                  float dndu, float* Ha, float* Hb, float* out, int side) {
+  WaveTw<E> tw;
+  wave_tw<E>(tw);
+  const int TT = j.T * j.T, PP = side * side, need = j.need;
+  const float* Lx = taps;
+  const float* dLx = taps + 7;
+  const float* Ly = taps + 14;
+  const float* dLy = taps + 21;
+  forward_plane<E>(j, j.P3, tw);
+  inverse_columns<E>(j, 0, tw);
+  inverse_rows_hshift<E>(j, cl, Lx, dLx, Ha, (need & 1) ? Hb : nullptr, tw);
+  vshift_plane<kPreBlock>(Ha, cl, Ly, out, side, 1.0f, false);
+  if (need & 2) vshift_plane<kPreBlock>(Ha, cl, dLy, out + (size_t)2 * PP, side, 1.0f, false);
+  if (need & 1) vshift_plane<kPreBlock>(Hb, cl, Ly, out + (size_t)PP, side, 1.0f, false);
+  for (int q = 0; q < 4; ++q) {
+    const int bit = q < 3 ? (4 << q) : 32;
+    if (!(need & bit)) continue;
+    inverse_columns<E>(j, 1 + q, tw);
+    inverse_rows_hshift<E>(j, cl, Lx, dLx, Ha, nullptr, tw);
+    vshift_plane<kPreBlock>(Ha, cl, Ly, out + (size_t)(3 + q) * PP, side, q < 3 ? 1.0f : dndu,
+                            false);
+  }
+  for (int q = 0; q < 2; ++q) {
+    if (!(need & (1 << q))) continue;
+    forward_plane<E>(j, j.P3 + (size_t)(1 + q) * TT, tw);
+    inverse_columns<E>(j, 0, tw);
+    inverse_rows_hshift<E>(j, cl, Lx, dLx, Ha, nullptr, tw);
+    vshift_plane<kPreBlock>(Ha, cl, Ly, out + (size_t)(1 + q) * PP, side, 1.0f, true);
+  }
+  __syncthreads();
+}
+
 // Separable Lanczos-3 shift of the raw planes R (plane 0 the stamp, 1 and 2
 // its x and y raw tangents, 3..6 the shape and n tangents) into out:
 // horizontal pass into Hh, vertical pass into out, over the window box of
@@ -644,7 +862,7 @@ __device__ __forceinline__ void psf_box(const float* Cf, int nt, int T, const fl
 // Extended members: (member, band) items [0, n_pre_ext * nb) from a work
 // queue; the whole stamp goes through the Fourier mixture in LDS.
 __global__ __launch_bounds__(kPreBlock) void prerender_planes_kernel(IslandArgsPre a) {
-  __shared__ float2 lds[kGalLdsComplex];
+  __shared__ float2 lds[kPoolFloats / 2];
   __shared__ float bas[63];
   __shared__ float taps[28];
   __shared__ int s_item, s_ok;
@@ -694,13 +912,41 @@ __global__ __launch_bounds__(kPreBlock) void prerender_planes_kernel(IslandArgsP
     const int g = s_it.g, b = s_it.b, T = s_it.T, need = s_it.need;
     const size_t slot = s_it.slot;
     if (tid < 28) taps[tid] = a.gtap[(size_t)g * 28 + tid];
+    const int pad = fft_pad_for(T), H = pad / 2 + 1;
+    const float* Cf = a.tier_coefs[a.shape[4 * (size_t)s_it.isl + 2]] + (size_t)b * nt * T * T;
+    if (planes_in_lds(T, pad)) {
+      // LDS pool: line buffer (H, T), spectrum (H, pad), Ha, Hb (T^2 each), P3
+      float* Ha = reinterpret_cast<float*>(lds + H * T + H * pad);
+      float* Hb = Ha + T * T;
+      float* P3 = Hb + T * T;
+      psf_box<kPreBlock>(Cf, nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
+      __syncthreads();
+      PlaneJob j;
+      j.g = g;
+      j.b = b;
+      j.T = T;
+      j.pad = pad;
+      j.logp = ilog2(pad);
+      j.H = H;
+      j.need = need;
+      j.coefs = nullptr;
+      j.p6 = psrc + slot * 6;
+      j.out = nullptr;
+      j.P3 = P3;
+      j.A = lds;
+      j.S = lds + H * T;
+      j.F = a.fspec + a.pre_spec[g];
+      j.dt = a.gdt + (size_t)g * 9;
+      const int side = s_it.side;
+      render_planes_lds<2>(j, s_it.clip, taps, s_dndu, Ha, Hb,
+                           a.pbuf + a.pre_off[g] + (size_t)b * 7 * side * side, side);
+      continue;
+    }
     const PreScratch L = pre_scratch(T);
     float* P3 = base + L.P3;
     float* R = base + L.R;
-    psf_box<kPreBlock>(a.tier_coefs[a.shape[4 * (size_t)s_it.isl + 2]] + (size_t)b * nt * T * T,
-                       nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
+    psf_box<kPreBlock>(Cf, nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
     __syncthreads();
-    const int pad = fft_pad_for(T), H = pad / 2 + 1;
     PlaneJob j;
     j.g = g;
     j.b = b;

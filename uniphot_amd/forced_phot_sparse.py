@@ -85,6 +85,20 @@ class BandProblem:
     origin: tuple | None = None   # (gx0, gy0) for the sub-pixel remainder
 
 
+def check_windows(win, n_bands: int, H: int, W: int, what: str) -> None:
+    """Raise ValueError unless every window (band, x0, y0, h, w) lies inside
+    an (n_bands, H, W) image stack (the kernels index the stack without
+    bounds checks)."""
+    win = np.asarray(win, np.int64).reshape(-1, 5)
+    b, x0, y0, h, w = (win[:, i] for i in range(5))
+    bad = ((b < 0) | (b >= n_bands) | (x0 < 0) | (y0 < 0) | (h < 0) | (w < 0)
+           | (x0 + w > W) | (y0 + h > H))
+    if bad.any():
+        q = int(np.nonzero(bad)[0][0])
+        raise ValueError(f"{what}: window {q} {win[q].tolist()} (band, x0, y0, h, w) is not "
+                         f"inside the ({n_bands}, {H}, {W}) image")
+
+
 def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, data_dev,
                    invvar_dev, good_dev, *, bg_order: int = 1, bg_geo=(0.0, 1.0, 0.0, 1.0),
                    img_origin=(0, 0), reg: float = 1e-9, k_max: int = DEFAULT_K_MAX,
@@ -122,6 +136,7 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
     nbr = np.concatenate([p.nbr_idx[:, :k_max] for p in problems]).astype(np.int32) if ntot \
         else np.zeros((0, k_max), np.int32)
     win = np.array([[p.band, p.x0, p.y0, p.h, p.w] for p in problems], np.int32)
+    check_windows(win, nb, H, W, "solve_problems")
     mixed_flag = np.array([p.stype is not None and bool(np.any(p.stype != SOURCE_STAR))
                            for p in problems])
     up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)

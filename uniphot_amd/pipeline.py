@@ -234,6 +234,7 @@ def blank_response_correct(fld, T, problems, owners, res, x, y, stype, logre, ee
         bg = np.stack([out[q].bg_coefs for q in qs]).astype(np.float32)
         win = np.array([[owners[q][1], problems[q].x0, problems[q].y0, problems[q].h,
                          problems[q].w] for q in qs], np.int32)
+        fps.check_windows(win, nb, H, W, "blank_response_correct")
         phi = np.empty((qs.size, T, T), np.float32)
         for j, q in enumerate(qs):
             pr = problems[q]
@@ -339,6 +340,7 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
         off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
         win = np.array([[owners[q][1], problems[q].x0, problems[q].y0, problems[q].h,
                          problems[q].w] for q in sel], np.int32)
+        fps.check_windows(win, nb, H, W, "wing_peel")
         wing = torch.zeros((nb, H, W), dtype=torch.float32, device=dev)
         keep = [up(off, np.int64), up(win, np.int32), up(ix, np.int32), up(iy, np.int32),
                 up(fl, np.float32)]

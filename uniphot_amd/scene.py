@@ -122,7 +122,7 @@ def subtract_stars(dstack, pad: int, psf_coefs, psf_geo, T: int, n_terms: int, x
     ys = np.asarray(y, np.float32)[stars]
     ix = _rint(xs) - T // 2 + pad
     iy = _rint(ys) - T // 2 + pad
-    off, win, lst = tile_lists(ix, iy, T, Hp, Wp, tile)
+    off, win, lst = tile_lists(ix, iy, T, Hp, Wp, tile)   # tiles inside the stack
     if win.shape[0] == 0:
         return 0
     n_st = stars.size
@@ -193,6 +193,9 @@ def island_mosaic(chunk, plan_of, dstack, wstack, scene: SceneStars, *,
     sx = np.concatenate(sx)
     sy = np.concatenate(sy)
     own = np.concatenate(own)
+    Hs, Ws = int(dstack.shape[1]), int(dstack.shape[2])
+    if (sx.min() < 0 or sy.min() < 0 or np.any(sx + sides > Ws) or np.any(sy + sides > Hs)):
+        raise ValueError("island_mosaic: an island window is not inside the field stack")
     mx, my, Hm, Wm = shelf_pack(sides)
     dmos = torch.empty((nb, Hm, Wm), dtype=torch.float32, device=dev)
     wmos = torch.empty_like(dmos)
