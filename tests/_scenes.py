@@ -72,8 +72,9 @@ def oracle_island(plan, host, row, d, w, psfs):
 
 
 def small_mixed_scene(seed=21, H=200, W=200, n_bands=2, n_src=16, order=2, flux_lo=5e3,
-                      flux_hi=2e5, types=(0, 1, 2, 3)):
-    """Scene with stars and EXP/DEV/SERSIC galaxies (Re 1.2-3 px)."""
+                      flux_hi=2e5, types=(0, 1, 2, 3), re_range=(1.2, 3.0)):
+    """Scene with stars and EXP/DEV/SERSIC galaxies (Re uniform in
+    ``re_range`` px)."""
     from uniphot_amd.source_model import ellipse_soft_from_re_ab_phi_np
     rng = np.random.default_rng(seed)
     fields = [synth.MixturePsfField(H, W, (1.0, 2.0, 4.0), (0.7, 0.25, 0.05), amp=0.1, rng=rng)
@@ -84,7 +85,7 @@ def small_mixed_scene(seed=21, H=200, W=200, n_bands=2, n_src=16, order=2, flux_
     tx, ty = pts[:, 0], pts[:, 1]
     n = tx.size
     st = np.asarray(types, np.int32)[rng.integers(len(types), size=n)]
-    lr, e1, e2 = ellipse_soft_from_re_ab_phi_np(rng.uniform(1.2, 3.0, n), rng.uniform(0.5, 1, n),
+    lr, e1, e2 = ellipse_soft_from_re_ab_phi_np(rng.uniform(*re_range, n), rng.uniform(0.5, 1, n),
                                                 rng.uniform(0, 180, n))
     gal = st != SOURCE_STAR
     lr = np.where(gal, lr, 0).astype(np.float32)

@@ -99,15 +99,22 @@ def test_mixed_band_split_equals_persistent(cuda):
             assert np.array_equal(ra[k], rb[k]), k
 
 
-def test_galaxy_prerender_matches_oracle(cuda):
+@pytest.mark.parametrize("tier", [55, 111])
+def test_galaxy_prerender_matches_oracle(cuda, tier):
     """Band-split fit with the galaxy planes rendered by their own launches
-    (float32 wave-level FFTs in LDS) against the float64 oracle, at the bar
-    of the in-place render."""
+    (float32 wave-level FFTs) against the float64 oracle, at the bar of the
+    in-place render: native-tier galaxies (planes in the LDS pool, pad 128)
+    and de Vaucouleurs galaxies of Re 3.6-5 px on the 111 stamp tier (pad
+    256, planes in global scratch)."""
     import torch
     from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
     from uniphot_amd.sersic_table import device_table
-    sc = small_mixed_scene()
+    if tier == 55:
+        sc = small_mixed_scene()
+    else:
+        sc = small_mixed_scene(seed=4, n_src=8, types=(0, 2), re_range=(3.6, 5.0))
     plans, hosts, pad, d, w = mixed_classes(sc, free_n=(sc.stype == 3))
+    assert any(p.tier == tier and p.kind != "star" for p in plans)
     batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms), h, np.arange(p.n_islands))
              for p, h in zip(plans, hosts)]
     trees = {T: psf_tree_device(sc.psfs, T, cuda) for T in {p.tier for p in plans}}

@@ -13,9 +13,10 @@
 //     wave-level FFTs: rows of P -> LDS line buffer -> columns -> spectrum
 //     S (LDS for pad <= 128, per-workgroup global columns for pad 256); each
 //     output plane is one column pass over S times its spectrum into the LDS
-//     line buffer and one Hermitian row pass.  The raw planes are then
-//     shifted by the member's sub-pixel offset (separable Lanczos-3 with its
-//     tap derivatives) into the member's output planes.
+//     line buffer and one Hermitian row pass over the rows the window box
+//     needs, whose outputs get the horizontal part of the member's sub-pixel
+//     shift (separable Lanczos-3 with its tap derivatives) while still in
+//     registers; the vertical part then writes the member's output planes.
 // The core term is folded into the spectra (irfft2(S (F + core)) =
 // irfft2(S F) + core P for the cropped stamp).
 // Algorithm followed: tract7dt/fit/grid_gpu/sersic_mog.py
@@ -46,11 +47,11 @@ __host__ __device__ inline bool galaxy_fits(int T, int pad) {
 
 __host__ __device__ inline size_t round64(size_t v) { return (v + 63) / 64 * 64; }
 
-// Per-workgroup scratch of the planes kernel (floats, 64-aligned regions):
-// P, dP/dx, dP/dy | 7 raw planes | 7 horizontally shifted planes |
-// spectrum columns (pad 256 only).
+// Per-workgroup global scratch of the planes kernel for stamps whose
+// planes do not fit the LDS pool (floats, 64-aligned regions): P, dP/dx,
+// dP/dy | two horizontally shifted planes | spectrum columns (pad 256).
 struct PreScratch {
-  size_t P3, R, Hh, S, total;
+  size_t P3, H2, S, total;
 };
 
 __host__ __device__ inline PreScratch pre_scratch(int T) {
@@ -58,9 +59,8 @@ __host__ __device__ inline PreScratch pre_scratch(int T) {
   const size_t TT = (size_t)T * T;
   PreScratch o;
   o.P3 = 0;
-  o.R = round64(3 * TT);
-  o.Hh = o.R + round64(7 * TT);
-  o.S = o.Hh + round64(7 * TT);
+  o.H2 = round64(3 * TT);
+  o.S = o.H2 + round64(2 * TT);
   o.total = o.S + (pad > 128 ? (size_t)2 * H * pad : 0);
   return o;
 }
@@ -281,10 +281,8 @@ __global__ __launch_bounds__(UPH_BLOCK) void prerender_spectra_kernel(IslandArgs
 // ---------------------------------------------------------------------------
 struct PlaneJob {
   int g, b, T, pad, logp, H, need;
-  const float* coefs;
   const float* p6;
-  float* out;     // the item's 7 raw planes (T^2 each)
-  float* P3;      // per-workgroup P, dP/dx, dP/dy (global scratch)
+  float* P3;      // P, dP/dx, dP/dy (LDS or the workgroup's global scratch)
   float2* S;      // spectrum (H, pad): LDS or global scratch
   float2* A;      // LDS line buffer (H, T)
   const float* F; // the galaxy's spectra
@@ -301,7 +299,8 @@ struct Batch {
 
 // Forward transform of the T x T real plane src into S (H columns of pad).
 template <int E>
-__device__ void forward_plane(const PlaneJob& j, const float* src, const WaveTw<E>& tw) {
+__device__ __forceinline__ void forward_plane(const PlaneJob& j, const float* src,
+                                              const WaveTw<E>& tw) {
   constexpr int G = Batch<E>::G;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int T = j.T, H = j.H, pad = j.pad, logp = j.logp;
@@ -362,120 +361,6 @@ __device__ void forward_plane(const PlaneJob& j, const float* src, const WaveTw<
   __syncthreads();
 }
 
-// Inverse transform of S times the spectrum of output kind `kind`
-// (0: F + core, 1..3: F1 d(expo)/d(logre, ee1, ee2), 4: Fn + dcore) into
-// the T x T plane dst: dst = irfft2(S X, (pad, pad))[:T, :T].
-template <int E>
-__device__ void inverse_plane(const PlaneJob& j, int kind, float* dst, const WaveTw<E>& tw) {
-  constexpr int G = Batch<E>::G;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int T = j.T, H = j.H, pad = j.pad, logp = j.logp, n = H * pad;
-  const float* X = j.F + (kind == 0 ? 0 : (kind == 4 ? 2 * n : n));
-  const double c2 = -2.0 * 9.869604401089358;
-  const int p = kind - 1;
-  // columns: Y[k][r] = sum_m S[k][m] X[k][m] e^{+2 pi i m r / pad}, r < T
-  for (int k0 = wid * G; k0 < H; k0 += kGalWaves * G) {
-    float re[G][E], im[G][E];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int k = k0 + g;
-      const double v = double(k) / pad;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int m = brev_bits(lane + 64 * e, logp);
-        float2 sv = make_float2(0.0f, 0.0f);
-        float x = 0.0f;
-        if (k < H) {
-          const size_t o = (size_t)k * pad + m;
-          sv = j.S[o];
-          x = X[o];
-          if (kind >= 1 && kind <= 3) {
-            const double w = double(m < pad / 2 ? m : m - pad) / pad;
-            const double dex = c2 * (j.dt[0 * 3 + p] * v * v + j.dt[1 * 3 + p] * w * w +
-                                     2.0 * j.dt[2 * 3 + p] * v * w);
-            x = float(double(x) * dex);
-          }
-        }
-        // inverse via conj(FFT(conj(z)))
-        re[g][e] = sv.x * x;
-        im[g][e] = -(sv.y * x);
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-      if (k0 + g < H) wave_fft<E>(re[g], im[g], tw);
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int k = k0 + g;
-      if (k >= H) continue;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int r = lane + 64 * e;
-        if (r < T) j.A[k * T + r] = make_float2(re[g][e], -im[g][e]);
-      }
-    }
-  }
-  __syncthreads();
-  // rows: out[r][c] = Re sum_k X_r[k] e^{+2 pi i k c / pad} / pad^2 with
-  // the Hermitian extension X_r[k] = Y[k][r] (k < H), conj(Y[pad - k][r])
-  const float norm = 1.0f / (float(pad) * float(pad));
-  for (int r0 = wid * G; r0 < T; r0 += kGalWaves * G) {
-    float re[G][E], im[G][E];
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int r = r0 + g;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int c = brev_bits(lane + 64 * e, logp);
-        float2 y = make_float2(0.0f, 0.0f);
-        if (r < T) {
-          if (c < H) {
-            y = j.A[c * T + r];
-          } else {
-            y = j.A[(pad - c) * T + r];
-            y.y = -y.y;
-          }
-        }
-        re[g][e] = y.x;
-        im[g][e] = -y.y;
-      }
-    }
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-      if (r0 + g < T) wave_fft<E>(re[g], im[g], tw);
-#pragma unroll
-    for (int g = 0; g < G; ++g) {
-      const int r = r0 + g;
-      if (r >= T) continue;
-#pragma unroll
-      for (int e = 0; e < E; ++e) {
-        const int c = lane + 64 * e;
-        if (c < T) dst[r * T + c] = re[g][e] * norm;
-      }
-    }
-  }
-  __syncthreads();
-}
-
-template <int E>
-__device__ void render_planes(const PlaneJob& j) {
-  WaveTw<E> tw;
-  wave_tw<E>(tw);
-  const int TT = j.T * j.T, need = j.need;
-  // source P: model (plane 0), shape tangents (planes 3..5), n (plane 6)
-  forward_plane<E>(j, j.P3, tw);
-  inverse_plane<E>(j, 0, j.out, tw);
-  for (int q = 0; q < 3; ++q)
-    if (need & (4 << q)) inverse_plane<E>(j, 1 + q, j.out + (size_t)(3 + q) * TT, tw);
-  if (need & 32) inverse_plane<E>(j, 4, j.out + (size_t)6 * TT, tw);
-  // sources dP/dx, dP/dy: position tangents (planes 1, 2)
-  for (int q = 0; q < 2; ++q) {
-    if (!(need & (1 << q))) continue;
-    forward_plane<E>(j, j.P3 + (size_t)(1 + q) * TT, tw);
-    inverse_plane<E>(j, 0, j.out + (size_t)(1 + q) * TT, tw);
-  }
-}
-
 // Part of a member's T x T stamp that lands in its island window: the
 // window-covered box [sx0, sx1) x [sy0, sy1) in stamp coordinates, and the
 // box [rx0, rx1) x [ry0, ry1) of raw-plane pixels the 7-tap shift reads
@@ -485,17 +370,19 @@ struct StampClip {
 };
 
 // ---------------------------------------------------------------------------
-// LDS-resident form of the planes pipeline (pad 128 with the raw planes,
-// the line buffer, the spectrum and two shifted planes all in LDS): the
-// horizontal Lanczos pass is applied to each output row of an inverse
-// transform while the row is in registers, and only the rows and columns
-// the window box needs are kept, so no raw plane is stored anywhere.
+// Planes pipeline: the horizontal Lanczos pass is applied to each output row
+// of an inverse transform while the row is in registers, and only the rows
+// and columns the window box needs are kept, so no raw plane is stored.
+// Up to pad 128 and T = 55 the PSF planes, the line buffer, the spectrum and
+// the two shifted-row buffers all live in one LDS pool; larger stamps keep
+// the line buffer (and the pad-128 spectrum) in LDS and the rest in the
+// workgroup's global scratch.
 // ---------------------------------------------------------------------------
 constexpr int kPoolFloats = 40704;  // LDS pool of the planes kernel (159 KiB)
 
 __host__ __device__ inline bool planes_in_lds(int T, int pad) {
   const int H = pad / 2 + 1;
-  return pad == 128 && 2 * (H * T + H * pad) + 5 * T * T <= kPoolFloats;
+  return pad <= 128 && 2 * (H * T + H * pad) + 5 * T * T <= kPoolFloats;
 }
 
 // Horizontal 7-tap pass of one row held across the wave (element e of lane
@@ -534,7 +421,8 @@ __device__ __forceinline__ void lanczos_row(const float (&v)[E], const float* L,
 // e^{+2 pi i m r / pad} for r < T, X the spectrum of output kind `kind`
 // (0: F + core, 1..3: F1 d(expo)/d(logre, ee1, ee2), 4: Fn + dcore).
 template <int E>
-__device__ void inverse_columns(const PlaneJob& j, int kind, const WaveTw<E>& tw) {
+__device__ __forceinline__ void inverse_columns(const PlaneJob& j, int kind,
+                                                const WaveTw<E>& tw) {
   constexpr int G = Batch<E>::G;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int T = j.T, H = j.H, pad = j.pad, logp = j.logp, n = H * pad;
@@ -589,8 +477,9 @@ __device__ void inverse_columns(const PlaneJob& j, int kind, const WaveTw<E>& tw
 // with L (and dL into Hb when Hb is given) and stored for the window-box
 // columns as Ha[(r - ry0) * SW + c - sx0], SW = sx1 - sx0.
 template <int E>
-__device__ void inverse_rows_hshift(const PlaneJob& j, const StampClip& cl, const float* L,
-                                    const float* dL, float* Ha, float* Hb, const WaveTw<E>& tw) {
+__device__ __forceinline__ void inverse_rows_hshift(const PlaneJob& j, const StampClip& cl,
+                                                    const float* L, const float* dL, float* Ha,
+                                                    float* Hb, const WaveTw<E>& tw) {
   constexpr int G = Batch<E>::G;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int T = j.T, pad = j.pad, logp = j.logp, H = j.H;
@@ -663,14 +552,21 @@ __device__ __forceinline__ void vshift_plane(const float* Hs, const StampClip& c
   }
 }
 
-// All planes of an extended member with the raw planes P3 and the shifted
-// buffers Ha, Hb in LDS: plane 0 = Ly (Lx R0); plane 1 = Ly (dLx R0) +
-// Ly (Lx R1); plane 2 = dLy (Lx R0) + Ly (Lx R2); planes 3..5 = Ly (Lx R);
-// plane 6 = dn/du Ly (Lx R6), R the raw planes of section render_planes.
-// `out` holds the member's 7 planes of side x side floats for this band.
-template <int E>
-__device__ void render_planes_lds(const PlaneJob& j, const StampClip& cl, const float* taps,
-                                  float dndu, float* Ha, float* Hb, float* out, int side) {
+// All planes of an extended member for one band.  The raw planes are R0 =
+// irfft2(S_P (F + core)) (the model), R1, R2 = irfft2(S_dP/dx (F + core)),
+// irfft2(S_dP/dy (F + core)) (position), R3..R5 = irfft2(S_P F1 d expo /
+// d(logre, ee1, ee2)) and R6 = irfft2(S_P (Fn + dcore)) (Sersic n), S_X the
+// spectrum of the PSF plane X (P3 holds P, dP/dx, dP/dy).  The shifted
+// planes: 0 = Ly (Lx R0); 1 = Ly (dLx R0) + Ly (Lx R1); 2 = dLy (Lx R0) +
+// Ly (Lx R2); 3..5 = Ly (Lx R); 6 = dn/du Ly (Lx R6), each over the window
+// box only; `out` holds the member's 7 planes of side x side floats for the
+// band.  Ha, Hb hold horizontally shifted rows (in LDS when kLds, else in
+// global scratch); each (E, kLds) form has its own call site so that it is
+// inlined with its pointers' address spaces known.
+template <int E, bool kLds>
+__device__ __forceinline__ void render_planes_fused(const PlaneJob& j, const StampClip& cl,
+                                                    const float* taps, float dndu, float* Ha,
+                                                    float* Hb, float* out, int side) {
   WaveTw<E> tw;
   wave_tw<E>(tw);
   const int TT = j.T * j.T, PP = side * side, need = j.need;
@@ -914,39 +810,6 @@ __global__ __launch_bounds__(kPreBlock) void prerender_planes_kernel(IslandArgsP
     if (tid < 28) taps[tid] = a.gtap[(size_t)g * 28 + tid];
     const int pad = fft_pad_for(T), H = pad / 2 + 1;
     const float* Cf = a.tier_coefs[a.shape[4 * (size_t)s_it.isl + 2]] + (size_t)b * nt * T * T;
-    if (planes_in_lds(T, pad)) {
-      // LDS pool: line buffer (H, T), spectrum (H, pad), Ha, Hb (T^2 each), P3
-      float* Ha = reinterpret_cast<float*>(lds + H * T + H * pad);
-      float* Hb = Ha + T * T;
-      float* P3 = Hb + T * T;
-      psf_box<kPreBlock>(Cf, nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
-      __syncthreads();
-      PlaneJob j;
-      j.g = g;
-      j.b = b;
-      j.T = T;
-      j.pad = pad;
-      j.logp = ilog2(pad);
-      j.H = H;
-      j.need = need;
-      j.coefs = nullptr;
-      j.p6 = psrc + slot * 6;
-      j.out = nullptr;
-      j.P3 = P3;
-      j.A = lds;
-      j.S = lds + H * T;
-      j.F = a.fspec + a.pre_spec[g];
-      j.dt = a.gdt + (size_t)g * 9;
-      const int side = s_it.side;
-      render_planes_lds<2>(j, s_it.clip, taps, s_dndu, Ha, Hb,
-                           a.pbuf + a.pre_off[g] + (size_t)b * 7 * side * side, side);
-      continue;
-    }
-    const PreScratch L = pre_scratch(T);
-    float* P3 = base + L.P3;
-    float* R = base + L.R;
-    psf_box<kPreBlock>(Cf, nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
-    __syncthreads();
     PlaneJob j;
     j.g = g;
     j.b = b;
@@ -955,29 +818,42 @@ __global__ __launch_bounds__(kPreBlock) void prerender_planes_kernel(IslandArgsP
     j.logp = ilog2(pad);
     j.H = H;
     j.need = need;
-    j.coefs = nullptr;
     j.p6 = psrc + slot * 6;
-    j.out = R;
-    j.P3 = P3;
     j.A = lds;
-    j.S = pad <= 128 ? lds + H * T : reinterpret_cast<float2*>(base + L.S);
     j.F = a.fspec + a.pre_spec[g];
     j.dt = a.gdt + (size_t)g * 9;
-    if (pad == 64)
-      render_planes<1>(j);
-    else if (pad == 128)
-      render_planes<2>(j);
-    else
-      render_planes<4>(j);
-    // the item fields are read from LDS again here, not held across the
-    // transforms
-    {
-      const int g2 = s_it.g, T2 = s_it.T;
-      const PreScratch L2 = pre_scratch(T2);
-      const int side = s_it.side;
-      shift_planes<kPreBlock>(base + L2.R, base + L2.Hh,
-                              a.pbuf + a.pre_off[g2] + (size_t)s_it.b * 7 * side * side, T2,
-                              side, s_it.need, taps, s_dndu, s_it.clip);
+    const int side = s_it.side;
+    float* out = a.pbuf + a.pre_off[g] + (size_t)b * 7 * side * side;
+    if (planes_in_lds(T, pad)) {
+      // LDS pool: line buffer (H, T), spectrum (H, pad), Ha, Hb (T^2 each), P3
+      float* Ha = reinterpret_cast<float*>(lds + H * T + H * pad);
+      float* Hb = Ha + T * T;
+      float* P3 = Hb + T * T;
+      psf_box<kPreBlock>(Cf, nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
+      __syncthreads();
+      j.P3 = P3;
+      j.S = lds + H * T;
+      if (pad == 64)
+        render_planes_fused<1, true>(j, s_it.clip, taps, s_dndu, Ha, Hb, out, side);
+      else
+        render_planes_fused<2, true>(j, s_it.clip, taps, s_dndu, Ha, Hb, out, side);
+    } else {
+      // line buffer (and the spectrum up to pad 128) in LDS, the rest in the
+      // workgroup's global scratch
+      const PreScratch L = pre_scratch(T);
+      float* P3 = base + L.P3;
+      float* Ha = base + L.H2;
+      float* Hb = Ha + T * T;
+      psf_box<kPreBlock>(Cf, nt, T, bas, s_xs, s_ys, need & 3, 0, 0, T, T, P3);
+      __syncthreads();
+      j.P3 = P3;
+      if (pad <= 128) {
+        j.S = lds + H * T;
+        render_planes_fused<2, false>(j, s_it.clip, taps, s_dndu, Ha, Hb, out, side);
+      } else {
+        j.S = reinterpret_cast<float2*>(base + L.S);
+        render_planes_fused<4, false>(j, s_it.clip, taps, s_dndu, Ha, Hb, out, side);
+      }
     }
   }
 }
@@ -1064,6 +940,162 @@ void prerender_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hi
   const int g_star = (int)(n_star < cap ? n_star : cap);
   if (g_star > 0)
     hipLaunchKernelGGL(prerender_stars_kernel, dim3(g_star), dim3(kStarBlock), 0, s, p);
+}
+
+// ---------------------------------------------------------------------------
+// Unit stamps of extended sources for the linear solve (source_model.py:
+// 264-328, fft backend; never renormalised): one (source, band) item per
+// workgroup visit through the planes pipeline with the model plane only --
+// the PSF at the source position, its forward transform, the product with
+// the mixture spectrum F + core (evaluated per item on the (H, pad) grid:
+// float64 exponents, float32 exponentials), the inverse transform with the
+// horizontal Lanczos pass fused into its rows, then the vertical pass into
+// the T x T stamp.  Stars are left to the star launch.  kLds: everything in
+// one LDS pool (pad <= 128 up to T = 55); otherwise the line buffer in LDS
+// and the spectrum, mixture, PSF plane and shifted rows in the workgroup's
+// global scratch.
+// ---------------------------------------------------------------------------
+constexpr int kStampPoolFloats = 39424;
+
+__host__ __device__ inline bool stamps_in_lds(int T, int pad) {
+  const int H = pad / 2 + 1;
+  return pad <= 128 && 2 * (H * T + H * pad) + H * pad + 2 * T * T <= kStampPoolFloats;
+}
+
+__host__ __device__ inline size_t stamp_scratch_floats(int T) {
+  const int pad = fft_pad_for(T), H = pad / 2 + 1;
+  return round64((size_t)2 * H * pad) + round64((size_t)H * pad) + round64((size_t)T * T) +
+         (size_t)T * T;
+}
+
+template <int E, bool kLds>
+__global__ __launch_bounds__(kPreBlock) void galaxy_stamps_kernel(
+    int n, int T, int nt, const float* __restrict__ coefs, const float* __restrict__ geo_all,
+    const double* __restrict__ sersic, const int32_t* __restrict__ src_band,
+    const float* __restrict__ xs_, const float* __restrict__ ys_, const float* __restrict__ gx0,
+    const float* __restrict__ gy0, const int32_t* __restrict__ stype,
+    const float* __restrict__ logre, const float* __restrict__ ee1,
+    const float* __restrict__ ee2, const float* __restrict__ sersic_n, float* __restrict__ out,
+    char* scratch, size_t per_block) {
+  __shared__ float2 lds[kStampPoolFloats / 2];
+  __shared__ float bas[21];
+  __shared__ float taps[28];
+  __shared__ float s_xs, s_ys;
+  __shared__ GalParams<float> gp;
+  __shared__ StampClip clip;
+  const int tid = threadIdx.x, TT = T * T;
+  const int pad = fft_pad_for(T), H = pad / 2 + 1, np = H * pad;
+  WaveTw<E> tw;
+  wave_tw<E>(tw);
+  if (tid == 0) {
+    clip.sx0 = clip.sy0 = clip.rx0 = clip.ry0 = 0;
+    clip.sx1 = clip.sy1 = clip.rx1 = clip.ry1 = T;
+  }
+  float2* S;
+  float *F, *P, *Ha;
+  if constexpr (kLds) {
+    S = lds + H * T;
+    F = reinterpret_cast<float*>(S + np);
+    P = F + np;
+    Ha = P + TT;
+  } else {
+    float* base = reinterpret_cast<float*>(scratch + (size_t)blockIdx.x * per_block);
+    S = reinterpret_cast<float2*>(base);
+    F = base + round64((size_t)2 * np);
+    P = F + round64((size_t)np);
+    Ha = P + round64((size_t)TT);
+  }
+  PlaneJob j;
+  j.g = 0;
+  j.b = 0;
+  j.T = T;
+  j.pad = pad;
+  j.logp = ilog2(pad);
+  j.H = H;
+  j.need = 0;
+  j.p6 = nullptr;
+  j.P3 = P;
+  j.S = S;
+  j.A = lds;
+  j.F = F;
+  j.dt = nullptr;
+  const double c2 = -2.0 * 9.869604401089358;  // -2 pi^2
+  for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    if (stype[s] == 0) continue;  // uniform per workgroup
+    const int b = src_band[s];
+    if (tid == 0) {
+      gp.n = double(sersic_n[s]);
+      gp.dndu = 0.0;
+      gp.K = int(sersic[3]);
+      mog_lookup(sersic, gp.n, gp.a, gp.v, gp.core, gp.da, gp.dv, gp.dcore);
+      ellipse_terms(double(logre[s]), double(ee1[s]), double(ee2[s]), gp.t, gp.dt);
+    }
+    if (tid == 64 || tid == 128) {
+      const float d = tid == 64 ? xs_[s] - gx0[s] : ys_[s] - gy0[s];
+      float* t = taps + (tid == 64 ? 0 : 14);
+      lanczos_taps(d - rintf(d), t, t + 7);
+    }
+    if (tid == 192) {
+      const float* geo = geo_all + 4 * b;
+      const float X = (xs_[s] - geo[0]) / geo[1], Y = (ys_[s] - geo[2]) / geo[3];
+      psf_basis(X, Y, nt, bas, (float*)nullptr, (float*)nullptr);
+      s_xs = geo[1];
+      s_ys = geo[3];
+    }
+    __syncthreads();
+    // mixture spectrum with the core folded in
+    for (int e = tid; e < np; e += kPreBlock) {
+      const int k = e / pad, m = e % pad;
+      const double v = double(k) / pad;
+      const double w = double(m < pad / 2 ? m : m - pad) / pad;
+      const double ex = c2 * (gp.t[0] * v * v + gp.t[1] * w * w + 2.0 * gp.t[2] * v * w);
+      float f = 0.0f;
+      for (int q = 0; q < gp.K; ++q) f = fmaf(float(gp.a[q]), expf(float(gp.v[q] * ex)), f);
+      F[e] = f + float(gp.core);
+    }
+    psf_box<kPreBlock>(coefs + (size_t)b * nt * TT, nt, T, bas, s_xs, s_ys, false, 0, 0, T, T,
+                       P);
+    __syncthreads();
+    forward_plane<E>(j, P, tw);
+    inverse_columns<E>(j, 0, tw);
+    inverse_rows_hshift<E>(j, clip, taps, taps + 7, Ha, nullptr, tw);
+    vshift_plane<kPreBlock>(Ha, clip, taps + 14, out + (size_t)s * TT, T, 1.0f, false);
+    __syncthreads();
+  }
+}
+
+bool launch_galaxy_stamps(int n, int T, int nt, const float* coefs, const float* geo,
+                          const double* sersic, const int32_t* src_band, const float* x,
+                          const float* y, const float* gx0, const float* gy0,
+                          const int32_t* stype, const float* logre, const float* ee1,
+                          const float* ee2, const float* sersic_n, float* out, char* scratch,
+                          size_t scratch_bytes, int n_cu, hipStream_t s) {
+  const int pad = fft_pad_for(T);
+  if (!galaxy_fits(T, pad)) return false;
+  const bool in_lds = stamps_in_lds(T, pad);
+  const size_t per = align256(stamp_scratch_floats(T) * sizeof(float));
+  long long nblk = 2LL * n_cu;
+  if (!in_lds) {
+    if (scratch == nullptr || scratch_bytes < per) return false;
+    const long long cap = (long long)(scratch_bytes / per);
+    nblk = nblk < cap ? nblk : cap;
+  }
+  nblk = nblk < n ? nblk : n;
+  if (nblk < 1) return true;
+#define UPH_GAL_STAMPS(E_, L_)                                                                   \
+  hipLaunchKernelGGL((galaxy_stamps_kernel<E_, L_>), dim3((int)nblk), dim3(kPreBlock), 0, s, n, \
+                     T, nt, coefs, geo, sersic, src_band, x, y, gx0, gy0, stype, logre, ee1,   \
+                     ee2, sersic_n, out, scratch, per)
+  if (in_lds && pad == 64)
+    UPH_GAL_STAMPS(1, true);
+  else if (in_lds)
+    UPH_GAL_STAMPS(2, true);
+  else if (pad == 128)
+    UPH_GAL_STAMPS(2, false);
+  else
+    UPH_GAL_STAMPS(4, false);
+#undef UPH_GAL_STAMPS
+  return true;
 }
 
 }  // namespace uph
