@@ -18,6 +18,7 @@
 #include "uph_device.hpp"
 #include "uph_galaxy.hpp"
 #include "uph_island.hpp"
+#include "uph_prerender.hpp"
 
 namespace uph {
 
@@ -633,58 +634,91 @@ __global__ __launch_bounds__(UPH_BLOCK) void blank_residual_kernel(BlankArgs a) 
   }
 }
 
-constexpr int kCorrTile = 16;
+// Output tile of the correlation launch: kCorrRows x kCorrCols pixels, one
+// row of 4 adjacent outputs per thread (kCorrBlock threads).
+constexpr int kCorrRows = 32, kCorrCols = 64, kCorrBlock = 512;
 
-// 'same' correlations of wr with phi and of w with phi^2 over one
-// kCorrTile^2 output tile per workgroup; inputs and template staged in LDS.
-__global__ __launch_bounds__(UPH_BLOCK) void blank_corr_kernel(BlankArgs a) {
-  extern __shared__ float lds[];
+__host__ __device__ inline int corr_tp(int T) { return (T + 3) / 4 * 4; }
+__host__ __device__ inline int corr_sx(int T) { return kCorrCols + corr_tp(T) + 4; }
+__host__ __device__ inline size_t corr_lds_bytes(int T) {
+  return ((size_t)2 * T * corr_tp(T) + (size_t)2 * (kCorrRows + T - 1) * corr_sx(T)) *
+         sizeof(float);
+}
+
+// 'same' correlations of wr with phi and of w with phi^2 over one output
+// tile per workgroup visit, inputs (with halo) and template staged in LDS;
+// each thread keeps 4 adjacent outputs of both correlations and reads the
+// template and the inputs 4 floats at a time, so every input value loaded
+// serves 4 outputs.  Template rows are zero-padded to a multiple of 4.
+__global__ __launch_bounds__(kCorrBlock) void blank_corr_kernel(BlankArgs a) {
+  extern __shared__ float4 lds4[];
+  float* lds = reinterpret_cast<float*>(lds4);
   const int q = blockIdx.x;
   const int band = a.win[q * 5], x0 = a.win[q * 5 + 1], y0 = a.win[q * 5 + 2];
   const int h = a.win[q * 5 + 3], w = a.win[q * 5 + 4], T = a.T, c = T / 2;
-  const int tiles_x = (w + kCorrTile - 1) / kCorrTile, tiles_y = (h + kCorrTile - 1) / kCorrTile;
-  const int S = kCorrTile + T - 1;
-  float* P1 = lds;                // phi
-  float* P2 = P1 + T * T;         // phi^2
-  float* A1 = P2 + T * T;         // wr tile with halo
-  float* A2 = A1 + S * S;         // w tile with halo
+  const int TP = corr_tp(T), SX = corr_sx(T), SY = kCorrRows + T - 1;
+  const int tiles_x = (w + kCorrCols - 1) / kCorrCols, tiles_y = (h + kCorrRows - 1) / kCorrRows;
+  float* P1 = lds;                // phi (T rows of TP)
+  float* P2 = P1 + T * TP;        // phi^2
+  float* A1 = P2 + T * TP;        // wr tile with halo (SY rows of SX)
+  float* A2 = A1 + SY * SX;       // w tile with halo
   const float* WR = a.work + (size_t)q * 3 * a.max_pix;
   const float* WW = WR + a.max_pix;
   float* RESP = a.work + (size_t)q * 3 * a.max_pix + 2 * a.max_pix;
   const float* ph = a.phi + (size_t)q * T * T;
-  for (int e = threadIdx.x; e < T * T; e += UPH_BLOCK) {
-    P1[e] = ph[e];
-    P2[e] = ph[e] * ph[e];
+  for (int e = threadIdx.x; e < T * TP; e += kCorrBlock) {
+    const int ky = e / TP, kx = e % TP;
+    const float v = kx < T ? ph[ky * T + kx] : 0.0f;
+    P1[e] = v;
+    P2[e] = v * v;
   }
   const size_t plane = (size_t)a.H * a.W, off = (size_t)band * plane;
+  const int ly = threadIdx.x / (kCorrCols / 4), lx = 4 * (threadIdx.x % (kCorrCols / 4));
   for (int tile = blockIdx.y; tile < tiles_x * tiles_y; tile += gridDim.y) {
-    const int ty0 = (tile / tiles_x) * kCorrTile, tx0 = (tile % tiles_x) * kCorrTile;
+    const int ty0 = (tile / tiles_x) * kCorrRows, tx0 = (tile % tiles_x) * kCorrCols;
     __syncthreads();
-    for (int e = threadIdx.x; e < S * S; e += UPH_BLOCK) {
-      const int yy = ty0 - c + e / S, xx = tx0 - c + e % S;
+    for (int e = threadIdx.x; e < SY * SX; e += kCorrBlock) {
+      const int yy = ty0 - c + e / SX, xx = tx0 - c + e % SX;
       const bool in = yy >= 0 && yy < h && xx >= 0 && xx < w;
       A1[e] = in ? WR[yy * w + xx] : 0.0f;
       A2[e] = in ? WW[yy * w + xx] : 0.0f;
     }
     __syncthreads();
-    const int ly = threadIdx.x / kCorrTile, lx = threadIdx.x % kCorrTile;
-    const int oy = ty0 + ly, ox = tx0 + lx;
-    if (oy < h && ox < w) {
-      float num = 0.0f, den = 0.0f;
-      for (int ky = 0; ky < T; ++ky) {
-        const float* r1 = A1 + (ly + ky) * S + lx;
-        const float* r2 = A2 + (ly + ky) * S + lx;
-        const float* k1 = P1 + ky * T;
-        const float* k2 = P2 + ky * T;
-        for (int kx = 0; kx < T; ++kx) {
-          num = fmaf(r1[kx], k1[kx], num);
-          den = fmaf(r2[kx], k2[kx], den);
+    float num[4] = {0.0f, 0.0f, 0.0f, 0.0f}, den[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int ky = 0; ky < T; ++ky) {
+      const float4* r1 = reinterpret_cast<const float4*>(A1 + (ly + ky) * SX + lx);
+      const float4* r2 = reinterpret_cast<const float4*>(A2 + (ly + ky) * SX + lx);
+      const float4* k1 = reinterpret_cast<const float4*>(P1 + ky * TP);
+      const float4* k2 = reinterpret_cast<const float4*>(P2 + ky * TP);
+      float4 u1 = r1[0], u2 = r2[0];
+      for (int k4 = 0; k4 < TP / 4; ++k4) {
+        const float4 v1 = r1[k4 + 1], v2 = r2[k4 + 1];
+        const float4 t1 = k1[k4], t2 = k2[k4];
+        const float x1[8] = {u1.x, u1.y, u1.z, u1.w, v1.x, v1.y, v1.z, v1.w};
+        const float x2[8] = {u2.x, u2.y, u2.z, u2.w, v2.x, v2.y, v2.z, v2.w};
+        const float c1[4] = {t1.x, t1.y, t1.z, t1.w};
+        const float c2[4] = {t2.x, t2.y, t2.z, t2.w};
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            num[j] = fmaf(x1[j + t], c1[t], num[j]);
+            den[j] = fmaf(x2[j + t], c2[t], den[j]);
+          }
         }
+        u1 = v1;
+        u2 = v2;
       }
+    }
+    const int oy = ty0 + ly;
+    const float nanv = __int_as_float(0x7fc00000);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int ox = tx0 + lx + j;
+      if (oy >= h || ox >= w) continue;
       const size_t gi = off + (size_t)(y0 + oy) * a.W + x0 + ox;
       const bool gd = a.good[gi] && a.invvar[gi] > 0.0f;
-      const float nanv = __int_as_float(0x7fc00000);
-      RESP[oy * w + ox] = (gd && den > 0.0f) ? num / fmaxf(den, 1e-30f) : nanv;
+      RESP[oy * w + ox] = (gd && den[j] > 0.0f) ? num[j] / fmaxf(den[j], 1e-30f) : nanv;
     }
   }
 }
@@ -805,6 +839,18 @@ extern "C" int uph_render_stamps(int n, int T, int n_terms, const float* psf_coe
                        out);
     int st = uph::hip_status(hipGetLastError(), "uph_render_stamps launch");
     if (st != UPH_OK || !ext) return st;
+  }
+  if (stars_lds) {
+    // extended sources through the planes pipeline where it serves T
+    int n_cu = 0;
+    st0 = uph::hip_status(
+        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev),
+        "uph_render_stamps attribute");
+    if (st0 != UPH_OK) return st0;
+    if (uph::launch_galaxy_stamps(n, T, n_terms, psf_coefs, psf_geo, sersic, src_band, x, y,
+                                  gx0, gy0, stype, logre, ee1, ee2, sersic_n, out,
+                                  static_cast<char*>(scratch), scratch_bytes, n_cu, s))
+      return uph::hip_status(hipGetLastError(), "uph_render_stamps galaxy launch");
   }
   const size_t per = uph::galaxy_scratch_bytes(T, sizeof(float));
   const size_t nb_s = scratch_bytes / per;
@@ -991,10 +1037,9 @@ extern "C" int uph_blank_response(int n_prob, int T, int bg_order, const int64_t
     hipLaunchKernelGGL(uph::blank_residual_kernel<3>, dim3(n_prob, 16), dim3(UPH_BLOCK), 0, s, a);
   else
     hipLaunchKernelGGL(uph::blank_residual_kernel<6>, dim3(n_prob, 16), dim3(UPH_BLOCK), 0, s, a);
-  const int S = uph::kCorrTile + T - 1;
-  const size_t lds = (size_t)(2 * T * T + 2 * S * S) * sizeof(float);
-  UPH_CHECK_ARG(lds <= 65536, "uph_blank_response: template T=%d too large", T);
-  hipLaunchKernelGGL(uph::blank_corr_kernel, dim3(n_prob, 32), dim3(UPH_BLOCK), lds, s, a);
+  const size_t lds = uph::corr_lds_bytes(T);
+  UPH_CHECK_ARG(lds <= 160 * 1024, "uph_blank_response: template T=%d too large", T);
+  hipLaunchKernelGGL(uph::blank_corr_kernel, dim3(n_prob, 16), dim3(uph::kCorrBlock), lds, s, a);
   hipLaunchKernelGGL(uph::blank_median_kernel, dim3(n_prob), dim3(UPH_BLOCK), 0, s, a);
   return uph::hip_status(hipGetLastError(), "uph_blank_response launch");
 }

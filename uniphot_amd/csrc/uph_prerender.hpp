@@ -24,4 +24,16 @@ size_t prerender_spectra_floats(int T);
 void prerender_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hipStream_t s,
                     hipEvent_t ev_spec, hipEvent_t ev_ext);
 
+// Unit stamps of the extended sources among n (source, band) items for the
+// linear solve through the planes pipeline (stars are skipped).  Returns
+// false, launching nothing, when stamp side T is not served (its line
+// buffer does not fit LDS) or the scratch is too small for one workgroup;
+// the caller then renders them another way.  n_cu sizes the grid.
+bool launch_galaxy_stamps(int n, int T, int nt, const float* coefs, const float* geo,
+                          const double* sersic, const int32_t* src_band, const float* x,
+                          const float* y, const float* gx0, const float* gy0,
+                          const int32_t* stype, const float* logre, const float* ee1,
+                          const float* ee2, const float* sersic_n, float* out, char* scratch,
+                          size_t scratch_bytes, int n_cu, hipStream_t s);
+
 }  // namespace uph
