@@ -22,9 +22,11 @@ from .island_plan import (DEFAULT_CHI2_FTOL_REL, DEFAULT_MAX_STEPS, DEFAULT_POS_
                           DEFAULT_STEP_TOL_PIX, DEFAULT_UN_STEP_MAX)
 
 # Cap on the persistent workgroups of one launch (4 per CU on 256 CUs) and
-# on the workspace they may hold.
+# on the workspace they may hold.  The per-workgroup workspace is sized for
+# the largest island of the batch (about 16 MB for 32 members in a 128^2
+# window), so the cap is set to keep 4 evaluation workgroups per CU there.
 MAX_BLOCKS = 1024
-MAX_SCRATCH_BYTES = 4 << 30
+MAX_SCRATCH_BYTES = 16 << 30
 
 
 @dataclass

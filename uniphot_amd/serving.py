@@ -20,6 +20,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
+from .island_solver import MAX_SCRATCH_BYTES
 from .pipeline import DeviceField, FitConfig, fit_field
 from .psf_polynomial import psf_device_arrays
 
@@ -60,15 +61,18 @@ class TileServer:
 
     def footprint(self, req) -> float:
         """Estimated device bytes while the tile is in flight: images and
-        padded stacks, grid-band linear-solve buffers, island work buffers
-        and, with scene subtraction, the island cutout mosaic (at most one
-        64 x 64 window of data and weights per source and band)."""
+        padded stacks, grid-band linear-solve buffers, island work buffers,
+        the island workspace and, with scene subtraction, the island cutout
+        mosaic (at most one 64 x 64 window of data and weights per source and
+        band)."""
         px = float(req.n_bands) * req.side * req.side
         stacks = px * 4.0 * 2 * 1.6
         linear = 4.0 * req.n_sources * req.n_bands * 111.0 ** 2
         island = 2e6 * req.n_sources * max(req.n_bands, 1) / 20.0 + (2 << 30)
         mosaic = 8.0 * req.n_bands * req.n_sources * 64.0 ** 2 if self.cfg.scene_subtract else 0.0
-        return px * 9.0 + stacks + linear + island + mosaic
+        # island workspace: at most 1024 workgroups of up to ~16 MB each
+        work = float(min(MAX_SCRATCH_BYTES, 16e6 * max(1, req.n_sources * req.n_bands)))
+        return px * 9.0 + stacks + linear + island + mosaic + work
 
     def _psf_dev(self, req):
         import torch
