@@ -22,11 +22,12 @@ from .island_plan import (DEFAULT_CHI2_FTOL_REL, DEFAULT_MAX_STEPS, DEFAULT_POS_
                           DEFAULT_STEP_TOL_PIX, DEFAULT_UN_STEP_MAX)
 
 # Cap on the persistent workgroups of one launch (4 per CU on 256 CUs) and
-# on the workspace they may hold.  The per-workgroup workspace is sized for
-# the largest island of the batch (about 16 MB for 32 members in a 128^2
-# window), so the cap is set to keep 4 evaluation workgroups per CU there.
+# the default cap on the workspace they may hold.  The per-workgroup
+# workspace is sized for the largest island of the batch (about 34 MB for
+# 32 members in a 192^2 window), so the default keeps 4 evaluation
+# workgroups per CU there; a caller sharing the device passes less.
 MAX_BLOCKS = 1024
-MAX_SCRATCH_BYTES = 16 << 30
+MAX_SCRATCH_BYTES = 40 << 30
 
 
 @dataclass
@@ -253,7 +254,8 @@ def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prer
 def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                          precision: str = "fp32", sersic_dev=None, timer=None,
                          split: bool = False, prerender: bool = True,
-                         post: bool = False, gram: str = "fp32") -> "PreparedIslandBatch":
+                         post: bool = False, gram: str = "fp32",
+                         workspace_bytes: float = MAX_SCRATCH_BYTES) -> "PreparedIslandBatch":
     """Stage the islands of several classes for ONE launch sequence.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
@@ -364,7 +366,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         nrej=torch.empty(n_isl, dtype=torch.int32, device=dev),
         n_good=torch.empty(n_isl, dtype=torch.int32, device=dev))
     n_work = n_isl * nb if split else n_isl
-    nblk = max(1, min(n_work, MAX_BLOCKS, MAX_SCRATCH_BYTES // per))
+    nblk = max(1, min(n_work, MAX_BLOCKS, int(workspace_bytes) // per))
     scratch = torch.empty(per * nblk, dtype=torch.uint8, device=dev)
     counter = torch.zeros(1, dtype=torch.int32, device=dev)
     P = _lib.ptr
@@ -587,13 +589,14 @@ class PreparedIslandBatch:
 def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                      precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
                      split: bool = False, prerender: bool = True, post: bool = False,
-                     gram: str = "fp32"):
+                     gram: str = "fp32", workspace_bytes: float = MAX_SCRATCH_BYTES):
     """Fit the islands of several classes in one launch sequence (see
     :func:`prepare_island_batch`) and return one result dict per class, in
     the same layout as :func:`run_class_chunk`."""
     pb = prepare_island_batch(classes, psf_trees, max_steps, dstack_dev, wstack_dev,
                               precision=precision, sersic_dev=sersic_dev, timer=timer,
-                              split=split, prerender=prerender, post=post, gram=gram)
+                              split=split, prerender=prerender, post=post, gram=gram,
+                              workspace_bytes=workspace_bytes)
     ev = timer.begin("island_lm_kernel") if timer is not None else None
     pb.launch(stream)
     if ev is not None:

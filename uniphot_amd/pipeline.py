@@ -62,6 +62,7 @@ class FitConfig:
     post_fit: bool = True        # parameter variances, marginal flux errors, source metrics
     gram: str = "fp32"           # island Gram operands: "fp32" or "bf16" (matrix cores)
     stamp_budget: float = 24e9   # bytes of rendered stamps per linear-solve launch
+    island_workspace: float = 40 * 2 ** 30  # bytes of island-fit workgroup workspace (cap)
 
 
 @dataclass
@@ -612,7 +613,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         results = run_island_batch(chunk, trees, cfg.max_steps, ds, ws,
                                    precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
                                    split=cfg.island_split, prerender=cfg.galaxy_prerender,
-                                   post=cfg.post_fit, gram=cfg.gram)
+                                   post=cfg.post_fit, gram=cfg.gram,
+                                   workspace_bytes=cfg.island_workspace)
         redo = [(kern, host, rows[res["gram_unsettled"]])
                 for (kern, host, rows), res in zip(chunk, results) if "gram_unsettled" in res
                 and res["gram_unsettled"].any()]
@@ -622,7 +624,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             again = run_island_batch(redo, trees, cfg.max_steps, ds, ws,
                                      precision=cfg.precision, sersic_dev=sersic_dev,
                                      split=cfg.island_split, prerender=cfg.galaxy_prerender,
-                                     post=cfg.post_fit, gram="fp32")
+                                     post=cfg.post_fit, gram="fp32",
+                                     workspace_bytes=cfg.island_workspace)
             pos = {id(h): j for j, (_, h, _) in enumerate(redo)}
             merged = []
             for (kern, host, rows), res in zip(chunk, results):

@@ -20,7 +20,6 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .island_solver import MAX_SCRATCH_BYTES
 from .pipeline import DeviceField, FitConfig, fit_field
 from .psf_polynomial import psf_device_arrays
 
@@ -49,7 +48,8 @@ class TileServer:
         self.hbm_budget = float(hbm_budget)
         self.max_inflight = int(max_inflight)
         self.device = device
-        self.cfg = cfg or FitConfig()
+        # tiles in flight share the device: a smaller island workspace each
+        self.cfg = cfg or FitConfig(island_workspace=16 * 2 ** 30)
         # resident PSF coefficients of every bank band at every stamp tier
         # (the frame-dependent geometry is per tile)
         self._coefs = {}
@@ -71,7 +71,7 @@ class TileServer:
         island = 2e6 * req.n_sources * max(req.n_bands, 1) / 20.0 + (2 << 30)
         mosaic = 8.0 * req.n_bands * req.n_sources * 64.0 ** 2 if self.cfg.scene_subtract else 0.0
         # island workspace: at most 1024 workgroups of up to ~16 MB each
-        work = float(min(MAX_SCRATCH_BYTES, 16e6 * max(1, req.n_sources * req.n_bands)))
+        work = float(min(self.cfg.island_workspace, 34e6 * max(1, req.n_sources * req.n_bands)))
         return px * 9.0 + stacks + linear + island + mosaic + work
 
     def _psf_dev(self, req):
