@@ -186,13 +186,14 @@ typedef struct uph_island_split {
   size_t hg_doubles;
   float* p_try;
   int32_t* counters;
-  void* const* eval_events; /* NULL, or 5 * (2 * max_steps + 2 + post)
+  void* const* eval_events; /* NULL, or 6 * (2 * max_steps + 2 + post)
                                hipEvent_t: per evaluation pass (chi2 at p0,
                                then H/g and trial per step, then final
                                fluxes, then the post-fit pass) one at its
                                start, one after the pre-render taps/spectra
                                launch, one after the extended-member planes,
-                               one after the star planes (before the
+                               one after the star planes, one after the
+                               large-island Gram launch (before the
                                (island, band) evaluation) and one at its end */
   /* member pre-render (float32 only; n_pre = 0 renders members in place):
      before each pass every real member of the active islands is rendered
@@ -249,6 +250,20 @@ typedef struct uph_island_split {
      near 1); the caller refits those islands with gram_bf16 = 0. */
   int gram_bf16;
   int32_t* gram_unsettled;
+  /* Gram of the large islands on many workgroups (float32 Gram only;
+     big_n = 0 turns it off): islands with big_slot[isl] = s >= 0 (s <
+     big_n) get their (island, band) Gram computed before each evaluation,
+     one 16 x 16 column tile pair per wave, into big_part + ((s * n_bands +
+     band) * big_ntp + pair) * 256 doubles (big_ntp >= the island's tile
+     pairs t (t + 1) / 2, t = ceil((1 + N + nLc) / 16)); big_isl (big_n)
+     names the island of each slot.  The evaluation then reads the Gram
+     instead of forming it in one workgroup; it is the same Gram, summed in
+     the same order. */
+  int big_n;
+  int big_ntp;
+  const int32_t* big_slot;
+  const int32_t* big_isl;
+  double* big_part;
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);

@@ -141,3 +141,37 @@ def test_galaxy_prerender_matches_oracle(cuda, tier):
                                                                           e[:, mm])
             assert rel.max() < 1e-5
     assert n_gal > 0
+
+
+def test_large_island_gram_launch_matches_in_kernel_gram(cuda):
+    """Every island's Gram taken by the many-workgroup Gram launch
+    (threshold 0) against the evaluation's own Gram on a mixed scene, with
+    the post-fit pass: same LM step counts and flags, parameters and fluxes
+    at float32 rounding (the two sum the same products in the same order,
+    except islands of <= 16 columns, which the in-kernel Gram splits over
+    pixels)."""
+    import torch
+    from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
+    from uniphot_amd.sersic_table import device_table
+    sc = small_mixed_scene(seed=21, n_src=24)
+    plans, hosts, pad, d, w = mixed_classes(sc, free_n=(sc.stype == 3))
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms), h, np.arange(p.n_islands))
+             for p, h in zip(plans, hosts)]
+    trees = {T: psf_tree_device(sc.psfs, T, cuda) for T in {p.tier for p in plans}}
+    args = (batch, trees, 30, torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda))
+    kw = dict(sersic_dev=device_table(cuda), split=True, prerender=True, post=True)
+    print("gram launch classes", [(p.kind, p.bbox_class, p.nsrc_class, p.n_islands)
+                                  for p in plans], flush=True)
+    ref = run_island_batch(*args, gram_split_min=float("inf"), **kw)
+    got = run_island_batch(*args, gram_split_min=0.0, **kw)
+    for (plan, host), ra, rb in zip(zip(plans, hosts), ref, got):
+        for k in ("nst", "conv", "stall", "n_good"):
+            assert np.array_equal(ra[k], rb[k]), k
+        for r in range(plan.n_islands):
+            mm = host["mmask"][r]
+            assert np.abs(ra["p"][r][mm] - rb["p"][r][mm]).max() < 1e-5
+            fa, fb = ra["fluxes"][r][:, mm], rb["fluxes"][r][:, mm]
+            scale = np.maximum(np.abs(fa), ra["ferr"][r][:, mm])
+            assert (np.abs(fa - fb) / scale).max() < 1e-5
+            va, vb = ra["post_var"][r][mm], rb["post_var"][r][mm]
+            assert np.allclose(va, vb, rtol=1e-4, atol=1e-12, equal_nan=True)

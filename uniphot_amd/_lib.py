@@ -81,7 +81,8 @@ class IslandSplit(C.Structure):
                 ("fspec_elems", C.c_size_t), ("gdt", _P), ("gtap", _P), ("post", C.c_int),
                 ("post_off", _P), ("post_buf", _P), ("post_doubles", C.c_size_t),
                 ("post_var", _P), ("post_dvar", _P), ("post_metrics", _P),
-                ("gram_bf16", C.c_int), ("gram_unsettled", _P)]
+                ("gram_bf16", C.c_int), ("gram_unsettled", _P), ("big_n", C.c_int),
+                ("big_ntp", C.c_int), ("big_slot", _P), ("big_isl", _P), ("big_part", _P)]
 
 
 _lib = None

@@ -247,6 +247,25 @@ __device__ void gram_mfma(const float* __restrict__ W, const float* __restrict__
   __syncthreads();
 }
 
+// The Gram out (C x C) from its 16 x 16 tile pairs (ti <= tj, pair_of
+// order), each computed over all pixels as one unit of gram_mfma.
+__device__ void gram_from_pairs(const double* __restrict__ pairs, int C,
+                                double* __restrict__ out) {
+  const int nt = (C + 15) >> 4, ntp = nt * (nt + 1) / 2;
+  for (int e = threadIdx.x; e < ntp * 256; e += UPH_BLOCK) {
+    const int tp = e >> 8, i = (e >> 4) & 15, j = e & 15;
+    int ti, tj;
+    pair_of(tp, nt, ti, tj);
+    if (ti == tj && i > j) continue;
+    const int a = ti * 16 + i, b = tj * 16 + j;
+    if (a >= C || b >= C) continue;
+    const double v = pairs[e];
+    out[a * C + b] = v;
+    out[b * C + a] = v;
+  }
+  __syncthreads();
+}
+
 // The same Gram with bfloat16 inputs: one v_mfma_f32_16x16x16_bf16 takes
 // each lane's four consecutive pixels (lane l: plane l % 16, pixels
 // 4 (l / 16) .. + 3 of the 16-pixel step), so a K-step is one instruction
@@ -697,8 +716,12 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
   int bf16 = 0;  // bfloat16 Gram operands (pre-render variant, on request)
   if constexpr (kPre) bf16 = static_cast<const IslandArgsPre&>(a).gram_bf16;
   if constexpr (kPre) {
+    const IslandArgsPre& ap = static_cast<const IslandArgsPre&>(a);
+    const int slot = (ap.big_n > 0 && !bf16) ? ap.big_slot[isl] : -1;
     if (bf16)
       gram_mfma_bf16(Wp, planes, P, C, 2, G, part);
+    else if (slot >= 0)
+      gram_from_pairs(ap.big_part + ((size_t)slot * a.nb + b) * ap.big_ntp * 256, C, G);
     else
       gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
   } else {
@@ -1220,6 +1243,134 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(typename EvalArg
   }
 }
 
+// Gram tile pairs of the large islands for one evaluation pass: workgroup
+// (q, g) takes island big_isl[q / nb] in band q % nb and its tile pairs
+// 4 g + wave over the columns [d, member models, live tangents] (C = 1 + Nr
+// + nL, no tangents on passes without them), each over all window pixels
+// as one unit of gram_mfma (float32 MFMA partials flushed to float64 every
+// 16 K-steps), with the columns read straight from the data stack and the
+// member planes (zero outside a member's stamp box) instead of placed
+// window planes.  Converged islands are skipped as the evaluation skips
+// them.
+constexpr int kMaxCols = 1 + kMaxMembers + kMaxLive;
+
+__global__ __launch_bounds__(UPH_BLOCK) void island_gram_kernel(IslandArgsPre a) {
+  __shared__ IslandSh<float> sh;
+  __shared__ long long m_blk[kMaxMembers];  // member's band block in pbuf
+  __shared__ int m_box[kMaxMembers][5];     // window box wx0, wx1, wy0, wy1; plane side
+  __shared__ int c_src[kMaxCols];           // -1: data; else member * 8 + plane
+  __shared__ int s_C;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int q = blockIdx.x, b = q % a.nb, isl = a.big_isl[q / a.nb];
+  const int mode = a.mode;
+  if ((mode == kSplitEvalGrad || mode == kSplitEvalTry) && a.conv_out[isl]) return;  // uniform
+  if (tid == 0) read_shape(a, sh, isl);
+  __syncthreads();
+  const float* psrc = mode == kSplitEvalInit ? a.p0 : (mode == kSplitEvalTry ? a.p_try : a.p_out);
+  describe_island(a, sh, psrc);
+  const bool grads = split_grads(mode);
+  const int Nr = sh.Nr, nL = grads ? sh.nL : 0, C = 1 + Nr + nL;
+  const int nt = (C + 15) >> 4, ntp = nt * (nt + 1) / 2;
+  if (4 * (int)blockIdx.y >= ntp || Nr == 0) return;  // uniform
+  const int B = sh.B, T = sh.T, P = B * B;
+  const float x0f = a.x0b[isl], y0f = a.y0b[isl];
+  for (int r = tid; r < Nr; r += UPH_BLOCK) {
+    const float* p6 = sh.p + 6 * sh.mem[r];
+    const int ix = int(rint(p6[0] - x0f)) - T / 2, iy = int(rint(p6[1] - y0f)) - T / 2;
+    const int side = pre_side(T, B);
+    m_box[r][0] = max(0, ix);
+    m_box[r][1] = min(B, ix + T);
+    m_box[r][2] = max(0, iy);
+    m_box[r][3] = min(B, iy + T);
+    m_box[r][4] = side;
+    const int g = a.pre_of_slot[(size_t)sh.moff + sh.mem[r]];
+    m_blk[r] = (long long)a.pre_off[g] +
+               (long long)b * pre_planes(sh.st[r] == 0) * side * side;
+  }
+  for (int c = tid; c < C; c += UPH_BLOCK) {
+    if (c == 0) {
+      c_src[c] = -1;
+    } else if (c <= Nr) {
+      c_src[c] = (c - 1) * 8;
+    } else {
+      const int l = c - 1 - Nr;
+      c_src[c] = sh.live_r[l] * 8 + 1 + sh.live_j[l];
+    }
+  }
+  __syncthreads();
+  const int tp = 4 * blockIdx.y + wid;
+  if (tp >= ntp) return;  // no barrier follows
+  int ti, tj;
+  pair_of(tp, nt, ti, tj);
+  const int col = lane & 15, grp = lane >> 4;
+  const int ca = ti * 16 + col, cb = tj * 16 + col;
+  const int sa = ca < C ? c_src[ca] : -2, sb = cb < C ? c_src[cb] : -2;
+  const int ox = a.ox[isl], oy = a.oy[isl];
+  const float* drow = a.dstack + ((size_t)b * a.Hp + oy) * a.Wp + ox;
+  const float* wrow = a.wstack + ((size_t)b * a.Hp + oy) * a.Wp + ox;
+  // the value of column source s at window pixels px .. px + 3 (one row)
+  auto column4 = [&](int s, int wr, int wc, float (&v)[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) v[t] = 0.0f;
+    if (s == -2) return;
+    if (s == -1) {
+      const float* d = drow + (size_t)wr * a.Wp + wc;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v[t] = d[t];
+      return;
+    }
+    const int r = s >> 3, pl = s & 7;
+    const int* bx = m_box[r];
+    if (wr < bx[2] || wr >= bx[3]) return;
+    const int side = bx[4];
+    const float* src = a.pbuf + m_blk[r] + (size_t)pl * side * side +
+                       (size_t)(wr - bx[2]) * side - bx[0];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = wc + t;
+      if (c >= bx[0] && c < bx[1]) v[t] = src[c];
+    }
+  };
+  const int KS = P >> 4;
+  double accd[4] = {0.0, 0.0, 0.0, 0.0};
+  uph_f4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+  int cnt = 0;
+  for (int ks = 0; ks < KS; ++ks) {
+    const int px = ks * 16 + 4 * grp;
+    const int wr = px / B, wc = px % B;
+    float xa[4], xb[4], wv[4];
+    const float* w = wrow + (size_t)wr * a.Wp + wc;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) wv[t] = w[t];
+    column4(sa, wr, wc, xa);
+    if (ti == tj) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) xb[t] = xa[t];
+    } else {
+      column4(sb, wr, wc, xb);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[0], wv[0] * xb[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[1], wv[1] * xb[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[2], wv[2] * xb[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[3], wv[3] * xb[3], acc, 0, 0, 0);
+    if (++cnt == 16) {
+      accd[0] += acc.x;
+      accd[1] += acc.y;
+      accd[2] += acc.z;
+      accd[3] += acc.w;
+      acc = uph_f4{0.0f, 0.0f, 0.0f, 0.0f};
+      cnt = 0;
+    }
+  }
+  accd[0] += acc.x;
+  accd[1] += acc.y;
+  accd[2] += acc.z;
+  accd[3] += acc.w;
+  double* pt = a.big_part + (((size_t)(q / a.nb) * a.nb + b) * a.big_ntp + tp) * 256;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) pt[(4 * grp + r) * 16 + col] = accd[r];
+}
+
 template <typename Real>
 __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
   __shared__ IslandSh<Real> sh;
@@ -1593,15 +1744,16 @@ static void split_step(uph::IslandArgs a, int mode, hipStream_t s) {
 }
 
 // Evaluation pass k of the sequence: the member pre-render (when `pre` is
-// given) then the (island, band) evaluation, on work-queue counters 2k
-// (evaluation) and 2k + 1 (extended-member planes); the caller's timing
-// events 5k (start), 5k + 1 (after the taps/spectra launch), 5k + 2 (after
-// the extended-member planes), 5k + 3 (after the star planes, before the
-// evaluation) and 5k + 4 (end) when given.
+// given), the large-island Gram launch (when it has islands), then the
+// (island, band) evaluation, on work-queue counters 2k (evaluation) and
+// 2k + 1 (extended-member planes); the caller's timing events 6k (start),
+// 6k + 1 (after the taps/spectra launch), 6k + 2 (after the extended-member
+// planes), 6k + 3 (after the star planes), 6k + 4 (after the large-island
+// Gram, before the evaluation) and 6k + 5 (end) when given.
 template <typename Real>
 static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, int mode,
                        int* counters, int nblk, hipStream_t s, void* const* ev, int& k) {
-  auto evk = [&](int q) { return ev ? static_cast<hipEvent_t>(ev[5 * k + q]) : nullptr; };
+  auto evk = [&](int q) { return ev ? static_cast<hipEvent_t>(ev[6 * k + q]) : nullptr; };
   auto rec = [&](int q) {
     if (ev) (void)hipEventRecord(evk(q), s);
   };
@@ -1613,11 +1765,18 @@ static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, 
     rec(2);
   }
   rec(3);
+  if (pre && pre->big_n > 0 && !pre->gram_bf16) {
+    uph::IslandArgsPre g = *pre;
+    g.mode = mode;
+    hipLaunchKernelGGL(uph::island_gram_kernel, dim3(g.big_n * g.nb, (g.big_ntp + 3) / 4),
+                       dim3(UPH_BLOCK), 0, s, g);
+  }
+  rec(4);
   if (pre)
     split_eval_pre(*pre, mode, counters + 2 * k, nblk, s);
   else
     split_eval<Real>(a, mode, counters + 2 * k, nblk, s);
-  rec(4);
+  rec(5);
   ++k;
 }
 
@@ -1717,6 +1876,16 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
     pre.post = sp->post;
     pre.gram_bf16 = sp->gram_bf16;
     pre.gram_unsettled = sp->gram_unsettled;
+    UPH_CHECK_ARG(sp->big_n >= 0 && (sp->big_n == 0 || (sp->big_slot && sp->big_isl &&
+                                                          sp->big_part && sp->big_ntp >= 1)),
+                  "uph_island_fit_split: big_n=%d needs big_slot, big_isl, big_part and "
+                  "big_ntp >= 1",
+                  sp->big_n);
+    pre.big_n = sp->big_n;
+    pre.big_ntp = sp->big_ntp;
+    pre.big_slot = sp->big_slot;
+    pre.big_isl = sp->big_isl;
+    pre.big_part = sp->big_part;
     UPH_CHECK_ARG(!sp->gram_bf16 || sp->gram_unsettled,
                   "uph_island_fit_split: the bfloat16 Gram needs gram_unsettled");
     if (sp->post) {

@@ -100,6 +100,12 @@ struct IslandArgsPre : IslandArgs {
   float* post_metrics;         // (member slots, n_bands, 6) per-band metric sums
   int gram_bf16;               // island Gram on bfloat16 matrix cores (float32 accumulate)
   int* gram_unsettled;         // (n_isl) set to 1 where a bfloat16 refinement did not settle
+  // large islands: Gram tile pairs computed by a separate many-workgroup launch
+  int big_n;                   // islands with a slot (0: off)
+  int big_ntp;                 // tile-pair capacity per (slot, band)
+  const int32_t* big_slot;     // (n_isl) slot or -1
+  const int32_t* big_isl;      // (big_n) island of each slot
+  double* big_part;            // (big_n, n_bands, big_ntp, 256) tile pairs
 };
 
 // Band-split phases.  Evaluation launches run one (island, band) item per

@@ -28,6 +28,11 @@ from .island_plan import (DEFAULT_CHI2_FTOL_REL, DEFAULT_MAX_STEPS, DEFAULT_POS_
 # workgroups per CU there; a caller sharing the device passes less.
 MAX_BLOCKS = 1024
 MAX_SCRATCH_BYTES = 40 << 30
+# Islands whose largest Gram (C^2 x window pixels, C = 1 + N + live
+# parameters) reaches this get it computed by the many-workgroup Gram
+# launch instead of inside their one evaluation workgroup.
+GRAM_SPLIT_MIN = 3e7
+GRAM_SPLIT_BYTES = 2e9  # cap on their Gram tile-pair buffer per launch sequence
 
 
 @dataclass
@@ -255,7 +260,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                          precision: str = "fp32", sersic_dev=None, timer=None,
                          split: bool = False, prerender: bool = True,
                          post: bool = False, gram: str = "fp32",
-                         workspace_bytes: float = MAX_SCRATCH_BYTES) -> "PreparedIslandBatch":
+                         workspace_bytes: float = MAX_SCRATCH_BYTES,
+                         gram_split_min: float = GRAM_SPLIT_MIN) -> "PreparedIslandBatch":
     """Stage the islands of several classes for ONE launch sequence.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
@@ -410,7 +416,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         ev_arr = None
         if timer is not None:
             # per-pass timing events (created by a first record)
-            evs = [torch.cuda.Event(enable_timing=True) for _ in range(5 * n_eval)]
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(6 * n_eval)]
             for e in evs:
                 e.record()
             ev_arr = (C.c_void_p * len(evs))(*[e.cuda_event for e in evs])
@@ -429,10 +435,32 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                           metrics=torch.full((m_acc * nb * 6,), float("nan"), **f32))
             post_args = (1, P(post_t["off"]), P(post_t["buf"]), int(post_t["buf"].numel()),
                          P(post_t["var"]), P(post_t["dvar"]), P(post_t["metrics"]))
+        # large islands: Gram on the many-workgroup launch (float32 Gram)
+        ncap = np.asarray([1 + s_[0] + s_[3] for s_ in shape], np.float64)
+        npx = np.asarray([s_[1] * s_[1] for s_ in shape], np.float64)
+        cost = ncap * ncap * npx
+        big = np.nonzero((cost >= gram_split_min) & (gal["args"][0] > 0) & (not bf16_on))[0]
+        # largest first, while the tile-pair buffer stays within its budget
+        big = big[np.argsort(-cost[big], kind="stable")]
+        ntp_of = ((ncap[big].astype(np.int64) + 15) // 16)
+        ntp_of = ntp_of * (ntp_of + 1) // 2
+        keep_n = np.arange(1, big.size + 1) * np.maximum.accumulate(ntp_of) * nb * 2048.0
+        big = np.sort(big[keep_n <= GRAM_SPLIT_BYTES])
+        big_args = (0, 0, None, None, None)
+        if big.size:
+            ntp_cap = int(max(((c_ + 15) // 16) * ((c_ + 15) // 16 + 1) // 2
+                              for c_ in ncap[big].astype(np.int64)))
+            slot = np.full(n_isl, -1, np.int32)
+            slot[big] = np.arange(big.size, dtype=np.int32)
+            big_t = dict(slot=up(slot, np.int32), isl=up(big.astype(np.int32), np.int32),
+                         part=torch.empty(big.size * nb * ntp_cap * 256, dtype=torch.float64,
+                                          device=dev))
+            big_args = (int(big.size), ntp_cap, P(big_t["slot"]), P(big_t["isl"]),
+                        P(big_t["part"]))
         sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
                               P(p_try), P(counters),
                               C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None,
-                              *gal["args"], *post_args, int(bf16_on), P(unsettled))
+                              *gal["args"], *post_args, int(bf16_on), P(unsettled), *big_args)
     else:
         evs = None
         post_on = False
@@ -497,7 +525,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
             for k, kind in enumerate(kinds):
                 it = (k - 1) // 2
                 sel = np.ones(n_isl, bool) if kind in ("init", "final", "post") else nst > it
-                e5 = evs[5 * k:5 * k + 5]
+                e5 = evs[6 * k:6 * k + 6]
                 grad = kind in ("grad", "post")
                 if g_isl.size:
                     gsel = sel[g_isl]
@@ -533,8 +561,20 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                                           flops=float(nb * np.sum(t_fl[st_])),
                                           n_pre=int(st_.sum())))
                 ncol = 1 + nr + (nlive if grad else 0)
+                gram_fl = nb * npix * ncol * (ncol + 1)
                 ev_fl = nb * npix * (ncol * (ncol + 1) + 4.0 * nr + (2.0 * (nr + nlive) if grad else 0))
-                timer.record("island_eval_kernel", e5[3], e5[4],
+                if big.size:
+                    # the large islands' Gram (and its data + weight reads) is
+                    # the Gram launch's, the rest of their evaluation stays
+                    bsel = np.zeros(n_isl, bool)
+                    bsel[big] = True
+                    bsel &= sel
+                    timer.record("island_gram_kernel", e5[3], e5[4],
+                                 dict(kind="island_eval", phase=kind,
+                                      bytes=float(box[bsel].sum()),
+                                      flops=float(gram_fl[bsel].sum()), n_isl=int(bsel.sum())))
+                    ev_fl = np.where(bsel, ev_fl - gram_fl, ev_fl)
+                timer.record("island_eval_kernel", e5[4], e5[5],
                              dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
                                   flops=float(ev_fl[sel].sum()), n_isl=int(sel.sum())))
         hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics")} if post_on
@@ -568,6 +608,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
             psf_trees, sersic_dev]
     if split:
         keep += [hg, p_try, counters, d_items, d_bands, d_hgoff, gal, unsettled]
+        if big.size:
+            keep.append(big_t)
         if post_on:
             keep.append(post_t)
     return PreparedIslandBatch(launch, timer_info, collect, keep)
@@ -589,14 +631,15 @@ class PreparedIslandBatch:
 def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                      precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
                      split: bool = False, prerender: bool = True, post: bool = False,
-                     gram: str = "fp32", workspace_bytes: float = MAX_SCRATCH_BYTES):
+                     gram: str = "fp32", workspace_bytes: float = MAX_SCRATCH_BYTES,
+                     gram_split_min: float = GRAM_SPLIT_MIN):
     """Fit the islands of several classes in one launch sequence (see
     :func:`prepare_island_batch`) and return one result dict per class, in
     the same layout as :func:`run_class_chunk`."""
     pb = prepare_island_batch(classes, psf_trees, max_steps, dstack_dev, wstack_dev,
                               precision=precision, sersic_dev=sersic_dev, timer=timer,
                               split=split, prerender=prerender, post=post, gram=gram,
-                              workspace_bytes=workspace_bytes)
+                              workspace_bytes=workspace_bytes, gram_split_min=gram_split_min)
     ev = timer.begin("island_lm_kernel") if timer is not None else None
     pb.launch(stream)
     if ev is not None:
