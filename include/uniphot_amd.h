@@ -353,6 +353,20 @@ int uph_stamp_sum(int n_win, int max_pix, int T, int n_st, int nbc, int band0, i
                   int W, float* out, void* stream);
 
 /*
+ * Per-grid background planes subtracted from the padded field stack
+ * (replaces the host loop `dmb[bi, core] -= basis @ bg_coefs[bi]`,
+ * island_stage.py:1731-1751): grid q's core window win[q] = (x0, y0, h, w)
+ * in image coordinates gets, in every band, out[band][pad + y][pad + x] -=
+ * sum_m bg[(q * n_bands + band) * M + m] * X^i Y^j, M = (o + 1)(o + 2) / 2
+ * terms of order o = bg_order in X = (x - bg_xo) / bg_xs, Y = (y - bg_yo) /
+ * bg_ys (float64, rounded to float32).  out (n_bands, Hp, Wp) f32; the
+ * windows (with pad) lie inside it and are disjoint; max_pix >= h * w.
+ */
+int uph_subtract_bg_planes(int n_grid, int max_pix, int n_bands, int bg_order, const int32_t* win,
+                           const float* bg, int pad, double bg_xo, double bg_xs, double bg_yo,
+                           double bg_ys, int Hp, int Wp, float* out, void* stream);
+
+/*
  * Island window gather (replaces _gather_jit / gather, island_solver.py:
  * 1820-1830): window q = win[q] = (sx, sy, dx, dy, side) copies the
  * side x side block at (sx, sy) of every band of src_a (n_bands, Hs, Ws) to

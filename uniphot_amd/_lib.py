@@ -21,7 +21,7 @@ EXPORTS = (
     "uph_prerender_scratch_bytes", "uph_prerender_spectra_floats",
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
     "uph_wing_images", "uph_blank_work_floats", "uph_blank_response", "uph_stamp_sum",
-    "uph_gather_windows",
+    "uph_gather_windows", "uph_subtract_bg_planes",
 )
 
 
@@ -142,6 +142,9 @@ def lib() -> C.CDLL:
                                   C.c_int, C.c_int, _P, _P]
     L.uph_stamp_sum.restype = C.c_int
     L.uph_stamp_sum.argtypes = [C.c_int] * 7 + [_P] * 7 + [C.c_float, C.c_int, C.c_int, _P, _P]
+    L.uph_subtract_bg_planes.restype = C.c_int
+    L.uph_subtract_bg_planes.argtypes = [C.c_int] * 4 + [_P, _P, C.c_int] + [C.c_double] * 4 + \
+        [C.c_int, C.c_int, _P, _P]
     L.uph_gather_windows.restype = C.c_int
     L.uph_gather_windows.argtypes = [C.c_int, C.c_int, C.c_int, _P, C.c_int, C.c_int, _P, _P,
                                      C.c_int, C.c_int, _P, _P, _P]

@@ -455,7 +455,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_solve_kernel(BandArgs a) {
     A[(size_t)i * NT + i] = d + double(a.reg) * fmax(fabs(d), 1e-30);
   }
   __syncthreads();
-  block_lu(A, NT, piv, &s_piv);
+  block_lu_wpiv(A, NT, piv, &s_piv);
   // inverse columns (one per thread) -> variance diagonal; then the solve
   for (int i = tid; i < NT; i += UPH_BLOCK) {
     double* x = X + (size_t)i * NT;
@@ -542,6 +542,40 @@ __global__ __launch_bounds__(UPH_BLOCK) void stamp_sum_kernel(
     }
     float* o = img + (size_t)py * W + px;
     *o = fmaf(scale, v, *o);
+  }
+}
+
+// Per-grid background planes subtracted from the padded data stack
+// (island_stage.py:1731-1751): grid q with core window win[q] = (x0, y0, h,
+// w) in image coordinates and coefficients bg[(q * nb + band) * M + m] gets
+// out[band][pad + y0 + py][pad + x0 + px] -= sum_m bg_m * basis_m(px, py),
+// the basis X^i Y^j of the linear solve (float64 normalisation rounded to
+// float32).  Core windows are disjoint.
+template <int M>
+__global__ __launch_bounds__(UPH_BLOCK) void bg_planes_kernel(
+    const int32_t* __restrict__ win, const float* __restrict__ bg, int nb, int pad, double xo,
+    double xs, double yo, double ys, int Hp, int Wp, float* __restrict__ out) {
+  const int q = blockIdx.x, band = blockIdx.y;
+  const int x0 = win[q * 4], y0 = win[q * 4 + 1], h = win[q * 4 + 2], w = win[q * 4 + 3];
+  const float* c = bg + ((size_t)q * nb + band) * M;
+  float cf[M];
+#pragma unroll
+  for (int m = 0; m < M; ++m) cf[m] = c[m];
+  constexpr int m_[6] = {0, 1, 0, 2, 1, 0};
+  constexpr int n_[6] = {0, 0, 1, 0, 1, 2};
+  float* img = out + (size_t)band * Hp * Wp;
+  for (int e = blockIdx.z * UPH_BLOCK + threadIdx.x; e < h * w; e += gridDim.z * UPH_BLOCK) {
+    const int py = y0 + e / w, px = x0 + e % w;
+    const double X = (double(px) - xo) / xs, Y = (double(py) - yo) / ys;
+    float v = 0.0f;
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+      double t = 1.0;
+      for (int i = 0; i < m_[m]; ++i) t *= X;
+      for (int i = 0; i < n_[m]; ++i) t *= Y;
+      v = fmaf(float(t), cf[m], v);
+    }
+    img[(size_t)(pad + py) * Wp + pad + px] -= v;
   }
 }
 
@@ -963,6 +997,32 @@ extern "C" int uph_stamp_sum(int n_win, int max_pix, int T, int n_st, int nbc, i
   hipLaunchKernelGGL(uph::stamp_sum_kernel, dim3(n_win, nbc, z), dim3(UPH_BLOCK), 0, s, T, n_st,
                      band0, off, win, list, ix, iy, stamps, flux, scale, H, W, out);
   return uph::hip_status(hipGetLastError(), "uph_stamp_sum launch");
+}
+
+extern "C" int uph_subtract_bg_planes(int n_grid, int max_pix, int n_bands, int bg_order,
+                                      const int32_t* win, const float* bg, int pad, double bg_xo,
+                                      double bg_xs, double bg_yo, double bg_ys, int Hp, int Wp,
+                                      float* out, void* stream) {
+  UPH_CHECK_ARG(n_grid >= 0 && max_pix >= 0 && n_bands >= 1 && bg_order >= 0 && bg_order <= 2 &&
+                    pad >= 0 && Hp >= 1 && Wp >= 1 && bg_xs != 0.0 && bg_ys != 0.0,
+                "uph_subtract_bg_planes: n_grid=%d n_bands=%d bg_order=%d pad=%d %d x %d",
+                n_grid, n_bands, bg_order, pad, Hp, Wp);
+  if (n_grid == 0 || max_pix == 0) return UPH_OK;
+  UPH_CHECK_ARG(win && bg && out, "uph_subtract_bg_planes: null argument");
+  const int per = UPH_BLOCK * 16;
+  const int z = std::min(64, (max_pix + per - 1) / per);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const dim3 grid(n_grid, n_bands, z);
+  if (bg_order == 0)
+    hipLaunchKernelGGL(uph::bg_planes_kernel<1>, grid, dim3(UPH_BLOCK), 0, s, win, bg, n_bands,
+                       pad, bg_xo, bg_xs, bg_yo, bg_ys, Hp, Wp, out);
+  else if (bg_order == 1)
+    hipLaunchKernelGGL(uph::bg_planes_kernel<3>, grid, dim3(UPH_BLOCK), 0, s, win, bg, n_bands,
+                       pad, bg_xo, bg_xs, bg_yo, bg_ys, Hp, Wp, out);
+  else
+    hipLaunchKernelGGL(uph::bg_planes_kernel<6>, grid, dim3(UPH_BLOCK), 0, s, win, bg, n_bands,
+                       pad, bg_xo, bg_xs, bg_yo, bg_ys, Hp, Wp, out);
+  return uph::hip_status(hipGetLastError(), "uph_subtract_bg_planes launch");
 }
 
 extern "C" int uph_gather_windows(int n_win, int max_side, int n_bands, const int32_t* win,

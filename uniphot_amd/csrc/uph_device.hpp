@@ -147,6 +147,61 @@ __device__ inline void block_lu(double* A, int n, int* piv, int* s_p) {
   }
 }
 
+// block_lu with the pivot search spread over wave 0 (each lane scans rows
+// k + lane, k + lane + 64, ..., then a shuffle reduction): the same pivots
+// (largest |A[i][k]|, the smallest i among ties, the diagonal when it is
+// NaN) and the same arithmetic, without one thread walking the column.
+__device__ inline void block_lu_wpiv(double* A, int n, int* piv, int* s_p) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < n; ++k) {
+    if (tid < 64) {
+      // lane 0 starts from the diagonal as the one-thread scan does (a NaN
+      // there stays the pivot); the other lanes from "no candidate"
+      double best = tid == 0 ? fabs(A[k * n + k]) : -1.0;
+      int p = tid == 0 ? k : n;
+      for (int i = k + tid + (tid == 0 ? 64 : 0); i < n; i += 64) {
+        const double v = fabs(A[i * n + k]);
+        if (v > best) {
+          best = v;
+          p = i;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int op = __shfl_down(p, off, 64);
+        if (ob > best || (ob == best && op < p)) {
+          best = ob;
+          p = op;
+        }
+      }
+      if (tid == 0) {
+        piv[k] = p;
+        *s_p = p;
+      }
+    }
+    __syncthreads();
+    int p = *s_p;
+    if (p != k) {
+      for (int j = tid; j < n; j += UPH_BLOCK) {
+        double t = A[k * n + j];
+        A[k * n + j] = A[p * n + j];
+        A[p * n + j] = t;
+      }
+      __syncthreads();
+    }
+    double akk = A[k * n + k];
+    for (int i = k + 1 + tid; i < n; i += UPH_BLOCK) A[i * n + k] /= akk;
+    __syncthreads();
+    int m = n - k - 1;
+    for (int e = tid; e < m * m; e += UPH_BLOCK) {
+      int i = k + 1 + e / m, j = k + 1 + e % m;
+      A[i * n + j] -= A[i * n + k] * A[k * n + j];
+    }
+    __syncthreads();
+  }
+}
+
 // Solve LU x = P b for one right-hand side (sequential, one thread).
 __device__ inline void lu_solve_one(const double* LU, const int* piv, int n, double* x) {
   for (int k = 0; k < n; ++k) {

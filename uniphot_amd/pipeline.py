@@ -435,7 +435,8 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
 
 def padded_stacks(fld: DeviceField, grids, bg, bg_order, pad):
     """(dstack, wstack) (nb, H+2 pad, W+2 pad): data minus each grid's S1
-    background on its core, inverse variance; zero border."""
+    background on its core (``uph_subtract_bg_planes``), inverse variance;
+    zero border."""
     import torch
     nb, H, W = fld.nb, fld.H, fld.W
     dev = fld.data.device
@@ -445,17 +446,18 @@ def padded_stacks(fld: DeviceField, grids, bg, bg_order, pad):
     w[:, pad:pad + H, pad:pad + W] = fld.invvar
     xo, xs = 0.5 * (W - 1), max(0.5 * W, 1.0)
     yo, ys = 0.5 * (H - 1), max(0.5 * H, 1.0)
-    from .psf_polynomial import terms_for_order
-    terms = terms_for_order(bg_order)
-    for g in grids:
-        h, ww = g.y1_core - g.y0_core, g.x1_core - g.x0_core
-        X = ((np.arange(ww) + g.x0_core - xo) / xs)[None, :]
-        Y = ((np.arange(h) + g.y0_core - yo) / ys)[:, None]
-        basis = np.stack([np.broadcast_to(X ** m * Y ** n, (h, ww)).astype(np.float32)
-                          for (m, n) in terms], axis=-1)
-        plane = np.einsum("hwm,bm->bhw", basis, bg[g.grid_id]).astype(np.float32)
-        d[:, pad + g.y0_core:pad + g.y1_core, pad + g.x0_core:pad + g.x1_core] -= \
-            torch.from_numpy(plane).to(dev)
+    from . import _lib
+    if grids:
+        win = np.array([[g.x0_core, g.y0_core, g.y1_core - g.y0_core, g.x1_core - g.x0_core]
+                        for g in grids], np.int32)
+        fps.check_windows(np.concatenate([np.zeros((len(grids), 1), np.int32), win], 1),
+                          1, H, W, "padded_stacks")
+        coef = np.stack([np.asarray(bg[g.grid_id], np.float32) for g in grids])
+        t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (win, coef)]
+        max_pix = int((win[:, 2] * win[:, 3]).max())
+        _lib.check(_lib.lib().uph_subtract_bg_planes(
+            len(grids), max_pix, nb, int(bg_order), _lib.ptr(t[0]), _lib.ptr(t[1]), int(pad),
+            xo, xs, yo, ys, H + 2 * pad, W + 2 * pad, _lib.ptr(d), _lib.stream_ptr(None)))
     return d, w
 
 
