@@ -15,7 +15,7 @@ def _psf(p):
                 y_origin=p.y_origin, y_scale=p.y_scale)
 
 
-@pytest.mark.parametrize("T", [55, 111])
+@pytest.mark.parametrize("T", [55, 111, 221])
 def test_galaxy_stamps_match_oracle(cuda, T):
     import torch
     from uniphot_amd import _lib

@@ -190,3 +190,39 @@ def test_blank_response_matches_oracle(cuda):
         assert abs(b_hat[b] - ref) < 1e-3 * sig + 1e-5 * abs(ref)
         np.testing.assert_allclose(res[b].fluxes - corr[b].fluxes, np.float32(b_hat[b]),
                                    rtol=1e-6, atol=2e-7 * np.abs(res[b].fluxes).max())
+
+
+def test_background_planes_subtracted_as_the_host_formula(cuda):
+    """uph_subtract_bg_planes against the plane each grid core gets in the
+    reference's stack build (island_stage.py:1731-1751): basis X^m Y^n in
+    float64 normalised coordinates rounded to float32, times the grid's
+    coefficients, subtracted on the core only (halo and border untouched)."""
+    import torch
+    from uniphot_amd.grid import divide_image_into_grids
+    from uniphot_amd.pipeline import DeviceField, padded_stacks
+    from uniphot_amd.psf_polynomial import terms_for_order
+    sc = small_star_scene(n_src=6, H=150, W=170, n_bands=3)
+    fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
+    nb, H, W = sc.data.shape
+    grids = divide_image_into_grids(H, W, 2, 3, 10)
+    rng = np.random.default_rng(2)
+    for order in (0, 1, 2):
+        M = (order + 1) * (order + 2) // 2
+        bg = rng.normal(0, 5, (len(grids), nb, M)).astype(np.float32)
+        pad = 12
+        d, w = padded_stacks(fld, grids, bg, order, pad)
+        want = np.zeros((nb, H + 2 * pad, W + 2 * pad), np.float64)
+        want[:, pad:pad + H, pad:pad + W] = sc.data
+        xo, xs, yo, ys = 0.5 * (W - 1), 0.5 * W, 0.5 * (H - 1), 0.5 * H
+        for g in grids:
+            h, ww = g.y1_core - g.y0_core, g.x1_core - g.x0_core
+            X = ((np.arange(ww) + g.x0_core - xo) / xs)[None, :]
+            Y = ((np.arange(h) + g.y0_core - yo) / ys)[:, None]
+            basis = np.stack([np.broadcast_to(X ** m * Y ** n, (h, ww)).astype(np.float32)
+                              for (m, n) in terms_for_order(order)], axis=-1)
+            plane = np.einsum("hwm,bm->bhw", basis.astype(np.float64),
+                              bg[g.grid_id].astype(np.float64))
+            want[:, pad + g.y0_core:pad + g.y1_core, pad + g.x0_core:pad + g.x1_core] -= plane
+        got = d.cpu().numpy()
+        assert np.max(np.abs(got - want)) < 1e-5 * (np.abs(want).max() + 1.0), order
+        assert np.array_equal(w.cpu().numpy()[:, pad:pad + H, pad:pad + W], sc.invvar)

@@ -95,15 +95,16 @@ __global__ __launch_bounds__(UPH_BLOCK) void render_stamps_kernel(
 // Extended-source stamps (EXP/DEV/SERSIC): raw PSF at (x, y), analytic-FT
 // galaxy convolution, separable Lanczos-3 shift; never renormalised
 // (source_model.py:264-328 fft backend, forced_phot_sparse.py:624-637).
-// Persistent: each workgroup loops over the stamps with its own workspace.
+// Persistent: each workgroup loops over the stamps with its own workspace;
+// stars only with stars_too, extended sources unless skip_gal.
 __global__ __launch_bounds__(UPH_BLOCK) void render_galaxy_stamps_kernel(
     int n, int T, int nt, const float* __restrict__ coefs, const float* __restrict__ geo_all,
     const double* __restrict__ sersic, const int32_t* __restrict__ src_band,
     const float* __restrict__ xs_, const float* __restrict__ ys_, const float* __restrict__ gx0,
     const float* __restrict__ gy0, const int32_t* __restrict__ stype,
     const float* __restrict__ logre, const float* __restrict__ ee1,
-    const float* __restrict__ ee2, const float* __restrict__ sersic_n, int stars_too, int renorm,
-    float* __restrict__ out, char* scratch, size_t per_block) {
+    const float* __restrict__ ee2, const float* __restrict__ sersic_n, int stars_too,
+    int skip_gal, int renorm, float* __restrict__ out, char* scratch, size_t per_block) {
   __shared__ GalParams<float> gp;
   __shared__ float taps[14];
   __shared__ float bas[21];
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void render_galaxy_stamps_kernel(
   const GalLayout L = galaxy_layout(T);
   for (int s = blockIdx.x; s < n; s += gridDim.x) {
     const bool star = stype == nullptr || stype[s] == 0;
-    if (star && !stars_too) continue;  // uniform per block
+    if (star ? !stars_too : skip_gal) continue;  // uniform per block
     const int b = src_band[s];
     const float* geo = geo_all + 4 * b;
     const float x = xs_[s], y = ys_[s];
@@ -874,18 +875,19 @@ extern "C" int uph_render_stamps(int n, int T, int n_terms, const float* psf_coe
     int st = uph::hip_status(hipGetLastError(), "uph_render_stamps launch");
     if (st != UPH_OK || !ext) return st;
   }
-  if (stars_lds) {
-    // extended sources through the planes pipeline where it serves T
-    int n_cu = 0;
-    st0 = uph::hip_status(
-        hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev),
-        "uph_render_stamps attribute");
-    if (st0 != UPH_OK) return st0;
-    if (uph::launch_galaxy_stamps(n, T, n_terms, psf_coefs, psf_geo, sersic, src_band, x, y,
-                                  gx0, gy0, stype, logre, ee1, ee2, sersic_n, out,
-                                  static_cast<char*>(scratch), scratch_bytes, n_cu, s))
-      return uph::hip_status(hipGetLastError(), "uph_render_stamps galaxy launch");
-  }
+  // extended sources through the planes pipeline where it serves T
+  int n_cu = 0;
+  st0 = uph::hip_status(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev),
+                        "uph_render_stamps attribute");
+  if (st0 != UPH_OK) return st0;
+  const bool gal_done = ext && uph::launch_galaxy_stamps(
+                                        n, T, n_terms, psf_coefs, psf_geo, sersic, src_band, x,
+                                        y, gx0, gy0, stype, logre, ee1, ee2, sersic_n, out,
+                                        static_cast<char*>(scratch), scratch_bytes, n_cu, s);
+  st0 = uph::hip_status(hipGetLastError(), "uph_render_stamps galaxy launch");
+  if (st0 != UPH_OK || (gal_done && stars_lds)) return st0;
+  // the workspace kernel: the extended sources the pipeline did not take,
+  // and the stars too large for the LDS kernel
   const size_t per = uph::galaxy_scratch_bytes(T, sizeof(float));
   const size_t nb_s = scratch_bytes / per;
   UPH_CHECK_ARG(nb_s >= 1, "uph_render_stamps: scratch of %zu bytes < one workgroup (%zu)",
@@ -893,9 +895,9 @@ extern "C" int uph_render_stamps(int n, int T, int n_terms, const float* psf_coe
   const int nblk = (int)(nb_s < (size_t)n ? nb_s : (size_t)n);
   hipLaunchKernelGGL(uph::render_galaxy_stamps_kernel, dim3(nblk), dim3(UPH_BLOCK), 0, s, n, T,
                      n_terms, psf_coefs, psf_geo, sersic, src_band, x, y, gx0, gy0, stype, logre,
-                     ee1, ee2, sersic_n, stars_lds ? 0 : 1, renorm, out,
+                     ee1, ee2, sersic_n, stars_lds ? 0 : 1, gal_done ? 1 : 0, renorm, out,
                      static_cast<char*>(scratch), per);
-  return uph::hip_status(hipGetLastError(), "uph_render_stamps galaxy launch");
+  return uph::hip_status(hipGetLastError(), "uph_render_stamps workspace launch");
 }
 
 extern "C" size_t uph_band_scratch_bytes(int n_prob, int max_src, int bg_order) {

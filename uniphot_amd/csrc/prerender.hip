@@ -965,8 +965,15 @@ __host__ __device__ inline bool stamps_in_lds(int T, int pad) {
 __host__ __device__ inline size_t stamp_scratch_floats(int T) {
   const int pad = fft_pad_for(T), H = pad / 2 + 1;
   return round64((size_t)2 * H * pad) + round64((size_t)H * pad) + round64((size_t)T * T) +
-         (size_t)T * T;
+         round64((size_t)T * T) + (pad > 256 ? (size_t)2 * H * T : 0);
 }
+
+// LDS of a galaxy-stamps form: the pool, or (pad 512) nothing beyond the
+// small arrays, its line buffer then being in scratch too.
+template <int E>
+struct StampLds {
+  static constexpr int floats = E == 8 ? 2 : kStampPoolFloats;
+};
 
 template <int E, bool kLds>
 __global__ __launch_bounds__(kPreBlock) void galaxy_stamps_kernel(
@@ -977,7 +984,7 @@ __global__ __launch_bounds__(kPreBlock) void galaxy_stamps_kernel(
     const float* __restrict__ logre, const float* __restrict__ ee1,
     const float* __restrict__ ee2, const float* __restrict__ sersic_n, float* __restrict__ out,
     char* scratch, size_t per_block) {
-  __shared__ float2 lds[kStampPoolFloats / 2];
+  __shared__ float2 lds[StampLds<E>::floats / 2];
   __shared__ float bas[21];
   __shared__ float taps[28];
   __shared__ float s_xs, s_ys;
@@ -992,6 +999,7 @@ __global__ __launch_bounds__(kPreBlock) void galaxy_stamps_kernel(
     clip.sx1 = clip.sy1 = clip.rx1 = clip.ry1 = T;
   }
   float2* S;
+  float2* A = lds;
   float *F, *P, *Ha;
   if constexpr (kLds) {
     S = lds + H * T;
@@ -1004,6 +1012,7 @@ __global__ __launch_bounds__(kPreBlock) void galaxy_stamps_kernel(
     F = base + round64((size_t)2 * np);
     P = F + round64((size_t)np);
     Ha = P + round64((size_t)TT);
+    if constexpr (E == 8) A = reinterpret_cast<float2*>(Ha + round64((size_t)TT));
   }
   PlaneJob j;
   j.g = 0;
@@ -1016,7 +1025,7 @@ __global__ __launch_bounds__(kPreBlock) void galaxy_stamps_kernel(
   j.p6 = nullptr;
   j.P3 = P;
   j.S = S;
-  j.A = lds;
+  j.A = A;
   j.F = F;
   j.dt = nullptr;
   const double c2 = -2.0 * 9.869604401089358;  // -2 pi^2
@@ -1071,7 +1080,7 @@ bool launch_galaxy_stamps(int n, int T, int nt, const float* coefs, const float*
                           const float* ee2, const float* sersic_n, float* out, char* scratch,
                           size_t scratch_bytes, int n_cu, hipStream_t s) {
   const int pad = fft_pad_for(T);
-  if (!galaxy_fits(T, pad)) return false;
+  if (!galaxy_fits(T, pad) && pad != 512) return false;
   const bool in_lds = stamps_in_lds(T, pad);
   const size_t per = align256(stamp_scratch_floats(T) * sizeof(float));
   long long nblk = 2LL * n_cu;
@@ -1092,8 +1101,10 @@ bool launch_galaxy_stamps(int n, int T, int nt, const float* coefs, const float*
     UPH_GAL_STAMPS(2, true);
   else if (pad == 128)
     UPH_GAL_STAMPS(2, false);
-  else
+  else if (pad == 256)
     UPH_GAL_STAMPS(4, false);
+  else
+    UPH_GAL_STAMPS(8, false);
 #undef UPH_GAL_STAMPS
   return true;
 }
