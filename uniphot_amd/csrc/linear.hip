@@ -218,6 +218,7 @@ struct BandArgs {
   char* scratch;
   size_t per_problem;
   int stage;  // weighted stamps staged in LDS (UPH_WAVES * T^2 floats of dynamic LDS)
+  int lu_apart;  // the LU, inverse diagonal and solve run in band_lu_kernel
 };
 
 // float32 background basis at full-image pixel (px, py): float64
@@ -333,9 +334,24 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_solve_kernel(BandArgs a) {
           const int ow = kx1 - kx0, oh = ky1 - ky0;
           float acc = 0.0f;
           if (ow > 0 && oh > 0) {
-            for (int e = lane; e < ow * oh; e += 64) {
-              const int ky = ky0 + e / ow, kx = kx0 + e % ow;
-              acc = fmaf(wsb[ky * T + kx], tt[(ky + di) * T + kx + dj], acc);
+            // four pixels per lane per step, their neighbour-stamp loads
+            // issued together
+            const int npx = ow * oh;
+            for (int e0 = lane; e0 < npx; e0 += 256) {
+              float tv[4], wv[4];
+#pragma unroll
+              for (int u = 0; u < 4; ++u) {
+                const int e = e0 + 64 * u;
+                tv[u] = 0.0f;
+                wv[u] = 0.0f;
+                if (e < npx) {
+                  const int ky = ky0 + e / ow, kx = kx0 + e % ow;
+                  tv[u] = tt[(ky + di) * T + kx + dj];
+                  wv[u] = wsb[ky * T + kx];
+                }
+              }
+#pragma unroll
+              for (int u = 0; u < 4; ++u) acc = fmaf(wv[u], tv[u], acc);
             }
           }
           acc = wave_sum(acc);
@@ -455,6 +471,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_solve_kernel(BandArgs a) {
     const double d = A[(size_t)i * NT + i];
     A[(size_t)i * NT + i] = d + double(a.reg) * fmax(fabs(d), 1e-30);
   }
+  if (a.lu_apart) return;  // band_lu_kernel factors and solves
   __syncthreads();
   block_lu_wpiv(A, NT, piv, &s_piv);
   // inverse columns (one per thread) -> variance diagonal; then the solve
@@ -629,6 +646,58 @@ struct BlankArgs {
   size_t max_pix;
   double* b_hat;            // (n_prob)
 };
+
+// Factor and solve of the grid-band systems assembled by band_solve_kernel
+// (problem q's A (NT x NT, ridge applied) and right-hand side in its
+// scratch): A copied into LDS when it fits (NT <= kLuLdsN), then the same
+// LU with partial pivoting, the inverse columns (one per thread) for the
+// variance diagonal and the solve, as band_solve_kernel does in place.
+constexpr int kLuLdsN = 137;
+
+template <int M>
+__global__ __launch_bounds__(UPH_BLOCK) void band_lu_kernel(BandArgs a) {
+  extern __shared__ double lu_lds[];
+  __shared__ int s_piv;
+  const int q = blockIdx.x, tid = threadIdx.x;
+  const int64_t s0 = a.src_off[q];
+  const int n = int(a.src_off[q + 1] - s0);
+  const int NT = n + M;
+  char* base = a.scratch + (size_t)q * a.per_problem;
+  if ((size_t)NT * NT * 16 + (size_t)NT * 12 > a.per_problem) return;  // NaN written already
+  double* Ag = reinterpret_cast<double*>(base);
+  double* X = Ag + (size_t)NT * NT;
+  double* rhs = X + (size_t)NT * NT;
+  int* piv = reinterpret_cast<int*>(rhs + NT);
+  auto finish = [&](double* A) {
+    for (int i = tid; i < NT; i += UPH_BLOCK) {
+      double* x = X + (size_t)i * NT;
+      for (int j = 0; j < NT; ++j) x[j] = 0.0;
+      x[i] = 1.0;
+      lu_solve_one(A, piv, NT, x);
+    }
+    if (tid == 0) lu_solve_one(A, piv, NT, rhs);
+    __syncthreads();
+    for (int i = tid; i < NT; i += UPH_BLOCK) {
+      const double v = X[(size_t)i * NT + i];
+      if (i < n) {
+        a.fluxes[s0 + i] = float(rhs[i]);
+        a.flux_var[s0 + i] = float(v);
+      } else {
+        a.bg[(size_t)q * M + i - n] = float(rhs[i]);
+        a.bg_var[(size_t)q * M + i - n] = float(v);
+      }
+    }
+  };
+  if (NT <= kLuLdsN) {
+    for (int e = tid; e < NT * NT; e += UPH_BLOCK) lu_lds[e] = Ag[e];
+    __syncthreads();
+    block_lu_wpiv(lu_lds, NT, piv, &s_piv);
+    finish(lu_lds);
+  } else {
+    block_lu_wpiv(Ag, NT, piv, &s_piv);
+    finish(Ag);
+  }
+}
 
 template <int M>
 __global__ __launch_bounds__(UPH_BLOCK) void blank_residual_kernel(BlankArgs a) {
@@ -955,14 +1024,20 @@ extern "C" int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const 
   a.per_problem = (scratch_bytes / (size_t)n_prob) & ~(size_t)255;
   // stage the weighted stamps in LDS up to T = 63 (64 KiB per workgroup)
   a.stage = T <= 63;
+  a.lu_apart = 1;
   const size_t lds = a.stage ? (size_t)UPH_WAVES * T * T * sizeof(float) : 0;
+  const size_t lds_lu = (size_t)uph::kLuLdsN * uph::kLuLdsN * sizeof(double);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (a.M == 1)
+  if (a.M == 1) {
     hipLaunchKernelGGL(uph::band_solve_kernel<1>, dim3(n_prob), dim3(UPH_BLOCK), lds, s, a);
-  else if (a.M == 3)
+    hipLaunchKernelGGL(uph::band_lu_kernel<1>, dim3(n_prob), dim3(UPH_BLOCK), lds_lu, s, a);
+  } else if (a.M == 3) {
     hipLaunchKernelGGL(uph::band_solve_kernel<3>, dim3(n_prob), dim3(UPH_BLOCK), lds, s, a);
-  else
+    hipLaunchKernelGGL(uph::band_lu_kernel<3>, dim3(n_prob), dim3(UPH_BLOCK), lds_lu, s, a);
+  } else {
     hipLaunchKernelGGL(uph::band_solve_kernel<6>, dim3(n_prob), dim3(UPH_BLOCK), lds, s, a);
+    hipLaunchKernelGGL(uph::band_lu_kernel<6>, dim3(n_prob), dim3(UPH_BLOCK), lds_lu, s, a);
+  }
   return uph::hip_status(hipGetLastError(), "uph_band_solve launch");
 }
 
