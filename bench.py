@@ -36,6 +36,7 @@ class KernelTimer:
 
     def __init__(self):
         self.recs = []
+        self.deferred = []
         self.enabled = True
 
     def begin(self, name):
@@ -58,9 +59,19 @@ class KernelTimer:
         """An event pair recorded inside the library (per-kernel launches)."""
         self.recs.append((name, e0, e1, info))
 
+    def defer(self, fn):
+        """Bookkeeping that records event pairs, run when the timer reports
+        rather than inside the timed steps."""
+        self.deferred.append(fn)
+
+    def _flush(self):
+        while self.deferred:
+            self.deferred.pop(0)()
+
     def per_launch(self):
         import torch
         torch.cuda.synchronize()
+        self._flush()
         rows = []
         for name, e0, e1, info in self.recs:
             rows.append((name, e0.elapsed_time(e1), {k: v for k, v in info.items()
@@ -70,6 +81,7 @@ class KernelTimer:
     def summary(self):
         import torch
         torch.cuda.synchronize()
+        self._flush()
         out = {}
         for name, e0, e1, info in self.recs:
             ms = e0.elapsed_time(e1)

@@ -483,6 +483,92 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         return dict(kind="island_batch", n=n_isl, nb=nb, outs=outs, area=area, nreal=mmc,
                     name="island_split_sequence" if split else None)
 
+    def _record_costs(nst):
+        # Algorithmic cost of every launch of the sequence, recorded with
+        # its events.  Bytes are the unique HBM bytes the launch must move;
+        # flops count multiply-adds as 2 and a complex FFT of n points as
+        # 5 n log2 n.
+        #   evaluation: data + weight of every island box it evaluates (INIT
+        #     and FINAL: all islands; the H/g and trial launches of step it:
+        #     the islands that took step it); per band the Gram of the
+        #     1 + N + nL planes, the residual and its projections.
+        #   taps/spectra: spectra written (all three on H/g passes, F alone
+        #     otherwise) and 28 taps per member.
+        #   extended planes: PSF coefficient table once, spectra read, window
+        #     part of the planes written; per (member, band) the polynomial
+        #     over the stamp, forward transforms of P (and dP/dx, dP/dy),
+        #     one spectral product and inverse transform per output plane,
+        #     the 7-tap shift of the window part.
+        #   stars: coefficient table once, planes written; the polynomial
+        #     over the shift-read box and the shift.
+        box = nb * area.astype(np.float64) * 8.0
+        kp = c["keep"].reshape(-1, 6).astype(np.float64) * c["mm"].reshape(-1, 1)
+        nlive = np.asarray([kp[moff[i]:moff[i] + shape[i][0]].sum() for i in range(n_isl)])
+        npix = area.astype(np.float64)
+        nr = mmc.astype(np.float64)
+        kinds = ["init"] + ["grad", "try"] * int(max_steps) + ["final"] + ["post"] * post_on
+        g_isl = gal["isl"]
+        pad = np.asarray([_fft_pad(int(t)) for t in gal["T"]], np.float64)
+        coef_tbl = {T: nb * nt * T * T * 4.0 for T in tiers}
+
+        def fft_cost(n):
+            return 5.0 * n * np.log2(n)
+
+        for k, kind in enumerate(kinds):
+            it = (k - 1) // 2
+            sel = np.ones(n_isl, bool) if kind in ("init", "final", "post") else nst > it
+            e5 = evs[6 * k:6 * k + 6]
+            grad = kind in ("grad", "post")
+            if g_isl.size:
+                gsel = sel[g_isl]
+                st_, ex_ = gsel & gal["star"], gsel & ~gal["star"]
+                tg = gal["T"].astype(np.float64)
+                planes = gal["grad_planes"] if grad else np.ones(len(g_isl))
+                nspec = 3.0 if grad else 1.0
+                spec_b = gal["spec_floats"] / 3.0 * nspec * 4.0
+                sbytes = float(np.sum(spec_b[ex_]) + 28 * 4.0 * gsel.sum())
+                hh = pad // 2 + 1
+                npoly = 3.0 if grad else 1.0
+                nfwd = np.where(grad & (gal["grad_planes"] > 1), 3.0, 1.0) if grad else 1.0
+                e_fl = (2.0 * nt * tg * tg * npoly + nfwd * (tg + hh) * fft_cost(pad)
+                        + planes * (6.0 * hh * pad + (hh + tg) * fft_cost(pad))
+                        + 28.0 * gal["win"] * planes)
+                t_fl = 2.0 * nt * gal["reg"] * npoly + 28.0 * gal["win"] * planes
+                tiers_of = lambda m: sum(coef_tbl[int(t)] for t in set(gal["T"][m].tolist()))
+                ebytes = float(tiers_of(ex_) + np.sum(spec_b[ex_])
+                               + nb * np.sum((gal["win"] * planes * 4.0)[ex_]))
+                tbytes = float(tiers_of(st_) + nb * np.sum((gal["win"] * planes * 4.0)[st_]))
+                timer.record("prerender_spectra_kernel", e5[0], e5[1],
+                             dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
+                                  n_pre=int(gsel.sum())))
+                if gal["n_ext"]:
+                    timer.record("prerender_planes_kernel", e5[1], e5[2],
+                                 dict(kind="island_eval", phase=kind, bytes=ebytes,
+                                      flops=float(nb * np.sum(e_fl[ex_])),
+                                      n_pre=int(ex_.sum())))
+                if gal["n_ext"] < g_isl.size:
+                    timer.record("prerender_stars_kernel", e5[2], e5[3],
+                                 dict(kind="island_eval", phase=kind, bytes=tbytes,
+                                      flops=float(nb * np.sum(t_fl[st_])),
+                                      n_pre=int(st_.sum())))
+            ncol = 1 + nr + (nlive if grad else 0)
+            gram_fl = nb * npix * ncol * (ncol + 1)
+            ev_fl = nb * npix * (ncol * (ncol + 1) + 4.0 * nr + (2.0 * (nr + nlive) if grad else 0))
+            if big.size:
+                # the large islands' Gram (and its data + weight reads) is
+                # the Gram launch's, the rest of their evaluation stays
+                bsel = np.zeros(n_isl, bool)
+                bsel[big] = True
+                bsel &= sel
+                timer.record("island_gram_kernel", e5[3], e5[4],
+                             dict(kind="island_eval", phase=kind,
+                                  bytes=float(box[bsel].sum()),
+                                  flops=float(gram_fl[bsel].sum()), n_isl=int(bsel.sum())))
+                ev_fl = np.where(bsel, ev_fl - gram_fl, ev_fl)
+            timer.record("island_eval_kernel", e5[4], e5[5],
+                         dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
+                              flops=float(ev_fl[sel].sum()), n_isl=int(sel.sum())))
+
     def collect():
         return _collect()
 
@@ -492,91 +578,14 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
             raise _lib.UphError(_lib.UPH_ERR_INTERNAL,
                                 "island band blocks overflowed their buffer")
         if split and evs is not None:
-            # Algorithmic cost of every launch of the sequence, recorded with
-            # its events.  Bytes are the unique HBM bytes the launch must move;
-            # flops count multiply-adds as 2 and a complex FFT of n points as
-            # 5 n log2 n.
-            #   evaluation: data + weight of every island box it evaluates (INIT
-            #     and FINAL: all islands; the H/g and trial launches of step it:
-            #     the islands that took step it); per band the Gram of the
-            #     1 + N + nL planes, the residual and its projections.
-            #   taps/spectra: spectra written (all three on H/g passes, F alone
-            #     otherwise) and 28 taps per member.
-            #   extended planes: PSF coefficient table once, spectra read, window
-            #     part of the planes written; per (member, band) the polynomial
-            #     over the stamp, forward transforms of P (and dP/dx, dP/dy),
-            #     one spectral product and inverse transform per output plane,
-            #     the 7-tap shift of the window part.
-            #   stars: coefficient table once, planes written; the polynomial
-            #     over the shift-read box and the shift.
-            nst = h["nst"].astype(np.int64)
-            box = nb * area.astype(np.float64) * 8.0
-            kp = c["keep"].reshape(-1, 6).astype(np.float64) * c["mm"].reshape(-1, 1)
-            nlive = np.asarray([kp[moff[i]:moff[i] + shape[i][0]].sum() for i in range(n_isl)])
-            npix = area.astype(np.float64)
-            nr = mmc.astype(np.float64)
-            kinds = ["init"] + ["grad", "try"] * int(max_steps) + ["final"] + ["post"] * post_on
-            g_isl = gal["isl"]
-            coef_tbl = {T: nb * nt * T * T * 4.0 for T in tiers}
-
-            def fft_cost(n):
-                return 5.0 * n * np.log2(n)
-
-            for k, kind in enumerate(kinds):
-                it = (k - 1) // 2
-                sel = np.ones(n_isl, bool) if kind in ("init", "final", "post") else nst > it
-                e5 = evs[6 * k:6 * k + 6]
-                grad = kind in ("grad", "post")
-                if g_isl.size:
-                    gsel = sel[g_isl]
-                    st_, ex_ = gsel & gal["star"], gsel & ~gal["star"]
-                    tg = gal["T"].astype(np.float64)
-                    planes = gal["grad_planes"] if grad else np.ones(len(g_isl))
-                    nspec = 3.0 if grad else 1.0
-                    spec_b = gal["spec_floats"] / 3.0 * nspec * 4.0
-                    sbytes = float(np.sum(spec_b[ex_]) + 28 * 4.0 * gsel.sum())
-                    pad = np.asarray([_fft_pad(int(t)) for t in gal["T"]], np.float64)
-                    hh = pad // 2 + 1
-                    npoly = 3.0 if grad else 1.0
-                    nfwd = np.where(grad & (gal["grad_planes"] > 1), 3.0, 1.0) if grad else 1.0
-                    e_fl = (2.0 * nt * tg * tg * npoly + nfwd * (tg + hh) * fft_cost(pad)
-                            + planes * (6.0 * hh * pad + (hh + tg) * fft_cost(pad))
-                            + 28.0 * gal["win"] * planes)
-                    t_fl = 2.0 * nt * gal["reg"] * npoly + 28.0 * gal["win"] * planes
-                    tiers_of = lambda m: sum(coef_tbl[int(t)] for t in set(gal["T"][m].tolist()))
-                    ebytes = float(tiers_of(ex_) + np.sum(spec_b[ex_])
-                                   + nb * np.sum((gal["win"] * planes * 4.0)[ex_]))
-                    tbytes = float(tiers_of(st_) + nb * np.sum((gal["win"] * planes * 4.0)[st_]))
-                    timer.record("prerender_spectra_kernel", e5[0], e5[1],
-                                 dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
-                                      n_pre=int(gsel.sum())))
-                    if gal["n_ext"]:
-                        timer.record("prerender_planes_kernel", e5[1], e5[2],
-                                     dict(kind="island_eval", phase=kind, bytes=ebytes,
-                                          flops=float(nb * np.sum(e_fl[ex_])),
-                                          n_pre=int(ex_.sum())))
-                    if gal["n_ext"] < g_isl.size:
-                        timer.record("prerender_stars_kernel", e5[2], e5[3],
-                                     dict(kind="island_eval", phase=kind, bytes=tbytes,
-                                          flops=float(nb * np.sum(t_fl[st_])),
-                                          n_pre=int(st_.sum())))
-                ncol = 1 + nr + (nlive if grad else 0)
-                gram_fl = nb * npix * ncol * (ncol + 1)
-                ev_fl = nb * npix * (ncol * (ncol + 1) + 4.0 * nr + (2.0 * (nr + nlive) if grad else 0))
-                if big.size:
-                    # the large islands' Gram (and its data + weight reads) is
-                    # the Gram launch's, the rest of their evaluation stays
-                    bsel = np.zeros(n_isl, bool)
-                    bsel[big] = True
-                    bsel &= sel
-                    timer.record("island_gram_kernel", e5[3], e5[4],
-                                 dict(kind="island_eval", phase=kind,
-                                      bytes=float(box[bsel].sum()),
-                                      flops=float(gram_fl[bsel].sum()), n_isl=int(bsel.sum())))
-                    ev_fl = np.where(bsel, ev_fl - gram_fl, ev_fl)
-                timer.record("island_eval_kernel", e5[4], e5[5],
-                             dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
-                                  flops=float(ev_fl[sel].sum()), n_isl=int(sel.sum())))
+            # the cost bookkeeping runs when the timer reports (``timer.defer``),
+            # not between the fit and the next pass
+            nst_h = h["nst"].astype(np.int64)
+            defer = getattr(timer, "defer", None)
+            if defer is not None:
+                defer(lambda: _record_costs(nst_h))
+            else:
+                _record_costs(nst_h)
         hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics")} if post_on
               else None)
         h_uns = unsettled.cpu().numpy() if split and bf16_on else None
