@@ -42,20 +42,34 @@ def build_neighbor_list(pos_x, pos_y, H_psf: int, W_psf: int, k_max: int = DEFAU
     fin = np.where(np.isfinite(px) & np.isfinite(py))[0]
     if fin.size == 0:
         return idx, msk
+    # every pair within the circumscribed radius once, both directions,
+    # filtered to overlapping boxes; per source the candidates in ascending
+    # index order, truncated to the k_max nearest where more qualify
     tree = cKDTree(np.column_stack([px[fin], py[fin]]))
-    lists = tree.query_ball_point(np.column_stack([px[fin], py[fin]]),
-                                  r=float(np.hypot(W_psf, H_psf)))
-    for a, s in enumerate(fin):
-        cand = fin[np.asarray(lists[a], np.int64)]
-        cand = cand[cand != s]
-        if cand.size == 0:
-            continue
-        cand = cand[(np.abs(px[cand] - px[s]) < W_psf) & (np.abs(py[cand] - py[s]) < H_psf)]
-        if cand.size > k_max:
-            d2 = (px[cand] - px[s]) ** 2 + (py[cand] - py[s]) ** 2
-            cand = cand[np.argsort(d2)[:k_max]]
-        idx[s, :cand.size] = cand
-        msk[s, :cand.size] = True
+    pr = tree.query_pairs(float(np.hypot(W_psf, H_psf)), output_type="ndarray")
+    if pr.size == 0:
+        return idx, msk
+    s = fin[np.concatenate([pr[:, 0], pr[:, 1]])]
+    c = fin[np.concatenate([pr[:, 1], pr[:, 0]])]
+    keep = (np.abs(px[c] - px[s]) < W_psf) & (np.abs(py[c] - py[s]) < H_psf)
+    s, c = s[keep], c[keep]
+    if s.size == 0:
+        return idx, msk
+    o = np.lexsort((c, s))
+    s, c = s[o], c[o]
+    starts = np.flatnonzero(np.r_[True, s[1:] != s[:-1]])
+    counts = np.diff(np.r_[starts, s.size])
+    rank = np.arange(s.size) - np.repeat(starts, counts)
+    fits = np.repeat(counts <= k_max, counts)
+    idx[s[fits], rank[fits]] = c[fits]
+    msk[s[fits], rank[fits]] = True
+    for g in np.flatnonzero(counts > k_max):
+        a, z = starts[g], starts[g] + counts[g]
+        src, cand = s[a], c[a:z]
+        d2 = (px[cand] - px[src]) ** 2 + (py[cand] - py[src]) ** 2
+        cand = cand[np.argsort(d2)[:k_max]]
+        idx[src, :cand.size] = cand
+        msk[src, :cand.size] = True
     return idx, msk
 
 
@@ -118,27 +132,26 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
         np.zeros(0, dt)
     x = cat(lambda p: p.pos_x, np.float32)
     y = cat(lambda p: p.pos_y, np.float32)
+    # per-problem scalars spread over the problem's sources
     org = [p.origin if p.origin is not None else (p.x0, p.y0) for p in problems]
-    gx0 = np.concatenate([np.full(p.pos_x.size, o[0], np.float32) for p, o in
-                          zip(problems, org)]) if ntot else np.zeros(0, np.float32)
-    gy0 = np.concatenate([np.full(p.pos_x.size, o[1], np.float32) for p, o in
-                          zip(problems, org)]) if ntot else np.zeros(0, np.float32)
-    band = cat(lambda p: np.full(p.pos_x.size, p.band, np.int32), np.int32)
+    gx0 = np.repeat(np.array([o[0] for o in org], np.float32), ns)
+    gy0 = np.repeat(np.array([o[1] for o in org], np.float32), ns)
+    band = np.repeat(np.array([p.band for p in problems], np.int32), ns)
     ixt = np.rint(x - gx0).astype(np.int32) - T // 2
     iyt = np.rint(y - gy0).astype(np.int32) - T // 2
     # window-local top-left (window origin may differ from the rounding origin)
-    wx0 = np.concatenate([np.full(p.pos_x.size, p.x0 - o[0], np.int32) for p, o in
-                          zip(problems, org)]) if ntot else np.zeros(0, np.int32)
-    wy0 = np.concatenate([np.full(p.pos_x.size, p.y0 - o[1], np.int32) for p, o in
-                          zip(problems, org)]) if ntot else np.zeros(0, np.int32)
-    ixt -= wx0
-    iyt -= wy0
+    ixt -= np.repeat(np.array([p.x0 - o[0] for p, o in zip(problems, org)], np.int32), ns)
+    iyt -= np.repeat(np.array([p.y0 - o[1] for p, o in zip(problems, org)], np.int32), ns)
     nbr = np.concatenate([p.nbr_idx[:, :k_max] for p in problems]).astype(np.int32) if ntot \
         else np.zeros((0, k_max), np.int32)
     win = np.array([[p.band, p.x0, p.y0, p.h, p.w] for p in problems], np.int32)
     check_windows(win, nb, H, W, "solve_problems")
-    mixed_flag = np.array([p.stype is not None and bool(np.any(p.stype != SOURCE_STAR))
-                           for p in problems])
+    # problems of one grid share their type array: test each array once
+    ext_of = {}
+    for p in problems:
+        if p.stype is not None and id(p.stype) not in ext_of:
+            ext_of[id(p.stype)] = bool(np.any(p.stype != SOURCE_STAR))
+    mixed_flag = np.array([p.stype is not None and ext_of[id(p.stype)] for p in problems])
     up = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
     L = _lib.lib()
     P = _lib.ptr
@@ -148,8 +161,7 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
     # source do not (forced_phot_sparse.py:624-637): one render call per kind
     for mixed in (False, True):
         sel = np.where(mixed_flag == mixed)[0]
-        idx = (np.concatenate([np.arange(off[q], off[q + 1]) for q in sel]).astype(np.int64)
-               if sel.size else np.zeros(0, np.int64))
+        idx = np.flatnonzero(np.repeat(mixed_flag == mixed, ns)).astype(np.int64)
         if idx.size == 0:
             continue
         r_x, r_y, r_gx, r_gy, r_b = (up(v[idx]) for v in (x, y, gx0, gy0, band))
@@ -204,14 +216,11 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
                            pix=int(sum(p.h * p.w for p in problems))))
     flh, fvh = fl.cpu().numpy(), fv.cpu().numpy()
     bgh, bvh = bgc.cpu().numpy(), bgv.cpu().numpy()
-    out = []
-    for q in range(len(problems)):
-        f = flh[off[q]:off[q + 1]]
-        v = fvh[off[q]:off[q + 1]]
-        out.append(SparseForcedPhotResult(f, bgh[q], v, bvh[q],
-                                          bool(np.all(np.isfinite(f)) and
-                                               np.all(np.isfinite(bgh[q])))))
-    return out
+    # converged: every flux and background coefficient of the problem finite
+    nbad = np.concatenate([[0], np.cumsum(~np.isfinite(flh[:ntot]))])
+    ok = (nbad[off[1:]] == nbad[off[:-1]]) & np.isfinite(bgh).all(axis=1)
+    return [SparseForcedPhotResult(flh[off[q]:off[q + 1]], bgh[q], fvh[off[q]:off[q + 1]],
+                                   bvh[q], bool(ok[q])) for q in range(len(problems))]
 
 
 def render_unit_stamps(psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, band, pos_x, pos_y,

@@ -391,12 +391,14 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
         for g in gl:
             fi = g.fit_indices
             nbr, _ = fps.build_neighbor_list(x[fi], y[fi], T, T, cfg.k_max)
+            # the bands of a grid share its source arrays
+            px, py = x[fi].astype(np.float32), y[fi].astype(np.float32)
+            gst, glr, ge1, ge2, gsn = stype[fi], logre[fi], ee1[fi], ee2[fi], sn[fi]
+            x0, y0, h, w = g.x0_pad, g.y0_pad, g.pad_h, g.pad_w
             for b in range(nb):
                 problems.append(fps.BandProblem(
-                    band=b, x0=g.x0_pad, y0=g.y0_pad, h=g.pad_h, w=g.pad_w,
-                    pos_x=x[fi].astype(np.float32), pos_y=y[fi].astype(np.float32),
-                    nbr_idx=nbr, stype=stype[fi], logre=logre[fi], ee1=ee1[fi], ee2=ee2[fi],
-                    sersic_n=sn[fi]))
+                    band=b, x0=x0, y0=y0, h=h, w=w, pos_x=px, pos_y=py, nbr_idx=nbr,
+                    stype=gst, logre=glr, ee1=ge1, ee2=ge2, sersic_n=gsn))
                 owners.append((g, b))
         coefs, geo = fld.psf(T)
         # launches of at most stamp_budget bytes of rendered stamps each
