@@ -131,16 +131,18 @@ template <int H>
 __device__ __forceinline__ float lane_xor(float x) {
   const int v = __float_as_int(x);
   int r;
+  // every lane of these DPP forms reads a lane of its own row, so the move
+  // needs no old value and can fold into the instruction that consumes it
   if constexpr (H == 1) {
-    r = __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);  // quad [1,0,3,2]
+    r = __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad [1,0,3,2]
   } else if constexpr (H == 2) {
-    r = __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);  // quad [2,3,0,1]
+    r = __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad [2,3,0,1]
   } else if constexpr (H == 4) {
-    const int t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // i ^ 7
-    r = __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);             // i ^ 3
+    const int t = __builtin_amdgcn_mov_dpp(v, 0x141, 0xF, 0xF, true);  // i ^ 7
+    r = __builtin_amdgcn_mov_dpp(t, 0x1B, 0xF, 0xF, true);             // i ^ 3
   } else if constexpr (H == 8) {
-    const int t = __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // i ^ 15
-    r = __builtin_amdgcn_update_dpp(0, t, 0x141, 0xF, 0xF, false);            // i ^ 7
+    const int t = __builtin_amdgcn_mov_dpp(v, 0x140, 0xF, 0xF, true);  // i ^ 15
+    r = __builtin_amdgcn_mov_dpp(t, 0x141, 0xF, 0xF, true);            // i ^ 7
   } else if constexpr (H == 16) {
     r = __builtin_amdgcn_ds_swizzle(v, (0x10 << 10) | 0x1F);  // bit mode: xor 16
   } else {
@@ -168,9 +170,14 @@ __device__ __forceinline__ void lane_stage(float (&re)[E], float (&im)[E], const
   }
 #pragma unroll
   for (int e = 0; e < E; ++e) {
+    // partner + sg * own as a product and a sum: sg = +-1 makes the product
+    // exact, so this is the fused result, and the partner's DPP move folds
+    // into the sum
+#pragma clang fp contract(off)
     const float qr = lane_xor<h>(zr[e]), qi = lane_xor<h>(zi[e]);
-    re[e] = fmaf(sg, zr[e], qr);
-    im[e] = fmaf(sg, zi[e], qi);
+    const float sr = sg * zr[e], si = sg * zi[e];
+    re[e] = qr + sr;
+    im[e] = qi + si;
   }
 }
 
