@@ -41,18 +41,16 @@ def test_render_stamps_match_oracle(cuda):
         assert np.max(np.abs(got[i] - ref)) < 2e-7 * np.max(np.abs(ref)) * 10
 
 
-def test_band_solve_matches_oracle(cuda):
+def _check_band_solve(sc, x0, y0, h, w, dev, min_src=0):
     import torch
     from uniphot_amd import forced_phot_sparse as fps
     from uniphot_amd.psf_polynomial import psf_device_arrays
-    sc = small_star_scene(n_src=20)
     nb, H, W = sc.data.shape
-    dev = cuda
     c, g = psf_device_arrays(sc.psfs, 55)
     # one grid = a sub-window of the image with all sources whose centre is inside
-    x0, y0, h, w = 10, 5, 140, 150
     inside = (sc.x >= x0) & (sc.x < x0 + w) & (sc.y >= y0) & (sc.y < y0 + h)
     xs, ys = sc.x[inside], sc.y[inside]
+    assert xs.size >= min_src
     nbr, msk = fps.build_neighbor_list(xs, ys, 55, 55)
     probs = [fps.BandProblem(b, x0, y0, h, w, xs, ys, nbr) for b in range(nb)]
     bg_geo = (0.5 * (W - 1), 0.5 * W, 0.5 * (H - 1), 0.5 * H)
@@ -76,6 +74,18 @@ def test_band_solve_matches_oracle(cuda):
         assert np.allclose(r.flux_var, fv, rtol=1e-5)
         assert np.allclose(r.bg_coefs, bgc, rtol=1e-4, atol=1e-4 * np.sqrt(bv).max())
         assert r.converged
+
+
+def test_band_solve_matches_oracle(cuda):
+    _check_band_solve(small_star_scene(n_src=20), 10, 5, 140, 150, cuda)
+
+
+def test_band_solve_large_system_matches_oracle(cuda):
+    """170 sources and a plane background in one window: a 173 x 173 system,
+    whose LU runs its first 36 steps in global memory and the rest with the
+    trailing block in LDS."""
+    sc = small_star_scene(seed=11, H=300, W=300, n_bands=1, n_src=170, clustered=False)
+    _check_band_solve(sc, 0, 0, 300, 300, cuda, min_src=138)
 
 
 def test_wing_peeling_matches_oracle(cuda):

@@ -474,14 +474,14 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_solve_kernel(BandArgs a) {
   if (a.lu_apart) return;  // band_lu_kernel factors and solves
   __syncthreads();
   block_lu_wpiv(A, NT, piv, &s_piv);
-  // inverse columns (one per thread) -> variance diagonal; then the solve
+  // inverse columns (one per thread, column i stored as X[j * NT + i]) ->
+  // variance diagonal; the solve on a thread without a column
   for (int i = tid; i < NT; i += UPH_BLOCK) {
-    double* x = X + (size_t)i * NT;
-    for (int j = 0; j < NT; ++j) x[j] = 0.0;
-    x[i] = 1.0;
-    lu_solve_one(A, piv, NT, x);
+    for (int j = 0; j < NT; ++j) X[(size_t)j * NT + i] = 0.0;
+    X[(size_t)i * NT + i] = 1.0;
+    lu_solve_strided(A, piv, NT, X + i, NT);
   }
-  if (tid == 0) lu_solve_one(A, piv, NT, rhs);
+  if (tid == (NT < UPH_BLOCK ? NT : 0)) lu_solve_one(A, piv, NT, rhs);
   __syncthreads();
   for (int i = tid; i < NT; i += UPH_BLOCK) {
     const double v = X[(size_t)i * NT + i];
@@ -649,10 +649,78 @@ struct BlankArgs {
 
 // Factor and solve of the grid-band systems assembled by band_solve_kernel
 // (problem q's A (NT x NT, ridge applied) and right-hand side in its
-// scratch): A copied into LDS when it fits (NT <= kLuLdsN), then the same
+// scratch): A copied into LDS when it fits (NT <= kLuLdsN; larger systems
+// keep their trailing kLuLdsN block there), then the same
 // LU with partial pivoting, the inverse columns (one per thread) for the
 // variance diagonal and the solve, as band_solve_kernel does in place.
 constexpr int kLuLdsN = 137;
+
+// block_lu_wpiv for n > kLuLdsN: steps k < k0 = n - kLuLdsN on the matrix
+// in global memory, then the trailing block [k0, n) x [k0, n) moves to L
+// (LDS, row stride m = n - k0) for the remaining steps, and back at the end.
+// The pivots, the operations and their order are those of block_lu_wpiv.
+__device__ void block_lu_lds_tail(double* A, double* L, int n, int* piv, int* s_p) {
+  const int tid = threadIdx.x, m = kLuLdsN, k0 = n - m;
+  for (int k = 0; k < n; ++k) {
+    if (k == k0) {
+      for (int e = tid; e < m * m; e += UPH_BLOCK)
+        L[e] = A[(size_t)(k0 + e / m) * n + k0 + e % m];
+      __syncthreads();
+    }
+    const bool in_l = k >= k0;
+    // element (i, j) with i, j >= k: the trailing block once in LDS
+    auto at = [&](int i, int j) -> double& {
+      return in_l ? L[(i - k0) * m + (j - k0)] : A[(size_t)i * n + j];
+    };
+    if (tid < 64) {
+      double best = tid == 0 ? fabs(at(k, k)) : -1.0;
+      int p = tid == 0 ? k : n;
+      for (int i = k + tid + (tid == 0 ? 64 : 0); i < n; i += 64) {
+        const double v = fabs(at(i, k));
+        if (v > best) {
+          best = v;
+          p = i;
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_down(best, off, 64);
+        const int op = __shfl_down(p, off, 64);
+        if (ob > best || (ob == best && op < p)) {
+          best = ob;
+          p = op;
+        }
+      }
+      if (tid == 0) {
+        piv[k] = p;
+        *s_p = p;
+      }
+    }
+    __syncthreads();
+    const int p = *s_p;
+    if (p != k) {
+      for (int j = tid; j < n; j += UPH_BLOCK) {
+        double* x = (in_l && j >= k0) ? &L[(k - k0) * m + j - k0] : &A[(size_t)k * n + j];
+        double* y = (in_l && j >= k0) ? &L[(p - k0) * m + j - k0] : &A[(size_t)p * n + j];
+        const double t = *x;
+        *x = *y;
+        *y = t;
+      }
+      __syncthreads();
+    }
+    const double akk = at(k, k);
+    for (int i = k + 1 + tid; i < n; i += UPH_BLOCK) at(i, k) /= akk;
+    __syncthreads();
+    const int mm = n - k - 1;
+    for (int e = tid; e < mm * mm; e += UPH_BLOCK) {
+      const int i = k + 1 + e / mm, j = k + 1 + e % mm;
+      at(i, j) -= at(i, k) * at(k, j);
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < m * m; e += UPH_BLOCK) A[(size_t)(k0 + e / m) * n + k0 + e % m] = L[e];
+  __syncthreads();
+}
 
 template <int M>
 __global__ __launch_bounds__(UPH_BLOCK) void band_lu_kernel(BandArgs a) {
@@ -669,13 +737,14 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_lu_kernel(BandArgs a) {
   double* rhs = X + (size_t)NT * NT;
   int* piv = reinterpret_cast<int*>(rhs + NT);
   auto finish = [&](double* A) {
+    // column i of the inverse as X[j * NT + i] (one column per thread); the
+    // solve on a thread without a column
     for (int i = tid; i < NT; i += UPH_BLOCK) {
-      double* x = X + (size_t)i * NT;
-      for (int j = 0; j < NT; ++j) x[j] = 0.0;
-      x[i] = 1.0;
-      lu_solve_one(A, piv, NT, x);
+      for (int j = 0; j < NT; ++j) X[(size_t)j * NT + i] = 0.0;
+      X[(size_t)i * NT + i] = 1.0;
+      lu_solve_strided(A, piv, NT, X + i, NT);
     }
-    if (tid == 0) lu_solve_one(A, piv, NT, rhs);
+    if (tid == (NT < UPH_BLOCK ? NT : 0)) lu_solve_one(A, piv, NT, rhs);
     __syncthreads();
     for (int i = tid; i < NT; i += UPH_BLOCK) {
       const double v = X[(size_t)i * NT + i];
@@ -694,7 +763,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void band_lu_kernel(BandArgs a) {
     block_lu_wpiv(lu_lds, NT, piv, &s_piv);
     finish(lu_lds);
   } else {
-    block_lu_wpiv(Ag, NT, piv, &s_piv);
+    block_lu_lds_tail(Ag, lu_lds, NT, piv, &s_piv);
     finish(Ag);
   }
 }

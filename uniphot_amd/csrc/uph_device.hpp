@@ -224,6 +224,31 @@ __device__ inline void lu_solve_one(const double* LU, const int* piv, int n, dou
   }
 }
 
+// lu_solve_one on a vector stored with stride st (x[k] at x[k * st]): the
+// same operations in the same order.  Columns of a row-major block solved
+// one per thread this way read consecutive addresses across the wave.
+__device__ inline void lu_solve_strided(const double* LU, const int* piv, int n, double* x,
+                                        int st) {
+  for (int k = 0; k < n; ++k) {
+    int p = piv[k];
+    if (p != k) {
+      double t = x[(size_t)k * st];
+      x[(size_t)k * st] = x[(size_t)p * st];
+      x[(size_t)p * st] = t;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    double s = x[(size_t)i * st];
+    for (int j = 0; j < i; ++j) s -= LU[i * n + j] * x[(size_t)j * st];
+    x[(size_t)i * st] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = x[(size_t)i * st];
+    for (int j = i + 1; j < n; ++j) s -= LU[i * n + j] * x[(size_t)j * st];
+    x[(size_t)i * st] = s / LU[i * n + i];
+  }
+}
+
 // Diagonal of A^-1 from its LU factors: column i of the inverse solved with
 // e_i, one column per thread; `col` is per-thread workspace (n doubles each).
 __device__ inline void lu_inverse_diag(const double* LU, const int* piv, int n, double* diag,
