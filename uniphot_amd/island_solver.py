@@ -483,7 +483,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         return dict(kind="island_batch", n=n_isl, nb=nb, outs=outs, area=area, nreal=mmc,
                     name="island_split_sequence" if split else None)
 
-    def _record_costs(nst):
+    def _record_costs(nst, gh):
         # Algorithmic cost of every launch of the sequence, recorded with
         # its events.  Bytes are the unique HBM bytes the launch must move;
         # flops count multiply-adds as 2 and a complex FFT of n points as
@@ -507,8 +507,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         npix = area.astype(np.float64)
         nr = mmc.astype(np.float64)
         kinds = ["init"] + ["grad", "try"] * int(max_steps) + ["final"] + ["post"] * post_on
-        g_isl = gal["isl"]
-        pad = np.asarray([_fft_pad(int(t)) for t in gal["T"]], np.float64)
+        g_isl = gh["isl"]
+        pad = np.asarray([_fft_pad(int(t)) for t in gh["T"]], np.float64)
         coef_tbl = {T: nb * nt * T * T * 4.0 for T in tiers}
 
         def fft_cost(n):
@@ -521,32 +521,32 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
             grad = kind in ("grad", "post")
             if g_isl.size:
                 gsel = sel[g_isl]
-                st_, ex_ = gsel & gal["star"], gsel & ~gal["star"]
-                tg = gal["T"].astype(np.float64)
-                planes = gal["grad_planes"] if grad else np.ones(len(g_isl))
+                st_, ex_ = gsel & gh["star"], gsel & ~gh["star"]
+                tg = gh["T"].astype(np.float64)
+                planes = gh["grad_planes"] if grad else np.ones(len(g_isl))
                 nspec = 3.0 if grad else 1.0
-                spec_b = gal["spec_floats"] / 3.0 * nspec * 4.0
+                spec_b = gh["spec_floats"] / 3.0 * nspec * 4.0
                 sbytes = float(np.sum(spec_b[ex_]) + 28 * 4.0 * gsel.sum())
                 hh = pad // 2 + 1
                 npoly = 3.0 if grad else 1.0
-                nfwd = np.where(grad & (gal["grad_planes"] > 1), 3.0, 1.0) if grad else 1.0
+                nfwd = np.where(grad & (gh["grad_planes"] > 1), 3.0, 1.0) if grad else 1.0
                 e_fl = (2.0 * nt * tg * tg * npoly + nfwd * (tg + hh) * fft_cost(pad)
                         + planes * (6.0 * hh * pad + (hh + tg) * fft_cost(pad))
-                        + 28.0 * gal["win"] * planes)
-                t_fl = 2.0 * nt * gal["reg"] * npoly + 28.0 * gal["win"] * planes
-                tiers_of = lambda m: sum(coef_tbl[int(t)] for t in set(gal["T"][m].tolist()))
+                        + 28.0 * gh["win"] * planes)
+                t_fl = 2.0 * nt * gh["reg"] * npoly + 28.0 * gh["win"] * planes
+                tiers_of = lambda m: sum(coef_tbl[int(t)] for t in set(gh["T"][m].tolist()))
                 ebytes = float(tiers_of(ex_) + np.sum(spec_b[ex_])
-                               + nb * np.sum((gal["win"] * planes * 4.0)[ex_]))
-                tbytes = float(tiers_of(st_) + nb * np.sum((gal["win"] * planes * 4.0)[st_]))
+                               + nb * np.sum((gh["win"] * planes * 4.0)[ex_]))
+                tbytes = float(tiers_of(st_) + nb * np.sum((gh["win"] * planes * 4.0)[st_]))
                 timer.record("prerender_spectra_kernel", e5[0], e5[1],
                              dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
                                   n_pre=int(gsel.sum())))
-                if gal["n_ext"]:
+                if gh["n_ext"]:
                     timer.record("prerender_planes_kernel", e5[1], e5[2],
                                  dict(kind="island_eval", phase=kind, bytes=ebytes,
                                       flops=float(nb * np.sum(e_fl[ex_])),
                                       n_pre=int(ex_.sum())))
-                if gal["n_ext"] < g_isl.size:
+                if gh["n_ext"] < g_isl.size:
                     timer.record("prerender_stars_kernel", e5[2], e5[3],
                                  dict(kind="island_eval", phase=kind, bytes=tbytes,
                                       flops=float(nb * np.sum(t_fl[st_])),
@@ -580,12 +580,16 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         if split and evs is not None:
             # the cost bookkeeping runs when the timer reports (``timer.defer``),
             # not between the fit and the next pass
+            # host fields only: the deferred call must not keep the batch's
+            # device buffers (gal holds them) alive
             nst_h = h["nst"].astype(np.int64)
+            gh = {k: gal.get(k) for k in ("isl", "star", "T", "grad_planes", "spec_floats",
+                                          "win", "reg", "n_ext")}
             defer = getattr(timer, "defer", None)
             if defer is not None:
-                defer(lambda: _record_costs(nst_h))
+                defer(lambda: _record_costs(nst_h, gh))
             else:
-                _record_costs(nst_h)
+                _record_costs(nst_h, gh)
         hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics")} if post_on
               else None)
         h_uns = unsettled.cpu().numpy() if split and bf16_on else None
