@@ -101,8 +101,8 @@ typedef struct uph_island_outputs {
   float* p;          /* (n_isl, 6N) fitted parameters                       */
   float* fluxes;     /* (n_isl, n_bands, N) VarPro member fluxes            */
   float* ferr;       /* (n_isl, n_bands, N) sqrt(diag(A^-1))                */
-  float* chi2;       /* (n_isl)                                             */
-  float* lam;        /* (n_isl) final LM damping                            */
+  double* chi2;      /* (n_isl) float64: the accept test compares float64 */
+  double* lam;       /* (n_isl) final LM damping, float64                   */
   uint8_t* conv;     /* (n_isl)                                             */
   uint8_t* stall;    /* (n_isl) converged by 4 consecutive rejects          */
   int32_t* nst;      /* (n_isl) LM steps taken                              */
@@ -232,7 +232,12 @@ typedef struct uph_island_split {
          of marginalising over extended members' parameters (0 for stars);
        post_metrics (member slots, n_bands, 6): per band sum m, sum (f m)^2,
          sum (model - f m) f m, sum m over zero-weight pixels,
-         sum m w (d - model)^2 and f, m the unit model on the window.     */
+         sum m w (d - model)^2 and f, m the unit model on the window;
+       post_kappa (n_isl): 1-norm condition number of the band-summed
+         Gauss-Newton matrix scaled to a unit diagonal (1 for an island
+         with nothing to fit) -- how strongly an LM step amplifies float32
+         model error; a caller refits islands above its threshold on the
+         float64 kernels.                                                  */
   int post;
   const int64_t* post_off;
   double* post_buf;
@@ -264,6 +269,7 @@ typedef struct uph_island_split {
   const int32_t* big_slot;
   const int32_t* big_isl;
   double* big_part;
+  float* post_kappa;          /* (n_isl), see post above                       */
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);

@@ -188,6 +188,19 @@ def hg(p, isl: Island, o: LMOptions):
     return H, g, chi2, live
 
 
+def kappa_scaled(H) -> float:
+    """1-norm condition number of H scaled to a unit diagonal,
+    |Hs|_1 |Hs^-1|_1 with Hs = D^-1/2 H D^-1/2, D = diag(H): the measure the
+    float32 fit uses to send an island to the float64 kernels (no
+    reference counterpart; the reference has no such policy)."""
+    H = np.asarray(H, np.float64)
+    if H.size == 0:
+        return 1.0
+    s = 1.0 / np.sqrt(np.diag(H))
+    Hs = H * s[:, None] * s[None, :]
+    return float(np.abs(Hs).sum(axis=0).max() * np.abs(np.linalg.inv(Hs)).sum(axis=0).max())
+
+
 def one_step(state: dict, isl: Island, o: LMOptions) -> dict:
     """Legacy damped LM step with accept/reject and the three convergence
     prongs (island_solver.py:921-1002).  ``state`` keys: p, lam, chi2,
@@ -209,11 +222,14 @@ def one_step(state: dict, isl: Island, o: LMOptions) -> dict:
     H = np.where(keep[:, None] & keep[None, :], H, 0.0) + np.diag(np.where(keep, 0.0, 1.0))
     g = np.where(keep, g, 0.0)
     Hd = H + state["lam"] * np.diag(np.diag(H)) + 1e-8 * np.eye(nP)
-    delta = -np.linalg.solve(Hd, g)
-    delta = np.where(keep, delta, 0.0)
-    cv = clip_vec(N, o)
+    # the parameter vector is float32 state, as in the reference: the
+    # float64 step is cast to float32, clipped and added in float32
+    # (island_solver.py:963-969, p_flat f32)
+    delta = (-np.linalg.solve(Hd, g)).astype(np.float32)
+    delta = np.where(keep, delta, np.float32(0.0))
+    cv = clip_vec(N, o).astype(np.float32)
     delta = np.clip(delta, -cv, cv)
-    p_try = np.clip(p + delta, isl.lower, isl.upper)
+    p_try = np.clip(p + delta, isl.lower.astype(np.float32), isl.upper.astype(np.float32))
     chi2_try = chi2_of(p_try, isl, o)
     chi2_cur = state["chi2"]
     accept = chi2_try < chi2_cur
@@ -223,9 +239,9 @@ def one_step(state: dict, isl: Island, o: LMOptions) -> dict:
     chi2_new = chi2_try if accept else chi2_cur
     dchi2 = chi2_cur - chi2_try
     step = (p_new - p).reshape(N, 6)
-    mmf = isl.mmask.astype(np.float64)
-    pos_step = float(np.max(mmf * np.hypot(step[:, 0], step[:, 1]))) if N else 0.0
-    shape_step = float(np.max(mmf[:, None] * np.abs(step[:, 2:6]))) if N else 0.0
+    mmf = isl.mmask.astype(np.float32)
+    pos_step = np.max(mmf * np.hypot(step[:, 0], step[:, 1])) if N else np.float32(0.0)
+    shape_step = np.max(mmf[:, None] * np.abs(step[:, 2:6])) if N else np.float32(0.0)
     small = accept and (dchi2 < o.ftol_rel * max(chi2_cur, 1.0))
     consec = state["consec"] + 1 if small else (0 if accept else state["consec"])
     step_small = accept and pos_step < o.step_tol and shape_step < o.shape_step_tol
@@ -239,9 +255,11 @@ def one_step(state: dict, isl: Island, o: LMOptions) -> dict:
 
 def run_island(p0, isl: Island, o: LMOptions, max_steps=None, trace=False):
     """Fresh-chunk LM loop for one island (run_class_chunk, :2890-2977):
-    lam = 1e-3, chi2 = chi2(p0), until converged or ``max_steps``."""
+    lam = 1e-3, chi2 = chi2(p0), until converged or ``max_steps``.  The
+    parameters are float32 state (the reference's ``p0``/``p_flat``
+    dtype); chi2 and the damping are float64."""
     max_steps = o.max_steps if max_steps is None else max_steps
-    st = dict(p=np.asarray(p0, np.float64).copy(), lam=LM_LAMBDA0,
+    st = dict(p=np.asarray(p0, np.float32).ravel().copy(), lam=LM_LAMBDA0,
               chi2=chi2_of(p0, isl, o), consec=0, nrej=0, conv=False,
               stall=False, nst=0)
     hist = []

@@ -143,7 +143,7 @@ def test_lm_stalls_after_four_rejects_when_steps_are_clipped_to_zero():
     assert [s["nrej"] for s in h] == [1, 2, 3, 4]
     np.testing.assert_allclose([s["lam"] for s in h], [1e-2, 1e-1, 1.0, 10.0])
     assert st["conv"] and st["stall"] and st["nst"] == 4
-    np.testing.assert_array_equal(st["p"], p0)
+    np.testing.assert_array_equal(st["p"], np.float32(p0))  # float32 state, unchanged
 
 
 def test_lm_invariants_on_a_noisy_mixed_island():
@@ -364,3 +364,17 @@ def test_scene_subtract_shifts_stars_to_their_subpixel_position():
     err = np.max(np.abs(img - true))
     assert err < 1e-2 * peak, err / peak          # measured 6.2e-3 at sigma 1.6 px
     assert np.max(np.abs(img - mirror)) > 0.2 * peak
+
+
+def test_kappa_scaled_closed_forms():
+    """Unit-diagonal 2x2 [[1, r], [r, 1]]: kappa_1 = (1 + |r|) / (1 - |r|);
+    a diagonal matrix of any scale: 1; invariant under diagonal scaling."""
+    for r in (0.0, 0.3, -0.9, 0.999):
+        H = np.array([[1.0, r], [r, 1.0]])
+        assert oisl.kappa_scaled(H) == pytest.approx((1 + abs(r)) / (1 - abs(r)), rel=1e-9)
+    assert oisl.kappa_scaled(np.diag([1e-6, 3.0, 1e8])) == pytest.approx(1.0)
+    rng = np.random.default_rng(0)
+    A = rng.normal(size=(5, 5))
+    H = A @ A.T + np.eye(5)
+    d = np.diag(rng.uniform(0.1, 10.0, 5))
+    assert oisl.kappa_scaled(d @ H @ d) == pytest.approx(oisl.kappa_scaled(H), rel=1e-9)

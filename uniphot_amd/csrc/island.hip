@@ -45,8 +45,9 @@ struct IslandSh {
   int N, B, T, nLc;          // this island's class shape
   long long moff, foff;      // member-slot and flux-block offsets
   const float* coefs;        // PSF coefficients of the island's tier
-  // LM state
-  float lam, chi2, chi2_try;
+  // LM state: chi2 and the damping in float64, so the accept test compares
+  // float64 sums (the per-band chi2 are float64 reductions of the residual)
+  double lam, chi2, chi2_try;
   int consec, nrej, nst, conv, stall;
 };
 
@@ -654,7 +655,7 @@ __device__ void place_member(const IslandArgs& a, IslandSh<Real>& sh, int b, con
 constexpr int kPreQ = 8;
 
 template <typename Real, bool kPre>
-__device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int b,
+__device__ double eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int b,
                           const float* pv, bool grads, bool accumulate, Real* planes, Real* ws,
                           Real* gws, double* G, double* part, const Small& sm, float* flux_out,
                           float* ferr_out, int* ngood) {
@@ -807,7 +808,7 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
       ferr_out[sh.mem[r]] = float(sqrt(fmax(sm.s[r], 1e-30)));
     }
     __syncthreads();
-    return Real(0);
+    return 0.0;
   }
   // residual R = d - sum_k f_k m_k ; chi2 = sum (sqrt(w) R)^2
   double acc = 0.0;
@@ -819,7 +820,7 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
     acc = fma(double(rw), double(rw), acc);
     Rp[p] = R;
   }
-  const Real chi2b = Real(block_sum(acc, sh.red));
+  const double chi2b = block_sum(acc, sh.red);
   if (!grads) return chi2b;
   // c = M^T W R, s = T^T W R (one pass over the Nr + nL planes)
   weighted_pairs<Real, false, kPre ? kPreQ : 32>(Wp, planes, P, Nr + nL, 3, 1, sm.colws, sh.red);
@@ -881,14 +882,14 @@ __device__ Real eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, int 
 }
 
 template <typename Real>
-__device__ float chi2_all(const IslandArgs& a, IslandSh<Real>& sh, int isl, const float* pv,
+__device__ double chi2_all(const IslandArgs& a, IslandSh<Real>& sh, int isl, const float* pv,
                           Real* planes, Real* ws, Real* gws, double* G, double* part,
                           const Small& sm) {
-  Real c = 0;
+  double c = 0.0;
   for (int b = 0; b < a.nb; ++b)
     c += eval_band<Real, false>(a, sh, isl, b, pv, false, false, planes, ws, gws, G, part, sm, nullptr,
                          nullptr, nullptr);
-  return float(c);
+  return c;
 }
 
 // Class shape and offsets of island `isl` into sh (called by one thread).
@@ -985,8 +986,8 @@ __device__ void lm_propose(const IslandArgs& a, IslandSh<Real>& sh, double* Hd, 
 // Accept / reject the trial with chi2 `ct` and update the convergence
 // prongs (one thread).
 template <typename Real>
-__device__ void lm_accept(const IslandArgs& a, IslandSh<Real>& sh, float ct) {
-  const float cur = sh.chi2;
+__device__ void lm_accept(const IslandArgs& a, IslandSh<Real>& sh, double ct) {
+  const double cur = sh.chi2;
   const bool accept = ct < cur;
   float pos_step = 0.0f, shape_step = 0.0f;
   if (accept) {
@@ -999,13 +1000,13 @@ __device__ void lm_accept(const IslandArgs& a, IslandSh<Real>& sh, float ct) {
         shape_step = fmaxf(shape_step, fabsf(sh.ptry[6 * k + j] - sh.p[6 * k + j]));
     }
     for (int q = 0; q < 6 * sh.N; ++q) sh.p[q] = sh.ptry[q];
-    sh.lam = fmaxf(sh.lam * 0.1f, 1e-9f);
+    sh.lam = fmax(sh.lam * 0.1, 1e-9);
     sh.chi2 = ct;
   } else {
-    sh.lam = fminf(sh.lam * 10.0f, 1e6f);
+    sh.lam = fmin(sh.lam * 10.0, 1e6);
   }
-  const float dchi2 = cur - ct;
-  const bool small = accept && (dchi2 < a.ftol_rel * fmaxf(cur, 1.0f));
+  const double dchi2 = cur - ct;
+  const bool small = accept && (dchi2 < double(a.ftol_rel) * fmax(cur, 1.0));
   sh.consec = small ? sh.consec + 1 : (accept ? 0 : sh.consec);
   const bool step_small = accept && pos_step < a.step_tol && shape_step < a.shape_step_tol;
   sh.nrej = accept ? 0 : sh.nrej + 1;
@@ -1042,18 +1043,18 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     // ---- island description into LDS ----
     describe_island(a, sh, a.p0);
     if (tid == 0) {
-      sh.lam = 1e-3f;
+      sh.lam = 1e-3;
       sh.consec = sh.nrej = sh.nst = 0;
       sh.conv = sh.stall = 0;
       if (sh.Nr == 0) sh.conv = 1;  // nothing to fit (all slots padded)
     }
     __syncthreads();
     if (sh.Nr > 0) {
-      const float c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, gws, G, part, sm);
+      const double c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, gws, G, part, sm);
       if (tid == 0) sh.chi2 = c0;
       __syncthreads();
     } else if (tid == 0) {
-      sh.chi2 = 0.0f;
+      sh.chi2 = 0.0;
     }
     __syncthreads();
     for (int it = 0; it < a.max_steps; ++it) {
@@ -1064,7 +1065,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
         eval_band<Real, false>(a, sh, isl, b, sh.p, true, b > 0, planes, ws, gws, G, part, sm, nullptr,
                         nullptr, nullptr);
       // damped float64 solve: (H + lam diag(H) + 1e-8 I) delta = -g
-      const double lam = double(sh.lam);
+      const double lam = sh.lam;
       for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
         const int i = e / nL, j = e % nL;
         const double v = sm.H[e];
@@ -1073,7 +1074,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
       for (int j = tid; j < nL; j += UPH_BLOCK) sm.delta[j] = sm.g[j];
       __syncthreads();
       lm_propose(a, sh, sm.Hd, sm.delta, sm.pivH);
-      const float ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, gws, G, part, sm);
+      const double ct = chi2_all<Real>(a, sh, isl, sh.ptry, planes, ws, gws, G, part, sm);
       if (tid == 0) lm_accept(a, sh, ct);
       __syncthreads();
     }
@@ -1228,13 +1229,13 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(typename EvalArg
       if (tid == 0) atomicAdd(a.ngood_out + isl, ngood);
     } else {
       const bool grads = split_grads(mode);
-      const Real c = eval_band<Real, kPre>(a, sh, isl, b, sh.p, grads, false, planes, ws, gws, G, part,
+      const double c = eval_band<Real, kPre>(a, sh, isl, b, sh.p, grads, false, planes, ws, gws, G, part,
                                      sm, nullptr, nullptr, nullptr);
       if (grads) {
         for (int e = tid; e < nL * nL; e += UPH_BLOCK) slot[e] = sm.H[e];
         for (int j = tid; j < nL; j += UPH_BLOCK) slot[(size_t)nL * nL + j] = sm.g[j];
       }
-      if (tid == 0) slot[(size_t)nL * nL + nL] = double(c);
+      if (tid == 0) slot[(size_t)nL * nL + nL] = c;
       if constexpr (kPre) {
         if (mode == kSplitEvalPost) post_band<Real>(a, sh, isl, b, planes, sm);
       }
@@ -1396,11 +1397,11 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
   const size_t c_at = (size_t)nL * nL + nL;  // chi2 entry of a band block
   if (mode == kSplitStepInit) {
     if (tid == 0) {
-      Real c = 0;
+      double c = 0.0;
       if (sh.Nr > 0)
-        for (int b = 0; b < nb; ++b) c += Real(h0[(size_t)b * blk + c_at]);
-      a.chi2_out[isl] = sh.Nr > 0 ? float(c) : 0.0f;
-      a.lam_out[isl] = 1e-3f;
+        for (int b = 0; b < nb; ++b) c += h0[(size_t)b * blk + c_at];
+      a.chi2_out[isl] = c;
+      a.lam_out[isl] = 1e-3;
       a.consec_out[isl] = 0;
       a.nrej_out[isl] = 0;
       a.nst_out[isl] = 0;
@@ -1429,7 +1430,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
   __syncthreads();
   if (mode == kSplitStepSolve) {
     // H = sum_b H_b and g = sum_b g_b in band order; damped in place in block 0
-    const double lam = double(sh.lam);
+    const double lam = sh.lam;
     for (int e = tid; e < nL * nL; e += UPH_BLOCK) {
       double v = h0[e];
       for (int b = 1; b < nb; ++b) v = v + h0[(size_t)b * blk + e];
@@ -1451,9 +1452,9 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
   for (int q = tid; q < nP; q += UPH_BLOCK) sh.ptry[q] = a.p_try[m0 * 6 + q];
   __syncthreads();
   if (tid == 0) {
-    Real c = 0;
-    for (int b = 0; b < nb; ++b) c += Real(h0[(size_t)b * blk + c_at]);
-    lm_accept(a, sh, float(c));
+    double c = 0.0;
+    for (int b = 0; b < nb; ++b) c += h0[(size_t)b * blk + c_at];
+    lm_accept(a, sh, c);
     a.chi2_out[isl] = sh.chi2;
     a.lam_out[isl] = sh.lam;
     a.consec_out[isl] = sh.consec;
@@ -1477,6 +1478,8 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_post_kernel(IslandArgsPre a)
   __shared__ IslandSh<float> sh;
   __shared__ int piv[kMaxLive];
   __shared__ int s_p;
+  __shared__ double scl[kMaxLive];   // 1 / sqrt(H_ll)
+  __shared__ double cmax[UPH_BLOCK];
   const int tid = threadIdx.x, isl = blockIdx.x, nb = a.nb;
   if (tid == 0) read_shape(a, sh, isl);
   __syncthreads();
@@ -1493,7 +1496,10 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_post_kernel(IslandArgsPre a)
   for (int e = tid; e < N * 7; e += UPH_BLOCK)
     a.post_var[(size_t)sh.moff * 7 + e] = e % 7 == 6 ? 0.0f : 1.0f;
   for (int e = tid; e < nb * N; e += UPH_BLOCK) a.post_dvar[(size_t)sh.foff + e] = 0.0f;
-  if (nL == 0 || sh.Nr == 0) return;  // uniform
+  if (nL == 0 || sh.Nr == 0) {
+    if (tid == 0) a.post_kappa[isl] = 1.0f;
+    return;  // uniform
+  }
   double* h0 = a.hg + off0;
   const double* D = a.post_buf + poff;
   double* S = a.post_buf + poff + (size_t)nb * sh.nLc * N;
@@ -1504,6 +1510,22 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_post_kernel(IslandArgsPre a)
     h0[e] = v;
   }
   __syncthreads();
+  // conditioning of the Gauss-Newton system in parameter units scaled to a
+  // unit diagonal: kappa_1 = |Hs|_1 |Hs^-1|_1, Hs = diag(s) H diag(s),
+  // s_l = H_ll^-1/2 (how much float32 model error the LM step amplifies)
+  for (int l = tid; l < nL; l += UPH_BLOCK) {
+    const double d = h0[(size_t)l * nL + l];
+    scl[l] = d > 0.0 ? 1.0 / sqrt(d) : 0.0;
+  }
+  __syncthreads();
+  double cm = 0.0;
+  for (int j = tid; j < nL; j += UPH_BLOCK) {
+    double c = 0.0;
+    for (int i = 0; i < nL; ++i) c += fabs(h0[(size_t)i * nL + j]) * scl[i];
+    cm = fmax(cm, c * scl[j]);
+  }
+  cmax[tid] = cm;
+  __syncthreads();
   block_lu(h0, nL, piv, &s_p);
   for (int j = tid; j < nL; j += UPH_BLOCK) {
     double* x = S + (size_t)j * nL;  // column j of the (symmetric) inverse
@@ -1511,6 +1533,19 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_post_kernel(IslandArgsPre a)
     lu_solve_one(h0, piv, nL, x);
   }
   __syncthreads();
+  if (tid == 0) {
+    double n1 = 0.0, n2 = 0.0;
+    bool ok = true;
+    for (int t = 0; t < UPH_BLOCK; ++t) n1 = fmax(n1, cmax[t]);
+    for (int j = 0; j < nL; ++j) {
+      if (!(scl[j] > 0.0)) ok = false;
+      double c = 0.0;
+      for (int i = 0; i < nL; ++i) c += fabs(S[(size_t)j * nL + i]) / (scl[i] * scl[j]);
+      n2 = fmax(n2, c);
+    }
+    const double k = n1 * n2;
+    a.post_kappa[isl] = ok && k == k ? float(fmin(k, 3.0e38)) : 3.0e38f;
+  }
   for (int l = tid; l < nL; l += UPH_BLOCK) {
     const int r = sh.live_r[l], q = sh.live_j[l];
     float* o = a.post_var + ((size_t)sh.moff + sh.mem[r]) * 7;
@@ -1890,7 +1925,7 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
                   "uph_island_fit_split: the bfloat16 Gram needs gram_unsettled");
     if (sp->post) {
       UPH_CHECK_ARG(sp->post_off && sp->post_buf && sp->post_var && sp->post_dvar &&
-                        sp->post_metrics,
+                        sp->post_metrics && sp->post_kappa,
                     "uph_island_fit_split: null post-fit arrays");
       pre.post_off = sp->post_off;
       pre.post_buf = sp->post_buf;
@@ -1898,6 +1933,7 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
       pre.post_var = sp->post_var;
       pre.post_dvar = sp->post_dvar;
       pre.post_metrics = sp->post_metrics;
+      pre.post_kappa = sp->post_kappa;
     }
   }
   hipStream_t s = static_cast<hipStream_t>(stream);

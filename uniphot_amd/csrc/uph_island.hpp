@@ -34,8 +34,8 @@ struct IslandArgs {
   float* p_out;
   float* flux_out;
   float* ferr_out;
-  float* chi2_out;
-  float* lam_out;
+  double* chi2_out;  // float64 LM state (accept test and damping)
+  double* lam_out;
   uint8_t* conv_out;
   uint8_t* stall_out;
   int* nst_out;
@@ -98,6 +98,7 @@ struct IslandArgsPre : IslandArgs {
   float* post_var;             // (member slots, 7) var x, y, logre, ee1, ee2, u, cov(ee1, ee2)
   float* post_dvar;            // flux layout: shape-marginalisation variance increment
   float* post_metrics;         // (member slots, n_bands, 6) per-band metric sums
+  float* post_kappa;           // (n_isl) 1-norm condition number of the diagonally scaled H
   int gram_bf16;               // island Gram on bfloat16 matrix cores (float32 accumulate)
   int* gram_unsettled;         // (n_isl) set to 1 where a bfloat16 refinement did not settle
   // large islands: Gram tile pairs computed by a separate many-workgroup launch

@@ -139,8 +139,8 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
     f32 = dict(dtype=torch.float32, device=dev)
     outs = dict(
         p=torch.empty(n_c, nP, **f32), fluxes=torch.empty(n_c, nb, N, **f32),
-        ferr=torch.empty(n_c, nb, N, **f32), chi2=torch.empty(n_c, **f32),
-        lam=torch.empty(n_c, **f32), conv=torch.empty(n_c, dtype=torch.uint8, device=dev),
+        ferr=torch.empty(n_c, nb, N, **f32), chi2=torch.empty(n_c, dtype=torch.float64, device=dev),
+        lam=torch.empty(n_c, dtype=torch.float64, device=dev), conv=torch.empty(n_c, dtype=torch.uint8, device=dev),
         stall=torch.empty(n_c, dtype=torch.uint8, device=dev),
         nst=torch.empty(n_c, dtype=torch.int32, device=dev),
         consec=torch.empty(n_c, dtype=torch.int32, device=dev),
@@ -282,7 +282,9 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     split sequence.  ``post`` (split + pre-render only) adds the post-fit
     pass: per-member parameter variances (``post_var`` (n, N, 7)), the
     shape-marginalisation flux variance increment (``post_dvar`` (n, nb, N))
-    and the per-band metric sums (``post_metrics`` (n, N, nb, 6)).
+    and the per-band metric sums (``post_metrics`` (n, N, nb, 6)), plus the
+    scaled condition number of each island's Gauss-Newton matrix
+    (``post_kappa`` (n,)).
     ``gram="bf16"`` (split + pre-render) builds the island Gram matrices on
     bfloat16 matrix cores with float32 accumulation.
     """
@@ -364,8 +366,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     f32 = dict(dtype=torch.float32, device=dev)
     outs = dict(
         p=torch.empty(m_acc * 6, **f32), fluxes=torch.empty(f_acc, **f32),
-        ferr=torch.empty(f_acc, **f32), chi2=torch.empty(n_isl, **f32),
-        lam=torch.empty(n_isl, **f32), conv=torch.empty(n_isl, dtype=torch.uint8, device=dev),
+        ferr=torch.empty(f_acc, **f32), chi2=torch.empty(n_isl, dtype=torch.float64, device=dev),
+        lam=torch.empty(n_isl, dtype=torch.float64, device=dev), conv=torch.empty(n_isl, dtype=torch.uint8, device=dev),
         stall=torch.empty(n_isl, dtype=torch.uint8, device=dev),
         nst=torch.empty(n_isl, dtype=torch.int32, device=dev),
         consec=torch.empty(n_isl, dtype=torch.int32, device=dev),
@@ -432,7 +434,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                                           device=dev),
                           var=torch.full((m_acc * 7,), float("nan"), **f32),
                           dvar=torch.full((f_acc,), float("nan"), **f32),
-                          metrics=torch.full((m_acc * nb * 6,), float("nan"), **f32))
+                          metrics=torch.full((m_acc * nb * 6,), float("nan"), **f32),
+                          kappa=torch.full((n_isl,), float("nan"), **f32))
             post_args = (1, P(post_t["off"]), P(post_t["buf"]), int(post_t["buf"].numel()),
                          P(post_t["var"]), P(post_t["dvar"]), P(post_t["metrics"]))
         # large islands: Gram on the many-workgroup launch (float32 Gram)
@@ -460,7 +463,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         sp = _lib.IslandSplit(n_isl * nb, P(d_items), P(d_bands), P(d_hgoff), P(hg), hg_n,
                               P(p_try), P(counters),
                               C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None,
-                              *gal["args"], *post_args, int(bf16_on), P(unsettled), *big_args)
+                              *gal["args"], *post_args, int(bf16_on), P(unsettled), *big_args,
+                              P(post_t["kappa"]) if post_on else None)
     else:
         evs = None
         post_on = False
@@ -590,7 +594,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 defer(lambda: _record_costs(nst_h, gh))
             else:
                 _record_costs(nst_h, gh)
-        hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics")} if post_on
+        hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics", "kappa")} if post_on
               else None)
         h_uns = unsettled.cpu().numpy() if split and bf16_on else None
         results = []
@@ -610,6 +614,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 r["post_dvar"] = hp["dvar"][f0:f0 + n_c * nb * N].reshape(n_c, nb, N)
                 r["post_metrics"] = hp["metrics"][m0 * nb * 6:(m0 + n_c * N) * nb * 6].reshape(
                     n_c, N, nb, 6)
+                r["post_kappa"] = hp["kappa"][i0:i0 + n_c]
             r["steps_run"] = int(r["nst"].max()) if n_c else 0
             r["wall_s"] = time.perf_counter() - t0
             results.append(r)
@@ -658,6 +663,72 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
     if ev is not None:
         timer.end(ev, pb.timer_info())
     return pb.collect()
+
+
+# Islands whose band-summed Gauss-Newton matrix, scaled to a unit diagonal,
+# has a 1-norm condition number above this are refitted on the float64
+# kernels: their LM steps amplify the float32 model error (about 1e-7 of
+# the model) enough to move the trajectory by more than 1e-5 of a flux.
+# Close pairs (separation below about 1 px) and flat shape directions land
+# there; single sources and resolved blends sit near 1-30.
+FP64_REFIT_KAPPA = 50.0
+_STATE_KEYS = ("p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
+               "n_good")
+_POST_KEYS = ("post_var", "post_dvar", "post_metrics", "post_kappa")
+
+
+def refit_ill_conditioned(classes, results, psf_trees: dict, max_steps: int, dstack_dev,
+                          wstack_dev, *, kappa: float = FP64_REFIT_KAPPA, sersic_dev=None,
+                          workspace_bytes: float = MAX_SCRATCH_BYTES):
+    """Refit, on the float64 kernels from the same seeds, the islands of a
+    float32 post-fit batch (``results`` of :func:`run_island_batch` with
+    ``post``) whose ``post_kappa`` exceeds ``kappa``.  Their LM state,
+    fluxes and flux errors are replaced by the float64 fit; the post-fit
+    pass is redone (float32, no LM step) at the float64 parameters.
+    Returns (results, number of islands refitted); refitted rows carry
+    ``fp64_refit`` True."""
+    import dataclasses
+    redo, sel_of = [], []
+    for (kern, host, rows), res in zip(classes, results):
+        rows = np.asarray(rows)
+        k = res.get("post_kappa")
+        sel = np.zeros(rows.size, bool) if k is None else ~(k <= kappa)  # NaN refits too
+        sel_of.append(sel)
+        if sel.any():
+            redo.append((dataclasses.replace(kern, precision="fp64"), host, rows[sel]))
+    out = []
+    for res, sel in zip(results, sel_of):
+        r = dict(res)
+        r["fp64_refit"] = sel.copy()
+        out.append(r)
+    if not redo:
+        return out, 0
+    r64 = run_island_batch(redo, psf_trees, max_steps, dstack_dev, wstack_dev,
+                           precision="fp64", sersic_dev=sersic_dev, split=False,
+                           workspace_bytes=workspace_bytes)
+    at64 = []
+    for (kern, host, rows), r in zip(redo, r64):
+        h2 = dict(host)
+        h2["p0"] = host["p0"].copy()
+        h2["p0"][rows] = r["p"]
+        at64.append((dataclasses.replace(kern, precision="fp32"), h2, rows))
+    rpost = run_island_batch(at64, psf_trees, 0, dstack_dev, wstack_dev, precision="fp32",
+                             sersic_dev=sersic_dev, split=True, prerender=True, post=True,
+                             workspace_bytes=workspace_bytes)
+    j = 0
+    for r, sel in zip(out, sel_of):
+        if not sel.any():
+            continue
+        idx = np.nonzero(sel)[0]
+        for keys, src in ((_STATE_KEYS, r64[j]), (_POST_KEYS, rpost[j])):
+            for k in keys:
+                if k in r and k in src:
+                    v = r[k].copy()
+                    v[idx] = src[k]
+                    r[k] = v
+        r["steps_run"] = max(int(r.get("steps_run", 0)), int(r64[j]["steps_run"]))
+        j += 1
+    return out, int(sum(int(x[2].size) for x in redo))
 
 
 class IslandBatchGraph:

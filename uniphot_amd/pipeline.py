@@ -30,7 +30,8 @@ from . import forced_phot_sparse as fps
 from .grid import assign_sources_to_grids, divide_image_into_grids
 from .island_plan import (DEFAULT_MAX_STEPS, JOINT_NSRC_CAP, class_host_arrays, free_n_gate,
                           plan_island_classes)
-from .island_solver import make_class_kernels, psf_tree_device, run_class_chunk
+from .island_solver import (FP64_REFIT_KAPPA, make_class_kernels, psf_tree_device,
+                           refit_ill_conditioned, run_class_chunk)
 from .islands import (ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands,
                       build_islands, flux_aware_radii, psf_radial_profile)
 from .psf_polynomial import evaluate_psf_at_np, psf_device_arrays
@@ -63,6 +64,9 @@ class FitConfig:
     gram: str = "fp32"           # island Gram operands: "fp32" or "bf16" (matrix cores)
     stamp_budget: float = 24e9   # bytes of rendered stamps per linear-solve launch
     island_workspace: float = 40 * 2 ** 30  # bytes of island-fit workgroup workspace (cap)
+    # float32 fits: islands whose scaled Gauss-Newton condition number
+    # exceeds this are refitted on the float64 kernels (None: never)
+    fp64_refit_kappa: float | None = FP64_REFIT_KAPPA
 
 
 @dataclass
@@ -488,6 +492,8 @@ class IslandSetFit:
     flux_err_marg: np.ndarray = None  # (n, nb) sqrt(flux_err^2 + shape-marginal increment)
     metrics: np.ndarray = None        # (n, 4) rchisq, fracflux, fracin, fracmasked
     flux_window: np.ndarray = None    # (n, nb) flux times the unit model's window sum
+    fp64_refit: np.ndarray = None     # (n,) bool: island refitted on the float64 kernels
+    n_fp64_islands: int = 0
 
 
 def metrics_from_sums(sums):
@@ -580,6 +586,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     ferr_m = np.full((n, nb), np.nan, np.float32)
     mets = np.full((n, 4), np.nan, np.float32)
     fwin = np.full((n, nb), np.nan, np.float32)
+    f64src = np.zeros(n, bool)
     classes, batch, trees = [], [], {}
     for plan in plans:
         host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
@@ -604,6 +611,10 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         chunks = [batch]
     plan_of = {id(h): p for p, h in classes}
     timing = []
+    # the float64 refit reads the conditioning the post-fit pass measures
+    refit_on = (cfg.fp64_refit_kappa is not None and cfg.precision == "fp32" and
+                cfg.island_split and cfg.galaxy_prerender)
+    n_fp64 = 0
     for chunk in chunks:
         if not chunk:
             continue
@@ -617,7 +628,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         results = run_island_batch(chunk, trees, cfg.max_steps, ds, ws,
                                    precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
                                    split=cfg.island_split, prerender=cfg.galaxy_prerender,
-                                   post=cfg.post_fit, gram=cfg.gram,
+                                   post=cfg.post_fit or refit_on, gram=cfg.gram,
                                    workspace_bytes=cfg.island_workspace)
         redo = [(kern, host, rows[res["gram_unsettled"]])
                 for (kern, host, rows), res in zip(chunk, results) if "gram_unsettled" in res
@@ -645,6 +656,12 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             results = merged
             n_redo = sum(int(r.size) for _, _, r in redo)
             timing.append((-n_redo, 0.0))
+        if refit_on:
+            results, n64 = refit_ill_conditioned(chunk, results, trees, cfg.max_steps, ds, ws,
+                                                 kappa=cfg.fp64_refit_kappa,
+                                                 sersic_dev=sersic_dev,
+                                                 workspace_bytes=cfg.island_workspace)
+            n_fp64 += n64
         if sync is not None:
             sync()
         timing.append((int(sum(np.asarray(r).size for _, _, r in chunk)),
@@ -669,6 +686,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             stall[src] = res["stall"][r_of]
             nst[src] = res["nst"][r_of]
             fitted[src] = True
+            if "fp64_refit" in res:
+                f64src[src] = res["fp64_refit"][r_of]
             if "post_var" in res:
                 pvar[src] = res["post_var"][mm]
                 dv = res["post_dvar"].transpose(0, 2, 1)[mm]
@@ -678,7 +697,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 fwin[src] = fw
     n_fit = sum(p.n_islands for p in plans)
     return IslandSetFit(fx, fy, flr, fe1, fe2, fsn, pfit, flux, ferr, conv, stall, nst, fitted,
-                        n_fit, classes, timing, pvar, ferr_m, mets, fwin)
+                        n_fit, classes, timing, pvar, ferr_m, mets, fwin, f64src, n_fp64)
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,
