@@ -19,8 +19,9 @@ Parity pins (tests/test_oracle_*.py):
   * Lanczos taps: sum to 1; d=0 is the identity; shift matches
     ``scipy.ndimage.correlate1d(mode="constant")``; tap derivative vs
     central finite differences.
-  * Sersic table: rows n>=0.8 equal the reference's committed table rows
-    (golden fixture) to 1e-12; Shajib amplitudes integrate to the row total.
+  * Sersic table: every row equals the reference's committed table rows
+    (golden fixture: n >= 0.8 to 1e-12, the Tractor rows below exactly);
+    Tractor rows sum to 1; Shajib amplitudes integrate to the row total.
   * Galaxy FFT render: Gaussian(1 component) x Gaussian PSF matches the
     closed-form convolved Gaussian away from the wrap; flux = total - wings.
   * Member-stamp derivatives: central finite differences in float64.
@@ -216,14 +217,21 @@ def shajib_row(n: float, K: int = K_MAX, P: int = 10):
 
 @lru_cache(maxsize=1)
 def sersic_table():
-    """Knots on a 0.005 grid over [0.8, 6.3] with Shajib rows.  The
-    reference's rows below 0.8 are The Tractor's fits (data only); lookups
-    below 0.8 clamp to the n = 0.8 mixture here (parity unpinned there)."""
+    """The reference's two-family table (sersic_mog_gen.py:45-52, 114-186):
+    The Tractor's rows for n in [0.29, 0.8], which exist only as data (read
+    from the data file the package ships, uniphot_amd/data), then Shajib rows
+    on a 0.005 grid over [0.8, 6.3] computed here."""
+    from pathlib import Path
+    low = np.load(Path(__file__).resolve().parents[1] / "uniphot_amd" / "data" /
+                  "sersic_tractor_rows.npz", allow_pickle=False)
     kn = np.linspace(0.8, SERSIC_N_MAX, int(round((SERSIC_N_MAX - 0.8) / 0.005)) + 1)
     rows = [shajib_row(float(n)) for n in kn]
-    return dict(n_knots=kn, amps=np.array([a for a, _ in rows]),
-                log_vars=np.log(np.array([v for _, v in rows])),
-                core_amp=np.zeros(kn.size), n_min=0.8, n_max=SERSIC_N_MAX)
+    return dict(n_knots=np.concatenate([low["n_knots"], kn]),
+                amps=np.concatenate([low["amps"], np.array([a for a, _ in rows])]),
+                log_vars=np.concatenate([low["log_vars"],
+                                         np.log(np.array([v for _, v in rows]))]),
+                core_amp=np.concatenate([low["core_amp"], np.zeros(kn.size)]),
+                n_min=SERSIC_N_MIN, n_max=SERSIC_N_MAX)
 
 
 def mog_params_at(n: float, grad: bool = False):

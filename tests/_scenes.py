@@ -72,9 +72,10 @@ def oracle_island(plan, host, row, d, w, psfs):
 
 
 def small_mixed_scene(seed=21, H=200, W=200, n_bands=2, n_src=16, order=2, flux_lo=5e3,
-                      flux_hi=2e5, types=(0, 1, 2, 3), re_range=(1.2, 3.0)):
+                      flux_hi=2e5, types=(0, 1, 2, 3), re_range=(1.2, 3.0),
+                      n_range=(1.0, 3.0)):
     """Scene with stars and EXP/DEV/SERSIC galaxies (Re uniform in
-    ``re_range`` px)."""
+    ``re_range`` px, SERSIC n uniform in ``n_range``)."""
     from uniphot_amd.source_model import ellipse_soft_from_re_ab_phi_np
     rng = np.random.default_rng(seed)
     fields = [synth.MixturePsfField(H, W, (1.0, 2.0, 4.0), (0.7, 0.25, 0.05), amp=0.1, rng=rng)
@@ -91,7 +92,7 @@ def small_mixed_scene(seed=21, H=200, W=200, n_bands=2, n_src=16, order=2, flux_
     lr = np.where(gal, lr, 0).astype(np.float32)
     e1 = np.where(gal, e1, 0).astype(np.float32)
     e2 = np.where(gal, e2, 0).astype(np.float32)
-    sn = np.where(st == 2, 4.0, np.where(st == 3, rng.uniform(1.0, 3.0, n), 1.0)).astype(np.float32)
+    sn = np.where(st == 2, 4.0, np.where(st == 3, rng.uniform(*n_range, n), 1.0)).astype(np.float32)
     flux = np.exp(rng.uniform(np.log(flux_lo), np.log(flux_hi), (n, 1))) * rng.uniform(
         0.5, 2.0, (1, n_bands))
     truths = [synth.render_truth(H, W, fields[b], tx, ty, st, lr, e1, e2, sn, flux[:, b])

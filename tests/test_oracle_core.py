@@ -80,11 +80,38 @@ def test_sersic_table_matches_reference_rows():
     z = np.load(GOLD / "sersic_rows.npz", allow_pickle=False)
     tb = core.sersic_table()
     for n, a, lv in zip(z["n"], z["amps"], z["log_vars"]):
-        i = int(np.argmin(np.abs(tb["n_knots"] - n)))
+        # the Shajib row of a knot listed twice (n = 0.8) is the second one
+        i = int(np.searchsorted(tb["n_knots"], n + 1e-12, side="right")) - 1
         assert abs(tb["n_knots"][i] - n) < 1e-12
         assert np.allclose(tb["amps"][i], a, rtol=0, atol=1e-12)
         assert np.allclose(tb["log_vars"][i], lv, rtol=0, atol=1e-12)
     assert np.all(z["core"] == 0.0)
+    # the Tractor family: the fixture's rows 0, 40, 80 of the reference table
+    for j, i in enumerate(range(0, 108, 40)):
+        assert tb["n_knots"][i] == z["low_n"][j]
+        assert np.array_equal(tb["amps"][i], z["low_amps"][j])
+        assert np.array_equal(tb["log_vars"][i], z["low_log_vars"][j])
+        assert tb["core_amp"][i] == z["low_core"][j]
+    assert tb["n_knots"].size == 1209 and tb["n_min"] == 0.29
+
+
+def test_sersic_tractor_rows_are_normalised_and_segmented():
+    """Tractor rows sum to 1 with the core (their own normalisation), the
+    knots rise with each segment boundary (0.42, 0.51, 0.57, 0.62, 0.71,
+    0.8) listed twice, and the product's table is the oracle's."""
+    from uniphot_amd import sersic_table
+    tb = core.sersic_table()
+    low = slice(0, 108)  # Tractor rows, through the first of the two n = 0.8 knots
+    assert tb["n_knots"][107] == 0.8 == tb["n_knots"][108]
+    tot = tb["amps"][low].sum(axis=1) + tb["core_amp"][low]
+    assert np.allclose(tot, 1.0, atol=1e-12)
+    kn = tb["n_knots"]
+    dup = kn[1:][np.diff(kn) == 0]
+    np.testing.assert_allclose(dup, [0.42, 0.51, 0.57, 0.62, 0.71, 0.8])
+    assert np.all(np.diff(kn) >= 0)
+    pt = sersic_table.table()   # the package's own generation of the same rows
+    for k in ("n_knots", "amps", "log_vars", "core_amp"):
+        np.testing.assert_allclose(pt[k], tb[k], rtol=0, atol=1e-13, err_msg=k)
 
 
 def test_mog_lookup_interpolates_and_clamps():
@@ -93,11 +120,20 @@ def test_mog_lookup_interpolates_and_clamps():
     am, vm, cm = core.mog_params_at(2.0025)
     assert np.allclose(am, 0.5 * (a0 + a1), atol=1e-14)
     assert np.allclose(np.log(vm), 0.5 * (np.log(v0) + np.log(v1)), atol=1e-12)
-    lo = core.mog_params_at(0.5)
-    at = core.mog_params_at(0.8)
+    # below the table the lookup clamps to n_min, with zero derivatives
+    lo = core.mog_params_at(0.2)
+    at = core.mog_params_at(0.29)
     assert np.array_equal(lo[0], at[0])
-    _, _, _, da, dv, dc = core.mog_params_at(0.5, grad=True)
+    _, _, _, da, dv, dc = core.mog_params_at(0.2, grad=True)
     assert np.all(da == 0) and np.all(dv == 0)
+    # a duplicated boundary knot resolves to the right segment's first row
+    tb = core.sersic_table()
+    i = int(np.searchsorted(tb["n_knots"], 0.51, side="left")) + 1
+    a51 = core.mog_params_at(0.51)
+    assert np.array_equal(a51[0], tb["amps"][i])
+    # inside a Tractor segment: linear in amplitude and log variance
+    b0, b1, bm = core.mog_params_at(0.6), core.mog_params_at(0.605), core.mog_params_at(0.6025)
+    assert np.allclose(bm[0], 0.5 * (b0[0] + b1[0]), atol=1e-14)
 
 
 def _pix_gauss(T, var, cx, cy):

@@ -1,16 +1,18 @@
 """Sersic-profile mixture-of-Gaussians table (host side) and its device block.
 
 The table format and lookup follow ``tract7dt/fit/grid_gpu/sersic_mog.py``
-(:48-100): knots in n with the segment-boundary knot duplicated, 15
+(:48-100): knots in n with each segment-boundary knot duplicated, 15
 Gaussian components (amplitude, log variance in Re^2 units) plus a delta
 core per knot, ``searchsorted(side="right") - 1`` and linear interpolation
-inside a segment.  Rows are generated here from the published Shajib (2019)
-decomposition of an untruncated Sersic profile (the method of
-``sersic_mog_gen.py:61-111``) on the 0.005 knot grid over [0.8, 6.3]; they
-equal the reference table's rows there to ~3e-14.  The reference's rows
-below n = 0.8 are fits from The Tractor that exist only as data (the
-expansion itself degrades there), so this table starts at n = 0.8 and
-lookups below it clamp to the n = 0.8 mixture.
+inside a segment.  Two families, spliced at n = 0.8 as in the reference
+(``sersic_mog_gen.py:45-52``):
+  * n in [0.29, 0.8]: The Tractor's fits, which exist only as data; they
+    are read from ``data/sersic_tractor_rows.npz`` (see ``data/NOTICE``);
+  * n in [0.8, 6.3]: generated here from the published Shajib (2019)
+    decomposition of an untruncated Sersic profile (the method of
+    ``sersic_mog_gen.py:61-111``) on the 0.005 knot grid; they equal the
+    reference table's rows to ~3e-14.
+The assembled table has the reference's 1209 knots.
 
 Device block layout (float64, see uph_galaxy.hpp):
   [0] n_knots, [1] n_min, [2] n_max, [3] K, then knots (n_knots),
@@ -19,13 +21,15 @@ Device block layout (float64, see uph_galaxy.hpp):
 from __future__ import annotations
 
 from functools import lru_cache
+from pathlib import Path
 
 import numpy as np
 from scipy.special import comb, gammaln
 
 K_MAX = 15
-N_MIN, N_MAX = 0.8, 6.3
+SPLICE_N, N_MAX = 0.8, 6.3
 DN = 0.005
+LOW_ROWS = Path(__file__).resolve().parent / "data" / "sersic_tractor_rows.npz"
 
 
 @lru_cache(maxsize=None)
@@ -73,13 +77,19 @@ def shajib_components(n: float, K: int = K_MAX, P: int = 10):
 
 @lru_cache(maxsize=1)
 def table() -> dict:
-    m = int(round((N_MAX - N_MIN) / DN)) + 1
-    kn = np.linspace(N_MIN, N_MAX, m)
+    low = np.load(LOW_ROWS, allow_pickle=False)
+    if float(low["n_knots"][-1]) != SPLICE_N or low["amps"].shape[1] != K_MAX:
+        raise ValueError(f"{LOW_ROWS}: unexpected layout")
+    m = int(round((N_MAX - SPLICE_N) / DN)) + 1
+    kn = np.linspace(SPLICE_N, N_MAX, m)
     rows = [shajib_components(float(n)) for n in kn]
     amps = np.array([a for a, _ in rows])
     lvs = np.log(np.array([v for _, v in rows]))
-    return dict(n_knots=kn, amps=amps, log_vars=lvs, core_amp=np.zeros(m),
-                n_min=N_MIN, n_max=N_MAX)
+    return dict(n_knots=np.concatenate([low["n_knots"], kn]),
+                amps=np.concatenate([low["amps"], amps]),
+                log_vars=np.concatenate([low["log_vars"], lvs]),
+                core_amp=np.concatenate([low["core_amp"], np.zeros(m)]),
+                n_min=float(low["n_min"]), n_max=N_MAX)
 
 
 def mog_params_at(n: float):

@@ -125,7 +125,7 @@ def make_tile(rng, rid, arrival, side, n_bands, bank: PsfBank, device="cuda", de
     st = np.where(u < 0.70, SOURCE_STAR, np.where(u < 0.85, SOURCE_EXP, np.where(
         u < 0.95, SOURCE_DEV, SOURCE_SERSIC))).astype(np.int32)
     sn = np.where(st == SOURCE_DEV, 4.0, np.where(st == SOURCE_SERSIC,
-                                                   rng.uniform(0.8, 6.0, n), 1.0))
+                                                   rng.uniform(0.5, 6.0, n), 1.0))
     sn = sn.astype(np.float32)
     base = np.exp(rng.uniform(np.log(1e2), np.log(1e6), n))
     flux = base[:, None] * rng.uniform(0.5, 2.0, (n, n_bands))
