@@ -16,6 +16,7 @@ usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c2-stars
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -28,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3  # MI355X dense fp32, vector and matrix alike
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense fp64, vector and matrix alike
 
 
 class KernelTimer:
@@ -161,10 +163,12 @@ def make_scene(config):
     raise SystemExit(f"unknown config {config}")
 
 
-def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0):
+def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0, oracle_cmp=None):
     """Float64 oracle timed on a bounded random sample of the same work:
     island LM fits (until ~2/3 of the budget) and grid-band linear solves
-    (the rest); extrapolated to seconds per source for the full field."""
+    (the rest); extrapolated to seconds per source for the full field.
+    ``oracle_cmp`` (a list) collects, per fitted island, (source ids, oracle
+    fluxes (n, nb), flux errors, parameters (n, 6), steps, converged)."""
     from oracle import island as oisl
     from oracle import core as ocore
     from oracle import linear as olin
@@ -201,8 +205,12 @@ def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0):
                           host["x0b"][r], host["y0b"][r], host["st"][r], host["sn"][r],
                           host["fn_flag"][r], host["mmask"][r], host["keep"][r],
                           host["lower"][r].ravel(), host["upper"][r].ravel(), ps, T)
-        st = oisl.run_island(host["p0"][r].ravel().astype(np.float64), isl, o)
-        oisl.final_fluxes(st["p"], isl, o)
+        st = oisl.run_island(host["p0"][r], isl, o)
+        f, e = oisl.final_fluxes(st["p"], isl, o)
+        if oracle_cmp is not None:
+            mm = host["mmask"][r]
+            oracle_cmp.append((plan.midx[r][mm], f[:, mm].T, e[:, mm].T,
+                               st["p"].reshape(-1, 6)[mm], st["nst"], bool(st["conv"])))
         n_src_isl += int(host["mmask"][r].sum())
         n_isl += 1
     t_isl = time.perf_counter() - t0
@@ -245,6 +253,44 @@ def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0):
                         f"extrapolated per source (islands + 2 passes x {fld.nb} bands)"))
 
 
+def oracle_accuracy(res, cmp):
+    """GPU island fit (island VarPro fluxes and positions of ``res``)
+    against the float64 oracle's fits of the same islands from the CPU
+    baseline's sample: per source, max over bands of |f - f_o| /
+    max(|f_o|, sigma_o), position error in px, and the share of islands
+    with the oracle's step count."""
+    from uniphot_amd.pipeline import _rel_stats
+    if not cmp:
+        return dict(n=0)
+    src = np.concatenate([c[0] for c in cmp])
+    fo = np.concatenate([c[1] for c in cmp])
+    eo = np.concatenate([c[2] for c in cmp])
+    po = np.concatenate([c[3] for c in cmp])
+    st = _rel_stats(res.island_flux[src].astype(np.float64), fo, eo)
+    st["position_max_px"] = float(np.max(np.hypot(res.x[src] - po[:, 0], res.y[src] - po[:, 1])))
+    same = [int(np.all(res.island_nst[c[0]] == c[4])) for c in cmp]
+    st["same_steps"] = float(np.mean(same))
+    st["islands"] = len(cmp)
+    return st
+
+
+def field_accuracy(fld, res, cat, cfg, n_sample, sync, seed=2):
+    """Float32 island fit (with its float64 refit) against the float64
+    kernels on ``n_sample`` random fitted islands of the field."""
+    from uniphot_amd.pipeline import island_accuracy
+    art = res.record["_artifacts"]
+    iset = art["island_set"]
+    rng = np.random.default_rng(seed)
+    fit_ids = np.setdiff1d(np.arange(iset.n_islands), art["giants"])
+    sample = np.sort(rng.choice(fit_ids, min(n_sample, fit_ids.size), replace=False))
+    acc = island_accuracy(fld.psf, fld.nb, fld.n_terms, fld.native, art["dstack"],
+                          art["wstack"], art["pad"], iset, sample, cat, art["free_n"], cfg,
+                          sync=sync, scene=art["scene"],
+                          flux_scale=(res.s1_flux, np.sqrt(np.maximum(res.s1_var, 0.0))))
+    acc.update(acc.pop("fp32"))
+    return acc
+
+
 def roofline_of(ks, config):
     """Dominant kernel (most total time; the split sequence's own record is
     for reference only) placed against the HBM or fp32 roof it is closer to."""
@@ -254,11 +300,12 @@ def roofline_of(ks, config):
     per_launch_bytes = isl["bytes"] / max(isl["launches"], 1)
     achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
     tflops = isl["flops"] / max(isl["ms"] / 1e3, 1e-12) / 1e12
-    compute = tflops / FP32_PEAK_TFLOPS > achieved / HBM_PEAK_GBS
+    peak = FP64_PEAK_TFLOPS if kern.endswith("_fp64") else FP32_PEAK_TFLOPS
+    compute = tflops / peak > achieved / HBM_PEAK_GBS
     traffic = measured_traffic(config, kern)
     if compute:
-        roof = {"bound": "mfma", "achieved": round(tflops, 4), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(tflops / FP32_PEAK_TFLOPS, 6),
+        roof = {"bound": "mfma", "achieved": round(tflops, 4), "peak": peak,
+                "unit": "TFLOP/s", "frac": round(tflops / peak, 6),
                 "traffic": traffic, "hbm_gbs": round(achieved, 3),
                 "hbm_frac": round(achieved / HBM_PEAK_GBS, 6)}
     else:
@@ -458,7 +505,8 @@ def bench_precision(args, torch):
     sample = np.sort(rng.choice(fit_ids, min(args.sample, fit_ids.size), replace=False))
     acc = island_accuracy(fld.psf, fld.nb, fld.n_terms, fld.native, art["dstack"],
                           art["wstack"], art["pad"], iset, sample, cat, art["free_n"], base,
-                          grams=("bf16", "fp32"), sync=sync, scene=art["scene"])
+                          grams=("bf16", "fp32"), sync=sync, scene=art["scene"],
+                          flux_scale=(res.s1_flux, np.sqrt(np.maximum(res.s1_var, 0.0))))
     print(f"[bench] accuracy {acc}", file=sys.stderr, flush=True)
     ks = timer.summary()
     line = {
@@ -493,7 +541,7 @@ def main():
     ap.add_argument("--inflight", type=int, default=4, help="c5: tiles in flight at most")
     ap.add_argument("--side", type=int, default=8192, help="c4: field side (px)")
     ap.add_argument("--sample", type=int, default=300,
-                    help="c3: islands of the FP64 accuracy and persistent-form comparisons")
+                    help="islands of the FP64 accuracy comparison (c2, c3, c4; 0 = none)")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--island-split", type=int, default=1,
@@ -567,8 +615,37 @@ def main():
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                         for kk, vv in v.items()} for k, v in ks.items()},
     }
+    if args.sample > 0:
+        t_acc = time.perf_counter()
+        cat = (sc.x, sc.y, sc.stype, sc.logre, sc.ee1, sc.ee2, sc.sersic_n)
+        acc = field_accuracy(fld, res, cat, cfg, args.sample, sync)
+        frac = acc.get("fit", {}).get("frac_le_1e5", float("nan"))
+        acc["value_within_1e5"] = round(value * frac, 2)
+        line["accuracy_vs_fp64"] = acc
+        print(f"[bench] accuracy vs fp64 ({time.perf_counter() - t_acc:.1f}s) {acc}",
+              file=sys.stderr, flush=True)
+        # the same fit with every island on the float32 kernels (no float64
+        # routing or refit): its throughput and its accuracy on the sample
+        c32 = dataclasses.replace(cfg, fp64_contrast=None, fp64_refit_kappa=None)
+        sync()
+        t32 = time.perf_counter()
+        r32 = fit_field(fld, *args_fit[:-1], c32, sync=sync, keep_artifacts=True)
+        sync()
+        el32 = time.perf_counter() - t32
+        a32 = field_accuracy(fld, r32, cat, c32, args.sample, sync)
+        v32 = n / el32
+        line["fp32_only"] = {"value": round(v32, 2), "ms_per_step": round(el32 * 1e3, 2),
+                             "stage_s": {k: round(v, 4) for k, v in r32.timings.items()},
+                             "accuracy_vs_fp64": a32,
+                             "value_within_1e5": round(
+                                 v32 * a32.get("fit", {}).get("frac_le_1e5", float("nan")), 2)}
+        print(f"[bench] fp32 only {line['fp32_only']}", file=sys.stderr, flush=True)
+    line["config"]["fp64_refit_islands"] = res.record["n_fp64_refit_islands"]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(fld, res, sc, cfg, budget_s=args.cpu_budget)
+        cmp = []
+        line["cpu_baseline"] = cpu_baseline(fld, res, sc, cfg, budget_s=args.cpu_budget,
+                                            oracle_cmp=cmp)
+        line["accuracy_vs_oracle"] = oracle_accuracy(res, cmp)
     print(json.dumps(line), flush=True)
 
 

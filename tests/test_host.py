@@ -269,3 +269,17 @@ def test_mosaic_shelves_are_disjoint():
         img[y:y + s, x:x + s] += 1
     assert img.max() == 1
     assert img.sum() / img.size > 0.7      # shelves of sorted sides pack densely
+
+
+def test_every_module_imports_without_a_gpu():
+    """Every package module, the bench and the entry points import on a
+    machine with no GPU (the library is loaded lazily)."""
+    import importlib
+    import pkgutil
+    import uniphot_amd
+    for m in pkgutil.iter_modules(uniphot_amd.__path__):
+        if m.name.startswith("lib"):
+            continue  # the HIP library itself (a C ABI loaded with ctypes)
+        importlib.import_module(f"uniphot_amd.{m.name}")
+    importlib.import_module("bench")
+    importlib.import_module("__graft_entry__")

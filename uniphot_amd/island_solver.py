@@ -569,7 +569,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                                   bytes=float(box[bsel].sum()),
                                   flops=float(gram_fl[bsel].sum()), n_isl=int(bsel.sum())))
                 ev_fl = np.where(bsel, ev_fl - gram_fl, ev_fl)
-            timer.record("island_eval_kernel", e5[4], e5[5],
+            timer.record("island_eval_kernel_fp64" if precision == "fp64" else
+                         "island_eval_kernel", e5[4], e5[5],
                          dict(kind="island_eval", phase=kind, bytes=float(box[sel].sum()),
                               flops=float(ev_fl[sel].sum()), n_isl=int(sel.sum())))
 
@@ -677,17 +678,49 @@ _STATE_KEYS = ("p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "co
 _POST_KEYS = ("post_var", "post_dvar", "post_metrics", "post_kappa")
 
 
+def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
+                sersic_dev=None, post: bool = True,
+                workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None):
+    """Fit ``classes`` (kernels, host, rows) on the float64 kernels
+    (band-split, members rendered in place), then, with ``post``, the
+    float32 post-fit pass (no LM step) at the float64 parameters for the
+    parameter variances, marginal flux errors, metrics and conditioning.
+    Fluxes, flux errors and the LM state are the float64 fit's.  Every row
+    carries ``fp64_refit`` True."""
+    import dataclasses
+    if not classes:
+        return []
+    k64 = [(dataclasses.replace(k, precision="fp64"), h, np.asarray(r)) for k, h, r in classes]
+    r64 = run_island_batch(k64, psf_trees, max_steps, dstack_dev, wstack_dev,
+                           precision="fp64", sersic_dev=sersic_dev, split=True,
+                           prerender=False, workspace_bytes=workspace_bytes, timer=timer)
+    if post:
+        at64 = []
+        for (kern, host, rows), r in zip(k64, r64):
+            h2 = dict(host)
+            h2["p0"] = host["p0"].copy()
+            h2["p0"][rows] = r["p"]
+            at64.append((dataclasses.replace(kern, precision="fp32"), h2, rows))
+        rpost = run_island_batch(at64, psf_trees, 0, dstack_dev, wstack_dev, precision="fp32",
+                                 sersic_dev=sersic_dev, split=True, prerender=True, post=True,
+                                 workspace_bytes=workspace_bytes)
+        for r, rp in zip(r64, rpost):
+            for k in _POST_KEYS:
+                if k in rp:
+                    r[k] = rp[k]
+    for (_, _, rows), r in zip(k64, r64):
+        r["fp64_refit"] = np.ones(rows.size, bool)
+    return r64
+
+
 def refit_ill_conditioned(classes, results, psf_trees: dict, max_steps: int, dstack_dev,
                           wstack_dev, *, kappa: float = FP64_REFIT_KAPPA, sersic_dev=None,
                           workspace_bytes: float = MAX_SCRATCH_BYTES):
     """Refit, on the float64 kernels from the same seeds, the islands of a
     float32 post-fit batch (``results`` of :func:`run_island_batch` with
-    ``post``) whose ``post_kappa`` exceeds ``kappa``.  Their LM state,
-    fluxes and flux errors are replaced by the float64 fit; the post-fit
-    pass is redone (float32, no LM step) at the float64 parameters.
-    Returns (results, number of islands refitted); refitted rows carry
-    ``fp64_refit`` True."""
-    import dataclasses
+    ``post``) whose ``post_kappa`` exceeds ``kappa`` (see
+    :func:`fit_float64`).  Returns (results, number of islands refitted);
+    every row carries ``fp64_refit``."""
     redo, sel_of = [], []
     for (kern, host, rows), res in zip(classes, results):
         rows = np.asarray(rows)
@@ -695,37 +728,26 @@ def refit_ill_conditioned(classes, results, psf_trees: dict, max_steps: int, dst
         sel = np.zeros(rows.size, bool) if k is None else ~(k <= kappa)  # NaN refits too
         sel_of.append(sel)
         if sel.any():
-            redo.append((dataclasses.replace(kern, precision="fp64"), host, rows[sel]))
+            redo.append((kern, host, rows[sel]))
     out = []
     for res, sel in zip(results, sel_of):
         r = dict(res)
-        r["fp64_refit"] = sel.copy()
+        r["fp64_refit"] = np.zeros(sel.size, bool)
         out.append(r)
     if not redo:
         return out, 0
-    r64 = run_island_batch(redo, psf_trees, max_steps, dstack_dev, wstack_dev,
-                           precision="fp64", sersic_dev=sersic_dev, split=False,
-                           workspace_bytes=workspace_bytes)
-    at64 = []
-    for (kern, host, rows), r in zip(redo, r64):
-        h2 = dict(host)
-        h2["p0"] = host["p0"].copy()
-        h2["p0"][rows] = r["p"]
-        at64.append((dataclasses.replace(kern, precision="fp32"), h2, rows))
-    rpost = run_island_batch(at64, psf_trees, 0, dstack_dev, wstack_dev, precision="fp32",
-                             sersic_dev=sersic_dev, split=True, prerender=True, post=True,
-                             workspace_bytes=workspace_bytes)
+    r64 = fit_float64(redo, psf_trees, max_steps, dstack_dev, wstack_dev,
+                      sersic_dev=sersic_dev, workspace_bytes=workspace_bytes)
     j = 0
     for r, sel in zip(out, sel_of):
         if not sel.any():
             continue
         idx = np.nonzero(sel)[0]
-        for keys, src in ((_STATE_KEYS, r64[j]), (_POST_KEYS, rpost[j])):
-            for k in keys:
-                if k in r and k in src:
-                    v = r[k].copy()
-                    v[idx] = src[k]
-                    r[k] = v
+        for k in _STATE_KEYS + _POST_KEYS + ("fp64_refit",):
+            if k in r and k in r64[j]:
+                v = r[k].copy()
+                v[idx] = r64[j][k]
+                r[k] = v
         r["steps_run"] = max(int(r.get("steps_run", 0)), int(r64[j]["steps_run"]))
         j += 1
     return out, int(sum(int(x[2].size) for x in redo))

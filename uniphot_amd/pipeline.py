@@ -30,13 +30,17 @@ from . import forced_phot_sparse as fps
 from .grid import assign_sources_to_grids, divide_image_into_grids
 from .island_plan import (DEFAULT_MAX_STEPS, JOINT_NSRC_CAP, class_host_arrays, free_n_gate,
                           plan_island_classes)
-from .island_solver import (FP64_REFIT_KAPPA, make_class_kernels, psf_tree_device,
-                           refit_ill_conditioned, run_class_chunk)
+from .island_solver import (FP64_REFIT_KAPPA, fit_float64, make_class_kernels,
+                           psf_tree_device, refit_ill_conditioned, run_class_chunk)
 from .islands import (ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands,
                       build_islands, flux_aware_radii, psf_radial_profile)
 from .psf_polynomial import evaluate_psf_at_np, psf_device_arrays
 from .scene import SceneStars, island_mosaic, mosaic_bytes, star_flux, subtract_stars
 from .source_model import SOURCE_STAR, sersic_u_to_n
+
+
+# float32 fits send islands above this flux contrast to the float64 kernels
+FP64_CONTRAST = 4.0
 
 
 @dataclass
@@ -67,6 +71,12 @@ class FitConfig:
     # float32 fits: islands whose scaled Gauss-Newton condition number
     # exceeds this are refitted on the float64 kernels (None: never)
     fp64_refit_kappa: float | None = FP64_REFIT_KAPPA
+    # float32 fits: islands whose flux contrast (the brightest member's flux
+    # over the faintest member's max(|flux|, sigma), in any band, from the
+    # pass-1 solve) exceeds this are fitted on the float64 kernels, where
+    # float32 model error (~1e-7 of the bright member's light) would move
+    # the faint members by more than 1e-5 (None: never)
+    fp64_contrast: float | None = FP64_CONTRAST
 
 
 @dataclass
@@ -494,6 +504,7 @@ class IslandSetFit:
     flux_window: np.ndarray = None    # (n, nb) flux times the unit model's window sum
     fp64_refit: np.ndarray = None     # (n,) bool: island refitted on the float64 kernels
     n_fp64_islands: int = 0
+    kappa: np.ndarray = None          # (n,) the island's scaled Gauss-Newton condition number
 
 
 def metrics_from_sums(sums):
@@ -556,10 +567,28 @@ def _chunk_batch(batch, nb, budget_bytes, mosaic=False):
     return chunks
 
 
+def island_contrast(plan, rows, flux, sigma) -> np.ndarray:
+    """Per island row of a class: max over bands of the brightest member's
+    |flux| over the faintest member's max(|flux|, sigma); 1 for a single
+    member.  ``flux``, ``sigma`` (n_sources, n_bands); NaN counts as 0 flux
+    and infinite sigma."""
+    rows = np.asarray(rows)
+    mm = plan.mmask[rows].astype(bool)
+    idx = np.where(mm, plan.midx[rows], 0)
+    f = np.abs(np.nan_to_num(np.asarray(flux, np.float64)[idx], nan=0.0))
+    s = np.nan_to_num(np.asarray(sigma, np.float64)[idx], nan=np.inf)
+    den = np.where(mm[:, :, None], np.maximum(f, s), np.inf)
+    num = np.where(mm[:, :, None], f, 0.0)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        c = num.max(axis=1) / den.min(axis=1)
+    c = np.where(np.isfinite(c), c, 1.0)
+    return np.maximum(c.max(axis=1), 1.0) if c.size else np.ones(rows.size)
+
+
 def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans, x, y, stype,
                    logre, ee1, ee2, sn, free_n, cfg: FitConfig, timer=None,
                    budget_bytes: float = 48e9, sync=None, p_start=None,
-                   scene: SceneStars | None = None) -> IslandSetFit:
+                   scene: SceneStars | None = None, flux_scale=None) -> IslandSetFit:
     """Island VarPro-LM over the planned classes of an island set
     (island_solver.py:3233-3595, one device): class host arrays, the device
     Sersic table and PSF trees, then the band-split fit in launch sequences
@@ -569,7 +598,9 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     classes (stamp tiers, bounds) were planned from the catalog seeds.  With
     ``scene`` (scene subtraction), dstack already has every catalogued star
     subtracted at its S1 flux and each launch sequence fits a mosaic of its
-    island cutouts with the members' own stars added back."""
+    island cutouts with the members' own stars added back.  ``flux_scale``
+    = (flux, sigma) per source and band (the pass-1 solve) routes the
+    islands above ``cfg.fp64_contrast`` to the float64 kernels."""
     from .island_solver import PsfTree, run_island_batch
     n = x.size
     fx, fy = x.copy(), y.copy()
@@ -587,6 +618,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     mets = np.full((n, 4), np.nan, np.float32)
     fwin = np.full((n, nb), np.nan, np.float32)
     f64src = np.zeros(n, bool)
+    kap = np.full(n, np.nan, np.float32)
     classes, batch, trees = [], [], {}
     for plan in plans:
         host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
@@ -614,6 +646,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     # the float64 refit reads the conditioning the post-fit pass measures
     refit_on = (cfg.fp64_refit_kappa is not None and cfg.precision == "fp32" and
                 cfg.island_split and cfg.galaxy_prerender)
+    route_on = (refit_on and cfg.fp64_contrast is not None and flux_scale is not None)
     n_fp64 = 0
     for chunk in chunks:
         if not chunk:
@@ -625,6 +658,17 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             for (_, h2, _), (_, h, _) in zip(moved, chunk):
                 plan_of[id(h2)] = plan_of[id(h)]
             chunk = moved
+        part64 = []
+        if route_on:
+            part32 = []
+            for kern, host, rows in chunk:
+                rows = np.asarray(rows)
+                hi = island_contrast(plan_of[id(host)], rows, *flux_scale) > cfg.fp64_contrast
+                if (~hi).any():
+                    part32.append((kern, host, rows[~hi]))
+                if hi.any():
+                    part64.append((kern, host, rows[hi]))
+            chunk = part32
         results = run_island_batch(chunk, trees, cfg.max_steps, ds, ws,
                                    precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
                                    split=cfg.island_split, prerender=cfg.galaxy_prerender,
@@ -662,6 +706,12 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                                                  sersic_dev=sersic_dev,
                                                  workspace_bytes=cfg.island_workspace)
             n_fp64 += n64
+        if part64:
+            results = results + fit_float64(part64, trees, cfg.max_steps, ds, ws,
+                                             sersic_dev=sersic_dev, post=cfg.post_fit,
+                                             workspace_bytes=cfg.island_workspace, timer=timer)
+            chunk = chunk + part64
+            n_fp64 += sum(int(r.size) for _, _, r in part64)
         if sync is not None:
             sync()
         timing.append((int(sum(np.asarray(r).size for _, _, r in chunk)),
@@ -688,6 +738,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             fitted[src] = True
             if "fp64_refit" in res:
                 f64src[src] = res["fp64_refit"][r_of]
+            if "post_kappa" in res:
+                kap[src] = res["post_kappa"][r_of]
             if "post_var" in res:
                 pvar[src] = res["post_var"][mm]
                 dv = res["post_dvar"].transpose(0, 2, 1)[mm]
@@ -697,7 +749,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 fwin[src] = fw
     n_fit = sum(p.n_islands for p in plans)
     return IslandSetFit(fx, fy, flr, fe1, fe2, fsn, pfit, flux, ferr, conv, stall, nst, fitted,
-                        n_fit, classes, timing, pvar, ferr_m, mets, fwin, f64src, n_fp64)
+                        n_fit, classes, timing, pvar, ferr_m, mets, fwin, f64src, n_fp64, kap)
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,
@@ -759,7 +811,7 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     t2 = time.perf_counter()
     isf = fit_island_set(fld.psf, fld.nb, fld.n_terms, fld.native, dstack, wstack, pad, iset,
                          plans, x, y, stype, logre, ee1, ee2, sn, free_n, cfg, timer=timer,
-                         scene=scene)
+                         scene=scene, flux_scale=(s1_flux, np.sqrt(np.maximum(s1_var, 0.0))))
     fx, fy, flr, fe1, fe2 = isf.x, isf.y, isf.logre, isf.ee1, isf.ee2
     fsn = isf.sersic_n
     conv, stall, nst = isf.conv, isf.stall, isf.nst
@@ -778,7 +830,8 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     tm["pass2"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
     rec = dict(n_sources=n, n_islands=int(iset.n_islands), n_fitted_islands=n_fitted_islands,
-               n_giant_islands=int(giants.size), classes=[
+               n_giant_islands=int(giants.size), n_fp64_refit_islands=int(isf.n_fp64_islands),
+               classes=[
                    (p.bbox_class, p.nsrc_class, p.kind, p.tier, p.n_islands) for p in plans],
                knobs_off=["priors"] + ([] if cfg.scene_subtract else ["scene_subtract"])
                + ([] if cfg.blank_response else ["blank_response_correction"])
@@ -832,7 +885,17 @@ def fit_cutouts(cut, cfg: FitConfig, timer=None, sync=None, joint_nsrc_cap=None,
     return fit_island_set(psf_of, cut.n_bands, int(cut.psfs[0].n_terms), native, cut.dstack,
                           cut.wstack, cut.pad, iset, plans, cut.x, cut.y, cut.stype, cut.logre,
                           cut.ee1, cut.ee2, cut.sersic_n, free_n, cfg, timer=timer,
-                          budget_bytes=budget_bytes, sync=sync, p_start=p_start)
+                          budget_bytes=budget_bytes, sync=sync, p_start=p_start,
+                          flux_scale=cutout_flux_scale(cut))
+
+
+def cutout_flux_scale(cut):
+    """(flux, sigma) per source and band of island cutouts for the float64
+    routing: with no linear pass before the fit, the true flux and the
+    sky-limited noise of a 50 px^2 aperture (the free-n gate's S/N)."""
+    var = (np.asarray(cut.sky, np.float64) + 25.0)[None, :] * 50.0
+    f = np.asarray(cut.true_flux, np.float64)
+    return f, np.sqrt(np.broadcast_to(var, f.shape))
 
 
 def _rel_stats(f, ref, ref_err):
@@ -856,7 +919,8 @@ def sub_island_set(iset, island_ids):
 
 def island_accuracy(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, island_ids, cat,
                     free_n, cfg: FitConfig, grams=("fp32",), sync=None,
-                    joint_nsrc_cap=JOINT_NSRC_CAP, scene: SceneStars | None = None) -> dict:
+                    joint_nsrc_cap=JOINT_NSRC_CAP, scene: SceneStars | None = None,
+                    flux_scale=None) -> dict:
     """Flux error of float32 island fits (one per Gram operand precision in
     ``grams``) against the float64 fit of the same islands, per source as
     max over bands of |f - f64| / max(|f64|, sigma64):
@@ -866,7 +930,9 @@ def island_accuracy(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, isla
       * ``flux_solve``: the flux solve (no LM step) at the float64 fit's
         parameters -- the arithmetic precision of the fluxes.
     ``cat`` = (x, y, stype, logre, ee1, ee2, sersic_n) catalog seeds;
-    ``scene`` as for :func:`fit_island_set` (dstack then scene-subtracted)."""
+    ``scene`` and ``flux_scale`` as for :func:`fit_island_set` (the float32
+    fits then route their high-contrast islands to the float64 kernels, as
+    the field fit does)."""
     import dataclasses
     x, y, st, lr, e1, e2, sn = cat
     sub = sub_island_set(iset, island_ids)
@@ -877,7 +943,7 @@ def island_accuracy(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, isla
     def run(c, p_start=None):
         return fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, sub, plans, x,
                               y, st, lr, e1, e2, sn, free_n, c, sync=sync, p_start=p_start,
-                              scene=scene)
+                              scene=scene, flux_scale=flux_scale)
     c64 = dataclasses.replace(cfg, precision="fp64", gram="fp32", post_fit=False)
     f64 = run(c64)
     fz64 = run(dataclasses.replace(c64, max_steps=0), f64.p)
@@ -911,7 +977,8 @@ def cutout_accuracy(cut, cfg: FitConfig, island_ids, sync=None, psf_of=None,
                           int(cut.psfs[0].H_psf), cut.dstack, cut.wstack, cut.pad,
                           cut.island_set, island_ids, (cut.x, cut.y, cut.stype, cut.logre,
                                                        cut.ee1, cut.ee2, cut.sersic_n),
-                          free_n, cfg, grams=grams, sync=sync, joint_nsrc_cap=None)
+                          free_n, cfg, grams=grams, sync=sync, joint_nsrc_cap=None,
+                          flux_scale=cutout_flux_scale(cut))
     if tuple(grams) == ("fp32",):
         acc.update(acc.pop("fp32"))   # flat layout for the single-precision comparison
     return acc
