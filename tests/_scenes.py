@@ -6,7 +6,7 @@ import numpy as np
 from oracle import island as oisl
 from uniphot_amd import synth
 from uniphot_amd.island_plan import class_host_arrays, free_n_gate, plan_island_classes
-from uniphot_amd.islands import bucket_islands, build_islands
+from uniphot_amd.islands import ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands, build_islands
 from uniphot_amd.source_model import SOURCE_STAR
 
 
@@ -42,7 +42,7 @@ def star_classes(sc, link_radius=6.0, pos_max_shift=5.0):
     n = sc.x.size
     iset = build_islands(sc.x, sc.y, sc.stype, sc.logre, sc.sersic_n, 27.5,
                          radius=np.full(n, link_radius))
-    bk = bucket_islands(iset)
+    bk = bucket_islands(iset, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
     plans, _ = plan_island_classes(iset, bk, sc.stype, sc.logre, sc.sersic_n, 55)
     pad = max(p.bbox_class for p in plans) + 8
     nb, H, W = sc.data.shape
@@ -108,7 +108,7 @@ def mixed_classes(sc, link_radius=7.0, pos_max_shift=5.0, free_n=None):
     n = sc.x.size
     iset = build_islands(sc.x, sc.y, sc.stype, sc.logre, sc.sersic_n, 27.5,
                          radius=np.full(n, link_radius))
-    bk = bucket_islands(iset)
+    bk = bucket_islands(iset, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
     plans, _ = plan_island_classes(iset, bk, sc.stype, sc.logre, sc.sersic_n, 55)
     pad = max(p.bbox_class for p in plans) + 8
     nb, H, W = sc.data.shape

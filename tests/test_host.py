@@ -169,7 +169,7 @@ def test_bucketing_picks_the_smallest_class_or_giant():
     pts = np.concatenate([par[i] + rng.normal(0, 2 + k[i] / 4, (k[i], 2)) for i in range(400)])
     r = np.full(len(pts), 4.0)
     iset = build_islands(pts[:, 0], pts[:, 1], None, None, None, 27.5, radius=r)
-    bk = bucket_islands(iset)
+    bk = bucket_islands(iset, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
     seen = np.zeros(iset.n_islands, int)
     for (B, N), b in bk.buckets.items():
         for row, i in enumerate(b.island_ids):
@@ -213,7 +213,7 @@ def test_stamp_tier_covers_twice_the_profile_extent():
     sn = np.where(st == 2, 4.0, np.where(st == 3, rng.uniform(0.5, 6, n), 1.0))
     lr = np.log(rng.uniform(1, 12, n))
     iset = build_islands(px, py, st, lr, sn, 27.5, radius=np.full(n, 20.0))
-    plans, capped = plan_island_classes(iset, bucket_islands(iset), st, lr, sn, 55)
+    plans, capped = plan_island_classes(iset, bucket_islands(iset, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES), st, lr, sn, 55)
     for p in plans:
         for row in range(p.n_islands):
             m = p.midx[row][p.mmask[row]]

@@ -1,84 +1,126 @@
-"""Island decomposition (host, NumPy/SciPy).
+"""Island decomposition: which catalogue sources are fitted jointly.
 
-Follows ``tract7dt/fit/grid_gpu/islands.py``: coupling radii (:68-103),
-PSF radial profile (:128-157), flux-aware radii (:234-302), connected
-components of the coupling graph with labels ordered by minimum member
-index and half-open integer bboxes (:423-550), and (bbox, nsrc) class
-bucketing (:614-663).  Island classes used by the island stage:
-``island_stage.py:133-134``.
+Two sources are coupled when their coupling disks overlap (distance below
+the sum of their radii); islands are the connected groups of coupled
+sources.  The results must equal the reference's bit for bit (membership
+decides every fit), so the radius rule, island order and box convention
+are the reference's (``tract7dt/fit/grid_gpu/islands.py``):
+  * geometric radius (:68-103): r_psf for stars, nre(n) * clip(Re) + r_psf
+    for extended sources;
+  * flux-aware radius (:234-302): where flux * unit profile falls to
+    kappa * sigma_sky -- the PSF's annulus profile (:128-179) for stars,
+    an analytic Sersic falloff for extended sources -- floored at the PSF
+    FWHM and capped at the geometric radius;
+  * islands numbered by their smallest member index, members ascending,
+    half-open integer boxes over the member disks (:423-550);
+  * the smallest (box side, member count) class from the island stage's
+    ladders (``island_stage.py:133-134``), larger islands "giant"
+    (:614-663).
+The search and grouping below are this module's own: sources split into a
+small-radius population (pairs from one kd-tree pair query at twice the
+population's largest radius) and the few large-radius sources (a ball query
+each); components by min-root union-find with pointer jumping, whose roots
+are the smallest member indices.
 """
 from __future__ import annotations
 
 from dataclasses import dataclass, field
 
 import numpy as np
-from scipy import sparse
-from scipy.sparse.csgraph import connected_components
 from scipy.spatial import cKDTree
 
 from .source_model import SOURCE_STAR
 
 RE_MIN, RE_MAX = 0.3, 200.0
-SERSIC_B1 = 1.6783469900166605
+SERSIC_B1 = 1.6783469900166605          # exact b for n = 1
+# the island stage's class ladders (island_stage.py:133-134) and the
+# bucketing function's own defaults (islands.py:59-60)
 ISLAND_BBOX_CLASSES = (32, 48, 64, 96, 128, 192, 256, 384)
 ISLAND_NSRC_CLASSES = (1, 2, 4, 8, 16, 32, 64)
+DEFAULT_BBOX_CLASSES = (64, 96, 128, 192, 256, 384)
+DEFAULT_NSRC_CLASSES = (1, 2, 4, 8, 16, 32, 64, 128)
+# share of sources (by radius) searched by the pair query; the rest by ball
+# queries of their own
+_SMALL_RADIUS_QUANTILE = 0.95
 
 
 def nre_multiplier(sersic_n) -> np.ndarray:
+    """Radius multiple of Re that bounds an extended source's light: 4 below
+    n = 1.5, 6 below n = 3, 8 above (non-finite n counts as 1)."""
     n = np.asarray(sersic_n, np.float64)
     n = np.where(np.isfinite(n), n, 1.0)
-    return np.where(n < 1.5, 4.0, np.where(n < 3.0, 6.0, 8.0))
+    return np.select([n < 1.5, n < 3.0], [4.0, 6.0], 8.0)
+
+
+def _clean_re(logre) -> np.ndarray:
+    re = np.exp(np.asarray(logre, np.float64))
+    return np.clip(np.where(np.isfinite(re), re, RE_MIN), RE_MIN, RE_MAX)
 
 
 def coupling_radii(source_type, logre, sersic_n, r_psf: float) -> np.ndarray:
-    """Stars r_psf; extended nre(n) * clip(Re) + r_psf."""
-    if r_psf <= 0:
+    """Geometric coupling radius per source."""
+    if not r_psf > 0:
         raise ValueError(f"r_psf must be positive, got {r_psf}")
-    re = np.exp(np.asarray(logre, np.float64))
-    re = np.clip(np.where(np.isfinite(re), re, RE_MIN), RE_MIN, RE_MAX)
-    ext = nre_multiplier(sersic_n) * re + float(r_psf)
-    return np.where(np.asarray(source_type) == SOURCE_STAR, float(r_psf), ext)
+    star = np.asarray(source_type) == SOURCE_STAR
+    ext = nre_multiplier(sersic_n) * _clean_re(logre) + float(r_psf)
+    return np.where(star, float(r_psf), ext)
+
+
+@dataclass
+class PsfProfile:
+    """Unit-flux annulus profile of a centred PSF stamp: p[k] is the mean
+    of the normalised stamp over pixels whose centre distance falls in bin
+    k of width ``bin_width``, taken at the bin centre, floored at 1e-12 and
+    made non-increasing; a first knot at r = 0 repeats the central value."""
+    r: np.ndarray
+    p: np.ndarray
+
+    @classmethod
+    def of_stamp(cls, stamp, bin_width: float = 0.5) -> "PsfProfile":
+        img = np.asarray(stamp, np.float64)
+        total = img.sum()
+        if not (np.isfinite(total) and total > 0):
+            raise ValueError("psf_radial_profile: stamp must have positive sum")
+        img = img / total
+        h, w = img.shape
+        dist = np.hypot(*np.meshgrid(np.arange(h) - (h - 1) / 2.0,
+                                     np.arange(w) - (w - 1) / 2.0, indexing="ij")).ravel()
+        n_bin = int(np.ceil(dist.max() / bin_width)) + 1
+        k = np.minimum((dist / bin_width).astype(np.int64), n_bin - 1)
+        total_in = np.bincount(k, weights=img.ravel(), minlength=n_bin)
+        pix_in = np.bincount(k, minlength=n_bin)
+        used = np.flatnonzero(pix_in)
+        mean = np.minimum.accumulate(np.maximum(total_in[used] / pix_in[used], 1e-12))
+        return cls(np.r_[0.0, (used + 0.5) * bin_width], np.r_[mean[0], mean])
+
+    def radius_at_level(self, level) -> np.ndarray:
+        """Radius where the profile falls to ``level`` (linear in log p):
+        0 at or above the central value and for +inf, the outermost radius
+        for levels below the profile or not positive."""
+        lev = np.asarray(level, np.float64)
+        out = np.full(lev.shape, self.r[-1])
+        ok = np.isfinite(lev) & (lev > 0)
+        logp = np.log(self.p[::-1])               # ascending
+        x = np.clip(np.log(np.where(ok, lev, 1.0)), logp[0], logp[-1])
+        out = np.where(ok, np.interp(x, logp, self.r[::-1]), out)
+        inside = (np.isfinite(lev) & (lev >= self.p[0])) | (np.isinf(lev) & (lev > 0))
+        return np.where(inside, 0.0, out)
 
 
 def psf_radial_profile(stamp, bin_width: float = 0.5):
-    """Monotone (running-min) annulus-mean unit-flux profile of a centred
-    stamp; returns (r, p) with r[0] = 0 anchored at the central value."""
-    st = np.asarray(stamp, np.float64)
-    tot = st.sum()
-    if not np.isfinite(tot) or tot <= 0:
-        raise ValueError("psf_radial_profile: stamp must have positive sum")
-    st = st / tot
-    H, W = st.shape
-    yy, xx = np.mgrid[0:H, 0:W]
-    rr = np.hypot(yy - (H - 1) / 2.0, xx - (W - 1) / 2.0).ravel()
-    nbin = int(np.ceil(rr.max() / bin_width)) + 1
-    idx = np.minimum((rr / bin_width).astype(np.int64), nbin - 1)
-    sums = np.bincount(idx, weights=st.ravel(), minlength=nbin)
-    cnts = np.bincount(idx, minlength=nbin)
-    ok = cnts > 0
-    r = (np.nonzero(ok)[0] + 0.5) * bin_width
-    p = np.minimum.accumulate(np.maximum(sums[ok] / cnts[ok], 1e-12))
-    return np.concatenate([[0.0], r]), np.concatenate([[p[0]], p])
+    """(r, p) of :class:`PsfProfile` (the reference's return layout)."""
+    prof = PsfProfile.of_stamp(stamp, bin_width)
+    return prof.r, prof.p
 
 
-def _invert_profile(psf_r, psf_p, thresh):
-    lp = np.log(psf_p[::-1])
-    rr = psf_r[::-1]
-    t = np.asarray(thresh, np.float64)
-    out = np.full(t.shape, psf_r[-1])
-    pos = np.isfinite(t) & (t > 0)
-    tc = np.clip(np.log(np.where(pos, t, 1.0)), lp[0], lp[-1])
-    out[pos] = np.interp(tc, lp, rr)[pos]
-    out[np.isfinite(t) & (t >= psf_p[0])] = 0.0
-    out[np.isinf(t) & (t > 0)] = 0.0
-    return out
-
-
-def _ext_shape(logre, sersic_n):
+def _sersic_falloff(logre, sersic_n):
+    """(Re, n_eff, b, I0) of the unit-flux profile I0 exp(-b (r/Re)^(1/n_eff))
+    that bounds an extended source's light: an exponential below n = 1.5,
+    b = 2n - 1/3 above."""
     from scipy.special import gamma
-    re = np.clip(np.where(np.isfinite(logre), np.exp(logre), RE_MIN), RE_MIN, RE_MAX)
-    n = np.where(np.isfinite(sersic_n), sersic_n, 1.0)
-    n_eff = np.where(n < 1.5, 1.0, n)
+    re = _clean_re(logre)
+    n = np.asarray(sersic_n, np.float64)
+    n_eff = np.where(np.isfinite(n) & (n >= 1.5), n, 1.0)
     b = np.where(n_eff < 1.5, SERSIC_B1, 2.0 * n_eff - 1.0 / 3.0)
     i0 = b ** (2.0 * n_eff) / (2.0 * np.pi * n_eff * gamma(2.0 * n_eff) * re ** 2)
     return re, n_eff, b, i0
@@ -86,39 +128,39 @@ def _ext_shape(logre, sersic_n):
 
 def flux_aware_radii(source_type, logre, sersic_n, flux, sigma_sky: float, kappa: float,
                      psf_r, psf_p, r_psf: float, r_core: float | None = None) -> np.ndarray:
-    """Radius where flux * unit profile = kappa * sigma_sky, floored at
-    r_core (default PSF FWHM) and capped at the geometric radius."""
-    if sigma_sky <= 0 or not np.isfinite(sigma_sky):
+    """Per-source radius where flux x (unit profile) equals kappa x sigma_sky,
+    floored at ``r_core`` (default the PSF FWHM) and capped at the
+    geometric radius; sources without a positive finite flux get the
+    floor."""
+    if not (np.isfinite(sigma_sky) and sigma_sky > 0):
         raise ValueError(f"sigma_sky must be positive, got {sigma_sky}")
-    if kappa <= 0:
+    if not kappa > 0:
         raise ValueError(f"kappa must be positive, got {kappa}")
-    st = np.asarray(source_type)
-    logre = np.asarray(logre, np.float64)
-    sersic_n = np.asarray(sersic_n, np.float64)
-    flux = np.asarray(flux, np.float64)
+    prof = PsfProfile(np.asarray(psf_r, np.float64), np.asarray(psf_p, np.float64))
     if r_core is None:
-        r_core = 2.0 * float(_invert_profile(psf_r, psf_p, np.array([0.5 * psf_p[0]]))[0])
-    if r_core <= 0:
+        r_core = 2.0 * float(prof.radius_at_level(np.array([0.5 * prof.p[0]]))[0])
+    if not r_core > 0:
         raise ValueError(f"r_core must be positive, got {r_core}")
-    good = np.isfinite(flux) & (flux > 0)
-    thresh = np.where(good, kappa * float(sigma_sky) / np.where(good, flux, 1.0), np.inf)
-    r_star = _invert_profile(psf_r, psf_p, thresh)
-    re, n_eff, b, i0 = _ext_shape(logre, sersic_n)
+    f = np.asarray(flux, np.float64)
+    lit = np.isfinite(f) & (f > 0)
+    level = np.where(lit, kappa * float(sigma_sky) / np.where(lit, f, 1.0), np.inf)
+    re, n_eff, b, i0 = _sersic_falloff(logre, sersic_n)
     with np.errstate(divide="ignore", invalid="ignore"):
-        arg = np.log(i0 / thresh)
-    r_ext = np.where(arg > 0, re * (np.maximum(arg, 0.0) / b) ** n_eff, 0.0)
-    cap = coupling_radii(st, logre, sersic_n, r_psf)
-    return np.clip(np.where(st == SOURCE_STAR, r_star, r_ext), r_core, cap)
+        depth = np.log(i0 / level)                   # -inf without flux
+    r_ext = np.where(depth > 0, re * (np.maximum(depth, 0.0) / b) ** n_eff, 0.0)
+    star = np.asarray(source_type) == SOURCE_STAR
+    r = np.where(star, prof.radius_at_level(level), r_ext)
+    return np.clip(r, r_core, coupling_radii(source_type, logre, sersic_n, r_psf))
 
 
 @dataclass
 class IslandSet:
     n_sources: int
     n_islands: int
-    labels: np.ndarray
-    radius: np.ndarray
-    members: list = field(repr=False)
-    n_src: np.ndarray = field(repr=False)
+    labels: np.ndarray                     # (n,) island of each source
+    radius: np.ndarray                     # (n,) coupling radius used
+    members: list = field(repr=False)      # per island, ascending source ids
+    n_src: np.ndarray = field(repr=False)  # (n_isl,) member counts
     bbox: np.ndarray = field(repr=False)   # (n_isl, 4) x0, y0, x1, y1 half-open
     r_psf: float = 0.0
 
@@ -128,65 +170,100 @@ class IslandSet:
 
 
 def coupled_pairs(px, py, radius):
-    """All i < j with dist(i, j) < r_i + r_j (kd-tree candidates at the
-    largest possible separation, exact predicate afterwards)."""
+    """Index pairs (i, j) with |p_i - p_j| < r_i + r_j, each coupled pair at
+    least once (a pair of two large-radius sources may appear twice)."""
     n = px.size
     if n < 2:
         return np.empty(0, np.int64), np.empty(0, np.int64)
-    tree = cKDTree(np.column_stack([px, py]))
-    cand = tree.query_pairs(r=2.0 * float(radius.max()), output_type="ndarray")
-    if cand.size == 0:
-        return np.empty(0, np.int64), np.empty(0, np.int64)
-    i, j = cand[:, 0].astype(np.int64), cand[:, 1].astype(np.int64)
-    d2 = (px[i] - px[j]) ** 2 + (py[i] - py[j]) ** 2
-    rs = radius[i] + radius[j]
-    keep = d2 < rs * rs
-    return i[keep], j[keep]
+    xy = np.column_stack([px, py])
+    cut = float(np.quantile(radius, _SMALL_RADIUS_QUANTILE))
+    small = np.flatnonzero(radius <= cut)
+    large = np.flatnonzero(radius > cut)
+    cand_i, cand_j = [], []
+    if small.size > 1:
+        pr = cKDTree(xy[small]).query_pairs(r=2.0 * float(radius[small].max()),
+                                            output_type="ndarray")
+        cand_i.append(small[pr[:, 0]])
+        cand_j.append(small[pr[:, 1]])
+    if large.size:
+        near = cKDTree(xy).query_ball_point(xy[large], radius[large] + float(radius.max()))
+        cnt = np.fromiter(map(len, near), np.int64, large.size)
+        cand_i.append(np.repeat(large, cnt))
+        cand_j.append(np.concatenate(near).astype(np.int64) if cnt.sum()
+                      else np.empty(0, np.int64))
+    i = np.concatenate(cand_i) if cand_i else np.empty(0, np.int64)
+    j = np.concatenate(cand_j) if cand_j else np.empty(0, np.int64)
+    reach = radius[i] + radius[j]
+    hit = (i != j) & ((px[i] - px[j]) ** 2 + (py[i] - py[j]) ** 2 < reach * reach)
+    return i[hit], j[hit]
+
+
+def _min_roots(n: int, i, j) -> np.ndarray:
+    """Root of every node after joining the pairs (i, j): union-find where a
+    root is always hooked under a smaller root, so each component's root is
+    its smallest index; full pointer jumping between rounds."""
+    parent = np.arange(n)
+    while True:
+        while True:                                   # flatten to the roots
+            up = parent[parent]
+            if np.array_equal(up, parent):
+                break
+            parent = up
+        ri, rj = parent[i], parent[j]
+        split = ri != rj
+        if not split.any():
+            return parent
+        lo, hi = np.minimum(ri, rj)[split], np.maximum(ri, rj)[split]
+        np.minimum.at(parent, hi, lo)
 
 
 def build_islands(pos_x, pos_y, source_type, logre, sersic_n, r_psf: float, *,
-                  radius=None) -> IslandSet:
+                  radius=None, forced_singleton=None) -> IslandSet:
+    """Islands of a catalogue; ``radius`` overrides the geometric radii and
+    sources flagged in ``forced_singleton`` couple to nothing (the
+    reference's subtract-only sources, islands.py:423-470)."""
     px = np.asarray(pos_x, np.float64)
     py = np.asarray(pos_y, np.float64)
     n = px.size
-    if not (np.all(np.isfinite(px)) and np.all(np.isfinite(py))):
-        raise ValueError("build_islands requires finite positions")
-    if source_type is None:
-        source_type = np.zeros(n, np.int32)
-    if logre is None:
-        logre = np.zeros(n)
-    if sersic_n is None:
-        sersic_n = np.ones(n)
+    if py.size != n:
+        raise ValueError("pos_x and pos_y must have the same length")
+    if not (np.isfinite(px).all() and np.isfinite(py).all()):
+        raise ValueError("build_islands needs finite positions")
     if radius is None:
-        radius = coupling_radii(source_type, logre, sersic_n, r_psf)
+        st = np.zeros(n, np.int32) if source_type is None else source_type
+        radius = coupling_radii(st, np.zeros(n) if logre is None else logre,
+                                np.ones(n) if sersic_n is None else sersic_n, r_psf)
     radius = np.asarray(radius, np.float64)
-    if n and not (np.all(np.isfinite(radius)) and np.all(radius > 0)):
+    if radius.size != n:
+        raise ValueError("radius must have one entry per source")
+    if n and not (np.isfinite(radius).all() and (radius > 0).all()):
         raise ValueError("radius must be finite and positive")
     if n == 0:
         return IslandSet(0, 0, np.empty(0, np.int32), radius, [], np.empty(0, np.int32),
                          np.empty((0, 4), np.int64), float(r_psf))
     i, j = coupled_pairs(px, py, radius)
-    if i.size:
-        g = sparse.coo_matrix((np.ones(i.size, np.int8), (i, j)), shape=(n, n))
-        _, raw = connected_components(g, directed=False)
-    else:
-        raw = np.arange(n)
-    n_isl = int(raw.max()) + 1
-    first = np.full(n_isl, n, np.int64)
-    np.minimum.at(first, raw, np.arange(n))
-    remap = np.empty(n_isl, np.int64)
-    remap[np.argsort(first, kind="stable")] = np.arange(n_isl)
-    labels = remap[raw].astype(np.int32)
-    order = np.argsort(labels, kind="stable")
-    counts = np.bincount(labels, minlength=n_isl).astype(np.int32)
-    starts = np.concatenate([[0], np.cumsum(counts[:-1])]).astype(np.int64)
-    members = [order[s:s + c].astype(np.int32) for s, c in zip(starts, counts)]
-    bbox = np.empty((n_isl, 4), np.int64)
-    bbox[:, 0] = np.floor(np.minimum.reduceat((px - radius)[order], starts))
-    bbox[:, 1] = np.floor(np.minimum.reduceat((py - radius)[order], starts))
-    bbox[:, 2] = np.ceil(np.maximum.reduceat((px + radius)[order], starts))
-    bbox[:, 3] = np.ceil(np.maximum.reduceat((py + radius)[order], starts))
-    return IslandSet(n, n_isl, labels, radius, members, counts, bbox, float(r_psf))
+    if forced_singleton is not None:
+        alone = np.asarray(forced_singleton, bool)
+        if alone.size != n:
+            raise ValueError("forced_singleton must have one entry per source")
+        free = ~(alone[i] | alone[j])
+        i, j = i[free], j[free]
+    root = _min_roots(n, i, j)
+    heads, labels = np.unique(root, return_inverse=True)   # heads ascending
+    labels = labels.astype(np.int32)
+    n_isl = heads.size
+    n_src = np.bincount(labels, minlength=n_isl).astype(np.int32)
+    by_island = np.argsort(labels, kind="stable").astype(np.int32)
+    members = np.split(by_island, np.cumsum(n_src)[:-1])
+    box = np.empty((n_isl, 4), np.float64)
+    box[:, :2] = np.inf
+    box[:, 2:] = -np.inf
+    np.minimum.at(box[:, 0], labels, px - radius)
+    np.minimum.at(box[:, 1], labels, py - radius)
+    np.maximum.at(box[:, 2], labels, px + radius)
+    np.maximum.at(box[:, 3], labels, py + radius)
+    bbox = np.concatenate([np.floor(box[:, :2]), np.ceil(box[:, 2:])], axis=1).astype(np.int64)
+    return IslandSet(n, n_isl, labels, radius, members, n_src, bbox, float(r_psf))
 
 
 @dataclass
@@ -211,27 +288,31 @@ class BucketedIslands:
     giant_ids: np.ndarray
 
 
-def bucket_islands(isl: IslandSet, bbox_classes=ISLAND_BBOX_CLASSES,
-                   nsrc_classes=ISLAND_NSRC_CLASSES) -> BucketedIslands:
-    """Smallest (bbox, nsrc) class holding each island; oversize -> giant."""
-    bc = tuple(sorted(int(c) for c in bbox_classes))
-    nc = tuple(sorted(int(c) for c in nsrc_classes))
-    bi = np.searchsorted(np.asarray(bc), isl.bbox_size, side="left")
-    ni = np.searchsorted(np.asarray(nc), isl.n_src, side="left")
-    giant = (bi >= len(bc)) | (ni >= len(nc))
+def bucket_islands(isl: IslandSet, bbox_classes=DEFAULT_BBOX_CLASSES,
+                   nsrc_classes=DEFAULT_NSRC_CLASSES) -> BucketedIslands:
+    """Each island into the smallest (box side, member count) class that
+    holds it; islands beyond either ladder are giants."""
+    bc = np.sort(np.asarray(bbox_classes, np.int64))
+    nc = np.sort(np.asarray(nsrc_classes, np.int64))
+    kb = np.searchsorted(bc, isl.bbox_size, side="left")
+    kn = np.searchsorted(nc, isl.n_src, side="left")
+    giant = (kb >= bc.size) | (kn >= nc.size)
+    # every source's slot within its island (members are ascending)
+    first = np.concatenate([[0], np.cumsum(isl.n_src)[:-1]])
+    slot = np.empty(isl.n_sources, np.int64)
+    if isl.n_sources:
+        order = np.concatenate(isl.members)
+        slot[order] = np.arange(isl.n_sources) - np.repeat(first, isl.n_src)
     buckets = {}
-    ok = np.where(~giant)[0]
-    if ok.size:
-        keys = bi[ok] * len(nc) + ni[ok]
-        for key in np.unique(keys):
-            sel = ok[keys == key]
-            B, N = bc[int(key) // len(nc)], nc[int(key) % len(nc)]
-            midx = np.full((sel.size, N), -1, np.int32)
-            mmask = np.zeros((sel.size, N), bool)
-            for row, k in enumerate(sel):
-                m = isl.members[k]
-                midx[row, :m.size] = m
-                mmask[row, :m.size] = True
-            buckets[(B, N)] = IslandBucket(B, N, sel.astype(np.int32), midx, mmask,
-                                           isl.bbox[sel])
-    return BucketedIslands(bc, nc, buckets, np.where(giant)[0].astype(np.int32))
+    cls = np.where(giant, -1, kb * nc.size + kn)
+    for key in np.unique(cls[~giant]):
+        ids = np.flatnonzero(cls == key)
+        B, N = int(bc[key // nc.size]), int(nc[key % nc.size])
+        src = np.concatenate([isl.members[k] for k in ids]).astype(np.int64)
+        row = np.repeat(np.arange(ids.size), isl.n_src[ids])
+        midx = np.full((ids.size, N), -1, np.int32)
+        midx[row, slot[src]] = src
+        buckets[(B, N)] = IslandBucket(B, N, ids.astype(np.int32), midx, midx >= 0,
+                                       isl.bbox[ids])
+    return BucketedIslands(tuple(int(c) for c in bc), tuple(int(c) for c in nc), buckets,
+                           np.flatnonzero(giant).astype(np.int32))
