@@ -283,3 +283,21 @@ def test_every_module_imports_without_a_gpu():
         importlib.import_module(f"uniphot_amd.{m.name}")
     importlib.import_module("bench")
     importlib.import_module("__graft_entry__")
+
+
+def test_bucketing_a_subset_of_islands_equals_bucketing_them_in_the_full_set():
+    """An island set restricted to some islands (as the accuracy and cutout
+    paths build) buckets those islands exactly as the full set does."""
+    from uniphot_amd.pipeline import sub_island_set
+    rng = np.random.default_rng(4)
+    pts = rng.uniform(0, 600, (300, 2))
+    iset = build_islands(pts[:, 0], pts[:, 1], None, None, None, 27.5,
+                         radius=np.full(300, 6.0))
+    ids = np.sort(rng.choice(iset.n_islands, iset.n_islands // 3, replace=False))
+    full = bucket_islands(iset, ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
+    part = bucket_islands(sub_island_set(iset, ids), ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES)
+    for key, b in part.buckets.items():
+        f = full.buckets[key]
+        rows = np.searchsorted(f.island_ids, ids[b.island_ids])
+        assert np.array_equal(f.island_ids[rows], ids[b.island_ids])
+        assert np.array_equal(f.member_idx[rows], b.member_idx)

@@ -297,12 +297,13 @@ def bucket_islands(isl: IslandSet, bbox_classes=DEFAULT_BBOX_CLASSES,
     kb = np.searchsorted(bc, isl.bbox_size, side="left")
     kn = np.searchsorted(nc, isl.n_src, side="left")
     giant = (kb >= bc.size) | (kn >= nc.size)
-    # every source's slot within its island (members are ascending)
-    first = np.concatenate([[0], np.cumsum(isl.n_src)[:-1]])
-    slot = np.empty(isl.n_sources, np.int64)
-    if isl.n_sources:
-        order = np.concatenate(isl.members)
-        slot[order] = np.arange(isl.n_sources) - np.repeat(first, isl.n_src)
+    # each member's slot within its island (members are ascending; the set
+    # may hold only some of the catalogue's islands)
+    first = np.concatenate([[0], np.cumsum(isl.n_src)[:-1]]).astype(np.int64)
+    slot = np.zeros(isl.n_sources, np.int64)
+    if isl.n_islands:
+        order = np.concatenate(isl.members).astype(np.int64)
+        slot[order] = np.arange(order.size) - np.repeat(first, isl.n_src)
     buckets = {}
     cls = np.where(giant, -1, kb * nc.size + kn)
     for key in np.unique(cls[~giant]):
