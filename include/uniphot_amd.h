@@ -415,6 +415,28 @@ int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const int64_t* sr
                    const int32_t* nbr_idx, float reg, float* fluxes, float* flux_var, float* bg,
                    float* bg_var, void* scratch, size_t scratch_bytes, void* stream);
 
+/*
+ * Allocation-free form of uph_band_solve.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  Same inputs, outputs and results as
+ * uph_band_solve; the workspace is a static device array of
+ * UPH_BAND_NOALLOC_BYTES per GPU (held from library load until the process
+ * ends), so the call makes no allocation.  ``max_src`` is the largest
+ * problem's source count; batches whose uph_band_scratch_bytes(n_prob,
+ * max_src, bg_order) exceed UPH_BAND_NOALLOC_BYTES are rejected with
+ * UPH_ERR_INVALID_ARGUMENT (split them).  Every call on one GPU uses the
+ * same array: calls that overlap on one GPU race.
+ */
+#define UPH_BAND_NOALLOC_BYTES (64u << 20)
+int uph_band_solve_noalloc(int n_prob, int T, int bg_order, int k_max, int max_src,
+                           const int64_t* src_off, const int32_t* win, int H, int W, int img_x0,
+                           int img_y0, const float* data, const float* invvar,
+                           const uint8_t* good, double bg_xo, double bg_xs, double bg_yo,
+                           double bg_ys, const float* stamps, const int32_t* ix_top,
+                           const int32_t* iy_top, const int32_t* nbr_idx, float reg,
+                           float* fluxes, float* flux_var, float* bg, float* bg_var,
+                           void* stream);
+
 #ifdef __cplusplus
 }
 #endif

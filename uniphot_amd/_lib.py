@@ -21,8 +21,9 @@ EXPORTS = (
     "uph_prerender_scratch_bytes", "uph_prerender_spectra_floats",
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
     "uph_wing_images", "uph_blank_work_floats", "uph_blank_response", "uph_stamp_sum",
-    "uph_gather_windows", "uph_subtract_bg_planes",
+    "uph_gather_windows", "uph_subtract_bg_planes", "uph_band_solve_noalloc",
 )
+UPH_BAND_NOALLOC_BYTES = 64 << 20
 
 
 class UphError(RuntimeError):
@@ -132,6 +133,9 @@ def lib() -> C.CDLL:
     L.uph_band_solve.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int,
                                  C.c_int, C.c_int, _P, _P, _P, C.c_double, C.c_double, C.c_double, C.c_double,
                                  _P, _P, _P, _P, C.c_float, _P, _P, _P, _P, _P, C.c_size_t, _P]
+    L.uph_band_solve_noalloc.restype = C.c_int
+    L.uph_band_solve_noalloc.argtypes = [C.c_int] * 5 + [_P, _P] + [C.c_int] * 4 + [_P] * 3 + \
+        [C.c_double] * 4 + [_P] * 4 + [C.c_float] + [_P] * 4 + [_P]
     L.uph_blank_work_floats.restype = C.c_size_t
     L.uph_blank_work_floats.argtypes = [C.c_int, C.c_int]
     L.uph_blank_response.restype = C.c_int

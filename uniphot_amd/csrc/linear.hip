@@ -14,6 +14,8 @@
 // forced_phot_sparse.py:342-554.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "uph_capi_internal.hpp"
 #include "uph_device.hpp"
 #include "uph_galaxy.hpp"
@@ -1043,15 +1045,14 @@ extern "C" size_t uph_band_scratch_bytes(int n_prob, int max_src, int bg_order) 
   return (size_t)n_prob * uph::band_scratch_layout(max_src, M).per_problem;
 }
 
-extern "C" int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const int64_t* src_off,
-                              const int32_t* win, int H, int W, int img_x0, int img_y0,
-                              const float* data, const float* invvar, const uint8_t* good,
-                              double bg_xo,
-                              double bg_xs, double bg_yo, double bg_ys, const float* stamps,
-                              const int32_t* ix_top, const int32_t* iy_top,
-                              const int32_t* nbr_idx, float reg, float* fluxes, float* flux_var,
-                              float* bg, float* bg_var, void* scratch, size_t scratch_bytes,
-                              void* stream) {
+static int band_solve_impl(int n_prob, int T, int bg_order, int k_max, const int64_t* src_off,
+                           const int32_t* win, int H, int W, int img_x0, int img_y0,
+                           const float* data, const float* invvar, const uint8_t* good,
+                           double bg_xo, double bg_xs, double bg_yo, double bg_ys,
+                           const float* stamps, const int32_t* ix_top, const int32_t* iy_top,
+                           const int32_t* nbr_idx, float reg, float* fluxes, float* flux_var,
+                           float* bg, float* bg_var, void* scratch, size_t scratch_bytes,
+                           void* stream) {
   UPH_CHECK_ARG(n_prob >= 0, "uph_band_solve: n_prob=%d", n_prob);
   UPH_CHECK_ARG(bg_order >= 0 && bg_order <= 2, "uph_band_solve: bg_order=%d outside 0..2",
                 bg_order);
@@ -1108,6 +1109,83 @@ extern "C" int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const 
     hipLaunchKernelGGL(uph::band_lu_kernel<6>, dim3(n_prob), dim3(UPH_BLOCK), lds_lu, s, a);
   }
   return uph::hip_status(hipGetLastError(), "uph_band_solve launch");
+}
+
+extern "C" int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const int64_t* src_off,
+                              const int32_t* win, int H, int W, int img_x0, int img_y0,
+                              const float* data, const float* invvar, const uint8_t* good,
+                              double bg_xo, double bg_xs, double bg_yo, double bg_ys,
+                              const float* stamps, const int32_t* ix_top, const int32_t* iy_top,
+                              const int32_t* nbr_idx, float reg, float* fluxes, float* flux_var,
+                              float* bg, float* bg_var, void* scratch, size_t scratch_bytes,
+                              void* stream) {
+  return band_solve_impl(n_prob, T, bg_order, k_max, src_off, win, H, W, img_x0, img_y0, data,
+                         invvar, good, bg_xo, bg_xs, bg_yo, bg_ys, stamps, ix_top, iy_top,
+                         nbr_idx, reg, fluxes, flux_var, bg, bg_var, scratch, scratch_bytes,
+                         stream);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free band solve, for callers that cannot allocate (code under
+// graph capture, serving loops): the launches of uph_band_solve with their
+// workspace in this file-scope device array of UPH_BAND_NOALLOC_BYTES.  Each
+// device holds its own copy from module load until the process ends.  In
+// host code the array's name is not a device address: the address is taken
+// once per device with hipGetSymbolAddress and cached.  The kernels write
+// every workspace value they read (the normal matrix is zeroed by the
+// assembly launch), so nothing needs clearing between calls; calls that
+// overlap on one device share the array and race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) char scratch[UPH_BAND_NOALLOC_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return uph::hip_status(e, "uph_band_solve_noalloc: hipGetDevice");
+  UPH_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "uph_band_solve_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess)
+      return uph::hip_status(e, "uph_band_solve_noalloc: hipGetSymbolAddress");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return UPH_OK;
+}
+}  // namespace
+
+extern "C" int uph_band_solve_noalloc(int n_prob, int T, int bg_order, int k_max, int max_src,
+                                      const int64_t* src_off, const int32_t* win, int H, int W,
+                                      int img_x0, int img_y0, const float* data,
+                                      const float* invvar, const uint8_t* good, double bg_xo,
+                                      double bg_xs, double bg_yo, double bg_ys,
+                                      const float* stamps, const int32_t* ix_top,
+                                      const int32_t* iy_top, const int32_t* nbr_idx, float reg,
+                                      float* fluxes, float* flux_var, float* bg, float* bg_var,
+                                      void* stream) {
+  UPH_CHECK_ARG(n_prob >= 0 && max_src >= 0, "uph_band_solve_noalloc: n_prob=%d max_src=%d",
+                n_prob, max_src);
+  UPH_CHECK_ARG(bg_order >= 0 && bg_order <= 2,
+                "uph_band_solve_noalloc: bg_order=%d outside 0..2", bg_order);
+  if (n_prob == 0) return UPH_OK;
+  const size_t need = uph_band_scratch_bytes(n_prob, max_src, bg_order);
+  UPH_CHECK_ARG(need <= (size_t)UPH_BAND_NOALLOC_BYTES,
+                "uph_band_solve_noalloc: %d problems of up to %d sources need %zu bytes of "
+                "workspace, more than the static %zu; split the batch",
+                n_prob, max_src, need, (size_t)UPH_BAND_NOALLOC_BYTES);
+  void* ws = nullptr;
+  const int st = noalloc_scratch(&ws);
+  if (st != UPH_OK) return st;
+  return band_solve_impl(n_prob, T, bg_order, k_max, src_off, win, H, W, img_x0, img_y0, data,
+                         invvar, good, bg_xo, bg_xs, bg_yo, bg_ys, stamps, ix_top, iy_top,
+                         nbr_idx, reg, fluxes, flux_var, bg, bg_var, ws, need, stream);
 }
 
 extern "C" int uph_wing_images(int n_prob, int Tb, int Ts, const int64_t* src_off,
