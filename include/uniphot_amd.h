@@ -270,6 +270,13 @@ typedef struct uph_island_split {
   const int32_t* big_isl;
   double* big_part;
   float* post_kappa;          /* (n_isl), see post above                       */
+  /* optional (NULL: not written): with post, the band sums of the
+     Gauss-Newton H (nL x nL over the island's live parameters, members in
+     slot order, parameters x, y, logre, ee1, ee2, u_n within a member) and
+     g (nL) at the converged parameters, from post_hg + post_hg_off[i] --
+     what the host's stall probes (island_solver.py:1335-1465) need        */
+  double* post_hg;
+  const int64_t* post_hg_off;
 } uph_island_split;
 
 size_t uph_island_split_counters(int max_steps);

@@ -3,6 +3,7 @@
 Reference: ``tract7dt/fit/grid_gpu/island_solver.py`` -- ``models_band``
 (:480-562), ``_fluxes_band`` / ``residual_band`` / ``chi2_of``
 (:570-684), ``hg_scan`` (:812-865), legacy ``one_step`` (:921-1002),
+``diverged_probe`` / ``plateau_probe`` (:1335-1465),
 ``final_fluxes`` (:1467-1482), ``final_errors`` (:1484-1547), ``final_flux_marg``
 (:1549-1612), ``source_metrics`` (:1643-1715), and the
 chunk loop of ``run_class_chunk`` (:2890-2977).  Host input layout:
@@ -251,6 +252,54 @@ def one_step(state: dict, isl: Island, o: LMOptions) -> dict:
     return dict(p=p_new, lam=lam_new, chi2=chi2_new, consec=consec, nrej=nrej,
                 conv=conv, stall=stall_now, nst=state["nst"] + 1,
                 accept=accept, chi2_try=chi2_try, p_try=p_try)
+
+
+def propose(p, lam, isl: Island, o: LMOptions, hg_at_p=None):
+    """The legacy step's trial point at damping ``lam`` (one_step,
+    island_solver.py:921-969, float32 parameter state).  ``hg_at_p`` reuses
+    an (H, g, chi2, live) of :func:`hg` at p."""
+    N = isl.N
+    nP = 6 * N
+    Hl, gl, _, live = hg(p, isl, o) if hg_at_p is None else hg_at_p
+    H = np.zeros((nP, nP))
+    g = np.zeros(nP)
+    idx = np.array([6 * k + j for (k, j) in live], dtype=np.int64)
+    if idx.size:
+        H[np.ix_(idx, idx)] = Hl
+        g[idx] = gl
+    keep = np.zeros(nP, bool)
+    keep[idx] = True
+    H = np.where(keep[:, None] & keep[None, :], H, 0.0) + np.diag(np.where(keep, 0.0, 1.0))
+    g = np.where(keep, g, 0.0)
+    Hd = H + lam * np.diag(np.diag(H)) + 1e-8 * np.eye(nP)
+    delta = (-np.linalg.solve(Hd, g)).astype(np.float32)
+    delta = np.where(keep, delta, np.float32(0.0))
+    cv = clip_vec(N, o).astype(np.float32)
+    delta = np.clip(delta, -cv, cv)
+    p = np.asarray(p, np.float32)
+    return np.clip(p + delta, isl.lower.astype(np.float32), isl.upper.astype(np.float32))
+
+
+def diverged_probe(p, lam_final, isl: Island, o: LMOptions) -> bool:
+    """Post-hoc divergence test of a stalled island (island_solver.py:
+    1335-1392): the STALL_REJECT_K rejected proposals, rebuilt at
+    lam_final * 10^(j - K), are all identical."""
+    h = hg(p, isl, o)
+    props = [propose(p, lam_final * 10.0 ** (j - STALL_REJECT_K), isl, o, h)
+             for j in range(STALL_REJECT_K)]
+    return all(np.array_equal(q, props[0]) for q in props[1:])
+
+
+PLATEAU_LAMBDAS = tuple(float(np.float32(v)) for v in [1e-9] + [10.0 ** e for e in range(-8, 7)])
+
+
+def plateau_probe(p, isl: Island, o: LMOptions) -> float:
+    """Best chi2 change over the damping ladder 1e-9 ... 1e6 of a stalled
+    island (island_solver.py:1394-1465): min_j chi2(proposal(lam_j)) -
+    chi2(p); negative when a damped step still improves."""
+    h = hg(p, isl, o)
+    c0 = chi2_of(p, isl, o)
+    return min(chi2_of(propose(p, lm, isl, o, h), isl, o) for lm in PLATEAU_LAMBDAS) - c0
 
 
 def run_island(p0, isl: Island, o: LMOptions, max_steps=None, trace=False):

@@ -303,3 +303,36 @@ def test_post_fit_pass_matches_oracle(cuda, kind):
     assert worst["var"] < 2e-3
     assert worst["dvar"] < 2e-3
     assert worst["met"] < 1e-3
+
+
+def test_stall_probes_match_oracle(cuda):
+    """LM run with the relative-chi2 and step prongs off (ftol 0, step
+    tolerance 0), so islands end by four rejects: the device probes of every
+    stalled island against the oracle's at the device's final parameters --
+    the divergence flag exactly, the plateau's best chi2 change within the
+    float32 chi2 noise (1e-6 of chi2 + 1e-2)."""
+    import torch
+    from uniphot_amd.island_solver import (make_class_kernels, psf_tree_device,
+                                           run_island_batch, stall_probes)
+    sc = small_star_scene()
+    plans, hosts, pad, d, w = star_classes(sc)
+    dd, ww = torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda)
+    trees = {55: psf_tree_device(sc.psfs, 55, cuda)}
+    batch = [(make_class_kernels(p, 2, sc.psfs[0].n_terms, chi2_ftol_rel=0.0, step_tol_pix=0.0),
+              h, np.arange(p.n_islands)) for p, h in zip(plans, hosts)]
+    res = run_island_batch(batch, trees, 30, dd, ww, split=True, prerender=True, post=True)
+    n = stall_probes(batch, res, trees, dd, ww)
+    print("stalled and probed:", n, flush=True)
+    assert n >= 3
+    o = oisl.LMOptions(ftol_rel=0.0, step_tol=0.0)
+    for (plan, host), rr in zip(zip(plans, hosts), res):
+        for r in range(plan.n_islands):
+            if not rr["stall"][r]:
+                assert np.isnan(rr["stall_dchi2"][r]) and not rr["diverged"][r]
+                continue
+            isl = oracle_island(plan, host, r, d, w, sc.psfs)
+            p = rr["p"][r].ravel()
+            assert rr["diverged"][r] == oisl.diverged_probe(p, float(rr["lam"][r]), isl, o)
+            best = oisl.plateau_probe(p, isl, o)
+            c2 = oisl.chi2_of(p, isl, o)
+            assert abs(rr["stall_dchi2"][r] - best) < 1e-6 * c2 + 1e-2, (rr["stall_dchi2"][r], best)

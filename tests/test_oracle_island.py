@@ -378,3 +378,37 @@ def test_kappa_scaled_closed_forms():
     H = A @ A.T + np.eye(5)
     d = np.diag(rng.uniform(0.1, 10.0, 5))
     assert oisl.kappa_scaled(d @ H @ d) == pytest.approx(oisl.kappa_scaled(H), rel=1e-9)
+
+
+def test_stall_probes_known_answers():
+    """Clip-pinned steps: every rejected proposal is p itself, so the
+    divergence probe fires and the plateau probe's best change is exactly 0.
+    A noiseless island 0.3 px off: the proposals differ with the damping
+    (no divergence) and the ladder's best rung recovers at least 99 % of
+    chi2 (the Gauss-Newton step of a nearly quadratic problem)."""
+    isl, _, p0 = _noiseless_island(0.3)
+    pinned = oisl.LMOptions(pos_step_max=0.0, shape_step_max=0.0, un_step_max=0.0)
+    st = oisl.run_island(p0, isl, pinned)
+    assert st["stall"]
+    assert oisl.diverged_probe(st["p"], st["lam"], isl, pinned)
+    assert oisl.plateau_probe(st["p"], isl, pinned) == 0.0
+    o = oisl.LMOptions()
+    p = np.float32(p0)
+    assert not oisl.diverged_probe(p, 10.0, isl, o)
+    best = oisl.plateau_probe(p, isl, o)
+    c0 = oisl.chi2_of(p, isl, o)
+    assert best < -0.99 * c0          # >= 99 % of chi2 recovered by one rung
+    # the best rung equals the brute-force minimum over the ladder
+    h = oisl.hg(p, isl, o)
+    brute = min(oisl.chi2_of(oisl.propose(p, lm, isl, o, h), isl, o) for lm in
+                [1e-9] + [10.0 ** e for e in range(-8, 7)]) - c0
+    assert best == pytest.approx(brute, rel=1e-6)
+
+
+def test_bound_pinned_island_diverges():
+    """Lower = upper = p: every proposal is clipped back to p."""
+    isl, _, p0 = _noiseless_island(0.3)
+    p = np.float32(p0)
+    isl.lower = p.astype(np.float64)
+    isl.upper = p.astype(np.float64)
+    assert oisl.diverged_probe(p, 1e-2, isl, oisl.LMOptions())

@@ -84,7 +84,7 @@ class IslandSplit(C.Structure):
                 ("post_var", _P), ("post_dvar", _P), ("post_metrics", _P),
                 ("gram_bf16", C.c_int), ("gram_unsettled", _P), ("big_n", C.c_int),
                 ("big_ntp", C.c_int), ("big_slot", _P), ("big_isl", _P), ("big_part", _P),
-                ("post_kappa", _P)]
+                ("post_kappa", _P), ("post_hg", _P), ("post_hg_off", _P)]
 
 
 _lib = None

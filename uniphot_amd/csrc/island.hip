@@ -1509,7 +1509,22 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_post_kernel(IslandArgsPre a)
     for (int b = 1; b < nb; ++b) v = v + h0[(size_t)b * blk + e];
     h0[e] = v;
   }
+  if (a.post_hg) {
+    // the band sums of H and g in band order (the step launch's sums), for
+    // the host's post-hoc stall probes
+    double* o = a.post_hg + a.post_hg_off[isl];
+    for (int j = tid; j < nL; j += UPH_BLOCK) {
+      const size_t at = (size_t)nL * nL + j;
+      double v = h0[at];
+      for (int b = 1; b < nb; ++b) v = v + h0[(size_t)b * blk + at];
+      o[(size_t)nL * nL + j] = v;
+    }
+  }
   __syncthreads();
+  if (a.post_hg) {
+    double* o = a.post_hg + a.post_hg_off[isl];
+    for (int e = tid; e < nL * nL; e += UPH_BLOCK) o[e] = h0[e];
+  }
   // conditioning of the Gauss-Newton system in parameter units scaled to a
   // unit diagonal: kappa_1 = |Hs|_1 |Hs^-1|_1, Hs = diag(s) H diag(s),
   // s_l = H_ll^-1/2 (how much float32 model error the LM step amplifies)
@@ -1934,6 +1949,8 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
       pre.post_dvar = sp->post_dvar;
       pre.post_metrics = sp->post_metrics;
       pre.post_kappa = sp->post_kappa;
+      pre.post_hg = sp->post_hg;
+      pre.post_hg_off = sp->post_hg_off;
     }
   }
   hipStream_t s = static_cast<hipStream_t>(stream);

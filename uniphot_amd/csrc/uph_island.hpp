@@ -99,6 +99,8 @@ struct IslandArgsPre : IslandArgs {
   float* post_dvar;            // flux layout: shape-marginalisation variance increment
   float* post_metrics;         // (member slots, n_bands, 6) per-band metric sums
   float* post_kappa;           // (n_isl) 1-norm condition number of the diagonally scaled H
+  double* post_hg;             // optional: band-summed H (nL x nL) then g (nL) per island
+  const int64_t* post_hg_off;  // (n_isl) first double of the island's block in post_hg
   int gram_bf16;               // island Gram on bfloat16 matrix cores (float32 accumulate)
   int* gram_unsettled;         // (n_isl) set to 1 where a bfloat16 refinement did not settle
   // large islands: Gram tile pairs computed by a separate many-workgroup launch

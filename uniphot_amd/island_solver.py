@@ -436,6 +436,11 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                           dvar=torch.full((f_acc,), float("nan"), **f32),
                           metrics=torch.full((m_acc * nb * 6,), float("nan"), **f32),
                           kappa=torch.full((n_isl,), float("nan"), **f32))
+            hblk = nlc * nlc + nlc
+            post_t["hg_off"] = up(np.concatenate([[0], np.cumsum(hblk)[:-1]]), np.int64)
+            post_t["hg"] = torch.full((max(int(hblk.sum()), 1),), float("nan"),
+                                      dtype=torch.float64, device=dev)
+            post_hblk = hblk
             post_args = (1, P(post_t["off"]), P(post_t["buf"]), int(post_t["buf"].numel()),
                          P(post_t["var"]), P(post_t["dvar"]), P(post_t["metrics"]))
         # large islands: Gram on the many-workgroup launch (float32 Gram)
@@ -464,7 +469,9 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                               P(p_try), P(counters),
                               C.cast(ev_arr, C.c_void_p) if ev_arr is not None else None,
                               *gal["args"], *post_args, int(bf16_on), P(unsettled), *big_args,
-                              P(post_t["kappa"]) if post_on else None)
+                              P(post_t["kappa"]) if post_on else None,
+                              P(post_t["hg"]) if post_on else None,
+                              P(post_t["hg_off"]) if post_on else None)
     else:
         evs = None
         post_on = False
@@ -595,7 +602,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 defer(lambda: _record_costs(nst_h, gh))
             else:
                 _record_costs(nst_h, gh)
-        hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics", "kappa")} if post_on
+        hp = ({k: post_t[k].cpu().numpy() for k in ("var", "dvar", "metrics", "kappa", "hg")}
+              if post_on
               else None)
         h_uns = unsettled.cpu().numpy() if split and bf16_on else None
         results = []
@@ -616,6 +624,11 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 r["post_metrics"] = hp["metrics"][m0 * nb * 6:(m0 + n_c * N) * nb * 6].reshape(
                     n_c, N, nb, 6)
                 r["post_kappa"] = hp["kappa"][i0:i0 + n_c]
+                # band sums of H (nL x nL) and g (nL) per island, nL the
+                # island's live parameters (rows padded to the class's block)
+                hb = int(post_hblk[i0]) if n_c else 0
+                h0 = int(np.sum(post_hblk[:i0]))
+                r["post_hg"] = hp["hg"][h0:h0 + n_c * hb].reshape(n_c, hb)
             r["steps_run"] = int(r["nst"].max()) if n_c else 0
             r["wall_s"] = time.perf_counter() - t0
             results.append(r)
@@ -675,7 +688,7 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
 FP64_REFIT_KAPPA = 50.0
 _STATE_KEYS = ("p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
                "n_good")
-_POST_KEYS = ("post_var", "post_dvar", "post_metrics", "post_kappa")
+_POST_KEYS = ("post_var", "post_dvar", "post_metrics", "post_kappa", "post_hg")
 
 
 def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
@@ -751,6 +764,114 @@ def refit_ill_conditioned(classes, results, psf_trees: dict, max_steps: int, dst
         r["steps_run"] = max(int(r.get("steps_run", 0)), int(r64[j]["steps_run"]))
         j += 1
     return out, int(sum(int(x[2].size) for x in redo))
+
+
+# Stall probes (island_solver.py:1335-1465): a stalled island took
+# STALL_REJECT_K consecutive rejected steps at its final parameters with the
+# damping raised 10x each time, so the rejected proposals are a function of
+# the final state; the plateau probe sweeps the full damping ladder.
+STALL_REJECT_K = 4
+PLATEAU_LAMBDAS = tuple(float(np.float32(v)) for v in
+                        [1e-9] + [10.0 ** e for e in range(-8, 7)])
+PLATEAU_DLNP_TOL = 0.5   # island_plateau: best dchi2 > -2 * tol (config.py:594-610)
+
+
+def _live_slots(host, row) -> np.ndarray:
+    """Flat (slot, parameter) indices of an island's live parameters in the
+    kernels' order: real members in slot order, then x, y, logre, ee1, ee2,
+    u_n as kept."""
+    keep = host["keep"][row] & host["mmask"][row][:, None]
+    return np.flatnonzero(keep.reshape(-1))
+
+
+def _damped_proposal(p, H, g, lam, live, clipv, lower, upper):
+    """The legacy step's trial point from (H, g) at p (one_step,
+    island_solver.py:921-969): float64 solve of (H + lam diag(H) + 1e-8 I)
+    delta = -g, the step cast to float32, clipped per parameter, added in
+    float32 and clipped to the box."""
+    Hd = H.astype(np.float64).copy()
+    dg = np.diag(Hd).copy()
+    Hd[np.diag_indices_from(Hd)] = lam * dg + dg + 1e-8
+    d = (-np.linalg.solve(Hd, g.astype(np.float64))).astype(np.float32)
+    cl = clipv[live % 6]
+    d = np.clip(d, -cl, cl)
+    pt = p.astype(np.float32).copy()
+    pt[live] = pt[live] + d
+    return np.clip(pt, lower, upper)
+
+
+def stall_probes(classes, results, psf_trees: dict, dstack_dev, wstack_dev, *, sersic_dev=None,
+                 workspace_bytes: float = MAX_SCRATCH_BYTES):
+    """Post-hoc probes of the stalled islands of a fitted batch whose
+    results carry the post-fit pass (``post_hg``): ``diverged`` -- the
+    STALL_REJECT_K rejected proposals, rebuilt at lam_final 10^(j - K), are
+    all identical (a clip- or bound-pinned runaway); ``stall_dchi2`` -- the
+    best chi2 change min_j chi2(proposal(lam_j)) - chi2(p) over the damping
+    ladder 1e-9 ... 1e6 (negative: a damped step would still improve).
+    chi2 values come from zero-step device passes on the island's own
+    kernels (float64 where the island was fitted in float64).  Adds
+    ``diverged`` (n_c,) and ``stall_dchi2`` (n_c,, NaN where not stalled) to
+    each result; returns the number of islands probed."""
+    import dataclasses
+    jobs = {"fp32": [], "fp64": []}
+    for ci, ((kern, host, rows), res) in enumerate(zip(classes, results)):
+        rows = np.asarray(rows)
+        res["diverged"] = np.zeros(rows.size, bool)
+        res["stall_dchi2"] = np.full(rows.size, np.nan, np.float32)
+        if "post_hg" not in res:
+            continue
+        clipv = np.array([kern.pos_step_max, kern.pos_step_max, kern.shape_step_max,
+                          kern.shape_step_max, kern.shape_step_max, kern.un_step_max],
+                         np.float32)
+        f64 = res.get("fp64_refit", np.zeros(rows.size, bool))
+        for i in np.flatnonzero(res["stall"]):
+            live = _live_slots(host, rows[i])
+            nL = live.size
+            blk = res["post_hg"][i]
+            if nL == 0 or not np.isfinite(blk[:nL * nL + nL]).all():
+                continue
+            H = blk[:nL * nL].reshape(nL, nL)
+            g = blk[nL * nL:nL * nL + nL]
+            p = res["p"][i].reshape(-1).astype(np.float32)
+            lo = host["lower"][rows[i]].reshape(-1)
+            hi = host["upper"][rows[i]].reshape(-1)
+            try:
+                lam = float(res["lam"][i])
+                rej = [_damped_proposal(p, H, g, lam * 10.0 ** (j - STALL_REJECT_K), live,
+                                        clipv, lo, hi) for j in range(STALL_REJECT_K)]
+                res["diverged"][i] = all(np.array_equal(r, rej[0]) for r in rej[1:])
+                trial = [p] + [_damped_proposal(p, H, g, lm, live, clipv, lo, hi)
+                               for lm in PLATEAU_LAMBDAS]
+            except np.linalg.LinAlgError:
+                continue
+            jobs["fp64" if f64[i] else "fp32"].append((ci, i, np.stack(trial)))
+    n_probed = 0
+    for prec, items in jobs.items():
+        if not items:
+            continue
+        by_class = {}
+        for ci, i, trial in items:
+            by_class.setdefault(ci, []).append((i, trial))
+        batch, back = [], []
+        for ci, lst in by_class.items():
+            kern, host, rows = classes[ci]
+            rows = np.asarray(rows)
+            n_host = host["p0"].shape[0]
+            rep = np.repeat(rows[[i for i, _ in lst]], len(PLATEAU_LAMBDAS) + 1)
+            h2 = {k: (v[rep] if isinstance(v, np.ndarray) and v.ndim and v.shape[0] == n_host
+                      else v) for k, v in host.items()}
+            h2["p0"] = np.concatenate([t for _, t in lst]).reshape(rep.size, -1, 6)
+            batch.append((dataclasses.replace(kern, precision=prec), h2, np.arange(rep.size)))
+            back.append((ci, [i for i, _ in lst]))
+        out = run_island_batch(batch, psf_trees, 0, dstack_dev, wstack_dev, precision=prec,
+                               sersic_dev=sersic_dev, split=True, prerender=prec == "fp32",
+                               workspace_bytes=workspace_bytes)
+        for (ci, idx), r in zip(back, out):
+            c2 = r["chi2"].reshape(len(idx), len(PLATEAU_LAMBDAS) + 1)
+            results[ci]["stall_dchi2"][idx] = (c2[:, 1:].min(axis=1) - c2[:, 0]).astype(
+                np.float32)
+            n_probed += len(idx)
+    return n_probed
 
 
 class IslandBatchGraph:

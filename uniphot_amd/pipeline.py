@@ -30,8 +30,9 @@ from . import forced_phot_sparse as fps
 from .grid import assign_sources_to_grids, divide_image_into_grids
 from .island_plan import (DEFAULT_MAX_STEPS, JOINT_NSRC_CAP, class_host_arrays, free_n_gate,
                           plan_island_classes)
-from .island_solver import (FP64_REFIT_KAPPA, fit_float64, make_class_kernels,
-                           psf_tree_device, refit_ill_conditioned, run_class_chunk)
+from .island_solver import (FP64_REFIT_KAPPA, PLATEAU_DLNP_TOL, fit_float64,
+                           make_class_kernels, psf_tree_device, refit_ill_conditioned,
+                           run_class_chunk, stall_probes)
 from .islands import (ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands,
                       build_islands, flux_aware_radii, psf_radial_profile)
 from .psf_polynomial import evaluate_psf_at_np, psf_device_arrays
@@ -77,6 +78,9 @@ class FitConfig:
     # float32 model error (~1e-7 of the bright member's light) would move
     # the faint members by more than 1e-5 (None: never)
     fp64_contrast: float | None = FP64_CONTRAST
+    # post-hoc diverged / plateau probes of stalled islands (the reference
+    # builds and runs both by default, island_solver.py:1762-1764, :2984-3009)
+    stall_probes: bool = True
 
 
 @dataclass
@@ -104,6 +108,10 @@ class FitResult:
     island_flux_err_marg: np.ndarray = None  # (n, nb) shape-marginalised island flux error
     metrics: np.ndarray = None        # (n, 4) rchisq, fracflux, fracin, fracmasked
     flux_window: np.ndarray = None    # (n, nb)
+    island_diverged: np.ndarray = None    # (n,) stall probes (island_solver.py:1335-1465)
+    island_stall_dchi2: np.ndarray = None
+    island_plateau: np.ndarray = None     # stalled with no damped step better than -2 tol
+    island_fp64: np.ndarray = None        # (n,) island fitted on the float64 kernels
 
 
 class DeviceField:
@@ -505,6 +513,8 @@ class IslandSetFit:
     fp64_refit: np.ndarray = None     # (n,) bool: island refitted on the float64 kernels
     n_fp64_islands: int = 0
     kappa: np.ndarray = None          # (n,) the island's scaled Gauss-Newton condition number
+    diverged: np.ndarray = None       # (n,) stalled on identical (clip-pinned) proposals
+    stall_dchi2: np.ndarray = None    # (n,) best damped-step chi2 change of a stalled island
 
 
 def metrics_from_sums(sums):
@@ -619,6 +629,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     fwin = np.full((n, nb), np.nan, np.float32)
     f64src = np.zeros(n, bool)
     kap = np.full(n, np.nan, np.float32)
+    div = np.zeros(n, bool)
+    sdc = np.full(n, np.nan, np.float32)
     classes, batch, trees = [], [], {}
     for plan in plans:
         host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
@@ -712,6 +724,9 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                                              workspace_bytes=cfg.island_workspace, timer=timer)
             chunk = chunk + part64
             n_fp64 += sum(int(r.size) for _, _, r in part64)
+        if cfg.stall_probes and any("post_hg" in r for r in results):
+            stall_probes(chunk, results, trees, ds, ws, sersic_dev=sersic_dev,
+                         workspace_bytes=cfg.island_workspace)
         if sync is not None:
             sync()
         timing.append((int(sum(np.asarray(r).size for _, _, r in chunk)),
@@ -740,6 +755,9 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 f64src[src] = res["fp64_refit"][r_of]
             if "post_kappa" in res:
                 kap[src] = res["post_kappa"][r_of]
+            if "stall_dchi2" in res:
+                div[src] = res["diverged"][r_of]
+                sdc[src] = res["stall_dchi2"][r_of]
             if "post_var" in res:
                 pvar[src] = res["post_var"][mm]
                 dv = res["post_dvar"].transpose(0, 2, 1)[mm]
@@ -749,7 +767,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 fwin[src] = fw
     n_fit = sum(p.n_islands for p in plans)
     return IslandSetFit(fx, fy, flr, fe1, fe2, fsn, pfit, flux, ferr, conv, stall, nst, fitted,
-                        n_fit, classes, timing, pvar, ferr_m, mets, fwin, f64src, n_fp64, kap)
+                        n_fit, classes, timing, pvar, ferr_m, mets, fwin, f64src, n_fp64, kap,
+                        div, sdc)
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,
@@ -845,7 +864,9 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                                  giants=giants)
     return FitResult(flux, np.sqrt(np.maximum(var, 0.0)), s1_flux, s1_var, fx, fy, flr, fe1,
                      fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec,
-                     isf.param_var, isf.flux, isf.flux_err_marg, isf.metrics, isf.flux_window)
+                     isf.param_var, isf.flux, isf.flux_err_marg, isf.metrics, isf.flux_window,
+                     isf.diverged, isf.stall_dchi2,
+                     stall & (isf.stall_dchi2 > -2.0 * PLATEAU_DLNP_TOL), isf.fp64_refit)
 
 
 def psf_bank(psfs, device="cuda"):
