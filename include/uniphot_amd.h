@@ -52,6 +52,12 @@ typedef struct uph_lm_options {
   float step_tol;       /* prong 2: position step < step_tol px (0.1)      */
   float shape_step_tol; /* prong 2: shape steps < shape_step_tol (0.01)    */
   int max_steps;        /* LM steps per island (8 main + 22 re-queue = 30) */
+  int resume;           /* 0: fresh state (lam 1e-3, chi2 at p0, counters 0).
+                           1: resume each island from p0 and the LM state the
+                           caller wrote into the outputs chi2, lam, consec,
+                           nrej and nst (run_class_chunk's init_state,
+                           island_solver.py:2902-2930); conv and stall start
+                           false                                               */
 } uph_lm_options;
 
 /*

@@ -86,7 +86,7 @@ def test_batch_launch_equals_per_class(cuda):
              for p, h in zip(plans, hosts)]
     got = run_island_batch(batch, {55: tree}, 30, dd, ww)
     for (kern, host, rows), rb in zip(batch, got):
-        ra = run_class_chunk(kern, tree, host, rows, None, 30, dd, ww, pad)
+        ra = run_class_chunk(kern, tree, host, rows, None, 30, dd, ww, pad, form="persistent")
         for k in ("p", "fluxes", "ferr", "chi2", "nst", "conv", "stall", "n_good"):
             assert np.array_equal(ra[k], rb[k]), k
 

@@ -301,3 +301,56 @@ def test_bucketing_a_subset_of_islands_equals_bucketing_them_in_the_full_set():
         rows = np.searchsorted(f.island_ids, ids[b.island_ids])
         assert np.array_equal(f.island_ids[rows], ids[b.island_ids])
         assert np.array_equal(f.member_idx[rows], b.member_idx)
+
+
+def test_catalog_source_params_rules():
+    """source_model.py:141-215 rules: TYPE strings (case and spaces
+    ignored, unknown -> STAR), Re fallback and clip, ab = 1 - ELL clipped
+    (NaN ELL -> 0, no column -> 0.8), THETA -> phi, DEV n fixed at 4,
+    SERSIC n from the column where finite else 2, clipped to [0.29, 6.3];
+    the same result from a DataFrame with lower-case columns."""
+    import pandas as pd
+    from uniphot_amd.source_model import (catalog_source_params,
+                                          ellipse_soft_from_re_ab_phi_np,
+                                          ellipse_soft_to_re_ab_phi_np, type_codes)
+    assert type_codes([" exp", "DEV", "ser", "Sersic", "psf", "QSO", ""]).tolist() == \
+        [1, 2, 3, 3, 0, 0, 0]
+    t = np.array(["STAR", "EXP", "DEV", "SERSIC", "SERSIC", "DEV"])
+    re = np.array([np.nan, 0.1, 500.0, 2.0, 3.0, 4.0])
+    ell = np.array([0.5, np.nan, 0.95, 0.2, 0.0, 0.3])
+    th = np.array([30.0, -30.0, np.nan, 0.0, 90.0, -90.0])
+    sn = np.array([np.nan, 5.0, 7.0, 9.0, np.nan, 1.0])
+    st, lr, e1, e2, n = catalog_source_params(t, re, ell, th, sn)
+    assert st.tolist() == [0, 1, 2, 3, 3, 2]
+    np.testing.assert_allclose(np.exp(lr), [3.0, 0.3, 100.0, 2.0, 3.0, 4.0], rtol=1e-6)
+    np.testing.assert_allclose(n, [1.0, 1.0, 4.0, 6.3, 2.0, 4.0], rtol=1e-7)
+    phi = -np.where(np.nan_to_num(th) >= 0, 90.0 - np.nan_to_num(th), -90.0 - np.nan_to_num(th))
+    ab = np.clip(1.0 - np.nan_to_num(ell), 0.1, 1.0)
+    want = ellipse_soft_from_re_ab_phi_np(np.clip(np.nan_to_num(re, nan=3.0), 0.3, 100), ab, phi)
+    for a, b in zip((lr, e1, e2), want):
+        np.testing.assert_array_equal(a, b)
+    df = pd.DataFrame(dict(type=t, re=re, ell=ell, theta=th, sersic_n=sn))
+    for a, b in zip(catalog_source_params(df), (st, lr, e1, e2, n)):
+        np.testing.assert_array_equal(a, b)
+    # no ELL column: ab = 0.8
+    _, _, f1, f2, _ = catalog_source_params(t, re, None, th, sn)
+    _, ab8, _ = ellipse_soft_to_re_ab_phi_np(lr, f1, f2)
+    np.testing.assert_allclose(ab8, 0.8, rtol=1e-6)
+
+
+def test_ellipse_soft_round_trip_and_round_case():
+    """to(from(re, ab, phi)) returns re, ab and phi mod 180 (float32); a
+    round source (ab = 1) has ee1 = ee2 = 0."""
+    from uniphot_amd.source_model import (ellipse_soft_from_re_ab_phi_np,
+                                          ellipse_soft_to_re_ab_phi_np)
+    rng = np.random.default_rng(5)
+    re = rng.uniform(0.2, 50, 200)
+    ab = rng.uniform(0.15, 1.0, 200)
+    phi = rng.uniform(0, 180, 200)
+    r2, ab2, p2 = ellipse_soft_to_re_ab_phi_np(*ellipse_soft_from_re_ab_phi_np(re, ab, phi))
+    np.testing.assert_allclose(r2, re, rtol=1e-6)
+    np.testing.assert_allclose(ab2, ab, rtol=1e-5)
+    d = np.abs(((p2 - phi) + 90.0) % 180.0 - 90.0)
+    assert d.max() < 1e-3
+    _, z1, z2 = ellipse_soft_from_re_ab_phi_np(np.array([2.0]), np.array([1.0]), np.array([33.0]))
+    assert z1[0] == 0.0 and z2[0] == 0.0

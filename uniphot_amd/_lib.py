@@ -42,7 +42,7 @@ class LMOptions(C.Structure):
     _fields_ = [("reg", C.c_float), ("pos_step_max", C.c_float),
                 ("shape_step_max", C.c_float), ("un_step_max", C.c_float),
                 ("ftol_rel", C.c_float), ("step_tol", C.c_float),
-                ("shape_step_tol", C.c_float), ("max_steps", C.c_int)]
+                ("shape_step_tol", C.c_float), ("max_steps", C.c_int), ("resume", C.c_int)]
 
 
 class IslandClass(C.Structure):

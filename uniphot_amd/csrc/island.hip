@@ -1043,13 +1043,23 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_lm_kernel(IslandArgs a) {
     // ---- island description into LDS ----
     describe_island(a, sh, a.p0);
     if (tid == 0) {
-      sh.lam = 1e-3;
-      sh.consec = sh.nrej = sh.nst = 0;
+      if (a.resume) {
+        sh.lam = a.lam_out[isl];
+        sh.chi2 = a.chi2_out[isl];
+        sh.consec = a.consec_out[isl];
+        sh.nrej = a.nrej_out[isl];
+        sh.nst = a.nst_out[isl];
+      } else {
+        sh.lam = 1e-3;
+        sh.consec = sh.nrej = sh.nst = 0;
+      }
       sh.conv = sh.stall = 0;
       if (sh.Nr == 0) sh.conv = 1;  // nothing to fit (all slots padded)
     }
     __syncthreads();
-    if (sh.Nr > 0) {
+    if (a.resume) {
+      if (sh.Nr == 0 && tid == 0) sh.chi2 = 0.0;
+    } else if (sh.Nr > 0) {
       const double c0 = chi2_all<Real>(a, sh, isl, sh.p, planes, ws, gws, G, part, sm);
       if (tid == 0) sh.chi2 = c0;
       __syncthreads();
@@ -1396,7 +1406,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
   double* h0 = a.hg + off0;
   const size_t c_at = (size_t)nL * nL + nL;  // chi2 entry of a band block
   if (mode == kSplitStepInit) {
-    if (tid == 0) {
+    if (tid == 0 && !a.resume) {
       double c = 0.0;
       if (sh.Nr > 0)
         for (int b = 0; b < nb; ++b) c += h0[(size_t)b * blk + c_at];
@@ -1405,6 +1415,10 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
       a.consec_out[isl] = 0;
       a.nrej_out[isl] = 0;
       a.nst_out[isl] = 0;
+    } else if (tid == 0 && sh.Nr == 0) {
+      a.chi2_out[isl] = 0.0;
+    }
+    if (tid == 0) {
       a.conv_out[isl] = uint8_t(sh.Nr == 0);
       a.stall_out[isl] = 0;
       a.ngood_out[isl] = 0;
@@ -1614,6 +1628,7 @@ static int fill_island_args(uph::IslandArgs& a, const uph_island_inputs* in,
                 "uph_island_fit: precision=%d", precision);
   UPH_CHECK_ARG(scratch != nullptr, "uph_island_fit: null scratch");
   a.max_steps = opt->max_steps;
+  a.resume = opt->resume;
   a.reg = opt->reg;
   a.pos_step_max = opt->pos_step_max;
   a.shape_step_max = opt->shape_step_max;

@@ -12,6 +12,7 @@ constexpr int kMaxLive = 6 * kMaxMembers;
 struct IslandArgs {
   int n_isl, N, B, T, nt, nb, nL_max, Hp, Wp;
   int max_steps;
+  int resume;  // LM state from the outputs (see uph_lm_options)
   float reg, pos_step_max, shape_step_max, un_step_max, ftol_rel, step_tol, shape_step_tol;
   const float* dstack;
   const float* wstack;

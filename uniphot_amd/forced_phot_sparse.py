@@ -2,11 +2,13 @@
 
 Mirrors ``tract7dt/fit/grid_gpu/forced_phot_sparse.py``:
 ``build_neighbor_list`` (:272-334, host kd-tree, rectangular stamp-overlap
-predicate, k_max nearest kept), ``stamp_top_left`` (:255-264) and the
-per-band solve ``solve_band_sparse`` (:683-709) with its result type.  The
-device work is batched: ``solve_problems`` renders every (source, band)
-stamp with ``uph_render_stamps`` and solves every (grid, band) problem in
-one ``uph_band_solve`` launch.
+predicate, k_max nearest kept), ``stamp_top_left`` (:255-264), the
+per-band inputs ``GridBandInputs`` (:65-105) and the per-band solve
+``solve_band_sparse`` with its launch / collect halves (:583-709) and
+result type.  The device work is batched: ``solve_problems`` renders every
+(source, band) stamp with ``uph_render_stamps`` and solves every (grid,
+band) problem in one ``uph_band_solve`` launch; the reference-shaped entry
+points are one-problem batches of it.
 """
 from __future__ import annotations
 
@@ -28,6 +30,109 @@ class SparseForcedPhotResult:
     flux_var: np.ndarray
     bg_var: np.ndarray
     converged: bool
+
+
+@dataclass
+class GridBandInputs:
+    """One band of one grid (forced_phot_sparse.py:65-105): ``data``,
+    ``invvar``, ``pix_good`` (band_h, band_w) of the grid's window (numpy or
+    device tensors), source positions in full-image pixels, the band PSF
+    (a :class:`PsfPolynomial`), the window's full-image corner (grid_x0,
+    grid_y0) and the background normalisation; optional source types and
+    shapes (EXP/DEV/SERSIC render through the Fourier mixture)."""
+    data: object
+    invvar: object
+    pix_good: object
+    pos_x: object
+    pos_y: object
+    psf_jax: object
+    band_h: int
+    band_w: int
+    grid_x0: int
+    grid_y0: int
+    bg_x_origin: float
+    bg_x_scale: float
+    bg_y_origin: float
+    bg_y_scale: float
+    bg_basis_flat: object = None
+    source_type: object = None
+    logre: object = None
+    ee1: object = None
+    ee2: object = None
+    sersic_n: object = None
+    render_backend: str = "direct"
+
+
+@dataclass
+class _PendingBandSolve:
+    """A launched one-problem band solve (``launch_band_sparse``)."""
+    problem: object
+    args: tuple
+    kw: dict
+
+
+def launch_band_sparse(inputs: GridBandInputs, neighbor_idx, neighbor_mask, *,
+                       reg: float = 1e-9, bg_poly_order: int = 2) -> _PendingBandSolve:
+    """First half of :func:`solve_band_sparse` (forced_phot_sparse.py:
+    583-680): validates and stages one (grid, band) problem; the device
+    work runs in :func:`collect_band_sparse`."""
+    import torch
+    from .psf_polynomial import psf_device_arrays
+    psf = inputs.psf_jax
+    T = int(psf.H_psf)
+    if int(psf.W_psf) != T:
+        raise ValueError(f"square PSF stamps only, got {psf.H_psf}x{psf.W_psf}")
+    h, w = int(inputs.band_h), int(inputs.band_w)
+    st = None if inputs.source_type is None else np.asarray(inputs.source_type, np.int32)
+    if st is not None and inputs.render_backend == "direct" and np.any(st != SOURCE_STAR):
+        raise NotImplementedError(
+            "render_backend='direct' for extended sources is not part of this build; "
+            "the reference's 'auto' resolves to 'fft' whenever extended sources are present")
+    nbr = np.where(np.asarray(neighbor_mask, bool), np.asarray(neighbor_idx, np.int64), -1)
+    px = np.asarray(inputs.pos_x, np.float32)
+    py = np.asarray(inputs.pos_y, np.float32)
+
+    def dev(a, dt):
+        if isinstance(a, torch.Tensor):
+            return a.to(device="cuda", dtype=dt).reshape(1, h, w).contiguous()
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(a), dt).reshape(1, h, w)).cuda()
+
+    good = dev(inputs.pix_good, np.uint8) if not isinstance(inputs.pix_good, torch.Tensor) \
+        else inputs.pix_good.to("cuda").to(torch.uint8).reshape(1, h, w).contiguous()
+    iv = dev(inputs.invvar, np.float32)
+    iv = torch.where(good.bool(), iv, torch.zeros((), dtype=iv.dtype, device=iv.device))
+    c, g = psf_device_arrays([psf], T)
+    gx0, gy0 = int(inputs.grid_x0), int(inputs.grid_y0)
+    prob = BandProblem(0, 0, 0, h, w, px, py, nbr.astype(np.int32), stype=st,
+                       logre=None if inputs.logre is None else np.asarray(inputs.logre, np.float32),
+                       ee1=None if inputs.ee1 is None else np.asarray(inputs.ee1, np.float32),
+                       ee2=None if inputs.ee2 is None else np.asarray(inputs.ee2, np.float32),
+                       sersic_n=None if inputs.sersic_n is None
+                       else np.asarray(inputs.sersic_n, np.float32))
+    up = lambda a: torch.from_numpy(a).cuda()   # noqa: E731
+    args = ([prob], up(c), up(g), T, c.shape[1], dev(inputs.data, np.float32), iv, good)
+    kw = dict(bg_order=int(bg_poly_order), img_origin=(gx0, gy0), reg=float(reg),
+              k_max=int(nbr.shape[1]) if nbr.ndim == 2 else DEFAULT_K_MAX,
+              bg_geo=(float(inputs.bg_x_origin), float(inputs.bg_x_scale),
+                      float(inputs.bg_y_origin), float(inputs.bg_y_scale)))
+    return _PendingBandSolve(prob, args, kw)
+
+
+def collect_band_sparse(pending: _PendingBandSolve) -> SparseForcedPhotResult:
+    """Second half of :func:`solve_band_sparse`: the device solve and the
+    copy back; ``converged`` = every flux and coefficient finite."""
+    return solve_problems(*pending.args, **pending.kw)[0]
+
+
+def solve_band_sparse(inputs: GridBandInputs, neighbor_idx, neighbor_mask, reg: float = 1e-9,
+                      bg_poly_order: int = 2) -> SparseForcedPhotResult:
+    """Fluxes and background coefficients of one band of one grid
+    (forced_phot_sparse.py:683-709): stamps rendered on the device (unit-sum
+    renormalised when every source is a star, forced_phot_sparse.py:
+    624-637), the normal equations over stamp overlaps and the background
+    basis, float64 solve and inverse diagonal."""
+    return collect_band_sparse(launch_band_sparse(inputs, neighbor_idx, neighbor_mask, reg=reg,
+                                                  bg_poly_order=bg_poly_order))
 
 
 def build_neighbor_list(pos_x, pos_y, H_psf: int, W_psf: int, k_max: int = DEFAULT_K_MAX):
@@ -132,16 +237,20 @@ def solve_problems(problems, psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, d
         np.zeros(0, dt)
     x = cat(lambda p: p.pos_x, np.float32)
     y = cat(lambda p: p.pos_y, np.float32)
-    # per-problem scalars spread over the problem's sources
-    org = [p.origin if p.origin is not None else (p.x0, p.y0) for p in problems]
+    # per-problem scalars spread over the problem's sources; windows are in
+    # the arrays' pixels, whose (0, 0) is full-image pixel img_origin, and
+    # the sub-pixel remainder is taken against the window's full-image
+    # corner unless a problem names another origin
+    ox0, oy0 = int(img_origin[0]), int(img_origin[1])
+    org = [p.origin if p.origin is not None else (p.x0 + ox0, p.y0 + oy0) for p in problems]
     gx0 = np.repeat(np.array([o[0] for o in org], np.float32), ns)
     gy0 = np.repeat(np.array([o[1] for o in org], np.float32), ns)
     band = np.repeat(np.array([p.band for p in problems], np.int32), ns)
     ixt = np.rint(x - gx0).astype(np.int32) - T // 2
     iyt = np.rint(y - gy0).astype(np.int32) - T // 2
     # window-local top-left (window origin may differ from the rounding origin)
-    ixt -= np.repeat(np.array([p.x0 - o[0] for p, o in zip(problems, org)], np.int32), ns)
-    iyt -= np.repeat(np.array([p.y0 - o[1] for p, o in zip(problems, org)], np.int32), ns)
+    ixt -= np.repeat(np.array([p.x0 + ox0 - o[0] for p, o in zip(problems, org)], np.int32), ns)
+    iyt -= np.repeat(np.array([p.y0 + oy0 - o[1] for p, o in zip(problems, org)], np.int32), ns)
     nbr = np.concatenate([p.nbr_idx[:, :k_max] for p in problems]).astype(np.int32) if ntot \
         else np.zeros((0, k_max), np.int32)
     win = np.array([[p.band, p.x0, p.y0, p.h, p.w] for p in problems], np.int32)

@@ -11,7 +11,8 @@ inverse sigma is zero, the reference's exact no-op).
 """
 from __future__ import annotations
 
-from dataclasses import dataclass
+import dataclasses
+from dataclasses import dataclass, field
 
 import numpy as np
 
@@ -27,6 +28,74 @@ DEFAULT_UN_STEP_MAX = 0.5
 DEFAULT_SHAPE_STEP_TOL = 0.01
 DEFAULT_REG = 1e-9
 JOINT_NSRC_CAP = 32
+
+
+@dataclass
+class IslandFitResult:
+    """Per-source results of every source in a fitted or giant island
+    (island_plan.py:220-319), rows aligned with ``source_idx``; island-level
+    quantities replicated onto each member."""
+    bandnames: list
+    source_idx: np.ndarray
+    pos_x: np.ndarray
+    pos_y: np.ndarray
+    logre: np.ndarray
+    ee1: np.ndarray
+    ee2: np.ndarray
+    sersic_n: np.ndarray
+    fluxes: np.ndarray              # (n_fit, n_bands)
+    flux_err: np.ndarray
+    chi2: np.ndarray
+    n_steps: np.ndarray
+    converged: np.ndarray
+    free_n: np.ndarray
+    stalled: np.ndarray = None
+    diverged: np.ndarray = None
+    stall_dchi2: np.ndarray = None
+    railed: np.ndarray = None
+    chi2_nu: np.ndarray = None
+    island_id: np.ndarray = None
+    island_nsrc: np.ndarray = None
+    island_giant: np.ndarray = None
+    re_upper_pix: np.ndarray = None
+    pos_x_err: np.ndarray = None
+    pos_y_err: np.ndarray = None
+    re_err: np.ndarray = None
+    re_prof_lo: np.ndarray = None
+    re_prof_hi: np.ndarray = None
+    ell_err: np.ndarray = None
+    theta_err: np.ndarray = None
+    sersic_n_err: np.ndarray = None
+    pos_x_prior_frac: np.ndarray = None
+    pos_y_prior_frac: np.ndarray = None
+    re_prior_frac: np.ndarray = None
+    ell_prior_frac: np.ndarray = None
+    theta_prior_frac: np.ndarray = None
+    sersic_n_prior_frac: np.ndarray = None
+    rchisq: np.ndarray = None
+    fracflux: np.ndarray = None
+    fracin: np.ndarray = None
+    fracmasked: np.ndarray = None
+    flux_win: np.ndarray = None
+    flux_marg_var: np.ndarray = None
+    bucket_stats: list = field(default_factory=list)
+
+    @property
+    def requeue_idx(self) -> np.ndarray:
+        return self.source_idx[~self.converged]
+
+    @classmethod
+    def from_rows(cls, bandnames, rows: list) -> "IslandFitResult":
+        """Concatenate unpacked / giant row dicts (source order within each)."""
+        keys = [f.name for f in dataclasses.fields(cls)
+                if f.name not in ("bandnames", "source_idx", "n_steps", "converged",
+                                  "stalled", "bucket_stats")]
+        cat = lambda k: np.concatenate([r[k] for r in rows]) if rows else np.empty(0)  # noqa
+        return cls(bandnames=list(bandnames), source_idx=cat("src").astype(np.int64),
+                   n_steps=cat("n_steps").astype(np.int32),
+                   converged=cat("converged").astype(bool),
+                   stalled=cat("stalled").astype(bool),
+                   **{k: cat(k) for k in keys if not rows or k in rows[0]})
 
 
 @dataclass

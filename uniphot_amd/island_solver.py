@@ -57,10 +57,10 @@ class ClassKernels:
     def n_live_max(self) -> int:
         return 2 * self.N if self.kind == "star" else 6 * self.N
 
-    def options(self, max_steps: int) -> _lib.LMOptions:
+    def options(self, max_steps: int, resume: bool = False) -> _lib.LMOptions:
         return _lib.LMOptions(self.reg, self.pos_step_max, self.shape_step_max,
                               self.un_step_max, self.chi2_ftol_rel, self.step_tol_pix,
-                              self.shape_step_tol, int(max_steps))
+                              self.shape_step_tol, int(max_steps), int(bool(resume)))
 
 
 def make_class_kernels(plan, n_bands: int, n_terms: int, *, reg: float = DEFAULT_REG,
@@ -94,15 +94,135 @@ def psf_tree_device(psfs, tier: int, device="cuda") -> PsfTree:
 
 def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
                     width: int | None, max_steps: int, dstack_dev, wstack_dev, pad: int,
-                    init_state=None, stream=None, sersic_dev=None, timer=None) -> dict:
+                    init_state=None, compute_param_errors: bool = False,
+                    want_residual: bool = False, compute_source_metrics: bool = False,
+                    compute_flux_marg: bool = False, compute_re_profile: bool = False, *,
+                    stream=None, sersic_dev=None, timer=None, form: str = "split",
+                    stall_probes_on: bool = True) -> dict:
     """Fit rows ``rows`` of a class host dict on the device and return the
-    reference's per-row output dict (p, fluxes, ferr, chi2, conv, stall,
-    nst, lam, consec, nrej, n_good, steps_run, wall_s)."""
-    import torch
-    if init_state is not None:
-        raise NotImplementedError("resuming LM state is not part of this build")
+    reference's per-row output dict (island_solver.py:2793-3131): p (n_c,
+    N, 6), fluxes and ferr (n_c, nb, N), perr (n_c, N, 13: marginal
+    variances of x, y, logre, ee1, ee2, u_n, cov(ee1, ee2), then six prior
+    fractions -- NaN, no priors in this build), femarg (n_c, nb, N),
+    smetrics (n_c, N, 4), swin (n_c, nb, N), n_good, chi2, conv, stall,
+    diverged, stall_dchi2, railed, n_same, prej, nst, lam, consec, nrej,
+    wall_s, steps_run; None for what was not requested.  ``init_state``
+    (p, lam, chi2, consec, nrej, nst per row) resumes the LM from that
+    state.  ``width`` is accepted for the call signature: the device work
+    queue replaces the vmap width.  ``form="persistent"`` runs the one-class
+    persistent kernel (no post-fit pass, no probes)."""
     if host.get("has_priors"):
         raise NotImplementedError("catalog priors are not part of this build")
+    if want_residual or compute_re_profile:
+        raise NotImplementedError("island residual export and the Re profile scan (opt-in in "
+                                  "the reference) are not part of this build")
+    if form == "persistent":
+        return _run_class_persistent(kernels, psf_tree, host, rows, max_steps, dstack_dev,
+                                     wstack_dev, init_state, stream, sersic_dev, timer)
+    if psf_tree.T != kernels.T:
+        raise ValueError(f"psf tree tier {psf_tree.T} != class tier {kernels.T}")
+    t0 = time.perf_counter()
+    rows = np.asarray(rows, np.int64)
+    n_c, N, nb = int(rows.size), kernels.N, kernels.n_bands
+    want_post = compute_param_errors or compute_source_metrics or compute_flux_marg
+    post = want_post or stall_probes_on
+    trees = {kernels.T: psf_tree}
+    cls = [(kernels, host, rows)]
+    inits = None if init_state is None else [init_state]
+    if kernels.precision == "fp64":
+        res = fit_float64(cls, trees, max_steps, dstack_dev, wstack_dev, sersic_dev=sersic_dev,
+                          post=post, init_states=inits)[0]
+    else:
+        res = run_island_batch(cls, trees, max_steps, dstack_dev, wstack_dev,
+                               sersic_dev=sersic_dev, timer=timer, split=True, prerender=True,
+                               post=post, init_states=inits, stream=stream)[0]
+    if stall_probes_on and "post_hg" in res:
+        stall_probes(cls, [res], trees, dstack_dev, wstack_dev, sersic_dev=sersic_dev)
+    out = {k: res[k] for k in ("p", "fluxes", "ferr", "n_good", "chi2", "conv", "stall", "nst",
+                               "lam", "consec", "nrej")}
+    out["diverged"] = res.get("diverged", np.zeros(n_c, bool))
+    out["stall_dchi2"] = res.get("stall_dchi2", np.full(n_c, np.nan, np.float32))
+    out["railed"] = np.zeros(n_c, bool)          # trust-region mode only
+    out["n_same"] = np.zeros(n_c, np.int32)      # "changed" stall counting only
+    out["prej"] = np.zeros((n_c, 6 * N), np.float32)
+    out["perr"] = out["femarg"] = out["smetrics"] = out["swin"] = None
+    out["reprof"] = out["resid"] = out["resid_w"] = None
+    if want_post and "post_var" in res:
+        from .pipeline import metrics_from_sums
+        if compute_param_errors:
+            perr = np.full((n_c, N, 13), np.nan, np.float32)
+            perr[:, :, :7] = res["post_var"]
+            out["perr"] = perr
+        if compute_flux_marg:
+            out["femarg"] = res["post_dvar"]
+        if compute_source_metrics:
+            m4, fw = metrics_from_sums(res["post_metrics"])
+            out["smetrics"] = m4.astype(np.float32)
+            out["swin"] = np.transpose(fw, (0, 2, 1)).astype(np.float32)
+    out["steps_run"] = res["steps_run"]
+    out["wall_s"] = time.perf_counter() - t0
+    return out
+
+
+def _is_device_mem_error(exc: BaseException) -> bool:
+    """A device-memory failure a smaller chunk may avoid: the library's
+    UPH_ERR_OOM, or torch's own out-of-memory error."""
+    if isinstance(exc, _lib.UphOOM):
+        return True
+    e = f"{type(exc).__name__}: {exc}".upper()
+    return any(k in e for k in ("OUT OF MEMORY", "OUT_OF_MEMORY", "RESOURCE_EXHAUSTED"))
+
+
+def _merge_chunk_outputs(parts: list) -> dict:
+    """Row-concatenate the result dicts of a split retry."""
+    out = dict(parts[0])
+    for k, v0 in parts[0].items():
+        if k in ("wall_s", "steps_run"):
+            continue
+        if v0 is None:
+            out[k] = None
+        elif isinstance(v0, np.ndarray):
+            out[k] = np.concatenate([p[k] for p in parts], axis=0)
+    out["wall_s"] = float(sum(p["wall_s"] for p in parts))
+    out["steps_run"] = int(max(p["steps_run"] for p in parts))
+    return out
+
+
+def run_class_chunk_resilient(kernels, psf_tree, host, rows, width, max_steps, dstack_dev,
+                              wstack_dev, pad, *, init_state=None, log_fn=None, **kw) -> dict:
+    """:func:`run_class_chunk`, retried as two half chunks (recursively) on
+    a device-memory failure (island_solver.py:3165-3230); anything else
+    propagates, and a single-row chunk that fails re-raises."""
+    rows = np.asarray(rows)
+    try:
+        out = run_class_chunk(kernels, psf_tree, host, rows, width, max_steps, dstack_dev,
+                              wstack_dev, pad, init_state=init_state, **kw)
+        out["halved"] = False
+        return out
+    except Exception as exc:                                    # noqa: BLE001
+        if not _is_device_mem_error(exc) or rows.size <= 1:
+            raise
+        half = (rows.size + 1) // 2
+        msg = (f"island chunk of {rows.size} hit {type(exc).__name__} ({str(exc)[:80]}) -- "
+               f"retrying as two chunks")
+        (log_fn or (lambda m: print("island: " + m, flush=True)))(msg)
+        parts = []
+        for sl in (slice(0, half), slice(half, rows.size)):
+            sub = None if init_state is None else {k: np.asarray(v)[sl]
+                                                   for k, v in init_state.items()}
+            parts.append(run_class_chunk_resilient(kernels, psf_tree, host, rows[sl],
+                                                   width, max_steps, dstack_dev, wstack_dev,
+                                                   pad, init_state=sub, log_fn=log_fn, **kw))
+        merged = _merge_chunk_outputs(parts)
+        merged["halved"] = True
+        return merged
+
+
+def _run_class_persistent(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
+                          max_steps: int, dstack_dev, wstack_dev, init_state, stream,
+                          sersic_dev, timer) -> dict:
+    """One class on the persistent single-class kernel (uph_island_fit)."""
+    import torch
     t0 = time.perf_counter()
     rows = np.asarray(rows, np.int64)
     n_c = int(rows.size)
@@ -133,7 +253,8 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
         y0b=up(host["y0b"][rows], np.float32), stype=up(host["st"][rows], np.int32),
         n_frozen=up(host["sn"][rows], np.float32), free_n=up(host["fn_flag"][rows], np.uint8),
         mmask=up(mm, np.uint8), keep=up(host["keep"][rows], np.uint8),
-        p0=up(host["p0"][rows].reshape(n_c, nP), np.float32),
+        p0=up((host["p0"][rows] if init_state is None else init_state["p"]).reshape(n_c, nP),
+              np.float32),
         lower=up(host["lower"][rows].reshape(n_c, nP), np.float32),
         upper=up(host["upper"][rows].reshape(n_c, nP), np.float32), order=up(order, np.int32))
     f32 = dict(dtype=torch.float32, device=dev)
@@ -146,6 +267,10 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
         consec=torch.empty(n_c, dtype=torch.int32, device=dev),
         nrej=torch.empty(n_c, dtype=torch.int32, device=dev),
         n_good=torch.empty(n_c, dtype=torch.int32, device=dev))
+    if init_state is not None:
+        for k, dt in (("chi2", np.float64), ("lam", np.float64), ("consec", np.int32),
+                      ("nrej", np.int32), ("nst", np.int32)):
+            outs[k].copy_(torch.from_numpy(np.asarray(init_state[k], dt).reshape(-1)))
     prec = _lib.UPH_PRECISION_FP64 if kernels.precision == "fp64" else _lib.UPH_PRECISION_FP32
     L = _lib.lib()
     per = L.uph_island_scratch_per_block(N, B, T, kernels.n_live_max, prec)
@@ -165,7 +290,8 @@ def run_class_chunk(kernels: ClassKernels, psf_tree: PsfTree, host: dict, rows,
     if n_c:
         ev = timer.begin("island_lm_kernel") if timer is not None else None
         _lib.check(L.uph_island_fit(C.byref(cls), C.byref(inp), C.byref(out),
-                                    C.byref(kernels.options(max_steps)), prec, P(scratch),
+                                    C.byref(kernels.options(max_steps, init_state is not None)),
+                                    prec, P(scratch),
                                     scratch.numel(), P(counter), _lib.stream_ptr(stream)))
         if ev is not None:
             timer.end(ev, dict(kind="island", n=n_c, B=B, N=N, nb=nb, outs=outs, host=host,
@@ -261,7 +387,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                          split: bool = False, prerender: bool = True,
                          post: bool = False, gram: str = "fp32",
                          workspace_bytes: float = MAX_SCRATCH_BYTES,
-                         gram_split_min: float = GRAM_SPLIT_MIN) -> "PreparedIslandBatch":
+                         gram_split_min: float = GRAM_SPLIT_MIN,
+                         init_states=None) -> "PreparedIslandBatch":
     """Stage the islands of several classes for ONE launch sequence.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
@@ -286,10 +413,16 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     scaled condition number of each island's Gauss-Newton matrix
     (``post_kappa`` (n,)).
     ``gram="bf16"`` (split + pre-render) builds the island Gram matrices on
-    bfloat16 matrix cores with float32 accumulation.
+    bfloat16 matrix cores with float32 accumulation.  ``init_states`` (one
+    dict per class with p (n_c, N, 6), lam, chi2, consec, nrej, nst) resumes
+    every island from that LM state (island_solver.py:2902-2930); all
+    classes or none carry one.
     """
     import torch
     t0 = time.perf_counter()
+    resume = init_states is not None and any(st is not None for st in init_states)
+    if resume and any(st is None for st in init_states):
+        raise ValueError("init_states: every class or none resumes")
     if not classes:
         return PreparedIslandBatch(lambda stream=None: None, lambda: None, lambda: [])
     k0 = classes[0][0]
@@ -336,7 +469,11 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         cat["fn"].append(host["fn_flag"][rows].ravel())
         cat["mm"].append(mm.ravel())
         cat["keep"].append(host["keep"][rows].ravel())
-        cat["p0"].append(host["p0"][rows].ravel())
+        if resume:
+            st0 = init_states[len(spans) - 1]
+            cat["p0"].append(np.asarray(st0["p"], np.float32).reshape(n_c, -1).ravel())
+        else:
+            cat["p0"].append(host["p0"][rows].ravel())
         cat["lo"].append(host["lower"][rows].ravel())
         cat["hi"].append(host["upper"][rows].ravel())
         per = max(per, L.uph_island_scratch_per_block(N, B, kern.T, kern.n_live_max, prec))
@@ -395,7 +532,13 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     out = _lib.IslandOutputs(*(P(outs[k]) for k in (
         "p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "consec", "nrej",
         "n_good")))
-    opts = k0.options(max_steps)
+    opts = k0.options(max_steps, resume=resume)
+    if resume:
+        # the LM state the kernels resume from, in the outputs they update
+        for k, dt in (("chi2", np.float64), ("lam", np.float64), ("consec", np.int32),
+                      ("nrej", np.int32), ("nst", np.int32)):
+            v = np.concatenate([np.asarray(st0[k], dt).reshape(-1) for st0 in init_states])
+            outs[k].copy_(torch.from_numpy(v))
     if split:
         nlc = np.asarray([s_[3] for s_ in shape], np.int64)
         blk = nlc * nlc + nlc + 2
@@ -664,14 +807,15 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
                      precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
                      split: bool = False, prerender: bool = True, post: bool = False,
                      gram: str = "fp32", workspace_bytes: float = MAX_SCRATCH_BYTES,
-                     gram_split_min: float = GRAM_SPLIT_MIN):
+                     gram_split_min: float = GRAM_SPLIT_MIN, init_states=None):
     """Fit the islands of several classes in one launch sequence (see
     :func:`prepare_island_batch`) and return one result dict per class, in
     the same layout as :func:`run_class_chunk`."""
     pb = prepare_island_batch(classes, psf_trees, max_steps, dstack_dev, wstack_dev,
                               precision=precision, sersic_dev=sersic_dev, timer=timer,
                               split=split, prerender=prerender, post=post, gram=gram,
-                              workspace_bytes=workspace_bytes, gram_split_min=gram_split_min)
+                              workspace_bytes=workspace_bytes, gram_split_min=gram_split_min,
+                              init_states=init_states)
     ev = timer.begin("island_lm_kernel") if timer is not None else None
     pb.launch(stream)
     if ev is not None:
@@ -693,7 +837,7 @@ _POST_KEYS = ("post_var", "post_dvar", "post_metrics", "post_kappa", "post_hg")
 
 def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                 sersic_dev=None, post: bool = True,
-                workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None):
+                workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None, init_states=None):
     """Fit ``classes`` (kernels, host, rows) on the float64 kernels
     (band-split, members rendered in place), then, with ``post``, the
     float32 post-fit pass (no LM step) at the float64 parameters for the
@@ -706,7 +850,8 @@ def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev
     k64 = [(dataclasses.replace(k, precision="fp64"), h, np.asarray(r)) for k, h, r in classes]
     r64 = run_island_batch(k64, psf_trees, max_steps, dstack_dev, wstack_dev,
                            precision="fp64", sersic_dev=sersic_dev, split=True,
-                           prerender=False, workspace_bytes=workspace_bytes, timer=timer)
+                           prerender=False, workspace_bytes=workspace_bytes, timer=timer,
+                           init_states=init_states)
     if post:
         at64 = []
         for (kern, host, rows), r in zip(k64, r64):
@@ -872,6 +1017,123 @@ def stall_probes(classes, results, psf_trees: dict, dstack_dev, wstack_dev, *, s
                 np.float32)
             n_probed += len(idx)
     return n_probed
+
+
+def island_fit(*, bandnames, data_minus_bg, invvar, psfs, island_set, buckets, pos_x, pos_y,
+               source_type, logre, ee1, ee2, sersic_n, s1_flux, s1_flux_var, pos_err=None,
+               re_prior_sig=None, ell_prior_sig=None, theta_prior_sig=None,
+               sersic_n_prior_sig=None, render_backend: str = "fft", fit_sersic_n: bool = True,
+               n_fit_min_snr: float = 30.0, pos_max_shift_pix: float = 5.0,
+               pos_step_max_pix: float = DEFAULT_POS_STEP_MAX,
+               max_steps: int = 30, max_steps_main=8,
+               chi2_ftol_rel: float = DEFAULT_CHI2_FTOL_REL,
+               step_tol_pix: float = DEFAULT_STEP_TOL_PIX, un_freeze_steps: int = 0,
+               reg: float = DEFAULT_REG, nsrc_max=None, joint_nsrc_cap: int = 32,
+               star_render_split: bool = True, stall_count_mode: str = "proposals",
+               convergence: str = "legacy", plateau_dlnp_tol: float = 0.5,
+               include_giants: bool = True, compute_param_errors: bool = False,
+               compute_source_metrics: bool = False, compute_re_profile: bool = False,
+               return_island_residual: bool = False, free_n_override=None, log_fn=None):
+    """Fit every bucketed island and return per-source rows, giants
+    included (island_solver.py:3233-3595): free-n gate from the S1 fluxes,
+    class planning with the joint member cap, padded field stacks (PAD =
+    largest window + 8), the band-split device fit of all classes (float64
+    for islands the precision policy routes there), the stall probes,
+    :func:`unpack_class_results` per class and :func:`giant_seed_rows`.
+    ``max_steps_main`` (the reference's main pass / re-queue split) gives
+    the same LM trajectories as one ``max_steps`` pass and is accepted for
+    the signature.  The legacy step only: priors, u_n annealing, the
+    "changed" stall counting, dlnp convergence, the Re profile scan and the
+    residual export raise NotImplementedError."""
+    import torch
+    from .island_plan import (IslandFitResult, free_n_gate, giant_seed_rows,
+                              plan_island_classes, unpack_class_results)
+    from .pipeline import FitConfig, fit_island_set, psf_bank
+
+    def has(a):
+        return a is not None and np.any(np.isfinite(np.asarray(a, np.float64)))
+    if any(has(a) for a in (pos_err, re_prior_sig, ell_prior_sig, theta_prior_sig,
+                            sersic_n_prior_sig)):
+        raise NotImplementedError("catalog priors are not part of this build")
+    if un_freeze_steps or stall_count_mode != "proposals" or convergence != "legacy":
+        raise NotImplementedError("only the legacy LM step (no u_n annealing, proposal stall "
+                                  "counting, legacy convergence) is part of this build")
+    if compute_re_profile or return_island_residual:
+        raise NotImplementedError("the Re profile scan and residual export are not part of "
+                                  "this build")
+    if render_backend != "fft" and np.any(np.asarray(source_type) != 0):
+        raise NotImplementedError("extended sources render through the 'fft' backend only")
+    nb = len(bandnames)
+    st = np.asarray(source_type, np.int32)
+    x = np.asarray(pos_x, np.float32)
+    y = np.asarray(pos_y, np.float32)
+    lr, e1, e2 = (np.asarray(a, np.float32) for a in (logre, ee1, ee2))
+    sn = np.asarray(sersic_n, np.float32)
+    fl = np.nan_to_num(np.asarray(s1_flux, np.float64), nan=0.0)
+    fv = np.nan_to_num(np.asarray(s1_flux_var, np.float64), nan=1e30)
+    free_n = (np.asarray(free_n_override, bool) if free_n_override is not None else
+              free_n_gate(st, fl, fv, bool(fit_sersic_n), float(n_fit_min_snr)))
+    native = int(psfs[0].H_psf)
+    sub = buckets
+    if nsrc_max is not None:
+        import dataclasses
+        sub = dataclasses.replace(buckets, buckets={k: b for k, b in buckets.buckets.items()
+                                                    if k[1] <= int(nsrc_max)})
+    plans, capped = plan_island_classes(island_set, sub, st, lr, sn, native,
+                                        joint_nsrc_cap=joint_nsrc_cap)
+    dev = torch.device("cuda")
+    pad = max((p.bbox_class for p in plans), default=0) + 8
+    d = np.asarray(data_minus_bg, np.float32)
+    w = np.asarray(invvar, np.float32)
+    _, H, W = d.shape
+    dstack = torch.zeros((nb, H + 2 * pad, W + 2 * pad), dtype=torch.float32, device=dev)
+    wstack = torch.zeros_like(dstack)
+    dstack[:, pad:pad + H, pad:pad + W] = torch.from_numpy(d).to(dev)
+    wstack[:, pad:pad + H, pad:pad + W] = torch.from_numpy(w).to(dev)
+    cfg = FitConfig(max_steps=int(max_steps), reg=float(reg),
+                    pos_max_shift_pix=float(pos_max_shift_pix),
+                    post_fit=bool(compute_param_errors or compute_source_metrics))
+    if float(pos_step_max_pix) != DEFAULT_POS_STEP_MAX or \
+            float(chi2_ftol_rel) != DEFAULT_CHI2_FTOL_REL or \
+            float(step_tol_pix) != DEFAULT_STEP_TOL_PIX:
+        raise NotImplementedError("non-default LM clip / tolerances through island_fit; use "
+                                  "make_class_kernels + run_class_chunk")
+    rows = []
+    if plans:
+        isf = fit_island_set(psf_bank(psfs, dev), nb, int(psfs[0].n_terms), native, dstack,
+                             wstack, pad, island_set, plans, x, y, st, lr, e1, e2, sn, free_n,
+                             cfg, flux_scale=(np.asarray(s1_flux, np.float64),
+                                              np.sqrt(np.maximum(fv, 0.0))),
+                             keep_class_results=True)
+        from .pipeline import metrics_from_sums
+        import dataclasses
+        for plan, host, r_idx, res in isf.class_results:
+            pl = dataclasses.replace(plan, isl_ids=plan.isl_ids[r_idx], midx=plan.midx[r_idx],
+                                     mmask=plan.mmask[r_idx], bbox=plan.bbox[r_idx],
+                                     n_src_of=plan.n_src_of[r_idx])
+            h = {k: (v[r_idx] if isinstance(v, np.ndarray) and v.ndim and
+                     v.shape[0] == plan.n_islands else v) for k, v in host.items()}
+            perr = sm = sw = fm = None
+            if "post_var" in res:
+                if compute_param_errors:
+                    perr = np.full(res["post_var"].shape[:2] + (13,), np.nan, np.float32)
+                    perr[:, :, :7] = res["post_var"]
+                    fm = res["post_dvar"]
+                if compute_source_metrics:
+                    m4, fw = metrics_from_sums(res["post_metrics"])
+                    sm, sw = m4.astype(np.float32), np.transpose(fw, (0, 2, 1)).astype(np.float32)
+            rows.append(unpack_class_results(
+                pl, h, island_set, res["p"], res["fluxes"], res["ferr"], res["chi2"],
+                res["conv"], res["stall"], res["nst"], res_perr=perr,
+                res_pdchi2=res.get("stall_dchi2"), res_smetrics=sm, res_swin=sw,
+                res_femarg=fm, res_div=res.get("diverged"), res_ngood=res["n_good"],
+                n_bands=nb))
+    if include_giants:
+        gids = np.concatenate([np.asarray(buckets.giant_ids, np.int64),
+                               np.asarray(capped, np.int64)])
+        rows += giant_seed_rows(gids, island_set, pos_x=x, pos_y=y, logre=lr, ee1=e1, ee2=e2,
+                                sersic_n=sn, n_bands=nb)
+    return IslandFitResult.from_rows(bandnames, rows)
 
 
 class IslandBatchGraph:

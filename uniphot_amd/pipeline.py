@@ -393,18 +393,23 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
 
 
 def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: FitConfig,
-                gtier: dict, timer=None, final=False):
+                gtier: dict, timer=None, final=False, bg_geo=None, with_bg_var=False):
     """Per-grid, per-band linear solve; returns fluxes/var (n, nb) for each
     source from the grid that owns it, and bg coefs (n_grid, nb, M).  The
     grids, their fit sets and stamp tiers are those of pass 1 (pass 2 only
-    replaces the source parameters, island_stage.py:3331-3363)."""
+    replaces the source parameters, island_stage.py:3331-3363).  ``bg_geo``
+    (x origin, x scale, y origin, y scale) of the background basis defaults
+    to the field's centre and half-size (pipeline.py:1693-1696)."""
     n = x.size
     nb = fld.nb
     M = (cfg.bg_order + 1) * (cfg.bg_order + 2) // 2
     flux = np.full((n, nb), np.nan, np.float32)
     var = np.full((n, nb), np.nan, np.float32)
     bg = np.zeros((len(grids), nb, M), np.float32)
-    bg_geo = (0.5 * (fld.W - 1), max(0.5 * fld.W, 1.0), 0.5 * (fld.H - 1), max(0.5 * fld.H, 1.0))
+    bgv = np.full((len(grids), nb, M), np.nan, np.float32)
+    if bg_geo is None:
+        bg_geo = (0.5 * (fld.W - 1), max(0.5 * fld.W, 1.0), 0.5 * (fld.H - 1),
+                  max(0.5 * fld.H, 1.0))
     by_tier = {}
     for g in grids:
         by_tier.setdefault(gtier[g.grid_id], []).append(g)
@@ -454,6 +459,9 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
             flux[g.source_indices, b] = r.fluxes[:own]
             var[g.source_indices, b] = r.flux_var[:own]
             bg[g.grid_id, b] = r.bg_coefs
+            bgv[g.grid_id, b] = r.bg_var
+    if with_bg_var:
+        return flux, var, bg, bgv
     return flux, var, bg
 
 
@@ -513,6 +521,7 @@ class IslandSetFit:
     fp64_refit: np.ndarray = None     # (n,) bool: island refitted on the float64 kernels
     n_fp64_islands: int = 0
     kappa: np.ndarray = None          # (n,) the island's scaled Gauss-Newton condition number
+    class_results: list = None        # (plan, host, rows, result dict) per launch part
     diverged: np.ndarray = None       # (n,) stalled on identical (clip-pinned) proposals
     stall_dchi2: np.ndarray = None    # (n,) best damped-step chi2 change of a stalled island
 
@@ -598,7 +607,8 @@ def island_contrast(plan, rows, flux, sigma) -> np.ndarray:
 def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans, x, y, stype,
                    logre, ee1, ee2, sn, free_n, cfg: FitConfig, timer=None,
                    budget_bytes: float = 48e9, sync=None, p_start=None,
-                   scene: SceneStars | None = None, flux_scale=None) -> IslandSetFit:
+                   scene: SceneStars | None = None, flux_scale=None,
+                   keep_class_results: bool = False) -> IslandSetFit:
     """Island VarPro-LM over the planned classes of an island set
     (island_solver.py:3233-3595, one device): class host arrays, the device
     Sersic table and PSF trees, then the band-split fit in launch sequences
@@ -660,6 +670,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 cfg.island_split and cfg.galaxy_prerender)
     route_on = (refit_on and cfg.fp64_contrast is not None and flux_scale is not None)
     n_fp64 = 0
+    kept = []
     for chunk in chunks:
         if not chunk:
             continue
@@ -733,6 +744,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                        time.perf_counter() - t0))
         for (kern, host, rows), res in zip(chunk, results):
             plan = plan_of[id(host)]
+            if keep_class_results:
+                kept.append((plan, host, np.asarray(rows), res))
             mm = plan.mmask[rows]
             src = plan.midx[rows][mm]
             fx[src] = res["p"][:, :, 0][mm]
@@ -767,8 +780,9 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 fwin[src] = fw
     n_fit = sum(p.n_islands for p in plans)
     return IslandSetFit(fx, fy, flr, fe1, fe2, fsn, pfit, flux, ferr, conv, stall, nst, fitted,
-                        n_fit, classes, timing, pvar, ferr_m, mets, fwin, f64src, n_fp64, kap,
-                        div, sdc)
+                        n_fit, classes, timing, pvar, ferr_m, mets, fwin, fp64_refit=f64src,
+                        n_fp64_islands=n_fp64, kappa=kap, diverged=div, stall_dchi2=sdc,
+                        class_results=kept if keep_class_results else None)
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,

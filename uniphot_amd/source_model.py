@@ -3,7 +3,9 @@
 Follows ``tract7dt/fit/grid_gpu/source_model.py``: type codes (:24-27),
 the Sersic-n logit bijection (:38-54), EllipseESoft conversions
 (:104-138) and the catalog-to-parameter rules of ``catalog_source_params``
-(:141-215), here taking arrays instead of a DataFrame.
+(:141-215), and the stamp renderers ``render_source_stamps`` /
+``render_star_stamps`` / ``render_source_stamp_at`` (:264-390) over the
+device kernels.
 """
 from __future__ import annotations
 
@@ -56,11 +58,37 @@ def type_codes(type_strings) -> np.ndarray:
                      for t in type_strings], dtype=np.int32)
 
 
+def _catalog_columns(catalog):
+    """TYPE, RE, ELL, THETA, SERSIC_N of a DataFrame-like catalogue, the
+    column names matched without case (None where absent)."""
+    names = {str(c).lower(): c for c in catalog.columns}
+
+    def col(k, numeric=True):
+        c = names.get(k)
+        if c is None:
+            return None
+        v = np.asarray(catalog[c])
+        if not numeric:
+            return v.astype(str)
+        import pandas as pd
+        return pd.to_numeric(catalog[c], errors="coerce").to_numpy(dtype=float)
+    typ = col("type", numeric=False)
+    return (np.full(len(catalog), "STAR", dtype=object) if typ is None else typ,
+            col("re"), col("ell"), col("theta"), col("sersic_n"))
+
+
 def catalog_source_params(types, re=None, ell=None, theta=None, sersic_n=None,
                           re_fallback_pix: float = 3.0):
-    """Catalog morphology -> (source_type, logre, ee1, ee2, sersic_n).
-    Re clipped to [0.3, 100] (fallback 3 px), ab = 1 - ELL in [0.1, 1]
-    (default 0.8), phi from THETA, DEV n = 4, SERSIC n = 2 unless given."""
+    """Catalog morphology -> (source_type, logre, ee1, ee2, sersic_n)
+    (source_model.py:141-215).  ``types`` is a DataFrame-like catalogue
+    (columns TYPE, RE, ELL, THETA, SERSIC_N in any case) or an array of
+    TYPE strings / type codes with the other columns as arrays.  Re NaN ->
+    ``re_fallback_pix``, clipped to [0.3, 100]; ab = 1 - ELL (NaN -> 0)
+    clipped to [0.1, 1], 0.8 without an ELL column; phi from THETA (NaN ->
+    0); n = 4 for DEV and 2 for SERSIC, a finite SERSIC_N used for SERSIC
+    rows only, clipped to [0.29, 6.3]."""
+    if hasattr(types, "columns"):
+        types, re, ell, theta, sersic_n = _catalog_columns(types)
     st = type_codes(types) if not np.issubdtype(np.asarray(types).dtype, np.integer) \
         else np.asarray(types, np.int32)
     n = st.size
@@ -83,3 +111,55 @@ def catalog_source_params(types, re=None, ell=None, theta=None, sersic_n=None,
         phi = -np.where(t >= 0, 90.0 - t, -90.0 - t)
     lr, e1, e2 = ellipse_soft_from_re_ab_phi_np(r, ab, phi)
     return st, lr, e1, e2, sn.astype(np.float32)
+
+
+def _render(psf, pos_x, pos_y, grid_x0, grid_y0, source_type, logre, ee1, ee2, sersic_n,
+            render_backend, stream=None):
+    from .forced_phot_sparse import render_unit_stamps
+    from .psf_polynomial import psf_device_arrays
+    import torch
+    st = np.asarray(source_type, np.int32)
+    if render_backend not in ("direct", "fft"):
+        raise ValueError(f"unknown render_backend: {render_backend!r}")
+    if render_backend == "direct" and np.any(st != SOURCE_STAR):
+        raise NotImplementedError(
+            "render_backend='direct' for extended sources is not part of this build; the "
+            "reference's 'auto' resolves to 'fft' whenever extended sources are present")
+    T = int(psf.H_psf)
+    c, g = psf_device_arrays([psf], T)
+    n = np.asarray(pos_x).size
+    dev = torch.device("cuda")
+    return render_unit_stamps(torch.from_numpy(c).to(dev), torch.from_numpy(g).to(dev), T,
+                              c.shape[1], np.zeros(n, np.int32), pos_x, pos_y,
+                              np.full(n, grid_x0), np.full(n, grid_y0), st,
+                              logre, ee1, ee2,
+                              np.ones(n) if sersic_n is None else sersic_n, stream=stream)
+
+
+def render_source_stamps(psf_jax, pos_x, pos_y, grid_x0: int, grid_y0: int, source_type,
+                         logre, ee1, ee2, sersic_n=None, render_backend: str = "direct"):
+    """Unit-flux stamps (n, T, T) on the device for mixed star/galaxy
+    sources (source_model.py:367-390): the PSF polynomial at the full-image
+    position, EXP/DEV/SERSIC through the Fourier mixture convolution
+    (``render_backend="fft"``), Lanczos-3 shift by the remainder of (pos -
+    grid origin); not renormalised.  ``psf_jax`` is a PsfPolynomial."""
+    return _render(psf_jax, pos_x, pos_y, grid_x0, grid_y0, source_type, logre, ee1, ee2,
+                   sersic_n, render_backend)
+
+
+def render_star_stamps(psf_jax, pos_x, pos_y, grid_x0: int, grid_y0: int):
+    """Unit-flux stamps of an all-star source set (source_model.py:331-364),
+    equal to the STAR rows of :func:`render_source_stamps`."""
+    n = np.asarray(pos_x).size
+    z = np.zeros(n, np.float32)
+    return _render(psf_jax, pos_x, pos_y, grid_x0, grid_y0, np.zeros(n, np.int32), z, z, z,
+                   None, "fft")
+
+
+def render_source_stamp_at(psf_jax, x, y, grid_x0: int, grid_y0: int, source_type, logre, ee1,
+                           ee2, sersic_n=None, render_backend: str = "direct"):
+    """One source's unit-flux stamp (T, T) (source_model.py:264-328)."""
+    return render_source_stamps(psf_jax, np.array([x]), np.array([y]), grid_x0, grid_y0,
+                                np.array([source_type]), np.array([logre]), np.array([ee1]),
+                                np.array([ee2]), None if sersic_n is None else
+                                np.array([sersic_n]), render_backend)[0]
