@@ -99,6 +99,18 @@ def test_fit_field_star_scene(cuda):
     # percent level, so this is a loose check)
     rel_truth = np.abs(res.flux - sc.true_flux) / sc.true_flux
     assert np.median(rel_truth) < 0.05
+    # catalogue writeback of the same fit
+    import pandas as pd
+    from uniphot_amd.output import write_fit_columns
+    out = write_fit_columns(pd.DataFrame(dict(ID=np.arange(n))), res, ["a", "b", "c"],
+                            x_init=sc.x, y_init=sc.y, source_type=sc.stype)
+    for b, bn in enumerate("abc"):
+        np.testing.assert_array_equal(out[f"FLUX_{bn}_fit"], res.flux[:, b])
+        np.testing.assert_allclose(out[f"FLUXERR_{bn}_fit"], np.sqrt(res.flux_var[:, b]),
+                                   rtol=1e-6)
+    assert out["opt_converged"].all() and (out["grid_id_fit"] >= 0).all()
+    fitted = (res.island_id >= 0) & ~res.island_giant
+    assert np.all(np.isfinite(out["x_pix_white_err"][fitted]))
 
 
 def test_scene_subtract_and_island_cutouts_match_oracle(cuda):

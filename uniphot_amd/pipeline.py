@@ -112,6 +112,11 @@ class FitResult:
     island_stall_dchi2: np.ndarray = None
     island_plateau: np.ndarray = None     # stalled with no damped step better than -2 tol
     island_fp64: np.ndarray = None        # (n,) island fitted on the float64 kernels
+    flux_var: np.ndarray = None           # (n, nb) pass-2 flux variance
+    flux_marg_var: np.ndarray = None      # (n, nb) shape-marginalisation variance increment
+    re_upper_pix: np.ndarray = None       # (n,) fitted-Re bound 0.75 x the island's tier
+    free_n: np.ndarray = None             # (n,) Sersic n fitted
+    grid_id: np.ndarray = None            # (n,) grid owning the source
 
 
 class DeviceField:
@@ -522,6 +527,8 @@ class IslandSetFit:
     n_fp64_islands: int = 0
     kappa: np.ndarray = None          # (n,) the island's scaled Gauss-Newton condition number
     class_results: list = None        # (plan, host, rows, result dict) per launch part
+    flux_marg_var: np.ndarray = None  # (n, nb) shape-marginalisation variance increment
+    re_upper_pix: np.ndarray = None   # (n,) exp(upper bound of logre)
     diverged: np.ndarray = None       # (n,) stalled on identical (clip-pinned) proposals
     stall_dchi2: np.ndarray = None    # (n,) best damped-step chi2 change of a stalled island
 
@@ -637,6 +644,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     ferr_m = np.full((n, nb), np.nan, np.float32)
     mets = np.full((n, 4), np.nan, np.float32)
     fwin = np.full((n, nb), np.nan, np.float32)
+    dvar = np.full((n, nb), np.nan, np.float32)
+    re_up = np.full(n, np.nan, np.float32)
     f64src = np.zeros(n, bool)
     kap = np.full(n, np.nan, np.float32)
     div = np.zeros(n, bool)
@@ -764,6 +773,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             stall[src] = res["stall"][r_of]
             nst[src] = res["nst"][r_of]
             fitted[src] = True
+            re_up[src] = np.exp(host["upper"][rows][:, :, 2][mm].astype(np.float64))
             if "fp64_refit" in res:
                 f64src[src] = res["fp64_refit"][r_of]
             if "post_kappa" in res:
@@ -775,6 +785,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 pvar[src] = res["post_var"][mm]
                 dv = res["post_dvar"].transpose(0, 2, 1)[mm]
                 ferr_m[src] = np.sqrt(np.square(ferr[src]) + dv)
+                dvar[src] = dv
                 m4, fw = metrics_from_sums(res["post_metrics"][mm])
                 mets[src] = m4
                 fwin[src] = fw
@@ -782,7 +793,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     return IslandSetFit(fx, fy, flr, fe1, fe2, fsn, pfit, flux, ferr, conv, stall, nst, fitted,
                         n_fit, classes, timing, pvar, ferr_m, mets, fwin, fp64_refit=f64src,
                         n_fp64_islands=n_fp64, kappa=kap, diverged=div, stall_dchi2=sdc,
-                        class_results=kept if keep_class_results else None)
+                        class_results=kept if keep_class_results else None,
+                        flux_marg_var=dvar, re_upper_pix=re_up)
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,
@@ -862,6 +874,9 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     sync()
     tm["pass2"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
+    owner = np.full(n, -1, np.int64)
+    for g in grids:
+        owner[g.source_indices] = g.grid_id
     rec = dict(n_sources=n, n_islands=int(iset.n_islands), n_fitted_islands=n_fitted_islands,
                n_giant_islands=int(giants.size), n_fp64_refit_islands=int(isf.n_fp64_islands),
                classes=[
@@ -880,7 +895,10 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                      fe2, fsn, iset.labels.astype(np.int64), conv, stall, nst, giant, tm, rec,
                      isf.param_var, isf.flux, isf.flux_err_marg, isf.metrics, isf.flux_window,
                      isf.diverged, isf.stall_dchi2,
-                     stall & (isf.stall_dchi2 > -2.0 * PLATEAU_DLNP_TOL), isf.fp64_refit)
+                     stall & (isf.stall_dchi2 > -2.0 * PLATEAU_DLNP_TOL), isf.fp64_refit,
+                     flux_var=var, flux_marg_var=isf.flux_marg_var,
+                     re_upper_pix=isf.re_upper_pix, free_n=np.asarray(free_n, bool),
+                     grid_id=owner)
 
 
 def psf_bank(psfs, device="cuda"):
