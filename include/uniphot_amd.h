@@ -181,7 +181,8 @@ int uph_island_fit_batch(const uph_island_batch* batch, const uph_island_inputs*
  *   counters: uph_island_split_counters(max_steps) int32 (reset here; the
  *     last one is set to 1 if an island's or a galaxy's blocks did not fit).
  * With the member pre-render, scratch_per_block must also cover
- * uph_prerender_scratch_bytes(T) of every stamp tier.
+ * uph_prerender_scratch_bytes(T) (float32) or uph_prerender64_scratch_bytes(T)
+ * (float64) of every stamp tier.
  */
 typedef struct uph_island_split {
   int n_items;
@@ -201,10 +202,14 @@ typedef struct uph_island_split {
                                one after the star planes, one after the
                                large-island Gram launch (before the
                                (island, band) evaluation) and one at its end */
-  /* member pre-render (float32 only; n_pre = 0 renders members in place):
-     before each pass every real member of the active islands is rendered
-     and shifted by (member, band) items into its planes, and the evaluation
-     only places their window part */
+  /* member pre-render (n_pre = 0 renders members in place): before each
+     pass every real member of the active islands is rendered and shifted
+     by (member, band) items into its planes, and the evaluation only
+     places their window part.  With precision UPH_PRECISION_FP64, pbuf and
+     fspec hold doubles (pbuf_elems, fspec_elems, pre_off and pre_spec count
+     doubles), an extended member's spectra block is
+     uph_prerender64_spectra_doubles(T) long, gdt and gtap may be NULL, and
+     post, gram_bf16 and big_n must be 0 */
   int n_pre;
   int n_pre_ext;              /* members [0, n_pre_ext) are extended, the
                                  rest stars                                    */
@@ -291,6 +296,14 @@ size_t uph_island_split_counters(int max_steps);
    extended member. */
 size_t uph_prerender_scratch_bytes(int T);
 size_t uph_prerender_spectra_floats(int T);
+/* The same for the float64 member pre-render: per-workgroup scratch bytes
+   for stamp tier T, and the doubles of one extended member's spectra block
+   (F, F1, Fn on the (pad, pad/2+1) grid, then core, dcore, the nine
+   ellipse-term derivatives and dn/du).  Sizing helpers of this build's
+   pre-render launches; the reference renders members inside its per-island
+   programs (island_solver.py:441-562) and has no counterpart. */
+size_t uph_prerender64_scratch_bytes(int T);
+size_t uph_prerender64_spectra_doubles(int T);
 int uph_island_fit_split(const uph_island_batch* batch, const uph_island_inputs* in,
                          const uph_island_outputs* out, const uph_lm_options* opt, int precision,
                          const uph_island_split* split, void* scratch, size_t scratch_bytes,

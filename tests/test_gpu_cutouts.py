@@ -67,3 +67,22 @@ def test_bf16_gram_against_fp64(cut):
     assert b16["flux_solve"]["p50"] < 1e-6
     assert b16["flux_solve"]["frac_le_1e5"] >= f32["flux_solve"]["frac_le_1e5"] - 0.05
     assert b16["same_trajectory"] >= 0.8
+
+
+def test_fp64_prerender_equals_inplace_with_larger_tiers(cut):
+    """The float64 member pre-render against the in-place float64 render
+    on the whole cutout batch, stamp tiers 55 and 111 included: bit for
+    bit."""
+    import dataclasses
+    import torch
+    from uniphot_amd.pipeline import FitConfig, fit_cutouts, psf_bank
+    psf_of = psf_bank(cut.psfs, cut.dstack.device)
+    c64 = FitConfig(precision="fp64", post_fit=False)
+    a = fit_cutouts(cut, dataclasses.replace(c64, galaxy_prerender=False), psf_of=psf_of,
+                    sync=torch.cuda.synchronize)
+    b = fit_cutouts(cut, c64, psf_of=psf_of, sync=torch.cuda.synchronize)
+    tiers = sorted({p.tier for p, _ in b.classes})
+    print("tiers", tiers, flush=True)
+    assert 111 in tiers
+    for k in ("flux", "flux_err", "p", "nst", "conv", "stall"):
+        assert np.array_equal(getattr(a, k), getattr(b, k), equal_nan=True), k

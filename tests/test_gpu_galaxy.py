@@ -185,3 +185,25 @@ def test_large_island_gram_launch_matches_in_kernel_gram(cuda):
             assert (np.abs(fa - fb) / scale).max() < 1e-5
             va, vb = ra["post_var"][r][mm], rb["post_var"][r][mm]
             assert np.allclose(va, vb, rtol=1e-4, atol=1e-12, equal_nan=True)
+
+
+@pytest.mark.parametrize("n_range", [(1.0, 3.0), (0.4, 0.9)])
+def test_fp64_prerender_equals_inplace(cuda, n_range):
+    """The float64 member pre-render (spectra + (member, band) planes
+    launches, then the evaluation places the planes) performs the in-place
+    float64 render's operations: the band-split fits agree bit for bit."""
+    import torch
+    from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
+    from uniphot_amd.sersic_table import device_table
+    sc = small_mixed_scene(n_range=n_range)
+    plans, hosts, pad, d, w = mixed_classes(sc, free_n=(sc.stype == 3))
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms, precision="fp64"), h,
+              np.arange(p.n_islands)) for p, h in zip(plans, hosts)]
+    trees = {T: psf_tree_device(sc.psfs, T, cuda) for T in {p.tier for p in plans}}
+    dd, ww = torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda)
+    kw = dict(precision="fp64", sersic_dev=device_table(cuda), split=True)
+    ref = run_island_batch(batch, trees, 30, dd, ww, prerender=False, **kw)
+    got = run_island_batch(batch, trees, 30, dd, ww, prerender=True, **kw)
+    for ra, rb in zip(ref, got):
+        for k in ("p", "fluxes", "ferr", "chi2", "lam", "nst", "conv", "stall", "n_good"):
+            assert np.array_equal(ra[k], rb[k], equal_nan=True), k

@@ -19,6 +19,7 @@ EXPORTS = (
     "uph_version", "uph_last_error", "uph_island_scratch_per_block", "uph_island_fit",
     "uph_island_fit_batch", "uph_island_split_counters", "uph_island_fit_split",
     "uph_prerender_scratch_bytes", "uph_prerender_spectra_floats",
+    "uph_prerender64_scratch_bytes", "uph_prerender64_spectra_doubles",
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
     "uph_wing_images", "uph_blank_work_floats", "uph_blank_response", "uph_stamp_sum",
     "uph_gather_windows", "uph_subtract_bg_planes", "uph_band_solve_noalloc",
@@ -118,6 +119,9 @@ def lib() -> C.CDLL:
     L.uph_prerender_scratch_bytes.argtypes = [C.c_int]
     L.uph_prerender_spectra_floats.restype = C.c_size_t
     L.uph_prerender_spectra_floats.argtypes = [C.c_int]
+    for name in ("uph_prerender64_scratch_bytes", "uph_prerender64_spectra_doubles"):
+        getattr(L, name).restype = C.c_size_t
+        getattr(L, name).argtypes = [C.c_int]
     L.uph_island_fit_split.restype = C.c_int
     L.uph_island_fit_split.argtypes = [C.POINTER(IslandBatch), C.POINTER(IslandInputs),
                                        C.POINTER(IslandOutputs), C.POINTER(LMOptions), C.c_int,

@@ -619,6 +619,15 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
 // member's shifted planes (model, then the live tangents; 7 planes per band
 // for an extended member, 3 for a star) into the island planes.  The stamp
 // sits at rint(x - x0) - T / 2 as in the in-place render.
+// The pre-rendered planes of the evaluation's precision.
+template <typename Real>
+__device__ __forceinline__ const Real* pre_planes_buf(const IslandArgsPre& ap) {
+  if constexpr (sizeof(Real) == 4)
+    return ap.pbuf;
+  else
+    return ap.pbuf64;
+}
+
 template <typename Real>
 __device__ void place_member(const IslandArgs& a, IslandSh<Real>& sh, int b, const float* p6,
                              int r, float x0f, float y0f, Real* __restrict__ m_out,
@@ -632,7 +641,7 @@ __device__ void place_member(const IslandArgs& a, IslandSh<Real>& sh, int b, con
   const int sx0 = wx0 - ix, sy0 = wy0 - iy, SW = wx1 - wx0, SH = wy1 - wy0;
   const int g = ap.pre_of_slot[(size_t)sh.moff + sh.mem[r]];
   const int side = pre_side(T, B), PP = side * side;
-  const float* blk = ap.pbuf + ap.pre_off[g] + (size_t)b * pre_planes(sh.st[r] == 0) * PP;
+  const Real* blk = pre_planes_buf<Real>(ap) + ap.pre_off[g] + (size_t)b * pre_planes(sh.st[r] == 0) * PP;
   const int l0 = sh.first_live[r], l1 = sh.first_live[r + 1];
   for (int e = tid; e < SH * SW; e += UPH_BLOCK) {
     const int rs = sy0 + e / SW, cs = sx0 + e % SW;
@@ -714,9 +723,9 @@ __device__ double eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, in
   }
   // Gram of [d, models, tangents] under W.
   const int C = 1 + Nr + nL;
-  int bf16 = 0;  // bfloat16 Gram operands (pre-render variant, on request)
-  if constexpr (kPre) bf16 = static_cast<const IslandArgsPre&>(a).gram_bf16;
-  if constexpr (kPre) {
+  int bf16 = 0;  // bfloat16 Gram operands (float32 pre-render variant, on request)
+  if constexpr (kPre && sizeof(Real) == 4) bf16 = static_cast<const IslandArgsPre&>(a).gram_bf16;
+  if constexpr (kPre && sizeof(Real) == 4) {
     const IslandArgsPre& ap = static_cast<const IslandArgsPre&>(a);
     const int slot = (ap.big_n > 0 && !bf16) ? ap.big_slot[isl] : -1;
     if (bf16)
@@ -765,7 +774,7 @@ __device__ double eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, in
     }
   }
   __syncthreads();
-  if constexpr (kPre) {
+  if constexpr (kPre && sizeof(Real) == 4) {
     // with the bfloat16 Gram two more refinement steps: each one multiplies
     // the flux error by about (Gram error) x (condition number)
     if (bf16) {
@@ -1246,7 +1255,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_eval_kernel(typename EvalArg
         for (int j = tid; j < nL; j += UPH_BLOCK) slot[(size_t)nL * nL + j] = sm.g[j];
       }
       if (tid == 0) slot[(size_t)nL * nL + nL] = c;
-      if constexpr (kPre) {
+      if constexpr (kPre && sizeof(Real) == 4) {
         if (mode == kSplitEvalPost) post_band<Real>(a, sh, isl, b, planes, sm);
       }
     }
@@ -1794,11 +1803,12 @@ static void split_eval(uph::IslandArgs a, int mode, int* counter, int nblk, hipS
   hipLaunchKernelGGL((uph::island_eval_kernel<Real, false>), dim3(nblk), dim3(UPH_BLOCK), 0, s, a);
 }
 
+template <typename Real>
 static void split_eval_pre(uph::IslandArgsPre a, int mode, int* counter, int nblk,
                            hipStream_t s) {
   a.mode = mode;
   a.counter = counter;
-  hipLaunchKernelGGL((uph::island_eval_kernel<float, true>), dim3(nblk), dim3(UPH_BLOCK), 0, s,
+  hipLaunchKernelGGL((uph::island_eval_kernel<Real, true>), dim3(nblk), dim3(UPH_BLOCK), 0, s,
                      a);
 }
 
@@ -1823,14 +1833,16 @@ static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, 
     if (ev) (void)hipEventRecord(evk(q), s);
   };
   rec(0);
-  if (pre) {
+  if (pre && sizeof(Real) == 8) {
+    uph::prerender64_pass(*pre, mode, counters + 2 * k + 1, nblk, s, evk(1), evk(2));
+  } else if (pre) {
     uph::prerender_pass(*pre, mode, counters + 2 * k + 1, nblk, s, evk(1), evk(2));
   } else {
     rec(1);
     rec(2);
   }
   rec(3);
-  if (pre && pre->big_n > 0 && !pre->gram_bf16) {
+  if (sizeof(Real) == 4 && pre && pre->big_n > 0 && !pre->gram_bf16) {
     uph::IslandArgsPre g = *pre;
     g.mode = mode;
     hipLaunchKernelGGL(uph::island_gram_kernel, dim3(g.big_n * g.nb, (g.big_ntp + 3) / 4),
@@ -1838,7 +1850,7 @@ static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, 
   }
   rec(4);
   if (pre)
-    split_eval_pre(*pre, mode, counters + 2 * k, nblk, s);
+    split_eval_pre<Real>(*pre, mode, counters + 2 * k, nblk, s);
   else
     split_eval<Real>(a, mode, counters + 2 * k, nblk, s);
   rec(5);
@@ -1910,20 +1922,31 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
                 "uph_island_fit_split: the post-fit pass needs the member pre-render");
   UPH_CHECK_ARG(!sp->gram_bf16 || use_pre,
                 "uph_island_fit_split: the bfloat16 Gram needs the member pre-render");
+  const bool fp64 = precision == UPH_PRECISION_FP64;
   if (use_pre) {
-    UPH_CHECK_ARG(precision == UPH_PRECISION_FP32,
-                  "uph_island_fit_split: the member pre-render is float32 only");
     UPH_CHECK_ARG(sp->pre_isl && sp->pre_slot && sp->pre_off && sp->pre_spec &&
-                      sp->pre_of_slot && sp->pbuf && sp->fspec && sp->gdt && sp->gtap,
+                      sp->pre_of_slot && sp->pbuf && sp->fspec && (fp64 || (sp->gdt && sp->gtap)),
                   "uph_island_fit_split: null pre-render arrays");
     UPH_CHECK_ARG(sp->n_pre_ext >= 0 && sp->n_pre_ext <= sp->n_pre,
                   "uph_island_fit_split: n_pre_ext=%d outside [0, n_pre=%d]", sp->n_pre_ext,
                   sp->n_pre);
-    for (int t = 0; t < bat->n_tiers; ++t)
-      UPH_CHECK_ARG(uph::prerender_supports(bat->tier_T[t]) &&
-                        uph::prerender_scratch(bat->tier_T[t]) <= scratch_per_block,
-                    "uph_island_fit_split: stamp tier %d not supported by the member pre-render",
-                    bat->tier_T[t]);
+    UPH_CHECK_ARG(!fp64 || (!sp->post && !sp->gram_bf16 && sp->big_n == 0),
+                  "uph_island_fit_split: the float64 pre-render takes no post-fit pass, "
+                  "bfloat16 Gram or large-island Gram launch");
+    for (int t = 0; t < bat->n_tiers; ++t) {
+      if (fp64)
+        UPH_CHECK_ARG(uph::prerender64_scratch(bat->tier_T[t]) <= scratch_per_block,
+                      "uph_island_fit_split: stamp tier %d needs %zu bytes of float64 "
+                      "pre-render scratch per workgroup (%zu given)",
+                      bat->tier_T[t], uph::prerender64_scratch(bat->tier_T[t]),
+                      scratch_per_block);
+      else
+        UPH_CHECK_ARG(uph::prerender_supports(bat->tier_T[t]) &&
+                          uph::prerender_scratch(bat->tier_T[t]) <= scratch_per_block,
+                      "uph_island_fit_split: stamp tier %d not supported by the member "
+                      "pre-render",
+                      bat->tier_T[t]);
+    }
     static_cast<uph::IslandArgs&>(pre) = a;
     pre.n_pre = sp->n_pre;
     pre.n_pre_ext = sp->n_pre_ext;
@@ -1932,9 +1955,14 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
     pre.pre_off = sp->pre_off;
     pre.pre_spec = sp->pre_spec;
     pre.pre_of_slot = sp->pre_of_slot;
-    pre.pbuf = sp->pbuf;
+    if (fp64) {
+      pre.pbuf64 = reinterpret_cast<double*>(sp->pbuf);
+      pre.fspec64 = reinterpret_cast<double*>(sp->fspec);
+    } else {
+      pre.pbuf = sp->pbuf;
+      pre.fspec = sp->fspec;
+    }
     pre.pbuf_elems = sp->pbuf_elems;
-    pre.fspec = sp->fspec;
     pre.fspec_elems = sp->fspec_elems;
     pre.gdt = sp->gdt;
     pre.gtap = sp->gtap;
@@ -1978,6 +2006,6 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
   if (precision == UPH_PRECISION_FP32)
     split_sequence<float>(a, use_pre ? &pre : nullptr, sp->counters, nblk, s, sp->eval_events);
   else
-    split_sequence<double>(a, nullptr, sp->counters, nblk, s, sp->eval_events);
+    split_sequence<double>(a, use_pre ? &pre : nullptr, sp->counters, nblk, s, sp->eval_events);
   return uph::hip_status(hipGetLastError(), "uph_island_fit_split launch");
 }

@@ -559,4 +559,67 @@ __device__ void galaxy_raw(const GalParams<Real>& gp, int T, int need, Real* ws)
   }
 }
 
+// Z = S * F1 * d(expo)/d(param p) with the ellipse-term derivatives given
+// as dt9[3 i + p] (the layout of GalParams::dt).
+template <typename Real>
+__device__ void spec_shape_dt(const Real* S, const Real* F1, const double* dt9, int p, int pad,
+                              Real* Z) {
+  const int H = pad / 2 + 1, n = pad * H;
+  const double c2 = -2.0 * 9.869604401089358;
+  const double d0 = dt9[p], d1 = dt9[3 + p], d2 = dt9[6 + p];
+  for (int e = threadIdx.x; e < n; e += UPH_BLOCK) {
+    const int m = e / H, k = e % H;
+    const double v = double(k) / pad;
+    const double w = double(m < pad / 2 ? m : m - pad) / pad;
+    const double dex = c2 * (d0 * v * v + d1 * w * w + 2.0 * d2 * v * w);
+    const Real s = Real(double(F1[e]) * dex);
+    Z[2 * e] = S[2 * e] * s;
+    Z[2 * e + 1] = S[2 * e + 1] * s;
+  }
+  __syncthreads();
+}
+
+// galaxy_raw with the Fourier mixtures computed beforehand: F, F1 and Fn on
+// the (pad, H) grid at spec, spec + n, spec + 2n (fill_fourier's layout),
+// the core terms and the ellipse-term derivatives given.  Same planes, same
+// operations in the same order as galaxy_raw.
+template <typename Real, bool kLds>
+__device__ void galaxy_raw_spec(const Real* spec, double core_d, double dcore_d,
+                                const double* dt9, int T, int need, Real* ws) {
+  const GalLayout L = galaxy_layout(T);
+  const int pad = L.pad, logp = ilog2(pad), n = pad * L.H, TT = T * T;
+  const bool nder = need & 32;
+  rfft2_pad<Real, kLds>(ws + L.P, T, pad, logp, ws + L.RB, ws + L.SP);
+  Real* Gm = ws + L.G;
+  spec_scale(ws + L.SP, spec, n, ws + L.Z);
+  irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, Gm);
+  const Real core = Real(core_d);
+  for (int e = threadIdx.x; e < TT; e += UPH_BLOCK) Gm[e] += core * ws[L.P + e];
+  __syncthreads();
+  for (int j = 0; j < 2; ++j) {
+    if (!(need & (1 << j))) continue;
+    Real* src = ws + (j == 0 ? L.Px : L.Py);
+    Real* sp = ws + (j == 0 ? L.SPx : L.SPy);
+    rfft2_pad<Real, kLds>(src, T, pad, logp, ws + L.RB, sp);
+    spec_scale(sp, spec, n, ws + L.Z);
+    Real* o = Gm + (size_t)(1 + j) * TT;
+    irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, o);
+    for (int e = threadIdx.x; e < TT; e += UPH_BLOCK) o[e] += core * src[e];
+    __syncthreads();
+  }
+  for (int p = 0; p < 3; ++p) {
+    if (!(need & (4 << p))) continue;
+    spec_shape_dt(ws + L.SP, spec + n, dt9, p, pad, ws + L.Z);
+    irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, Gm + (size_t)(3 + p) * TT);
+  }
+  if (nder) {
+    spec_scale(ws + L.SP, spec + 2 * n, n, ws + L.Z);
+    Real* o = Gm + (size_t)6 * TT;
+    irfft2_crop<Real, kLds>(ws + L.Z, T, pad, logp, ws + L.RB, o);
+    const Real dcore = Real(dcore_d);
+    for (int e = threadIdx.x; e < TT; e += UPH_BLOCK) o[e] += dcore * ws[L.P + e];
+    __syncthreads();
+  }
+}
+
 }  // namespace uph

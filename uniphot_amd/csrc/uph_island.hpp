@@ -86,6 +86,9 @@ struct IslandArgsPre : IslandArgs {
   size_t pbuf_elems;
   float* fspec;                // per extended member: F + core, F1, Fn + dcore on the (H, pad) grid
   size_t fspec_elems;
+  double* pbuf64;              // float64 islands: shifted planes (pre_off in doubles)
+  double* fspec64;             // float64 islands: F, F1, Fn on the (pad, H) grid, then core,
+                               // dcore, 9 ellipse-term derivatives, dn/du (pre_spec in doubles)
   double* gdt;                 // (n_pre, 9) ellipse-term derivatives of extended members
   float* gtap;                 // (n_pre, 28) Lanczos taps and tap derivatives, x then y
   size_t star_stride;          // per-workgroup scratch bytes of the star launch

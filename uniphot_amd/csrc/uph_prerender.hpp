@@ -36,4 +36,12 @@ bool launch_galaxy_stamps(int n, int T, int nt, const float* coefs, const float*
                           const float* ee2, const float* sersic_n, float* out, char* scratch,
                           size_t scratch_bytes, int n_cu, hipStream_t s);
 
+// Float64 member pre-render (prerender64.hip): per-workgroup scratch bytes
+// for stamp tier T, doubles of one extended member's spectra block, and
+// the spectra + planes launches of one evaluation pass (pbuf64, fspec64).
+size_t prerender64_scratch(int T);
+size_t prerender64_spectra_doubles(int T);
+void prerender64_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hipStream_t s,
+                      hipEvent_t ev_spec, hipEvent_t ev_planes);
+
 }  // namespace uph

@@ -327,8 +327,9 @@ def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prer
     empty = dict(args=(0, 0, *([None] * 6), 0, None, 0, None, None),
                  isl=np.zeros(0, np.int64), T=np.zeros(0, np.int64), star=np.zeros(0, bool),
                  spec_floats=np.zeros(0, np.float64), n_ext=0)
-    if not prerender or precision == "fp64":
+    if not prerender:
         return empty
+    f64 = precision == "fp64"
     st_all, mm_all = cat["st"], cat["mm"].astype(bool)
     ext, stars = [], []
     for i in order:
@@ -345,20 +346,23 @@ def _prerender_arrays(order, shape, moff, cat, tiers, nb, m_acc, precision, prer
         slot_pre[moff[i] + k] = g
     gi = np.asarray(pre_isl, np.int64)
     T = np.asarray([tiers[shape[i][2]] for i in pre_isl], np.int64)
-    if any(L.uph_prerender_scratch_bytes(int(t)) == 0 for t in set(T.tolist())):
+    if not f64 and any(L.uph_prerender_scratch_bytes(int(t)) == 0 for t in set(T.tolist())):
         return empty
     star = np.arange(len(members)) >= len(ext)
     # one plane holds the window part of the stamp: at most min(T, B) a side
     side = np.minimum(T, np.asarray([shape[i][1] for i in pre_isl], np.int64))
     psz = nb * np.where(star, 3, 7) * side * side
-    ssz = np.where(star, 0, [L.uph_prerender_spectra_floats(int(t)) for t in T]).astype(np.int64)
+    spec_of = L.uph_prerender64_spectra_doubles if f64 else L.uph_prerender_spectra_floats
+    ssz = np.where(star, 0, [spec_of(int(t)) for t in T]).astype(np.int64)
     poff = np.concatenate([[0], np.cumsum(psz)[:-1]]).astype(np.int64)
     soff = np.concatenate([[0], np.cumsum(ssz)[:-1]]).astype(np.int64)
     soff = np.where(star, -1, soff).astype(np.int64)
-    pbuf = torch.empty(int(psz.sum()), dtype=torch.float32, device=dev)
-    fspec = torch.empty(max(int(ssz.sum()), 1), dtype=torch.float32, device=dev)
-    gdt = torch.empty(len(members) * 9, dtype=torch.float64, device=dev)
-    gtap = torch.empty(len(members) * 28, dtype=torch.float32, device=dev)
+    # float64 islands: planes and spectra in doubles, no taps/derivative scratch
+    fdt = torch.float64 if f64 else torch.float32
+    pbuf = torch.empty(int(psz.sum()), dtype=fdt, device=dev)
+    fspec = torch.empty(max(int(ssz.sum()), 1), dtype=fdt, device=dev)
+    gdt = torch.empty(1 if f64 else len(members) * 9, dtype=torch.float64, device=dev)
+    gtap = torch.empty(1 if f64 else len(members) * 28, dtype=torch.float32, device=dev)
     keep = [up(np.asarray(pre_isl, np.int32), np.int32), up(np.asarray(pre_slot, np.int32), np.int32),
             up(poff, np.int64), up(soff, np.int64), up(slot_pre, np.int32)]
     args = (len(members), len(ext), *(P(t) for t in keep), P(pbuf), int(pbuf.numel()), P(fspec),
@@ -401,11 +405,12 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     workgroup per island).  ``split=True`` runs the band-split form
     (``uph_island_fit_split``): every evaluation pass is spread over
     (island, band) items, so one large island no longer sets the wall time;
-    results are bit-identical.  With ``prerender`` (float32 only) every
-    member is rendered and shifted by separate (member, band) launches
-    before each pass (extended members through LDS wave-level FFTs) and the
-    evaluation only places the planes; results then agree with the in-place
-    render to float32 rounding.  A ``timer`` gets per-launch events of the
+    results are bit-identical.  With ``prerender`` every member is rendered
+    and shifted by separate (member, band) launches before each pass and
+    the evaluation only places the planes: float32 (extended members
+    through LDS wave-level FFTs; results agree with the in-place render to
+    float32 rounding) or float64 (the in-place render's operations, so the
+    results are bit-identical to it).  A ``timer`` gets per-launch events of the
     split sequence.  ``post`` (split + pre-render only) adds the post-fit
     pass: per-member parameter variances (``post_var`` (n, N, 7)), the
     shape-marginalisation flux variance increment (``post_dvar`` (n, nb, N))
@@ -478,9 +483,10 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         cat["hi"].append(host["upper"][rows].ravel())
         per = max(per, L.uph_island_scratch_per_block(N, B, kern.T, kern.n_live_max, prec))
     n_isl = len(shape)
-    if split and prerender and precision != "fp64":
+    if split and prerender:
         for T in tiers:
-            need = int(L.uph_prerender_scratch_bytes(int(T)))
+            need = int(L.uph_prerender64_scratch_bytes(int(T)) if precision == "fp64" else
+                       L.uph_prerender_scratch_bytes(int(T)))
             if need == 0:
                 prerender = False  # a stamp tier the pre-render does not serve
             per = max(per, need)
@@ -555,7 +561,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         n_eval = 2 * int(max_steps) + 2
         gal = _prerender_arrays(order, shape, moff, c, tiers, nb, m_acc, precision, prerender,
                                 dev, up)
-        post_on = bool(post and gal["args"][0] > 0)
+        post_on = bool(post and gal["args"][0] > 0 and precision != "fp64")
         n_eval += int(post_on)
         evs = None
         ev_arr = None
@@ -566,7 +572,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 e.record()
             ev_arr = (C.c_void_p * len(evs))(*[e.cuda_event for e in evs])
         post_args = (0, None, None, 0, None, None, None)
-        bf16_on = bool(gram == "bf16" and gal["args"][0] > 0)
+        bf16_on = bool(gram == "bf16" and gal["args"][0] > 0 and precision != "fp64")
         unsettled = torch.zeros(n_isl, dtype=torch.int32, device=dev)
         if post_on:
             Ns = np.asarray([s_[0] for s_ in shape], np.int64)
@@ -590,7 +596,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         ncap = np.asarray([1 + s_[0] + s_[3] for s_ in shape], np.float64)
         npx = np.asarray([s_[1] * s_[1] for s_ in shape], np.float64)
         cost = ncap * ncap * npx
-        big = np.nonzero((cost >= gram_split_min) & (gal["args"][0] > 0) & (not bf16_on))[0]
+        big = np.nonzero((cost >= gram_split_min) & (gal["args"][0] > 0) & (not bf16_on)
+                         & (precision != "fp64"))[0]
         # largest first, while the tile-pair buffer stays within its budget
         big = big[np.argsort(-cost[big], kind="stable")]
         ntp_of = ((ncap[big].astype(np.int64) + 15) // 16)
@@ -835,11 +842,42 @@ _STATE_KEYS = ("p", "fluxes", "ferr", "chi2", "lam", "conv", "stall", "nst", "co
 _POST_KEYS = ("post_var", "post_dvar", "post_metrics", "post_kappa", "post_hg")
 
 
+FP64_PLANE_BUDGET = 16e9  # bytes of float64 member planes per launch sequence
+
+
+def _fp64_groups(k64, budget: float = FP64_PLANE_BUDGET):
+    """Split (kernels, host, rows) classes into launch sequences whose
+    float64 member planes (n_bands x (7 extended | 3 star) x min(T, B)^2
+    doubles per real member) stay within ``budget`` bytes: a list of groups
+    of (class index, first row, end row)."""
+    groups, used = [[]], 0.0
+    for ci, (kern, host, rows) in enumerate(k64):
+        side = min(int(kern.T), int(kern.B))
+        mm = np.asarray(host["mmask"])[rows].astype(bool)
+        st = np.asarray(host["st"])[rows]
+        planes = (np.where(st == 0, 3, 7) * mm).sum(axis=1)
+        b = kern.n_bands * side * side * 8.0 * planes
+        start = 0
+        for i in range(rows.size):
+            if used > 0 and used + b[i] > budget:
+                if i > start:
+                    groups[-1].append((ci, start, i))
+                groups.append([])
+                used, start = 0.0, i
+            used += b[i]
+        if rows.size > start:
+            groups[-1].append((ci, start, rows.size))
+    return [g for g in groups if g]
+
+
 def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                 sersic_dev=None, post: bool = True,
-                workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None, init_states=None):
+                workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None, init_states=None,
+                prerender: bool = True):
     """Fit ``classes`` (kernels, host, rows) on the float64 kernels
-    (band-split, members rendered in place), then, with ``post``, the
+    (band-split; members pre-rendered by the float64 (member, band)
+    launches, or in place with ``prerender=False``: the same planes, bit
+    for bit), then, with ``post``, the
     float32 post-fit pass (no LM step) at the float64 parameters for the
     parameter variances, marginal flux errors, metrics and conditioning.
     Fluxes, flux errors and the LM state are the float64 fit's.  Every row
@@ -848,10 +886,24 @@ def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev
     if not classes:
         return []
     k64 = [(dataclasses.replace(k, precision="fp64"), h, np.asarray(r)) for k, h, r in classes]
-    r64 = run_island_batch(k64, psf_trees, max_steps, dstack_dev, wstack_dev,
-                           precision="fp64", sersic_dev=sersic_dev, split=True,
-                           prerender=False, workspace_bytes=workspace_bytes, timer=timer,
-                           init_states=init_states)
+    # launch sequences of at most FP64_PLANE_BUDGET bytes of member planes
+    groups = _fp64_groups(k64) if prerender else [[(ci, 0, r.size) for ci, (_, _, r)
+                                                   in enumerate(k64)]]
+    parts = [[] for _ in k64]
+    for grp in groups:
+        sub = [(k64[ci][0], k64[ci][1], k64[ci][2][a_:b_]) for ci, a_, b_ in grp]
+        ist = None
+        if init_states is not None:
+            ist = [None if init_states[ci] is None else
+                   {k: (np.asarray(v)[a_:b_] if np.ndim(v) else v)
+                    for k, v in init_states[ci].items()} for ci, a_, b_ in grp]
+        res = run_island_batch(sub, psf_trees, max_steps, dstack_dev, wstack_dev,
+                               precision="fp64", sersic_dev=sersic_dev, split=True,
+                               prerender=prerender, workspace_bytes=workspace_bytes,
+                               timer=timer, init_states=ist)
+        for (ci, _, _), r in zip(grp, res):
+            parts[ci].append(r)
+    r64 = [p[0] if len(p) == 1 else _merge_chunk_outputs(p) for p in parts]
     if post:
         at64 = []
         for (kern, host, rows), r in zip(k64, r64):
