@@ -191,7 +191,9 @@ def test_large_island_gram_launch_matches_in_kernel_gram(cuda):
 def test_fp64_prerender_equals_inplace(cuda, n_range):
     """The float64 member pre-render (spectra + (member, band) planes
     launches, then the evaluation places the planes) performs the in-place
-    float64 render's operations: the band-split fits agree bit for bit."""
+    float64 render's operations, and its Gram runs on float64 matrix cores
+    (the in-place evaluation's is a VALU loop): the band-split fits take the
+    same steps and flags, and agree to float64 rounding."""
     import torch
     from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
     from uniphot_amd.sersic_table import device_table
@@ -205,5 +207,15 @@ def test_fp64_prerender_equals_inplace(cuda, n_range):
     ref = run_island_batch(batch, trees, 30, dd, ww, prerender=False, **kw)
     got = run_island_batch(batch, trees, 30, dd, ww, prerender=True, **kw)
     for ra, rb in zip(ref, got):
-        for k in ("p", "fluxes", "ferr", "chi2", "lam", "nst", "conv", "stall", "n_good"):
-            assert np.array_equal(ra[k], rb[k], equal_nan=True), k
+        _same_fit(ra, rb)
+
+
+def _same_fit(ra, rb):
+    """Two float64 fits of one batch: identical steps and flags, float32
+    parameter state within a few ulps, fluxes and chi2 to float64 rounding."""
+    for k in ("nst", "conv", "stall", "n_good"):
+        assert np.array_equal(ra[k], rb[k]), k
+    assert np.nanmax(np.abs(ra["p"] - rb["p"]), initial=0.0) <= 1e-5
+    scale = np.maximum(np.abs(ra["fluxes"]), ra["ferr"])
+    assert np.nanmax(np.abs(ra["fluxes"] - rb["fluxes"]) / scale, initial=0.0) <= 1e-8
+    assert np.nanmax(np.abs(ra["chi2"] - rb["chi2"]) / np.abs(ra["chi2"]), initial=0.0) <= 1e-10

@@ -248,6 +248,81 @@ __device__ void gram_mfma(const float* __restrict__ W, const float* __restrict__
   __syncthreads();
 }
 
+// The float64 Gram on the matrix cores (the float64 pre-render evaluation):
+// v_mfma_f64_16x16x4f64 tiles of 16 planes with the operand layout of
+// gram_mfma (lane l feeds plane l % 16 at four consecutive pixels, two
+// double2 loads, lane group l / 16 selects which four); the result tile
+// comes back as D[4 r + l / 16][l % 16] in register r of lane l.  Products
+// and sums are float64 throughout.  Per 16-pixel step a lane loads 96 bytes
+// for 16 products of its tile entries, where the pair loop loads 16 bytes
+// per product.
+typedef double uph_d2 __attribute__((ext_vector_type(2)));
+typedef double uph_d4 __attribute__((ext_vector_type(4)));
+
+__device__ void gram_mfma_f64(const double* __restrict__ W, const double* __restrict__ Z, int P,
+                              int C, int z0, double* __restrict__ out,
+                              double* __restrict__ part) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nt = (C + 15) >> 4, ntp = nt * (nt + 1) / 2;
+  const int S = ntp >= UPH_WAVES ? 1 : UPH_WAVES / ntp;
+  const int units = ntp * S;
+  const int KS = P >> 4;
+  const int col = lane & 15, grp = lane >> 4;
+  for (int u = wid; u < units; u += UPH_WAVES) {
+    const int tp = u / S, sp = u % S;
+    int ti, tj;
+    pair_of(tp, nt, ti, tj);
+    const int ks0 = (KS * sp) / S, ks1 = (KS * (sp + 1)) / S;
+    const int a = ti * 16 + col, b = tj * 16 + col;
+    const bool va = a < C, vb = b < C;
+    const double* za = Z + (size_t)(z0 + (va ? a : 0)) * P;
+    const double* zb = Z + (size_t)(z0 + (vb ? b : 0)) * P;
+    uph_d4 acc = {0.0, 0.0, 0.0, 0.0};
+    for (int ks = ks0; ks < ks1; ++ks) {
+      const int px = ks * 16 + 4 * grp;
+      const uph_d2 w0 = *reinterpret_cast<const uph_d2*>(W + px);
+      const uph_d2 w1 = *reinterpret_cast<const uph_d2*>(W + px + 2);
+      uph_d2 a0 = {0.0, 0.0}, a1 = {0.0, 0.0}, b0 = {0.0, 0.0}, b1 = {0.0, 0.0};
+      if (va) {
+        a0 = *reinterpret_cast<const uph_d2*>(za + px);
+        a1 = *reinterpret_cast<const uph_d2*>(za + px + 2);
+      }
+      if (ti == tj) {
+        b0 = a0;
+        b1 = a1;
+      } else if (vb) {
+        b0 = *reinterpret_cast<const uph_d2*>(zb + px);
+        b1 = *reinterpret_cast<const uph_d2*>(zb + px + 2);
+      }
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.x, w0.x * b0.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a0.y, w0.y * b0.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.x, w1.x * b1.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1.y, w1.y * b1.y, acc, 0, 0, 0);
+    }
+    // register r of lane l holds D[4 r + l / 16][l % 16] (the float64
+    // layout differs from the float32 one)
+    double* pt = part + (size_t)u * 256;
+    pt[(0 + grp) * 16 + col] = acc.x;
+    pt[(4 + grp) * 16 + col] = acc.y;
+    pt[(8 + grp) * 16 + col] = acc.z;
+    pt[(12 + grp) * 16 + col] = acc.w;
+  }
+  __syncthreads();
+  for (int e = tid; e < ntp * 256; e += UPH_BLOCK) {
+    const int tp = e >> 8, i = (e >> 4) & 15, j = e & 15;
+    int ti, tj;
+    pair_of(tp, nt, ti, tj);
+    if (ti == tj && i > j) continue;  // diagonal tiles: upper entries only
+    const int a = ti * 16 + i, b = tj * 16 + j;
+    if (a >= C || b >= C) continue;
+    double v = 0.0;
+    for (int sp = 0; sp < S; ++sp) v += part[((size_t)tp * S + sp) * 256 + i * 16 + j];
+    out[a * C + b] = v;
+    out[b * C + a] = v;
+  }
+  __syncthreads();
+}
+
 // The Gram out (C x C) from its 16 x 16 tile pairs (ti <= tj, pair_of
 // order), each computed over all pixels as one unit of gram_mfma.
 __device__ void gram_from_pairs(const double* __restrict__ pairs, int C,
@@ -734,6 +809,8 @@ __device__ double eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, in
       gram_from_pairs(ap.big_part + ((size_t)slot * a.nb + b) * ap.big_ntp * 256, C, G);
     else
       gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
+  } else if constexpr (kPre) {
+    gram_mfma_f64(Wp, planes, P, C, 2, G, part);
   } else {
     gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
   }

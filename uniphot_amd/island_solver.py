@@ -870,23 +870,12 @@ def _fp64_groups(k64, budget: float = FP64_PLANE_BUDGET):
     return [g for g in groups if g]
 
 
-def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
-                sersic_dev=None, post: bool = True,
-                workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None, init_states=None,
-                prerender: bool = True):
-    """Fit ``classes`` (kernels, host, rows) on the float64 kernels
-    (band-split; members pre-rendered by the float64 (member, band)
-    launches, or in place with ``prerender=False``: the same planes, bit
-    for bit), then, with ``post``, the
-    float32 post-fit pass (no LM step) at the float64 parameters for the
-    parameter variances, marginal flux errors, metrics and conditioning.
-    Fluxes, flux errors and the LM state are the float64 fit's.  Every row
-    carries ``fp64_refit`` True."""
-    import dataclasses
-    if not classes:
-        return []
-    k64 = [(dataclasses.replace(k, precision="fp64"), h, np.asarray(r)) for k, h, r in classes]
-    # launch sequences of at most FP64_PLANE_BUDGET bytes of member planes
+def _run_fp64_split(k64, psf_trees, max_steps, dstack_dev, wstack_dev, *, sersic_dev=None,
+                    prerender=True, workspace_bytes=MAX_SCRATCH_BYTES, timer=None,
+                    init_states=None):
+    """:func:`run_island_batch` on the float64 band-split kernels in launch
+    sequences of at most FP64_PLANE_BUDGET bytes of member planes (see
+    :func:`_fp64_groups`); one result dict per class, rows in order."""
     groups = _fp64_groups(k64) if prerender else [[(ci, 0, r.size) for ci, (_, _, r)
                                                    in enumerate(k64)]]
     parts = [[] for _ in k64]
@@ -903,7 +892,28 @@ def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev
                                timer=timer, init_states=ist)
         for (ci, _, _), r in zip(grp, res):
             parts[ci].append(r)
-    r64 = [p[0] if len(p) == 1 else _merge_chunk_outputs(p) for p in parts]
+    return [p[0] if len(p) == 1 else _merge_chunk_outputs(p) for p in parts]
+
+
+def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
+                sersic_dev=None, post: bool = True,
+                workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None, init_states=None,
+                prerender: bool = True):
+    """Fit ``classes`` (kernels, host, rows) on the float64 kernels
+    (band-split; members pre-rendered by the float64 (member, band)
+    launches, or in place with ``prerender=False``: the same planes, bit
+    for bit), then, with ``post``, the
+    float32 post-fit pass (no LM step) at the float64 parameters for the
+    parameter variances, marginal flux errors, metrics and conditioning.
+    Fluxes, flux errors and the LM state are the float64 fit's.  Every row
+    carries ``fp64_refit`` True."""
+    import dataclasses
+    if not classes:
+        return []
+    k64 = [(dataclasses.replace(k, precision="fp64"), h, np.asarray(r)) for k, h, r in classes]
+    r64 = _run_fp64_split(k64, psf_trees, max_steps, dstack_dev, wstack_dev,
+                          sersic_dev=sersic_dev, prerender=prerender,
+                          workspace_bytes=workspace_bytes, timer=timer, init_states=init_states)
     if post:
         at64 = []
         for (kern, host, rows), r in zip(k64, r64):
@@ -1060,9 +1070,13 @@ def stall_probes(classes, results, psf_trees: dict, dstack_dev, wstack_dev, *, s
             h2["p0"] = np.concatenate([t for _, t in lst]).reshape(rep.size, -1, 6)
             batch.append((dataclasses.replace(kern, precision=prec), h2, np.arange(rep.size)))
             back.append((ci, [i for i, _ in lst]))
-        out = run_island_batch(batch, psf_trees, 0, dstack_dev, wstack_dev, precision=prec,
-                               sersic_dev=sersic_dev, split=True, prerender=prec == "fp32",
-                               workspace_bytes=workspace_bytes)
+        if prec == "fp64":
+            out = _run_fp64_split(batch, psf_trees, 0, dstack_dev, wstack_dev,
+                                  sersic_dev=sersic_dev, workspace_bytes=workspace_bytes)
+        else:
+            out = run_island_batch(batch, psf_trees, 0, dstack_dev, wstack_dev, precision=prec,
+                                   sersic_dev=sersic_dev, split=True, prerender=True,
+                                   workspace_bytes=workspace_bytes)
         for (ci, idx), r in zip(back, out):
             c2 = r["chi2"].reshape(len(idx), len(PLATEAU_LAMBDAS) + 1)
             results[ci]["stall_dchi2"][idx] = (c2[:, 1:].min(axis=1) - c2[:, 0]).astype(
