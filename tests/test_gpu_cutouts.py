@@ -72,9 +72,9 @@ def test_bf16_gram_against_fp64(cut):
 def test_fp64_prerender_equals_inplace_with_larger_tiers(cut):
     """The float64 member pre-render against the in-place float64 render
     on the whole cutout batch, stamp tiers 55 and 111 included: the same
-    steps and flags, parameters within float32 ulps, fluxes to float64
-    rounding (the pre-render evaluation's Gram is on float64 matrix
-    cores)."""
+    steps and flags, parameters within float32 ulps, fluxes as far as an
+    ulp of a position moves them (the pre-render's tier-55 transforms run
+    in LDS with paired real rows and its Gram on float64 matrix cores)."""
     import dataclasses
     import torch
     from uniphot_amd.pipeline import FitConfig, fit_cutouts, psf_bank
@@ -90,4 +90,4 @@ def test_fp64_prerender_equals_inplace_with_larger_tiers(cut):
         assert np.array_equal(getattr(a, k), getattr(b, k)), k
     assert np.nanmax(np.abs(a.p - b.p)) <= 1e-5
     scale = np.maximum(np.abs(a.flux), a.flux_err)
-    assert np.nanmax(np.abs(a.flux - b.flux) / scale) <= 1e-8
+    assert np.nanmax(np.abs(a.flux - b.flux) / scale) <= 2e-6

@@ -211,11 +211,13 @@ def test_fp64_prerender_equals_inplace(cuda, n_range):
 
 
 def _same_fit(ra, rb):
-    """Two float64 fits of one batch: identical steps and flags, float32
-    parameter state within a few ulps, fluxes and chi2 to float64 rounding."""
+    """Two float64 fits of one batch that differ only in float64 rounding:
+    identical steps and flags, the float32 parameter state within a few
+    ulps, and the fluxes (float32 outputs) and chi2 as far as one ulp of a
+    position moves them (~3e-7 of a bright flux)."""
     for k in ("nst", "conv", "stall", "n_good"):
         assert np.array_equal(ra[k], rb[k]), k
     assert np.nanmax(np.abs(ra["p"] - rb["p"]), initial=0.0) <= 1e-5
     scale = np.maximum(np.abs(ra["fluxes"]), ra["ferr"])
-    assert np.nanmax(np.abs(ra["fluxes"] - rb["fluxes"]) / scale, initial=0.0) <= 1e-8
-    assert np.nanmax(np.abs(ra["chi2"] - rb["chi2"]) / np.abs(ra["chi2"]), initial=0.0) <= 1e-10
+    assert np.nanmax(np.abs(ra["fluxes"] - rb["fluxes"]) / scale, initial=0.0) <= 2e-6
+    assert np.nanmax(np.abs(ra["chi2"] - rb["chi2"]) / np.abs(ra["chi2"]), initial=0.0) <= 1e-7

@@ -36,6 +36,39 @@ __host__ __device__ inline size_t spectra64_doubles(int T) {
 }
 
 size_t prerender64_scratch(int T) { return galaxy_scratch_bytes(T, sizeof(double)); }
+
+constexpr int kLine = 128, kHalf = kLine / 2 + 1;  // pad and pad / 2 + 1 of tier 55
+constexpr int kLdsRows = 64;                       // rows of the LDS line buffer (T, box <= 64)
+
+// Stamp tiers whose transforms run in LDS (pad 128); their spectra are
+// stored frequency-column-major (k * 128 + m), the others row-major
+// (m * (pad / 2 + 1) + k, fill_fourier's layout).
+__host__ __device__ inline bool lds_tier(int T) { return fft_pad_for(T) == kLine && T <= kLdsRows; }
+
+// fill_fourier's F, F1 and Fn in the column-major layout of an LDS tier.
+__device__ void fill_fourier_kmajor(const GalParams<double>& gp, bool grads, double* F) {
+  const int n = kLine * kHalf;
+  const double c2 = -2.0 * 9.869604401089358;  // -2 pi^2
+  for (int e = threadIdx.x; e < n; e += UPH_BLOCK) {
+    const int k = e / kLine, m = e % kLine;
+    const double v = double(k) / kLine;
+    const double w = double(m < kLine / 2 ? m : m - kLine) / kLine;
+    const double ex = c2 * (gp.t[0] * v * v + gp.t[1] * w * w + 2.0 * gp.t[2] * v * w);
+    double f = 0.0, f1 = 0.0, fn = 0.0;
+    for (int j = 0; j < gp.K; ++j) {
+      const double E = exp(gp.v[j] * ex);
+      f += gp.a[j] * E;
+      f1 += gp.a[j] * gp.v[j] * E;
+      fn += gp.da[j] * E + gp.a[j] * E * ex * gp.dv[j];
+    }
+    F[e] = f;
+    if (grads) {
+      F[n + e] = f1;
+      F[2 * n + e] = fn;
+    }
+  }
+  __syncthreads();
+}
 size_t prerender64_spectra_doubles(int T) { return spectra64_doubles(T); }
 
 __device__ __forceinline__ const float* pass_params(const IslandArgsPre& a) {
@@ -85,7 +118,10 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_spectra_kernel(IslandArgsPre 
   const int n = pad * (pad / 2 + 1);
   double* F = a.fspec64 + a.pre_spec[g];
   const bool grads = split_grads(a.mode);
-  fill_fourier(gp, pad, grads, grads, F);
+  if (lds_tier(T))
+    fill_fourier_kmajor(gp, grads, F);
+  else
+    fill_fourier(gp, pad, grads, grads, F);
   if (tid == 0) {
     double* tail = F + 3 * (size_t)n;
     tail[0] = gp.core;
@@ -216,39 +252,206 @@ __device__ void star_planes64(const IslandArgsPre& a, const float* coefs, int T,
   __syncthreads();
 }
 
-// Extended member: PSF over the whole stamp, the convolved mixture and its
-// raw tangents (galaxy_raw_spec), then the shift of the window part into
-// planes 0 (model) and 1 + j for the parameters j in `need`.
-__device__ void galaxy_planes64(const IslandArgsPre& a, const float* coefs, int T, int b,
-                                const MemberWindow& w, int need, const double* spec,
-                                const double* taps, const double* bas, double xs, double ys,
-                                double* ws, double* blk, int side) {
-  const int tid = threadIdx.x, nt = a.nt, TT = T * T;
-  const bool gxy = need & 3;
-  const bool grads = need != 0;
-  const GalLayout L = galaxy_layout(T);
-  const float* Cf = coefs + (size_t)b * nt * TT;
-  for (int e = tid; e < TT; e += UPH_BLOCK) {
-    double v = 0, vx = 0, vy = 0;
-    for (int t = 0; t < nt; ++t) {
-      const double cv = double(Cf[(size_t)t * TT + e]);
-      v = fma(cv, bas[t], v);
-      if (gxy) {
-        vx = fma(cv, bas[21 + t], vx);
-        vy = fma(cv, bas[42 + t], vy);
+// ---------------------------------------------------------------------------
+// Pad-128 transforms in LDS (stamp tier 55).  One wave owns a 128-point
+// line: lane l holds elements l and l + 64 in registers.  Radix-2
+// decimation in time on bit-reversed input: spans 1..32 across lanes
+// (__shfl_xor), span 64 inside the lane.  Row transforms of real data take
+// two rows per complex line (forward: split through conj(Z[N - k]); inverse:
+// Hermitian rows a + i b give real a in the real and real b in the
+// imaginary part).
+// ---------------------------------------------------------------------------
+struct Tw128 {
+  double c[6], s[6];  // cross-lane stage h = 2^s: cos, sin of pi (lane & (h - 1)) / h
+  double ic, is;      // in-lane stage: cos, sin of pi lane / 64
+};
+
+__device__ inline void tw128(Tw128& t) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int h = 1 << q;
+    sincospi(double(lane & (h - 1)) / double(h), &t.s[q], &t.c[q]);
+  }
+  sincospi(double(lane) / 64.0, &t.is, &t.ic);
+}
+
+__device__ __forceinline__ int brev7(int i) { return (int)(__brev((unsigned)i) >> 25); }
+
+// In-register transform of one line (sign -1 forward, +1 inverse,
+// unnormalised): on entry re/im[e] = x[brev7(lane + 64 e)], on exit the
+// outputs lane + 64 e in natural order.
+__device__ __forceinline__ void fft128(double (&re)[2], double (&im)[2], const Tw128& t,
+                                       double sign) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const int h = 1 << q;
+    const double wr = t.c[q], wi = sign * t.s[q];
+    const bool upper = lane & h;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const double pr = __shfl_xor(re[e], h, 64), pim = __shfl_xor(im[e], h, 64);
+      if (!upper) {
+        re[e] = re[e] + (wr * pr - wi * pim);
+        im[e] = im[e] + (wr * pim + wi * pr);
+      } else {
+        const double tr = wr * re[e] - wi * im[e], ti = wr * im[e] + wi * re[e];
+        re[e] = pr - tr;
+        im[e] = pim - ti;
       }
     }
-    ws[L.P + e] = v;
-    if (gxy) {
-      ws[L.Px + e] = vx / xs;
-      ws[L.Py + e] = vy / ys;
+  }
+  const double wr = t.ic, wi = sign * t.is;
+  const double br = wr * re[1] - wi * im[1], bi = wr * im[1] + wi * re[1];
+  const double ar = re[0], ai = im[0];
+  re[0] = ar + br;
+  im[0] = ai + bi;
+  re[1] = ar - br;
+  im[1] = ai - bi;
+}
+
+// Forward rfft2 of the (T, T) real plane X zero-padded to 128 x 128 into
+// S (65 frequency columns k of 128 rows m, column-major: k * 128 + m) in
+// global scratch; A is the LDS line buffer (T x 65 complex), zw a
+// 128-complex LDS line per wave.
+__device__ void rfft2_128(const double* X, int T, double2* S, double2* A, double2* zw) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  Tw128 tw;
+  tw128(tw);
+  double2* z = zw + wid * kLine;
+  const int c0 = brev7(lane), c1 = brev7(lane + 64);
+  // rows, two per line
+  for (int j = wid; 2 * j < T; j += UPH_WAVES) {
+    const int ra = 2 * j, rb = 2 * j + 1;
+    double re[2], im[2];
+    re[0] = c0 < T ? X[ra * T + c0] : 0.0;
+    re[1] = c1 < T ? X[ra * T + c1] : 0.0;
+    im[0] = (rb < T && c0 < T) ? X[rb * T + c0] : 0.0;
+    im[1] = (rb < T && c1 < T) ? X[rb * T + c1] : 0.0;
+    fft128(re, im, tw, -1.0);
+    z[lane] = make_double2(re[0], im[0]);
+    z[lane + 64] = make_double2(re[1], im[1]);
+    __builtin_amdgcn_wave_barrier();
+    for (int k = lane; k < kHalf; k += 64) {
+      const double2 zk = z[k], zm = z[(kLine - k) & (kLine - 1)];
+      A[ra * kHalf + k] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+      if (rb < T) A[rb * kHalf + k] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // columns
+  for (int k = wid; k < kHalf; k += UPH_WAVES) {
+    double re[2], im[2];
+    const double2 v0 = c0 < T ? A[c0 * kHalf + k] : make_double2(0.0, 0.0);
+    const double2 v1 = c1 < T ? A[c1 * kHalf + k] : make_double2(0.0, 0.0);
+    re[0] = v0.x;
+    im[0] = v0.y;
+    re[1] = v1.x;
+    im[1] = v1.y;
+    fft128(re, im, tw, -1.0);
+    S[k * kLine + lane] = make_double2(re[0], im[0]);
+    S[k * kLine + lane + 64] = make_double2(re[1], im[1]);
+  }
+  __syncthreads();
+}
+
+// Spectral weight of output plane kind `kind` at (m, k): 0 F, 1 F1 times
+// the ellipse-term derivative of parameter p, 2 Fn (spec_scale /
+// spec_shape_dt of galaxy_raw_spec), spectra column-major.
+__device__ __forceinline__ double spec_weight(const double* spec, int n, int kind, int p,
+                                              const double* dt9, int m, int k) {
+  const int e = k * kLine + m;
+  if (kind == 0) return spec[e];
+  if (kind == 2) return spec[2 * n + e];
+  const double c2 = -2.0 * 9.869604401089358;
+  const double v = double(k) / kLine;
+  const double w = double(m < kLine / 2 ? m : m - kLine) / kLine;
+  const double dex = c2 * (dt9[p] * v * v + dt9[3 + p] * w * w + 2.0 * dt9[6 + p] * v * w);
+  return double(spec[n + e]) * dex;
+}
+
+// Raw plane box [ry0, ry1) x [rx0, rx1) of irfft2(S * W)[:T, :T] (W by
+// kind/p), plus `core` times the real plane Q (T x T) when Q is given, into
+// out (row stride RW).  B is the LDS line buffer (box rows x 65 complex).
+__device__ void irfft2_box_128(const double2* S, const double* spec, int kind, int p,
+                               const double* dt9, int ry0, int ry1, int rx0, int rx1,
+                               const double* Q, double core, int T, double* out, double2* B) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  Tw128 tw;
+  tw128(tw);
+  const int n = kLine * kHalf, RH = ry1 - ry0, RW = rx1 - rx0;
+  const int m0 = brev7(lane), m1 = brev7(lane + 64);
+  // columns: inverse along m, keep the box rows
+  for (int k = wid; k < kHalf; k += UPH_WAVES) {
+    double re[2], im[2];
+    const double w0 = spec_weight(spec, n, kind, p, dt9, m0, k);
+    const double w1 = spec_weight(spec, n, kind, p, dt9, m1, k);
+    const double2 s0 = S[k * kLine + m0], s1 = S[k * kLine + m1];
+    re[0] = s0.x * w0;
+    im[0] = s0.y * w0;
+    re[1] = s1.x * w1;
+    im[1] = s1.y * w1;
+    fft128(re, im, tw, 1.0);
+    if (lane >= ry0 && lane < ry1) B[(lane - ry0) * kHalf + k] = make_double2(re[0], im[0]);
+    if (lane + 64 >= ry0 && lane + 64 < ry1)
+      B[(lane + 64 - ry0) * kHalf + k] = make_double2(re[1], im[1]);
+  }
+  __syncthreads();
+  // rows of the box, two per line: the Hermitian extension of row a plus i
+  // times that of row b; the inverse of that line holds row a's real
+  // output in its real part and row b's in its imaginary part
+  const double norm = 1.0 / (double(kLine) * double(kLine));
+  const int c0 = brev7(lane), c1 = brev7(lane + 64);
+  for (int j = wid; 2 * j < RH; j += UPH_WAVES) {
+    const int ia = 2 * j, ib = 2 * j + 1;
+    const bool hb = ib < RH;
+    double re[2], im[2];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = e ? c1 : c0;
+      const bool lo = c < kHalf;
+      const int cc = lo ? c : kLine - c;
+      const double2 a = B[ia * kHalf + cc];
+      const double2 b = hb ? B[ib * kHalf + cc] : make_double2(0.0, 0.0);
+      // the DC and Nyquist bins enter with their real parts only, as in a
+      // complex-to-real transform (which a pair would otherwise mix)
+      const bool edge = cc == 0 || cc == kLine / 2;
+      const double ai = edge ? 0.0 : (lo ? a.y : -a.y);
+      const double bi = edge ? 0.0 : (lo ? b.y : -b.y);
+      re[e] = a.x - bi;
+      im[e] = ai + b.x;
+    }
+    fft128(re, im, tw, 1.0);
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      const int c = lane + 64 * e;
+      if (c < rx0 || c >= rx1) continue;
+      const int ra = ry0 + ia, rb = ry0 + ib;
+      double va = re[e] * norm;
+      if (Q) va += core * Q[ra * T + c];
+      out[ia * RW + (c - rx0)] = va;
+      if (hb) {
+        double vb = im[e] * norm;
+        if (Q) vb += core * Q[rb * T + c];
+        out[ib * RW + (c - rx0)] = vb;
+      }
     }
   }
   __syncthreads();
-  const int n = L.pad * L.H;
-  const double* tail = spec + 3 * (size_t)n;
-  galaxy_raw_spec<double, false>(spec, tail[0], tail[1], tail + 2, T, need, ws);
-  const double dndu = tail[11];
+}
+
+// Shift of the raw planes into the window part: horizontal then vertical
+// Lanczos-3 pass (with the tap derivatives on the x / y tangents), planes
+// 0 (model) and 1 + j for the parameters j in `need`.  Raw plane q is read
+// at G + q * gq + (row - gy0) * gs + (col - gx0) for rows in [ry0, ry1) and
+// columns in [rx0, rx1).
+__device__ void shift_planes64(const double* G, size_t gq, int gs, int gy0, int gx0, int T,
+                               const MemberWindow& w, int need, const double* taps, double dndu,
+                               double* Hh, double* blk, int side) {
+  const int tid = threadIdx.x, TT = T * T;
+  const bool grads = need != 0;
   const int RH = w.ry1 - w.ry0, SW = w.sx1 - w.sx0, SH = w.sy1 - w.sy0;
   const size_t PP = (size_t)side * side;
   double Lx[7], dLx[7], Ly[7], dLy[7];
@@ -259,8 +462,6 @@ __device__ void galaxy_planes64(const IslandArgsPre& a, const float* coefs, int 
     dLx[k] = grads ? taps[7 + k] : 0.0;
     dLy[k] = grads ? taps[21 + k] : 0.0;
   }
-  const double* G = ws + L.G;
-  double* Hh = ws + L.Hh;  // plane q of the horizontal pass at Hh + q T^2
   for (int e = tid; e < RH * SW; e += UPH_BLOCK) {
     const int rr = w.ry0 + e / SW, c = w.sx0 + e % SW;
     double h[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -268,12 +469,12 @@ __device__ void galaxy_planes64(const IslandArgsPre& a, const float* coefs, int 
     for (int k = 0; k < 7; ++k) {
       const int cc = c + k - 3;
       if (cc < 0 || cc >= T) continue;
-      const int o = rr * T + cc;
+      const size_t o = (size_t)(rr - gy0) * gs + (cc - gx0);
       h[0] = fma(Lx[k], G[o], h[0]);
-      if (need & 1) h[1] = fma(Lx[k], G[TT + o], fma(dLx[k], G[o], h[1]));
+      if (need & 1) h[1] = fma(Lx[k], G[gq + o], fma(dLx[k], G[o], h[1]));
 #pragma unroll
       for (int j = 1; j < 6; ++j)
-        if (need & (1 << j)) h[1 + j] = fma(Lx[k], G[(size_t)(1 + j) * TT + o], h[1 + j]);
+        if (need & (1 << j)) h[1 + j] = fma(Lx[k], G[(size_t)(1 + j) * gq + o], h[1 + j]);
     }
     Hh[e] = h[0];
 #pragma unroll
@@ -306,55 +507,168 @@ __device__ void galaxy_planes64(const IslandArgsPre& a, const float* coefs, int 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_kernel(IslandArgsPre a) {
-  __shared__ double taps[28];
-  __shared__ double bas[63];
-  __shared__ int s_item;
-  const int tid = threadIdx.x;
-  double* ws = reinterpret_cast<double*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
-  const long long n_items = (long long)a.n_pre * a.nb;
-  const float* psrc = pass_params(a);
-  const bool grads = split_grads(a.mode);
-  for (;;) {
-    __syncthreads();  // the previous item's readers of taps/bas are done
-    if (tid == 0) s_item = atomicAdd(a.pre_counter, 1);
-    __syncthreads();
-    const long long item = s_item;
-    if (item >= n_items) break;
-    const int g = (int)(item / a.nb), b = (int)(item % a.nb);
-    const int isl = a.pre_isl[g];
-    if (pass_skips(a, isl)) continue;
-    const int32_t* sp = a.shape + 4 * (size_t)isl;
-    const int B = sp[1], T = a.tier_T[sp[2]];
-    const float* coefs = a.tier_coefs[sp[2]];
-    const size_t slot = (size_t)a.mem_off[isl] + a.pre_slot[g];
-    const float* p6 = psrc + slot * 6;
-    const bool star = a.stype[slot] == 0;
-    MemberWindow w;
-    if (!member_window(p6[0], p6[1], a.x0b[isl], a.y0b[isl], T, B, w)) continue;  // uniform
-    int need = 0;
-    if (grads)
-      for (int j = 0; j < 6; ++j) need |= a.keep[slot * 6 + j] ? (1 << j) : 0;
-    const int side = pre_side(T, B);
-    double* blk = a.pbuf64 + a.pre_off[g] + (size_t)b * pre_planes(star) * side * side;
-    const float* geo = a.psf_geo + 4 * b;
-    const double xs = double(geo[1]), ys = double(geo[3]);
-    const double x = double(p6[0]), y = double(p6[1]);
-    const double xg = double(p6[0] - a.x0b[isl]), yg = double(p6[1] - a.y0b[isl]);
-    const double dx = xg - rint(xg), dy = yg - rint(yg);
-    const bool tg = star ? grads : need != 0;
-    if (tid == 0) lanczos_taps(dx, taps, tg ? taps + 7 : (double*)nullptr);
-    if (tid == 64) lanczos_taps(dy, taps + 14, tg ? taps + 21 : (double*)nullptr);
-    if (tid == 128) {
-      const double X = (x - double(geo[0])) / xs, Y = (y - double(geo[2])) / ys;
-      psf_basis(X, Y, a.nt, bas, bas + 21, bas + 42);
+// PSF planes of an extended member over the whole stamp: P, and dP/dx,
+// dP/dy when the x or y tangent is needed, at the layout's P, Px, Py.
+__device__ void galaxy_psf64(const IslandArgsPre& a, const float* coefs, int T, int b, int need,
+                             const double* bas, double xs, double ys, double* ws) {
+  const int tid = threadIdx.x, nt = a.nt, TT = T * T;
+  const bool gxy = need & 3;
+  const GalLayout L = galaxy_layout(T);
+  const float* Cf = coefs + (size_t)b * nt * TT;
+  for (int e = tid; e < TT; e += UPH_BLOCK) {
+    double v = 0, vx = 0, vy = 0;
+    for (int t = 0; t < nt; ++t) {
+      const double cv = double(Cf[(size_t)t * TT + e]);
+      v = fma(cv, bas[t], v);
+      if (gxy) {
+        vx = fma(cv, bas[21 + t], vx);
+        vy = fma(cv, bas[42 + t], vy);
+      }
     }
-    __syncthreads();
+    ws[L.P + e] = v;
+    if (gxy) {
+      ws[L.Px + e] = vx / xs;
+      ws[L.Py + e] = vy / ys;
+    }
+  }
+  __syncthreads();
+}
+
+// Extended member of an LDS tier (pad 128): the convolved mixture and its
+// raw tangents over the box the shift reads only, by transforms in LDS,
+// then the shift of the window part into planes 0 (model) and 1 + j for
+// the parameters j in `need`.
+__device__ void galaxy_planes64_lds(const IslandArgsPre& a, const float* coefs, int T, int b,
+                                    const MemberWindow& w, int need, const double* spec,
+                                    const double* taps, const double* bas, double xs, double ys,
+                                    double* ws, double* blk, int side, double2* lds) {
+  const int TT = T * T;
+  const GalLayout L = galaxy_layout(T);
+  galaxy_psf64(a, coefs, T, b, need, bas, xs, ys, ws);
+  const int n = L.pad * L.H;
+  const double* tail = spec + 3 * (size_t)n;
+  const double core = tail[0], dcore = tail[1], dndu = tail[11];
+  const double* dt9 = tail + 2;
+  // raw-plane boxes (RH x RW, row stride RW) at ws + L.G + q T^2; the
+  // spectrum in the (16-byte aligned) spectra region
+  const int RW = w.rx1 - w.rx0;
+  double2* S = reinterpret_cast<double2*>(ws + (L.SP + 1) / 2 * 2);
+  double2* A = lds;
+  double2* zw = lds + kLdsRows * kHalf;
+  double* G = ws + L.G;
+  rfft2_128(ws + L.P, T, S, A, zw);
+  irfft2_box_128(S, spec, 0, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, ws + L.P, core, T, G, A);
+  for (int p = 0; p < 3; ++p)
+    if (need & (4 << p))
+      irfft2_box_128(S, spec, 1, p, dt9, w.ry0, w.ry1, w.rx0, w.rx1, nullptr, 0.0, T,
+                     G + (size_t)(3 + p) * TT, A);
+  if (need & 32)
+    irfft2_box_128(S, spec, 2, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, ws + L.P, dcore, T,
+                   G + (size_t)6 * TT, A);
+  for (int j = 0; j < 2; ++j) {
+    if (!(need & (1 << j))) continue;
+    const double* Q = ws + (j == 0 ? L.Px : L.Py);
+    rfft2_128(Q, T, S, A, zw);
+    irfft2_box_128(S, spec, 0, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, Q, core, T,
+                   G + (size_t)(1 + j) * TT, A);
+  }
+  shift_planes64(G, TT, RW, w.ry0, w.rx0, T, w, need, taps, dndu, ws + L.Hh, blk, side);
+}
+
+// Extended member of a larger tier: galaxy_raw_spec's block transforms in
+// the workgroup's scratch over the whole stamp, then the shift.
+__device__ void galaxy_planes64_block(const IslandArgsPre& a, const float* coefs, int T, int b,
+                                      const MemberWindow& w, int need, const double* spec,
+                                      const double* taps, const double* bas, double xs,
+                                      double ys, double* ws, double* blk, int side) {
+  const int TT = T * T;
+  const GalLayout L = galaxy_layout(T);
+  galaxy_psf64(a, coefs, T, b, need, bas, xs, ys, ws);
+  const int n = L.pad * L.H;
+  const double* tail = spec + 3 * (size_t)n;
+  galaxy_raw_spec<double, false>(spec, tail[0], tail[1], tail + 2, T, need, ws);
+  shift_planes64(ws + L.G, TT, T, 0, 0, T, w, need, taps, tail[11], ws + L.Hh, blk, side);
+}
+
+// One (member, band) item.  kBig = false takes stars and the extended
+// members of LDS tiers, kBig = true the extended members of larger tiers;
+// an item of the other kind is left alone.  Block-uniform; ends after a
+// barrier.
+template <bool kBig>
+__device__ void plane_item(const IslandArgsPre& a, long long item, double* taps, double* bas,
+                           double2* lds, double* ws) {
+  const int tid = threadIdx.x;
+  const bool grads = split_grads(a.mode);
+  const int g = (int)(item / a.nb), b = (int)(item % a.nb);
+  const int isl = a.pre_isl[g];
+  if (pass_skips(a, isl)) return;
+  const int32_t* sp = a.shape + 4 * (size_t)isl;
+  const int B = sp[1], T = a.tier_T[sp[2]];
+  const size_t slot = (size_t)a.mem_off[isl] + a.pre_slot[g];
+  const bool star = a.stype[slot] == 0;
+  if (kBig != (!star && !lds_tier(T))) return;
+  const float* coefs = a.tier_coefs[sp[2]];
+  const float* p6 = pass_params(a) + slot * 6;
+  MemberWindow w;
+  if (!member_window(p6[0], p6[1], a.x0b[isl], a.y0b[isl], T, B, w)) return;
+  int need = 0;
+  if (grads)
+    for (int j = 0; j < 6; ++j) need |= a.keep[slot * 6 + j] ? (1 << j) : 0;
+  const int side = pre_side(T, B);
+  double* blk = a.pbuf64 + a.pre_off[g] + (size_t)b * pre_planes(star) * side * side;
+  const float* geo = a.psf_geo + 4 * b;
+  const double xs = double(geo[1]), ys = double(geo[3]);
+  const double x = double(p6[0]), y = double(p6[1]);
+  const double xg = double(p6[0] - a.x0b[isl]), yg = double(p6[1] - a.y0b[isl]);
+  const double dx = xg - rint(xg), dy = yg - rint(yg);
+  const bool tg = star ? grads : need != 0;
+  if (tid == 0) lanczos_taps(dx, taps, tg ? taps + 7 : (double*)nullptr);
+  if (tid == 64) lanczos_taps(dy, taps + 14, tg ? taps + 21 : (double*)nullptr);
+  if (tid == 128) {
+    const double X = (x - double(geo[0])) / xs, Y = (y - double(geo[2])) / ys;
+    psf_basis(X, Y, a.nt, bas, bas + 21, bas + 42);
+  }
+  __syncthreads();
+  const double* spec = star ? nullptr : a.fspec64 + a.pre_spec[g];
+  if constexpr (kBig) {
+    galaxy_planes64_block(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side);
+  } else {
     if (star)
       star_planes64(a, coefs, T, b, w, grads, taps, bas, xs, ys, ws, blk, side);
     else
-      galaxy_planes64(a, coefs, T, b, w, need, a.fspec64 + a.pre_spec[g], taps, bas, xs, ys, ws,
-                      blk, side);
+      galaxy_planes64_lds(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side, lds);
+  }
+}
+
+// Stars and the extended members of LDS tiers, from the pass's work queue
+// (extended members first).
+__global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_kernel(IslandArgsPre a) {
+  __shared__ double taps[28];
+  __shared__ double bas[63];
+  __shared__ long long s_item;
+  // line buffer (kLdsRows x 65 complex) and one 128-complex line per wave
+  __shared__ double2 lds[kLdsRows * kHalf + UPH_WAVES * kLine];
+  double* ws = reinterpret_cast<double*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
+  const long long n_items = (long long)a.n_pre * a.nb;
+  for (;;) {
+    __syncthreads();  // the previous item's readers of s_item, taps and bas are done
+    if (threadIdx.x == 0) s_item = atomicAdd(a.pre_counter, 1);
+    __syncthreads();
+    const long long item = s_item;
+    if (item >= n_items) break;
+    plane_item<false>(a, item, taps, bas, lds, ws);
+  }
+}
+
+// The extended members of larger stamp tiers: items strided over the grid.
+__global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_big_kernel(IslandArgsPre a) {
+  __shared__ double taps[28];
+  __shared__ double bas[63];
+  double* ws = reinterpret_cast<double*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
+  const long long n_items = (long long)a.n_pre_ext * a.nb;
+  for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
+    __syncthreads();  // the previous item's readers of taps and bas are done
+    plane_item<true>(a, item, taps, bas, nullptr, ws);
   }
 }
 
@@ -369,6 +683,12 @@ void prerender64_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, 
   const long long items = (long long)p.n_pre * p.nb;
   const int g = (int)(items < nblk ? items : nblk);
   if (g > 0) hipLaunchKernelGGL(pre64_planes_kernel, dim3(g), dim3(UPH_BLOCK), 0, s, p);
+  bool big = false;
+  for (int t = 0; t < 3; ++t) big = big || (p.tier_T[t] > 0 && !lds_tier(p.tier_T[t]));
+  const long long items_ext = (long long)p.n_pre_ext * p.nb;
+  const int gb = (int)(items_ext < nblk ? items_ext : nblk);
+  if (big && gb > 0)
+    hipLaunchKernelGGL(pre64_planes_big_kernel, dim3(gb), dim3(UPH_BLOCK), 0, s, p);
   if (ev_planes) (void)hipEventRecord(ev_planes, s);
 }
 
