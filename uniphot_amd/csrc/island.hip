@@ -1897,8 +1897,9 @@ static void split_step(uph::IslandArgs a, int mode, hipStream_t s) {
 
 // Evaluation pass k of the sequence: the member pre-render (when `pre` is
 // given), the large-island Gram launch (when it has islands), then the
-// (island, band) evaluation, on work-queue counters 2k (evaluation) and
-// 2k + 1 (extended-member planes); the caller's timing events 6k (start),
+// (island, band) evaluation, on work-queue counters 3k (evaluation), 3k + 1
+// (member planes) and 3k + 2 (float64 planes of the larger stamp tiers); the
+// caller's timing events 6k (start),
 // 6k + 1 (after the taps/spectra launch), 6k + 2 (after the extended-member
 // planes), 6k + 3 (after the star planes), 6k + 4 (after the large-island
 // Gram, before the evaluation) and 6k + 5 (end) when given.
@@ -1911,9 +1912,9 @@ static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, 
   };
   rec(0);
   if (pre && sizeof(Real) == 8) {
-    uph::prerender64_pass(*pre, mode, counters + 2 * k + 1, nblk, s, evk(1), evk(2));
+    uph::prerender64_pass(*pre, mode, counters + 3 * k + 1, nblk, s, evk(1), evk(2));
   } else if (pre) {
-    uph::prerender_pass(*pre, mode, counters + 2 * k + 1, nblk, s, evk(1), evk(2));
+    uph::prerender_pass(*pre, mode, counters + 3 * k + 1, nblk, s, evk(1), evk(2));
   } else {
     rec(1);
     rec(2);
@@ -1927,9 +1928,9 @@ static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, 
   }
   rec(4);
   if (pre)
-    split_eval_pre<Real>(*pre, mode, counters + 2 * k, nblk, s);
+    split_eval_pre<Real>(*pre, mode, counters + 3 * k, nblk, s);
   else
-    split_eval<Real>(a, mode, counters + 2 * k, nblk, s);
+    split_eval<Real>(a, mode, counters + 3 * k, nblk, s);
   rec(5);
   ++k;
 }
@@ -1956,7 +1957,9 @@ static void split_sequence(const uph::IslandArgs& a, const uph::IslandArgsPre* p
 extern "C" size_t uph_island_split_counters(int max_steps) {
   // evaluation passes: initial, two per step, final, post-fit
   const size_t passes = (size_t)(2 * (max_steps > 0 ? max_steps : 0) + 3);
-  return 2 * passes + 1;  // evaluation + planes queue per pass, then the error flag
+  // per pass: evaluation queue, planes queue, large-tier planes queue; then
+  // the error flag
+  return 3 * passes + 1;
 }
 
 extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_island_inputs* in,

@@ -36,23 +36,28 @@ __host__ __device__ inline size_t spectra64_doubles(int T) {
 }
 
 size_t prerender64_scratch(int T) { return galaxy_scratch_bytes(T, sizeof(double)); }
+size_t prerender64_spectra_doubles(int T) { return spectra64_doubles(T); }
 
-constexpr int kLine = 128, kHalf = kLine / 2 + 1;  // pad and pad / 2 + 1 of tier 55
-constexpr int kLdsRows = 64;                       // rows of the LDS line buffer (T, box <= 64)
+constexpr int kLdsRows = 64;  // rows of the LDS line buffer of the pad-128 tier (T, box)
 
-// Stamp tiers whose transforms run in LDS (pad 128); their spectra are
-// stored frequency-column-major (k * 128 + m), the others row-major
+// Stamp tiers whose transforms run as wave-held lines with only the raw
+// box the shift reads: pad 128 (line buffers in LDS, T <= kLdsRows) and pad
+// 256 (line buffers in the workgroup's scratch).  Their spectra are stored
+// frequency-column-major (k * pad + m), the others row-major
 // (m * (pad / 2 + 1) + k, fill_fourier's layout).
-__host__ __device__ inline bool lds_tier(int T) { return fft_pad_for(T) == kLine && T <= kLdsRows; }
+__host__ __device__ inline bool lds_tier(int T) { return fft_pad_for(T) == 128 && T <= kLdsRows; }
+__host__ __device__ inline bool box_tier(int T) {
+  return lds_tier(T) || fft_pad_for(T) == 256;
+}
 
-// fill_fourier's F, F1 and Fn in the column-major layout of an LDS tier.
-__device__ void fill_fourier_kmajor(const GalParams<double>& gp, bool grads, double* F) {
-  const int n = kLine * kHalf;
+// fill_fourier's F, F1 and Fn in the column-major layout of a box tier.
+__device__ void fill_fourier_kmajor(const GalParams<double>& gp, int pad, bool grads, double* F) {
+  const int n = pad * (pad / 2 + 1);
   const double c2 = -2.0 * 9.869604401089358;  // -2 pi^2
   for (int e = threadIdx.x; e < n; e += UPH_BLOCK) {
-    const int k = e / kLine, m = e % kLine;
-    const double v = double(k) / kLine;
-    const double w = double(m < kLine / 2 ? m : m - kLine) / kLine;
+    const int k = e / pad, m = e % pad;
+    const double v = double(k) / pad;
+    const double w = double(m < pad / 2 ? m : m - pad) / pad;
     const double ex = c2 * (gp.t[0] * v * v + gp.t[1] * w * w + 2.0 * gp.t[2] * v * w);
     double f = 0.0, f1 = 0.0, fn = 0.0;
     for (int j = 0; j < gp.K; ++j) {
@@ -69,7 +74,6 @@ __device__ void fill_fourier_kmajor(const GalParams<double>& gp, bool grads, dou
   }
   __syncthreads();
 }
-size_t prerender64_spectra_doubles(int T) { return spectra64_doubles(T); }
 
 __device__ __forceinline__ const float* pass_params(const IslandArgsPre& a) {
   return a.mode == kSplitEvalInit ? a.p0 : (a.mode == kSplitEvalTry ? a.p_try : a.p_out);
@@ -118,8 +122,8 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_spectra_kernel(IslandArgsPre 
   const int n = pad * (pad / 2 + 1);
   double* F = a.fspec64 + a.pre_spec[g];
   const bool grads = split_grads(a.mode);
-  if (lds_tier(T))
-    fill_fourier_kmajor(gp, grads, F);
+  if (box_tier(T))
+    fill_fourier_kmajor(gp, pad, grads, F);
   else
     fill_fourier(gp, pad, grads, grads, F);
   if (tid == 0) {
@@ -253,36 +257,55 @@ __device__ void star_planes64(const IslandArgsPre& a, const float* coefs, int T,
 }
 
 // ---------------------------------------------------------------------------
-// Pad-128 transforms in LDS (stamp tier 55).  One wave owns a 128-point
-// line: lane l holds elements l and l + 64 in registers.  Radix-2
+// Box-tier transforms.  One wave owns an N = 64 E point line (E = 2: pad 128,
+// E = 4: pad 256): lane l holds elements l + 64 e in registers.  Radix-2
 // decimation in time on bit-reversed input: spans 1..32 across lanes
-// (__shfl_xor), span 64 inside the lane.  Row transforms of real data take
+// (__shfl_xor), spans 64.. inside the lane.  Row transforms of real data take
 // two rows per complex line (forward: split through conj(Z[N - k]); inverse:
 // Hermitian rows a + i b give real a in the real and real b in the
 // imaginary part).
 // ---------------------------------------------------------------------------
-struct Tw128 {
-  double c[6], s[6];  // cross-lane stage h = 2^s: cos, sin of pi (lane & (h - 1)) / h
-  double ic, is;      // in-lane stage: cos, sin of pi lane / 64
+template <int E>
+struct LineE {
+  static constexpr int N = 64 * E, H = N / 2 + 1, LOGE = E == 2 ? 1 : 2, LOGN = 6 + LOGE;
 };
 
-__device__ inline void tw128(Tw128& t) {
+template <int E>
+struct TwN {
+  double c[6], s[6];                    // cross-lane stage h = 2^q: pi (lane & (h - 1)) / h
+  double ic[LineE<E>::LOGE][E], is[LineE<E>::LOGE][E];  // in-lane stage t, element e
+};
+
+template <int E>
+__device__ inline void twN(TwN<E>& t) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     const int h = 1 << q;
     sincospi(double(lane & (h - 1)) / double(h), &t.s[q], &t.c[q]);
   }
-  sincospi(double(lane) / 64.0, &t.is, &t.ic);
+#pragma unroll
+  for (int q = 0; q < LineE<E>::LOGE; ++q) {
+    const int he = 1 << q;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int j = lane + 64 * (e & (he - 1));
+      sincospi(double(j) / double(64 * he), &t.is[q][e], &t.ic[q][e]);
+    }
+  }
 }
 
-__device__ __forceinline__ int brev7(int i) { return (int)(__brev((unsigned)i) >> 25); }
+template <int E>
+__device__ __forceinline__ int brevN(int i) {
+  return (int)(__brev((unsigned)i) >> (32 - LineE<E>::LOGN));
+}
 
 // In-register transform of one line (sign -1 forward, +1 inverse,
-// unnormalised): on entry re/im[e] = x[brev7(lane + 64 e)], on exit the
+// unnormalised): on entry re/im[e] = x[brevN(lane + 64 e)], on exit the
 // outputs lane + 64 e in natural order.
-__device__ __forceinline__ void fft128(double (&re)[2], double (&im)[2], const Tw128& t,
-                                       double sign) {
+template <int E>
+__device__ __forceinline__ void fftN(double (&re)[E], double (&im)[E], const TwN<E>& t,
+                                     double sign) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
@@ -290,7 +313,7 @@ __device__ __forceinline__ void fft128(double (&re)[2], double (&im)[2], const T
     const double wr = t.c[q], wi = sign * t.s[q];
     const bool upper = lane & h;
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
+    for (int e = 0; e < E; ++e) {
       const double pr = __shfl_xor(re[e], h, 64), pim = __shfl_xor(im[e], h, 64);
       if (!upper) {
         re[e] = re[e] + (wr * pr - wi * pim);
@@ -302,57 +325,72 @@ __device__ __forceinline__ void fft128(double (&re)[2], double (&im)[2], const T
       }
     }
   }
-  const double wr = t.ic, wi = sign * t.is;
-  const double br = wr * re[1] - wi * im[1], bi = wr * im[1] + wi * re[1];
-  const double ar = re[0], ai = im[0];
-  re[0] = ar + br;
-  im[0] = ai + bi;
-  re[1] = ar - br;
-  im[1] = ai - bi;
+#pragma unroll
+  for (int q = 0; q < LineE<E>::LOGE; ++q) {
+    const int he = 1 << q;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      if (e & he) continue;
+      const int f = e + he;
+      const double wr = t.ic[q][e], wi = sign * t.is[q][e];
+      const double br = wr * re[f] - wi * im[f], bi = wr * im[f] + wi * re[f];
+      const double ar = re[e], ai = im[e];
+      re[e] = ar + br;
+      im[e] = ai + bi;
+      re[f] = ar - br;
+      im[f] = ai - bi;
+    }
+  }
 }
 
-// Forward rfft2 of the (T, T) real plane X zero-padded to 128 x 128 into
-// S (65 frequency columns k of 128 rows m, column-major: k * 128 + m) in
-// global scratch; A is the LDS line buffer (T x 65 complex), zw a
-// 128-complex LDS line per wave.
-__device__ void rfft2_128(const double* X, int T, double2* S, double2* A, double2* zw) {
+// Forward rfft2 of the (T, T) real plane X zero-padded to N x N into S
+// (H frequency columns k of N rows m, column-major: k * N + m); A is the
+// line buffer (T x H complex, LDS or scratch), zw an N-complex LDS line
+// per wave.
+template <int E>
+__device__ void rfft2_N(const double* X, int T, double2* S, double2* A, double2* zw) {
+  constexpr int N = LineE<E>::N, H = LineE<E>::H;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  Tw128 tw;
-  tw128(tw);
-  double2* z = zw + wid * kLine;
-  const int c0 = brev7(lane), c1 = brev7(lane + 64);
+  TwN<E> tw;
+  twN<E>(tw);
+  double2* z = zw + wid * N;
+  int ci[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) ci[e] = brevN<E>(lane + 64 * e);
   // rows, two per line
   for (int j = wid; 2 * j < T; j += UPH_WAVES) {
     const int ra = 2 * j, rb = 2 * j + 1;
-    double re[2], im[2];
-    re[0] = c0 < T ? X[ra * T + c0] : 0.0;
-    re[1] = c1 < T ? X[ra * T + c1] : 0.0;
-    im[0] = (rb < T && c0 < T) ? X[rb * T + c0] : 0.0;
-    im[1] = (rb < T && c1 < T) ? X[rb * T + c1] : 0.0;
-    fft128(re, im, tw, -1.0);
-    z[lane] = make_double2(re[0], im[0]);
-    z[lane + 64] = make_double2(re[1], im[1]);
+    double re[E], im[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int c = ci[e];
+      re[e] = c < T ? X[ra * T + c] : 0.0;
+      im[e] = (rb < T && c < T) ? X[rb * T + c] : 0.0;
+    }
+    fftN<E>(re, im, tw, -1.0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) z[lane + 64 * e] = make_double2(re[e], im[e]);
     __builtin_amdgcn_wave_barrier();
-    for (int k = lane; k < kHalf; k += 64) {
-      const double2 zk = z[k], zm = z[(kLine - k) & (kLine - 1)];
-      A[ra * kHalf + k] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-      if (rb < T) A[rb * kHalf + k] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
+    for (int k = lane; k < H; k += 64) {
+      const double2 zk = z[k], zm = z[(N - k) & (N - 1)];
+      A[ra * H + k] = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
+      if (rb < T) A[rb * H + k] = make_double2(0.5 * (zk.y + zm.y), 0.5 * (zm.x - zk.x));
     }
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
   // columns
-  for (int k = wid; k < kHalf; k += UPH_WAVES) {
-    double re[2], im[2];
-    const double2 v0 = c0 < T ? A[c0 * kHalf + k] : make_double2(0.0, 0.0);
-    const double2 v1 = c1 < T ? A[c1 * kHalf + k] : make_double2(0.0, 0.0);
-    re[0] = v0.x;
-    im[0] = v0.y;
-    re[1] = v1.x;
-    im[1] = v1.y;
-    fft128(re, im, tw, -1.0);
-    S[k * kLine + lane] = make_double2(re[0], im[0]);
-    S[k * kLine + lane + 64] = make_double2(re[1], im[1]);
+  for (int k = wid; k < H; k += UPH_WAVES) {
+    double re[E], im[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double2 v = ci[e] < T ? A[ci[e] * H + k] : make_double2(0.0, 0.0);
+      re[e] = v.x;
+      im[e] = v.y;
+    }
+    fftN<E>(re, im, tw, -1.0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) S[k * N + lane + 64 * e] = make_double2(re[e], im[e]);
   }
   __syncthreads();
 }
@@ -360,72 +398,80 @@ __device__ void rfft2_128(const double* X, int T, double2* S, double2* A, double
 // Spectral weight of output plane kind `kind` at (m, k): 0 F, 1 F1 times
 // the ellipse-term derivative of parameter p, 2 Fn (spec_scale /
 // spec_shape_dt of galaxy_raw_spec), spectra column-major.
-__device__ __forceinline__ double spec_weight(const double* spec, int n, int kind, int p,
+template <int E>
+__device__ __forceinline__ double spec_weight(const double* spec, int kind, int p,
                                               const double* dt9, int m, int k) {
-  const int e = k * kLine + m;
+  constexpr int N = LineE<E>::N, n = N * LineE<E>::H;
+  const int e = k * N + m;
   if (kind == 0) return spec[e];
   if (kind == 2) return spec[2 * n + e];
   const double c2 = -2.0 * 9.869604401089358;
-  const double v = double(k) / kLine;
-  const double w = double(m < kLine / 2 ? m : m - kLine) / kLine;
+  const double v = double(k) / N;
+  const double w = double(m < N / 2 ? m : m - N) / N;
   const double dex = c2 * (dt9[p] * v * v + dt9[3 + p] * w * w + 2.0 * dt9[6 + p] * v * w);
   return double(spec[n + e]) * dex;
 }
 
 // Raw plane box [ry0, ry1) x [rx0, rx1) of irfft2(S * W)[:T, :T] (W by
 // kind/p), plus `core` times the real plane Q (T x T) when Q is given, into
-// out (row stride RW).  B is the LDS line buffer (box rows x 65 complex).
-__device__ void irfft2_box_128(const double2* S, const double* spec, int kind, int p,
-                               const double* dt9, int ry0, int ry1, int rx0, int rx1,
-                               const double* Q, double core, int T, double* out, double2* B) {
+// out (row stride RW).  B is the line buffer (box rows x H complex, LDS or
+// scratch).
+template <int E>
+__device__ void irfft2_box_N(const double2* S, const double* spec, int kind, int p,
+                             const double* dt9, int ry0, int ry1, int rx0, int rx1,
+                             const double* Q, double core, int T, double* out, double2* B) {
+  constexpr int N = LineE<E>::N, H = LineE<E>::H;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  Tw128 tw;
-  tw128(tw);
-  const int n = kLine * kHalf, RH = ry1 - ry0, RW = rx1 - rx0;
-  const int m0 = brev7(lane), m1 = brev7(lane + 64);
+  TwN<E> tw;
+  twN<E>(tw);
+  const int RH = ry1 - ry0, RW = rx1 - rx0;
+  int mi[E];
+#pragma unroll
+  for (int e = 0; e < E; ++e) mi[e] = brevN<E>(lane + 64 * e);
   // columns: inverse along m, keep the box rows
-  for (int k = wid; k < kHalf; k += UPH_WAVES) {
-    double re[2], im[2];
-    const double w0 = spec_weight(spec, n, kind, p, dt9, m0, k);
-    const double w1 = spec_weight(spec, n, kind, p, dt9, m1, k);
-    const double2 s0 = S[k * kLine + m0], s1 = S[k * kLine + m1];
-    re[0] = s0.x * w0;
-    im[0] = s0.y * w0;
-    re[1] = s1.x * w1;
-    im[1] = s1.y * w1;
-    fft128(re, im, tw, 1.0);
-    if (lane >= ry0 && lane < ry1) B[(lane - ry0) * kHalf + k] = make_double2(re[0], im[0]);
-    if (lane + 64 >= ry0 && lane + 64 < ry1)
-      B[(lane + 64 - ry0) * kHalf + k] = make_double2(re[1], im[1]);
+  for (int k = wid; k < H; k += UPH_WAVES) {
+    double re[E], im[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const double w = spec_weight<E>(spec, kind, p, dt9, mi[e], k);
+      const double2 sv = S[k * N + mi[e]];
+      re[e] = sv.x * w;
+      im[e] = sv.y * w;
+    }
+    fftN<E>(re, im, tw, 1.0);
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int m = lane + 64 * e;
+      if (m >= ry0 && m < ry1) B[(m - ry0) * H + k] = make_double2(re[e], im[e]);
+    }
   }
   __syncthreads();
   // rows of the box, two per line: the Hermitian extension of row a plus i
   // times that of row b; the inverse of that line holds row a's real
   // output in its real part and row b's in its imaginary part
-  const double norm = 1.0 / (double(kLine) * double(kLine));
-  const int c0 = brev7(lane), c1 = brev7(lane + 64);
+  const double norm = 1.0 / (double(N) * double(N));
   for (int j = wid; 2 * j < RH; j += UPH_WAVES) {
     const int ia = 2 * j, ib = 2 * j + 1;
     const bool hb = ib < RH;
-    double re[2], im[2];
+    double re[E], im[E];
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
-      const int c = e ? c1 : c0;
-      const bool lo = c < kHalf;
-      const int cc = lo ? c : kLine - c;
-      const double2 a = B[ia * kHalf + cc];
-      const double2 b = hb ? B[ib * kHalf + cc] : make_double2(0.0, 0.0);
+    for (int e = 0; e < E; ++e) {
+      const int c = mi[e];
+      const bool lo = c < H;
+      const int cc = lo ? c : N - c;
+      const double2 a = B[ia * H + cc];
+      const double2 b = hb ? B[ib * H + cc] : make_double2(0.0, 0.0);
       // the DC and Nyquist bins enter with their real parts only, as in a
       // complex-to-real transform (which a pair would otherwise mix)
-      const bool edge = cc == 0 || cc == kLine / 2;
+      const bool edge = cc == 0 || cc == N / 2;
       const double ai = edge ? 0.0 : (lo ? a.y : -a.y);
       const double bi = edge ? 0.0 : (lo ? b.y : -b.y);
       re[e] = a.x - bi;
       im[e] = ai + b.x;
     }
-    fft128(re, im, tw, 1.0);
+    fftN<E>(re, im, tw, 1.0);
 #pragma unroll
-    for (int e = 0; e < 2; ++e) {
+    for (int e = 0; e < E; ++e) {
       const int c = lane + 64 * e;
       if (c < rx0 || c >= rx1) continue;
       const int ra = ry0 + ia, rb = ry0 + ib;
@@ -534,14 +580,17 @@ __device__ void galaxy_psf64(const IslandArgsPre& a, const float* coefs, int T, 
   __syncthreads();
 }
 
-// Extended member of an LDS tier (pad 128): the convolved mixture and its
-// raw tangents over the box the shift reads only, by transforms in LDS,
-// then the shift of the window part into planes 0 (model) and 1 + j for
-// the parameters j in `need`.
-__device__ void galaxy_planes64_lds(const IslandArgsPre& a, const float* coefs, int T, int b,
+// Extended member of a box tier: the convolved mixture and its raw tangents
+// over the box the shift reads only, by wave-held line transforms (line
+// buffers in LDS for pad 128, in the workgroup's scratch for pad 256), then
+// the shift of the window part into planes 0 (model) and 1 + j for the
+// parameters j in `need`.
+template <int E>
+__device__ void galaxy_planes64_box(const IslandArgsPre& a, const float* coefs, int T, int b,
                                     const MemberWindow& w, int need, const double* spec,
                                     const double* taps, const double* bas, double xs, double ys,
-                                    double* ws, double* blk, int side, double2* lds) {
+                                    double* ws, double* blk, int side, double2* lds,
+                                    double2* zw) {
   const int TT = T * T;
   const GalLayout L = galaxy_layout(T);
   galaxy_psf64(a, coefs, T, b, need, bas, xs, ys, ws);
@@ -550,27 +599,27 @@ __device__ void galaxy_planes64_lds(const IslandArgsPre& a, const float* coefs, 
   const double core = tail[0], dcore = tail[1], dndu = tail[11];
   const double* dt9 = tail + 2;
   // raw-plane boxes (RH x RW, row stride RW) at ws + L.G + q T^2; the
-  // spectrum in the (16-byte aligned) spectra region
+  // spectrum in the (16-byte aligned) spectra region; for pad 256 the line
+  // buffer in the next spectra region
   const int RW = w.rx1 - w.rx0;
   double2* S = reinterpret_cast<double2*>(ws + (L.SP + 1) / 2 * 2);
-  double2* A = lds;
-  double2* zw = lds + kLdsRows * kHalf;
+  double2* A = E == 2 ? lds : reinterpret_cast<double2*>(ws + (L.SPx + 1) / 2 * 2);
   double* G = ws + L.G;
-  rfft2_128(ws + L.P, T, S, A, zw);
-  irfft2_box_128(S, spec, 0, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, ws + L.P, core, T, G, A);
+  rfft2_N<E>(ws + L.P, T, S, A, zw);
+  irfft2_box_N<E>(S, spec, 0, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, ws + L.P, core, T, G, A);
   for (int p = 0; p < 3; ++p)
     if (need & (4 << p))
-      irfft2_box_128(S, spec, 1, p, dt9, w.ry0, w.ry1, w.rx0, w.rx1, nullptr, 0.0, T,
-                     G + (size_t)(3 + p) * TT, A);
+      irfft2_box_N<E>(S, spec, 1, p, dt9, w.ry0, w.ry1, w.rx0, w.rx1, nullptr, 0.0, T,
+                      G + (size_t)(3 + p) * TT, A);
   if (need & 32)
-    irfft2_box_128(S, spec, 2, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, ws + L.P, dcore, T,
-                   G + (size_t)6 * TT, A);
+    irfft2_box_N<E>(S, spec, 2, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, ws + L.P, dcore, T,
+                    G + (size_t)6 * TT, A);
   for (int j = 0; j < 2; ++j) {
     if (!(need & (1 << j))) continue;
     const double* Q = ws + (j == 0 ? L.Px : L.Py);
-    rfft2_128(Q, T, S, A, zw);
-    irfft2_box_128(S, spec, 0, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, Q, core, T,
-                   G + (size_t)(1 + j) * TT, A);
+    rfft2_N<E>(Q, T, S, A, zw);
+    irfft2_box_N<E>(S, spec, 0, 0, dt9, w.ry0, w.ry1, w.rx0, w.rx1, Q, core, T,
+                    G + (size_t)(1 + j) * TT, A);
   }
   shift_planes64(G, TT, RW, w.ry0, w.rx0, T, w, need, taps, dndu, ws + L.Hh, blk, side);
 }
@@ -591,12 +640,12 @@ __device__ void galaxy_planes64_block(const IslandArgsPre& a, const float* coefs
 }
 
 // One (member, band) item.  kBig = false takes stars and the extended
-// members of LDS tiers, kBig = true the extended members of larger tiers;
-// an item of the other kind is left alone.  Block-uniform; ends after a
-// barrier.
+// members of the LDS tier, kBig = true the extended members of larger
+// tiers; an item of the other kind is left alone.  Block-uniform; ends
+// after a barrier.
 template <bool kBig>
 __device__ void plane_item(const IslandArgsPre& a, long long item, double* taps, double* bas,
-                           double2* lds, double* ws) {
+                           double2* lds, double2* zw, double* ws) {
   const int tid = threadIdx.x;
   const bool grads = split_grads(a.mode);
   const int g = (int)(item / a.nb), b = (int)(item % a.nb);
@@ -631,12 +680,17 @@ __device__ void plane_item(const IslandArgsPre& a, long long item, double* taps,
   __syncthreads();
   const double* spec = star ? nullptr : a.fspec64 + a.pre_spec[g];
   if constexpr (kBig) {
-    galaxy_planes64_block(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side);
+    if (box_tier(T))
+      galaxy_planes64_box<4>(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side,
+                             nullptr, zw);
+    else
+      galaxy_planes64_block(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side);
   } else {
     if (star)
       star_planes64(a, coefs, T, b, w, grads, taps, bas, xs, ys, ws, blk, side);
     else
-      galaxy_planes64_lds(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side, lds);
+      galaxy_planes64_box<2>(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side,
+                             lds, zw);
   }
 }
 
@@ -647,7 +701,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_kernel(IslandArgsPre a
   __shared__ double bas[63];
   __shared__ long long s_item;
   // line buffer (kLdsRows x 65 complex) and one 128-complex line per wave
-  __shared__ double2 lds[kLdsRows * kHalf + UPH_WAVES * kLine];
+  __shared__ double2 lds[kLdsRows * LineE<2>::H + UPH_WAVES * LineE<2>::N];
   double* ws = reinterpret_cast<double*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
   const long long n_items = (long long)a.n_pre * a.nb;
   for (;;) {
@@ -656,19 +710,26 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_kernel(IslandArgsPre a
     __syncthreads();
     const long long item = s_item;
     if (item >= n_items) break;
-    plane_item<false>(a, item, taps, bas, lds, ws);
+    plane_item<false>(a, item, taps, bas, lds, lds + kLdsRows * LineE<2>::H, ws);
   }
 }
 
-// The extended members of larger stamp tiers: items strided over the grid.
+// The extended members of larger stamp tiers, from their own work queue
+// (a.big_counter; only extended-member items are visited).
 __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_big_kernel(IslandArgsPre a) {
   __shared__ double taps[28];
   __shared__ double bas[63];
+  __shared__ long long s_item;
+  __shared__ double2 zw[UPH_WAVES * LineE<4>::N];  // one 256-complex line per wave
   double* ws = reinterpret_cast<double*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
   const long long n_items = (long long)a.n_pre_ext * a.nb;
-  for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
-    __syncthreads();  // the previous item's readers of taps and bas are done
-    plane_item<true>(a, item, taps, bas, nullptr, ws);
+  for (;;) {
+    __syncthreads();  // the previous item's readers of s_item, taps and bas are done
+    if (threadIdx.x == 0) s_item = atomicAdd(a.pre_counter + 1, 1);
+    __syncthreads();
+    const long long item = s_item;
+    if (item >= n_items) break;
+    plane_item<true>(a, item, taps, bas, nullptr, zw, ws);
   }
 }
 

@@ -92,7 +92,8 @@ struct IslandArgsPre : IslandArgs {
   double* gdt;                 // (n_pre, 9) ellipse-term derivatives of extended members
   float* gtap;                 // (n_pre, 28) Lanczos taps and tap derivatives, x then y
   size_t star_stride;          // per-workgroup scratch bytes of the star launch
-  int* pre_counter;            // work queue of the planes kernel
+  int* pre_counter;            // work queue of the planes kernel (float64: the next counter
+                               // is the queue of the larger-tier planes kernel)
   // post-fit pass at the converged parameters (post != 0):
   int post;
   const int64_t* post_off;     // (n_isl) first double of the island's block in post_buf:
