@@ -300,7 +300,8 @@ def roofline_of(ks, config):
     per_launch_bytes = isl["bytes"] / max(isl["launches"], 1)
     achieved = per_launch_bytes / avg_s / 1e9 if avg_s > 0 else 0.0
     tflops = isl["flops"] / max(isl["ms"] / 1e3, 1e-12) / 1e12
-    peak = FP64_PEAK_TFLOPS if kern.endswith("_fp64") else FP32_PEAK_TFLOPS
+    peak = (FP64_PEAK_TFLOPS if kern.endswith("_fp64") or kern.startswith("pre64")
+            else FP32_PEAK_TFLOPS)
     compute = tflops / peak > achieved / HBM_PEAK_GBS
     traffic = measured_traffic(config, kern)
     if compute:

@@ -272,8 +272,9 @@ struct LineE {
 
 template <int E>
 struct TwN {
-  double c[6], s[6];                    // cross-lane stage h = 2^q: pi (lane & (h - 1)) / h
-  double ic[LineE<E>::LOGE][E], is[LineE<E>::LOGE][E];  // in-lane stage t, element e
+  double c[6], s[6];    // cross-lane stage h = 2^q: pi (lane & (h - 1)) / h
+  double ic[E - 1], is[E - 1];  // in-lane stage of span 64 he: pi (lane + 64 i) / (64 he)
+                                // at [he - 1 + i], i < he (the he distinct values)
 };
 
 template <int E>
@@ -288,10 +289,8 @@ __device__ inline void twN(TwN<E>& t) {
   for (int q = 0; q < LineE<E>::LOGE; ++q) {
     const int he = 1 << q;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int j = lane + 64 * (e & (he - 1));
-      sincospi(double(j) / double(64 * he), &t.is[q][e], &t.ic[q][e]);
-    }
+    for (int i = 0; i < he; ++i)
+      sincospi(double(lane + 64 * i) / double(64 * he), &t.is[he - 1 + i], &t.ic[he - 1 + i]);
   }
 }
 
@@ -332,7 +331,8 @@ __device__ __forceinline__ void fftN(double (&re)[E], double (&im)[E], const TwN
     for (int e = 0; e < E; ++e) {
       if (e & he) continue;
       const int f = e + he;
-      const double wr = t.ic[q][e], wi = sign * t.is[q][e];
+      const int ti = he - 1 + (e & (he - 1));
+      const double wr = t.ic[ti], wi = sign * t.is[ti];
       const double br = wr * re[f] - wi * im[f], bi = wr * im[f] + wi * re[f];
       const double ar = re[e], ai = im[e];
       re[e] = ar + br;
@@ -639,11 +639,13 @@ __device__ void galaxy_planes64_block(const IslandArgsPre& a, const float* coefs
   shift_planes64(ws + L.G, TT, T, 0, 0, T, w, need, taps, tail[11], ws + L.Hh, blk, side);
 }
 
-// One (member, band) item.  kBig = false takes stars and the extended
-// members of the LDS tier, kBig = true the extended members of larger
-// tiers; an item of the other kind is left alone.  Block-uniform; ends
-// after a barrier.
-template <bool kBig>
+// Item kinds of the planes kernels: stars and the extended members of the
+// LDS tier; extended members of tier 111 (pad 256); of larger tiers.
+constexpr int kItemLds = 0, kItemPad256 = 1, kItemBlock = 2;
+
+// One (member, band) item of kind kKind; an item of another kind is left
+// alone.  Block-uniform; ends after a barrier.
+template <int kKind>
 __device__ void plane_item(const IslandArgsPre& a, long long item, double* taps, double* bas,
                            double2* lds, double2* zw, double* ws) {
   const int tid = threadIdx.x;
@@ -655,7 +657,8 @@ __device__ void plane_item(const IslandArgsPre& a, long long item, double* taps,
   const int B = sp[1], T = a.tier_T[sp[2]];
   const size_t slot = (size_t)a.mem_off[isl] + a.pre_slot[g];
   const bool star = a.stype[slot] == 0;
-  if (kBig != (!star && !lds_tier(T))) return;
+  const int kind = (star || lds_tier(T)) ? kItemLds : (box_tier(T) ? kItemPad256 : kItemBlock);
+  if (kind != kKind) return;
   const float* coefs = a.tier_coefs[sp[2]];
   const float* p6 = pass_params(a) + slot * 6;
   MemberWindow w;
@@ -679,12 +682,11 @@ __device__ void plane_item(const IslandArgsPre& a, long long item, double* taps,
   }
   __syncthreads();
   const double* spec = star ? nullptr : a.fspec64 + a.pre_spec[g];
-  if constexpr (kBig) {
-    if (box_tier(T))
-      galaxy_planes64_box<4>(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side,
-                             nullptr, zw);
-    else
-      galaxy_planes64_block(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side);
+  if constexpr (kKind == kItemPad256) {
+    galaxy_planes64_box<4>(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side,
+                           nullptr, zw);
+  } else if constexpr (kKind == kItemBlock) {
+    galaxy_planes64_block(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side);
   } else {
     if (star)
       star_planes64(a, coefs, T, b, w, grads, taps, bas, xs, ys, ws, blk, side);
@@ -710,12 +712,13 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_kernel(IslandArgsPre a
     __syncthreads();
     const long long item = s_item;
     if (item >= n_items) break;
-    plane_item<false>(a, item, taps, bas, lds, lds + kLdsRows * LineE<2>::H, ws);
+    plane_item<kItemLds>(a, item, taps, bas, lds, lds + kLdsRows * LineE<2>::H, ws);
   }
 }
 
-// The extended members of larger stamp tiers, from their own work queue
-// (a.big_counter; only extended-member items are visited).
+// The extended members of tier 111 (pad 256), from their own work queue
+// (the counter after the planes queue; only extended-member items are
+// visited).
 __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_big_kernel(IslandArgsPre a) {
   __shared__ double taps[28];
   __shared__ double bas[63];
@@ -729,7 +732,20 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_big_kernel(IslandArgsP
     __syncthreads();
     const long long item = s_item;
     if (item >= n_items) break;
-    plane_item<true>(a, item, taps, bas, nullptr, zw, ws);
+    plane_item<kItemPad256>(a, item, taps, bas, nullptr, zw, ws);
+  }
+}
+
+// The extended members of the largest tier (pad 512, galaxy_raw_spec's
+// block transforms): items strided over the grid.
+__global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_block_kernel(IslandArgsPre a) {
+  __shared__ double taps[28];
+  __shared__ double bas[63];
+  double* ws = reinterpret_cast<double*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
+  const long long n_items = (long long)a.n_pre_ext * a.nb;
+  for (long long item = blockIdx.x; item < n_items; item += gridDim.x) {
+    __syncthreads();  // the previous item's readers of taps and bas are done
+    plane_item<kItemBlock>(a, item, taps, bas, nullptr, nullptr, ws);
   }
 }
 
@@ -744,12 +760,18 @@ void prerender64_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, 
   const long long items = (long long)p.n_pre * p.nb;
   const int g = (int)(items < nblk ? items : nblk);
   if (g > 0) hipLaunchKernelGGL(pre64_planes_kernel, dim3(g), dim3(UPH_BLOCK), 0, s, p);
-  bool big = false;
-  for (int t = 0; t < 3; ++t) big = big || (p.tier_T[t] > 0 && !lds_tier(p.tier_T[t]));
+  bool mid = false, large = false;
+  for (int t = 0; t < 3; ++t) {
+    const int T = p.tier_T[t];
+    mid = mid || (T > 0 && !lds_tier(T) && box_tier(T));
+    large = large || (T > 0 && !box_tier(T));
+  }
   const long long items_ext = (long long)p.n_pre_ext * p.nb;
   const int gb = (int)(items_ext < nblk ? items_ext : nblk);
-  if (big && gb > 0)
+  if (mid && gb > 0)
     hipLaunchKernelGGL(pre64_planes_big_kernel, dim3(gb), dim3(UPH_BLOCK), 0, s, p);
+  if (large && gb > 0)
+    hipLaunchKernelGGL(pre64_planes_block_kernel, dim3(gb), dim3(UPH_BLOCK), 0, s, p);
   if (ev_planes) (void)hipEventRecord(ev_planes, s);
 }
 

@@ -686,8 +686,9 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 tg = gh["T"].astype(np.float64)
                 planes = gh["grad_planes"] if grad else np.ones(len(g_isl))
                 nspec = 3.0 if grad else 1.0
-                spec_b = gh["spec_floats"] / 3.0 * nspec * 4.0
-                sbytes = float(np.sum(spec_b[ex_]) + 28 * 4.0 * gsel.sum())
+                wb = 8.0 if precision == "fp64" else 4.0  # bytes per plane / spectra element
+                spec_b = gh["spec_floats"] / 3.0 * nspec * wb
+                sbytes = float(np.sum(spec_b[ex_]) + (0.0 if wb == 8.0 else 28 * 4.0 * gsel.sum()))
                 hh = pad // 2 + 1
                 npoly = 3.0 if grad else 1.0
                 nfwd = np.where(grad & (gh["grad_planes"] > 1), 3.0, 1.0) if grad else 1.0
@@ -697,21 +698,32 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 t_fl = 2.0 * nt * gh["reg"] * npoly + 28.0 * gh["win"] * planes
                 tiers_of = lambda m: sum(coef_tbl[int(t)] for t in set(gh["T"][m].tolist()))
                 ebytes = float(tiers_of(ex_) + np.sum(spec_b[ex_])
-                               + nb * np.sum((gh["win"] * planes * 4.0)[ex_]))
-                tbytes = float(tiers_of(st_) + nb * np.sum((gh["win"] * planes * 4.0)[st_]))
-                timer.record("prerender_spectra_kernel", e5[0], e5[1],
-                             dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
-                                  n_pre=int(gsel.sum())))
-                if gh["n_ext"]:
-                    timer.record("prerender_planes_kernel", e5[1], e5[2],
-                                 dict(kind="island_eval", phase=kind, bytes=ebytes,
-                                      flops=float(nb * np.sum(e_fl[ex_])),
+                               + nb * np.sum((gh["win"] * planes * wb)[ex_]))
+                tbytes = float(tiers_of(st_) + nb * np.sum((gh["win"] * planes * wb)[st_]))
+                if wb == 8.0:
+                    # float64: the spectra launch, then the planes launches
+                    # (stars and tier 55 in LDS, tier 111, larger tiers)
+                    timer.record("pre64_spectra_kernel", e5[0], e5[1],
+                                 dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
                                       n_pre=int(ex_.sum())))
-                if gh["n_ext"] < g_isl.size:
-                    timer.record("prerender_stars_kernel", e5[2], e5[3],
-                                 dict(kind="island_eval", phase=kind, bytes=tbytes,
-                                      flops=float(nb * np.sum(t_fl[st_])),
-                                      n_pre=int(st_.sum())))
+                    timer.record("pre64_planes_kernels", e5[1], e5[2],
+                                 dict(kind="island_eval", phase=kind, bytes=ebytes + tbytes,
+                                      flops=float(nb * (np.sum(e_fl[ex_]) + np.sum(t_fl[st_]))),
+                                      n_pre=int(gsel.sum())))
+                else:
+                    timer.record("prerender_spectra_kernel", e5[0], e5[1],
+                                 dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
+                                      n_pre=int(gsel.sum())))
+                    if gh["n_ext"]:
+                        timer.record("prerender_planes_kernel", e5[1], e5[2],
+                                     dict(kind="island_eval", phase=kind, bytes=ebytes,
+                                          flops=float(nb * np.sum(e_fl[ex_])),
+                                          n_pre=int(ex_.sum())))
+                    if gh["n_ext"] < g_isl.size:
+                        timer.record("prerender_stars_kernel", e5[2], e5[3],
+                                     dict(kind="island_eval", phase=kind, bytes=tbytes,
+                                          flops=float(nb * np.sum(t_fl[st_])),
+                                          n_pre=int(st_.sum())))
             ncol = 1 + nr + (nlive if grad else 0)
             gram_fl = nb * npix * ncol * (ncol + 1)
             ev_fl = nb * npix * (ncol * (ncol + 1) + 4.0 * nr + (2.0 * (nr + nlive) if grad else 0))
