@@ -750,7 +750,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_block_kernel(IslandArg
 }
 
 void prerender64_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hipStream_t s,
-                      hipEvent_t ev_spec, hipEvent_t ev_planes) {
+                      hipEvent_t ev_spec, hipEvent_t ev_lds) {
   IslandArgsPre p = a;
   p.mode = mode;
   p.pre_counter = counter;
@@ -760,6 +760,7 @@ void prerender64_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, 
   const long long items = (long long)p.n_pre * p.nb;
   const int g = (int)(items < nblk ? items : nblk);
   if (g > 0) hipLaunchKernelGGL(pre64_planes_kernel, dim3(g), dim3(UPH_BLOCK), 0, s, p);
+  if (ev_lds) (void)hipEventRecord(ev_lds, s);
   bool mid = false, large = false;
   for (int t = 0; t < 3; ++t) {
     const int T = p.tier_T[t];
@@ -772,7 +773,6 @@ void prerender64_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, 
     hipLaunchKernelGGL(pre64_planes_big_kernel, dim3(gb), dim3(UPH_BLOCK), 0, s, p);
   if (large && gb > 0)
     hipLaunchKernelGGL(pre64_planes_block_kernel, dim3(gb), dim3(UPH_BLOCK), 0, s, p);
-  if (ev_planes) (void)hipEventRecord(ev_planes, s);
 }
 
 }  // namespace uph

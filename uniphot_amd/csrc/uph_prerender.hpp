@@ -38,10 +38,13 @@ bool launch_galaxy_stamps(int n, int T, int nt, const float* coefs, const float*
 
 // Float64 member pre-render (prerender64.hip): per-workgroup scratch bytes
 // for stamp tier T, doubles of one extended member's spectra block, and
-// the spectra + planes launches of one evaluation pass (pbuf64, fspec64).
+// the spectra + planes launches of one evaluation pass (pbuf64, fspec64);
+// `ev_spec` and `ev_lds`, when given, are recorded after the spectra launch
+// and after the stars / tier-55 planes launch (the larger-tier launches
+// follow it).
 size_t prerender64_scratch(int T);
 size_t prerender64_spectra_doubles(int T);
 void prerender64_pass(const IslandArgsPre& a, int mode, int* counter, int nblk, hipStream_t s,
-                      hipEvent_t ev_spec, hipEvent_t ev_planes);
+                      hipEvent_t ev_spec, hipEvent_t ev_lds);
 
 }  // namespace uph

@@ -701,15 +701,26 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                                + nb * np.sum((gh["win"] * planes * wb)[ex_]))
                 tbytes = float(tiers_of(st_) + nb * np.sum((gh["win"] * planes * wb)[st_]))
                 if wb == 8.0:
-                    # float64: the spectra launch, then the planes launches
-                    # (stars and tier 55 in LDS, tier 111, larger tiers)
+                    # float64: the spectra launch, the planes launch of the
+                    # stars and tier-55 members (transforms in LDS), then
+                    # the launches of the larger tiers
+                    small = ex_ & (gh["T"] <= 64)
+                    large = ex_ & (gh["T"] > 64)
+                    bx = lambda m: float(tiers_of(m) + np.sum(spec_b[m])
+                                         + nb * np.sum((gh["win"] * planes * wb)[m]))
                     timer.record("pre64_spectra_kernel", e5[0], e5[1],
                                  dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
                                       n_pre=int(ex_.sum())))
-                    timer.record("pre64_planes_kernels", e5[1], e5[2],
-                                 dict(kind="island_eval", phase=kind, bytes=ebytes + tbytes,
-                                      flops=float(nb * (np.sum(e_fl[ex_]) + np.sum(t_fl[st_]))),
-                                      n_pre=int(gsel.sum())))
+                    timer.record("pre64_planes_kernel", e5[1], e5[2],
+                                 dict(kind="island_eval", phase=kind,
+                                      bytes=bx(small) + tbytes,
+                                      flops=float(nb * (np.sum(e_fl[small]) + np.sum(t_fl[st_]))),
+                                      n_pre=int(small.sum() + st_.sum())))
+                    if large.any():
+                        timer.record("pre64_planes_big_kernel", e5[2], e5[3],
+                                     dict(kind="island_eval", phase=kind, bytes=bx(large),
+                                          flops=float(nb * np.sum(e_fl[large])),
+                                          n_pre=int(large.sum())))
                 else:
                     timer.record("prerender_spectra_kernel", e5[0], e5[1],
                                  dict(kind="island_eval", phase=kind, bytes=sbytes, flops=0.0,
