@@ -463,6 +463,28 @@ int uph_band_solve_noalloc(int n_prob, int T, int bg_order, int k_max, int max_s
                            float* fluxes, float* flux_var, float* bg, float* bg_var,
                            void* stream);
 
+/*
+ * PSF diagnostics of the band flags at fitted positions (replaces
+ * psf_lobe_zone and psf_uncovered_fraction, tract7dt/fit/grid_gpu/
+ * band_flags.py:115-196, both of which evaluate evaluate_psf_at_batch_np,
+ * psf_polynomial.py:229-270).  For source s and band b: the band's PSF
+ * model stamp at (x[s], y[s]) (size[b] x size[b] native pixels, centred in
+ * the planar coefs (n_bands, n_terms, T, T) f32; geo (n_bands, 4) f64 =
+ * x_origin, x_scale, y_origin, y_scale; basis in f64 rounded to f32),
+ * peak[b*n+s] = its maximum and vmin[b*n+s] its minimum (NaN for a
+ * non-finite position); where has_nodata[b] is set, uncovered[b*n+s] = the
+ * fraction of the stamp's positive part on pixels of nodata (n_bands, H, W)
+ * u8 (the stamp centred on the pixel nearest (x, y), round half to even;
+ * pixels beyond the image count as no data): 0 for a non-finite position
+ * or a stamp touching no such pixel, 1 for a position off the image or a
+ * stamp with no positive part; 0 where has_nodata[b] is clear.  nodata and
+ * has_nodata are both null or both given.
+ */
+int uph_band_psf_flags(int n, int n_bands, int n_terms, int T, const int32_t* size,
+                       const float* coefs, const double* geo, const double* x, const double* y,
+                       int H, int W, const uint8_t* nodata, const uint8_t* has_nodata,
+                       float* peak, float* vmin, double* uncovered, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,9 +14,12 @@ shape-marginalised error columns of ``error_columns.py`` (:255-341):
     metrics and window fluxes (post-fit pass);
   * FLUXERR_MARG_{b}_fit = sqrt(FLUXERR^2 + increment) and
     FLUX_MARG_FRAC_{b}_fit = increment / FLUXERR^2 (0 for stars);
-  * flag_bound_stalled, flag_shape_bound, flag_slow_converge.
+  * flag_bound_stalled, flag_shape_bound, flag_slow_converge;
+  * with ``band_psfs`` given, the band-level contract
+    (:mod:`uniphot_amd.band_flags`, ``pipeline.py:3084-3130``): no-data
+    cells blanked, ``FLAGS_{b}_fit`` and ``n_bands_usable``.
 The error-model totals (confusion, correlated noise, FLUXERR_TOT and the
-*_err_tot family) and the band flags are not part of this build.
+*_err_tot family) are not part of this build.
 """
 from __future__ import annotations
 
@@ -49,12 +52,17 @@ def write_fit_columns(catalog, fit, bandnames, *, x_init, y_init, source_type,
                       pos_max_shift_pix: float = 5.0, psf_stamp_size: int = 55, wcs=None,
                       param_errors: bool = True, source_metrics: bool = True,
                       marginal_flux_err: bool = True, max_steps_main: int = 8,
-                      max_steps: int = 30):
+                      max_steps: int = 30, band_psfs=None, band_nodata=None,
+                      band_saturated=None, device=None):
     """A copy of ``catalog`` (a DataFrame, rows aligned with the fit's
     sources) with the fit's columns added.  ``fit`` is a FitResult of
     :func:`uniphot_amd.pipeline.fit_field`; ``x_init``, ``y_init`` the
     catalogue seeds in the fit frame; ``wcs`` (optional) converts fitted
-    pixels with ``all_pix2world(x, y, 0)``."""
+    pixels with ``all_pix2world(x, y, 0)``.  ``band_psfs`` (one
+    PsfPolynomial or None per band) turns on the band flags, with the
+    optional per-band no-data masks ``band_nodata`` and saturation hits
+    ``band_saturated`` ({band: (n,) bool}); their counts are in
+    ``out.attrs["band_flags"]``."""
     out = catalog.copy()
     n = len(out)
     st = np.asarray(source_type, np.int32)
@@ -159,4 +167,9 @@ def write_fit_columns(catalog, fit, bandnames, *, x_init, y_init, source_type,
     has_fit = np.isfinite(out["x_pix_white_fit"].to_numpy(float))
     out["flag_slow_converge"] = slow_converge_flag(out["opt_niters"].to_numpy(), has_fit,
                                                    max_steps_main, max_steps)
+    if band_psfs is not None:
+        from .band_flags import apply_band_flags
+        out.attrs["band_flags"] = apply_band_flags(out, list(bandnames), list(band_psfs),
+                                                   nodata=band_nodata, saturated=band_saturated,
+                                                   device=device)
     return out
