@@ -185,3 +185,42 @@ def test_band_flags_argument_errors():
                                         None))
     assert "null argument" in str(ei.value)
     assert L.uph_band_psf_flags(0, 1, 1, 21, *([None] * 5), 10, 10, *([None] * 5), None) == 0
+
+
+def test_reliability_contract():
+    from uniphot_amd.band_flags import compute_reliability_flag
+    out = pd.DataFrame({
+        "x_pix_white_fit": [1.0, np.nan, 3.0, 4.0, 5.0, 6.0, 7.0, 8.0],
+        "excluded_any": [False, False, False, False, False, False, False, True],
+        "opt_hit_max_iters": [False, False, True, False, False, False, False, False],
+        "opt_converged": [True, False, False, False, True, True, True, True],
+        "flag_bound_stalled": [False] * 8,
+        "island_stalled": [False, False, False, False, True, True, False, False],
+        "island_plateau": [False, False, False, False, True, False, False, False],
+        "island_diverged": [False, False, False, False, False, True, False, False],
+        "island_giant": [False, False, False, False, False, False, True, False],
+        "n_bands_usable": [2, 0, 2, 0, 2, 2, 2, 0],
+        "shape_frozen": [False, False, False, True, False, False, False, False],
+    })
+    a = out.copy()
+    compute_reliability_flag(a)
+    assert a["fit_is_reliable"].tolist() == [True, False, False, False, False, False, False,
+                                             False]
+    assert a["fit_quality_issues"].tolist() == [
+        "", "no_fit_result", "hit_max_iters", "no_usable_band,not_converged,shape_frozen",
+        "island_stalled", "island_stalled,island_diverged", "giant_frozen_geometry",
+        "excluded"]
+    b = out.copy()
+    compute_reliability_flag(b, island_plateau_mode="converged", island_giant_mode="warn")
+    assert b["fit_is_reliable"].tolist() == [True, False, False, False, True, False, True, False]
+    assert b["fit_quality_issues"].tolist()[4] == "island_plateau"
+    assert b["fit_quality_issues"].tolist()[6] == "giant_frozen_geometry"
+    c = out.copy()
+    compute_reliability_flag(c, island_stalled_mode="ignore", island_giant_mode="ignore")
+    assert c["fit_is_reliable"].tolist()[4:7] == [True, True, True]
+    assert c["fit_quality_issues"].tolist()[5] == "island_diverged"
+    with pytest.raises(ValueError):
+        compute_reliability_flag(out.copy(), high_rchisq_mode="loud")
+    empty = pd.DataFrame({"x_pix_white_fit": []})
+    compute_reliability_flag(empty)
+    assert list(empty.columns) == ["x_pix_white_fit", "fit_is_reliable", "fit_quality_issues"]

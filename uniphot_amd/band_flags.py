@@ -214,3 +214,118 @@ def apply_band_flags(out, bandnames, psfs, *, nodata=None, saturated=None, devic
                 part[band] = unc[j] > coverage_max
     flags = compose_band_flags(out, bandnames, psf_zone=zone, saturated=saturated, partial=part)
     return {"blanked": blanked, "flags": flags}
+
+
+def _bool_col(out, col, default=False):
+    if col not in out.columns:
+        return None
+    return out[col].astype(object).where(out[col].notna(), default).astype(bool).to_numpy()
+
+
+def compute_reliability_flag(out, *, island_stalled_mode: str = "fail",
+                             island_giant_mode: str = "fail", high_rchisq_mode: str = "warn",
+                             blend_unresolved_mode: str = "fail",
+                             island_plateau_mode: str = "stalled") -> None:
+    """``fit_is_reliable`` and ``fit_quality_issues`` in place, the shared
+    reliability contract of ``tract7dt/merge.py:16-344`` for a catalogue of
+    this build's columns.
+
+    Hard reasons (unreliable): excluded, no_fit_result, hit_max_iters,
+    bound_stalled, bright_mask, island_stalled, blend_unresolved,
+    giant_frozen_geometry, no_usable_band (``n_bands_usable`` = 0 on a
+    fitted source), high_rchisq (in "fail" mode).  Soft tags: island_plateau
+    (plateau-certified stalls in "converged" mode), island_diverged,
+    blend_suspect, blend_resolved, not_converged, all_psf_fallback,
+    shape_frozen, and the mode-selectable reasons in "warn" mode.  Tags are
+    joined with commas, hard reasons first, in the order listed.  A missing
+    column is no signal."""
+    import pandas as pd
+    for name, v, ok in (("island_stalled_mode", island_stalled_mode, ("fail", "warn", "ignore")),
+                        ("island_plateau_mode", island_plateau_mode, ("stalled", "converged")),
+                        ("blend_unresolved_mode", blend_unresolved_mode,
+                         ("fail", "warn", "ignore")),
+                        ("island_giant_mode", island_giant_mode, ("fail", "warn", "ignore")),
+                        ("high_rchisq_mode", high_rchisq_mode, ("fail", "warn", "ignore"))):
+        if v not in ok:
+            raise ValueError(f"{name} must be one of {'|'.join(ok)}; got {v!r}")
+    n = len(out)
+    if n == 0:
+        out["fit_is_reliable"] = pd.Series(dtype=bool)
+        out["fit_quality_issues"] = pd.Series(dtype=str)
+        return
+    zero = np.zeros(n, bool)
+    hard, soft = [], []
+
+    def moded(tag, col, mode, fail="fail"):
+        m = _bool_col(out, col)
+        if m is None or mode == "ignore":
+            return
+        (hard if mode == fail else soft).append((tag, m))
+
+    excluded = _bool_col(out, "excluded_any")
+    excluded = zero if excluded is None else excluded
+    hard.append(("excluded", excluded))
+    if "x_pix_white_fit" in out.columns:
+        no_fit = out["x_pix_white_fit"].isna().to_numpy(bool)
+    else:
+        per_band = [c for c in out.columns
+                    if c.startswith("x_pix_white_") and c.endswith("_fit") and c != "x_pix_white_fit"]
+        no_fit = (out[per_band].isna().all(axis=1).to_numpy(bool) if per_band
+                  else np.ones(n, bool))
+    no_fit &= ~excluded
+    hard.append(("no_fit_result", no_fit))
+    hmi = _bool_col(out, "opt_hit_max_iters")
+    if hmi is None:
+        cols = [c for c in out.columns if c.startswith("opt_hit_max_iters_")]
+        hmi = (out[cols].fillna(False).astype(bool).any(axis=1).to_numpy() if cols else zero)
+    hard.append(("hit_max_iters", hmi))
+    bst = _bool_col(out, "flag_bound_stalled")
+    if bst is None:
+        cols = [c for c in out.columns
+                if c.startswith("flag_bound_stalled_") and "which_parameter" not in c]
+        bst = (out[cols].fillna(False).astype(bool).any(axis=1).to_numpy() if cols else zero)
+    hard.append(("bound_stalled", bst))
+    bm = _bool_col(out, "affected_by_bright_mask")
+    if bm is not None:
+        hard.append(("bright_mask", bm))
+    stalled = _bool_col(out, "island_stalled")
+    if island_stalled_mode != "ignore" and stalled is not None:
+        plat = _bool_col(out, "island_plateau")
+        if island_plateau_mode == "converged" and plat is not None and plat.any():
+            soft.append(("island_plateau", plat & stalled))
+            stalled = stalled & ~plat
+        if stalled.any():
+            (soft if island_stalled_mode == "warn" else hard).append(("island_stalled", stalled))
+    for tag, col in (("island_diverged", "island_diverged"), ("blend_suspect", "flag_blend_suspect"),
+                     ("blend_resolved", "blend_resolved")):
+        m = _bool_col(out, col)
+        if m is not None and m.any():
+            soft.append((tag, m))
+    moded("blend_unresolved", "blend_unresolved", blend_unresolved_mode)
+    moded("giant_frozen_geometry", "island_giant", island_giant_mode)
+    if "n_bands_usable" in out.columns:
+        nbu = pd.to_numeric(out["n_bands_usable"], errors="coerce").fillna(0).to_numpy(float)
+        hard.append(("no_usable_band", (nbu <= 0) & ~excluded & ~no_fit))
+    moded("high_rchisq", "flag_high_rchisq", high_rchisq_mode)
+    conv = _bool_col(out, "opt_converged", default=True)
+    if conv is None:
+        cols = [c for c in out.columns if c.startswith("opt_converged_")]
+        conv = (out[cols].fillna(True).astype(bool).all(axis=1).to_numpy() if cols
+                else np.ones(n, bool))
+    nc = ~conv & ~hmi & ~excluded & ~no_fit
+    if nc.any():
+        soft.append(("not_converged", nc))
+    if "psf_fallback_band_count" in out.columns and "psf_used_epsf_band_count" in out.columns:
+        fb = pd.to_numeric(out["psf_fallback_band_count"], errors="coerce").fillna(0).to_numpy()
+        used = pd.to_numeric(out["psf_used_epsf_band_count"], errors="coerce").fillna(0).to_numpy()
+        allfb = (fb > 0) & (used == 0)
+        if allfb.any():
+            soft.append(("all_psf_fallback", allfb))
+    shp = _bool_col(out, "shape_frozen")
+    if shp is not None and shp.any():
+        soft.append(("shape_frozen", shp))
+    items = hard + soft
+    mat = np.column_stack([m for _, m in items])
+    out["fit_is_reliable"] = ~mat[:, :len(hard)].any(axis=1)
+    tags = [t for t, _ in items]
+    out["fit_quality_issues"] = [",".join(t for t, on in zip(tags, row) if on) for row in mat]

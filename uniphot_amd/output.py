@@ -17,7 +17,8 @@ shape-marginalised error columns of ``error_columns.py`` (:255-341):
   * flag_bound_stalled, flag_shape_bound, flag_slow_converge;
   * with ``band_psfs`` given, the band-level contract
     (:mod:`uniphot_amd.band_flags`, ``pipeline.py:3084-3130``): no-data
-    cells blanked, ``FLAGS_{b}_fit`` and ``n_bands_usable``.
+    cells blanked, ``FLAGS_{b}_fit`` and ``n_bands_usable``;
+  * ``fit_is_reliable`` / ``fit_quality_issues`` (``merge.py:16-344``).
 The error-model totals (confusion, correlated noise, FLUXERR_TOT and the
 *_err_tot family) are not part of this build.
 """
@@ -53,7 +54,7 @@ def write_fit_columns(catalog, fit, bandnames, *, x_init, y_init, source_type,
                       param_errors: bool = True, source_metrics: bool = True,
                       marginal_flux_err: bool = True, max_steps_main: int = 8,
                       max_steps: int = 30, band_psfs=None, band_nodata=None,
-                      band_saturated=None, device=None):
+                      band_saturated=None, device=None, reliability: bool = True):
     """A copy of ``catalog`` (a DataFrame, rows aligned with the fit's
     sources) with the fit's columns added.  ``fit`` is a FitResult of
     :func:`uniphot_amd.pipeline.fit_field`; ``x_init``, ``y_init`` the
@@ -62,7 +63,8 @@ def write_fit_columns(catalog, fit, bandnames, *, x_init, y_init, source_type,
     PsfPolynomial or None per band) turns on the band flags, with the
     optional per-band no-data masks ``band_nodata`` and saturation hits
     ``band_saturated`` ({band: (n,) bool}); their counts are in
-    ``out.attrs["band_flags"]``."""
+    ``out.attrs["band_flags"]``.  ``reliability`` adds ``fit_is_reliable``
+    and ``fit_quality_issues`` (merge.py:16-344, default modes)."""
     out = catalog.copy()
     n = len(out)
     st = np.asarray(source_type, np.int32)
@@ -172,4 +174,7 @@ def write_fit_columns(catalog, fit, bandnames, *, x_init, y_init, source_type,
         out.attrs["band_flags"] = apply_band_flags(out, list(bandnames), list(band_psfs),
                                                    nodata=band_nodata, saturated=band_saturated,
                                                    device=device)
+    if reliability:
+        from .band_flags import compute_reliability_flag
+        compute_reliability_flag(out)
     return out
