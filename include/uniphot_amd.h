@@ -333,6 +333,25 @@ int uph_render_stamps(int n, int T, int n_terms, const float* psf_coefs, const f
                       void* scratch, size_t scratch_bytes, void* stream);
 
 /*
+ * Direct render backend (replaces render_source_stamp_at with
+ * render_backend="direct", source_model.py:218-256, :298-303): STAR = PSF
+ * polynomial at (x, y); EXP (stype 1) = the pixel-centre 6-Gaussian
+ * exponential profile (unit sum) convolved directly with the PSF stamp
+ * ('same' mode) and renormalised to unit box sum; then the Lanczos-3 shift
+ * by the remainder of (pos - origin).  Arguments as uph_render_stamps
+ * (stype NULL = all stars); DEV/SERSIC have no direct form and must not be
+ * passed.  Stamps above 63 px need scratch of
+ * uph_render_direct_scratch_bytes(T, k) bytes (k >= 1 workgroups).
+ */
+size_t uph_render_direct_scratch_bytes(int T, int n_blocks);
+int uph_render_stamps_direct(int n, int T, int n_terms, const float* psf_coefs,
+                             const float* psf_geo, const int32_t* src_band, const float* x,
+                             const float* y, const float* gx0, const float* gy0,
+                             const int32_t* stype, const float* logre, const float* ee1,
+                             const float* ee2, float* out, void* scratch, size_t scratch_bytes,
+                             void* stream);
+
+/*
  * Batched per grid-band linear solve (member fluxes + polynomial background
  * in one normal-equation system, float64 solve and inverse diagonal),
  * replacing solve_band_sparse / launch_band_sparse

@@ -25,6 +25,11 @@ Parity pins (tests/test_oracle_*.py):
   * Galaxy FFT render: Gaussian(1 component) x Gaussian PSF matches the
     closed-form convolved Gaussian away from the wrap; flux = total - wings.
   * Member-stamp derivatives: central finite differences in float64.
+  * Direct backend (``direct_stamp``): the shift-and-add convolution equals
+    ``scipy.signal.fftconvolve(mode="same")``; a one-component round
+    profile convolved with a Gaussian PSF matches the sampled Gaussian of
+    the summed variance; the stamp sums to 1 before the shift; STAR rows
+    equal the PSF render.
 """
 from __future__ import annotations
 
@@ -467,6 +472,69 @@ def member_stamp(psf: dict, x, y, x0, y0, stype, logre=0.0, ee1=0.0, ee2=0.0,
     if "y" in grads:
         grads["y"] = grads["y"] + correlate_y(hx, dty)
     return stamp, grads
+
+
+# Exponential-profile mixture of the direct backend (source_model.py:59-75).
+EXP_AMP_RAW = np.array([2.34853813e-03, 3.07995260e-02, 2.23364214e-01, 1.17949102e00,
+                        4.33873750e00, 5.99820770e00])
+EXP_VAR = np.array([1.20078965e-03, 8.84526493e-03, 3.91463084e-02, 1.39976817e-01,
+                    4.60962500e-01, 1.50159566e00])
+
+
+def exp_profile(T: int, logre, ee1, ee2, amps=None, vars_=None):
+    """Unit-sum pixel-centre exponential profile on a T x T grid centred at
+    T//2 (render_exp_profile_stamp_jax, source_model.py:218-251)."""
+    amps = EXP_AMP_RAW / EXP_AMP_RAW.sum() if amps is None else np.asarray(amps, float)
+    vars_ = EXP_VAR if vars_ is None else np.asarray(vars_, float)
+    yy, xx = np.mgrid[:T, :T].astype(float) - T // 2
+    re = min(max(math.exp(logre), 0.1), 1000.0)
+    ee = math.hypot(ee1, ee2)
+    e = 1.0 - math.exp(-ee)
+    inv_ba = min((1.0 + e) / max(1.0 - e, 1e-4), 1000.0)
+    th = 0.5 * math.atan2(ee2, ee1)
+    ct, st = math.cos(th), math.sin(th)
+    a, b, c, d = re * ct / inv_ba, re * st, -re * st / inv_ba, re * ct
+    prof = np.zeros((T, T))
+    for amp, var in zip(amps, vars_):
+        c00, c01, c11 = var * (a * a + b * b), var * (a * c + b * d), var * (c * c + d * d)
+        det = max(c00 * c11 - c01 * c01, 1e-12)
+        r2 = (c11 * xx * xx - 2.0 * c01 * xx * yy + c00 * yy * yy) / det
+        prof += amp / (2.0 * np.pi * math.sqrt(det)) * np.exp(-0.5 * np.minimum(r2, 200.0))
+    tot = prof.sum()
+    return prof / tot if tot > 0 else prof
+
+
+def convolve_same(a: np.ndarray, k: np.ndarray) -> np.ndarray:
+    """'same' 2-D convolution of equal odd-sized arrays by shift-and-add
+    over the kernel's pixels: out[i, j] = sum a[p, q] k[i + h - p, j + h - q]."""
+    T = a.shape[0]
+    h = T // 2
+    full = np.zeros((2 * T - 1, 2 * T - 1))
+    for p in range(T):
+        for q in range(T):
+            if a[p, q] != 0.0:
+                full[p:p + T, q:q + T] += a[p, q] * k
+    return full[h:h + T, h:h + T]
+
+
+def direct_stamp(psf: dict, x, y, x0, y0, stype, logre=0.0, ee1=0.0, ee2=0.0):
+    """Unit-flux stamp of the direct backend (source_model.py:264-328,
+    "direct" branch): STAR = PSF at (x, y); EXP = exp_profile convolved
+    ('same') with it and renormalised to unit sum; Lanczos-3 shift."""
+    raw_psf = eval_psf(psf, x, y)
+    T = raw_psf.shape[0]
+    if stype == SOURCE_EXP:
+        conv = convolve_same(exp_profile(T, logre, ee1, ee2), raw_psf)
+        s = conv.sum()
+        raw = conv / s if s > 0 else conv
+    elif stype == SOURCE_STAR:
+        raw = raw_psf
+    else:
+        raise ValueError("the direct backend renders STAR and EXP only")
+    xg, yg = x - x0, y - y0
+    tx = lanczos_taps(xg - _round_half_even(xg))
+    ty = lanczos_taps(yg - _round_half_even(yg))
+    return correlate_y(correlate_x(raw, tx), ty)
 
 
 def place(stamp: np.ndarray, ix_top: int, iy_top: int, B: int) -> np.ndarray:

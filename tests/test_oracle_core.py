@@ -204,3 +204,33 @@ def test_member_stamp_derivatives_fd(stype, names):
         fd = (render(vp) - render(vm)) / (2 * h)
         scale = np.abs(fd).max()
         assert np.max(np.abs(g[nm] - fd)) < 1e-5 * scale + 1e-12, nm
+
+
+def test_direct_backend_oracle_pins():
+    """Direct-backend restatement: the shift-and-add convolution equals
+    scipy's FFT convolution; a one-component round profile convolved with a
+    Gaussian PSF is the sampled Gaussian of the summed variance; the EXP
+    stamp has unit sum before the shift; a STAR equals the FFT backend's."""
+    from scipy.signal import fftconvolve
+    from oracle import core as oc
+    rng = np.random.default_rng(3)
+    a, k = rng.normal(size=(15, 15)), rng.normal(size=(15, 15))
+    np.testing.assert_allclose(oc.convolve_same(a, k), fftconvolve(a, k, mode="same"),
+                               rtol=0, atol=1e-12)
+    T = 41
+    yy, xx = np.mgrid[:T, :T] - T // 2
+    sp, re, v = 1.8, 2.0, 0.9
+    psf = np.exp(-0.5 * (xx ** 2 + yy ** 2) / sp ** 2)
+    psf /= psf.sum()
+    prof = oc.exp_profile(T, np.log(re), 0.0, 0.0, amps=[1.0], vars_=[v])
+    got = oc.convolve_same(prof, psf)
+    s2 = v * re * re + sp * sp
+    want = np.exp(-0.5 * (xx ** 2 + yy ** 2) / s2)
+    want /= want.sum()
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * want.max())
+    pd = dict(coefs=psf[..., None], x_origin=0.0, x_scale=1.0, y_origin=0.0, y_scale=1.0)
+    raw = oc.direct_stamp(pd, 20.0, 30.0, 0.0, 0.0, oc.SOURCE_EXP, 0.7, 0.2, -0.1)
+    np.testing.assert_allclose(raw.sum(), 1.0, atol=1e-12)  # integer position: no shift
+    st_d = oc.direct_stamp(pd, 20.3, 30.6, 0.0, 0.0, oc.SOURCE_STAR)
+    st_f, _ = oc.member_stamp(pd, 20.3, 30.6, 0.0, 0.0, oc.SOURCE_STAR)
+    np.testing.assert_array_equal(st_d, st_f)

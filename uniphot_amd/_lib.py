@@ -23,7 +23,7 @@ EXPORTS = (
     "uph_render_scratch_bytes", "uph_render_stamps", "uph_band_scratch_bytes", "uph_band_solve",
     "uph_wing_images", "uph_blank_work_floats", "uph_blank_response", "uph_stamp_sum",
     "uph_gather_windows", "uph_subtract_bg_planes", "uph_band_solve_noalloc",
-    "uph_band_psf_flags",
+    "uph_band_psf_flags", "uph_render_direct_scratch_bytes", "uph_render_stamps_direct",
 )
 UPH_BAND_NOALLOC_BYTES = 64 << 20
 
@@ -160,6 +160,10 @@ def lib() -> C.CDLL:
                                      C.c_int, C.c_int, _P, _P, _P]
     L.uph_band_psf_flags.restype = C.c_int
     L.uph_band_psf_flags.argtypes = [C.c_int] * 4 + [_P] * 5 + [C.c_int] * 2 + [_P] * 5 + [_P]
+    L.uph_render_direct_scratch_bytes.restype = C.c_size_t
+    L.uph_render_direct_scratch_bytes.argtypes = [C.c_int, C.c_int]
+    L.uph_render_stamps_direct.restype = C.c_int
+    L.uph_render_stamps_direct.argtypes = [C.c_int] * 3 + [_P] * 12 + [_P, C.c_size_t, _P]
     _lib = L
     return L
 

@@ -121,14 +121,17 @@ def _render(psf, pos_x, pos_y, grid_x0, grid_y0, source_type, logre, ee1, ee2, s
     st = np.asarray(source_type, np.int32)
     if render_backend not in ("direct", "fft"):
         raise ValueError(f"unknown render_backend: {render_backend!r}")
-    if render_backend == "direct" and np.any(st != SOURCE_STAR):
-        raise NotImplementedError(
-            "render_backend='direct' for extended sources is not part of this build; the "
-            "reference's 'auto' resolves to 'fft' whenever extended sources are present")
+    if render_backend == "direct" and np.any((st == SOURCE_DEV) | (st == SOURCE_SERSIC)):
+        raise ValueError("render_backend='direct' cannot render DEV/SERSIC sources; use 'fft'")
     T = int(psf.H_psf)
     c, g = psf_device_arrays([psf], T)
     n = np.asarray(pos_x).size
     dev = torch.device("cuda")
+    if render_backend == "direct":
+        return render_direct_stamps(torch.from_numpy(c).to(dev), torch.from_numpy(g).to(dev), T,
+                                    c.shape[1], np.zeros(n, np.int32), pos_x, pos_y,
+                                    np.full(n, grid_x0), np.full(n, grid_y0), st, logre, ee1,
+                                    ee2, stream=stream)
     return render_unit_stamps(torch.from_numpy(c).to(dev), torch.from_numpy(g).to(dev), T,
                               c.shape[1], np.zeros(n, np.int32), pos_x, pos_y,
                               np.full(n, grid_x0), np.full(n, grid_y0), st,
@@ -136,13 +139,47 @@ def _render(psf, pos_x, pos_y, grid_x0, grid_y0, source_type, logre, ee1, ee2, s
                               np.ones(n) if sersic_n is None else sersic_n, stream=stream)
 
 
+def render_direct_stamps(psf_coefs_dev, psf_geo_dev, T: int, n_terms: int, band, pos_x, pos_y,
+                         origin_x, origin_y, stype, logre, ee1, ee2, *, stream=None):
+    """(n, T, T) unit-flux stamps of the direct backend on the device
+    (``uph_render_stamps_direct``): the PSF at the position for stars; for
+    EXP the pixel-centre 6-Gaussian profile convolved directly with the PSF
+    stamp and renormalised to unit sum (source_model.py:218-256, :298-303);
+    Lanczos-3 shift by the remainder of (pos - origin)."""
+    import torch
+    from . import _lib
+    dev = psf_coefs_dev.device
+    L = _lib.lib()
+    P = _lib.ptr
+    n = int(np.asarray(pos_x).size)
+    out = torch.empty((max(n, 1), T, T), dtype=torch.float32, device=dev)
+    if n == 0:
+        return out[:0]
+    st = np.asarray(stype, np.int32)
+    if np.any((st == SOURCE_DEV) | (st == SOURCE_SERSIC)):
+        raise ValueError("render_backend='direct' cannot render DEV/SERSIC sources; use 'fft'")
+    up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+    k = max(1, min(n, 1024))
+    sb = L.uph_render_direct_scratch_bytes(T, k)
+    scratch = torch.empty(max(sb, 1), dtype=torch.uint8, device=dev) if sb else None
+    args = [up(band, np.int32), up(pos_x, np.float32), up(pos_y, np.float32),
+            up(origin_x, np.float32), up(origin_y, np.float32), up(st, np.int32),
+            up(logre, np.float32), up(ee1, np.float32), up(ee2, np.float32)]
+    _lib.check(L.uph_render_stamps_direct(n, T, n_terms, P(psf_coefs_dev), P(psf_geo_dev),
+                                          *(P(a) for a in args), P(out), P(scratch), sb,
+                                          _lib.stream_ptr(stream)))
+    return out
+
+
 def render_source_stamps(psf_jax, pos_x, pos_y, grid_x0: int, grid_y0: int, source_type,
                          logre, ee1, ee2, sersic_n=None, render_backend: str = "direct"):
     """Unit-flux stamps (n, T, T) on the device for mixed star/galaxy
     sources (source_model.py:367-390): the PSF polynomial at the full-image
     position, EXP/DEV/SERSIC through the Fourier mixture convolution
-    (``render_backend="fft"``), Lanczos-3 shift by the remainder of (pos -
-    grid origin); not renormalised.  ``psf_jax`` is a PsfPolynomial."""
+    (``render_backend="fft"``, not renormalised) or EXP through the direct
+    pixel-space convolution (``render_backend="direct"``, unit box sum),
+    Lanczos-3 shift by the remainder of (pos - grid origin).  ``psf_jax`` is
+    a PsfPolynomial."""
     return _render(psf_jax, pos_x, pos_y, grid_x0, grid_y0, source_type, logre, ee1, ee2,
                    sersic_n, render_backend)
 
