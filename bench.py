@@ -8,8 +8,9 @@ Metric: sources fitted per second (whole job).  Also reported: the
 dominant kernel's roofline position -- algorithmic HBM bytes and fp32 flops
 per launch over its measured average duration (HIP events on its stream),
 placed against whichever of the 8 TB/s HBM and 157.3 TFLOP/s fp32 peaks it
-is closer to -- and a CPU baseline (the float64 oracle timed on a bounded
-sample of the same workload).
+is closer to -- and a CPU baseline (the CPU restatement of the same method,
+float32 pixels and OpenMP, timed on a stratified sample of the same
+workload).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c2|c2-stars|c1|c3|c4|c5]
 """
@@ -163,100 +164,161 @@ def make_scene(config):
     raise SystemExit(f"unknown config {config}")
 
 
-def cpu_baseline(fld, res, sc, cfg, budget_s=12.0, seed=0, oracle_cmp=None):
-    """Float64 oracle timed on a bounded random sample of the same work:
-    island LM fits (until ~2/3 of the budget) and grid-band linear solves
-    (the rest); extrapolated to seconds per source for the full field.
-    ``oracle_cmp`` (a list) collects, per fitted island, (source ids, oracle
-    fluxes (n, nb), flux errors, parameters (n, 6), steps, converged)."""
-    from oracle import island as oisl
+def _cpu_threads():
+    """Threads of the CPU baseline: the process's CPU share (OMP_NUM_THREADS
+    when set, else the affinity mask)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return max(1, min(n, int(env))) if env and env.isdigit() else n
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _island_items(art, scene, psfs, picks):
+    """CPU-restatement island inputs (window cutouts with the scene
+    subtraction's member add-back) for (class, row) picks."""
     from oracle import core as ocore
+    from oracle import island as oisl
+    d, w = art["dstack"], art["wstack"]
+    items, meta = [], []
+    for pi, r in picks:
+        plan, host = art["classes"][pi]
+        B, T = plan.bbox_class, plan.tier
+        ox, oy = int(host["ox"][r]), int(host["oy"][r])
+        cut = d[:, oy:oy + B, ox:ox + B]
+        mm = host["mmask"][r]
+        if scene is not None:
+            mem = plan.midx[r][mm]
+            ps = [dict(q, coefs=ocore.pad_psf_coefs(q["coefs"], T)) for q in psfs]
+            cut = cut + oisl.member_addback(ps, host["x0b"][r], host["y0b"][r], B,
+                                            scene.x[mem], scene.y[mem], scene.stype[mem],
+                                            scene.flux[mem])
+        items.append(dict(data=cut, w=w[:, oy:oy + B, ox:ox + B], x0b=host["x0b"][r],
+                          y0b=host["y0b"][r], st=host["st"][r], sn=host["sn"][r],
+                          fn=host["fn_flag"][r], mmask=mm, keep=host["keep"][r],
+                          p0=host["p0"][r].ravel(), lower=host["lower"][r].ravel(),
+                          upper=host["upper"][r].ravel(), T=T))
+        meta.append((pi, plan.midx[r][mm], int(mm.sum())))
+    return items, meta
+
+
+def cpu_baseline(fld, res, sc, cfg, n_islands=120, n_problems=48, seed=0, oracle_cmp=None):
+    """The CPU restatement (oracle/cpu_fit.cpp: float32 pixels, float64
+    solves, OpenMP over islands and grid-band problems) timed on a sample
+    of the same field: ``n_islands`` islands stratified by class (each
+    class in proportion to its island count, at least one) fitted on all
+    threads, each island's time measured on its own thread, extrapolated
+    per class; ``n_problems`` random grid-band problems (stamp render +
+    solve) on one thread, extrapolated to the two linear passes.  value =
+    sources/s on all threads (per-island thread times over the thread
+    count); one_core = the same on one thread.  ``oracle_cmp`` (a list)
+    collects float64 fits of 150 random islands by the float64 build of the
+    same source, which reproduces the float64 oracle (tests/
+    test_oracle_cpu.py)."""
+    from oracle import core as ocore
+    from oracle import cpu as ocpu
+    from oracle import island as oisl
     from oracle import linear as olin
     from uniphot_amd import forced_phot_sparse as fps
     rng = np.random.default_rng(seed)
     art = res.record["_artifacts"]
-    d = art["dstack"].cpu().numpy()
-    w = art["wstack"].cpu().numpy()
+    art = dict(art, dstack=art["dstack"].cpu().numpy(), wstack=art["wstack"].cpu().numpy())
     scene = art.get("scene")
-    o = oisl.LMOptions(reg=cfg.reg)
     psfs = [dict(coefs=p.coefs.astype(np.float64), x_origin=p.x_origin, x_scale=p.x_scale,
                  y_origin=p.y_origin, y_scale=p.y_scale) for p in fld.psfs]
-    items = [(pi, r) for pi, (plan, host) in enumerate(art["classes"])
-             for r in range(plan.n_islands)]
-    rng.shuffle(items)
+    C = _cpu_threads()
+    o = oisl.LMOptions(reg=cfg.reg)
+    ocore.sersic_table()  # cached before any timing
+    counts = np.array([plan.n_islands for plan, _ in art["classes"]])
+    n_tot = int(counts.sum())
+    picks = []
+    for pi, nc in enumerate(counts):
+        k = int(min(nc, max(1, round(n_islands * nc / max(n_tot, 1)))))
+        picks += [(pi, int(r)) for r in rng.choice(nc, k, replace=False)]
+    items, meta = _island_items(art, scene, psfs, picks)
+    # each sampled island runs at most `cap` LM steps; its time is scaled to
+    # the step count the GPU fit took, counting the initial chi2 and the
+    # final fluxes as one step together
+    cap = 3
+    nst_gpu = np.array([max(int(res.island_nst[src[0]]), 0) for _, src, _ in meta])
+    out = []
     t0 = time.perf_counter()
-    n_src_isl = 0
-    n_isl = 0
-    for pi, r in items:
-        if time.perf_counter() - t0 > budget_s * 2 / 3:
-            break
-        plan, host = art["classes"][pi]
-        B, T = plan.bbox_class, plan.tier
-        ox, oy = int(host["ox"][r]), int(host["oy"][r])
-        ps = [dict(q, coefs=ocore.pad_psf_coefs(q["coefs"], T)) for q in psfs]
-        cut = d[:, oy:oy + B, ox:ox + B]
-        if scene is not None:
-            # member add-back of the scene subtraction
-            mem = plan.midx[r][host["mmask"][r]]
-            cut = cut + oisl.member_addback(ps, host["x0b"][r], host["y0b"][r], B,
-                                            scene.x[mem], scene.y[mem], scene.stype[mem],
-                                            scene.flux[mem])
-        isl = oisl.Island(cut, w[:, oy:oy + B, ox:ox + B],
-                          host["x0b"][r], host["y0b"][r], host["st"][r], host["sn"][r],
-                          host["fn_flag"][r], host["mmask"][r], host["keep"][r],
-                          host["lower"][r].ravel(), host["upper"][r].ravel(), ps, T)
-        st = oisl.run_island(host["p0"][r], isl, o)
-        f, e = oisl.final_fluxes(st["p"], isl, o)
-        if oracle_cmp is not None:
-            mm = host["mmask"][r]
-            oracle_cmp.append((plan.midx[r][mm], f[:, mm].T, e[:, mm].T,
-                               st["p"].reshape(-1, 6)[mm], st["nst"], bool(st["conv"])))
-        n_src_isl += int(host["mmask"][r].sum())
-        n_isl += 1
-    t_isl = time.perf_counter() - t0
-    # linear solves: random (grid, band) problems
+    for s_cap in sorted(set(np.minimum(nst_gpu, cap).tolist())):
+        sel = np.flatnonzero(np.minimum(nst_gpu, cap) == s_cap)
+        rr = ocpu.fit_islands([items[i] for i in sel], psfs, dataclasses.replace(o, max_steps=s_cap),
+                              n_threads=C)
+        out += [(int(i), r) for i, r in zip(sel, rr)]
+    wall_isl = time.perf_counter() - t0
+    per_class = {}
+    for i, r in out:
+        scale = (nst_gpu[i] + 1.0) / (min(nst_gpu[i], cap) + 1.0)
+        per_class.setdefault(meta[i][0], []).append(r["sec"] * scale)
+    t1_isl = float(sum(counts[pi] * np.mean(v) for pi, v in per_class.items()))
+    # linear passes: random (grid, band) problems, one thread
     grids = art["grids"]
-    t1 = time.perf_counter()
-    n_src_lin = 0
-    n_lin = 0
     data = fld.data.cpu().numpy()
     iv = fld.invvar.cpu().numpy()
     gd = fld.good.cpu().numpy().astype(bool)
-    bg_geo = (0.5 * (fld.W - 1), 0.5 * fld.W, 0.5 * (fld.H - 1), 0.5 * fld.H)
-    while time.perf_counter() - t1 < budget_s / 3:
+    geo = (0.5 * (fld.W - 1), 0.5 * fld.W, 0.5 * (fld.H - 1), 0.5 * fld.H)
+    t_lin = 0.0
+    for _ in range(n_problems):
         g = grids[rng.integers(len(grids))]
         b = int(rng.integers(fld.nb))
         fi = g.fit_indices
         T = art["gtier"][g.grid_id]
-        ps = dict(psfs[b], coefs=ocore.pad_psf_coefs(psfs[b]["coefs"], T))
         xs, ys = sc.x[fi], sc.y[fi]
+        t0 = time.perf_counter()
         all_star = bool(np.all(sc.stype[fi] == 0))
         nbr, msk = fps.build_neighbor_list(xs, ys, T, T)
-        stamps = np.stack([ocore.member_stamp(ps, float(sc.x[i]), float(sc.y[i]), g.x0_pad,
-                                              g.y0_pad, int(sc.stype[i]), float(sc.logre[i]),
-                                              float(sc.ee1[i]), float(sc.ee2[i]),
-                                              float(sc.sersic_n[i]), renorm=all_star)[0]
-                           for i in fi])
+        st = ocpu.render_stamps([psfs[b]], T, 0, xs, ys, g.x0_pad, g.y0_pad, sc.stype[fi],
+                                sc.logre[fi], sc.ee1[fi], sc.ee2[fi], sc.sersic_n[fi],
+                                renorm=all_star, n_threads=1)
         ix, iy = fps.stamp_top_left(xs, ys, g.x0_pad, g.y0_pad, T, T)
-        basis = olin.bg_basis(g.pad_h, g.pad_w, g.x0_pad, g.y0_pad, *bg_geo, order=1)
         sl = (b, slice(g.y0_pad, g.y1_pad), slice(g.x0_pad, g.x1_pad))
-        olin.solve_band(data[sl].astype(np.float64), iv[sl].astype(np.float64), gd[sl], stamps,
-                        ix, iy, nbr, msk, basis)
-        n_src_lin += int(g.source_indices.size)
-        n_lin += 1
-    t_lin = time.perf_counter() - t1
-    s_per_src = t_isl / max(n_src_isl, 1) + 2 * fld.nb * t_lin / max(n_src_lin, 1)
-    return dict(value=1.0 / s_per_src, unit="sources/s", cores=1, kind="port",
-                sample=(f"float64 oracle, 1 thread: {n_isl} random islands ({n_src_isl} sources) "
-                        f"of the field's island fit in {t_isl:.1f}s + {n_lin} random grid-band "
-                        f"linear solves ({n_src_lin} owned sources) in {t_lin:.1f}s; "
-                        f"extrapolated per source (islands + 2 passes x {fld.nb} bands)"))
+        ocpu.band_solve([dict(data=data[sl], invvar=iv[sl], good=gd[sl], stamps=st, ix=ix,
+                              iy=iy, nbr=np.where(msk, nbr, -1), gx0=g.x0_pad, gy0=g.y0_pad)],
+                        geo, k_max=nbr.shape[1], reg=cfg.reg, n_threads=1)
+        t_lin += time.perf_counter() - t0
+    t1_lin = t_lin / n_problems * 2 * len(grids) * fld.nb
+    n_src = int(sc.x.size)
+    one = n_src / (t1_isl + t1_lin)
+    if oracle_cmp is not None:
+        order = [(pi, r) for pi, (plan, _) in enumerate(art["classes"])
+                 for r in range(plan.n_islands)]
+        sel = rng.choice(len(order), min(150, len(order)), replace=False)
+        items64, meta64 = _island_items(art, scene, psfs, [order[i] for i in sel])
+        fits = ocpu.fit_islands(items64, psfs, o, n_threads=C, precision="fp64")
+        for it, (_, src, _), r in zip(items64, meta64, fits):
+            mm = np.asarray(it["mmask"], bool)
+            oracle_cmp.append((src, r["fluxes"][:, mm].T, r["ferr"][:, mm].T,
+                               r["p"].reshape(-1, 6)[mm], r["nst"], r["conv"]))
+    return dict(value=one * C, unit="sources/s", cores=C, kind="port",
+                one_core=one, cpu_model=_cpu_model(),
+                sample=(f"CPU restatement of the per-island VarPro-LM and the grid-band solve "
+                        f"(oracle/cpu_fit.cpp, g++ -O3 x86-64-v3, float32 pixels, float64 "
+                        f"solves, OpenMP): {len(items)} islands stratified over "
+                        f"{len(counts)} classes on {C} threads in {wall_isl:.1f}s (per-island "
+                        f"thread time {sum(r['sec'] for _, r in out):.1f}s for at most {cap} LM "
+                        f"steps each, scaled to the fit's step count and per class to {n_tot} "
+                        f"islands: {t1_isl:.1f} core-s) + {n_problems} random "
+                        f"grid-band problems (stamps + solve) on 1 thread in {t_lin:.1f}s "
+                        f"(extrapolated to 2 passes x {len(grids)} grids x {fld.nb} bands: "
+                        f"{t1_lin:.1f} core-s); value = one-core rate x {C} threads"))
 
 
 def oracle_accuracy(res, cmp):
     """GPU island fit (island VarPro fluxes and positions of ``res``)
-    against the float64 oracle's fits of the same islands from the CPU
-    baseline's sample: per source, max over bands of |f - f_o| /
+    against float64 fits of random islands by the float64 build of the CPU
+    restatement (which reproduces the float64 oracle's fits): per source, max over bands of |f - f_o| /
     max(|f_o|, sigma_o), position error in px, and the share of islands
     with the oracle's step count."""
     from uniphot_amd.pipeline import _rel_stats
@@ -543,7 +605,10 @@ def main():
     ap.add_argument("--side", type=int, default=8192, help="c4: field side (px)")
     ap.add_argument("--sample", type=int, default=300,
                     help="islands of the FP64 accuracy comparison (c2, c3, c4; 0 = none)")
-    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--cpu-budget", type=float, default=12.0,
+                    help="c3: seconds of float64-oracle CPU work for its baseline")
+    ap.add_argument("--cpu-islands", type=int, default=120,
+                    help="islands of the CPU baseline's stratified sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--island-split", type=int, default=1,
                     help="1 = band-split island LM (each island's bands spread over all CUs), "
@@ -644,8 +709,11 @@ def main():
     line["config"]["fp64_refit_islands"] = res.record["n_fp64_refit_islands"]
     if not args.no_cpu:
         cmp = []
-        line["cpu_baseline"] = cpu_baseline(fld, res, sc, cfg, budget_s=args.cpu_budget,
+        t_cpu = time.perf_counter()
+        line["cpu_baseline"] = cpu_baseline(fld, res, sc, cfg, n_islands=args.cpu_islands,
                                             oracle_cmp=cmp)
+        print(f"[bench] cpu baseline ({time.perf_counter() - t_cpu:.1f}s) "
+              f"{line['cpu_baseline']}", file=sys.stderr, flush=True)
         line["accuracy_vs_oracle"] = oracle_accuracy(res, cmp)
     print(json.dumps(line), flush=True)
 
