@@ -183,9 +183,32 @@ def _cpu_model():
     return "unknown"
 
 
-def _island_items(art, scene, psfs, picks):
-    """CPU-restatement island inputs (window cutouts with the scene
-    subtraction's member add-back) for (class, row) picks."""
+def _device_cutouts(art, picks):
+    """The island cutouts the GPU fit used, rebuilt by its own mosaic code
+    (window of the scene-subtracted stack plus the member add-back at the
+    class tier): {(class, row): (data, weights)}."""
+    from uniphot_amd.scene import island_mosaic
+    by_cls = {}
+    for pi, r in picks:
+        by_cls.setdefault(pi, []).append(int(r))
+    keys = sorted(by_cls)
+    chunk = [(None, art["classes"][pi][1], np.asarray(by_cls[pi])) for pi in keys]
+    plan_of = {id(h): p for p, h in art["classes"]}
+    dmos, wmos, moved = island_mosaic(chunk, plan_of, art["dstack"], art["wstack"], art["scene"])
+    dm, wm = dmos.cpu().numpy(), wmos.cpu().numpy()
+    out = {}
+    for pi, (_, h2, rows) in zip(keys, moved):
+        B = art["classes"][pi][0].bbox_class
+        for r in rows:
+            mx, my = int(h2["ox"][r]), int(h2["oy"][r])
+            out[(pi, int(r))] = (dm[:, my:my + B, mx:mx + B], wm[:, my:my + B, mx:mx + B])
+    return out
+
+
+def _island_items(art, scene, psfs, picks, cuts=None):
+    """CPU-restatement island inputs for (class, row) picks: the window
+    cutouts with the scene subtraction's member add-back (restated in
+    float64), or the given ``cuts`` {(class, row): (data, weights)}."""
     from oracle import core as ocore
     from oracle import island as oisl
     d, w = art["dstack"], art["wstack"]
@@ -195,14 +218,17 @@ def _island_items(art, scene, psfs, picks):
         B, T = plan.bbox_class, plan.tier
         ox, oy = int(host["ox"][r]), int(host["oy"][r])
         cut = d[:, oy:oy + B, ox:ox + B]
+        wcut = w[:, oy:oy + B, ox:ox + B]
         mm = host["mmask"][r]
-        if scene is not None:
+        if cuts is not None:
+            cut, wcut = cuts[(pi, int(r))]
+        elif scene is not None:
             mem = plan.midx[r][mm]
             ps = [dict(q, coefs=ocore.pad_psf_coefs(q["coefs"], T)) for q in psfs]
             cut = cut + oisl.member_addback(ps, host["x0b"][r], host["y0b"][r], B,
                                             scene.x[mem], scene.y[mem], scene.stype[mem],
                                             scene.flux[mem])
-        items.append(dict(data=cut, w=w[:, oy:oy + B, ox:ox + B], x0b=host["x0b"][r],
+        items.append(dict(data=cut, w=wcut, x0b=host["x0b"][r],
                           y0b=host["y0b"][r], st=host["st"][r], sn=host["sn"][r],
                           fn=host["fn_flag"][r], mmask=mm, keep=host["keep"][r],
                           p0=host["p0"][r].ravel(), lower=host["lower"][r].ravel(),
@@ -221,17 +247,18 @@ def cpu_baseline(fld, res, sc, cfg, n_islands=120, n_problems=48, seed=0, oracle
     solve) on one thread, extrapolated to the two linear passes.  value =
     sources/s on all threads (per-island thread times over the thread
     count); one_core = the same on one thread.  ``oracle_cmp`` (a list)
-    collects float64 fits of 150 random islands by the float64 build of the
-    same source, which reproduces the float64 oracle (tests/
-    test_oracle_cpu.py)."""
+    collects float64 fits of 150 random islands, on the cutouts the GPU fit
+    used, by the float64 build of the same source, which reproduces the
+    float64 oracle (tests/test_oracle_cpu.py)."""
     from oracle import core as ocore
     from oracle import cpu as ocpu
     from oracle import island as oisl
     from oracle import linear as olin
     from uniphot_amd import forced_phot_sparse as fps
     rng = np.random.default_rng(seed)
-    art = res.record["_artifacts"]
-    art = dict(art, dstack=art["dstack"].cpu().numpy(), wstack=art["wstack"].cpu().numpy())
+    art_dev = res.record["_artifacts"]
+    art = dict(art_dev, dstack=art_dev["dstack"].cpu().numpy(),
+               wstack=art_dev["wstack"].cpu().numpy())
     scene = art.get("scene")
     psfs = [dict(coefs=p.coefs.astype(np.float64), x_origin=p.x_origin, x_scale=p.x_scale,
                  y_origin=p.y_origin, y_scale=p.y_scale) for p in fld.psfs]
@@ -295,7 +322,9 @@ def cpu_baseline(fld, res, sc, cfg, n_islands=120, n_problems=48, seed=0, oracle
         order = [(pi, r) for pi, (plan, _) in enumerate(art["classes"])
                  for r in range(plan.n_islands)]
         sel = rng.choice(len(order), min(150, len(order)), replace=False)
-        items64, meta64 = _island_items(art, scene, psfs, [order[i] for i in sel])
+        picks64 = [order[i] for i in sel]
+        items64, meta64 = _island_items(art, scene, psfs, picks64,
+                                        cuts=_device_cutouts(art_dev, picks64))
         fits = ocpu.fit_islands(items64, psfs, o, n_threads=C, precision="fp64")
         for it, (_, src, _), r in zip(items64, meta64, fits):
             mm = np.asarray(it["mmask"], bool)
@@ -667,6 +696,7 @@ def main():
         for name, ms, info in timer.per_launch()[:400]:
             print(f"[bench] {name} {ms:.3f} ms {info}", file=sys.stderr)
     roof = roofline_of(ks, args.config)
+    from uniphot_amd import autotune
     line = {
         "metric": "sources fitted/sec at <=1e-5 flux rel-err vs FP64; p50/p99 tile latency",
         "value": round(value, 2), "unit": "sources/s", "n_gpus": 1, "steps": args.steps,
@@ -678,6 +708,7 @@ def main():
                    "grid": [cfg.n_rows, cfg.n_cols], "islands": res.record["n_islands"],
                    "stage_s": {k: round(v, 4) for k, v in res.timings.items()}},
         "roofline": roof,
+        "autotune": autotune.log(),
         "kernels": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv)
                         for kk, vv in v.items()} for k, v in ks.items()},
     }
@@ -692,7 +723,11 @@ def main():
               file=sys.stderr, flush=True)
         # the same fit with every island on the float32 kernels (no float64
         # routing or refit): its throughput and its accuracy on the sample
-        c32 = dataclasses.replace(cfg, fp64_contrast=None, fp64_refit_kappa=None)
+        from uniphot_amd import autotune
+        b32 = [t["choice"] for t in autotune.log() if "|fp32|" in t["key"]]
+        c32 = dataclasses.replace(cfg, fp64_contrast=None, fp64_refit_kappa=None,
+                                  island_blocks=dict(fp32=b32[-1]) if b32 else None,
+                                  autotune=not b32)
         sync()
         t32 = time.perf_counter()
         r32 = fit_field(fld, *args_fit[:-1], c32, sync=sync, keep_artifacts=True)

@@ -171,3 +171,36 @@ def test_scene_subtract_and_island_cutouts_match_oracle(cuda):
         for r in rows:
             cover[int(h2["oy"][r]):int(h2["oy"][r]) + B, int(h2["ox"][r]):int(h2["ox"][r]) + B] += 1
     assert cover.max() == 1
+
+
+def test_autotuned_launch_shape_leaves_the_fit_unchanged(cuda, tmp_path, monkeypatch):
+    """The autotuner times the island launch sequence at each candidate grid
+    size, caches its choice under the workload key, and a second fit reuses
+    it; the grid size changes no result (each (island, band) item runs on
+    one workgroup whatever the grid)."""
+    import dataclasses
+    import json
+    import torch
+    from uniphot_amd import autotune
+    from uniphot_amd.pipeline import DeviceField, FitConfig, fit_field
+    monkeypatch.setenv("UPH_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    sc = small_mixed_scene(seed=3, n_src=24)
+    fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
+    args = (sc.x, sc.y, sc.stype, sc.logre, sc.ee1, sc.ee2, sc.sersic_n)
+    cfg = FitConfig(n_rows=1, n_cols=1)
+    sync = torch.cuda.synchronize
+    n_log = len(autotune.log())
+    r0 = fit_field(fld, *args, cfg, sync=sync)
+    tuned = autotune.log()[n_log:]
+    assert tuned and not any(t["cached"] for t in tuned)
+    cache = json.loads((tmp_path / "tune.json").read_text())
+    for t in tuned:
+        assert cache[t["key"]]["choice"] == t["choice"] in autotune.BLOCK_CANDIDATES
+        assert sorted(cache[t["key"]]["ms"]) == sorted(map(str, autotune.BLOCK_CANDIDATES))
+    fit_field(fld, *args, cfg, sync=sync)
+    assert all(t["cached"] for t in autotune.log()[n_log + len(tuned):])
+    outs = [r0] + [fit_field(fld, *args, dataclasses.replace(cfg, island_blocks=dict(fp32=b, fp64=b)),
+                             sync=sync) for b in (256, 2048)]
+    for r in outs[1:]:
+        for k in ("flux", "x", "y", "logre", "ee1", "ee2", "island_nst", "island_flux"):
+            assert np.array_equal(getattr(r, k), getattr(r0, k), equal_nan=True), k

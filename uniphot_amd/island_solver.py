@@ -392,7 +392,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                          post: bool = False, gram: str = "fp32",
                          workspace_bytes: float = MAX_SCRATCH_BYTES,
                          gram_split_min: float = GRAM_SPLIT_MIN,
-                         init_states=None) -> "PreparedIslandBatch":
+                         init_states=None, max_blocks: int | None = None
+                         ) -> "PreparedIslandBatch":
     """Stage the islands of several classes for ONE launch sequence.
 
     ``classes`` is a list of (ClassKernels, host dict, rows); ``psf_trees``
@@ -417,6 +418,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
     and the per-band metric sums (``post_metrics`` (n, N, nb, 6)), plus the
     scaled condition number of each island's Gauss-Newton matrix
     (``post_kappa`` (n,)).
+    ``max_blocks`` caps the persistent workgroups of the launches (default
+    MAX_BLOCKS; :mod:`uniphot_amd.autotune` measures it per workload).
     ``gram="bf16"`` (split + pre-render) builds the island Gram matrices on
     bfloat16 matrix cores with float32 accumulation.  ``init_states`` (one
     dict per class with p (n_c, N, 6), lam, chi2, consec, nrej, nst) resumes
@@ -517,7 +520,7 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
         nrej=torch.empty(n_isl, dtype=torch.int32, device=dev),
         n_good=torch.empty(n_isl, dtype=torch.int32, device=dev))
     n_work = n_isl * nb if split else n_isl
-    nblk = max(1, min(n_work, MAX_BLOCKS, int(workspace_bytes) // per))
+    nblk = max(1, min(n_work, int(max_blocks or MAX_BLOCKS), int(workspace_bytes) // per))
     scratch = torch.empty(per * nblk, dtype=torch.uint8, device=dev)
     counter = torch.zeros(1, dtype=torch.int32, device=dev)
     P = _lib.ptr
@@ -837,7 +840,8 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
                      precision: str = "fp32", stream=None, sersic_dev=None, timer=None,
                      split: bool = False, prerender: bool = True, post: bool = False,
                      gram: str = "fp32", workspace_bytes: float = MAX_SCRATCH_BYTES,
-                     gram_split_min: float = GRAM_SPLIT_MIN, init_states=None):
+                     gram_split_min: float = GRAM_SPLIT_MIN, init_states=None,
+                     max_blocks: int | None = None):
     """Fit the islands of several classes in one launch sequence (see
     :func:`prepare_island_batch`) and return one result dict per class, in
     the same layout as :func:`run_class_chunk`."""
@@ -845,7 +849,7 @@ def run_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, wstac
                               precision=precision, sersic_dev=sersic_dev, timer=timer,
                               split=split, prerender=prerender, post=post, gram=gram,
                               workspace_bytes=workspace_bytes, gram_split_min=gram_split_min,
-                              init_states=init_states)
+                              init_states=init_states, max_blocks=max_blocks)
     ev = timer.begin("island_lm_kernel") if timer is not None else None
     pb.launch(stream)
     if ev is not None:
@@ -895,7 +899,7 @@ def _fp64_groups(k64, budget: float = FP64_PLANE_BUDGET):
 
 def _run_fp64_split(k64, psf_trees, max_steps, dstack_dev, wstack_dev, *, sersic_dev=None,
                     prerender=True, workspace_bytes=MAX_SCRATCH_BYTES, timer=None,
-                    init_states=None):
+                    init_states=None, max_blocks=None):
     """:func:`run_island_batch` on the float64 band-split kernels in launch
     sequences of at most FP64_PLANE_BUDGET bytes of member planes (see
     :func:`_fp64_groups`); one result dict per class, rows in order."""
@@ -912,7 +916,7 @@ def _run_fp64_split(k64, psf_trees, max_steps, dstack_dev, wstack_dev, *, sersic
         res = run_island_batch(sub, psf_trees, max_steps, dstack_dev, wstack_dev,
                                precision="fp64", sersic_dev=sersic_dev, split=True,
                                prerender=prerender, workspace_bytes=workspace_bytes,
-                               timer=timer, init_states=ist)
+                               timer=timer, init_states=ist, max_blocks=max_blocks)
         for (ci, _, _), r in zip(grp, res):
             parts[ci].append(r)
     return [p[0] if len(p) == 1 else _merge_chunk_outputs(p) for p in parts]
@@ -921,7 +925,7 @@ def _run_fp64_split(k64, psf_trees, max_steps, dstack_dev, wstack_dev, *, sersic
 def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev, *,
                 sersic_dev=None, post: bool = True,
                 workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None, init_states=None,
-                prerender: bool = True):
+                prerender: bool = True, max_blocks: int | None = None):
     """Fit ``classes`` (kernels, host, rows) on the float64 kernels
     (band-split; members pre-rendered by the float64 (member, band)
     launches, or in place with ``prerender=False``: the same planes, bit
@@ -936,7 +940,8 @@ def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev
     k64 = [(dataclasses.replace(k, precision="fp64"), h, np.asarray(r)) for k, h, r in classes]
     r64 = _run_fp64_split(k64, psf_trees, max_steps, dstack_dev, wstack_dev,
                           sersic_dev=sersic_dev, prerender=prerender,
-                          workspace_bytes=workspace_bytes, timer=timer, init_states=init_states)
+                          workspace_bytes=workspace_bytes, timer=timer, init_states=init_states,
+                          max_blocks=max_blocks)
     if post:
         at64 = []
         for (kern, host, rows), r in zip(k64, r64):

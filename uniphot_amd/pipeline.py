@@ -81,6 +81,12 @@ class FitConfig:
     # post-hoc diverged / plateau probes of stalled islands (the reference
     # builds and runs both by default, island_solver.py:1762-1764, :2984-3009)
     stall_probes: bool = True
+    # persistent workgroups of the band-split launch sequences per precision
+    # ({"fp32": n, "fp64": n}); None: measured per workload by the
+    # autotuner (uniphot_amd/autotune.py) and cached, or MAX_BLOCKS with
+    # autotune off
+    island_blocks: dict | None = None
+    autotune: bool = True
 
 
 @dataclass
@@ -611,6 +617,45 @@ def island_contrast(plan, rows, flux, sigma) -> np.ndarray:
     return np.maximum(c.max(axis=1), 1.0) if c.size else np.ones(rows.size)
 
 
+def _island_blocks(cfg, part32, part64, trees, nb, ds, ws, sersic_dev, post, sync):
+    """Persistent workgroups per precision for one launch sequence: the
+    configured ones, or the autotuner's choice for this workload (measured
+    on the sequence itself the first time, then cached)."""
+    from . import autotune
+    from .island_solver import fit_float64, run_island_batch
+    if cfg.island_blocks is not None or not cfg.autotune or not cfg.island_split:
+        return dict(cfg.island_blocks or {})
+    import torch
+    sync = sync or torch.cuda.synchronize
+    out = {}
+
+    def timed(fn):
+        sync()
+        t0 = time.perf_counter()
+        fn()
+        sync()
+        return time.perf_counter() - t0
+
+    for prec, part in ((cfg.precision, part32), ("fp64", part64)):
+        if not part:
+            continue
+        n_isl = int(sum(np.asarray(r).size for _, _, r in part))
+        tiers = sorted({int(k.T) for k, _, _ in part})
+        mixed = any(k.kind != "star" for k, _, _ in part)
+        key = autotune.workload_key(prec, nb, tiers, mixed, n_isl, ds.device)
+        if prec == "fp64" and part is part64:
+            run = lambda c: timed(lambda: fit_float64(
+                part64, trees, cfg.max_steps, ds, ws, sersic_dev=sersic_dev, post=False,
+                workspace_bytes=cfg.island_workspace, max_blocks=c))
+        else:
+            run = lambda c: timed(lambda: run_island_batch(
+                part32, trees, cfg.max_steps, ds, ws, precision=cfg.precision,
+                sersic_dev=sersic_dev, split=True, prerender=cfg.galaxy_prerender, post=post,
+                gram=cfg.gram, workspace_bytes=cfg.island_workspace, max_blocks=c))
+        out[prec] = autotune.choose(key, run)
+    return out
+
+
 def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans, x, y, stype,
                    logre, ee1, ee2, sn, free_n, cfg: FitConfig, timer=None,
                    budget_bytes: float = 48e9, sync=None, p_start=None,
@@ -701,11 +746,14 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                 if hi.any():
                     part64.append((kern, host, rows[hi]))
             chunk = part32
+        blocks = _island_blocks(cfg, chunk, part64, trees, nb, ds, ws, sersic_dev, refit_on,
+                                sync)
         results = run_island_batch(chunk, trees, cfg.max_steps, ds, ws,
                                    precision=cfg.precision, timer=timer, sersic_dev=sersic_dev,
                                    split=cfg.island_split, prerender=cfg.galaxy_prerender,
                                    post=cfg.post_fit or refit_on, gram=cfg.gram,
-                                   workspace_bytes=cfg.island_workspace)
+                                   workspace_bytes=cfg.island_workspace,
+                                   max_blocks=blocks.get(cfg.precision))
         redo = [(kern, host, rows[res["gram_unsettled"]])
                 for (kern, host, rows), res in zip(chunk, results) if "gram_unsettled" in res
                 and res["gram_unsettled"].any()]
@@ -741,7 +789,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
         if part64:
             results = results + fit_float64(part64, trees, cfg.max_steps, ds, ws,
                                              sersic_dev=sersic_dev, post=cfg.post_fit,
-                                             workspace_bytes=cfg.island_workspace, timer=timer)
+                                             workspace_bytes=cfg.island_workspace, timer=timer,
+                                             max_blocks=blocks.get("fp64"))
             chunk = chunk + part64
             n_fp64 += sum(int(r.size) for _, _, r in part64)
         if cfg.stall_probes and any("post_hg" in r for r in results):
