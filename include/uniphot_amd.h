@@ -452,6 +452,37 @@ int uph_blank_response(int n_prob, int T, int bg_order, const int64_t* src_off,
                        double bg_yo, double bg_ys, const float* phi, float* work,
                        size_t work_floats, int max_pix, double* b_hat, void* stream);
 
+/*
+ * Confusion-noise flux error of the post-fit error model (replaces
+ * _confusion_band_numeric / compute_grid_confusion, tract7dt/fit/grid_gpu/
+ * error_model.py:268-325, :663-930).  Problem q = one (grid, band) window
+ * win[q] = (band, x0, y0, h, w) of the (n_bands, H, W) images with sources
+ * [src_off[q], src_off[q+1]) (n_src in all): the model is flux x unit stamp
+ * (T x T at ix_top, iy_top) plus the background bg[q] (basis as in
+ * uph_band_solve), wing images subtracted from the data where use_wing[q];
+ * residual r = d - model on good pixels (0 elsewhere).  With phi[q] the
+ * unit-sum band PSF at the window centre (sumsq[q] = sum(phi^2) in float32):
+ * MF = corr(r, phi) / corr(good, phi^2) and Vstat = corr(var, phi^2) /
+ * corr(good, phi^2)^2 ('same' correlations; var = 1/invvar on good pixels
+ * with invvar > 0) where corr(good, phi^2) > 0.6 sumsq (valid).  For source s
+ * at window pixel (xi[s], yi[s]): the float64 means of MF, MF^2 and Vstat over
+ * the valid pixels of the square annulus radii[q] = (r_in, r_out) (clipped to
+ * the window; NaN below 50 pixels), excess = max(0, <MF^2> - <MF>^2 -
+ * <Vstat>), and sigma[s] = sqrt(R excess), R = 2 g^2 / ((g + sp2) sp2),
+ * g = sp2[q] + sgal2[s]; NaN where no_fit[s].  work: uph_confusion_work_floats
+ * (n_prob, max_pix) floats, max_pix >= every h * w; T up to 63 (LDS).
+ */
+size_t uph_confusion_work_floats(int n_prob, int max_pix);
+int uph_confusion(int n_prob, int64_t n_src, int T, int bg_order, const int64_t* src_off,
+                  const int32_t* win, int H, int W, const float* data, const float* invvar,
+                  const uint8_t* good, const float* wing, const uint8_t* use_wing,
+                  const float* stamps, const int32_t* ix_top, const int32_t* iy_top,
+                  const float* flux, const float* bg, double bg_xo, double bg_xs, double bg_yo,
+                  double bg_ys, const float* phi, const float* sumsq, const int32_t* radii,
+                  const double* sp2, const int32_t* xi, const int32_t* yi, const double* sgal2,
+                  const uint8_t* no_fit, float* work, size_t work_floats, int max_pix,
+                  float* sigma, void* stream);
+
 size_t uph_band_scratch_bytes(int n_prob, int max_src, int bg_order);
 int uph_band_solve(int n_prob, int T, int bg_order, int k_max, const int64_t* src_off,
                    const int32_t* win, int H, int W, int img_x0, int img_y0, const float* data,

@@ -549,6 +549,17 @@ def place(stamp: np.ndarray, ix_top: int, iy_top: int, B: int) -> np.ndarray:
     return out
 
 
+def place_window(stamp: np.ndarray, ix_top: int, iy_top: int, h: int, w: int) -> np.ndarray:
+    """(T,T) stamp -> (h,w) window at integer top-left offset, clipped."""
+    T = stamp.shape[0]
+    out = np.zeros((h, w))
+    y0, x0 = max(0, iy_top), max(0, ix_top)
+    y1, x1 = min(h, iy_top + T), min(w, ix_top + T)
+    if y1 > y0 and x1 > x0:
+        out[y0:y1, x0:x1] = stamp[y0 - iy_top:y1 - iy_top, x0 - ix_top:x1 - ix_top]
+    return out
+
+
 def stamp_top_left(x, y, x0, y0, T):
     """round(pos - origin) - T//2 (forced_phot_sparse.py:255-264)."""
     return (int(_round_half_even(x - x0)) - T // 2,

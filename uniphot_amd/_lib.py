@@ -24,6 +24,7 @@ EXPORTS = (
     "uph_wing_images", "uph_blank_work_floats", "uph_blank_response", "uph_stamp_sum",
     "uph_gather_windows", "uph_subtract_bg_planes", "uph_band_solve_noalloc",
     "uph_band_psf_flags", "uph_render_direct_scratch_bytes", "uph_render_stamps_direct",
+    "uph_confusion_work_floats", "uph_confusion",
 )
 UPH_BAND_NOALLOC_BYTES = 64 << 20
 
@@ -164,6 +165,12 @@ def lib() -> C.CDLL:
     L.uph_render_direct_scratch_bytes.argtypes = [C.c_int, C.c_int]
     L.uph_render_stamps_direct.restype = C.c_int
     L.uph_render_stamps_direct.argtypes = [C.c_int] * 3 + [_P] * 12 + [_P, C.c_size_t, _P]
+    L.uph_confusion_work_floats.restype = C.c_size_t
+    L.uph_confusion_work_floats.argtypes = [C.c_int, C.c_int]
+    L.uph_confusion.restype = C.c_int
+    L.uph_confusion.argtypes = ([C.c_int, C.c_int64, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int]
+                                + [_P] * 10 + [C.c_double] * 4 + [_P] * 9
+                                + [C.c_size_t, C.c_int, _P, _P])
     _lib = L
     return L
 

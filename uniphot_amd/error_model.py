@@ -1,0 +1,166 @@
+"""Post-fit flux-error model: the confusion term and the error columns.
+
+Mirrors ``tract7dt/fit/grid_gpu/error_model.py`` for the confusion
+measurement of each (grid, band) after pass 2 (``compute_grid_confusion``,
+:663-930, and its per-band numerics ``_confusion_band_numeric``,
+:268-325): the residual of the converged model (fluxes x unit stamps,
+the big sources' wings, the background plane) is matched-filtered with the
+band's central PSF and the variance of that map in an annulus around each
+source, above the statistical expectation, is the source's confusion
+variance, scaled by the extended-template factor.  The device does the
+whole per-problem computation (``uph_confusion``).
+
+``add_error_columns`` follows ``error_columns.py:112-363`` for the flux
+columns: FLUXERR_CONF_{b}_fit, FLUXERR_TOT_{b}_fit =
+sqrt(FLUXERR^2 + CONF^2) and FLUXERR_MARG_TOT_{b}_fit =
+sqrt(FLUXERR^2 + dvar + CONF^2) x sqrt(max(1, rchisq)).  The correlated-
+noise factor and the per-island chi2_nu (which only scale the geometric
+error columns and the marginal increment) and the source-Poisson term
+(which needs the images' gain) are not part of this build: their factors
+are 1, as the reference's are when those legs are off.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import forced_phot_sparse as fps
+from .source_model import SOURCE_STAR
+
+OCC_MIN = 0.6
+R_IN_FWHM, R_OUT_FWHM = 2.0, 6.0
+GAUSS_HALF_LIGHT = 1.1774
+MAX_TEMPLATE = 63  # largest stamp the correlation launch stages in LDS
+
+
+def psf_scale(phi32: np.ndarray):
+    """sum(phi^2) (float32), the Gaussian-equivalent PSF variance and the
+    annulus radii ceil(2 FWHM), ceil(6 FWHM) (error_model.py:293-300)."""
+    sumsq = np.float32(np.sum(np.square(phi32, dtype=np.float32), dtype=np.float32))
+    sp2 = 1.0 / (4.0 * math.pi * float(sumsq))
+    fwhm = 2.355 * math.sqrt(sp2)
+    return sumsq, sp2, int(math.ceil(R_IN_FWHM * fwhm)), int(math.ceil(R_OUT_FWHM * fwhm))
+
+
+def confusion_sigma(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn, cfg,
+                    bg_geo, wing=None, use_wing=None):
+    """sigma_conf of every fit source of each (grid, band) problem after the
+    final solve ``res`` (a list of per-problem arrays, NaN where the annulus
+    is too masked or the position is not finite).  ``wing`` (nb, H, W) is
+    subtracted from the data of the problems flagged in ``use_wing``."""
+    import torch
+    from . import _lib
+    from .pipeline import _pad_center
+    from .psf_polynomial import evaluate_psf_at_np
+    if T > MAX_TEMPLATE:
+        raise ValueError(f"confusion: stamp {T} px exceeds the correlation's {MAX_TEMPLATE} px")
+    L = _lib.lib()
+    P = _lib.ptr
+    dev = fld.data.device
+    nb, H, W = fld.nb, fld.H, fld.W
+    up = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+    cs, gs = fld.psf(T)
+    out = [None] * len(problems)
+    n_of = np.array([owners[q][0].fit_indices.size for q in range(len(problems))])
+    cost = n_of * T * T * 4.0 + np.array([pr.h * pr.w * 24.0 for pr in problems])
+    cut = np.concatenate([[0], np.nonzero(np.diff(np.floor(np.cumsum(cost) /
+                                                         cfg.stamp_budget)))[0] + 1,
+                          [len(problems)]])
+    for a, z in zip(cut[:-1], cut[1:]):
+        qs = np.arange(a, z)
+        if qs.size == 0:
+            continue
+        src = np.concatenate([owners[q][0].fit_indices for q in qs]).astype(np.int64)
+        counts = n_of[qs]
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+        band = np.repeat([owners[q][1] for q in qs], counts).astype(np.int32)
+        ox = np.repeat([problems[q].x0 for q in qs], counts).astype(np.float32)
+        oy = np.repeat([problems[q].y0 for q in qs], counts).astype(np.float32)
+        ext = stype[src] != SOURCE_STAR
+        stamps = fps.render_unit_stamps(
+            cs, gs, T, fld.n_terms, band, x[src], y[src], ox, oy, stype[src],
+            np.where(ext, logre[src], 0.0), np.where(ext, ee1[src], 1.0),
+            np.where(ext, ee2[src], 0.0), sn[src])
+        ix, iy = fps.stamp_top_left(x[src], y[src], ox, oy, T, T)
+        flux = np.concatenate([res[q].fluxes for q in qs]).astype(np.float32)
+        flux = np.where(np.isfinite(flux), flux, 0.0).astype(np.float32)
+        bg = np.stack([res[q].bg_coefs for q in qs]).astype(np.float32)
+        win = np.array([[owners[q][1], problems[q].x0, problems[q].y0, problems[q].h,
+                         problems[q].w] for q in qs], np.int32)
+        fps.check_windows(win, nb, H, W, "confusion_sigma")
+        phi = np.empty((qs.size, T, T), np.float32)
+        sumsq = np.empty(qs.size, np.float32)
+        sp2 = np.empty(qs.size, np.float64)
+        radii = np.empty((qs.size, 2), np.int32)
+        for j, q in enumerate(qs):
+            pr = problems[q]
+            t = evaluate_psf_at_np(fld.psfs[owners[q][1]], pr.x0 + 0.5 * pr.w,
+                                   pr.y0 + 0.5 * pr.h).astype(np.float32)
+            t = _pad_center(t, T).astype(np.float32)
+            phi[j] = t / np.sum(t, dtype=np.float32)
+            sumsq[j], sp2[j], radii[j, 0], radii[j, 1] = psf_scale(phi[j])
+        # rounded window-local source pixel, clipped into the window
+        wq = np.repeat([problems[q].w for q in qs], counts)
+        hq = np.repeat([problems[q].h for q in qs], counts)
+        with np.errstate(invalid="ignore"):
+            xi = np.clip(np.rint(x[src].astype(np.float32) - ox), 0, wq - 1)
+            yi = np.clip(np.rint(y[src].astype(np.float32) - oy), 0, hq - 1)
+        no_fit = ~np.isfinite(x[src]) | ~np.isfinite(y[src])
+        xi = np.where(no_fit, 0, xi).astype(np.int32)
+        yi = np.where(no_fit, 0, yi).astype(np.int32)
+        lr = np.asarray(logre[src], np.float64)
+        re = np.where(ext & np.isfinite(lr), np.exp(np.clip(lr, -5.0, 5.0)), 0.0)
+        sgal2 = (re / GAUSS_HALF_LIGHT) ** 2
+        max_pix = int(max(problems[q].h * problems[q].w for q in qs))
+        nw = int(L.uph_confusion_work_floats(int(qs.size), max_pix))
+        work = torch.empty(max(nw, 1), dtype=torch.float32, device=dev)
+        sig = torch.empty(max(int(src.size), 1), dtype=torch.float32, device=dev)
+        uw = (np.asarray(use_wing, np.uint8)[qs] if use_wing is not None
+              else np.zeros(qs.size, np.uint8))
+        keep = [up(off, np.int64), up(win, np.int32), up(ix, np.int32), up(iy, np.int32),
+                up(flux, np.float32), up(bg, np.float32), up(phi, np.float32),
+                up(uw, np.uint8), up(sumsq, np.float32), up(radii, np.int32),
+                up(sp2, np.float64), up(xi, np.int32), up(yi, np.int32),
+                up(sgal2, np.float64), up(no_fit, np.uint8)]
+        _lib.check(L.uph_confusion(
+            int(qs.size), int(src.size), T, cfg.bg_order, P(keep[0]), P(keep[1]), H, W,
+            P(fld.data), P(fld.invvar), P(fld.good), P(wing) if wing is not None else None,
+            P(keep[7]), P(stamps), P(keep[2]), P(keep[3]), P(keep[4]), P(keep[5]),
+            *map(float, bg_geo), P(keep[6]), P(keep[8]), P(keep[9]), P(keep[10]),
+            P(keep[11]), P(keep[12]), P(keep[13]), P(keep[14]), P(work), nw, max_pix, P(sig),
+            _lib.stream_ptr(None)))
+        s = sig.cpu().numpy()
+        for j, q in enumerate(qs):
+            out[q] = s[off[j]:off[j + 1]].copy()
+    return out
+
+
+def add_error_columns(out, bandnames, flux_err_conf, *, flux_marg_var=None,
+                      rchisq_scale_marg_tot: bool = True) -> None:
+    """FLUXERR_CONF / FLUXERR_TOT / FLUXERR_MARG_TOT per band, in place
+    (error_columns.py:134-150, :250-363 with the correlated-noise,
+    chi2_nu and source-Poisson factors at their neutral values).
+    ``flux_err_conf`` (n, nb); ``flux_marg_var`` (n, nb) the shape-
+    marginalisation increment (0 for stars, NaN off-island)."""
+    import pandas as pd
+    num = lambda c: pd.to_numeric(out[c], errors="coerce").to_numpy(dtype=np.float64)
+    conf = np.asarray(flux_err_conf, np.float64)
+    n = len(out)
+    scale = None
+    if rchisq_scale_marg_tot and "rchisq" in out.columns:
+        rq = num("rchisq")
+        scale = np.sqrt(np.where(np.isfinite(rq), np.maximum(rq, 1.0), 1.0))
+    for b, bn in enumerate(bandnames):
+        fe = num(f"FLUXERR_{bn}_fit")
+        fc = conf[:, b]
+        out[f"FLUXERR_CONF_{bn}_fit"] = fc.astype(np.float32)
+        fc2 = np.where(np.isfinite(fc), fc * fc, 0.0)
+        out[f"FLUXERR_TOT_{bn}_fit"] = np.sqrt(fe * fe + fc2).astype(np.float32)
+        if flux_marg_var is None:
+            continue
+        dv = np.asarray(flux_marg_var, np.float64)[:, b] if n else np.zeros(0)
+        tot = np.where(dv == 0.0, np.sqrt(fe * fe + fc2), np.sqrt(fe * fe + dv + fc2))
+        if scale is not None:
+            tot = tot * scale
+        out[f"FLUXERR_MARG_TOT_{bn}_fit"] = tot.astype(np.float32)

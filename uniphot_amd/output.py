@@ -19,8 +19,11 @@ shape-marginalised error columns of ``error_columns.py`` (:255-341):
     (:mod:`uniphot_amd.band_flags`, ``pipeline.py:3084-3130``): no-data
     cells blanked, ``FLAGS_{b}_fit`` and ``n_bands_usable``;
   * ``fit_is_reliable`` / ``fit_quality_issues`` (``merge.py:16-344``).
-The error-model totals (confusion, correlated noise, FLUXERR_TOT and the
-*_err_tot family) are not part of this build.
+  * the error model's flux columns (FLUXERR_CONF, FLUXERR_TOT,
+    FLUXERR_MARG_TOT; :mod:`uniphot_amd.error_model`) when the fit carries
+    the confusion term.
+The correlated-noise and chi2_nu scalings of the geometric errors (the
+*_err_corr / *_err_tot family) are not part of this build.
 """
 from __future__ import annotations
 
@@ -169,6 +172,10 @@ def write_fit_columns(catalog, fit, bandnames, *, x_init, y_init, source_type,
     has_fit = np.isfinite(out["x_pix_white_fit"].to_numpy(float))
     out["flag_slow_converge"] = slow_converge_flag(out["opt_niters"].to_numpy(), has_fit,
                                                    max_steps_main, max_steps)
+    if getattr(fit, "flux_err_conf", None) is not None:
+        from .error_model import add_error_columns
+        add_error_columns(out, list(bandnames), fit.flux_err_conf,
+                          flux_marg_var=fit.flux_marg_var if marginal_flux_err else None)
     if band_psfs is not None:
         from .band_flags import apply_band_flags
         out.attrs["band_flags"] = apply_band_flags(out, list(bandnames), list(band_psfs),

@@ -27,6 +27,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import forced_phot_sparse as fps
+from .error_model import confusion_sigma
 from .grid import assign_sources_to_grids, divide_image_into_grids
 from .island_plan import (DEFAULT_MAX_STEPS, JOINT_NSRC_CAP, class_host_arrays, free_n_gate,
                           plan_island_classes)
@@ -87,6 +88,9 @@ class FitConfig:
     # autotune off
     island_blocks: dict | None = None
     autotune: bool = True
+    # post-fit error model: confusion flux error of every source after
+    # pass 2 (error_model.py:663-930, on by default in the reference)
+    confusion: bool = True
 
 
 @dataclass
@@ -123,6 +127,7 @@ class FitResult:
     re_upper_pix: np.ndarray = None       # (n,) fitted-Re bound 0.75 x the island's tier
     free_n: np.ndarray = None             # (n,) Sersic n fitted
     grid_id: np.ndarray = None            # (n,) grid owning the source
+    flux_err_conf: np.ndarray = None      # (n, nb) confusion flux error (error model)
 
 
 class DeviceField:
@@ -315,7 +320,7 @@ def _pad_center(t, T):
 
 
 def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
-              cfg: FitConfig, bg_geo, timer=None, blank=False):
+              cfg: FitConfig, bg_geo, timer=None, blank=False, post=None):
     """Wing peeling of a grid pass at the native stamp (solver_grid.py:
     2495-2546, :2826-2846): per grid, the big sources' wings -- their S1
     flux times the tier-wing_tier render minus the native render -- are
@@ -325,7 +330,9 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
     (``uph_wing_images``, grids of one colour at a time so windows are
     disjoint), solved, and their fluxes and background subtracted.  With
     ``blank`` (pass 2) the blank-response correction follows, against data
-    minus each grid's own wing.  Returns (results, problems corrected)."""
+    minus each grid's own wing; ``post(sel, results, wing)`` (pass 2) is then
+    called with the corrected results of the local problems ``sel`` and the
+    wing image they saw.  Returns (results, problems corrected)."""
     import torch
     from . import _lib
     done = np.zeros(len(problems), bool)
@@ -400,17 +407,24 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
             for q, r in zip(sel, corr):
                 out[q] = r
             done[sel] = True
+        if post is not None:
+            post(sel, [out[q] for q in sel], wing)
+            done[sel] = True
     return out, done
 
 
 def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: FitConfig,
-                gtier: dict, timer=None, final=False, bg_geo=None, with_bg_var=False):
+                gtier: dict, timer=None, final=False, bg_geo=None, with_bg_var=False,
+                conf_out=None):
     """Per-grid, per-band linear solve; returns fluxes/var (n, nb) for each
     source from the grid that owns it, and bg coefs (n_grid, nb, M).  The
     grids, their fit sets and stamp tiers are those of pass 1 (pass 2 only
     replaces the source parameters, island_stage.py:3331-3363).  ``bg_geo``
     (x origin, x scale, y origin, y scale) of the background basis defaults
-    to the field's centre and half-size (pipeline.py:1693-1696)."""
+    to the field's centre and half-size (pipeline.py:1693-1696).  With
+    ``final`` and ``conf_out`` (an (n, nb) array) the confusion flux error
+    of each owned source is written there (error_model.confusion_sigma,
+    after the blank-response correction; native stamps only)."""
     n = x.size
     nb = fld.nb
     M = (cfg.bg_order + 1) * (cfg.bg_order + 2) // 2
@@ -426,6 +440,7 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
         by_tier.setdefault(gtier[g.grid_id], []).append(g)
     for T, gl in sorted(by_tier.items()):
         problems, owners = [], []
+        conf_q = {}
         for g in gl:
             fi = g.fit_indices
             nbr, _ = fps.build_neighbor_list(x[fi], y[fi], T, T, cfg.k_max)
@@ -452,21 +467,38 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
                                           bg_geo=bg_geo, reg=cfg.reg, k_max=cfg.k_max,
                                           timer=timer)
                 blank = final and cfg.blank_response and T == fld.native
+                conf = final and conf_out is not None and T == fld.native
                 done = np.zeros(len(part), bool)
+
+                def post(sel, rs, wing, a=a):
+                    sig = confusion_sigma(fld, T, [problems[a + q] for q in sel],
+                                          [owners[a + q] for q in sel], rs, x, y, stype, logre,
+                                          ee1, ee2, sn, cfg, bg_geo, wing=wing,
+                                          use_wing=(None if wing is None
+                                                    else np.ones(len(sel), np.uint8)))
+                    for q, sg in zip(sel, sig):
+                        conf_q[a + q] = sg
+
                 if cfg.wing_peeling:
                     part, done = wing_peel(fld, T, problems[a:z], owners[a:z], part, x, y,
                                            stype, logre, ee1, ee2, sn, cfg, bg_geo, timer,
-                                           blank=blank)
-                if blank and not done.all():
+                                           blank=blank, post=post if conf else None)
+                if (blank or conf) and not done.all():
                     rest = np.nonzero(~done)[0]
-                    corr, _ = blank_response_correct(
-                        fld, T, [problems[a + q] for q in rest], [owners[a + q] for q in rest],
-                        [part[q] for q in rest], x, y, stype, logre, ee1, ee2, sn, cfg, bg_geo)
-                    for q, r in zip(rest, corr):
-                        part[q] = r
+                    if blank:
+                        corr, _ = blank_response_correct(
+                            fld, T, [problems[a + q] for q in rest],
+                            [owners[a + q] for q in rest], [part[q] for q in rest], x, y,
+                            stype, logre, ee1, ee2, sn, cfg, bg_geo)
+                        for q, r in zip(rest, corr):
+                            part[q] = r
+                    if conf:
+                        post(list(rest), [part[q] for q in rest], None)
                 res += part
-        for (g, b), r in zip(owners, res):
+        for q, ((g, b), r) in enumerate(zip(owners, res)):
             own = g.source_indices.size
+            if conf_out is not None and q in conf_q:
+                conf_out[g.source_indices, b] = conf_q[q][:own]
             flux[g.source_indices, b] = r.fluxes[:own]
             var[g.source_indices, b] = r.flux_var[:own]
             bg[g.grid_id, b] = r.bg_coefs
@@ -918,8 +950,9 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     tm["island_fit"] = time.perf_counter() - t2
     # ---- pass 2 ----
     t3 = time.perf_counter()
+    conf = np.full((n, fld.nb), np.nan, np.float32) if cfg.confusion else None
     flux, var, _ = linear_pass(fld, grids, fx, fy, stype, flr, fe1, fe2, fsn, cfg, gtier, timer,
-                               final=True)
+                               final=True, conf_out=conf)
     sync()
     tm["pass2"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
@@ -933,6 +966,7 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                knobs_off=["priors"] + ([] if cfg.scene_subtract else ["scene_subtract"])
                + ([] if cfg.blank_response else ["blank_response_correction"])
                + ([] if cfg.wing_peeling else ["wing_peeling"])
+               + ([] if cfg.confusion else ["confusion"])
                + ([] if cfg.post_fit else ["param_errors", "source_metrics",
                                            "marginal_flux_err"]))
     if keep_artifacts:
@@ -947,7 +981,7 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                      stall & (isf.stall_dchi2 > -2.0 * PLATEAU_DLNP_TOL), isf.fp64_refit,
                      flux_var=var, flux_marg_var=isf.flux_marg_var,
                      re_upper_pix=isf.re_upper_pix, free_n=np.asarray(free_n, bool),
-                     grid_id=owner)
+                     grid_id=owner, flux_err_conf=conf)
 
 
 def psf_bank(psfs, device="cuda"):
