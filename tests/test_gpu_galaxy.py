@@ -187,6 +187,35 @@ def test_large_island_gram_launch_matches_in_kernel_gram(cuda):
             assert np.allclose(va, vb, rtol=1e-4, atol=1e-12, equal_nan=True)
 
 
+def test_fp64_large_island_gram_launch_matches_in_kernel_gram(cuda):
+    """The float64 sequence with every island's Gram on the many-workgroup
+    float64 Gram launch (threshold 0) against the evaluation's own float64
+    matrix-core Gram: same steps and flags, fluxes to float64 rounding
+    (islands above 16 columns sum the same products in the same order;
+    smaller ones are split over pixels in the evaluation)."""
+    import torch
+    from uniphot_amd.island_solver import make_class_kernels, psf_tree_device, run_island_batch
+    from uniphot_amd.sersic_table import device_table
+    sc = small_mixed_scene(seed=21, n_src=24)
+    plans, hosts, pad, d, w = mixed_classes(sc, free_n=(sc.stype == 3))
+    batch = [(make_class_kernels(p, d.shape[0], sc.psfs[0].n_terms, precision="fp64"), h,
+              np.arange(p.n_islands)) for p, h in zip(plans, hosts)]
+    trees = {T: psf_tree_device(sc.psfs, T, cuda) for T in {p.tier for p in plans}}
+    args = (batch, trees, 30, torch.from_numpy(d).to(cuda), torch.from_numpy(w).to(cuda))
+    kw = dict(sersic_dev=device_table(cuda), split=True, prerender=True, precision="fp64")
+    ref = run_island_batch(*args, gram_split_min=float("inf"), **kw)
+    got = run_island_batch(*args, gram_split_min=0.0, **kw)
+    for (plan, host), ra, rb in zip(zip(plans, hosts), ref, got):
+        for k in ("nst", "conv", "stall", "n_good"):
+            assert np.array_equal(ra[k], rb[k]), k
+        for r in range(plan.n_islands):
+            mm = host["mmask"][r]
+            assert np.abs(ra["p"][r][mm] - rb["p"][r][mm]).max() < 1e-5
+            fa, fb = ra["fluxes"][r][:, mm], rb["fluxes"][r][:, mm]
+            scale = np.maximum(np.abs(fa), ra["ferr"][r][:, mm])
+            assert (np.abs(fa - fb) / scale).max() < 1e-7
+
+
 @pytest.mark.parametrize("n_range", [(1.0, 3.0), (0.4, 0.9)])
 def test_fp64_prerender_equals_inplace(cuda, n_range):
     """The float64 member pre-render (spectra + (member, band) planes

@@ -33,15 +33,19 @@ _hash = None
 _log = []
 
 
+# the sources the island launch sequences are built from
+ISLAND_SOURCES = ("island.hip", "prerender.hip", "prerender64.hip", "uph_device.hpp",
+                  "uph_galaxy.hpp", "uph_island.hpp", "uph_prerender.hpp")
+
+
 def kernel_hash() -> str:
-    """Hash of the HIP sources the launches run (16 hex digits)."""
+    """Hash of the HIP sources the tuned launches run (16 hex digits)."""
     global _hash
     if _hash is None:
         h = hashlib.sha256()
-        for p in sorted((HERE / "csrc").glob("*")):
-            if p.suffix in (".hip", ".hpp"):
-                h.update(p.name.encode())
-                h.update(p.read_bytes())
+        for name in ISLAND_SOURCES:
+            h.update(name.encode())
+            h.update((HERE / "csrc" / name).read_bytes())
         _hash = h.hexdigest()[:16]
     return _hash
 

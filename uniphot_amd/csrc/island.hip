@@ -810,7 +810,12 @@ __device__ double eval_band(const IslandArgs& a, IslandSh<Real>& sh, int isl, in
     else
       gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
   } else if constexpr (kPre) {
-    gram_mfma_f64(Wp, planes, P, C, 2, G, part);
+    const IslandArgsPre& ap = static_cast<const IslandArgsPre&>(a);
+    const int slot = ap.big_n > 0 ? ap.big_slot[isl] : -1;
+    if (slot >= 0)
+      gram_from_pairs(ap.big_part + ((size_t)slot * a.nb + b) * ap.big_ntp * 256, C, G);
+    else
+      gram_mfma_f64(Wp, planes, P, C, 2, G, part);
   } else {
     gram<Real>(Wp, planes, P, C, 2, G, part, sh.red);
   }
@@ -1468,6 +1473,118 @@ __global__ __launch_bounds__(UPH_BLOCK) void island_gram_kernel(IslandArgsPre a)
   for (int r = 0; r < 4; ++r) pt[(4 * grp + r) * 16 + col] = accd[r];
 }
 
+// The float64 form of island_gram_kernel for the float64 band-split
+// sequence: columns read as doubles from the float64 member planes (data and
+// weights widened from the float32 stacks), products on
+// v_mfma_f64_16x16x4f64 with the accumulation chain of gram_mfma_f64 (one
+// unit per tile pair over all pixels, the pixels of a K-step in the same
+// order), so the Gram equals the evaluation's own.  The float64 result tile
+// comes back as D[4 r + l / 16][l % 16] in register r of lane l.
+__global__ __launch_bounds__(UPH_BLOCK) void island_gram64_kernel(IslandArgsPre a) {
+  __shared__ IslandSh<float> sh;
+  __shared__ long long m_blk[kMaxMembers];
+  __shared__ int m_box[kMaxMembers][5];
+  __shared__ int c_src[kMaxCols];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int q = blockIdx.x, b = q % a.nb, isl = a.big_isl[q / a.nb];
+  const int mode = a.mode;
+  if ((mode == kSplitEvalGrad || mode == kSplitEvalTry) && a.conv_out[isl]) return;  // uniform
+  if (tid == 0) read_shape(a, sh, isl);
+  __syncthreads();
+  const float* psrc = mode == kSplitEvalInit ? a.p0 : (mode == kSplitEvalTry ? a.p_try : a.p_out);
+  describe_island(a, sh, psrc);
+  const bool grads = split_grads(mode);
+  const int Nr = sh.Nr, nL = grads ? sh.nL : 0, C = 1 + Nr + nL;
+  const int nt = (C + 15) >> 4, ntp = nt * (nt + 1) / 2;
+  if (4 * (int)blockIdx.y >= ntp || Nr == 0) return;  // uniform
+  const int B = sh.B, T = sh.T, P = B * B;
+  const float x0f = a.x0b[isl], y0f = a.y0b[isl];
+  const double* pb = a.pbuf64;
+  for (int r = tid; r < Nr; r += UPH_BLOCK) {
+    const float* p6 = sh.p + 6 * sh.mem[r];
+    const int ix = int(rint(double(p6[0] - x0f))) - T / 2;
+    const int iy = int(rint(double(p6[1] - y0f))) - T / 2;
+    const int side = pre_side(T, B);
+    m_box[r][0] = max(0, ix);
+    m_box[r][1] = min(B, ix + T);
+    m_box[r][2] = max(0, iy);
+    m_box[r][3] = min(B, iy + T);
+    m_box[r][4] = side;
+    const int g = a.pre_of_slot[(size_t)sh.moff + sh.mem[r]];
+    m_blk[r] = (long long)a.pre_off[g] +
+               (long long)b * pre_planes(sh.st[r] == 0) * side * side;
+  }
+  for (int c = tid; c < C; c += UPH_BLOCK) {
+    if (c == 0) {
+      c_src[c] = -1;
+    } else if (c <= Nr) {
+      c_src[c] = (c - 1) * 8;
+    } else {
+      const int l = c - 1 - Nr;
+      c_src[c] = sh.live_r[l] * 8 + 1 + sh.live_j[l];
+    }
+  }
+  __syncthreads();
+  const int tp = 4 * blockIdx.y + wid;
+  if (tp >= ntp) return;  // no barrier follows
+  int ti, tj;
+  pair_of(tp, nt, ti, tj);
+  const int col = lane & 15, grp = lane >> 4;
+  const int ca = ti * 16 + col, cb = tj * 16 + col;
+  const int sa = ca < C ? c_src[ca] : -2, sb = cb < C ? c_src[cb] : -2;
+  const int ox = a.ox[isl], oy = a.oy[isl];
+  const float* drow = a.dstack + ((size_t)b * a.Hp + oy) * a.Wp + ox;
+  const float* wrow = a.wstack + ((size_t)b * a.Hp + oy) * a.Wp + ox;
+  auto column4 = [&](int s, int wr, int wc, double (&v)[4]) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) v[t] = 0.0;
+    if (s == -2) return;
+    if (s == -1) {
+      const float* d = drow + (size_t)wr * a.Wp + wc;
+#pragma unroll
+      for (int t = 0; t < 4; ++t) v[t] = double(d[t]);
+      return;
+    }
+    const int r = s >> 3, pl = s & 7;
+    const int* bx = m_box[r];
+    if (wr < bx[2] || wr >= bx[3]) return;
+    const int side = bx[4];
+    const double* src = pb + m_blk[r] + (size_t)pl * side * side +
+                        (size_t)(wr - bx[2]) * side - bx[0];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int c = wc + t;
+      if (c >= bx[0] && c < bx[1]) v[t] = src[c];
+    }
+  };
+  const int KS = P >> 4;
+  uph_d4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int ks = 0; ks < KS; ++ks) {
+    const int px = ks * 16 + 4 * grp;
+    const int wr = px / B, wc = px % B;
+    double xa[4], xb[4], wv[4];
+    const float* w = wrow + (size_t)wr * a.Wp + wc;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) wv[t] = double(w[t]);
+    column4(sa, wr, wc, xa);
+    if (ti == tj) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) xb[t] = xa[t];
+    } else {
+      column4(sb, wr, wc, xb);
+    }
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[0], wv[0] * xb[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[1], wv[1] * xb[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[2], wv[2] * xb[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(xa[3], wv[3] * xb[3], acc, 0, 0, 0);
+  }
+  double* pt = a.big_part + (((size_t)(q / a.nb) * a.nb + b) * a.big_ntp + tp) * 256;
+  pt[(0 + grp) * 16 + col] = acc.x;
+  pt[(4 + grp) * 16 + col] = acc.y;
+  pt[(8 + grp) * 16 + col] = acc.z;
+  pt[(12 + grp) * 16 + col] = acc.w;
+}
+
 template <typename Real>
 __global__ __launch_bounds__(UPH_BLOCK) void island_step_kernel(IslandArgs a) {
   __shared__ IslandSh<Real> sh;
@@ -1925,6 +2042,11 @@ static void split_pass(const uph::IslandArgs& a, const uph::IslandArgsPre* pre, 
     g.mode = mode;
     hipLaunchKernelGGL(uph::island_gram_kernel, dim3(g.big_n * g.nb, (g.big_ntp + 3) / 4),
                        dim3(UPH_BLOCK), 0, s, g);
+  } else if (sizeof(Real) == 8 && pre && pre->big_n > 0) {
+    uph::IslandArgsPre g = *pre;
+    g.mode = mode;
+    hipLaunchKernelGGL(uph::island_gram64_kernel, dim3(g.big_n * g.nb, (g.big_ntp + 3) / 4),
+                       dim3(UPH_BLOCK), 0, s, g);
   }
   rec(4);
   if (pre)
@@ -2010,9 +2132,9 @@ extern "C" int uph_island_fit_split(const uph_island_batch* bat, const uph_islan
     UPH_CHECK_ARG(sp->n_pre_ext >= 0 && sp->n_pre_ext <= sp->n_pre,
                   "uph_island_fit_split: n_pre_ext=%d outside [0, n_pre=%d]", sp->n_pre_ext,
                   sp->n_pre);
-    UPH_CHECK_ARG(!fp64 || (!sp->post && !sp->gram_bf16 && sp->big_n == 0),
-                  "uph_island_fit_split: the float64 pre-render takes no post-fit pass, "
-                  "bfloat16 Gram or large-island Gram launch");
+    UPH_CHECK_ARG(!fp64 || (!sp->post && !sp->gram_bf16),
+                  "uph_island_fit_split: the float64 pre-render takes no post-fit pass or "
+                  "bfloat16 Gram");
     for (int t = 0; t < bat->n_tiers; ++t) {
       if (fp64)
         UPH_CHECK_ARG(uph::prerender64_scratch(bat->tier_T[t]) <= scratch_per_block,

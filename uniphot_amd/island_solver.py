@@ -595,12 +595,11 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
             post_hblk = hblk
             post_args = (1, P(post_t["off"]), P(post_t["buf"]), int(post_t["buf"].numel()),
                          P(post_t["var"]), P(post_t["dvar"]), P(post_t["metrics"]))
-        # large islands: Gram on the many-workgroup launch (float32 Gram)
+        # large islands: Gram on the many-workgroup launch (float32 or float64)
         ncap = np.asarray([1 + s_[0] + s_[3] for s_ in shape], np.float64)
         npx = np.asarray([s_[1] * s_[1] for s_ in shape], np.float64)
         cost = ncap * ncap * npx
-        big = np.nonzero((cost >= gram_split_min) & (gal["args"][0] > 0) & (not bf16_on)
-                         & (precision != "fp64"))[0]
+        big = np.nonzero((cost >= gram_split_min) & (gal["args"][0] > 0) & (not bf16_on))[0]
         # largest first, while the tile-pair buffer stays within its budget
         big = big[np.argsort(-cost[big], kind="stable")]
         ntp_of = ((ncap[big].astype(np.int64) + 15) // 16)
@@ -747,7 +746,8 @@ def prepare_island_batch(classes, psf_trees: dict, max_steps: int, dstack_dev, w
                 bsel = np.zeros(n_isl, bool)
                 bsel[big] = True
                 bsel &= sel
-                timer.record("island_gram_kernel", e5[3], e5[4],
+                timer.record("island_gram64_kernel" if precision == "fp64" else
+                             "island_gram_kernel", e5[3], e5[4],
                              dict(kind="island_eval", phase=kind,
                                   bytes=float(box[bsel].sum()),
                                   flops=float(gram_fl[bsel].sum()), n_isl=int(bsel.sum())))
