@@ -154,7 +154,7 @@ def measured_traffic(workload, kernel):
 
 
 def make_scene(config):
-    from uniphot_amd import synth
+    from scenes import field as synth
     if config == "c2-stars":
         return synth.scene_c2(star_only=True), dict(n_rows=8, n_cols=8)
     if config == "c2":
@@ -454,7 +454,7 @@ def bench_cutouts(args, torch):
     """Batching workload: independent island cutouts fitted in band-split
     launch sequences; the step is the whole batch's island fit."""
     from uniphot_amd.pipeline import FitConfig, cutout_accuracy, fit_cutouts, psf_bank
-    from uniphot_amd.synth_cutouts import scene_cutouts
+    from scenes.cutouts import scene_cutouts
     sync = torch.cuda.synchronize
     t_scene = time.perf_counter()
     cut = scene_cutouts(n_isl=args.islands, n_bands=10, device="cuda")
@@ -520,7 +520,7 @@ def bench_serving(args, torch):
     before the stream starts.  value = sources served per second of
     makespan; p50/p99 request latency beside it."""
     from uniphot_amd.serving import TileServer, latency_summary
-    from uniphot_amd.synth_tiles import PsfBank, make_tile, tile_stream
+    from scenes.tiles import PsfBank, make_tile, tile_stream
     t_scene = time.perf_counter()
     reqs, bank = tile_stream(n_req=args.requests, rate=args.rate)
     torch.cuda.synchronize()
@@ -559,8 +559,8 @@ def bench_precision(args, torch):
     both against the float64 island fit on a sample of islands."""
     import dataclasses
     from uniphot_amd.pipeline import DeviceField, FitConfig, fit_field, island_accuracy
-    from uniphot_amd.synth import SEEDS
-    from uniphot_amd.synth_tiles import PsfBank, make_tile
+    from scenes.field import SEEDS
+    from scenes.tiles import PsfBank, make_tile
     sync = torch.cuda.synchronize
     t_scene = time.perf_counter()
     bank = PsfBank(SEEDS["c4"], n_bands=40)

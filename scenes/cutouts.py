@@ -20,10 +20,10 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .islands import ISLAND_BBOX_CLASSES, IslandSet
-from .sersic_table import mog_params_at
-from .source_model import SOURCE_DEV, SOURCE_EXP, SOURCE_SERSIC, SOURCE_STAR
-from .synth import SEEDS, MixturePsfField, _cov_terms, _galaxy_shapes, _seed_catalog
+from uniphot_amd.islands import ISLAND_BBOX_CLASSES, IslandSet
+from uniphot_amd.sersic_table import mog_params_at
+from uniphot_amd.source_model import SOURCE_DEV, SOURCE_EXP, SOURCE_SERSIC, SOURCE_STAR
+from .field import SEEDS, MixturePsfField, _cov_terms, _galaxy_shapes, _seed_catalog
 
 
 @dataclass

@@ -24,9 +24,9 @@ from dataclasses import dataclass, field
 import numpy as np
 from scipy.special import erf
 
-from .psf_polynomial import PsfPolynomial, terms_for_order
-from .sersic_table import mog_params_at
-from .source_model import (SOURCE_DEV, SOURCE_EXP, SOURCE_SERSIC, SOURCE_STAR,
+from uniphot_amd.psf_polynomial import PsfPolynomial, terms_for_order
+from uniphot_amd.sersic_table import mog_params_at
+from uniphot_amd.source_model import (SOURCE_DEV, SOURCE_EXP, SOURCE_SERSIC, SOURCE_STAR,
                            ellipse_soft_from_re_ab_phi_np)
 
 SEEDS = dict(c1=20261201, c2=20261202, c3=20261203, c4=20261204, c5=20261205)

@@ -14,10 +14,10 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .psf_polynomial import PsfPolynomial
-from .source_model import SOURCE_DEV, SOURCE_EXP, SOURCE_SERSIC, SOURCE_STAR
-from .synth import SEEDS, MixturePsfField, _cov_terms, _galaxy_shapes, _seed_catalog
-from .synth_cutouts import galaxy_mixtures, render_profiles
+from uniphot_amd.psf_polynomial import PsfPolynomial
+from uniphot_amd.source_model import SOURCE_DEV, SOURCE_EXP, SOURCE_SERSIC, SOURCE_STAR
+from .field import SEEDS, MixturePsfField, _cov_terms, _galaxy_shapes, _seed_catalog
+from .cutouts import galaxy_mixtures, render_profiles
 
 
 class PsfBank:

@@ -4,7 +4,7 @@ from __future__ import annotations
 import numpy as np
 
 from oracle import island as oisl
-from uniphot_amd import synth
+from scenes import field as synth
 from uniphot_amd.island_plan import class_host_arrays, free_n_gate, plan_island_classes
 from uniphot_amd.islands import ISLAND_BBOX_CLASSES, ISLAND_NSRC_CLASSES, bucket_islands, build_islands
 from uniphot_amd.source_model import SOURCE_STAR

@@ -162,7 +162,7 @@ def test_c1_field_in_full_against_the_oracle(cuda):
     """1 band, 256^2, 40 STAR + 10 EXP, 8 px linking, 128^2 background
     tiles: every island's fit and every pass-2 grid solve."""
     from uniphot_amd import forced_phot_sparse as fps
-    from uniphot_amd import synth
+    from scenes import field as synth
     from uniphot_amd.pipeline import _pad_center, wing_sources
     from uniphot_amd.psf_polynomial import evaluate_psf_at_np
     sc = synth.scene_c1()
@@ -239,7 +239,7 @@ def test_field_island_sample_against_the_oracle(cuda, n_bands, side, n_src, budg
     fluxes 1e2-1e6, quadratic PSF variation, 256^2 background tiles) at the
     workloads' density; one island of every class first, then random ones,
     within a time budget of the oracle."""
-    from uniphot_amd import synth
+    from scenes import field as synth
     sc = synth.scene_c2(H=side, W=side, n_bands=n_bands, n_src=n_src)
     print(f"field {side}^2 x {n_bands} bands, {sc.x.size} sources", flush=True)
     fld, cfg, res = _fit_field(sc, n_rows=side // 256, n_cols=side // 256)
@@ -260,7 +260,7 @@ def test_cutout_island_sample_against_the_oracle(cuda):
     batch; a sample of the islands against the oracle."""
     import torch
     from uniphot_amd.pipeline import FitConfig, fit_cutouts
-    from uniphot_amd.synth_cutouts import scene_cutouts
+    from scenes.cutouts import scene_cutouts
     cut = scene_cutouts(seed=9, n_isl=400, n_bands=10, device="cuda", mosaic_w=4096)
     res = fit_cutouts(cut, FitConfig(), sync=torch.cuda.synchronize)
     d = cut.dstack.cpu().numpy().astype(np.float64)
@@ -283,7 +283,7 @@ def test_serving_stream_tiles_alone_equal_in_the_mix(cuda):
     1e-3 sources/px^2), first eight requests: each tile's result alone equals
     its result among the others in flight."""
     from uniphot_amd.serving import TileServer, latency_summary
-    from uniphot_amd.synth_tiles import tile_stream
+    from scenes.tiles import tile_stream
     reqs, bank = tile_stream(n_req=8, rate=50.0)
     print("tiles", [(r.side, r.n_bands, r.n_sources) for r in reqs], flush=True)
     alone = [TileServer(bank, max_inflight=1).serve([r], realtime=False)[0] for r in reqs]

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def cut(cuda):
-    from uniphot_amd.synth_cutouts import scene_cutouts
+    from scenes.cutouts import scene_cutouts
     print("cutout scene seed 5, 240 islands, 3 bands", flush=True)
     return scene_cutouts(seed=5, n_isl=240, n_bands=3, device="cuda", mosaic_w=4096)
 

@@ -9,7 +9,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def tiles(cuda):
-    from uniphot_amd.synth_tiles import PsfBank, make_tile
+    from scenes.tiles import PsfBank, make_tile
     bank = PsfBank(n_bands=8)
     rng = np.random.default_rng(11)
     shapes = [(256, 3), (512, 2), (256, 5), (512, 4)]

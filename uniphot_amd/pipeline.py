@@ -1002,7 +1002,7 @@ def fit_cutouts(cut, cfg: FitConfig, timer=None, sync=None, joint_nsrc_cap=None,
                 budget_bytes: float = 48e9, psf_of=None, island_ids=None,
                 p_start=None) -> IslandSetFit:
     """Batched island fit of independent island cutouts (each island in its
-    own zero-weight-bordered cell of a mosaic; see synth_cutouts): class
+    own zero-weight-bordered cell of a mosaic; see scenes/cutouts.py): class
     planning with every island fitted jointly up to 64 members
     (``joint_nsrc_cap=None``), then :func:`fit_island_set`.  With no linear
     pass before it, the free-n gate's S/N is the true flux over the

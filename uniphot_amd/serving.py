@@ -2,7 +2,7 @@
 admitted tile fitted on its own HIP stream so the work of tiles in flight
 overlaps on the device, PSF coefficients resident across requests.
 
-A request (``synth_tiles.TileRequest``) carries its images on the device,
+A request (``scenes.tiles.TileRequest``) carries its images on the device,
 its band PSFs (frames of the server's resident bank) and its catalog.  The
 server admits arrived requests in arrival order while the sum of their
 estimated device footprints stays within ``hbm_budget`` and fewer than
