@@ -51,8 +51,7 @@ def confusion_sigma(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2,
     subtracted from the data of the problems flagged in ``use_wing``."""
     import torch
     from . import _lib
-    from .pipeline import _pad_center
-    from .psf_polynomial import evaluate_psf_at_np
+    from .pipeline import window_psf_stamps
     if T > MAX_TEMPLATE:
         raise ValueError(f"confusion: stamp {T} px exceeds the correlation's {MAX_TEMPLATE} px")
     L = _lib.lib()
@@ -93,11 +92,8 @@ def confusion_sigma(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2,
         sumsq = np.empty(qs.size, np.float32)
         sp2 = np.empty(qs.size, np.float64)
         radii = np.empty((qs.size, 2), np.int32)
-        for j, q in enumerate(qs):
-            pr = problems[q]
-            t = evaluate_psf_at_np(fld.psfs[owners[q][1]], pr.x0 + 0.5 * pr.w,
-                                   pr.y0 + 0.5 * pr.h).astype(np.float32)
-            t = _pad_center(t, T).astype(np.float32)
+        for j, t in enumerate(window_psf_stamps(fld, problems, owners, qs, T)):
+            t = t.astype(np.float32)
             phi[j] = t / np.sum(t, dtype=np.float32)
             sumsq[j], sp2[j], radii[j, 0], radii[j, 1] = psf_scale(phi[j])
         # rounded window-local source pixel, clipped into the window

@@ -275,11 +275,8 @@ def blank_response_correct(fld, T, problems, owners, res, x, y, stype, logre, ee
                          problems[q].w] for q in qs], np.int32)
         fps.check_windows(win, nb, H, W, "blank_response_correct")
         phi = np.empty((qs.size, T, T), np.float32)
-        for j, q in enumerate(qs):
-            pr = problems[q]
-            t = np.clip(evaluate_psf_at_np(fld.psfs[owners[q][1]], pr.x0 + 0.5 * pr.w,
-                                           pr.y0 + 0.5 * pr.h).astype(np.float64), 0.0, None)
-            t = _pad_center(t, T)
+        for j, t in enumerate(window_psf_stamps(fld, problems, owners, qs, T)):
+            t = np.clip(t.astype(np.float64), 0.0, None)
             phi[j] = (t / t.sum()).astype(np.float32)
         max_pix = int(max(problems[q].h * problems[q].w for q in qs))
         nw = int(L.uph_blank_work_floats(int(qs.size), max_pix))
@@ -306,6 +303,22 @@ def blank_response_correct(fld, T, problems, owners, res, x, y, stype, logre, ee
             out[q] = fps.SparseForcedPhotResult(f, r0.bg_coefs, r0.flux_var, r0.bg_var,
                                                 r0.converged)
     return out, b_hat
+
+
+def window_psf_stamps(fld, problems, owners, qs, T):
+    """The band PSF at the centre of each window ``qs`` (float32, padded to
+    T x T), evaluated per band in one product."""
+    from .psf_polynomial import evaluate_psf_at_batch_np
+    out = [None] * len(qs)
+    bands = np.array([owners[q][1] for q in qs])
+    for b in np.unique(bands):
+        sel = np.nonzero(bands == b)[0]
+        xc = np.array([problems[qs[j]].x0 + 0.5 * problems[qs[j]].w for j in sel])
+        yc = np.array([problems[qs[j]].y0 + 0.5 * problems[qs[j]].h for j in sel])
+        st = evaluate_psf_at_batch_np(fld.psfs[int(b)], xc, yc)
+        for k, j in enumerate(sel):
+            out[j] = _pad_center(st[k], T)
+    return out
 
 
 def _pad_center(t, T):

@@ -55,6 +55,19 @@ def evaluate_psf_at_np(psf: PsfPolynomial, x: float, y: float) -> np.ndarray:
     return np.tensordot(psf.coefs.astype(np.float32), basis, axes=([-1], [-1])).astype(np.float32)
 
 
+def evaluate_psf_at_batch_np(psf: PsfPolynomial, xs, ys) -> np.ndarray:
+    """float32 stamps (n, H, W) at n positions in one product: basis in
+    float64 rounded to float32, coefficients (H W, n_t) times basis (n_t, n)
+    (psf_polynomial.py:236-270)."""
+    X = (np.asarray(xs, np.float64) - psf.x_origin) / psf.x_scale
+    Y = (np.asarray(ys, np.float64) - psf.y_origin) / psf.y_scale
+    terms = terms_for_order(ORDER_FOR_N_TERMS[psf.n_terms])
+    basis = np.stack([(X ** m * Y ** n).astype(np.float32) for (m, n) in terms])
+    H, W = psf.coefs.shape[:2]
+    flat = psf.coefs.reshape(H * W, psf.n_terms).astype(np.float32, copy=False)
+    return (flat @ basis).T.reshape(-1, H, W).astype(np.float32, copy=False)
+
+
 def pad_psf(psf: PsfPolynomial, tier: int) -> PsfPolynomial:
     """Zero-pad to a larger odd tier keeping the centre pixel at tier//2
     (island_solver.py:193-206, runner.py:327-350)."""
