@@ -63,7 +63,8 @@ def test_render_source_stamps_match_oracle(cuda):
     star = sc.stype == 0
     only = render_star_stamps(psf, sc.x[star], sc.y[star], 7, 11).cpu().numpy()
     assert np.array_equal(only, got[star])      # the STAR rows, bit for bit
-    with pytest.raises(NotImplementedError):
+    # the default backend is "direct", which takes STAR and EXP only
+    with pytest.raises(ValueError, match="DEV/SERSIC"):
         render_source_stamps(psf, sc.x, sc.y, 7, 11, sc.stype, sc.logre, sc.ee1, sc.ee2)
 
 
