@@ -481,18 +481,8 @@ __device__ void render_star(const IslandArgs& a, int T, int B, const float* coef
   Real* Hy = ws + 5 * TT;
   for (int e = tid; e < RH * RW; e += UPH_BLOCK) {
     const int r = ry0 + e / RW, c = rx0 + e % RW;
-    const float* cp = Cf + r * T + c;
-    Real s = 0, sx = 0, sy = 0;
-    {
-      for (int t = 0; t < nt; ++t) {
-        const Real cv = Real(cp[(size_t)t * TT]);
-        s = fma(cv, bas[t], s);
-        if (grad) {
-          sx = fma(cv, bas[21 + t], sx);
-          sy = fma(cv, bas[42 + t], sy);
-        }
-      }
-    }
+    Real s, sx, sy;
+    psf_poly_at(Cf + r * T + c, (size_t)TT, nt, bas, grad, s, sx, sy);
     P[e] = s;
     if (grad) {
       Px[e] = sx / xs;
@@ -606,15 +596,8 @@ __device__ void render_galaxy(const IslandArgs& a, IslandSh<Real>& sh, int b, co
     const GalLayout L = galaxy_layout(T);
     const float* Cf = sh.coefs + (size_t)b * nt * TT;
     for (int e = tid; e < TT; e += UPH_BLOCK) {
-      Real v = 0, vx = 0, vy = 0;
-      for (int t = 0; t < nt; ++t) {
-        const Real cv = Real(Cf[(size_t)t * TT + e]);
-        v = fma(cv, sh.bas[t], v);
-        if (gxy) {
-          vx = fma(cv, sh.bas[21 + t], vx);
-          vy = fma(cv, sh.bas[42 + t], vy);
-        }
-      }
+      Real v, vx, vy;
+      psf_poly_at(Cf + e, (size_t)TT, nt, (const Real*)sh.bas, gxy, v, vx, vy);
       gws[L.P + e] = v;
       if (gxy) {
         gws[L.Px + e] = vx / xs;

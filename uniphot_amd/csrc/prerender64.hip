@@ -184,16 +184,8 @@ __device__ void star_planes64(const IslandArgsPre& a, const float* coefs, int T,
   double* Hy = ws + 5 * TT;
   for (int e = tid; e < RH * RW; e += UPH_BLOCK) {
     const int r = w.ry0 + e / RW, c = w.rx0 + e % RW;
-    const float* cp = Cf + r * T + c;
-    double s = 0, sx = 0, sy = 0;
-    for (int t = 0; t < nt; ++t) {
-      const double cv = double(cp[(size_t)t * TT]);
-      s = fma(cv, bas[t], s);
-      if (grad) {
-        sx = fma(cv, bas[21 + t], sx);
-        sy = fma(cv, bas[42 + t], sy);
-      }
-    }
+    double s, sx, sy;
+    psf_poly_at(Cf + r * T + c, (size_t)TT, nt, bas, grad, s, sx, sy);
     P[e] = s;
     if (grad) {
       Px[e] = sx / xs;
@@ -299,31 +291,69 @@ __device__ __forceinline__ int brevN(int i) {
   return (int)(__brev((unsigned)i) >> (32 - LineE<E>::LOGN));
 }
 
+// Lane l ^ H's value of v (H = 1 .. 32) without the LDS crossbar: quad
+// permutes for 1 and 2, a quad reversal then a half-row mirror for 4 (lane
+// 7 - ((l & 7) ^ 3) = l ^ 4 inside 8 lanes), a row rotation by 8 for 8, and
+// the gfx950 row / half-wave swaps for 16 and 32 (the swap of a register
+// with a copy of itself leaves the partner's value in one of the two
+// results, by lane).
+template <int H>
+__device__ __forceinline__ int lane_xor(int v) {
+  if constexpr (H == 1) {
+    return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1, 0, 3, 2]
+  } else if constexpr (H == 2) {
+    return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2, 3, 0, 1]
+  } else if constexpr (H == 4) {
+    const int r = __builtin_amdgcn_mov_dpp(v, 0x1B, 0xF, 0xF, false);  // quad_perm [3, 2, 1, 0]
+    return __builtin_amdgcn_mov_dpp(r, 0x141, 0xF, 0xF, false);         // row_half_mirror
+  } else if constexpr (H == 8) {
+    return __builtin_amdgcn_mov_dpp(v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (H == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (threadIdx.x & 16) ? (int)r[0] : (int)r[1];
+  } else {
+    static_assert(H == 32, "lane_xor: H is a power of two below 64");
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? (int)r[0] : (int)r[1];
+  }
+}
+
+template <int H>
+__device__ __forceinline__ double lane_xor(double v) {
+  return __hiloint2double(lane_xor<H>(__double2hiint(v)), lane_xor<H>(__double2loint(v)));
+}
+
+// One cross-lane radix-2 stage of span H = 2^Q: the upper lane of each pair
+// multiplies its value by the twiddle (the lower lane by 1), the pair
+// exchanges products, and lower = own + partner's, upper = partner's - own.
+template <int Q, int E>
+__device__ __forceinline__ void cross_stage(double (&re)[E], double (&im)[E], double c, double s,
+                                            double sign) {
+  constexpr int H = 1 << Q;
+  const bool upper = threadIdx.x & H;
+  const double wr = upper ? c : 1.0, wi = upper ? sign * s : 0.0;
+  const double g = upper ? -1.0 : 1.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const double zr = wr * re[e] - wi * im[e], zi = wr * im[e] + wi * re[e];
+    const double pr = lane_xor<H>(zr), pi = lane_xor<H>(zi);
+    re[e] = fma(g, zr, pr);
+    im[e] = fma(g, zi, pi);
+  }
+}
+
 // In-register transform of one line (sign -1 forward, +1 inverse,
 // unnormalised): on entry re/im[e] = x[brevN(lane + 64 e)], on exit the
 // outputs lane + 64 e in natural order.
 template <int E>
 __device__ __forceinline__ void fftN(double (&re)[E], double (&im)[E], const TwN<E>& t,
                                      double sign) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int q = 0; q < 6; ++q) {
-    const int h = 1 << q;
-    const double wr = t.c[q], wi = sign * t.s[q];
-    const bool upper = lane & h;
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const double pr = __shfl_xor(re[e], h, 64), pim = __shfl_xor(im[e], h, 64);
-      if (!upper) {
-        re[e] = re[e] + (wr * pr - wi * pim);
-        im[e] = im[e] + (wr * pim + wi * pr);
-      } else {
-        const double tr = wr * re[e] - wi * im[e], ti = wr * im[e] + wi * re[e];
-        re[e] = pr - tr;
-        im[e] = pim - ti;
-      }
-    }
-  }
+  cross_stage<0>(re, im, t.c[0], t.s[0], sign);
+  cross_stage<1>(re, im, t.c[1], t.s[1], sign);
+  cross_stage<2>(re, im, t.c[2], t.s[2], sign);
+  cross_stage<3>(re, im, t.c[3], t.s[3], sign);
+  cross_stage<4>(re, im, t.c[4], t.s[4], sign);
+  cross_stage<5>(re, im, t.c[5], t.s[5], sign);
 #pragma unroll
   for (int q = 0; q < LineE<E>::LOGE; ++q) {
     const int he = 1 << q;
@@ -357,16 +387,29 @@ __device__ void rfft2_N(const double* X, int T, double2* S, double2* A, double2*
   int ci[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) ci[e] = brevN<E>(lane + 64 * e);
-  // rows, two per line
-  for (int j = wid; 2 * j < T; j += UPH_WAVES) {
-    const int ra = 2 * j, rb = 2 * j + 1;
-    double re[E], im[E];
+  // rows, two per line.  Pad 128: the next pair is loaded before this
+  // pair's transform
+  constexpr bool kAhead = E == 2;
+  double xa[E], xb[E];
+  auto load_rows = [&](int j) {
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int c = ci[e];
-      re[e] = c < T ? X[ra * T + c] : 0.0;
-      im[e] = (rb < T && c < T) ? X[rb * T + c] : 0.0;
+      xa[e] = c < T ? X[2 * j * T + c] : 0.0;
+      xb[e] = (2 * j + 1 < T && c < T) ? X[(2 * j + 1) * T + c] : 0.0;
     }
+  };
+  if (kAhead && 2 * wid < T) load_rows(wid);
+  for (int j = wid; 2 * j < T; j += UPH_WAVES) {
+    const int ra = 2 * j, rb = 2 * j + 1;
+    if (!kAhead) load_rows(j);
+    double re[E], im[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      re[e] = xa[e];
+      im[e] = xb[e];
+    }
+    if (kAhead && 2 * (j + UPH_WAVES) < T) load_rows(j + UPH_WAVES);
     fftN<E>(re, im, tw, -1.0);
 #pragma unroll
     for (int e = 0; e < E; ++e) z[lane + 64 * e] = make_double2(re[e], im[e]);
@@ -428,16 +471,29 @@ __device__ void irfft2_box_N(const double2* S, const double* spec, int kind, int
   int mi[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) mi[e] = brevN<E>(lane + 64 * e);
-  // columns: inverse along m, keep the box rows
+  // columns: inverse along m, keep the box rows.  Pad 128: the next
+  // column's spectrum is loaded before this column's transform (pad 256
+  // has no registers to spare for it)
+  constexpr bool kAhead = E == 2;
+  double2 sv[E];
+  double wv[E];
+  auto load_col = [&](int k) {
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      sv[e] = S[k * N + mi[e]];
+      wv[e] = spec_weight<E>(spec, kind, p, dt9, mi[e], k);
+    }
+  };
+  if (kAhead && wid < H) load_col(wid);
   for (int k = wid; k < H; k += UPH_WAVES) {
+    if (!kAhead) load_col(k);
     double re[E], im[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) {
-      const double w = spec_weight<E>(spec, kind, p, dt9, mi[e], k);
-      const double2 sv = S[k * N + mi[e]];
-      re[e] = sv.x * w;
-      im[e] = sv.y * w;
+      re[e] = sv[e].x * wv[e];
+      im[e] = sv[e].y * wv[e];
     }
+    if (kAhead && k + UPH_WAVES < H) load_col(k + UPH_WAVES);
     fftN<E>(re, im, tw, 1.0);
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -562,15 +618,8 @@ __device__ void galaxy_psf64(const IslandArgsPre& a, const float* coefs, int T, 
   const GalLayout L = galaxy_layout(T);
   const float* Cf = coefs + (size_t)b * nt * TT;
   for (int e = tid; e < TT; e += UPH_BLOCK) {
-    double v = 0, vx = 0, vy = 0;
-    for (int t = 0; t < nt; ++t) {
-      const double cv = double(Cf[(size_t)t * TT + e]);
-      v = fma(cv, bas[t], v);
-      if (gxy) {
-        vx = fma(cv, bas[21 + t], vx);
-        vy = fma(cv, bas[42 + t], vy);
-      }
-    }
+    double v, vx, vy;
+    psf_poly_at(Cf + e, (size_t)TT, nt, bas, gxy, v, vx, vy);
     ws[L.P + e] = v;
     if (gxy) {
       ws[L.Px + e] = vx / xs;

@@ -42,6 +42,33 @@ __device__ __forceinline__ void psf_basis(Real X, Real Y, int nt, Real* b, Real*
   }
 }
 
+// The nt-term polynomial PSF at one stamp pixel (coefficients cp[t * TT],
+// t < nt, float) and, with `grad`, its X / Y derivative sums over the basis
+// b[t], b[21 + t], b[42 + t].  The coefficient loads go out in groups of 8
+// before their FMAs, which run in term order.
+template <typename Real>
+__device__ __forceinline__ void psf_poly_at(const float* cp, size_t TT, int nt, const Real* b,
+                                            bool grad, Real& s, Real& sx, Real& sy) {
+  s = Real(0);
+  sx = Real(0);
+  sy = Real(0);
+  for (int t0 = 0; t0 < nt; t0 += 8) {
+    float cv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) cv[u] = t0 + u < nt ? cp[(size_t)(t0 + u) * TT] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (t0 + u >= nt) break;
+      const Real c = Real(cv[u]);
+      s = fma(c, b[t0 + u], s);
+      if (grad) {
+        sx = fma(c, b[21 + t0 + u], sx);
+        sy = fma(c, b[42 + t0 + u], sy);
+      }
+    }
+  }
+}
+
 // Lanczos-3 window and its derivative at x, float64
 // (tract7dt/fit/optimize/gpu/kernels/lanczos.py:50-99).
 __device__ __forceinline__ void l3_val_der(double x, double& v, double& d) {

@@ -951,7 +951,7 @@ def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev
             at64.append((dataclasses.replace(kern, precision="fp32"), h2, rows))
         rpost = run_island_batch(at64, psf_trees, 0, dstack_dev, wstack_dev, precision="fp32",
                                  sersic_dev=sersic_dev, split=True, prerender=True, post=True,
-                                 workspace_bytes=workspace_bytes)
+                                 workspace_bytes=workspace_bytes, timer=timer)
         for r, rp in zip(r64, rpost):
             for k in _POST_KEYS:
                 if k in rp:
@@ -963,7 +963,8 @@ def fit_float64(classes, psf_trees: dict, max_steps: int, dstack_dev, wstack_dev
 
 def refit_ill_conditioned(classes, results, psf_trees: dict, max_steps: int, dstack_dev,
                           wstack_dev, *, kappa: float = FP64_REFIT_KAPPA, sersic_dev=None,
-                          workspace_bytes: float = MAX_SCRATCH_BYTES):
+                          workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None,
+                          max_blocks: int | None = None):
     """Refit, on the float64 kernels from the same seeds, the islands of a
     float32 post-fit batch (``results`` of :func:`run_island_batch` with
     ``post``) whose ``post_kappa`` exceeds ``kappa`` (see
@@ -985,7 +986,8 @@ def refit_ill_conditioned(classes, results, psf_trees: dict, max_steps: int, dst
     if not redo:
         return out, 0
     r64 = fit_float64(redo, psf_trees, max_steps, dstack_dev, wstack_dev,
-                      sersic_dev=sersic_dev, workspace_bytes=workspace_bytes)
+                      sersic_dev=sersic_dev, workspace_bytes=workspace_bytes, timer=timer,
+                      max_blocks=max_blocks)
     j = 0
     for r, sel in zip(out, sel_of):
         if not sel.any():
@@ -1036,7 +1038,7 @@ def _damped_proposal(p, H, g, lam, live, clipv, lower, upper):
 
 
 def stall_probes(classes, results, psf_trees: dict, dstack_dev, wstack_dev, *, sersic_dev=None,
-                 workspace_bytes: float = MAX_SCRATCH_BYTES):
+                 workspace_bytes: float = MAX_SCRATCH_BYTES, timer=None):
     """Post-hoc probes of the stalled islands of a fitted batch whose
     results carry the post-fit pass (``post_hg``): ``diverged`` -- the
     STALL_REJECT_K rejected proposals, rebuilt at lam_final 10^(j - K), are
@@ -1100,11 +1102,12 @@ def stall_probes(classes, results, psf_trees: dict, dstack_dev, wstack_dev, *, s
             back.append((ci, [i for i, _ in lst]))
         if prec == "fp64":
             out = _run_fp64_split(batch, psf_trees, 0, dstack_dev, wstack_dev,
-                                  sersic_dev=sersic_dev, workspace_bytes=workspace_bytes)
+                                  sersic_dev=sersic_dev, workspace_bytes=workspace_bytes,
+                                  timer=timer)
         else:
             out = run_island_batch(batch, psf_trees, 0, dstack_dev, wstack_dev, precision=prec,
                                    sersic_dev=sersic_dev, split=True, prerender=True,
-                                   workspace_bytes=workspace_bytes)
+                                   workspace_bytes=workspace_bytes, timer=timer)
         for (ci, idx), r in zip(back, out):
             c2 = r["chi2"].reshape(len(idx), len(PLATEAU_LAMBDAS) + 1)
             results[ci]["stall_dchi2"][idx] = (c2[:, 1:].min(axis=1) - c2[:, 0]).astype(

@@ -809,7 +809,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                                      precision=cfg.precision, sersic_dev=sersic_dev,
                                      split=cfg.island_split, prerender=cfg.galaxy_prerender,
                                      post=cfg.post_fit, gram="fp32",
-                                     workspace_bytes=cfg.island_workspace)
+                                     workspace_bytes=cfg.island_workspace, timer=timer,
+                                     max_blocks=blocks.get(cfg.precision))
             pos = {id(h): j for j, (_, h, _) in enumerate(redo)}
             merged = []
             for (kern, host, rows), res in zip(chunk, results):
@@ -829,7 +830,8 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             results, n64 = refit_ill_conditioned(chunk, results, trees, cfg.max_steps, ds, ws,
                                                  kappa=cfg.fp64_refit_kappa,
                                                  sersic_dev=sersic_dev,
-                                                 workspace_bytes=cfg.island_workspace)
+                                                 workspace_bytes=cfg.island_workspace,
+                                                 timer=timer, max_blocks=blocks.get("fp64"))
             n_fp64 += n64
         if part64:
             results = results + fit_float64(part64, trees, cfg.max_steps, ds, ws,
@@ -840,7 +842,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             n_fp64 += sum(int(r.size) for _, _, r in part64)
         if cfg.stall_probes and any("post_hg" in r for r in results):
             stall_probes(chunk, results, trees, ds, ws, sersic_dev=sersic_dev,
-                         workspace_bytes=cfg.island_workspace)
+                         workspace_bytes=cfg.island_workspace, timer=timer)
         if sync is not None:
             sync()
         timing.append((int(sum(np.asarray(r).size for _, _, r in chunk)),
