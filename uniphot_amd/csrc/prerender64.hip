@@ -39,6 +39,9 @@ size_t prerender64_scratch(int T) { return galaxy_scratch_bytes(T, sizeof(double
 size_t prerender64_spectra_doubles(int T) { return spectra64_doubles(T); }
 
 constexpr int kLdsRows = 64;  // rows of the LDS line buffer of the pad-128 tier (T, box)
+// doubles of the planes kernel's LDS buffer (line buffer + per-wave lines),
+// which star items use for their box planes
+constexpr size_t kLdsDoubles = (size_t)2 * (kLdsRows * 65 + UPH_WAVES * 128);
 
 // Stamp tiers whose transforms run as wave-held lines with only the raw
 // box the shift reads: pad 128 (line buffers in LDS, T <= kLdsRows) and pad
@@ -163,6 +166,135 @@ __device__ inline bool member_window(float xf, float yf, float x0f, float y0f, i
   w.ry0 = max(0, w.sy0 - 3);
   w.ry1 = min(T, w.sy1 + 3);
   return true;
+}
+
+// PSF polynomial (and its x/y derivatives with `grad`) over the box rows
+// [ry0, ry0 + RH) x columns [rx0, rx0 + RW) of the T x T stamp into P, Px,
+// Py (row stride RW): 4 box pixels per thread per round, the coefficient
+// loads of all 4 issued before their FMAs (term order per pixel, as
+// psf_poly_at).  Loads past the box or the last term read an in-bounds
+// coefficient and are zeroed, so none is predicated.
+__device__ __forceinline__ void psf_box64(const float* Cf, int T, int nt, const double* bas,
+                                          bool grad, double xs, double ys, int ry0, int rx0,
+                                          int RH, int RW, double* P, double* Px, double* Py) {
+  const int n = RH * RW;
+  const size_t TT = (size_t)T * T;
+  for (int e0 = threadIdx.x; e0 < n; e0 += 4 * UPH_BLOCK) {
+    int off[4];
+    bool ok[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = e0 + j * UPH_BLOCK;
+      ok[j] = e < n;
+      off[j] = ok[j] ? (ry0 + e / RW) * T + rx0 + e % RW : 0;
+    }
+    double v[4] = {0, 0, 0, 0}, vx[4] = {0, 0, 0, 0}, vy[4] = {0, 0, 0, 0};
+    for (int t0 = 0; t0 < nt; t0 += 8) {
+      float cv[4][8];
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const int t = min(t0 + u, nt - 1);
+          const float c = Cf[(size_t)t * TT + off[j]];
+          cv[j][u] = (ok[j] && t0 + u < nt) ? c : 0.0f;
+        }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (t0 + u >= nt) break;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const double c = double(cv[j][u]);
+          v[j] = fma(c, bas[t0 + u], v[j]);
+          if (grad) {
+            vx[j] = fma(c, bas[21 + t0 + u], vx[j]);
+            vy[j] = fma(c, bas[42 + t0 + u], vy[j]);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (!ok[j]) continue;
+      const int e = e0 + j * UPH_BLOCK;
+      P[e] = v[j];
+      if (grad) {
+        Px[e] = vx[j] / xs;
+        Py[e] = vy[j] / ys;
+      }
+    }
+  }
+}
+
+// Star through LDS: the box planes (P, and Px, Py with `grad`) in sl, then
+// the horizontal and vertical Lanczos passes fused: each thread takes 4
+// vertically adjacent window pixels of one column, forms the horizontal
+// sums of the 10 box rows they read (the same FMA chains as the separate
+// horizontal pass) and accumulates them in row order (as the vertical
+// pass), so the planes are those of star_planes64 bit for bit.
+__device__ void star_planes64_lds(const IslandArgsPre& a, const float* coefs, int T, int b,
+                                  const MemberWindow& w, bool grad, const double* taps,
+                                  const double* bas, double xs, double ys, double* sl,
+                                  double* blk, int side) {
+  const int RW = w.rx1 - w.rx0, RH = w.ry1 - w.ry0, SW = w.sx1 - w.sx0, SH = w.sy1 - w.sy0;
+  const size_t PP = (size_t)side * side;
+  double* P = sl;
+  double* Px = sl + RH * RW;
+  double* Py = sl + 2 * RH * RW;
+  psf_box64(coefs + (size_t)b * a.nt * T * T, T, a.nt, bas, grad, xs, ys, w.ry0, w.rx0, RH, RW,
+            P, Px, Py);
+  __syncthreads();
+  double Lx[7], dLx[7], Ly[7], dLy[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    Lx[k] = taps[k];
+    Ly[k] = taps[14 + k];
+    dLx[k] = grad ? taps[7 + k] : 0.0;
+    dLy[k] = grad ? taps[21 + k] : 0.0;
+  }
+  const int ngrp = (SH + 3) / 4;
+  for (int q = threadIdx.x; q < ngrp * SW; q += UPH_BLOCK) {
+    const int cc = q % SW, r0 = w.sy0 + 4 * (q / SW), c = w.sx0 + cc;
+    double m[4] = {0, 0, 0, 0}, tx[4] = {0, 0, 0, 0}, ty[4] = {0, 0, 0, 0};
+    for (int rr = r0 - 3; rr <= r0 + 6; ++rr) {
+      if (rr < w.ry0 || rr >= w.ry1) continue;
+      double h = 0, hx = 0, hy = 0;
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const int cx = c + k - 3;
+        if (cx < w.rx0 || cx >= w.rx1) continue;
+        const int o = (rr - w.ry0) * RW + cx - w.rx0;
+        const double v = P[o];
+        h = fma(Lx[k], v, h);
+        if (grad) {
+          hx = fma(Lx[k], Px[o], fma(dLx[k], v, hx));
+          hy = fma(Lx[k], Py[o], hy);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int k = rr - (r0 + j) + 3;
+        if (k < 0 || k > 6) continue;
+        m[j] = fma(Ly[k], h, m[j]);
+        if (grad) {
+          tx[j] = fma(Ly[k], hx, tx[j]);
+          ty[j] = fma(Ly[k], hy, fma(dLy[k], h, ty[j]));
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + j;
+      if (r >= w.sy1) break;
+      const size_t so = (size_t)(r - w.sy0) * side + cc;
+      blk[so] = m[j];
+      if (grad) {
+        blk[PP + so] = tx[j];
+        blk[2 * PP + so] = ty[j];
+      }
+    }
+  }
+  __syncthreads();
 }
 
 // Star: PSF polynomial over the shift-read box (and its x/y derivatives),
@@ -342,11 +474,12 @@ __device__ __forceinline__ void cross_stage(double (&re)[E], double (&im)[E], do
   }
 }
 
-// In-register transform of one line (sign -1 forward, +1 inverse,
-// unnormalised): on entry re/im[e] = x[brevN(lane + 64 e)], on exit the
-// outputs lane + 64 e in natural order.
-template <int E>
-__device__ __forceinline__ void fftN(double (&re)[E], double (&im)[E], const TwN<E>& t,
+// In-register transform of L lines of 64 E points held in re/im[l E + e]
+// (sign -1 forward, +1 inverse, unnormalised): on entry element e of line l
+// is x_l[brevN(lane + 64 e)], on exit the outputs lane + 64 e in natural
+// order.  The cross-lane stages of the L lines interleave.
+template <int E, int L = 1>
+__device__ __forceinline__ void fftN(double (&re)[L * E], double (&im)[L * E], const TwN<E>& t,
                                      double sign) {
   cross_stage<0>(re, im, t.c[0], t.s[0], sign);
   cross_stage<1>(re, im, t.c[1], t.s[1], sign);
@@ -358,17 +491,20 @@ __device__ __forceinline__ void fftN(double (&re)[E], double (&im)[E], const TwN
   for (int q = 0; q < LineE<E>::LOGE; ++q) {
     const int he = 1 << q;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      if (e & he) continue;
-      const int f = e + he;
-      const int ti = he - 1 + (e & (he - 1));
-      const double wr = t.ic[ti], wi = sign * t.is[ti];
-      const double br = wr * re[f] - wi * im[f], bi = wr * im[f] + wi * re[f];
-      const double ar = re[e], ai = im[e];
-      re[e] = ar + br;
-      im[e] = ai + bi;
-      re[f] = ar - br;
-      im[f] = ai - bi;
+    for (int l = 0; l < L; ++l) {
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        if (e & he) continue;
+        const int f = l * E + e + he, g = l * E + e;
+        const int ti = he - 1 + (e & (he - 1));
+        const double wr = t.ic[ti], wi = sign * t.is[ti];
+        const double br = wr * re[f] - wi * im[f], bi = wr * im[f] + wi * re[f];
+        const double ar = re[g], ai = im[g];
+        re[g] = ar + br;
+        im[g] = ai + bi;
+        re[f] = ar - br;
+        im[f] = ai - bi;
+      }
     }
   }
 }
@@ -422,18 +558,30 @@ __device__ void rfft2_N(const double* X, int T, double2* S, double2* A, double2*
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
-  // columns
-  for (int k = wid; k < H; k += UPH_WAVES) {
-    double re[E], im[E];
+  // columns; pad 128 transforms columns k and k + UPH_WAVES together
+  constexpr int L = E == 2 ? 2 : 1;
+  for (int k = wid; k < H; k += L * UPH_WAVES) {
+    double re[L * E], im[L * E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const double2 v = ci[e] < T ? A[ci[e] * H + k] : make_double2(0.0, 0.0);
-      re[e] = v.x;
-      im[e] = v.y;
+    for (int l = 0; l < L; ++l) {
+      const int kl = k + l * UPH_WAVES;  // wave-uniform
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const double2 v =
+            (kl < H && ci[e] < T) ? A[ci[e] * H + kl] : make_double2(0.0, 0.0);
+        re[l * E + e] = v.x;
+        im[l * E + e] = v.y;
+      }
     }
-    fftN<E>(re, im, tw, -1.0);
+    fftN<E, L>(re, im, tw, -1.0);
 #pragma unroll
-    for (int e = 0; e < E; ++e) S[k * N + lane + 64 * e] = make_double2(re[e], im[e]);
+    for (int l = 0; l < L; ++l) {
+      const int kl = k + l * UPH_WAVES;
+      if (kl >= H) continue;
+#pragma unroll
+      for (int e = 0; e < E; ++e)
+        S[kl * N + lane + 64 * e] = make_double2(re[l * E + e], im[l * E + e]);
+    }
   }
   __syncthreads();
 }
@@ -471,34 +619,38 @@ __device__ void irfft2_box_N(const double2* S, const double* spec, int kind, int
   int mi[E];
 #pragma unroll
   for (int e = 0; e < E; ++e) mi[e] = brevN<E>(lane + 64 * e);
-  // columns: inverse along m, keep the box rows.  Pad 128: the next
-  // column's spectrum is loaded before this column's transform (pad 256
-  // has no registers to spare for it)
-  constexpr bool kAhead = E == 2;
-  double2 sv[E];
-  double wv[E];
-  auto load_col = [&](int k) {
+  // columns: inverse along m, keep the box rows.  Pad 128 transforms
+  // columns k and k + UPH_WAVES together (both loaded before either
+  // transform); pad 256 has no registers to spare for it
+  constexpr int L = E == 2 ? 2 : 1;
+  for (int k = wid; k < H; k += L * UPH_WAVES) {
+    double re[L * E], im[L * E];
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      sv[e] = S[k * N + mi[e]];
-      wv[e] = spec_weight<E>(spec, kind, p, dt9, mi[e], k);
+    for (int l = 0; l < L; ++l) {
+      const int kl = k + l * UPH_WAVES;  // wave-uniform
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        double2 v = make_double2(0.0, 0.0);
+        double w = 0.0;
+        if (kl < H) {
+          v = S[kl * N + mi[e]];
+          w = spec_weight<E>(spec, kind, p, dt9, mi[e], kl);
+        }
+        re[l * E + e] = v.x * w;
+        im[l * E + e] = v.y * w;
+      }
     }
-  };
-  if (kAhead && wid < H) load_col(wid);
-  for (int k = wid; k < H; k += UPH_WAVES) {
-    if (!kAhead) load_col(k);
-    double re[E], im[E];
+    fftN<E, L>(re, im, tw, 1.0);
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      re[e] = sv[e].x * wv[e];
-      im[e] = sv[e].y * wv[e];
-    }
-    if (kAhead && k + UPH_WAVES < H) load_col(k + UPH_WAVES);
-    fftN<E>(re, im, tw, 1.0);
+    for (int l = 0; l < L; ++l) {
+      const int kl = k + l * UPH_WAVES;
+      if (kl >= H) continue;
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-      const int m = lane + 64 * e;
-      if (m >= ry0 && m < ry1) B[(m - ry0) * H + k] = make_double2(re[e], im[e]);
+      for (int e = 0; e < E; ++e) {
+        const int m = lane + 64 * e;
+        if (m >= ry0 && m < ry1)
+          B[(m - ry0) * H + kl] = make_double2(re[l * E + e], im[l * E + e]);
+      }
     }
   }
   __syncthreads();
@@ -613,11 +765,11 @@ __device__ void shift_planes64(const double* G, size_t gq, int gs, int gy0, int 
 // dP/dy when the x or y tangent is needed, at the layout's P, Px, Py.
 __device__ void galaxy_psf64(const IslandArgsPre& a, const float* coefs, int T, int b, int need,
                              const double* bas, double xs, double ys, double* ws) {
-  const int tid = threadIdx.x, nt = a.nt, TT = T * T;
+  const int nt = a.nt, TT = T * T;
   const bool gxy = need & 3;
   const GalLayout L = galaxy_layout(T);
   const float* Cf = coefs + (size_t)b * nt * TT;
-  for (int e = tid; e < TT; e += UPH_BLOCK) {
+  for (int e = threadIdx.x; e < TT; e += UPH_BLOCK) {
     double v, vx, vy;
     psf_poly_at(Cf + e, (size_t)TT, nt, bas, gxy, v, vx, vy);
     ws[L.P + e] = v;
@@ -737,7 +889,11 @@ __device__ void plane_item(const IslandArgsPre& a, long long item, double* taps,
   } else if constexpr (kKind == kItemBlock) {
     galaxy_planes64_block(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side);
   } else {
-    if (star)
+    const int nplanes = grads ? 3 : 1;
+    if (star && (size_t)nplanes * (w.ry1 - w.ry0) * (w.rx1 - w.rx0) <= kLdsDoubles)
+      star_planes64_lds(a, coefs, T, b, w, grads, taps, bas, xs, ys,
+                        reinterpret_cast<double*>(lds), blk, side);
+    else if (star)
       star_planes64(a, coefs, T, b, w, grads, taps, bas, xs, ys, ws, blk, side);
     else
       galaxy_planes64_box<2>(a, coefs, T, b, w, need, spec, taps, bas, xs, ys, ws, blk, side,
@@ -753,6 +909,7 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_kernel(IslandArgsPre a
   __shared__ long long s_item;
   // line buffer (kLdsRows x 65 complex) and one 128-complex line per wave
   __shared__ double2 lds[kLdsRows * LineE<2>::H + UPH_WAVES * LineE<2>::N];
+  static_assert(sizeof(lds) == kLdsDoubles * sizeof(double), "star planes LDS capacity");
   double* ws = reinterpret_cast<double*>(a.scratch + (size_t)blockIdx.x * a.scratch_per_block);
   const long long n_items = (long long)a.n_pre * a.nb;
   for (;;) {
