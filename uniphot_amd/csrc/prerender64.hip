@@ -845,13 +845,25 @@ __device__ void galaxy_planes64_block(const IslandArgsPre& a, const float* coefs
 constexpr int kItemLds = 0, kItemPad256 = 1, kItemBlock = 2;
 
 // One (member, band) item of kind kKind; an item of another kind is left
-// alone.  Block-uniform; ends after a barrier.
+// alone.  Block-uniform; ends after a barrier.  Items of extended members
+// ([0, n_pre_ext nb)) run member-major, so the bands of one member share
+// its spectra in L2; star items run band-major, so the workgroups working
+// at once share one band's PSF coefficient table in L2.
 template <int kKind>
 __device__ void plane_item(const IslandArgsPre& a, long long item, double* taps, double* bas,
                            double2* lds, double2* zw, double* ws) {
   const int tid = threadIdx.x;
   const bool grads = split_grads(a.mode);
-  const int g = (int)(item / a.nb), b = (int)(item % a.nb);
+  const long long n_ext_items = (long long)a.n_pre_ext * a.nb;
+  int g, b;
+  if (item < n_ext_items) {
+    g = (int)(item / a.nb);
+    b = (int)(item % a.nb);
+  } else {
+    const long long s = item - n_ext_items, n_star = a.n_pre - a.n_pre_ext;
+    b = (int)(s / n_star);
+    g = a.n_pre_ext + (int)(s % n_star);
+  }
   const int isl = a.pre_isl[g];
   if (pass_skips(a, isl)) return;
   const int32_t* sp = a.shape + 4 * (size_t)isl;
