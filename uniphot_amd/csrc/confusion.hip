@@ -8,8 +8,8 @@
 //     pixel variance v = good & invvar > 0 ? 1 / invvar : 0;
 //  2. conf_corr_kernel: the 'same' correlations corr(r, phi), corr(g, phi^2)
 //     and corr(v, phi^2) with the band's unit-sum central PSF phi, tiled
-//     through LDS like the blank-response correlation (4 vertically adjacent
-//     outputs per thread); the matched filter MF = corr(r, phi) / corr(g, phi^2) and
+//     through LDS like the blank-response correlation (4 adjacent outputs
+//     per thread); the matched filter MF = corr(r, phi) / corr(g, phi^2) and
 //     Vstat = corr(v, phi^2) / corr(g, phi^2)^2 where the kernel-weighted
 //     occupancy corr(g, phi^2) exceeds 0.6 sum(phi^2) (valid), else 0;
 //  3. conf_annulus_kernel: one wavefront per source: float64 sums of valid,
@@ -55,7 +55,7 @@ constexpr int kConfRows = 16, kConfCols = 64, kConfBlock = 256;
 __host__ __device__ inline int conf_tp(int T) { return (T + 3) / 4 * 4; }
 __host__ __device__ inline int conf_sx(int T) { return kConfCols + conf_tp(T) + 4; }
 __host__ __device__ inline size_t conf_lds_bytes(int T) {
-  return ((size_t)2 * (T + 3) * conf_tp(T) + (size_t)3 * (kConfRows + T - 1) * conf_sx(T)) *
+  return ((size_t)2 * T * conf_tp(T) + (size_t)3 * (kConfRows + T - 1) * conf_sx(T)) *
          sizeof(float);
 }
 
@@ -106,9 +106,9 @@ __global__ __launch_bounds__(kConfBlock) void conf_corr_kernel(ConfArgs a) {
   const int h = a.win[q * 5 + 3], w = a.win[q * 5 + 4], T = a.T, c = T / 2;
   const int TP = conf_tp(T), SX = conf_sx(T), SY = kConfRows + T - 1;
   const int tiles_x = (w + kConfCols - 1) / kConfCols, tiles_y = (h + kConfRows - 1) / kConfRows;
-  float* P1 = lds;                // phi (T + 3 rows of TP, the last 3 zero)
-  float* P2 = P1 + (T + 3) * TP;  // phi^2
-  float* A1 = P2 + (T + 3) * TP;  // r tile with halo (SY rows of SX)
+  float* P1 = lds;          // phi (T rows of TP)
+  float* P2 = P1 + T * TP;  // phi^2
+  float* A1 = P2 + T * TP;  // r tile with halo (SY rows of SX)
   float* A2 = A1 + SY * SX; // g tile
   float* A3 = A2 + SY * SX; // v tile
   float* base = a.work + (size_t)q * 6 * a.max_pix;
@@ -120,13 +120,13 @@ __global__ __launch_bounds__(kConfBlock) void conf_corr_kernel(ConfArgs a) {
   float* VL = base + 5 * (size_t)a.max_pix;
   const float* ph = a.phi + (size_t)q * T * T;
   const float thr = 0.6f * a.sumsq[q];
-  for (int e = threadIdx.x; e < (T + 3) * TP; e += kConfBlock) {
+  for (int e = threadIdx.x; e < T * TP; e += kConfBlock) {
     const int ky = e / TP, kx = e % TP;
-    const float v = (kx < T && ky < T) ? ph[ky * T + kx] : 0.0f;
+    const float v = kx < T ? ph[ky * T + kx] : 0.0f;
     P1[e] = v;
     P2[e] = v * v;
   }
-  const int lx = threadIdx.x % kConfCols, ly4 = 4 * (threadIdx.x / kConfCols);
+  const int ly = threadIdx.x / (kConfCols / 4), lx = 4 * (threadIdx.x % (kConfCols / 4));
   for (int tile = blockIdx.y; tile < tiles_x * tiles_y; tile += gridDim.y) {
     const int ty0 = (tile / tiles_x) * kConfRows, tx0 = (tile % tiles_x) * kConfCols;
     __syncthreads();
@@ -138,41 +138,40 @@ __global__ __launch_bounds__(kConfBlock) void conf_corr_kernel(ConfArgs a) {
       A3[e] = in ? V[yy * w + xx] : 0.0f;
     }
     __syncthreads();
-    // 4 vertically adjacent outputs per thread: for each template column,
-    // one pass down the tile column reads each input once (64 consecutive
-    // floats per wave: no bank conflicts) and serves the 4 outputs through a
-    // sliding window of template values (template rows T .. T + 2 are zero)
     float num[4] = {0, 0, 0, 0}, den[4] = {0, 0, 0, 0}, vn[4] = {0, 0, 0, 0};
-    for (int kx = 0; kx < T; ++kx) {
-      const float* k1 = P1 + kx;
-      const float* k2 = P2 + kx;
-      const float* b1 = A1 + ly4 * SX + lx + kx;
-      const float* b2 = A2 + ly4 * SX + lx + kx;
-      const float* b3 = A3 + ly4 * SX + lx + kx;
-      float p1[4] = {0, 0, 0, 0}, p2[4] = {0, 0, 0, 0};
-#pragma unroll 4
-      for (int r = 0; r < T + 3; ++r) {
-        p1[3] = p1[2];
-        p1[2] = p1[1];
-        p1[1] = p1[0];
-        p1[0] = k1[r * TP];
-        p2[3] = p2[2];
-        p2[2] = p2[1];
-        p2[1] = p2[0];
-        p2[0] = k2[r * TP];
-        const float x1 = b1[r * SX], x2 = b2[r * SX], x3 = b3[r * SX];
+    for (int ky = 0; ky < T; ++ky) {
+      const float4* r1 = reinterpret_cast<const float4*>(A1 + (ly + ky) * SX + lx);
+      const float4* r2 = reinterpret_cast<const float4*>(A2 + (ly + ky) * SX + lx);
+      const float4* r3 = reinterpret_cast<const float4*>(A3 + (ly + ky) * SX + lx);
+      const float4* k1 = reinterpret_cast<const float4*>(P1 + ky * TP);
+      const float4* k2 = reinterpret_cast<const float4*>(P2 + ky * TP);
+      float4 u1 = r1[0], u2 = r2[0], u3 = r3[0];
+      for (int k4 = 0; k4 < TP / 4; ++k4) {
+        const float4 v1 = r1[k4 + 1], v2 = r2[k4 + 1], v3 = r3[k4 + 1];
+        const float4 t1 = k1[k4], t2 = k2[k4];
+        const float x1[8] = {u1.x, u1.y, u1.z, u1.w, v1.x, v1.y, v1.z, v1.w};
+        const float x2[8] = {u2.x, u2.y, u2.z, u2.w, v2.x, v2.y, v2.z, v2.w};
+        const float x3[8] = {u3.x, u3.y, u3.z, u3.w, v3.x, v3.y, v3.z, v3.w};
+        const float c1[4] = {t1.x, t1.y, t1.z, t1.w};
+        const float c2[4] = {t2.x, t2.y, t2.z, t2.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          num[j] = fmaf(x1, p1[j], num[j]);
-          den[j] = fmaf(x2, p2[j], den[j]);
-          vn[j] = fmaf(x3, p2[j], vn[j]);
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            num[j] = fmaf(x1[j + t], c1[t], num[j]);
+            den[j] = fmaf(x2[j + t], c2[t], den[j]);
+            vn[j] = fmaf(x3[j + t], c2[t], vn[j]);
+          }
         }
+        u1 = v1;
+        u2 = v2;
+        u3 = v3;
       }
     }
-    const int ox = tx0 + lx;
+    const int oy = ty0 + ly;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int oy = ty0 + ly4 + j;
+      const int ox = tx0 + lx + j;
       if (oy >= h || ox >= w) continue;
       const bool valid = den[j] > thr && isfinite(num[j]);
       const float ds = fmaxf(den[j], 1e-30f);

@@ -809,22 +809,22 @@ __global__ __launch_bounds__(UPH_BLOCK) void blank_residual_kernel(BlankArgs a) 
   }
 }
 
-// Output tile of the correlation launch: kCorrRows x kCorrCols pixels, a
-// column of 4 vertically adjacent outputs per thread (kCorrBlock threads).
+// Output tile of the correlation launch: kCorrRows x kCorrCols pixels, one
+// row of 4 adjacent outputs per thread (kCorrBlock threads).
 constexpr int kCorrRows = 32, kCorrCols = 64, kCorrBlock = 512;
 
 __host__ __device__ inline int corr_tp(int T) { return (T + 3) / 4 * 4; }
 __host__ __device__ inline int corr_sx(int T) { return kCorrCols + corr_tp(T) + 4; }
 __host__ __device__ inline size_t corr_lds_bytes(int T) {
-  return ((size_t)2 * (T + 3) * corr_tp(T) + (size_t)2 * (kCorrRows + T - 1) * corr_sx(T)) *
+  return ((size_t)2 * T * corr_tp(T) + (size_t)2 * (kCorrRows + T - 1) * corr_sx(T)) *
          sizeof(float);
 }
 
 // 'same' correlations of wr with phi and of w with phi^2 over one output
 // tile per workgroup visit, inputs (with halo) and template staged in LDS;
-// each thread keeps 4 vertically adjacent outputs of both correlations, so
-// every input value loaded serves 4 outputs.  Template rows are zero-padded
-// to a multiple of 4 and 3 zero rows follow the template.
+// each thread keeps 4 adjacent outputs of both correlations and reads the
+// template and the inputs 4 floats at a time, so every input value loaded
+// serves 4 outputs.  Template rows are zero-padded to a multiple of 4.
 __global__ __launch_bounds__(kCorrBlock) void blank_corr_kernel(BlankArgs a) {
   extern __shared__ float4 lds4[];
   float* lds = reinterpret_cast<float*>(lds4);
@@ -833,22 +833,22 @@ __global__ __launch_bounds__(kCorrBlock) void blank_corr_kernel(BlankArgs a) {
   const int h = a.win[q * 5 + 3], w = a.win[q * 5 + 4], T = a.T, c = T / 2;
   const int TP = corr_tp(T), SX = corr_sx(T), SY = kCorrRows + T - 1;
   const int tiles_x = (w + kCorrCols - 1) / kCorrCols, tiles_y = (h + kCorrRows - 1) / kCorrRows;
-  float* P1 = lds;                // phi (T + 3 rows of TP, the last 3 zero)
-  float* P2 = P1 + (T + 3) * TP;  // phi^2
-  float* A1 = P2 + (T + 3) * TP;  // wr tile with halo (SY rows of SX)
+  float* P1 = lds;                // phi (T rows of TP)
+  float* P2 = P1 + T * TP;        // phi^2
+  float* A1 = P2 + T * TP;        // wr tile with halo (SY rows of SX)
   float* A2 = A1 + SY * SX;       // w tile with halo
   const float* WR = a.work + (size_t)q * 3 * a.max_pix;
   const float* WW = WR + a.max_pix;
   float* RESP = a.work + (size_t)q * 3 * a.max_pix + 2 * a.max_pix;
   const float* ph = a.phi + (size_t)q * T * T;
-  for (int e = threadIdx.x; e < (T + 3) * TP; e += kCorrBlock) {
+  for (int e = threadIdx.x; e < T * TP; e += kCorrBlock) {
     const int ky = e / TP, kx = e % TP;
-    const float v = (kx < T && ky < T) ? ph[ky * T + kx] : 0.0f;
+    const float v = kx < T ? ph[ky * T + kx] : 0.0f;
     P1[e] = v;
     P2[e] = v * v;
   }
   const size_t plane = (size_t)a.H * a.W, off = (size_t)band * plane;
-  const int lx = threadIdx.x % kCorrCols, ly4 = 4 * (threadIdx.x / kCorrCols);
+  const int ly = threadIdx.x / (kCorrCols / 4), lx = 4 * (threadIdx.x % (kCorrCols / 4));
   for (int tile = blockIdx.y; tile < tiles_x * tiles_y; tile += gridDim.y) {
     const int ty0 = (tile / tiles_x) * kCorrRows, tx0 = (tile % tiles_x) * kCorrCols;
     __syncthreads();
@@ -859,40 +859,37 @@ __global__ __launch_bounds__(kCorrBlock) void blank_corr_kernel(BlankArgs a) {
       A2[e] = in ? WW[yy * w + xx] : 0.0f;
     }
     __syncthreads();
-    // for each template column, one pass down the tile column: each input
-    // read once (64 consecutive floats per wave: no bank conflicts) serves
-    // the thread's 4 outputs through a sliding window of template values
-    // (template rows T .. T + 2 are zero)
     float num[4] = {0.0f, 0.0f, 0.0f, 0.0f}, den[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int kx = 0; kx < T; ++kx) {
-      const float* k1 = P1 + kx;
-      const float* k2 = P2 + kx;
-      const float* b1 = A1 + ly4 * SX + lx + kx;
-      const float* b2 = A2 + ly4 * SX + lx + kx;
-      float p1[4] = {0.0f, 0.0f, 0.0f, 0.0f}, p2[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll 4
-      for (int r = 0; r < T + 3; ++r) {
-        p1[3] = p1[2];
-        p1[2] = p1[1];
-        p1[1] = p1[0];
-        p1[0] = k1[r * TP];
-        p2[3] = p2[2];
-        p2[2] = p2[1];
-        p2[1] = p2[0];
-        p2[0] = k2[r * TP];
-        const float x1 = b1[r * SX], x2 = b2[r * SX];
+    for (int ky = 0; ky < T; ++ky) {
+      const float4* r1 = reinterpret_cast<const float4*>(A1 + (ly + ky) * SX + lx);
+      const float4* r2 = reinterpret_cast<const float4*>(A2 + (ly + ky) * SX + lx);
+      const float4* k1 = reinterpret_cast<const float4*>(P1 + ky * TP);
+      const float4* k2 = reinterpret_cast<const float4*>(P2 + ky * TP);
+      float4 u1 = r1[0], u2 = r2[0];
+      for (int k4 = 0; k4 < TP / 4; ++k4) {
+        const float4 v1 = r1[k4 + 1], v2 = r2[k4 + 1];
+        const float4 t1 = k1[k4], t2 = k2[k4];
+        const float x1[8] = {u1.x, u1.y, u1.z, u1.w, v1.x, v1.y, v1.z, v1.w};
+        const float x2[8] = {u2.x, u2.y, u2.z, u2.w, v2.x, v2.y, v2.z, v2.w};
+        const float c1[4] = {t1.x, t1.y, t1.z, t1.w};
+        const float c2[4] = {t2.x, t2.y, t2.z, t2.w};
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          num[j] = fmaf(x1, p1[j], num[j]);
-          den[j] = fmaf(x2, p2[j], den[j]);
+        for (int t = 0; t < 4; ++t) {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            num[j] = fmaf(x1[j + t], c1[t], num[j]);
+            den[j] = fmaf(x2[j + t], c2[t], den[j]);
+          }
         }
+        u1 = v1;
+        u2 = v2;
       }
     }
-    const int ox = tx0 + lx;
+    const int oy = ty0 + ly;
     const float nanv = __int_as_float(0x7fc00000);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int oy = ty0 + ly4 + j;
+      const int ox = tx0 + lx + j;
       if (oy >= h || ox >= w) continue;
       const size_t gi = off + (size_t)(y0 + oy) * a.W + x0 + ox;
       const bool gd = a.good[gi] && a.invvar[gi] > 0.0f;
