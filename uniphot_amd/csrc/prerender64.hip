@@ -174,11 +174,59 @@ __device__ inline bool member_window(float xf, float yf, float x0f, float y0f, i
 // loads of all 4 issued before their FMAs (term order per pixel, as
 // psf_poly_at).  Loads past the box or the last term read an in-bounds
 // coefficient and are zeroed, so none is predicated.
-__device__ __forceinline__ void psf_box64(const float* Cf, int T, int nt, const double* bas,
-                                          bool grad, double xs, double ys, int ry0, int rx0,
-                                          int RH, int RW, double* P, double* Px, double* Py) {
+// NT > 0: the term count known at compile time, 3 pixels per round with
+// all their 3 NT loads in flight together; NT = 0: any nt, 4 pixels per
+// round in groups of 8 terms.
+template <int NT>
+__device__ __forceinline__ void psf_box64_t(const float* Cf, int T, int nt, const double* bas,
+                                            bool grad, double xs, double ys, int ry0, int rx0,
+                                            int RH, int RW, double* P, double* Px, double* Py) {
   const int n = RH * RW;
   const size_t TT = (size_t)T * T;
+  if constexpr (NT > 0) {
+    constexpr int kPx = 3;  // pixels per round: 3 NT loads in flight per thread
+    for (int e0 = threadIdx.x; e0 < n; e0 += kPx * UPH_BLOCK) {
+      int off[kPx];
+      bool ok[kPx];
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) {
+        const int e = e0 + j * UPH_BLOCK;
+        ok[j] = e < n;
+        off[j] = ok[j] ? (ry0 + e / RW) * T + rx0 + e % RW : 0;
+      }
+      float cv[kPx][NT];
+#pragma unroll
+      for (int j = 0; j < kPx; ++j)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) cv[j][t] = Cf[(size_t)t * TT + off[j]];
+      double v[kPx], vx[kPx], vy[kPx];
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) v[j] = vx[j] = vy[j] = 0.0;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int j = 0; j < kPx; ++j) {
+          const double c = double(cv[j][t]);
+          v[j] = fma(c, bas[t], v[j]);
+          if (grad) {
+            vx[j] = fma(c, bas[21 + t], vx[j]);
+            vy[j] = fma(c, bas[42 + t], vy[j]);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kPx; ++j) {
+        if (!ok[j]) continue;
+        const int e = e0 + j * UPH_BLOCK;
+        P[e] = v[j];
+        if (grad) {
+          Px[e] = vx[j] / xs;
+          Py[e] = vy[j] / ys;
+        }
+      }
+    }
+    return;
+  }
   for (int e0 = threadIdx.x; e0 < n; e0 += 4 * UPH_BLOCK) {
     int off[4];
     bool ok[4];
@@ -224,6 +272,17 @@ __device__ __forceinline__ void psf_box64(const float* Cf, int T, int nt, const 
       }
     }
   }
+}
+
+__device__ __forceinline__ void psf_box64(const float* Cf, int T, int nt, const double* bas,
+                                          bool grad, double xs, double ys, int ry0, int rx0,
+                                          int RH, int RW, double* P, double* Px, double* Py) {
+  if (nt == 15)
+    psf_box64_t<15>(Cf, T, nt, bas, grad, xs, ys, ry0, rx0, RH, RW, P, Px, Py);
+  else if (nt == 10)
+    psf_box64_t<10>(Cf, T, nt, bas, grad, xs, ys, ry0, rx0, RH, RW, P, Px, Py);
+  else
+    psf_box64_t<0>(Cf, T, nt, bas, grad, xs, ys, ry0, rx0, RH, RW, P, Px, Py);
 }
 
 // Star through LDS: the box planes (P, and Px, Py with `grad`) in sl, then
