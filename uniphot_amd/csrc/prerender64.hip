@@ -996,7 +996,9 @@ __global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_kernel(IslandArgsPre a
 // The extended members of tier 111 (pad 256), from their own work queue
 // (the counter after the planes queue; only extended-member items are
 // visited).
-__global__ __launch_bounds__(UPH_BLOCK) void pre64_planes_big_kernel(IslandArgsPre a) {
+// 3 workgroups per CU: the register cap (168) costs a few spills and buys
+// 3 waves per SIMD for this latency-bound kernel
+__global__ __launch_bounds__(UPH_BLOCK, 3) void pre64_planes_big_kernel(IslandArgsPre a) {
   __shared__ double taps[28];
   __shared__ double bas[63];
   __shared__ long long s_item;
