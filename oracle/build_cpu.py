@@ -19,7 +19,7 @@ FLAGS = ["-O3", "-march=x86-64-v3", "-mtune=generic", "-fcx-limited-range", "-fo
 
 def build() -> Path:
     for lib, extra in ((LIB, []), (LIB64, ["-DUPHC_REAL=double"])):
-        tmp = lib.with_suffix(".so.tmp")
+        tmp = lib.with_suffix(f".so.tmp{os.getpid():x}")   # one per process: parallel test workers
         cmd = [os.environ.get("CXX", "g++"), *FLAGS, *extra, str(SRC), "-o", str(tmp)]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
