@@ -535,6 +535,73 @@ int uph_band_psf_flags(int n, int n_bands, int n_terms, int T, const int32_t* si
                        int H, int W, const uint8_t* nodata, const uint8_t* has_nodata,
                        float* peak, float* vmin, double* uncovered, void* stream);
 
+/*
+ * Correlated-noise variance factor F_corr of the post-fit error model
+ * (replaces the numerics of measure_correlated_noise_factor and
+ * measure_correlated_noise_factor_residual, tract7dt/fit/grid_gpu/
+ * error_model.py:933-1113, :1116-1302; the host combines the passes below
+ * as those functions do).  Maps: float64 images packed back to back, map k
+ * of map_hw[k] = (h, w) pixels starting at map_off[k]; max_pix >= every
+ * h * w.
+ *
+ * uph_fcorr_load: v[k] = float64(src[src_off[k] + y * src_stride[k] + x]),
+ * g0 = finite & good (good_u8 != 0 at the same index, or good_f32 > 0.5;
+ * exactly one of the two is given); n_good[k] = count of g0.
+ */
+int uph_fcorr_load(int n_maps, int64_t max_pix, const int64_t* map_off, const int32_t* map_hw,
+                   const float* src, const int64_t* src_off, const int32_t* src_stride,
+                   const uint8_t* good_u8, const float* good_f32, double* v, uint8_t* g0,
+                   int64_t* n_good, void* stream);
+/*
+ * uph_fcorr_select: out[q] = the q_rank[q]-th smallest (0-based) of v over
+ * g0 of map q_map[q], or of |v - q_center[q]| where q_center[q] is not NaN
+ * (exact: radix select on order-preserving keys).  The medians of np.median
+ * are the mean of ranks (n-1)/2 and n/2.  work: uph_fcorr_select_work_words
+ * (n_q) 32-bit words.
+ */
+size_t uph_fcorr_select_work_words(int n_q);
+int uph_fcorr_select(int n_q, int64_t max_pix, const int32_t* q_map, const int64_t* q_rank,
+                     const double* q_center, const int64_t* map_off, const int32_t* map_hw,
+                     const double* v, const uint8_t* g0, uint32_t* work, size_t work_words,
+                     double* out, void* stream);
+/*
+ * uph_fcorr_clean: struct = !g0 | (symmetric ? |v - med| > clip mad :
+ * v > med + clip mad | v < med - low mad) with med[k], mad[k];
+ * clean = no struct pixel within |dy| + |dx| <= 2 (scipy binary_dilation by
+ * the 4-neighbour cross, iterations = 2, border not struct), inverted.
+ */
+int uph_fcorr_clean(int n_maps, int64_t max_pix, const int64_t* map_off, const int32_t* map_hw,
+                    const double* v, const uint8_t* g0, const double* med, const double* mad,
+                    double clip, double low, int symmetric, uint8_t* clean, void* stream);
+/*
+ * uph_fcorr_tile_stats: tile t = the ts x ts block at tile_yx[t] = (y0, x0)
+ * of map tile_map[t] (clipped at the map edge): stats[t] = (count, sum, sum
+ * of squared deviations from the tile mean) of v over mask.
+ */
+int uph_fcorr_tile_stats(int n_tiles, const int32_t* tile_map, const int32_t* tile_yx, int ts,
+                         const int64_t* map_off, const int32_t* map_hw, const double* v,
+                         const uint8_t* mask, double* stats, void* stream);
+/*
+ * uph_fcorr_mask: with tile t = tile_base[k] + (y / ts) * ceil(w / ts) +
+ * x / ts: m = mask & keep[t]; v = m ? v - tile_mean[t] : 0; mask = m.
+ */
+int uph_fcorr_mask(int n_maps, int64_t max_pix, const int64_t* map_off, const int32_t* map_hw,
+                   const int32_t* tile_base, int ts, const uint8_t* keep, const double* tile_mean,
+                   double* v, uint8_t* mask, void* stream);
+/*
+ * uph_fcorr_autocov: circular autocovariance of (x, m) over the half window
+ * of lags: A(d) = sum_i x_i x_{i+d}, K(d) = sum_i m_i m_{i+d} (indices mod
+ * h, w) for lag (0, dx), dx = 0..L at index dx, and (dy, dx), dy = 1..L,
+ * dx = -L..L at index (L + 1) + (dy - 1)(2L + 1) + dx + L; out[k] =
+ * (A[nlag], K[nlag]) with nlag = uph_fcorr_nlag(L); L <= 56, L < h / 2 and
+ * w / 2.  bpm workgroups per map each write partial sums (partial: n_maps *
+ * bpm * 2 * nlag doubles), reduced in a fixed order.
+ */
+int uph_fcorr_nlag(int L);
+int uph_fcorr_autocov(int n_maps, int bpm, int L, const int64_t* map_off, const int32_t* map_hw,
+                      const double* x, const uint8_t* m, double* partial, size_t partial_doubles,
+                      double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -24,7 +24,9 @@ EXPORTS = (
     "uph_wing_images", "uph_blank_work_floats", "uph_blank_response", "uph_stamp_sum",
     "uph_gather_windows", "uph_subtract_bg_planes", "uph_band_solve_noalloc",
     "uph_band_psf_flags", "uph_render_direct_scratch_bytes", "uph_render_stamps_direct",
-    "uph_confusion_work_floats", "uph_confusion",
+    "uph_confusion_work_floats", "uph_confusion", "uph_fcorr_load", "uph_fcorr_select_work_words",
+    "uph_fcorr_select", "uph_fcorr_clean", "uph_fcorr_tile_stats", "uph_fcorr_mask",
+    "uph_fcorr_nlag", "uph_fcorr_autocov",
 )
 UPH_BAND_NOALLOC_BYTES = 64 << 20
 
@@ -171,6 +173,23 @@ def lib() -> C.CDLL:
     L.uph_confusion.argtypes = ([C.c_int, C.c_int64, C.c_int, C.c_int, _P, _P, C.c_int, C.c_int]
                                 + [_P] * 10 + [C.c_double] * 4 + [_P] * 9
                                 + [C.c_size_t, C.c_int, _P, _P])
+    L.uph_fcorr_load.restype = C.c_int
+    L.uph_fcorr_load.argtypes = [C.c_int, C.c_int64] + [_P] * 11
+    L.uph_fcorr_select_work_words.restype = C.c_size_t
+    L.uph_fcorr_select_work_words.argtypes = [C.c_int]
+    L.uph_fcorr_select.restype = C.c_int
+    L.uph_fcorr_select.argtypes = [C.c_int, C.c_int64] + [_P] * 8 + [C.c_size_t, _P, _P]
+    L.uph_fcorr_clean.restype = C.c_int
+    L.uph_fcorr_clean.argtypes = [C.c_int, C.c_int64] + [_P] * 6 + [C.c_double, C.c_double,
+                                                                    C.c_int, _P, _P]
+    L.uph_fcorr_tile_stats.restype = C.c_int
+    L.uph_fcorr_tile_stats.argtypes = [C.c_int, _P, _P, C.c_int] + [_P] * 6
+    L.uph_fcorr_mask.restype = C.c_int
+    L.uph_fcorr_mask.argtypes = [C.c_int, C.c_int64, _P, _P, _P, C.c_int] + [_P] * 5
+    L.uph_fcorr_nlag.restype = C.c_int
+    L.uph_fcorr_nlag.argtypes = [C.c_int]
+    L.uph_fcorr_autocov.restype = C.c_int
+    L.uph_fcorr_autocov.argtypes = [C.c_int] * 3 + [_P] * 5 + [C.c_size_t, _P, _P]
     _lib = L
     return L
 
