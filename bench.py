@@ -742,6 +742,9 @@ def main():
                                  v32 * a32.get("fit", {}).get("frac_le_1e5", float("nan")), 2)}
         print(f"[bench] fp32 only {line['fp32_only']}", file=sys.stderr, flush=True)
     line["config"]["fp64_refit_islands"] = res.record["n_fp64_refit_islands"]
+    if "fcorr" in res.record:
+        fc = res.record["fcorr"]
+        line["config"]["fcorr"] = {"F": [round(f, 5) for f in fc["F"]], "source": fc["source"]}
     if not args.no_cpu:
         cmp = []
         t_cpu = time.perf_counter()

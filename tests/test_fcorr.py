@@ -155,3 +155,35 @@ def test_shape_gate_matches_the_oracle():
     w = of.psf_autocorr(_gauss_psf(), L)
     assert uf.shape_gate(rho, w, L)[:2] == pytest.approx(of.shape_gate(rho, w, L)[:2],
                                                          nan_ok=True)
+
+
+def test_geometric_error_columns_known_answer():
+    import pandas as pd
+    from uniphot_amd.error_model import add_error_columns, add_geometric_error_columns
+    out = pd.DataFrame({
+        "FLUX_g_fit": [10.0, 0.0, np.nan], "FLUXERR_g_fit": [1.0, 1.0, 1.0],
+        "FLUX_r_fit": [30.0, 0.0, 5.0], "FLUXERR_r_fit": [1.0, 1.0, 0.0],
+        "x_pix_white_err": [0.1, 0.2, 0.3], "Re_err": [1.0, 2.0, np.nan]})
+    fcorr = np.array([2.0, 4.0])
+    chi2_nu = np.array([4.0, 0.5, np.nan])
+    add_geometric_error_columns(out, ["g", "r"], fcorr, chi2_nu)
+    # row 0: SNR^2 100 and 900 -> F_eff = (200 + 3600) / 1000 = 3.8; row 1: no
+    # band constrains it -> median 3.0; row 2: neither band valid -> 3.0
+    feff = np.array([3.8, 3.0, 3.0])
+    dfac = np.sqrt([4.0, 1.0, 1.0])
+    np.testing.assert_allclose(out["x_pix_white_err_corr"], np.sqrt(feff) * [0.1, 0.2, 0.3],
+                               rtol=1e-6)
+    np.testing.assert_allclose(out["x_pix_white_err_tot"],
+                               np.sqrt(feff) * dfac * [0.1, 0.2, 0.3], rtol=1e-6)
+    assert np.isnan(out["Re_err_tot"][2]) and "ELL_err_tot" not in out.columns
+    np.testing.assert_allclose(out["chi2_nu_nl"], chi2_nu.astype(np.float32))
+    # FLUXERR_MARG_TOT: the increment scaled by G = max(1, F_eff) max(1, chi2_nu)
+    conf = np.array([[0.5, np.nan], [0.0, 0.0], [1.0, 1.0]])
+    dv = np.array([[2.0, 0.0], [1.0, 1.0], [np.nan, 0.0]])
+    add_error_columns(out, ["g", "r"], conf, flux_marg_var=dv, rchisq_scale_marg_tot=False,
+                      fcorr=fcorr)
+    G = feff * np.array([4.0, 1.0, 1.0])
+    np.testing.assert_allclose(out["FLUXERR_MARG_TOT_g_fit"][:2],
+                               np.sqrt(1.0 + dv[:2, 0] * G[:2] + np.nan_to_num(conf[:2, 0]) ** 2),
+                               rtol=1e-6)
+    np.testing.assert_allclose(out["FLUXERR_MARG_TOT_r_fit"][0], np.sqrt(1.0), rtol=1e-6)

@@ -107,3 +107,32 @@ def test_residual_window_factor_matches_oracle():
         assert nc == do["n_clean"]
         assert F == pytest.approx(Fo, rel=1e-10)
     assert out[1][2] == "too few clean tiles" and out[0][0] > 1.5
+
+
+def test_fit_field_carries_the_band_factors():
+    import torch
+    from tests._scenes import small_star_scene
+    from uniphot_amd.fcorr import choose_band_fcorr
+    from uniphot_amd.pipeline import DeviceField, FitConfig, fit_field
+    from uniphot_amd.psf_polynomial import evaluate_psf_at_batch_np
+    sc = small_star_scene(seed=13, H=600, W=600, n_bands=2, n_src=30)
+    fld = DeviceField(sc.data, sc.invvar, sc.good, sc.psfs)
+    res = fit_field(fld, sc.x, sc.y, sc.stype, sc.logre, sc.ee1, sc.ee2, sc.sersic_n,
+                    FitConfig(n_rows=1, n_cols=1), sync=torch.cuda.synchronize)
+    rec = res.record["fcorr"]
+    assert res.fcorr.shape == (2,) and np.all(res.fcorr >= 1.0)
+    for b in range(2):
+        print(f"band {b}: {rec['source'][b]} F={res.fcorr[b]}", flush=True)
+        phi = evaluate_psf_at_batch_np(sc.psfs[b], np.array([300.0]), np.array([300.0]))[0]
+        Fo, do = of.fcorr_sky(np.asarray(sc.data[b], np.float64), ~np.asarray(sc.good[b], bool),
+                              phi.astype(np.float64))
+        assert rec["raw_sky"][b] == pytest.approx(Fo, rel=1e-10)
+        assert rec["raw_sky_fallback"][b] == do["fallback"]
+    # one 600-px grid: 9 residual tiles, so the residual estimator ran on each band
+    assert all(np.isfinite(rec["residual"][b]) or rec["source"][b] != "residual"
+               for b in range(2))
+    F, src, _ = choose_band_fcorr([[(rec["residual"][b], 1, None) if np.isfinite(
+        rec["residual"][b]) else None for b in range(2)]], np.array(rec["raw_sky"]))
+    np.testing.assert_allclose(F, res.fcorr)
+    fitted = np.isfinite(res.x)
+    assert np.isfinite(res.chi2_nu[fitted & (res.island_id >= 0)]).any()

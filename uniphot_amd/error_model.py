@@ -14,10 +14,12 @@ whole per-problem computation (``uph_confusion``).
 columns: FLUXERR_CONF_{b}_fit, FLUXERR_TOT_{b}_fit =
 sqrt(FLUXERR^2 + CONF^2) and FLUXERR_MARG_TOT_{b}_fit =
 sqrt(FLUXERR^2 + dvar + CONF^2) x sqrt(max(1, rchisq)).  The correlated-
-noise factor and the per-island chi2_nu (which only scale the geometric
-error columns and the marginal increment) and the source-Poisson term
-(which needs the images' gain) are not part of this build: their factors
-are 1, as the reference's are when those legs are off.
+noise factor F_corr (measured by :mod:`uniphot_amd.fcorr`) and the
+per-island chi2_nu scale the geometric error columns (``*_err_corr``,
+``*_err_tot``, ``add_geometric_error_columns``, error_columns.py:153-246)
+and the marginal increment inside FLUXERR_MARG_TOT.  The source-Poisson
+term needs the images' gain and is not part of this build: its variance is
+0, as the reference's is when that leg is off.
 """
 from __future__ import annotations
 
@@ -44,11 +46,14 @@ def psf_scale(phi32: np.ndarray):
 
 
 def confusion_sigma(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn, cfg,
-                    bg_geo, wing=None, use_wing=None):
+                    bg_geo, wing=None, use_wing=None, fcorr_out=None):
     """sigma_conf of every fit source of each (grid, band) problem after the
     final solve ``res`` (a list of per-problem arrays, NaN where the annulus
     is too masked or the position is not finite).  ``wing`` (nb, H, W) is
-    subtracted from the data of the problems flagged in ``use_wing``."""
+    subtracted from the data of the problems flagged in ``use_wing``.  With
+    ``fcorr_out`` (a dict) the residual correlated-noise estimate of problem
+    j (error_model.py:1116-1302 on the same residual and good maps and the
+    same unit-sum window PSF) is stored under j as (F, n_clean, fallback)."""
     import torch
     from . import _lib
     from .pipeline import window_psf_stamps
@@ -129,20 +134,81 @@ def confusion_sigma(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2,
         s = sig.cpu().numpy()
         for j, q in enumerate(qs):
             out[q] = s[off[j]:off[j + 1]].copy()
+        if fcorr_out is not None:
+            from .fcorr import fcorr_residual_windows
+            for j, r in enumerate(fcorr_residual_windows(work, max_pix, win, phi)):
+                fcorr_out[int(qs[j])] = r
     return out
 
 
+GEOMETRIC_ERR_BASES = ("x_pix_white_err", "y_pix_white_err", "Re_err", "ELL_err", "THETA_err",
+                       "sersic_n_err")
+
+
+def fcorr_effective(out, bandnames, fcorr):
+    """Per-source F_corr_eff: the SNR^2-weighted mean of the band factors
+    over the bands with a finite flux and a positive flux error, else the
+    median band factor (error_columns.py:166-187)."""
+    import pandas as pd
+    num = lambda c: pd.to_numeric(out[c], errors="coerce").to_numpy(dtype=np.float64)
+    n = len(out)
+    wsum, fnum = np.zeros(n), np.zeros(n)
+    for b, bn in enumerate(bandnames):
+        fl, fe = num(f"FLUX_{bn}_fit"), num(f"FLUXERR_{bn}_fit")
+        with np.errstate(invalid="ignore", divide="ignore"):
+            snr2 = np.where(np.isfinite(fl) & np.isfinite(fe) & (fe > 0), (fl / fe) ** 2, 0.0)
+        wsum += snr2
+        fnum += snr2 * float(fcorr[b])
+    fmed = float(np.median(np.asarray(fcorr, np.float64)))
+    return np.where(wsum > 0, fnum / np.maximum(wsum, 1e-30), fmed)
+
+
+def add_geometric_error_columns(out, bandnames, fcorr, chi2_nu) -> None:
+    """The correlated-noise and reduced-chi2 legs of the geometric errors,
+    in place (error_columns.py:153-246, pipeline.py:2548-2562, :2877-2881):
+    chi2_nu_nl = the island's reduced chi2; {base}_corr = sqrt(max(1,
+    F_corr_eff)) x {base} (with ``fcorr``) and {base}_tot = sqrt(max(1,
+    F_corr_eff)) x sqrt(max(1, chi2_nu_nl)) x {base} (F_corr_eff = 1 without
+    ``fcorr``; the chi2 factor 1 where chi2_nu_nl is not finite) for the
+    position, shape and Sersic-n errors."""
+    import pandas as pd
+    num = lambda c: pd.to_numeric(out[c], errors="coerce").to_numpy(dtype=np.float64)
+    n = len(out)
+    cn = (np.asarray(chi2_nu, np.float64) if chi2_nu is not None else np.full(n, np.nan))
+    out["chi2_nu_nl"] = cn.astype(np.float32)
+    if fcorr is not None:
+        c_fac = np.sqrt(np.maximum(fcorr_effective(out, bandnames, fcorr), 1.0))
+    else:
+        c_fac = np.ones(n)
+    d_fac = np.sqrt(np.where(np.isfinite(cn), np.maximum(cn, 1.0), 1.0))
+    for base in GEOMETRIC_ERR_BASES:
+        if base not in out.columns:
+            continue
+        raw = num(base)
+        if fcorr is not None:
+            out[f"{base}_corr"] = (c_fac * raw).astype(np.float32)
+        out[f"{base}_tot"] = (c_fac * d_fac * raw).astype(np.float32)
+
+
 def add_error_columns(out, bandnames, flux_err_conf, *, flux_marg_var=None,
-                      rchisq_scale_marg_tot: bool = True) -> None:
+                      rchisq_scale_marg_tot: bool = True, fcorr=None) -> None:
     """FLUXERR_CONF / FLUXERR_TOT / FLUXERR_MARG_TOT per band, in place
-    (error_columns.py:134-150, :250-363 with the correlated-noise,
-    chi2_nu and source-Poisson factors at their neutral values).
-    ``flux_err_conf`` (n, nb); ``flux_marg_var`` (n, nb) the shape-
-    marginalisation increment (0 for stars, NaN off-island)."""
+    (error_columns.py:134-150, :250-363; the source-Poisson term, which needs
+    the images' gain, at its neutral value).  ``flux_err_conf`` (n, nb);
+    ``flux_marg_var`` (n, nb) the shape-marginalisation increment (0 for
+    stars, NaN off-island), scaled in FLUXERR_MARG_TOT by the geometric
+    inflation G = max(1, F_corr_eff) x max(1, chi2_nu_nl) (``fcorr`` the
+    band factors, or None for F_corr_eff = 1; the chi2 factor where the
+    chi2_nu_nl column exists)."""
     import pandas as pd
     num = lambda c: pd.to_numeric(out[c], errors="coerce").to_numpy(dtype=np.float64)
     conf = np.asarray(flux_err_conf, np.float64)
     n = len(out)
+    g_corr = (np.maximum(fcorr_effective(out, bandnames, fcorr), 1.0) if fcorr is not None
+              else np.ones(n))
+    if "chi2_nu_nl" in out.columns:
+        cnu = num("chi2_nu_nl")
+        g_corr = g_corr * np.where(np.isfinite(cnu), np.maximum(cnu, 1.0), 1.0)
     scale = None
     if rchisq_scale_marg_tot and "rchisq" in out.columns:
         rq = num("rchisq")
@@ -156,7 +222,7 @@ def add_error_columns(out, bandnames, flux_err_conf, *, flux_marg_var=None,
         if flux_marg_var is None:
             continue
         dv = np.asarray(flux_marg_var, np.float64)[:, b] if n else np.zeros(0)
-        tot = np.where(dv == 0.0, np.sqrt(fe * fe + fc2), np.sqrt(fe * fe + dv + fc2))
+        tot = np.where(dv == 0.0, np.sqrt(fe * fe + fc2), np.sqrt(fe * fe + dv * g_corr + fc2))
         if scale is not None:
             tot = tot * scale
         out[f"FLUXERR_MARG_TOT_{bn}_fit"] = tot.astype(np.float32)

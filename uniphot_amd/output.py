@@ -21,9 +21,10 @@ shape-marginalised error columns of ``error_columns.py`` (:255-341):
   * ``fit_is_reliable`` / ``fit_quality_issues`` (``merge.py:16-344``).
   * the error model's flux columns (FLUXERR_CONF, FLUXERR_TOT,
     FLUXERR_MARG_TOT; :mod:`uniphot_amd.error_model`) when the fit carries
-    the confusion term.
-The correlated-noise and chi2_nu scalings of the geometric errors (the
-*_err_corr / *_err_tot family) are not part of this build.
+    the confusion term;
+  * chi2_nu_nl and the geometric error columns scaled by the correlated-
+    noise factor and the island's reduced chi2 (``*_err_corr``,
+    ``*_err_tot``; error_columns.py:153-246).
 """
 from __future__ import annotations
 
@@ -172,10 +173,13 @@ def write_fit_columns(catalog, fit, bandnames, *, x_init, y_init, source_type,
     has_fit = np.isfinite(out["x_pix_white_fit"].to_numpy(float))
     out["flag_slow_converge"] = slow_converge_flag(out["opt_niters"].to_numpy(), has_fit,
                                                    max_steps_main, max_steps)
+    from .error_model import add_error_columns, add_geometric_error_columns
+    fcorr = getattr(fit, "fcorr", None)
+    add_geometric_error_columns(out, list(bandnames), fcorr, getattr(fit, "chi2_nu", None))
     if getattr(fit, "flux_err_conf", None) is not None:
-        from .error_model import add_error_columns
         add_error_columns(out, list(bandnames), fit.flux_err_conf,
-                          flux_marg_var=fit.flux_marg_var if marginal_flux_err else None)
+                          flux_marg_var=fit.flux_marg_var if marginal_flux_err else None,
+                          fcorr=fcorr)
     if band_psfs is not None:
         from .band_flags import apply_band_flags
         out.attrs["band_flags"] = apply_band_flags(out, list(bandnames), list(band_psfs),

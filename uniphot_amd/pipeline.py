@@ -91,6 +91,10 @@ class FitConfig:
     # post-fit error model: confusion flux error of every source after
     # pass 2 (error_model.py:663-930, on by default in the reference)
     confusion: bool = True
+    # correlated-noise factor F_corr per band (error_model.py:933-1302,
+    # pipeline.py:2411-2458; correlated_noise_enabled, on by default): the
+    # residual estimate of each pass-2 confusion window, else the raw-sky one
+    correlated_noise: bool = True
 
 
 @dataclass
@@ -128,6 +132,8 @@ class FitResult:
     free_n: np.ndarray = None             # (n,) Sersic n fitted
     grid_id: np.ndarray = None            # (n,) grid owning the source
     flux_err_conf: np.ndarray = None      # (n, nb) confusion flux error (error model)
+    fcorr: np.ndarray = None              # (nb,) correlated-noise variance factor F_corr
+    chi2_nu: np.ndarray = None            # (n,) reduced chi2 of the source's island
 
 
 class DeviceField:
@@ -428,7 +434,7 @@ def wing_peel(fld, T, problems, owners, res, x, y, stype, logre, ee1, ee2, sn,
 
 def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: FitConfig,
                 gtier: dict, timer=None, final=False, bg_geo=None, with_bg_var=False,
-                conf_out=None):
+                conf_out=None, fcorr_out=None):
     """Per-grid, per-band linear solve; returns fluxes/var (n, nb) for each
     source from the grid that owns it, and bg coefs (n_grid, nb, M).  The
     grids, their fit sets and stamp tiers are those of pass 1 (pass 2 only
@@ -437,7 +443,10 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
     to the field's centre and half-size (pipeline.py:1693-1696).  With
     ``final`` and ``conf_out`` (an (n, nb) array) the confusion flux error
     of each owned source is written there (error_model.confusion_sigma,
-    after the blank-response correction; native stamps only)."""
+    after the blank-response correction; native stamps only); with
+    ``fcorr_out`` (a dict) the residual correlated-noise estimate of each
+    such (grid, band) window is stored there under (grid_id, band) as
+    (F, n_clean, fallback)."""
     n = x.size
     nb = fld.nb
     M = (cfg.bg_order + 1) * (cfg.bg_order + 2) // 2
@@ -484,13 +493,18 @@ def linear_pass(fld: DeviceField, grids, x, y, stype, logre, ee1, ee2, sn, cfg: 
                 done = np.zeros(len(part), bool)
 
                 def post(sel, rs, wing, a=a):
+                    fq = {} if fcorr_out is not None else None
                     sig = confusion_sigma(fld, T, [problems[a + q] for q in sel],
                                           [owners[a + q] for q in sel], rs, x, y, stype, logre,
                                           ee1, ee2, sn, cfg, bg_geo, wing=wing,
                                           use_wing=(None if wing is None
-                                                    else np.ones(len(sel), np.uint8)))
-                    for q, sg in zip(sel, sig):
+                                                    else np.ones(len(sel), np.uint8)),
+                                          fcorr_out=fq)
+                    for j, (q, sg) in enumerate(zip(sel, sig)):
                         conf_q[a + q] = sg
+                        if fq is not None:
+                            g_, b_ = owners[a + q]
+                            fcorr_out[(g_.grid_id, b_)] = fq[j]
 
                 if cfg.wing_peeling:
                     part, done = wing_peel(fld, T, problems[a:z], owners[a:z], part, x, y,
@@ -582,6 +596,7 @@ class IslandSetFit:
     re_upper_pix: np.ndarray = None   # (n,) exp(upper bound of logre)
     diverged: np.ndarray = None       # (n,) stalled on identical (clip-pinned) proposals
     stall_dchi2: np.ndarray = None    # (n,) best damped-step chi2 change of a stalled island
+    chi2_nu: np.ndarray = None        # (n,) the island's chi2 / (good pixels - free parameters)
 
 
 def metrics_from_sums(sums):
@@ -740,6 +755,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
     kap = np.full(n, np.nan, np.float32)
     div = np.zeros(n, bool)
     sdc = np.full(n, np.nan, np.float32)
+    c2nu = np.full(n, np.nan, np.float32)
     classes, batch, trees = [], [], {}
     for plan in plans:
         host = class_host_arrays(plan, pos_x=x, pos_y=y, source_type=stype, logre=logre,
@@ -877,6 +893,16 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
             if "stall_dchi2" in res:
                 div[src] = res["diverged"][r_of]
                 sdc[src] = res["stall_dchi2"][r_of]
+            if "n_good" in res:
+                # reduced chi2 of the island (island_plan.py:888-911): dof = good pixels -
+                # fitted nonlinear parameters - n_bands x members
+                n_par = (host["keep"][rows] & mm[:, :, None]).reshape(rows.size, -1).sum(axis=1)
+                dof = (np.asarray(res["n_good"], np.float64) - n_par
+                       - float(res["fluxes"].shape[1]) * mm.sum(axis=1))
+                c2 = np.asarray(res["chi2"], np.float64)
+                with np.errstate(invalid="ignore", divide="ignore"):
+                    cn = np.where((dof > 0) & np.isfinite(c2), c2 / np.maximum(dof, 1.0), np.nan)
+                c2nu[src] = cn[r_of]
             if "post_var" in res:
                 pvar[src] = res["post_var"][mm]
                 dv = res["post_dvar"].transpose(0, 2, 1)[mm]
@@ -890,7 +916,7 @@ def fit_island_set(psf_of, nb, n_terms, native, dstack, wstack, pad, iset, plans
                         n_fit, classes, timing, pvar, ferr_m, mets, fwin, fp64_refit=f64src,
                         n_fp64_islands=n_fp64, kappa=kap, diverged=div, stall_dchi2=sdc,
                         class_results=kept if keep_class_results else None,
-                        flux_marg_var=dvar, re_upper_pix=re_up)
+                        flux_marg_var=dvar, re_upper_pix=re_up, chi2_nu=c2nu)
 
 
 def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: FitConfig,
@@ -966,8 +992,17 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
     # ---- pass 2 ----
     t3 = time.perf_counter()
     conf = np.full((n, fld.nb), np.nan, np.float32) if cfg.confusion else None
+    fc_res = {} if (cfg.confusion and cfg.correlated_noise) else None
     flux, var, _ = linear_pass(fld, grids, fx, fy, stype, flr, fe1, fe2, fsn, cfg, gtier, timer,
-                               final=True, conf_out=conf)
+                               final=True, conf_out=conf, fcorr_out=fc_res)
+    fcorr = None
+    if cfg.correlated_noise:
+        from .fcorr import choose_band_fcorr, fcorr_sky
+        raw = fcorr_sky(fld)
+        per_grid = []
+        for g in grids:
+            per_grid.append([(fc_res or {}).get((g.grid_id, b)) for b in range(fld.nb)])
+        fcorr, fc_src, fc_resid = choose_band_fcorr(per_grid, np.array([f for f, _ in raw]))
     sync()
     tm["pass2"] = time.perf_counter() - t3
     tm["total"] = time.perf_counter() - t0
@@ -982,8 +1017,14 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                + ([] if cfg.blank_response else ["blank_response_correction"])
                + ([] if cfg.wing_peeling else ["wing_peeling"])
                + ([] if cfg.confusion else ["confusion"])
+               + ([] if cfg.correlated_noise else ["correlated_noise"])
                + ([] if cfg.post_fit else ["param_errors", "source_metrics",
                                            "marginal_flux_err"]))
+    if fcorr is not None:
+        rec["fcorr"] = dict(F=[float(f) for f in fcorr], source=fc_src,
+                            residual=[float(f) for f in fc_resid],
+                            raw_sky=[float(f) for f, _ in raw],
+                            raw_sky_fallback=[d.get("fallback") for _, d in raw])
     if keep_artifacts:
         rec["_artifacts"] = dict(classes=classes, dstack=dstack, dstack_raw=dstack_raw,
                                  scene=scene, wstack=wstack, pad=pad,
@@ -996,7 +1037,7 @@ def fit_field(fld: DeviceField, x, y, stype, logre, ee1, ee2, sersic_n, cfg: Fit
                      stall & (isf.stall_dchi2 > -2.0 * PLATEAU_DLNP_TOL), isf.fp64_refit,
                      flux_var=var, flux_marg_var=isf.flux_marg_var,
                      re_upper_pix=isf.re_upper_pix, free_n=np.asarray(free_n, bool),
-                     grid_id=owner, flux_err_conf=conf)
+                     grid_id=owner, flux_err_conf=conf, fcorr=fcorr, chi2_nu=isf.chi2_nu)
 
 
 def psf_bank(psfs, device="cuda"):
