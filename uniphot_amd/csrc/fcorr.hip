@@ -248,7 +248,13 @@ __global__ __launch_bounds__(UPH_BLOCK) void fc_mask_kernel(
 __host__ __device__ inline int fc_nr0(int L) { return (L + 1 + kFcLagRun - 1) / kFcLagRun; }
 __host__ __device__ inline int fc_nr1(int L) { return (2 * L + 1 + kFcLagRun - 1) / kFcLagRun; }
 __host__ __device__ inline int fc_nlag(int L) { return (L + 1) + L * (2 * L + 1); }
-__host__ __device__ inline int fc_sw(int L) { return kFcCols + 2 * L + kFcLagRun; }
+// LDS row stride of the staged rows: at least 64 + 2L + 8 columns, rounded up to 1
+// mod 32 doubles so that lanes reading the same column of adjacent rows (adjacent
+// row lags) hit distinct banks
+__host__ __device__ inline int fc_sw(int L) {
+  const int need = kFcCols + 2 * L + kFcLagRun;
+  return (need + 30) / 32 * 32 + 1;
+}
 __host__ __device__ inline size_t fc_lds_bytes(int L) {
   return (size_t)(kFcRows + L) * fc_sw(L) * (sizeof(double) + sizeof(float)) +
          (size_t)kFcRows * kFcCols * (sizeof(double) + sizeof(float));
@@ -270,7 +276,9 @@ __global__ __launch_bounds__(kFcBlock) void fc_autocov_kernel(const int64_t* map
   const int64_t o = map_off[k];
   const int nr0 = fc_nr0(L), nr1 = fc_nr1(L), nruns = nr0 + L * nr1, nlag = fc_nlag(L);
   const int ntx = (w + kFcCols - 1) / kFcCols, nty = (h + kFcRows - 1) / kFcRows;
-  // this thread's lag runs (up to two): dy, first dx
+  // this thread's lag runs (up to two): dy, first dx.  Adjacent runs past the
+  // dy = 0 row step through dy first, so a wave's lanes read one column of
+  // adjacent staged rows
   int dy[2], dx0[2];
   bool act[2];
 #pragma unroll
@@ -282,8 +290,8 @@ __global__ __launch_bounds__(kFcBlock) void fc_autocov_kernel(const int64_t* map
       dx0[j] = r * kFcLagRun;
     } else {
       const int rr = r - nr0;
-      dy[j] = act[j] ? 1 + rr / nr1 : 0;
-      dx0[j] = act[j] ? -L + (rr % nr1) * kFcLagRun : 0;
+      dy[j] = act[j] ? 1 + rr % L : 0;
+      dx0[j] = act[j] ? -L + (rr / L) * kFcLagRun : 0;
     }
   }
   double accA[2][kFcLagRun];
@@ -317,26 +325,27 @@ __global__ __launch_bounds__(kFcBlock) void fc_autocov_kernel(const int64_t* map
       AM[e] = (in && m[i]) ? 1.0f : 0.0f;
     }
     __syncthreads();
+    for (int y = 0; y < kFcRows; ++y) {
+      const double* arow = AX + y * kFcCols;
+      const float* amrow = AM + y * kFcCols;
+      for (int xc = 0; xc < kFcCols; xc += kFcXu) {
+        double a[kFcXu];
+        float am[kFcXu];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      if (!act[j]) continue;
-      for (int y = 0; y < kFcRows; ++y) {
-        const double* arow = AX + y * kFcCols;
-        const float* amrow = AM + y * kFcCols;
-        const double* brow = BX + (size_t)(y + dy[j]) * SW + L + dx0[j];
-        const float* bmrow = BM + (size_t)(y + dy[j]) * SW + L + dx0[j];
-        for (int xc = 0; xc < kFcCols; xc += kFcXu) {
-          double a[kFcXu], b[kFcXu + kFcLagRun - 1];
-          float am[kFcXu], bm[kFcXu + kFcLagRun - 1];
+        for (int i = 0; i < kFcXu; ++i) {  // one address across the wave: broadcast
+          a[i] = arow[xc + i];
+          am[i] = amrow[xc + i];
+        }
 #pragma unroll
-          for (int i = 0; i < kFcXu; ++i) {
-            a[i] = arow[xc + i];
-            am[i] = amrow[xc + i];
-          }
+        for (int j = 0; j < 2; ++j) {
+          if (!act[j]) continue;
+          const int bo = (y + dy[j]) * SW + L + dx0[j] + xc;
+          double b[kFcXu + kFcLagRun - 1];
+          float bm[kFcXu + kFcLagRun - 1];
 #pragma unroll
           for (int i = 0; i < kFcXu + kFcLagRun - 1; ++i) {
-            b[i] = brow[xc + i];
-            bm[i] = bmrow[xc + i];
+            b[i] = BX[bo + i];
+            bm[i] = BM[bo + i];
           }
 #pragma unroll
           for (int i = 0; i < kFcXu; ++i)
