@@ -602,6 +602,61 @@ int uph_fcorr_autocov(int n_maps, int bpm, int L, const int64_t* map_off, const 
                       const double* x, const uint8_t* m, double* partial, size_t partial_doubles,
                       double* out, void* stream);
 
+/*
+ * PSF model build from training stars (replaces the numerics of
+ * _extract_centered_stamps_gw, _annulus_median, _gw_centroid_vec,
+ * fourier_shift, _fit_polynomial_one_iter, _per_star_chi2 and the pixelwise
+ * median of _replace_model_wings_with_median, tract7dt/fit/grid_gpu/
+ * psf_from_stars.py:1478-2147).  image (H, W) f32, bad (H, W) u8.
+ *
+ * uph_psf_cutouts: star k at the pixel (ox[k], oy[k]): status[k] = 1 if the
+ * (2 half + 1)^2 box leaves the image, 2 if it holds a bad pixel, else 0 with
+ * bkg[k] = the sigma-clipped annulus median (float32 values, finite and not
+ * bad, with rin^2 <= r^2 <= rout^2; median, 1.4826 MAD, keep med +- sigma MAD,
+ * the kept median; the median where the MAD is 0 or fewer than minpix are
+ * kept; NaN below minpix; NaN without do_bkg) and cut[k] = box - float32(bkg)
+ * (the box where bkg is NaN).  half <= 30, (2 ceil(rout) + 1)^2 <= 4096, minpix >= 1.
+ */
+int uph_psf_cutouts(int n, int H, int W, const float* image, const uint8_t* bad,
+                    const int32_t* ox, const int32_t* oy, int half, double rin, double rout,
+                    double sigma, int minpix, int do_bkg, uint8_t* status, double* bkg, float* cut,
+                    void* stream);
+/*
+ * uph_psf_centroids: Gaussian-weighted centroid of max(cut[k], 0) (weight
+ * FWHM fwhm) from the box centre, iters iterations; hist[(k * iters + i) * 3]
+ * = (x, y, largest coordinate move) after iteration i (the centre kept
+ * where the weighted sum is 0).
+ */
+int uph_psf_centroids(int n, int half, double fwhm, int iters, const float* cut, double* hist,
+                      void* stream);
+/*
+ * uph_psf_stamps: star k centred at (fx[k], fy[k]): the box at the rounded
+ * centre (round half to even; status 1 off the image, 2 on a bad pixel, 3 for
+ * a non-finite centre), minus its background (bkg[k] where the rounded centre
+ * is (ox[k], oy[k]), else the annulus median there), moved by minus the
+ * fractional part with the Fourier shift of the box zero-padded by pad
+ * (2 half + 1 + 2 pad odd), the central psf x psf crop divided by its sum
+ * into stamps[k] (status 4 where the sum is not positive and finite).
+ */
+int uph_psf_stamps(int n, int H, int W, const float* image, const uint8_t* bad, const int32_t* ox,
+                   const int32_t* oy, const double* fx, const double* fy, const double* bkg,
+                   int half, int psf, int pad, double rin, double rout, double sigma, int minpix,
+                   int do_bkg, uint8_t* status, float* stamps, void* stream);
+/*
+ * uph_psf_polyfit: coefs[p * nt + t] = float32(sum_s pinv[t * ns + s] *
+ * stamps[sel[s] * P + p]) (float64 sums; pinv the pseudo-inverse of the
+ * selected stars' design matrix).  uph_psf_chi2: chi2[s] = sum_p
+ * (stamps[s * P + p] - sum_t A[s * nt + t] coefs[p * nt + t])^2.  nt <= 21.
+ * uph_psf_pixel_median: med[p] = median of stamps[sel[s] * P + p] over the
+ * ns <= 4096 selected stamps, in float64.
+ */
+int uph_psf_polyfit(int ns, int nt, int P, const int32_t* sel, const double* pinv,
+                    const float* stamps, float* coefs, void* stream);
+int uph_psf_chi2(int n, int nt, int P, const double* A, const float* stamps, const float* coefs,
+                 double* chi2, void* stream);
+int uph_psf_pixel_median(int ns, int P, const int32_t* sel, const float* stamps, double* med,
+                         void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -26,7 +26,8 @@ EXPORTS = (
     "uph_band_psf_flags", "uph_render_direct_scratch_bytes", "uph_render_stamps_direct",
     "uph_confusion_work_floats", "uph_confusion", "uph_fcorr_load", "uph_fcorr_select_work_words",
     "uph_fcorr_select", "uph_fcorr_clean", "uph_fcorr_tile_stats", "uph_fcorr_mask",
-    "uph_fcorr_nlag", "uph_fcorr_autocov",
+    "uph_fcorr_nlag", "uph_fcorr_autocov", "uph_psf_cutouts", "uph_psf_centroids",
+    "uph_psf_stamps", "uph_psf_polyfit", "uph_psf_chi2", "uph_psf_pixel_median",
 )
 UPH_BAND_NOALLOC_BYTES = 64 << 20
 
@@ -190,6 +191,17 @@ def lib() -> C.CDLL:
     L.uph_fcorr_nlag.argtypes = [C.c_int]
     L.uph_fcorr_autocov.restype = C.c_int
     L.uph_fcorr_autocov.argtypes = [C.c_int] * 3 + [_P] * 5 + [C.c_size_t, _P, _P]
+    L.uph_psf_cutouts.argtypes = [C.c_int] * 3 + [_P] * 4 + [C.c_int] + [C.c_double] * 3 + \
+        [C.c_int] * 2 + [_P] * 4
+    L.uph_psf_centroids.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, _P, _P, _P]
+    L.uph_psf_stamps.argtypes = [C.c_int] * 3 + [_P] * 7 + [C.c_int] * 3 + [C.c_double] * 3 + \
+        [C.c_int] * 2 + [_P] * 3
+    L.uph_psf_polyfit.argtypes = [C.c_int] * 3 + [_P] * 5
+    L.uph_psf_chi2.argtypes = [C.c_int] * 3 + [_P] * 5
+    L.uph_psf_pixel_median.argtypes = [C.c_int] * 2 + [_P] * 4
+    for f in ("uph_psf_cutouts", "uph_psf_centroids", "uph_psf_stamps", "uph_psf_polyfit",
+              "uph_psf_chi2", "uph_psf_pixel_median"):
+        getattr(L, f).restype = C.c_int
     _lib = L
     return L
 
