@@ -92,3 +92,27 @@ def test_wing_blend_keeps_the_core_and_the_constant_sum():
     np.testing.assert_allclose(ratio, ratio[0], rtol=1e-6)    # one renormalisation factor
     np.testing.assert_allclose(out[:, :, 1:][r >= 11.5], 0.0)
     assert ob.wing_blend(coefs, stamps[:19], None, 3.0) is coefs   # below 20 stars: unchanged
+
+
+def test_psf_polynomial_npz_round_trip(tmp_path):
+    from uniphot_amd.psf_build import (PsfBuildResult, load_psf_polynomial_for_band,
+                                       save_psf_polynomial)
+    from uniphot_amd.psf_polynomial import PsfPolynomial
+    rng = np.random.default_rng(5)
+    psf = PsfPolynomial(rng.random((55, 55, 15)).astype(np.float32), 600.0, 600.0, 550.0,
+                        550.0, 55, 55)
+    acc = np.array([True, False, True])
+    res = PsfBuildResult(psf, np.zeros((3, 55, 55), np.float32), np.array([1.0, 2.0, 3.0]),
+                         np.array([4.0, 5.0, 6.0]), np.arange(3), acc, 2, np.zeros(3), 4)
+    path = save_psf_polynomial(tmp_path, "m400", res, 4)
+    with np.load(path) as z:
+        assert sorted(z.files) == sorted(["coefs", "x_origin", "x_scale", "y_origin", "y_scale",
+                                          "H_psf", "W_psf", "polynomial_order", "star_x",
+                                          "star_y"])
+        assert z["x_origin"].dtype == np.float32 and z["H_psf"].dtype == np.int32
+        np.testing.assert_array_equal(z["star_x"], [1.0, 3.0])
+    back = load_psf_polynomial_for_band(tmp_path, "m400")
+    np.testing.assert_array_equal(back.coefs, psf.coefs)
+    assert (back.x_origin, back.y_scale, back.H_psf) == (600.0, 550.0, 55)
+    with pytest.raises(FileNotFoundError):
+        load_psf_polynomial_for_band(tmp_path, "m425")

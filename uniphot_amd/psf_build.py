@@ -192,3 +192,35 @@ def _wing_blend(coefs, med, fwhm, start_fwhm, width=4.0):
     if nt > 1:
         c[:, :, 1:] *= (1.0 - w)[:, :, None]
     return c.astype(np.float32)
+
+
+def save_psf_polynomial(out_dir, band: str, result: PsfBuildResult, polynomial_order: int):
+    """``{out_dir}/{band}/psf_polynomial.npz`` in the layout the reference's
+    stage writes and its loader reads (psf_pipeline_stage.py:555-567,
+    :1066-1084): coefs f32, the four geometry scalars f32, H_psf / W_psf /
+    polynomial_order i32 and the accepted training stars f64."""
+    from pathlib import Path
+    d = Path(out_dir) / band
+    d.mkdir(parents=True, exist_ok=True)
+    p = result.psf
+    np.savez(d / "psf_polynomial.npz", coefs=p.coefs.astype(np.float32),
+             x_origin=np.float32(p.x_origin), x_scale=np.float32(p.x_scale),
+             y_origin=np.float32(p.y_origin), y_scale=np.float32(p.y_scale),
+             H_psf=np.int32(p.H_psf), W_psf=np.int32(p.W_psf),
+             polynomial_order=np.int32(polynomial_order),
+             star_x=np.asarray(result.fitted_x[result.accepted], np.float64),
+             star_y=np.asarray(result.fitted_y[result.accepted], np.float64))
+    return d / "psf_polynomial.npz"
+
+
+def load_psf_polynomial_for_band(psf_dir, band: str) -> PsfPolynomial:
+    """The band's PsfPolynomial from ``{psf_dir}/{band}/psf_polynomial.npz``
+    (psf_pipeline_stage.py:1066-1084); FileNotFoundError when absent."""
+    from pathlib import Path
+    path = Path(psf_dir) / band / "psf_polynomial.npz"
+    if not path.is_file():
+        raise FileNotFoundError(f"PSF polynomial not found for band {band}: {path}")
+    with np.load(path, allow_pickle=False) as z:
+        return PsfPolynomial(z["coefs"].astype(np.float32), float(z["x_origin"]),
+                             float(z["x_scale"]), float(z["y_origin"]), float(z["y_scale"]),
+                             int(z["H_psf"]), int(z["W_psf"]))
