@@ -18,6 +18,7 @@ where new results are written; the user cache wins.
 from __future__ import annotations
 
 import hashlib
+import sys
 import json
 import math
 import os
@@ -113,6 +114,9 @@ def choose(key: str, run, candidates=BLOCK_CANDIDATES, reps: int = 1) -> int:
     ms = {}
     for c in candidates:
         ms[int(c)] = round(1e3 * min(run(c) for _ in range(max(1, reps))), 3)
+        # one line per measured candidate: a long tuning run shows its progress
+        print(f"[autotune] {key}: {int(c)} workgroups {ms[int(c)]} ms", file=sys.stderr,
+              flush=True)
     best = min(ms, key=ms.get)
     entry = dict(choice=int(best), ms=ms, tuned_at=time.strftime("%Y-%m-%d %H:%M:%S"))
     store(key, entry)
