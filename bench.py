@@ -54,9 +54,10 @@ class KernelTimer:
         import torch
         if ev is None:
             return
+        if not info:               # a launch with no cost record (the persistent batch)
+            return
         e1 = torch.cuda.Event(enable_timing=True)
         e1.record()
-        info = info or {}          # launches with no cost record (the persistent batch)
         self.recs.append((info.get("name") or ev[0], ev[1], e1, info))
 
     def record(self, name, e0, e1, info):
